@@ -1,0 +1,162 @@
+/* zfv.h — C ABI of the MI355X-native zephyr finite-volume step.
+ *
+ * Plain pointers and sizes, int status returns (0 = ok), no exceptions and
+ * no torch types across the boundary.  One context owns one device mesh, one
+ * solver state and one HIP stream; a context is not thread-safe.
+ *
+ * The reference has no C ABI: its hot path sits behind C++ classes.  Each
+ * entry point below names the reference interface it replaces.  A host C++
+ * facade with the reference's class shapes (zephyr_amd/host/zfv.hpp:
+ * EuMesh, SmFluid, Rectangle, IdealGas/StiffenedGas) is built on these calls.
+ */
+#ifndef ZFV_H
+#define ZFV_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------ */
+enum {
+    ZFV_OK = 0,
+    ZFV_ERR_ARG = 1,         /* bad argument (size, null pointer, enum value)         */
+    ZFV_ERR_HIP = 2,         /* HIP runtime failure (allocation, launch, copy)        */
+    ZFV_ERR_NUMERIC = 3,     /* a NaN flux: where the reference throws
+                                (hllc.cpp:72-80, hll.cpp:46-54, riemann.cpp:222-229)  */
+    ZFV_ERR_UNSUPPORTED = 4, /* input outside the supported mesh class                */
+    ZFV_ERR_STATE = 5        /* call order (no mesh, no state, not configured)        */
+};
+
+/* Numerical flux ids = zephyr::math::Fluxes values (math/cfd/flux/num_flux.h:11-21). */
+enum { ZFV_GODUNOV = 0, ZFV_RUSANOV = 1, ZFV_HLL = 2, ZFV_HLLC = 3 };
+
+/* Limiter ids; names parsed as Limiter(const std::string&) (math/cfd/limiter.cxx:163-252). */
+enum {
+    ZFV_LIM_NONE = 0, ZFV_LIM_UNIT = 1, ZFV_LIM_MINMOD = 2, ZFV_LIM_MC = 3,
+    ZFV_LIM_VANLEER = 4, ZFV_LIM_VANALBADA1 = 5, ZFV_LIM_VANALBADA2 = 6,
+    ZFV_LIM_CHARM = 7, ZFV_LIM_HCUS = 8, ZFV_LIM_HQUICK = 9, ZFV_LIM_KOREN = 10,
+    ZFV_LIM_SMART = 11, ZFV_LIM_SUPERBEE = 12
+};
+
+/* Boundary ids = zephyr::geom::Boundary values (geom/boundary.h:9-16). */
+enum { ZFV_UNDEFINED = -1, ZFV_INNER = 0, ZFV_WALL = 1, ZFV_ZOE = 2, ZFV_PERIODIC = 3, ZFV_FUNCTION = 4 };
+
+/* Host-side AMR-2D mesh in the reference's AmrCells layout
+ * (mesh/euler/amr_cells.h:74-104, amr_faces.h:83-101): n cells, 8 face slots
+ * per cell (slot s of cell c at index 8*c + s, sides L,R,B,T then their
+ * second sub-faces), 9 vertices per cell in SqQuad order (geom/primitives/quad.h:186-191).
+ * Supported class: Cartesian AMR-2D (every defined slot normal is exactly
+ * -e_x, +e_x, -e_y, +e_y for sides L, R, B, T); anything else is rejected
+ * with ZFV_ERR_UNSUPPORTED. */
+typedef struct {
+    int32_t n_cells;
+    const double* cx;        /* [n] cell centre                                    */
+    const double* cy;
+    const double* volume;    /* [n]                                                */
+    const double* vx;        /* [9n] vertices (incircle diameter, AMR geometry)    */
+    const double* vy;
+    const int32_t* level;    /* [n] may be NULL (all 0)                            */
+    const int32_t* flag;     /* [n] may be NULL (all 0)                            */
+    const int32_t* b_idx;    /* [n] may be NULL (identity)                         */
+    const int32_t* z_idx;    /* [n] may be NULL (all 0)                            */
+    const int32_t* boundary; /* [8n] Boundary id, -1 = undefined slot               */
+    const int32_t* adj;      /* [8n] neighbour cell (the cell itself on boundaries) */
+    const double* nx;        /* [8n] outward normal                                */
+    const double* ny;
+    const double* fcx;       /* [8n] face centre                                   */
+    const double* fcy;
+    const double* area;      /* [8n]                                               */
+} zfv_mesh_soa;
+
+/* Solver parameters: SmFluid setters (math/solver/sm_fluid.h:40-71, sm_fluid.cpp:16-67)
+ * and a stiffened-gas EOS {gamma, P0, e0} (IdealGas = P0 = e0 = 0). */
+typedef struct {
+    int32_t acc;        /* 1 or 2 (clamped as set_accuracy)                 */
+    int32_t flux;       /* ZFV_GODUNOV..ZFV_HLLC                             */
+    int32_t limiter;    /* ZFV_LIM_*                                         */
+    int32_t reserved;
+    double cfl;         /* clamped to [0,1] as set_CFL                      */
+    double max_dt;      /* set_max_dt                                        */
+    double gamma, P0, e0;
+} zfv_smf_params;
+
+typedef struct zfv_ctx zfv_ctx;
+
+/* Text of the last error on this thread. */
+const char* zfv_last_error(void);
+/* Library version string. */
+const char* zfv_version(void);
+
+/* Create / destroy a context on HIP device `device`.  No reference counterpart
+ * (EuMesh + SmFluid construction, eu_mesh.cpp:113-115, sm_fluid.cpp:16-35). */
+int zfv_create(int device, zfv_ctx** out);
+int zfv_destroy(zfv_ctx* ctx);
+/* Run the context's work on an external HIP stream (hipStream_t as void*);
+ * NULL restores the context's own stream. */
+int zfv_set_stream(zfv_ctx* ctx, void* stream);
+int zfv_synchronize(zfv_ctx* ctx);
+
+/* Upload a host AmrCells-layout mesh (replaces EuMesh(Generator&), eu_mesh.cpp:70-104). */
+int zfv_mesh_upload(zfv_ctx* ctx, const zfv_mesh_soa* mesh);
+/* Build the Rectangle AMR-2D base mesh on the host and upload it
+ * (replaces generator::Rectangle + Rectangle::initialize, rectangle.cpp:465-...).
+ * bounds = {left, right, bottom, top} Boundary ids. */
+int zfv_mesh_rectangle(zfv_ctx* ctx, int nx, int ny, double xmin, double xmax,
+                       double ymin, double ymax, const int32_t* bounds);
+/* Number of cells of the current mesh (EuMesh::n_cells). */
+int zfv_mesh_n_cells(zfv_ctx* ctx, int32_t* n);
+/* Download the current mesh into caller arrays sized as zfv_mesh_soa
+ * (pointers of a zfv_mesh_soa used as outputs; NULL members are skipped). */
+int zfv_mesh_download(zfv_ctx* ctx, const zfv_mesh_soa* out);
+
+/* Configure the solver (SmFluid(Eos) + setters + add_types, sm_fluid.cpp:16-67). */
+int zfv_smf_configure(zfv_ctx* ctx, const zfv_smf_params* params);
+/* set_max_dt (sm_fluid.cpp:65-67). */
+int zfv_smf_set_max_dt(zfv_ctx* ctx, double max_dt);
+
+/* Primitive state of the "init" layer, SoA host arrays of n doubles
+ * (cell[part.init] = PState{rho, {u, v, 0}, p, e}, sm_fluid.cpp:25-35). */
+int zfv_state_upload(zfv_ctx* ctx, const double* rho, const double* u, const double* v,
+                     const double* p, const double* e);
+int zfv_state_download(zfv_ctx* ctx, double* rho, double* u, double* v, double* p, double* e);
+/* Same, with device pointers (device-to-device copies on the context stream). */
+int zfv_state_upload_device(zfv_ctx* ctx, const double* rho, const double* u, const double* v,
+                            const double* p, const double* e);
+int zfv_state_download_device(zfv_ctx* ctx, double* rho, double* u, double* v, double* p, double* e);
+/* Isolation counters (Parts::wait, sm_fluid.h:34-35). */
+int zfv_wait_upload(zfv_ctx* ctx, const int32_t* wait);
+int zfv_wait_download(zfv_ctx* ctx, int32_t* wait);
+/* Limited gradient layers d_dx, d_dy of {rho, u, v, p}: 4n doubles each,
+ * component-major (compute_grad, sm_fluid.cpp:116-125). */
+int zfv_grad_download(zfv_ctx* ctx, double* d_dx, double* d_dy);
+
+/* `n_steps` calls of SmFluid::update (sm_fluid.cpp:81-102), enqueued without
+ * a host synchronisation.  Numeric errors are sticky on the device and are
+ * reported by the next synchronising call (dt, download, synchronize). */
+int zfv_smf_step(zfv_ctx* ctx, int n_steps);
+/* SmFluid::dt(): dt of the last step (synchronises). */
+int zfv_smf_dt(zfv_ctx* ctx, double* dt);
+/* Only the gradient stage (compute_grad) on the current state. */
+int zfv_smf_compute_grad(zfv_ctx* ctx);
+/* Only the CFL reduction (compute_dt) on the current state. */
+int zfv_smf_compute_dt(zfv_ctx* ctx, double* dt);
+
+/* Conserved totals {mass, mom_x, mom_y, energy} = sum V*q (synchronises). */
+int zfv_conserved_totals(zfv_ctx* ctx, double* out4);
+
+/* Limiter name -> id with the reference's parsing rules; -1 if unknown
+ * (the reference throws, limiter.cxx:249-251). */
+int zfv_limiter_from_name(const char* name);
+
+/* Kernel timing: average device time (ms) per launch of each step kernel
+ * over the last zfv_smf_step call when enabled (HIP events on the context
+ * stream).  names/ms arrays of length *count (in: capacity, out: used). */
+int zfv_profile_enable(zfv_ctx* ctx, int on);
+int zfv_profile_read(zfv_ctx* ctx, const char** names, double* ms, int32_t* launches, int32_t* count);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* ZFV_H */

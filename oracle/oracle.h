@@ -1,0 +1,110 @@
+// ORACLE — test infrastructure only.
+//
+// A plain-C++ CPU restatement of zephyr's single-material finite-volume step
+// (SmFluid) on AMR-2D meshes.  It is the checker for the HIP product path:
+// only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may
+// load it.  The product library never links or calls it.
+//
+// Every function cites the reference file:line whose behaviour it restates.
+// Operation order: where the reference's order is fixed by Eigen expression
+// templates, the oracle fixes one documented order (see ORDER notes in
+// oracle.cpp).  On Cartesian AMR meshes (axis-aligned normals, z = 0) every
+// such ambiguous sum has at most one non-zero term, so the chosen order does
+// not change any value.
+//
+// Parity pins (tests/test_oracle_*.py): EOS closed forms and round trips,
+// flux consistency F(z,z) = F_phys(z), supersonic early-outs, mirror
+// antisymmetry, HLL/Rusanov closed forms, exact Riemann star states against
+// Toro's published tables, the Limiter::check table, LSM exactness on linear
+// fields, uniform-state preservation, row symmetry, conservation, Sod
+// convergence against the exact solution, and brute-force hand-computed
+// 2x2 steps.
+#pragma once
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+// Numerical flux ids: the values of zephyr::math::Fluxes
+// (reference zephyr/math/cfd/flux/num_flux.h:11-21).
+enum { ORC_GODUNOV = 0, ORC_RUSANOV = 1, ORC_HLL = 2, ORC_HLLC = 3 };
+
+// Limiter ids (reference zephyr/math/cfd/limiter.cxx:163-252 name table).
+enum {
+    ORC_LIM_NONE = 0, ORC_LIM_UNIT = 1, ORC_LIM_MINMOD = 2, ORC_LIM_MC = 3,
+    ORC_LIM_VANLEER = 4, ORC_LIM_VANALBADA1 = 5, ORC_LIM_VANALBADA2 = 6,
+    ORC_LIM_CHARM = 7, ORC_LIM_HCUS = 8, ORC_LIM_HQUICK = 9, ORC_LIM_KOREN = 10,
+    ORC_LIM_SMART = 11, ORC_LIM_SUPERBEE = 12
+};
+
+// Boundary ids: the values of zephyr::geom::Boundary (reference geom/boundary.h:9-16).
+enum { ORC_UNDEFINED = -1, ORC_INNER = 0, ORC_WALL = 1, ORC_ZOE = 2, ORC_PERIODIC = 3 };
+
+// ---- point functions (known-answer tests) --------------------------------
+// z = {rho, vx, vy, vz, p, e} (smf::PState order), eos = {gamma, P0, e0}.
+// Returns 0, or 1 if the reference would have thrown (NaN flux).
+int orc_flux(int kind, const double* zL, const double* zR, const double* eos, double* F);
+double orc_limiter2(int kind, double a, double b);
+double orc_limiter1(int kind, double r);
+// Exact Riemann solver (riemann.cpp:366-460).  out = {rho, Uf, Pf, P, U, conv}.
+void orc_riemann(double rL, double uL, double pL, double gL, double p0L,
+                 double rR, double uR, double pR, double gR, double p0R, double* out);
+// Exact self-similar solution at xi = x/t (riemann.cpp density/velocity/pressure
+// samplers): out = {rho, u, p}.
+void orc_riemann_sample(double rL, double uL, double pL, double rR, double uR, double pR,
+                        double gamma, double p0, double xi, double* out);
+// EOS: out = {p(rho,e), e(rho,p), c(rho,p), min_pressure}.
+void orc_eos(const double* eos, double rho, double e, double p, double* out);
+
+// ---- mesh ----------------------------------------------------------------
+typedef struct orc_mesh orc_mesh;
+orc_mesh* orc_mesh_rectangle(int nx, int ny, double xmin, double xmax,
+                             double ymin, double ymax, const int* bounds /*L,R,B,T*/);
+void orc_mesh_free(orc_mesh* m);
+int  orc_mesh_size(const orc_mesh* m);
+// Cell arrays (n): cx, cy, volume, level, flag, b_idx, z_idx.
+void orc_mesh_get_cells(const orc_mesh* m, double* cx, double* cy, double* vol,
+                        int* level, int* flag, int* b_idx, int* z_idx);
+// Slot arrays (8n, slot s of cell c at 8c+s): boundary, adj, normal, centre, area.
+void orc_mesh_get_slots(const orc_mesh* m, int* boundary, int* adj, double* nx, double* ny,
+                        double* fcx, double* fcy, double* area);
+// Vertex arrays (9n, SqQuad order).
+void orc_mesh_get_verts(const orc_mesh* m, double* vx, double* vy);
+// Incircle diameter per cell (amr_cells.cpp:107-114).
+void orc_mesh_incircle(const orc_mesh* m, double* h);
+
+// ---- solver (SmFluid restatement) ----------------------------------------
+typedef struct orc_smf orc_smf;
+orc_smf* orc_smf_create(orc_mesh* m, double gamma, double P0, double e0,
+                        int flux, int limiter, int acc, double cfl);
+void orc_smf_free(orc_smf* s);
+void orc_smf_set_max_dt(orc_smf* s, double max_dt);
+void orc_smf_set_state(orc_smf* s, const double* rho, const double* u, const double* v,
+                       const double* p, const double* e);
+void orc_smf_get_state(const orc_smf* s, double* rho, double* u, double* v,
+                       double* p, double* e);
+void orc_smf_get_wait(const orc_smf* s, int* wait);
+void orc_smf_set_wait(orc_smf* s, const int* wait);
+// Gradient layers d_dx, d_dy for components {rho, vx, vy, p} (4n each).
+void orc_smf_get_grad(const orc_smf* s, double* dx, double* dy);
+// One SmFluid::update.  Returns 0, or 1 on a reference-thrown numeric error.
+int orc_smf_update(orc_smf* s, int nthreads);
+double orc_smf_dt(const orc_smf* s);
+// Stage entry points for stage-level tests.
+double orc_smf_compute_dt(orc_smf* s, int nthreads);
+void orc_smf_compute_grad(orc_smf* s, int nthreads);
+// Conserved totals {mass, mom_x, mom_y, energy} = sum V*q.
+void orc_smf_totals(const orc_smf* s, double* out);
+
+// ---- synthetic inputs -----------------------------------------------------
+// SplitMix64 uniform stream, u = (x >> 11) * 2^-53.
+double orc_splitmix_uniform(uint64_t* state);
+// Four-quadrant Riemann data drawn from `seed` (quadrants numbered as
+// Riemann2D, phys/tests/test_2D.cpp:72-106); evaluated at cell centres.
+void orc_init_quadrants(const orc_mesh* m, uint64_t seed, double gamma,
+                        double* rho, double* u, double* v, double* p, double* e);
+
+#ifdef __cplusplus
+}
+#endif

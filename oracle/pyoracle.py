@@ -1,0 +1,240 @@
+"""ctypes binding of the oracle (test infrastructure only).
+
+Loads oracle/_build/liborc.so, the CPU restatement of zephyr's SmFluid step.
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg import
+this module; the product package never does.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "_build", "liborc.so")
+
+GODUNOV, RUSANOV, HLL, HLLC = 0, 1, 2, 3
+LIMITERS = ["none", "unit", "minmod", "MC", "vanLeer", "vanAlbada1", "vanAlbada2",
+            "CHARM", "HCUS", "HQUICK", "Koren", "smart", "superbee"]
+UNDEFINED, INNER, WALL, ZOE, PERIODIC = -1, 0, 1, 2, 3
+
+_lib = None
+
+_D = C.c_double
+_I = C.c_int
+_P = C.c_void_p
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int)
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        L = C.CDLL(_LIB_PATH)
+        sig = {
+            "orc_flux": (_I, [_I, _dp, _dp, _dp, _dp]),
+            "orc_limiter2": (_D, [_I, _D, _D]),
+            "orc_limiter1": (_D, [_I, _D]),
+            "orc_riemann": (None, [_D] * 10 + [_dp]),
+            "orc_riemann_sample": (None, [_D] * 9 + [_dp]),
+            "orc_eos": (None, [_dp, _D, _D, _D, _dp]),
+            "orc_mesh_rectangle": (_P, [_I, _I, _D, _D, _D, _D, _ip]),
+            "orc_mesh_free": (None, [_P]),
+            "orc_mesh_size": (_I, [_P]),
+            "orc_mesh_get_cells": (None, [_P, _dp, _dp, _dp, _ip, _ip, _ip, _ip]),
+            "orc_mesh_get_slots": (None, [_P, _ip, _ip, _dp, _dp, _dp, _dp, _dp]),
+            "orc_mesh_get_verts": (None, [_P, _dp, _dp]),
+            "orc_mesh_incircle": (None, [_P, _dp]),
+            "orc_smf_create": (_P, [_P, _D, _D, _D, _I, _I, _I, _D]),
+            "orc_smf_free": (None, [_P]),
+            "orc_smf_set_max_dt": (None, [_P, _D]),
+            "orc_smf_set_state": (None, [_P, _dp, _dp, _dp, _dp, _dp]),
+            "orc_smf_get_state": (None, [_P, _dp, _dp, _dp, _dp, _dp]),
+            "orc_smf_get_wait": (None, [_P, _ip]),
+            "orc_smf_set_wait": (None, [_P, _ip]),
+            "orc_smf_get_grad": (None, [_P, _dp, _dp]),
+            "orc_smf_update": (_I, [_P, _I]),
+            "orc_smf_dt": (_D, [_P]),
+            "orc_smf_compute_dt": (_D, [_P, _I]),
+            "orc_smf_compute_grad": (None, [_P, _I]),
+            "orc_smf_totals": (None, [_P, _dp]),
+            "orc_splitmix_uniform": (_D, [C.POINTER(C.c_uint64)]),
+            "orc_init_quadrants": (None, [_P, C.c_uint64, _D, _dp, _dp, _dp, _dp, _dp]),
+        }
+        for name, (res, args) in sig.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _d(a):
+    return a.ctypes.data_as(_dp)
+
+
+def _i(a):
+    return a.ctypes.data_as(_ip)
+
+
+def flux(kind, zL, zR, eos=(1.4, 0.0, 0.0)):
+    """Numerical flux in the face frame; z = (rho, vx, vy, vz, p, e)."""
+    zL = np.ascontiguousarray(zL, dtype=np.float64)
+    zR = np.ascontiguousarray(zR, dtype=np.float64)
+    e = np.ascontiguousarray(eos, dtype=np.float64)
+    F = np.zeros(5)
+    err = lib().orc_flux(kind, _d(zL), _d(zR), _d(e), _d(F))
+    return F, err
+
+
+def limiter2(kind, a, b):
+    return lib().orc_limiter2(kind, a, b)
+
+
+def limiter1(kind, r):
+    return lib().orc_limiter1(kind, r)
+
+
+def riemann(L, R, gL=1.4, p0L=0.0, gR=None, p0R=None):
+    """Exact Riemann solver face state: dict(rho, Uf, Pf, P, U, conv)."""
+    gR = gL if gR is None else gR
+    p0R = p0L if p0R is None else p0R
+    out = np.zeros(6)
+    lib().orc_riemann(L[0], L[1], L[2], gL, p0L, R[0], R[1], R[2], gR, p0R, _d(out))
+    return dict(rho=out[0], Uf=out[1], Pf=out[2], P=out[3], U=out[4], conv=bool(out[5]))
+
+
+def riemann_sample(L, R, xi, gamma=1.4, p0=0.0):
+    out = np.zeros(3)
+    lib().orc_riemann_sample(L[0], L[1], L[2], R[0], R[1], R[2], gamma, p0, xi, _d(out))
+    return out
+
+
+def eos(params, rho, e, p):
+    out = np.zeros(4)
+    prm = np.ascontiguousarray(params, dtype=np.float64)
+    lib().orc_eos(_d(prm), rho, e, p, _d(out))
+    return dict(p=out[0], e=out[1], c=out[2], pmin=out[3])
+
+
+def splitmix_stream(seed, count):
+    st = C.c_uint64(seed)
+    return np.array([lib().orc_splitmix_uniform(C.byref(st)) for _ in range(count)])
+
+
+class Mesh:
+    """Oracle AMR-2D mesh (8 face slots, 9 vertices per cell)."""
+
+    def __init__(self, handle):
+        self.h = handle
+
+    @classmethod
+    def rectangle(cls, nx, ny, xmin=0.0, xmax=1.0, ymin=0.0, ymax=1.0, bounds=(ZOE,) * 4):
+        b = np.ascontiguousarray(bounds, dtype=np.int32)
+        return cls(lib().orc_mesh_rectangle(nx, ny, xmin, xmax, ymin, ymax, _i(b)))
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.orc_mesh_free(self.h)
+            self.h = None
+
+    @property
+    def n(self):
+        return lib().orc_mesh_size(self.h)
+
+    def cells(self):
+        n = self.n
+        cx, cy, vol = np.zeros(n), np.zeros(n), np.zeros(n)
+        lv, fl, bi, zi = (np.zeros(n, np.int32) for _ in range(4))
+        lib().orc_mesh_get_cells(self.h, _d(cx), _d(cy), _d(vol), _i(lv), _i(fl), _i(bi), _i(zi))
+        return dict(cx=cx, cy=cy, volume=vol, level=lv, flag=fl, b_idx=bi, z_idx=zi)
+
+    def slots(self):
+        ns = 8 * self.n
+        b, a = np.zeros(ns, np.int32), np.zeros(ns, np.int32)
+        nx, ny, fx, fy, ar = (np.zeros(ns) for _ in range(5))
+        lib().orc_mesh_get_slots(self.h, _i(b), _i(a), _d(nx), _d(ny), _d(fx), _d(fy), _d(ar))
+        return dict(boundary=b, adj=a, nx=nx, ny=ny, fcx=fx, fcy=fy, area=ar)
+
+    def verts(self):
+        nv = 9 * self.n
+        vx, vy = np.zeros(nv), np.zeros(nv)
+        lib().orc_mesh_get_verts(self.h, _d(vx), _d(vy))
+        return vx, vy
+
+    def incircle(self):
+        h = np.zeros(self.n)
+        lib().orc_mesh_incircle(self.h, _d(h))
+        return h
+
+    def quadrant_state(self, seed, gamma=1.4):
+        n = self.n
+        arrs = [np.zeros(n) for _ in range(5)]
+        lib().orc_init_quadrants(self.h, C.c_uint64(seed), gamma, *[_d(a) for a in arrs])
+        return arrs
+
+
+class SmFluid:
+    """Oracle SmFluid (math/solver/sm_fluid.h:22-112 restated)."""
+
+    def __init__(self, mesh, gamma=1.4, P0=0.0, e0=0.0, flux=HLLC, limiter="MC", acc=1, cfl=0.5):
+        self.mesh = mesh
+        lim = LIMITERS.index(limiter) if isinstance(limiter, str) else int(limiter)
+        self.h = lib().orc_smf_create(mesh.h, gamma, P0, e0, flux, lim, acc, cfl)
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.orc_smf_free(self.h)
+            self.h = None
+
+    def set_state(self, rho, u, v, p, e):
+        arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (rho, u, v, p, e)]
+        lib().orc_smf_set_state(self.h, *[_d(a) for a in arrs])
+
+    def state(self):
+        n = self.mesh.n
+        arrs = [np.zeros(n) for _ in range(5)]
+        lib().orc_smf_get_state(self.h, *[_d(a) for a in arrs])
+        return arrs
+
+    def wait(self):
+        w = np.zeros(self.mesh.n, np.int32)
+        lib().orc_smf_get_wait(self.h, _i(w))
+        return w
+
+    def set_wait(self, w):
+        w = np.ascontiguousarray(w, dtype=np.int32)
+        lib().orc_smf_set_wait(self.h, _i(w))
+
+    def grad(self):
+        n = self.mesh.n
+        dx, dy = np.zeros(4 * n), np.zeros(4 * n)
+        lib().orc_smf_get_grad(self.h, _d(dx), _d(dy))
+        return dx.reshape(4, n), dy.reshape(4, n)
+
+    def set_max_dt(self, v):
+        lib().orc_smf_set_max_dt(self.h, v)
+
+    def update(self, nthreads=None):
+        nt = nthreads or min(8, os.cpu_count() or 1)
+        return lib().orc_smf_update(self.h, nt)
+
+    def dt(self):
+        return lib().orc_smf_dt(self.h)
+
+    def compute_dt(self, nthreads=1):
+        return lib().orc_smf_compute_dt(self.h, nthreads)
+
+    def compute_grad(self, nthreads=1):
+        lib().orc_smf_compute_grad(self.h, nthreads)
+
+    def totals(self):
+        out = np.zeros(4)
+        lib().orc_smf_totals(self.h, _d(out))
+        return out

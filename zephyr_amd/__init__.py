@@ -1,0 +1,244 @@
+"""zephyr_amd — MI355X-native finite-volume step of zephyr (SmFluid) on AMR-2D meshes.
+
+Python plumbing over the C ABI of libzfv.so (include/zfv.h).  All compute
+runs in the HIP kernels of the library; there is no CPU fallback: if the
+library is missing or no GPU is present, the calls raise.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+from . import build as _build
+
+LIB_PATH = _build.LIB_PATH
+
+GODUNOV, RUSANOV, HLL, HLLC = 0, 1, 2, 3
+FLUXES = {"godunov": GODUNOV, "rusanov": RUSANOV, "hll": HLL, "hllc": HLLC}
+UNDEFINED, INNER, WALL, ZOE, PERIODIC, FUNCTION = -1, 0, 1, 2, 3, 4
+
+ZFV_OK, ZFV_ERR_ARG, ZFV_ERR_HIP, ZFV_ERR_NUMERIC, ZFV_ERR_UNSUPPORTED, ZFV_ERR_STATE = range(6)
+
+# Every symbol include/zfv.h declares.
+EXPORTS = [
+    "zfv_last_error", "zfv_version", "zfv_create", "zfv_destroy", "zfv_set_stream",
+    "zfv_synchronize", "zfv_mesh_upload", "zfv_mesh_rectangle", "zfv_mesh_n_cells",
+    "zfv_mesh_download", "zfv_smf_configure", "zfv_smf_set_max_dt", "zfv_state_upload",
+    "zfv_state_download", "zfv_state_upload_device", "zfv_state_download_device",
+    "zfv_wait_upload", "zfv_wait_download", "zfv_grad_download", "zfv_smf_step", "zfv_smf_dt",
+    "zfv_smf_compute_grad", "zfv_smf_compute_dt", "zfv_conserved_totals",
+    "zfv_limiter_from_name", "zfv_profile_enable", "zfv_profile_read",
+]
+
+
+class ZfvError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"zfv error {code}: {msg}")
+        self.code = code
+
+
+class MeshSoA(C.Structure):
+    _fields_ = [
+        ("n_cells", C.c_int32),
+        ("cx", C.c_void_p), ("cy", C.c_void_p), ("volume", C.c_void_p),
+        ("vx", C.c_void_p), ("vy", C.c_void_p),
+        ("level", C.c_void_p), ("flag", C.c_void_p), ("b_idx", C.c_void_p), ("z_idx", C.c_void_p),
+        ("boundary", C.c_void_p), ("adj", C.c_void_p),
+        ("nx", C.c_void_p), ("ny", C.c_void_p), ("fcx", C.c_void_p), ("fcy", C.c_void_p),
+        ("area", C.c_void_p),
+    ]
+
+
+class SmfParams(C.Structure):
+    _fields_ = [
+        ("acc", C.c_int32), ("flux", C.c_int32), ("limiter", C.c_int32), ("reserved", C.c_int32),
+        ("cfl", C.c_double), ("max_dt", C.c_double),
+        ("gamma", C.c_double), ("P0", C.c_double), ("e0", C.c_double),
+    ]
+
+
+_lib = None
+
+
+def lib():
+    """Load libzfv.so (build it first if absent)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            _build.build()
+        L = C.CDLL(LIB_PATH)
+        vp, i32, d = C.c_void_p, C.c_int32, C.c_double
+        L.zfv_last_error.restype = C.c_char_p
+        L.zfv_version.restype = C.c_char_p
+        L.zfv_limiter_from_name.restype = C.c_int
+        L.zfv_limiter_from_name.argtypes = [C.c_char_p]
+        for name in EXPORTS:
+            f = getattr(L, name)
+            if name not in ("zfv_last_error", "zfv_version", "zfv_limiter_from_name"):
+                f.restype = C.c_int
+        L.zfv_create.argtypes = [C.c_int, C.POINTER(vp)]
+        L.zfv_mesh_rectangle.argtypes = [vp, C.c_int, C.c_int, d, d, d, d, vp]
+        L.zfv_smf_set_max_dt.argtypes = [vp, d]
+        L.zfv_smf_step.argtypes = [vp, C.c_int]
+        L.zfv_profile_enable.argtypes = [vp, C.c_int]
+        _lib = L
+    return _lib
+
+
+def _check(code):
+    if code != ZFV_OK:
+        raise ZfvError(code, lib().zfv_last_error().decode())
+
+
+def limiter_id(name):
+    if isinstance(name, int):
+        return name
+    v = lib().zfv_limiter_from_name(name.encode())
+    if v < 0:
+        raise ValueError(f"Unknown limiter '{name}'")
+    return v
+
+
+def _ptr(a):
+    return C.c_void_p(a.ctypes.data) if a is not None else None
+
+
+class Context:
+    """One device mesh + SmFluid solver state (a zfv_ctx)."""
+
+    def __init__(self, device=0):
+        h = C.c_void_p()
+        _check(lib().zfv_create(device, C.byref(h)))
+        self.h = h
+        self.n = 0
+        self._keep = []
+
+    def close(self):
+        if getattr(self, "h", None) and self.h.value:
+            lib().zfv_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ---- mesh
+    def mesh_rectangle(self, nx, ny, xmin=0.0, xmax=1.0, ymin=0.0, ymax=1.0, bounds=(ZOE,) * 4):
+        b = np.ascontiguousarray(bounds, dtype=np.int32)
+        _check(lib().zfv_mesh_rectangle(self.h, nx, ny, xmin, xmax, ymin, ymax, _ptr(b)))
+        self.n = nx * ny
+
+    def mesh_upload(self, cells, slots, verts):
+        n = len(cells["cx"])
+        arrs = dict(
+            cx=np.ascontiguousarray(cells["cx"], np.float64), cy=np.ascontiguousarray(cells["cy"], np.float64),
+            volume=np.ascontiguousarray(cells["volume"], np.float64),
+            vx=np.ascontiguousarray(verts[0], np.float64), vy=np.ascontiguousarray(verts[1], np.float64),
+            level=np.ascontiguousarray(cells.get("level", np.zeros(n)), np.int32),
+            flag=np.ascontiguousarray(cells.get("flag", np.zeros(n)), np.int32),
+            b_idx=np.ascontiguousarray(cells.get("b_idx", np.arange(n)), np.int32),
+            z_idx=np.ascontiguousarray(cells.get("z_idx", np.zeros(n)), np.int32),
+            boundary=np.ascontiguousarray(slots["boundary"], np.int32),
+            adj=np.ascontiguousarray(slots["adj"], np.int32),
+            nx=np.ascontiguousarray(slots["nx"], np.float64), ny=np.ascontiguousarray(slots["ny"], np.float64),
+            fcx=np.ascontiguousarray(slots["fcx"], np.float64), fcy=np.ascontiguousarray(slots["fcy"], np.float64),
+            area=np.ascontiguousarray(slots["area"], np.float64),
+        )
+        m = MeshSoA(n_cells=n, **{k: v.ctypes.data for k, v in arrs.items()})
+        _check(lib().zfv_mesh_upload(self.h, C.byref(m)))
+        self.n = n
+
+    def mesh_download(self):
+        n = C.c_int32()
+        _check(lib().zfv_mesh_n_cells(self.h, C.byref(n)))
+        n = n.value
+        cells = dict(cx=np.zeros(n), cy=np.zeros(n), volume=np.zeros(n),
+                     level=np.zeros(n, np.int32), flag=np.zeros(n, np.int32),
+                     b_idx=np.zeros(n, np.int32), z_idx=np.zeros(n, np.int32))
+        slots = dict(boundary=np.zeros(8 * n, np.int32), adj=np.zeros(8 * n, np.int32),
+                     nx=np.zeros(8 * n), ny=np.zeros(8 * n), fcx=np.zeros(8 * n), fcy=np.zeros(8 * n),
+                     area=np.zeros(8 * n))
+        vx, vy = np.zeros(9 * n), np.zeros(9 * n)
+        m = MeshSoA(n_cells=n, vx=vx.ctypes.data, vy=vy.ctypes.data,
+                    **{k: v.ctypes.data for k, v in list(cells.items()) + list(slots.items())})
+        _check(lib().zfv_mesh_download(self.h, C.byref(m)))
+        return cells, slots, (vx, vy)
+
+    # ---- solver
+    def configure(self, acc=1, flux=HLLC, limiter="MC", cfl=0.5, gamma=1.4, P0=0.0, e0=0.0, max_dt=None):
+        if isinstance(flux, str):
+            flux = FLUXES[flux.lower()]
+        p = SmfParams(acc=acc, flux=flux, limiter=limiter_id(limiter), reserved=0, cfl=cfl,
+                      max_dt=np.finfo(np.float64).max if max_dt is None else max_dt,
+                      gamma=gamma, P0=P0, e0=e0)
+        _check(lib().zfv_smf_configure(self.h, C.byref(p)))
+
+    def set_max_dt(self, v):
+        _check(lib().zfv_smf_set_max_dt(self.h, v))
+
+    def set_state(self, rho, u, v, p, e):
+        arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (rho, u, v, p, e)]
+        _check(lib().zfv_state_upload(self.h, *[_ptr(a) for a in arrs]))
+
+    def set_state_device(self, ptrs):
+        _check(lib().zfv_state_upload_device(self.h, *[C.c_void_p(p) for p in ptrs]))
+
+    def state(self):
+        arrs = [np.zeros(self.n) for _ in range(5)]
+        _check(lib().zfv_state_download(self.h, *[_ptr(a) for a in arrs]))
+        return arrs
+
+    def set_wait(self, w):
+        w = np.ascontiguousarray(w, dtype=np.int32)
+        _check(lib().zfv_wait_upload(self.h, _ptr(w)))
+
+    def wait(self):
+        w = np.zeros(self.n, np.int32)
+        _check(lib().zfv_wait_download(self.h, _ptr(w)))
+        return w
+
+    def grad(self):
+        dx, dy = np.zeros(4 * self.n), np.zeros(4 * self.n)
+        _check(lib().zfv_grad_download(self.h, _ptr(dx), _ptr(dy)))
+        return dx.reshape(4, self.n), dy.reshape(4, self.n)
+
+    def step(self, n=1):
+        _check(lib().zfv_smf_step(self.h, n))
+
+    def dt(self):
+        v = C.c_double()
+        _check(lib().zfv_smf_dt(self.h, C.byref(v)))
+        return v.value
+
+    def compute_grad(self):
+        _check(lib().zfv_smf_compute_grad(self.h))
+
+    def compute_dt(self):
+        v = C.c_double()
+        _check(lib().zfv_smf_compute_dt(self.h, C.byref(v)))
+        return v.value
+
+    def totals(self):
+        out = np.zeros(4)
+        _check(lib().zfv_conserved_totals(self.h, _ptr(out)))
+        return out
+
+    def synchronize(self):
+        _check(lib().zfv_synchronize(self.h))
+
+    def set_stream(self, stream_ptr):
+        _check(lib().zfv_set_stream(self.h, C.c_void_p(stream_ptr)))
+
+    def profile_enable(self, on=True):
+        _check(lib().zfv_profile_enable(self.h, 1 if on else 0))
+
+    def profile_read(self):
+        cap = 16
+        names = (C.c_char_p * cap)()
+        ms = (C.c_double * cap)()
+        cnt = (C.c_int32 * cap)()
+        count = C.c_int32(cap)
+        _check(lib().zfv_profile_read(self.h, names, ms, cnt, C.byref(count)))
+        return {names[i].decode(): (ms[i], cnt[i]) for i in range(count.value)}

@@ -1,0 +1,757 @@
+// C ABI of the MI355X finite-volume step (include/zfv.h): context, mesh
+// upload and validation, the Rectangle AMR-2D generator, and the step
+// orchestration over the HIP kernels of zfv_kernels.hip.
+#include "../../include/zfv.h"
+#include "zfv_internal.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <limits>
+#include <map>
+#include <string>
+#include <vector>
+
+using namespace zfv;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int fail(int code, const std::string& msg) {
+    g_last_error = msg;
+    return code;
+}
+
+#define ZFV_HIP(call)                                                                     \
+    do {                                                                                  \
+        hipError_t e_ = (call);                                                           \
+        if (e_ != hipSuccess)                                                             \
+            return fail(ZFV_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+template <class T>
+struct DBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    ~DBuf() { release(); }
+    void release() {
+        if (p) hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    hipError_t alloc(size_t count) {
+        if (count == n && p) return hipSuccess;
+        release();
+        if (count == 0) return hipSuccess;
+        hipError_t e = hipMalloc(&p, count * sizeof(T));
+        if (e == hipSuccess) n = count;
+        return e;
+    }
+};
+
+// Host copy of the uploaded mesh in the reference's AmrCells layout.
+struct HostMesh {
+    int n = 0;
+    std::vector<double> cx, cy, vol, vx, vy;
+    std::vector<int32_t> level, flag, b_idx, z_idx;
+    std::vector<int32_t> bnd, adj;
+    std::vector<double> nx, ny, fcx, fcy, area;
+};
+
+struct KernelTimer {
+    std::string name;
+    hipEvent_t t0, t1;
+};
+
+}  // namespace
+
+struct zfv_ctx {
+    int device = 0;
+    hipStream_t own_stream = nullptr;
+    hipStream_t stream = nullptr;
+
+    HostMesh hm;
+    bool have_mesh = false, have_state = false, configured = false;
+
+    // device mesh
+    DBuf<double> cx, cy, vol, h, area, fcx, fcy, w;
+    DBuf<int> code, adj;
+    // device state
+    DBuf<double> st[3][5];   // init / next / half layers
+    DBuf<double> gdx, gdy;
+    DBuf<int> wait[2];
+    int cur = 0, wcur = 0;   // index of the init layer / current wait buffer
+    DBuf<unsigned long long> dtmin;
+    DBuf<double> dt_out, partial;
+    DBuf<int> err;
+
+    zfv_smf_params prm{};
+    bool dt_valid = false;
+    long long step_no = 0;
+
+    bool profile = false;
+    std::vector<KernelTimer> timers;   // pending event pairs of the last zfv_smf_step
+    std::map<std::string, std::pair<double, int>> prof;   // name -> (ms, launches)
+
+    DevMesh dmesh() const {
+        DevMesh M;
+        M.n = hm.n;
+        M.cx = cx.p; M.cy = cy.p; M.vol = vol.p; M.h = h.p;
+        M.code = code.p; M.adj = adj.p; M.area = area.p; M.fcx = fcx.p; M.fcy = fcy.p; M.w = w.p;
+        return M;
+    }
+    DevState layer(int i) const {
+        return DevState{st[i][0].p, st[i][1].p, st[i][2].p, st[i][3].p, st[i][4].p};
+    }
+    Eos eos() const { return Eos{prm.gamma, prm.P0, prm.e0}; }
+};
+
+namespace {
+
+int check_ctx(zfv_ctx* ctx) {
+    if (!ctx) return fail(ZFV_ERR_ARG, "null context");
+    hipError_t e = hipSetDevice(ctx->device);
+    if (e != hipSuccess) return fail(ZFV_ERR_HIP, std::string("hipSetDevice: ") + hipGetErrorString(e));
+    return ZFV_OK;
+}
+
+int code_of_boundary(int b) {
+    switch (b) {
+        case ZFV_UNDEFINED: return SC_UNDEF;
+        case ZFV_INNER: return SC_INNER;
+        case ZFV_PERIODIC: return SC_PERIODIC;
+        case ZFV_WALL: return SC_WALL;
+        case ZFV_ZOE:
+        case ZFV_FUNCTION: return SC_COPY;
+        default: return -1;
+    }
+}
+
+// Canonical outward normals of the AMR-2D sides L, R, B, T.
+const double kNX[4] = {-1.0, 1.0, 0.0, 0.0};
+const double kNY[4] = {0.0, 0.0, -1.0, 1.0};
+
+// Incircle diameter of an adaptive 2D cell from its SqQuad vertices
+// (AmrCells::incircle_diameter, mesh/euler/amr_cells.cpp:107-114).
+double incircle_2d(const double* X, const double* Y) {
+    double ax = X[5] - X[3], ay = Y[5] - Y[3];
+    double bx = X[7] - X[1], by = Y[7] - Y[1];
+    double a = ax * ax + ay * ay, b = bx * bx + by * by;
+    return std::sqrt((b < a) ? b : a);
+}
+
+// Allocate device buffers and copy the validated host mesh.
+int upload_mesh(zfv_ctx* ctx) {
+    const HostMesh& m = ctx->hm;
+    const int n = m.n;
+    const size_t ns = 8 * (size_t)n;
+    std::vector<int> code(n, 0), adj(ns, 0);
+    std::vector<double> area(ns, 0.0), fcx(ns, 0.0), fcy(ns, 0.0), w(ns, 0.0), h(n, 0.0);
+
+    for (int c = 0; c < n; ++c) {
+        int cw = 0;
+        for (int s = 0; s < 8; ++s) {
+            size_t i = 8 * (size_t)c + s;
+            int t = code_of_boundary(m.bnd[i]);
+            if (t < 0) return fail(ZFV_ERR_ARG, "unknown boundary id " + std::to_string(m.bnd[i]));
+            cw |= t << (4 * s);
+            size_t j = (size_t)s * n + c;   // slot-major
+            if (t == SC_UNDEF) {
+                adj[j] = c;
+                continue;
+            }
+            int side = s & 3;
+            if (m.nx[i] != kNX[side] || m.ny[i] != kNY[side])
+                return fail(ZFV_ERR_UNSUPPORTED, "cell " + std::to_string(c) + " slot " + std::to_string(s) +
+                                                     ": normal is not the canonical AMR-2D side normal");
+            int nb = m.adj[i];
+            if (nb < 0 || nb >= n) return fail(ZFV_ERR_ARG, "adjacent index out of range");
+            if ((t == SC_WALL || t == SC_COPY) && nb != c)
+                return fail(ZFV_ERR_ARG, "boundary slot must reference its own cell");
+            adj[j] = nb;
+            area[j] = m.area[i];
+            fcx[j] = m.fcx[i];
+            fcy[j] = m.fcy[i];
+            // LSM weight w = area / |drn|^3, geometry-only (gradient.h:251-256):
+            // dr = (INNER ? neib_c : symm_point(cell_c)) - cell_c; drn keeps
+            // the normal component of dr for a canonical normal.
+            double px, py;
+            if (t == SC_INNER) {
+                px = m.cx[nb];
+                py = m.cy[nb];
+            } else if (side < 2) {
+                px = m.cx[c] + 2.0 * (m.fcx[i] - m.cx[c]);
+                py = m.cy[c];
+            } else {
+                px = m.cx[c];
+                py = m.cy[c] + 2.0 * (m.fcy[i] - m.cy[c]);
+            }
+            double dn = side < 2 ? (px - m.cx[c]) : (py - m.cy[c]);
+            double len = std::sqrt(dn * dn);
+            w[j] = (1.0 * m.area[i]) / std::pow(len, 3);
+        }
+        code[c] = cw;
+        h[c] = incircle_2d(&m.vx[9 * (size_t)c], &m.vy[9 * (size_t)c]);
+    }
+
+    hipStream_t s = ctx->stream;
+    auto put = [&](auto& buf, const auto& host) -> hipError_t {
+        hipError_t e = buf.alloc(host.size());
+        if (e != hipSuccess) return e;
+        return hipMemcpyAsync(buf.p, host.data(), host.size() * sizeof(host[0]), hipMemcpyHostToDevice, s);
+    };
+    ZFV_HIP(put(ctx->cx, m.cx));
+    ZFV_HIP(put(ctx->cy, m.cy));
+    ZFV_HIP(put(ctx->vol, m.vol));
+    ZFV_HIP(put(ctx->h, h));
+    ZFV_HIP(put(ctx->code, code));
+    ZFV_HIP(put(ctx->adj, adj));
+    ZFV_HIP(put(ctx->area, area));
+    ZFV_HIP(put(ctx->fcx, fcx));
+    ZFV_HIP(put(ctx->fcy, fcy));
+    ZFV_HIP(put(ctx->w, w));
+    for (int l = 0; l < 3; ++l)
+        for (int k = 0; k < 5; ++k) ZFV_HIP(ctx->st[l][k].alloc(n));
+    ZFV_HIP(ctx->gdx.alloc(4 * (size_t)n));
+    ZFV_HIP(ctx->gdy.alloc(4 * (size_t)n));
+    for (int k = 0; k < 2; ++k) {
+        ZFV_HIP(ctx->wait[k].alloc(n));
+        ZFV_HIP(hipMemsetAsync(ctx->wait[k].p, 0, n * sizeof(int), s));
+    }
+    ZFV_HIP(ctx->dtmin.alloc(3));
+    ZFV_HIP(ctx->dt_out.alloc(1));
+    ZFV_HIP(ctx->err.alloc(1));
+    ZFV_HIP(hipMemsetAsync(ctx->err.p, 0, sizeof(int), s));
+    ZFV_HIP(ctx->partial.alloc(4 * (size_t)((n + kBlock - 1) / kBlock)));
+    ZFV_HIP(hipStreamSynchronize(s));
+    ctx->have_mesh = true;
+    ctx->have_state = false;
+    ctx->dt_valid = false;
+    ctx->cur = 0;
+    ctx->wcur = 0;
+    return ZFV_OK;
+}
+
+int read_err(zfv_ctx* ctx) {
+    int e = 0;
+    ZFV_HIP(hipMemcpyAsync(&e, ctx->err.p, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    ZFV_HIP(hipStreamSynchronize(ctx->stream));
+    if (e) return fail(ZFV_ERR_NUMERIC, "numeric error: a flux or Riemann solve produced NaN / did not converge");
+    return ZFV_OK;
+}
+
+void collect_timers(zfv_ctx* ctx) {
+    for (auto& t : ctx->timers) {
+        float ms = 0.0f;
+        hipEventSynchronize(t.t1);
+        hipEventElapsedTime(&ms, t.t0, t.t1);
+        auto& slot = ctx->prof[t.name];
+        slot.first += ms;
+        slot.second += 1;
+        hipEventDestroy(t.t0);
+        hipEventDestroy(t.t1);
+    }
+    ctx->timers.clear();
+}
+
+template <class F>
+hipError_t timed(zfv_ctx* ctx, const char* name, F&& launch) {
+    if (!ctx->profile) return launch();
+    KernelTimer t;
+    t.name = name;
+    hipEventCreate(&t.t0);
+    hipEventCreate(&t.t1);
+    hipEventRecord(t.t0, ctx->stream);
+    hipError_t e = launch();
+    hipEventRecord(t.t1, ctx->stream);
+    ctx->timers.push_back(t);
+    return e;
+}
+
+int ensure_dt(zfv_ctx* ctx) {
+    if (ctx->dt_valid) return ZFV_OK;
+    int rd = (int)(ctx->step_no % 3), wr = (int)((ctx->step_no + 1) % 3);
+    ZFV_HIP(launch_fill_u64(ctx->dtmin.p + rd, 1, 0x7ff0000000000000ULL, ctx->stream));
+    ZFV_HIP(launch_fill_u64(ctx->dtmin.p + wr, 1, 0x7ff0000000000000ULL, ctx->stream));
+    DevMesh M = ctx->dmesh();
+    DevState z = ctx->layer(ctx->cur);
+    Eos eos = ctx->eos();
+    ZFV_HIP(timed(ctx, "dt", [&] { return launch_dt(M, z, eos, ctx->dtmin.p + rd, ctx->stream); }));
+    ctx->dt_valid = true;
+    return ZFV_OK;
+}
+
+StepCtl make_ctl(zfv_ctx* ctx) {
+    StepCtl c;
+    c.dtmin = ctx->dtmin.p;
+    c.rd = (int)(ctx->step_no % 3);
+    c.wr = (int)((ctx->step_no + 1) % 3);
+    c.clr = (int)((ctx->step_no + 2) % 3);
+    c.cfl = ctx->prm.cfl;
+    c.max_dt = ctx->prm.max_dt;
+    c.dt_out = ctx->dt_out.p;
+    c.err = ctx->err.p;
+    return c;
+}
+
+int need_ready(zfv_ctx* ctx, bool state = true, bool config = false) {
+    int r = check_ctx(ctx);
+    if (r) return r;
+    if (!ctx->have_mesh) return fail(ZFV_ERR_STATE, "no mesh uploaded");
+    if (state && !ctx->have_state) return fail(ZFV_ERR_STATE, "no state uploaded");
+    if (config && !ctx->configured) return fail(ZFV_ERR_STATE, "solver not configured");
+    return ZFV_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* zfv_last_error(void) { return g_last_error.c_str(); }
+const char* zfv_version(void) { return "zfv 0.1 gfx950"; }
+
+int zfv_create(int device, zfv_ctx** out) {
+    if (!out) return fail(ZFV_ERR_ARG, "null output pointer");
+    int count = 0;
+    hipError_t e = hipGetDeviceCount(&count);
+    if (e != hipSuccess || count == 0) return fail(ZFV_ERR_HIP, "no HIP device available");
+    if (device < 0 || device >= count) return fail(ZFV_ERR_ARG, "device index out of range");
+    ZFV_HIP(hipSetDevice(device));
+    zfv_ctx* ctx = new zfv_ctx();
+    ctx->device = device;
+    e = hipStreamCreateWithFlags(&ctx->own_stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        delete ctx;
+        return fail(ZFV_ERR_HIP, std::string("hipStreamCreate: ") + hipGetErrorString(e));
+    }
+    ctx->stream = ctx->own_stream;
+    *out = ctx;
+    return ZFV_OK;
+}
+
+int zfv_destroy(zfv_ctx* ctx) {
+    if (!ctx) return ZFV_OK;
+    hipSetDevice(ctx->device);
+    hipStreamSynchronize(ctx->stream);
+    collect_timers(ctx);
+    hipStream_t s = ctx->own_stream;
+    delete ctx;
+    if (s) hipStreamDestroy(s);
+    return ZFV_OK;
+}
+
+int zfv_set_stream(zfv_ctx* ctx, void* stream) {
+    int r = check_ctx(ctx);
+    if (r) return r;
+    ZFV_HIP(hipStreamSynchronize(ctx->stream));
+    ctx->stream = stream ? (hipStream_t)stream : ctx->own_stream;
+    return ZFV_OK;
+}
+
+int zfv_synchronize(zfv_ctx* ctx) {
+    int r = check_ctx(ctx);
+    if (r) return r;
+    ZFV_HIP(hipStreamSynchronize(ctx->stream));
+    collect_timers(ctx);
+    if (ctx->have_mesh) return read_err(ctx);
+    return ZFV_OK;
+}
+
+int zfv_mesh_upload(zfv_ctx* ctx, const zfv_mesh_soa* M) {
+    int r = check_ctx(ctx);
+    if (r) return r;
+    if (!M) return fail(ZFV_ERR_ARG, "null mesh");
+    const int n = M->n_cells;
+    if (n <= 0) return fail(ZFV_ERR_ARG, "mesh has no cells");
+    if ((long long)n * 8 > (long long)std::numeric_limits<int>::max())
+        return fail(ZFV_ERR_ARG, "mesh too large for 32-bit slot indexing");
+    if (!M->cx || !M->cy || !M->volume || !M->vx || !M->vy || !M->boundary || !M->adj || !M->nx ||
+        !M->ny || !M->fcx || !M->fcy || !M->area)
+        return fail(ZFV_ERR_ARG, "mesh array missing");
+    HostMesh& m = ctx->hm;
+    const size_t ns = 8 * (size_t)n, nv = 9 * (size_t)n;
+    m.n = n;
+    m.cx.assign(M->cx, M->cx + n);
+    m.cy.assign(M->cy, M->cy + n);
+    m.vol.assign(M->volume, M->volume + n);
+    m.vx.assign(M->vx, M->vx + nv);
+    m.vy.assign(M->vy, M->vy + nv);
+    auto opt = [n](std::vector<int32_t>& dst, const int32_t* src, int fill, bool iota) {
+        if (src) dst.assign(src, src + n);
+        else {
+            dst.assign(n, fill);
+            if (iota)
+                for (int i = 0; i < n; ++i) dst[i] = i;
+        }
+    };
+    opt(m.level, M->level, 0, false);
+    opt(m.flag, M->flag, 0, false);
+    opt(m.b_idx, M->b_idx, 0, true);
+    opt(m.z_idx, M->z_idx, 0, false);
+    m.bnd.assign(M->boundary, M->boundary + ns);
+    m.adj.assign(M->adj, M->adj + ns);
+    m.nx.assign(M->nx, M->nx + ns);
+    m.ny.assign(M->ny, M->ny + ns);
+    m.fcx.assign(M->fcx, M->fcx + ns);
+    m.fcy.assign(M->fcy, M->fcy + ns);
+    m.area.assign(M->area, M->area + ns);
+    return upload_mesh(ctx);
+}
+
+// Rectangle AMR-2D base mesh (generator::Rectangle, Rectangle::initialize,
+// geom/generator/rectangle.cpp:465-...; AmrCells(const Rectangle&),
+// mesh/euler/amr_cells_gen.cpp:128-281): cell ic = i*ny + j, slots L,R,B,T
+// active, boundary slots reference the cell itself, 9 SqQuad vertices.
+int zfv_mesh_rectangle(zfv_ctx* ctx, int nx, int ny, double xmin, double xmax,
+                       double ymin, double ymax, const int32_t* bounds) {
+    int r = check_ctx(ctx);
+    if (r) return r;
+    if (nx < 1 || ny < 1) return fail(ZFV_ERR_ARG, "Rectangle: nx, ny must be >= 1");
+    if (!(xmax > xmin) || !(ymax > ymin)) return fail(ZFV_ERR_ARG, "Rectangle: empty domain");
+    if (!bounds) return fail(ZFV_ERR_ARG, "null bounds");
+    for (int k = 0; k < 4; ++k)
+        if (bounds[k] < ZFV_INNER || bounds[k] > ZFV_FUNCTION || bounds[k] == ZFV_INNER)
+            return fail(ZFV_ERR_ARG, "Rectangle: bad boundary id");
+    if ((long long)nx * ny * 9 > (long long)std::numeric_limits<int>::max())
+        return fail(ZFV_ERR_ARG, "Rectangle too large");
+    const bool xper = bounds[0] == ZFV_PERIODIC && bounds[1] == ZFV_PERIODIC;
+    const bool yper = bounds[2] == ZFV_PERIODIC && bounds[3] == ZFV_PERIODIC;
+    const int n = nx * ny;
+    HostMesh& m = ctx->hm;
+    m.n = n;
+    const size_t ns = 8 * (size_t)n;
+    m.cx.resize(n); m.cy.resize(n); m.vol.resize(n);
+    m.vx.resize(9 * (size_t)n); m.vy.resize(9 * (size_t)n);
+    m.level.assign(n, 0); m.flag.assign(n, 0); m.b_idx.resize(n); m.z_idx.assign(n, 0);
+    m.bnd.assign(ns, ZFV_UNDEFINED); m.adj.assign(ns, -1);
+    m.nx.assign(ns, 0.0); m.ny.assign(ns, 0.0); m.fcx.assign(ns, 0.0); m.fcy.assign(ns, 0.0);
+    m.area.assign(ns, 0.0);
+    const double hx = (xmax - xmin) / nx, hy = (ymax - ymin) / ny;
+    const double Lx = xmax - xmin, Ly = ymax - ymin;
+    for (int i = 0; i < nx; ++i) {
+        const double x0 = xmin + (Lx * i) / nx, x1 = xmin + (Lx * (i + 1)) / nx;
+        for (int j = 0; j < ny; ++j) {
+            const int c = i * ny + j;
+            const double y0 = ymin + (Ly * j) / ny, y1 = ymin + (Ly * (j + 1)) / ny;
+            // SqQuad(v00, v01, v10, v11): rows by y, columns by x (quad.cpp:429-436)
+            double* X = &m.vx[9 * (size_t)c];
+            double* Y = &m.vy[9 * (size_t)c];
+            X[0] = x0; Y[0] = y0;
+            X[2] = x1; Y[2] = y0;
+            X[6] = x0; Y[6] = y1;
+            X[8] = x1; Y[8] = y1;
+            X[1] = 0.5 * (x0 + x1); Y[1] = 0.5 * (y0 + y0);
+            X[3] = 0.5 * (x0 + x0); Y[3] = 0.5 * (y0 + y1);
+            X[5] = 0.5 * (x1 + x1); Y[5] = 0.5 * (y0 + y1);
+            X[7] = 0.5 * (x0 + x1); Y[7] = 0.5 * (y1 + y1);
+            X[4] = 0.25 * (((x0 + x1) + x0) + x1);
+            Y[4] = 0.25 * (((y0 + y0) + y1) + y1);
+            m.cx[c] = X[4];
+            m.cy[c] = Y[4];
+            m.vol[c] = hx * hy;
+            m.b_idx[c] = c;
+            const size_t f = 8 * (size_t)c;
+            m.bnd[f + 0] = i > 0 ? ZFV_INNER : bounds[0];
+            m.bnd[f + 1] = i < nx - 1 ? ZFV_INNER : bounds[1];
+            m.bnd[f + 2] = j > 0 ? ZFV_INNER : bounds[2];
+            m.bnd[f + 3] = j < ny - 1 ? ZFV_INNER : bounds[3];
+            m.adj[f + 0] = (i == 0 && !xper) ? c : ((i - 1 + nx) % nx) * ny + j;
+            m.adj[f + 1] = (i == nx - 1 && !xper) ? c : ((i + 1) % nx) * ny + j;
+            m.adj[f + 2] = (j == 0 && !yper) ? c : i * ny + (j - 1 + ny) % ny;
+            m.adj[f + 3] = (j == ny - 1 && !yper) ? c : i * ny + (j + 1) % ny;
+            const int cen[4] = {3, 5, 1, 7};
+            for (int s = 0; s < 4; ++s) {
+                m.nx[f + s] = kNX[s];
+                m.ny[f + s] = kNY[s];
+                m.fcx[f + s] = X[cen[s]];
+                m.fcy[f + s] = Y[cen[s]];
+                m.area[f + s] = s < 2 ? hy : hx;
+            }
+        }
+    }
+    return upload_mesh(ctx);
+}
+
+int zfv_mesh_n_cells(zfv_ctx* ctx, int32_t* n) {
+    if (!ctx || !n) return fail(ZFV_ERR_ARG, "null argument");
+    *n = ctx->have_mesh ? ctx->hm.n : 0;
+    return ZFV_OK;
+}
+
+int zfv_mesh_download(zfv_ctx* ctx, const zfv_mesh_soa* o) {
+    if (!ctx || !o) return fail(ZFV_ERR_ARG, "null argument");
+    if (!ctx->have_mesh) return fail(ZFV_ERR_STATE, "no mesh uploaded");
+    const HostMesh& m = ctx->hm;
+    auto cp = [](const void* dst, const void* src, size_t bytes) {
+        if (dst) std::memcpy(const_cast<void*>(dst), src, bytes);
+    };
+    const size_t n = m.n, ns = 8 * n, nv = 9 * n;
+    cp(o->cx, m.cx.data(), n * 8); cp(o->cy, m.cy.data(), n * 8); cp(o->volume, m.vol.data(), n * 8);
+    cp(o->vx, m.vx.data(), nv * 8); cp(o->vy, m.vy.data(), nv * 8);
+    cp(o->level, m.level.data(), n * 4); cp(o->flag, m.flag.data(), n * 4);
+    cp(o->b_idx, m.b_idx.data(), n * 4); cp(o->z_idx, m.z_idx.data(), n * 4);
+    cp(o->boundary, m.bnd.data(), ns * 4); cp(o->adj, m.adj.data(), ns * 4);
+    cp(o->nx, m.nx.data(), ns * 8); cp(o->ny, m.ny.data(), ns * 8);
+    cp(o->fcx, m.fcx.data(), ns * 8); cp(o->fcy, m.fcy.data(), ns * 8);
+    cp(o->area, m.area.data(), ns * 8);
+    return ZFV_OK;
+}
+
+int zfv_smf_configure(zfv_ctx* ctx, const zfv_smf_params* p) {
+    int r = check_ctx(ctx);
+    if (r) return r;
+    if (!p) return fail(ZFV_ERR_ARG, "null params");
+    if (p->flux < ZFV_GODUNOV || p->flux > ZFV_HLLC)
+        return fail(ZFV_ERR_ARG, "unsupported numerical flux id " + std::to_string(p->flux));
+    if (p->limiter < ZFV_LIM_NONE || p->limiter > ZFV_LIM_SUPERBEE)
+        return fail(ZFV_ERR_ARG, "unknown limiter id " + std::to_string(p->limiter));
+    if (!(p->gamma > 1.0)) return fail(ZFV_ERR_ARG, "EOS gamma must be > 1");
+    ctx->prm = *p;
+    ctx->prm.acc = std::min(std::max(1, p->acc), 2);               // set_accuracy
+    ctx->prm.cfl = std::max(0.0, std::min(p->cfl, 1.0));            // set_CFL
+    if (std::isnan(ctx->prm.max_dt)) ctx->prm.max_dt = std::numeric_limits<double>::max();
+    ctx->configured = true;
+    ctx->dt_valid = false;
+    return ZFV_OK;
+}
+
+int zfv_smf_set_max_dt(zfv_ctx* ctx, double max_dt) {
+    if (!ctx) return fail(ZFV_ERR_ARG, "null context");
+    ctx->prm.max_dt = max_dt;
+    return ZFV_OK;
+}
+
+static int state_copy(zfv_ctx* ctx, const double* const* src, bool to_device, hipMemcpyKind kind) {
+    const int n = ctx->hm.n;
+    for (int k = 0; k < 5; ++k)
+        if (!src[k]) return fail(ZFV_ERR_ARG, "null state array");
+    for (int k = 0; k < 5; ++k) {
+        double* d = ctx->st[ctx->cur][k].p;
+        if (to_device) ZFV_HIP(hipMemcpyAsync(d, src[k], n * sizeof(double), kind, ctx->stream));
+        else ZFV_HIP(hipMemcpyAsync(const_cast<double*>(src[k]), d, n * sizeof(double), kind, ctx->stream));
+    }
+    ZFV_HIP(hipStreamSynchronize(ctx->stream));
+    return ZFV_OK;
+}
+
+int zfv_state_upload(zfv_ctx* ctx, const double* rho, const double* u, const double* v, const double* p,
+                     const double* e) {
+    int r = need_ready(ctx, false);
+    if (r) return r;
+    const double* a[5] = {rho, u, v, p, e};
+    r = state_copy(ctx, a, true, hipMemcpyHostToDevice);
+    if (r) return r;
+    ctx->have_state = true;
+    ctx->dt_valid = false;
+    return ZFV_OK;
+}
+
+int zfv_state_upload_device(zfv_ctx* ctx, const double* rho, const double* u, const double* v,
+                            const double* p, const double* e) {
+    int r = need_ready(ctx, false);
+    if (r) return r;
+    const double* a[5] = {rho, u, v, p, e};
+    r = state_copy(ctx, a, true, hipMemcpyDeviceToDevice);
+    if (r) return r;
+    ctx->have_state = true;
+    ctx->dt_valid = false;
+    return ZFV_OK;
+}
+
+int zfv_state_download(zfv_ctx* ctx, double* rho, double* u, double* v, double* p, double* e) {
+    int r = need_ready(ctx);
+    if (r) return r;
+    const double* a[5] = {rho, u, v, p, e};
+    r = state_copy(ctx, a, false, hipMemcpyDeviceToHost);
+    if (r) return r;
+    return read_err(ctx);
+}
+
+int zfv_state_download_device(zfv_ctx* ctx, double* rho, double* u, double* v, double* p, double* e) {
+    int r = need_ready(ctx);
+    if (r) return r;
+    const double* a[5] = {rho, u, v, p, e};
+    r = state_copy(ctx, a, false, hipMemcpyDeviceToDevice);
+    if (r) return r;
+    return read_err(ctx);
+}
+
+int zfv_wait_upload(zfv_ctx* ctx, const int32_t* wait) {
+    int r = need_ready(ctx, false);
+    if (r) return r;
+    if (!wait) return fail(ZFV_ERR_ARG, "null wait");
+    ZFV_HIP(hipMemcpyAsync(ctx->wait[ctx->wcur].p, wait, ctx->hm.n * sizeof(int), hipMemcpyHostToDevice, ctx->stream));
+    ZFV_HIP(hipStreamSynchronize(ctx->stream));
+    return ZFV_OK;
+}
+
+int zfv_wait_download(zfv_ctx* ctx, int32_t* wait) {
+    int r = need_ready(ctx, false);
+    if (r) return r;
+    if (!wait) return fail(ZFV_ERR_ARG, "null wait");
+    ZFV_HIP(hipMemcpyAsync(wait, ctx->wait[ctx->wcur].p, ctx->hm.n * sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    ZFV_HIP(hipStreamSynchronize(ctx->stream));
+    return ZFV_OK;
+}
+
+int zfv_grad_download(zfv_ctx* ctx, double* dx, double* dy) {
+    int r = need_ready(ctx);
+    if (r) return r;
+    if (!dx || !dy) return fail(ZFV_ERR_ARG, "null output");
+    const size_t bytes = 4 * (size_t)ctx->hm.n * sizeof(double);
+    ZFV_HIP(hipMemcpyAsync(dx, ctx->gdx.p, bytes, hipMemcpyDeviceToHost, ctx->stream));
+    ZFV_HIP(hipMemcpyAsync(dy, ctx->gdy.p, bytes, hipMemcpyDeviceToHost, ctx->stream));
+    ZFV_HIP(hipStreamSynchronize(ctx->stream));
+    return ZFV_OK;
+}
+
+// SmFluid::update x n_steps (sm_fluid.cpp:81-102).
+int zfv_smf_step(zfv_ctx* ctx, int n_steps) {
+    int r = need_ready(ctx, true, true);
+    if (r) return r;
+    if (n_steps < 0) return fail(ZFV_ERR_ARG, "negative step count");
+    r = ensure_dt(ctx);
+    if (r) return r;
+    const DevMesh M = ctx->dmesh();
+    const Eos eos = ctx->eos();
+    const zfv_smf_params& P = ctx->prm;
+    for (int it = 0; it < n_steps; ++it) {
+        StepCtl ctl = make_ctl(ctx);
+        const int nxt = (ctx->cur + 1) % 2;   // layers 0/1 alternate as init/next, 2 is half
+        DevState in = ctx->layer(ctx->cur), out = ctx->layer(nxt);
+        if (P.acc == 1) {
+            ZFV_HIP(timed(ctx, "flux1", [&] { return launch_flux1(P.flux, M, in, out, eos, ctl, ctx->stream); }));
+        } else {
+            DevGrad G{ctx->gdx.p, ctx->gdy.p};
+            DevState half = ctx->layer(2);
+            int* w_in = ctx->wait[ctx->wcur].p;
+            int* w_out = ctx->wait[1 - ctx->wcur].p;
+            ZFV_HIP(timed(ctx, "grad_pred", [&] {
+                return launch_grad_pred(P.limiter, true, M, in, G, half, w_in, eos, ctl, ctx->stream);
+            }));
+            ZFV_HIP(timed(ctx, "corr", [&] {
+                return launch_corr(P.flux, M, in, half, G, w_in, w_out, out, eos, ctl, ctx->stream);
+            }));
+            ctx->wcur = 1 - ctx->wcur;
+        }
+        ctx->cur = nxt;
+        ctx->step_no += 1;
+    }
+    return ZFV_OK;
+}
+
+int zfv_smf_dt(zfv_ctx* ctx, double* dt) {
+    int r = need_ready(ctx, false);
+    if (r) return r;
+    if (!dt) return fail(ZFV_ERR_ARG, "null output");
+    ZFV_HIP(hipMemcpyAsync(dt, ctx->dt_out.p, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    ZFV_HIP(hipStreamSynchronize(ctx->stream));
+    return read_err(ctx);
+}
+
+int zfv_smf_compute_grad(zfv_ctx* ctx) {
+    int r = need_ready(ctx, true, true);
+    if (r) return r;
+    r = ensure_dt(ctx);
+    if (r) return r;
+    StepCtl ctl = make_ctl(ctx);
+    DevGrad G{ctx->gdx.p, ctx->gdy.p};
+    ZFV_HIP(launch_grad_pred(ctx->prm.limiter, false, ctx->dmesh(), ctx->layer(ctx->cur), G, ctx->layer(2),
+                             ctx->wait[ctx->wcur].p, ctx->eos(), ctl, ctx->stream));
+    ZFV_HIP(hipStreamSynchronize(ctx->stream));
+    return ZFV_OK;
+}
+
+int zfv_smf_compute_dt(zfv_ctx* ctx, double* dt) {
+    int r = need_ready(ctx, true, true);
+    if (r) return r;
+    ctx->dt_valid = false;
+    r = ensure_dt(ctx);
+    if (r) return r;
+    unsigned long long bits = 0;
+    ZFV_HIP(hipMemcpyAsync(&bits, ctx->dtmin.p + (ctx->step_no % 3), 8, hipMemcpyDeviceToHost, ctx->stream));
+    ZFV_HIP(hipStreamSynchronize(ctx->stream));
+    double dmin;
+    std::memcpy(&dmin, &bits, 8);
+    double v = ctx->prm.cfl * dmin;
+    if (dt) *dt = (ctx->prm.max_dt < v) ? ctx->prm.max_dt : v;
+    return ZFV_OK;
+}
+
+int zfv_conserved_totals(zfv_ctx* ctx, double* out4) {
+    int r = need_ready(ctx, true, true);
+    if (r) return r;
+    if (!out4) return fail(ZFV_ERR_ARG, "null output");
+    const int nb = (ctx->hm.n + kBlock - 1) / kBlock;
+    ZFV_HIP(launch_totals(ctx->dmesh(), ctx->layer(ctx->cur), ctx->partial.p, nb, ctx->stream));
+    std::vector<double> part(4 * (size_t)nb);
+    ZFV_HIP(hipMemcpyAsync(part.data(), ctx->partial.p, part.size() * 8, hipMemcpyDeviceToHost, ctx->stream));
+    ZFV_HIP(hipStreamSynchronize(ctx->stream));
+    for (int k = 0; k < 4; ++k) out4[k] = 0.0;
+    for (int b = 0; b < nb; ++b)
+        for (int k = 0; k < 4; ++k) out4[k] += part[4 * (size_t)b + k];
+    return read_err(ctx);
+}
+
+// Limiter(const std::string&) name parsing (math/cfd/limiter.cxx:163-252).
+int zfv_limiter_from_name(const char* orig) {
+    if (!orig) return -1;
+    std::string name(orig);
+    for (auto& ch : name) ch = (char)std::tolower((unsigned char)ch);
+    auto erase_all = [&](const std::string& pat) {
+        size_t pos;
+        while ((pos = name.find(pat)) != std::string::npos) name.erase(pos, pat.size());
+    };
+    erase_all(" ");
+    erase_all("-");
+    erase_all("_");
+    erase_all("lim");
+    erase_all("limiter");
+    auto has = [&](const char* s) { return name.find(s) != std::string::npos; };
+    if (name.empty() || has("off") || has("none") || has("zero") || has("empty")) return ZFV_LIM_NONE;
+    if (has("one") || has("unit")) return ZFV_LIM_UNIT;
+    if (has("minmod")) return ZFV_LIM_MINMOD;
+    if (has("mc")) return ZFV_LIM_MC;
+    if (has("leer")) return ZFV_LIM_VANLEER;
+    if (has("albada")) return has("2") ? ZFV_LIM_VANALBADA2 : ZFV_LIM_VANALBADA1;
+    if (has("charm")) return ZFV_LIM_CHARM;
+    if (has("hcus")) return ZFV_LIM_HCUS;
+    if (has("hquick")) return ZFV_LIM_HQUICK;
+    if (has("koren")) return ZFV_LIM_KOREN;
+    if (has("smart")) return ZFV_LIM_SMART;
+    if (has("superbee")) return ZFV_LIM_SUPERBEE;
+    g_last_error = std::string("Unknown limiter '") + orig + "'";
+    return -1;
+}
+
+int zfv_profile_enable(zfv_ctx* ctx, int on) {
+    int r = check_ctx(ctx);
+    if (r) return r;
+    ZFV_HIP(hipStreamSynchronize(ctx->stream));
+    collect_timers(ctx);
+    ctx->prof.clear();
+    ctx->profile = on != 0;
+    return ZFV_OK;
+}
+
+int zfv_profile_read(zfv_ctx* ctx, const char** names, double* ms, int32_t* launches, int32_t* count) {
+    int r = check_ctx(ctx);
+    if (r) return r;
+    if (!count) return fail(ZFV_ERR_ARG, "null count");
+    ZFV_HIP(hipStreamSynchronize(ctx->stream));
+    collect_timers(ctx);
+    int cap = *count, i = 0;
+    for (auto& kv : ctx->prof) {
+        if (i < cap) {
+            if (names) names[i] = kv.first.c_str();
+            if (ms) ms[i] = kv.second.first;
+            if (launches) launches[i] = kv.second.second;
+        }
+        ++i;
+    }
+    *count = std::min(i, cap);
+    return ZFV_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,79 @@
+"""Synthetic initial data for the benchmark workloads (seeded, host-side setup).
+
+Draw order and quadrant numbering follow the reference's Riemann2D test
+(phys/tests/test_2D.cpp:72-106); the stream is SplitMix64 with
+u = (x >> 11) * 2^-53.
+"""
+import numpy as np
+
+_M64 = (1 << 64) - 1
+
+
+class SplitMix64:
+    def __init__(self, seed):
+        self.x = seed & _M64
+
+    def next_u64(self):
+        self.x = (self.x + 0x9E3779B97F4A7C15) & _M64
+        z = self.x
+        z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & _M64
+        z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & _M64
+        return z ^ (z >> 31)
+
+    def uniform(self):
+        return (self.next_u64() >> 11) * (1.0 / 9007199254740992.0)
+
+
+def energy_rP(rho, p, gamma, P0=0.0, e0=0.0):
+    """StiffenedGas::energy_rP (phys/matter/eos/stiffened_gas.cpp:121-122)."""
+    return e0 + (p + gamma * P0) / ((gamma - 1.0) * rho)
+
+
+def quadrant_states(seed):
+    """Per-quadrant (rho, u, v, p) for quadrants 1..4: rho~U[0.1,2], u,v~U[-1,1], p~U[0.1,2]."""
+    g = SplitMix64(seed)
+    out = []
+    for _ in range(4):
+        r = 0.1 + 1.9 * g.uniform()
+        u = -1.0 + 2.0 * g.uniform()
+        v = -1.0 + 2.0 * g.uniform()
+        p = 0.1 + 1.9 * g.uniform()
+        out.append((r, u, v, p))
+    return out
+
+
+def four_quadrant(cx, cy, seed, gamma=1.4):
+    """Four-quadrant Riemann data at cell centres -> (rho, u, v, p, e)."""
+    q = quadrant_states(seed)
+    cx = np.asarray(cx)
+    cy = np.asarray(cy)
+    idx = np.full(cx.shape, 3)
+    idx[(cx >= 0.5) & (cy >= 0.5)] = 0
+    idx[(cx <= 0.5) & (cy >= 0.5) & (idx == 3)] = 1
+    idx[(cx <= 0.5) & (cy <= 0.5) & (idx == 3)] = 2
+    tab = np.array(q)
+    rho, u, v, p = (tab[idx, k].copy() for k in range(4))
+    e = energy_rP(rho, p, gamma)
+    return rho, u, v, p, e
+
+
+def gaussian_random_field(nx, ny, seed, corr_len=0.05, Lx=1.0, Ly=1.0):
+    """Unit-variance Gaussian random field on the cell grid (index i*ny + j):
+    white noise filtered by a Gaussian kernel of correlation length corr_len
+    in Fourier space, then normalised to mean 0, std 1."""
+    rng = np.random.default_rng(seed)
+    w = rng.standard_normal((nx, ny))
+    kx = np.fft.fftfreq(nx, d=Lx / nx) * 2.0 * np.pi
+    ky = np.fft.fftfreq(ny, d=Ly / ny) * 2.0 * np.pi
+    K2 = kx[:, None] ** 2 + ky[None, :] ** 2
+    f = np.real(np.fft.ifft2(np.fft.fft2(w) * np.exp(-0.25 * K2 * corr_len ** 2)))
+    f = (f - f.mean()) / f.std()
+    return f.reshape(-1)
+
+
+def smooth_density_field(nx, ny, seed, gamma=1.4):
+    """Adjoint / uniform-mesh workload: rho = 1 + 0.1 g, p = 1, u = v = 0."""
+    rho = 1.0 + 0.1 * gaussian_random_field(nx, ny, seed)
+    p = np.ones_like(rho)
+    z = np.zeros_like(rho)
+    return rho, z, z.copy(), p, energy_rP(rho, p, gamma)
