@@ -521,9 +521,9 @@ double lim_binary(int kind, double a, double b) {
 // =============================================================== mesh
 struct orc_mesh {
     int n = 0;
-    // cells (amr_cells.h:74-104)
+    // cells (amr_cells.h:74-104); idx < 0 marks an undefined cell during AMR
     std::vector<double> cx, cy, vol;
-    std::vector<int> level, flag, b_idx, z_idx;
+    std::vector<int> level, flag, b_idx, z_idx, next, idx;
     std::vector<double> vx, vy;   // 9 per cell, SqQuad order (quad.h:186-191)
     // face slots, 8 per cell (amr_faces.h:83-101)
     std::vector<int> bnd, adj;
@@ -567,6 +567,8 @@ orc_mesh* orc_mesh_rectangle(int nx, int ny, double xmin, double xmax,
     m->n = n;
     m->cx.resize(n); m->cy.resize(n); m->vol.resize(n);
     m->level.assign(n, 0); m->flag.assign(n, 0); m->b_idx.resize(n); m->z_idx.assign(n, 0);
+    m->next.resize(n); m->idx.resize(n);
+    for (int c = 0; c < n; ++c) { m->next[c] = c; m->idx[c] = c; }
     m->vx.resize(9 * (size_t)n); m->vy.resize(9 * (size_t)n);
     size_t ns = 8 * (size_t)n;
     m->bnd.assign(ns, ORC_UNDEFINED); m->adj.assign(ns, -1);
@@ -638,6 +640,15 @@ void orc_mesh_get_slots(const orc_mesh* m, int* boundary, int* adj, double* nx, 
     std::memcpy(nx, m->fnx.data(), ns * 8); std::memcpy(ny, m->fny.data(), ns * 8);
     std::memcpy(fcx, m->fcx.data(), ns * 8); std::memcpy(fcy, m->fcy.data(), ns * 8);
     std::memcpy(area, m->area.data(), ns * 8);
+}
+
+void orc_mesh_set_slots(orc_mesh* m, const int* boundary, const int* adj, const double* nx, const double* ny,
+                        const double* fcx, const double* fcy, const double* area) {
+    size_t ns = 8 * (size_t)m->n;
+    std::memcpy(m->bnd.data(), boundary, ns * 4); std::memcpy(m->adj.data(), adj, ns * 4);
+    std::memcpy(m->fnx.data(), nx, ns * 8); std::memcpy(m->fny.data(), ny, ns * 8);
+    std::memcpy(m->fcx.data(), fcx, ns * 8); std::memcpy(m->fcy.data(), fcy, ns * 8);
+    std::memcpy(m->area.data(), area, ns * 8);
 }
 
 void orc_mesh_get_verts(const orc_mesh* m, double* vx, double* vy) {
@@ -1117,3 +1128,5 @@ void orc_init_quadrants(const orc_mesh* m, uint64_t seed, double gamma,
 }
 
 }  // extern "C"
+
+#include "oracle_amr.hpp"

@@ -69,6 +69,8 @@ void orc_mesh_get_cells(const orc_mesh* m, double* cx, double* cy, double* vol,
 // Slot arrays (8n, slot s of cell c at 8c+s): boundary, adj, normal, centre, area.
 void orc_mesh_get_slots(const orc_mesh* m, int* boundary, int* adj, double* nx, double* ny,
                         double* fcx, double* fcy, double* area);
+void orc_mesh_set_slots(orc_mesh* m, const int* boundary, const int* adj, const double* nx, const double* ny,
+                        const double* fcx, const double* fcy, const double* area);
 // Vertex arrays (9n, SqQuad order).
 void orc_mesh_get_verts(const orc_mesh* m, double* vx, double* vy);
 // Incircle diameter per cell (amr_cells.cpp:107-114).
@@ -96,6 +98,16 @@ double orc_smf_compute_dt(orc_smf* s, int nthreads);
 void orc_smf_compute_grad(orc_smf* s, int nthreads);
 // Conserved totals {mass, mom_x, mom_y, energy} = sum V*q.
 void orc_smf_totals(const orc_smf* s, double* out);
+
+// ---- AMR (oracle_amr.hpp) ---------------------------------------------------
+// SmFluid::set_flags: criterion 0 = relative jumps (sm_fluid.cpp:517-643),
+// 1 = normalised limited density gradient |grad rho| h / rho vs threshold.
+void orc_smf_set_flags(orc_smf* s, int criterion, double threshold, int nthreads);
+void orc_mesh_set_flags(orc_mesh* m, const int* flags);
+void orc_mesh_balance_flags(orc_mesh* m, int max_level);
+// EuMesh::refine: balance + apply; s may be NULL (mesh only); slope: distributor.
+int orc_mesh_refine(orc_mesh* m, orc_smf* s, int max_level, int slope);
+void orc_mesh_get_next(const orc_mesh* m, int* next);
 
 // ---- synthetic inputs -----------------------------------------------------
 // SplitMix64 uniform stream, u = (x >> 11) * 2^-53.

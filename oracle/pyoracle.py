@@ -50,6 +50,7 @@ def lib():
             "orc_mesh_get_cells": (None, [_P, _dp, _dp, _dp, _ip, _ip, _ip, _ip]),
             "orc_mesh_get_slots": (None, [_P, _ip, _ip, _dp, _dp, _dp, _dp, _dp]),
             "orc_mesh_get_verts": (None, [_P, _dp, _dp]),
+            "orc_mesh_set_slots": (None, [_P, _ip, _ip, _dp, _dp, _dp, _dp, _dp]),
             "orc_mesh_incircle": (None, [_P, _dp]),
             "orc_smf_create": (_P, [_P, _D, _D, _D, _I, _I, _I, _D]),
             "orc_smf_free": (None, [_P]),
@@ -64,6 +65,11 @@ def lib():
             "orc_smf_compute_dt": (_D, [_P, _I]),
             "orc_smf_compute_grad": (None, [_P, _I]),
             "orc_smf_totals": (None, [_P, _dp]),
+            "orc_smf_set_flags": (None, [_P, _I, _D, _I]),
+            "orc_mesh_set_flags": (None, [_P, _ip]),
+            "orc_mesh_balance_flags": (None, [_P, _I]),
+            "orc_mesh_refine": (_I, [_P, _P, _I, _I]),
+            "orc_mesh_get_next": (None, [_P, _ip]),
             "orc_splitmix_uniform": (_D, [C.POINTER(C.c_uint64)]),
             "orc_init_quadrants": (None, [_P, C.c_uint64, _D, _dp, _dp, _dp, _dp, _dp]),
         }
@@ -162,6 +168,12 @@ class Mesh:
         lib().orc_mesh_get_slots(self.h, _i(b), _i(a), _d(nx), _d(ny), _d(fx), _d(fy), _d(ar))
         return dict(boundary=b, adj=a, nx=nx, ny=ny, fcx=fx, fcy=fy, area=ar)
 
+    def set_slots(self, sl):
+        a = {k: np.ascontiguousarray(sl[k], dtype=np.int32 if k in ("boundary", "adj") else np.float64)
+             for k in ("boundary", "adj", "nx", "ny", "fcx", "fcy", "area")}
+        lib().orc_mesh_set_slots(self.h, _i(a["boundary"]), _i(a["adj"]), _d(a["nx"]), _d(a["ny"]),
+                                 _d(a["fcx"]), _d(a["fcy"]), _d(a["area"]))
+
     def verts(self):
         nv = 9 * self.n
         vx, vy = np.zeros(nv), np.zeros(nv)
@@ -172,6 +184,16 @@ class Mesh:
         h = np.zeros(self.n)
         lib().orc_mesh_incircle(self.h, _d(h))
         return h
+
+    def set_flags(self, flags):
+        f = np.ascontiguousarray(flags, dtype=np.int32)
+        lib().orc_mesh_set_flags(self.h, _i(f))
+
+    def balance_flags(self, max_level):
+        lib().orc_mesh_balance_flags(self.h, max_level)
+
+    def refine(self, max_level, solver=None, slope=True):
+        return lib().orc_mesh_refine(self.h, solver.h if solver is not None else None, max_level, int(slope))
 
     def quadrant_state(self, seed, gamma=1.4):
         n = self.n
@@ -217,6 +239,13 @@ class SmFluid:
         dx, dy = np.zeros(4 * n), np.zeros(4 * n)
         lib().orc_smf_get_grad(self.h, _d(dx), _d(dy))
         return dx.reshape(4, n), dy.reshape(4, n)
+
+    def set_flags(self, criterion=0, threshold=0.1, nthreads=1):
+        lib().orc_smf_set_flags(self.h, criterion, threshold, nthreads)
+
+    def refine(self, max_level, slope=True):
+        """EuMesh::refine with this solver's distributor."""
+        return self.mesh.refine(max_level, self, slope)
 
     def set_max_dt(self, v):
         lib().orc_smf_set_max_dt(self.h, v)

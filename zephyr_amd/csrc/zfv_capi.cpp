@@ -74,8 +74,9 @@ struct zfv_ctx {
     HostMesh hm;
     bool have_mesh = false, have_state = false, configured = false;
 
-    // device mesh
-    DBuf<double> cx, cy, vol, h, area, fcx, fcy, w;
+    // device mesh (capacity `cap` cells; slot-major arrays have stride cap)
+    int cap = 0;
+    DBuf<double> cx, cy, vol, h, area, fcx, fcy, w, nx, ny;
     DBuf<int> code, adj;
     // device state
     DBuf<double> st[3][5];   // init / next / half layers
@@ -97,8 +98,10 @@ struct zfv_ctx {
     DevMesh dmesh() const {
         DevMesh M;
         M.n = hm.n;
+        M.stride = cap;
         M.cx = cx.p; M.cy = cy.p; M.vol = vol.p; M.h = h.p;
         M.code = code.p; M.adj = adj.p; M.area = area.p; M.fcx = fcx.p; M.fcy = fcy.p; M.w = w.p;
+        M.nx = nx.p; M.ny = ny.p;
         return M;
     }
     DevState layer(int i) const {
@@ -148,6 +151,7 @@ int upload_mesh(zfv_ctx* ctx) {
     const size_t ns = 8 * (size_t)n;
     std::vector<int> code(n, 0), adj(ns, 0);
     std::vector<double> area(ns, 0.0), fcx(ns, 0.0), fcy(ns, 0.0), w(ns, 0.0), h(n, 0.0);
+    std::vector<double> nxs(ns, 0.0), nys(ns, 0.0);
 
     for (int c = 0; c < n; ++c) {
         int cw = 0;
@@ -155,16 +159,18 @@ int upload_mesh(zfv_ctx* ctx) {
             size_t i = 8 * (size_t)c + s;
             int t = code_of_boundary(m.bnd[i]);
             if (t < 0) return fail(ZFV_ERR_ARG, "unknown boundary id " + std::to_string(m.bnd[i]));
-            cw |= t << (4 * s);
             size_t j = (size_t)s * n + c;   // slot-major
             if (t == SC_UNDEF) {
                 adj[j] = c;
                 continue;
             }
-            int side = s & 3;
-            if (m.nx[i] != kNX[side] || m.ny[i] != kNY[side])
-                return fail(ZFV_ERR_UNSUPPORTED, "cell " + std::to_string(c) + " slot " + std::to_string(s) +
-                                                     ": normal is not the canonical AMR-2D side normal");
+            const int side = s & 3;
+            const double nx = m.nx[i], ny = m.ny[i];
+            if (!std::isfinite(nx) || !std::isfinite(ny) || !(nx * nx + ny * ny > 0.0))
+                return fail(ZFV_ERR_ARG, "cell " + std::to_string(c) + " slot " + std::to_string(s) +
+                                             ": degenerate normal");
+            const bool gen = !(nx == kNX[side] && ny == kNY[side]);
+            cw |= (t | (gen ? kGeneralNormal : 0)) << (4 * s);
             int nb = m.adj[i];
             if (nb < 0 || nb >= n) return fail(ZFV_ERR_ARG, "adjacent index out of range");
             if ((t == SC_WALL || t == SC_COPY) && nb != c)
@@ -173,27 +179,30 @@ int upload_mesh(zfv_ctx* ctx) {
             area[j] = m.area[i];
             fcx[j] = m.fcx[i];
             fcy[j] = m.fcy[i];
+            nxs[j] = nx;
+            nys[j] = ny;
             // LSM weight w = area / |drn|^3, geometry-only (gradient.h:251-256):
-            // dr = (INNER ? neib_c : symm_point(cell_c)) - cell_c; drn keeps
-            // the normal component of dr for a canonical normal.
+            // dr = (INNER ? neib_c : symm_point(cell_c)) - cell_c, drn = (dr.n) n.
+            const double cx = m.cx[c], cy = m.cy[c];
             double px, py;
             if (t == SC_INNER) {
                 px = m.cx[nb];
                 py = m.cy[nb];
-            } else if (side < 2) {
-                px = m.cx[c] + 2.0 * (m.fcx[i] - m.cx[c]);
-                py = m.cy[c];
             } else {
-                px = m.cx[c];
-                py = m.cy[c] + 2.0 * (m.fcy[i] - m.cy[c]);
+                double d2 = 2.0 * ((m.fcx[i] - cx) * nx + (m.fcy[i] - cy) * ny);
+                px = cx + d2 * nx;
+                py = cy + d2 * ny;
             }
-            double dn = side < 2 ? (px - m.cx[c]) : (py - m.cy[c]);
-            double len = std::sqrt(dn * dn);
+            const double drx = px - cx, dry = py - cy;
+            const double d = drx * nx + dry * ny;
+            const double dnx = d * nx, dny = d * ny;
+            const double len = std::sqrt(dnx * dnx + dny * dny);
             w[j] = (1.0 * m.area[i]) / std::pow(len, 3);
         }
         code[c] = cw;
         h[c] = incircle_2d(&m.vx[9 * (size_t)c], &m.vy[9 * (size_t)c]);
     }
+    ctx->cap = n;
 
     hipStream_t s = ctx->stream;
     auto put = [&](auto& buf, const auto& host) -> hipError_t {
@@ -211,6 +220,8 @@ int upload_mesh(zfv_ctx* ctx) {
     ZFV_HIP(put(ctx->fcx, fcx));
     ZFV_HIP(put(ctx->fcy, fcy));
     ZFV_HIP(put(ctx->w, w));
+    ZFV_HIP(put(ctx->nx, nxs));
+    ZFV_HIP(put(ctx->ny, nys));
     for (int l = 0; l < 3; ++l)
         for (int k = 0; k < 5; ++k) ZFV_HIP(ctx->st[l][k].alloc(n));
     ZFV_HIP(ctx->gdx.alloc(4 * (size_t)n));

@@ -8,22 +8,31 @@
 
 namespace zfv {
 
-// Device mesh: cell arrays [n], slot arrays slot-major [8][n] so that a
-// wavefront of 64 consecutive cells reads slot s of each cell from 64
-// consecutive addresses.  code[c] packs the 8 slot codes, 4 bits each.
+// Device mesh: cell arrays [cap], slot arrays slot-major [8][cap] (slot s
+// of cell c at s*stride + c) so that a wavefront of 64 consecutive cells
+// reads slot s of each cell from 64 consecutive addresses.  code[c] packs the
+// 8 slot nibbles: bits 0-2 the slot type (SlotCode), bit 3 set when the
+// slot's normal is not exactly the canonical side normal (AMR faces whose
+// vertices carry round-off); such slots read nx/ny and use the general
+// rotation.
 struct DevMesh {
     int n;
+    int stride;
     const double* __restrict__ cx;
     const double* __restrict__ cy;
     const double* __restrict__ vol;
     const double* __restrict__ h;       // incircle diameter
     const int* __restrict__ code;
-    const int* __restrict__ adj;        // [8][n]
-    const double* __restrict__ area;    // [8][n]
-    const double* __restrict__ fcx;     // [8][n]
-    const double* __restrict__ fcy;     // [8][n]
-    const double* __restrict__ w;       // [8][n] LSM weight area / |drn|^3
+    const int* __restrict__ adj;        // [8][stride]
+    const double* __restrict__ area;    // [8][stride]
+    const double* __restrict__ fcx;     // [8][stride]
+    const double* __restrict__ fcy;     // [8][stride]
+    const double* __restrict__ w;       // [8][stride] LSM weight area / |drn|^3
+    const double* __restrict__ nx;      // [8][stride] outward normal
+    const double* __restrict__ ny;
 };
+
+constexpr int kGeneralNormal = 8;
 
 struct DevState {
     double* r;
@@ -34,7 +43,7 @@ struct DevState {
 };
 
 struct DevGrad {
-    double* dx;   // [4][n]: rho, u, v, p
+    double* dx;   // [4][stride]: rho, u, v, p
     double* dy;
 };
 

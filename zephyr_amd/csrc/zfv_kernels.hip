@@ -80,24 +80,22 @@ __device__ __forceinline__ void block_min_atomic(unsigned long long* slot, doubl
     }
 }
 
-// EuFace::symm_point (mesh/euler/amr_faces.h:197-199) for a canonical normal:
-// p + 2 ((f - p).n) n reduces to p + 2 (f - p) along the normal axis.
-__device__ __forceinline__ void symm_point(int side, double px, double py, double fx, double fy,
-                                           double& sx, double& sy) {
-    if (side < 2) {
-        sx = px + 2.0 * (fx - px);
-        sy = py;
-    } else {
-        sx = px;
-        sy = py + 2.0 * (fy - py);
+// Frame of slot s of cell c (code nibble `nib`).
+__device__ __forceinline__ Frame slot_frame(const DevMesh& M, int c, int s, int nib) {
+    const bool gen = (nib & kGeneralNormal) != 0;
+    double nx = 0.0, ny = 0.0;
+    if (gen) {
+        nx = M.nx[s * M.stride + c];
+        ny = M.ny[s * M.stride + c];
     }
+    return frame_of(s & 3, gen, nx, ny);
 }
 
 // Neighbour state through a slot and the LSM offset dr (gradient.h:245-252).
-__device__ __forceinline__ void lsm_neib(const DevMesh& M, const DevState& in, int n, int c, int s,
-                                         int t, int side, const Z& zc, double ccx, double ccy,
+__device__ __forceinline__ void lsm_neib(const DevMesh& M, const DevState& in, int c, int s,
+                                         int t, const Frame& f, const Z& zc, double ccx, double ccy,
                                          Z& zn, double& drx, double& dry) {
-    const int nb = M.adj[s * n + c];
+    const int nb = M.adj[s * M.stride + c];
     if (t == SC_INNER || t == SC_PERIODIC) {
         zn.r = in.r[nb];
         zn.u = in.u[nb];
@@ -105,17 +103,30 @@ __device__ __forceinline__ void lsm_neib(const DevMesh& M, const DevState& in, i
         zn.p = in.p[nb];
         zn.e = 0.0;
     } else {
-        zn = boundary_value(zc, side, t == SC_WALL);
+        zn = frame_boundary(f, zc, t == SC_WALL);
     }
     double px, py;
     if (t == SC_INNER) {
         px = M.cx[nb];
         py = M.cy[nb];
     } else {
-        symm_point(side, ccx, ccy, M.fcx[s * n + c], M.fcy[s * n + c], px, py);
+        frame_symm(f, ccx, ccy, M.fcx[s * M.stride + c], M.fcy[s * M.stride + c], px, py);
     }
     drx = px - ccx;
     dry = py - ccy;
+}
+
+// drn = (dr.n) n of the LSM (gradient.h:254); a canonical normal keeps one
+// component of dr exactly.
+__device__ __forceinline__ void lsm_drn(const Frame& f, double drx, double dry, double& dnx, double& dny) {
+    if (!f.gen) {
+        dnx = f.side < 2 ? drx : 0.0;
+        dny = f.side < 2 ? 0.0 : dry;
+        return;
+    }
+    double d = drx * f.nx + dry * f.ny;
+    dnx = d * f.nx;
+    dny = d * f.ny;
 }
 
 // FaceExtraDirect::m/p (math/cfd/face_extra.h:143-177) on {rho, u, v, p};
@@ -160,15 +171,16 @@ __global__ void __launch_bounds__(kBlock) k_flux1(DevMesh M, DevState in, DevSta
         bool bad = false;
 #pragma unroll
         for (int s = 0; s < 8; ++s) {
-            const int t = (code >> (4 * s)) & 15;
+            const int nib = (code >> (4 * s)) & 15;
+            const int t = nib & 7;
             if (t == SC_UNDEF) continue;
-            const int side = s & 3;
+            const Frame fr = slot_frame(M, c, s, nib);
             Z zn;
-            if (t <= SC_PERIODIC) zn = load_z(in, M.adj[s * n + c]);
-            else zn = boundary_value(zc, side, t == SC_WALL);
-            Q f = num_flux<FLUX>(to_local(zc, side), to_local(zn, side), eos, bad);
-            f = flux_to_global(f, side);
-            const double a = M.area[s * n + c];
+            if (t <= SC_PERIODIC) zn = load_z(in, M.adj[s * M.stride + c]);
+            else zn = frame_boundary(fr, zc, t == SC_WALL);
+            Q f = num_flux<FLUX>(frame_local(fr, zc), frame_local(fr, zn), eos, bad);
+            f = frame_global(fr, f);
+            const double a = M.area[s * M.stride + c];
             F0 = F0 + f.r * a;
             F1 = F1 + f.mx * a;
             F2 = F2 + f.my * a;
@@ -215,26 +227,29 @@ __global__ void __launch_bounds__(kBlock) k_grad_pred(DevMesh M, DevState in, De
     double A00 = 0.0, A01 = 0.0, A10 = 0.0, A11 = 0.0;
     double Fx[4] = {0.0, 0.0, 0.0, 0.0}, Fy[4] = {0.0, 0.0, 0.0, 0.0};
     for (int s = 0; s < 8; ++s) {
-        const int t = (code >> (4 * s)) & 15;
+        const int nib = (code >> (4 * s)) & 15;
+        const int t = nib & 7;
         if (t == SC_UNDEF) continue;
-        const int side = s & 3;
+        const Frame fr = slot_frame(M, c, s, nib);
         Z zn;
-        double drx, dry;
-        lsm_neib(M, in, n, c, s, t, side, zc, ccx, ccy, zn, drx, dry);
-        const double w = M.w[s * n + c];
+        double drx, dry, dnx, dny;
+        lsm_neib(M, in, c, s, t, fr, zc, ccx, ccy, zn, drx, dry);
+        lsm_drn(fr, drx, dry, dnx, dny);
+        const double w = M.w[s * M.stride + c];
         const double dF[4] = {zn.r - zc.r, zn.u - zc.u, zn.v - zc.v, zn.p - zc.p};
-        if (side < 2) {
-            const double wd = w * drx;
+        if (fr.gen || fr.side < 2) {
+            const double wd = w * dnx;
             A00 = A00 + wd * drx;
             A01 = A01 + wd * dry;
 #pragma unroll
-            for (int k = 0; k < 4; ++k) Fx[k] = Fx[k] + (w * dF[k]) * drx;
-        } else {
-            const double wd = w * dry;
+            for (int k = 0; k < 4; ++k) Fx[k] = Fx[k] + (w * dF[k]) * dnx;
+        }
+        if (fr.gen || fr.side >= 2) {
+            const double wd = w * dny;
             A10 = A10 + wd * drx;
             A11 = A11 + wd * dry;
 #pragma unroll
-            for (int k = 0; k < 4; ++k) Fy[k] = Fy[k] + (w * dF[k]) * dry;
+            for (int k = 0; k < 4; ++k) Fy[k] = Fy[k] + (w * dF[k]) * dny;
         }
     }
     if (fabs(A11) < 1e-8) A11 = 1.0;   // A(2,2) = 0 -> 1 always holds in 2D (gradient.h:268-273)
@@ -252,29 +267,31 @@ __global__ void __launch_bounds__(kBlock) k_grad_pred(DevMesh M, DevState in, De
     // limiting: limiter(2 du_n - dF_p, dF_p) * dF_p re-solved with the same A
     double Lx[4] = {0.0, 0.0, 0.0, 0.0}, Ly[4] = {0.0, 0.0, 0.0, 0.0};
     for (int s = 0; s < 8; ++s) {
-        const int t = (code >> (4 * s)) & 15;
+        const int nib = (code >> (4 * s)) & 15;
+        const int t = nib & 7;
         if (t == SC_UNDEF) continue;
-        const int side = s & 3;
+        const Frame fr = slot_frame(M, c, s, nib);
         Z zn;
-        double drx, dry;
-        lsm_neib(M, in, n, c, s, t, side, zc, ccx, ccy, zn, drx, dry);
-        const double w = M.w[s * n + c];
+        double drx, dry, dnx, dny;
+        lsm_neib(M, in, c, s, t, fr, zc, ccx, ccy, zn, drx, dry);
+        lsm_drn(fr, drx, dry, dnx, dny);
+        const double w = M.w[s * M.stride + c];
         const double dFp[4] = {zn.r - zc.r, zn.u - zc.u, zn.v - zc.v, zn.p - zc.p};
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
             const double du_n = gx[k] * drx + gy[k] * dry;
             const double dFm = 2.0 * du_n - dFp[k];
             const double lim = limiter2<LIM>(dFm, dFp[k]) * dFp[k];
-            if (side < 2) Lx[k] = Lx[k] + (w * lim) * drx;
-            else          Ly[k] = Ly[k] + (w * lim) * dry;
+            if (fr.gen || fr.side < 2) Lx[k] = Lx[k] + (w * lim) * dnx;
+            if (fr.gen || fr.side >= 2) Ly[k] = Ly[k] + (w * lim) * dny;
         }
     }
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
         gx[k] = B00 * Lx[k] + B01 * Ly[k];
         gy[k] = B10 * Lx[k] + B11 * Ly[k];
-        G.dx[k * n + c] = gx[k];
-        G.dy[k * n + c] = gy[k];
+        G.dx[k * M.stride + c] = gx[k];
+        G.dy[k * M.stride + c] = gy[k];
     }
     if (!PRED) return;
 
@@ -283,17 +300,17 @@ __global__ void __launch_bounds__(kBlock) k_grad_pred(DevMesh M, DevState in, De
     if (wait[c] <= 0) {
         double F0 = 0.0, F1 = 0.0, F2 = 0.0, F3 = 0.0;
         for (int s = 0; s < 8; ++s) {
-            const int t = (code >> (4 * s)) & 15;
-            if (t == SC_UNDEF) continue;
-            const int side = s & 3;
-            const double dx = M.fcx[s * n + c] - ccx;
-            const double dy = M.fcy[s * n + c] - ccy;
+            const int nib = (code >> (4 * s)) & 15;
+            if ((nib & 7) == SC_UNDEF) continue;
+            const Frame fr = slot_frame(M, c, s, nib);
+            const double dx = M.fcx[s * M.stride + c] - ccx;
+            const double dy = M.fcy[s * M.stride + c] - ccy;
             Z zm = extrap(zc, gx, gy, dx, dy);
             zm.e = eos_e(eos, zm.r, zm.p);
             if (is_bad(zm, eos)) zm = zc;
-            Q f = phys_flux(to_local(zm, side));
-            f = flux_to_global(f, side);
-            const double a = M.area[s * n + c];
+            Q f = phys_flux(frame_local(fr, zm));
+            f = frame_global(fr, f);
+            const double a = M.area[s * M.stride + c];
             F0 = F0 + f.r * a;
             F1 = F1 + f.mx * a;
             F2 = F2 + f.my * a;
@@ -332,26 +349,27 @@ __global__ void __launch_bounds__(kBlock) k_corr(DevMesh M, DevState init, DevSt
         double gx[4], gy[4];
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
-            gx[k] = G.dx[k * n + c];
-            gy[k] = G.dy[k * n + c];
+            gx[k] = G.dx[k * M.stride + c];
+            gy[k] = G.dy[k * M.stride + c];
         }
         double F0 = 0.0, F1 = 0.0, F2 = 0.0, F3 = 0.0;
         bool bad = false;
         for (int s = 0; s < 8; ++s) {
-            const int t = (code >> (4 * s)) & 15;
+            const int nib = (code >> (4 * s)) & 15;
+            const int t = nib & 7;
             if (t == SC_UNDEF) continue;
-            const int side = s & 3;
-            const int nb = M.adj[s * n + c];
+            const Frame fr = slot_frame(M, c, s, nib);
+            const int nb = M.adj[s * M.stride + c];
             const bool bound = t >= SC_WALL;
             const bool wall = t == SC_WALL;
-            const Z znh = bound ? boundary_value(zch, side, wall) : load_z(half, nb);
+            const Z znh = bound ? frame_boundary(fr, zch, wall) : load_z(half, nb);
             const bool robust = wc > 0 || wait_in[nb] > 0;
             Z zm, zp;
             if (robust) {
                 zm = zch;
                 zp = znh;
             } else {
-                const double fx = M.fcx[s * n + c], fy = M.fcy[s * n + c];
+                const double fx = M.fcx[s * M.stride + c], fy = M.fcy[s * M.stride + c];
                 zm = extrap(zch, gx, gy, fx - ccx, fy - ccy);
                 zm.e = eos_e(eos, zm.r, zm.p);
                 if (is_bad(zm, eos)) zm = zch;
@@ -361,25 +379,25 @@ __global__ void __launch_bounds__(kBlock) k_corr(DevMesh M, DevState init, DevSt
                         nx = M.cx[nb];
                         ny = M.cy[nb];
                     } else {
-                        symm_point(side, ccx, ccy, fx, fy, nx, ny);
+                        frame_symm(fr, ccx, ccy, fx, fy, nx, ny);
                     }
                     double ngx[4], ngy[4];
 #pragma unroll
                     for (int k = 0; k < 4; ++k) {
-                        ngx[k] = G.dx[k * n + nb];
-                        ngy[k] = G.dy[k * n + nb];
+                        ngx[k] = G.dx[k * M.stride + nb];
+                        ngy[k] = G.dy[k * M.stride + nb];
                     }
                     zp = extrap(znh, ngx, ngy, fx - nx, fy - ny);
                     zp.e = eos_e(eos, zp.r, zp.p);
                     if (is_bad(zp, eos)) zp = znh;
                 } else {
-                    zp = boundary_value(zm, side, wall);
+                    zp = frame_boundary(fr, zm, wall);
                 }
             }
-            const Z lm = to_local(zm, side), lp = to_local(zp, side);
+            const Z lm = frame_local(fr, zm), lp = frame_local(fr, zp);
             Q f = robust ? flux_hll(lm, lp, eos, bad) : num_flux<FLUX>(lm, lp, eos, bad);
-            f = flux_to_global(f, side);
-            const double a = M.area[s * n + c];
+            f = frame_global(fr, f);
+            const double a = M.area[s * M.stride + c];
             F0 = F0 + f.r * a;
             F1 = F1 + f.mx * a;
             F2 = F2 + f.my * a;

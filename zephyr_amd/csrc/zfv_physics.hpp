@@ -134,6 +134,85 @@ __device__ __forceinline__ Z boundary_value(const Z& z, int side, bool wall) {
     return r;
 }
 
+// ---- face frames ------------------------------------------------------------
+// A slot's frame is its canonical side normal, or, for AMR faces whose
+// vertices carry round-off, its stored normal n.  The general formulas are
+// the reference's with n_z = 0 and Eigen's order fixed as
+// ((a0 b0 + a1 b1) + a2 b2): Rotate (rotate.cpp:7-53) with c, s from
+// n.normalized(); boundary_value v - 2 (v.n) n (sm_fluid.cpp:69-79);
+// symm_point p + (2 (f - p).n) n (amr_faces.h:197-199); drn = (dr.n) n
+// (gradient.h:254).  For a canonical normal they give the same values as the
+// side-specialised maps above.
+struct Frame {
+    int side;
+    bool gen;
+    double nx, ny;   // stored normal
+    double c, s;     // normalised normal
+};
+
+__device__ __forceinline__ Frame frame_of(int side, bool gen, double nx, double ny) {
+    Frame f;
+    f.side = side;
+    f.gen = gen;
+    f.nx = nx;
+    f.ny = ny;
+    f.c = nx;
+    f.s = ny;
+    if (gen) {
+        double z = nx * nx + ny * ny;
+        if (z > 0.0) {
+            double r = sqrt(z);
+            f.c = nx / r;
+            f.s = ny / r;
+        }
+    }
+    return f;
+}
+
+__device__ __forceinline__ Z frame_local(const Frame& f, const Z& z) {
+    if (!f.gen) return to_local(z, f.side);
+    Z r = z;
+    r.u = f.c * z.u + f.s * z.v;
+    r.v = (-f.s) * z.u + f.c * z.v;
+    return r;
+}
+
+__device__ __forceinline__ Q frame_global(const Frame& f, const Q& q) {
+    if (!f.gen) return flux_to_global(q, f.side);
+    Q r = q;
+    r.mx = f.c * q.mx + (-f.s) * q.my;
+    r.my = f.s * q.mx + f.c * q.my;
+    return r;
+}
+
+__device__ __forceinline__ Z frame_boundary(const Frame& f, const Z& z, bool wall) {
+    if (!f.gen) return boundary_value(z, f.side, wall);
+    Z r = z;
+    if (wall) {
+        double d = z.u * f.nx + z.v * f.ny;
+        r.u = z.u - 2.0 * (f.nx * d);
+        r.v = z.v - 2.0 * (f.ny * d);
+    }
+    return r;
+}
+
+__device__ __forceinline__ void frame_symm(const Frame& f, double px, double py, double fx, double fy,
+                                           double& sx, double& sy) {
+    if (!f.gen) {
+        if (f.side < 2) {
+            sx = px + 2.0 * (fx - px);
+            sy = py;
+        } else {
+            sx = px;
+            sy = py + 2.0 * (fy - py);
+        }
+        return;
+    }
+    double d2 = 2.0 * ((fx - px) * f.nx + (fy - py) * f.ny);
+    sx = px + d2 * f.nx;
+    sy = py + d2 * f.ny;
+}
+
 // ---- numerical fluxes in the face frame ------------------------------------
 // HLLC::calc_flux, smf (math/cfd/flux/hllc.cpp:23-83).
 __device__ __forceinline__ Q flux_hllc(const Z& L, const Z& R, const Eos& eos, bool& nan_out) {
