@@ -95,6 +95,8 @@ def main():
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--cpu-budget", type=float, default=10.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="also run device-to-device copies of the state (known bytes) for HBM counter scaling")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -113,6 +115,20 @@ def main():
 
     ctx.step(args.warmup)
     ctx.synchronize()
+
+    if args.copy_calibration:
+        # 5 device-to-device copies of n doubles (the init layer) into fresh
+        # buffers: 8n bytes read and 8n bytes written per copy dispatch.
+        import ctypes as C
+        hip = C.CDLL("libamdhip64.so")
+        bufs = [C.c_void_p() for _ in range(5)]
+        for b in bufs:
+            if hip.hipMalloc(C.byref(b), C.c_size_t(8 * n)) != 0:
+                raise RuntimeError("hipMalloc failed")
+        for _ in range(3):
+            Z._check(Z.lib().zfv_state_download_device(ctx.h, *bufs))
+        for b in bufs:
+            hip.hipFree(b)
 
     ctx.profile_enable(True)
     ctx.synchronize()

@@ -47,9 +47,10 @@ enum { ZFV_UNDEFINED = -1, ZFV_INNER = 0, ZFV_WALL = 1, ZFV_ZOE = 2, ZFV_PERIODI
  * (mesh/euler/amr_cells.h:74-104, amr_faces.h:83-101): n cells, 8 face slots
  * per cell (slot s of cell c at index 8*c + s, sides L,R,B,T then their
  * second sub-faces), 9 vertices per cell in SqQuad order (geom/primitives/quad.h:186-191).
- * Supported class: Cartesian AMR-2D (every defined slot normal is exactly
- * -e_x, +e_x, -e_y, +e_y for sides L, R, B, T); anything else is rejected
- * with ZFV_ERR_UNSUPPORTED. */
+ * Normals are arbitrary unit vectors in the plane; slots whose normal is
+ * exactly the canonical side normal (-e_x, +e_x, -e_y, +e_y for L, R, B, T)
+ * take an exact specialised path.  The adaptation entry points assume a
+ * Cartesian base (adjacency rotation 0, as Rectangle builds). */
 typedef struct {
     int32_t n_cells;
     const double* cx;        /* [n] cell centre                                    */
@@ -142,6 +143,23 @@ int zfv_smf_dt(zfv_ctx* ctx, double* dt);
 int zfv_smf_compute_grad(zfv_ctx* ctx);
 /* Only the CFL reduction (compute_dt) on the current state. */
 int zfv_smf_compute_dt(zfv_ctx* ctx, double* dt);
+
+/* ---- adaptation (AMR-2D, one process, Cartesian base) ------------------- */
+/* Refinement flags per cell (AmrCells::flag, amr_cells.h:81): upload / download. */
+int zfv_flags_upload(zfv_ctx* ctx, const int32_t* flags);
+int zfv_flags_download(zfv_ctx* ctx, int32_t* flags);
+/* SmFluid::set_flags (sm_fluid.cpp:517-643): limited gradient, then flags.
+ * criterion 0: the reference's relative jumps of density and pressure
+ * (xi = 0.05, hysteresis 0.4); criterion 1: |grad rho| h / rho of the limited
+ * gradient against `threshold` (+1 above, 0 above 0.4 threshold, else -1). */
+int zfv_set_flags(zfv_ctx* ctx, int criterion, double threshold);
+/* amr::balance_flags (mesh/amr/balancing.h:15-35, balancing_fast.h:412-451). */
+int zfv_balance_flags(zfv_ctx* ctx, int max_level);
+/* EuMesh::refine (eu_mesh.cpp:290-359): balance, then amr::apply
+ * (mesh/amr/apply.h:43-94) on the device, moving the state with the SmFluid
+ * distributor (sm_fluid.cpp:372-470): distributor 0 = "const", 1 = "slope"
+ * (uses the gradient of the last zfv_set_flags / zfv_smf_compute_grad). */
+int zfv_refine(zfv_ctx* ctx, int max_level, int distributor);
 
 /* Conserved totals {mass, mom_x, mom_y, energy} = sum V*q (synchronises). */
 int zfv_conserved_totals(zfv_ctx* ctx, double* out4);

@@ -1,0 +1,142 @@
+"""Device AMR (zfv_set_flags / zfv_balance_flags / zfv_refine) against the
+oracle's restatement of zephyr's adaptation.
+
+Bar: refinement flags, cell order, levels, z/b indices, adjacency, boundary
+ids and all geometry (vertices, centres, volumes, face centres, areas,
+normals) bit-exact; states bit-exact through the "const" distributor and
+the conservative merge.  Fields after MUSCL steps on an adapted mesh agree
+within 1e-12 relative: the device derives the LSM weights area / |drn|^3 of
+adapted meshes with a correctly rounded cube, while glibc's pow(x, 3) (the
+reference's std::pow, gradient.h:256) is off by one ulp for about 0.1 % of
+arguments.
+"""
+import numpy as np
+import pytest
+
+from amr_checks import check_mesh
+
+pytestmark = pytest.mark.gpu
+
+GAMMA = 1.4
+
+
+def compare_meshes(ctx, m, what):
+    cells, slots, verts = ctx.mesh_download()
+    oc, osl = m.cells(), m.slots()
+    assert len(cells["cx"]) == m.n, f"{what}: cell count {len(cells['cx'])} vs {m.n}"
+    for k in ("level", "b_idx", "z_idx", "cx", "cy", "volume"):
+        bad = np.flatnonzero(cells[k] != oc[k])
+        assert bad.size == 0, f"{what}: {k} differs at cells {bad[:5]}"
+    assert np.array_equal(slots["boundary"], osl["boundary"]), f"{what}: boundary ids"
+    defined = osl["boundary"] >= 0
+    for k in ("adj", "nx", "ny", "fcx", "fcy", "area"):
+        bad = np.flatnonzero((slots[k] != osl[k]) & defined)
+        assert bad.size == 0, f"{what}: slot {k} differs at {bad[:5]}"
+    ovx, ovy = m.verts()
+    assert np.array_equal(verts[0], ovx) and np.array_equal(verts[1], ovy), f"{what}: vertices"
+    check_mesh(cells, slots)
+
+
+def pair(oracle, zfv, nx, ny, bounds=(2, 2, 2, 2), acc=2, limiter="MC"):
+    m = oracle.Mesh.rectangle(nx, ny, bounds=bounds)
+    s = oracle.SmFluid(m, acc=acc, flux=3, limiter=limiter)
+    ctx = zfv.Context()
+    ctx.mesh_rectangle(nx, ny, bounds=bounds)
+    ctx.configure(acc=acc, flux=3, limiter=limiter, cfl=0.5, gamma=GAMMA)
+    return m, s, ctx
+
+
+@pytest.mark.parametrize("seed", [0, 1])
+def test_random_flags_refine_bit_exact(oracle, zfv, seed):
+    """tests/amr/random.cpp-style stress: random flags, balance, apply."""
+    m, s, ctx = pair(oracle, zfv, 12, 10, bounds=(2, 1, 3, 3))
+    st = m.quadrant_state(seed + 5)
+    s.set_state(*st)
+    ctx.set_state(*st)
+    rng = np.random.default_rng(seed)
+    for it in range(25):
+        p = [0.2, 0.3, 0.5] if it < 5 else [0.80, 0.18, 0.02]
+        f = rng.choice([-1, 0, 1], size=m.n, p=p).astype(np.int32)
+        m.set_flags(f)
+        ctx.set_flags_array(f)
+        s.refine(3, slope=False)
+        ctx.refine(3, slope=False)
+        compare_meshes(ctx, m, f"cycle {it}")
+        got, ref = ctx.state(), s.state()
+        for a, b in zip(got, ref):
+            assert np.array_equal(a, b), f"cycle {it}: state after const split / merge"
+
+
+def test_balance_flags_bit_exact(oracle, zfv):
+    m, s, ctx = pair(oracle, zfv, 16, 16)
+    rng = np.random.default_rng(7)
+    st = m.quadrant_state(3)
+    s.set_state(*st)
+    ctx.set_state(*st)
+    for it in range(6):
+        f = rng.choice([-1, 0, 1], size=m.n, p=[0.3, 0.3, 0.4]).astype(np.int32)
+        m.set_flags(f)
+        ctx.set_flags_array(f)
+        s.refine(3, slope=False)
+        ctx.refine(3, slope=False)
+    f = rng.choice([-1, 0, 1], size=m.n, p=[0.6, 0.2, 0.2]).astype(np.int32)
+    m.set_flags(f)
+    ctx.set_flags_array(f)
+    m.balance_flags(3)
+    ctx.balance_flags(3)
+    assert np.array_equal(ctx.flags(), m.cells()["flag"])
+
+
+@pytest.mark.parametrize("criterion", [0, 1])
+def test_set_flags_bit_exact(oracle, zfv, criterion):
+    m, s, ctx = pair(oracle, zfv, 24, 20, limiter="minmod")
+    st = m.quadrant_state(11)
+    s.set_state(*st)
+    ctx.set_state(*st)
+    s.set_flags(criterion, 0.1)
+    ctx.set_flags(criterion, 0.1)
+    assert np.array_equal(ctx.flags(), m.cells()["flag"])
+
+
+def test_slope_distributor_bit_exact(oracle, zfv):
+    m, s, ctx = pair(oracle, zfv, 20, 16, limiter="minmod")
+    st = m.quadrant_state(13)
+    s.set_state(*st)
+    ctx.set_state(*st)
+    for it in range(3):
+        s.set_flags(0)
+        ctx.set_flags(0)
+        assert np.array_equal(ctx.flags(), m.cells()["flag"])
+        s.refine(2, slope=True)
+        ctx.refine(2, slope=True)
+        compare_meshes(ctx, m, f"slope cycle {it}")
+        for a, b in zip(ctx.state(), s.state()):
+            assert np.array_equal(a, b), f"slope cycle {it}: state"
+
+
+def test_adaptive_run_within_tolerance(oracle, zfv):
+    """update; set_flags; refine (smf_test_2D.cpp:141-144), MUSCL + HLLC, max level 2."""
+    m, s, ctx = pair(oracle, zfv, 24, 24, limiter="minmod")
+    st = m.quadrant_state(17)
+    s.set_state(*st)
+    ctx.set_state(*st)
+    for k in range(3):   # initial adaptation (smf_test_2D.cpp:112-117)
+        s.set_flags(0)
+        ctx.set_flags(0)
+        s.refine(2, slope=True)
+        ctx.refine(2, slope=True)
+    compare_meshes(ctx, m, "initial adaptation")
+    for it in range(12):
+        assert s.update(4) == 0
+        ctx.step(1)
+        s.set_flags(0)
+        ctx.set_flags(0)
+        assert np.array_equal(ctx.flags(), m.cells()["flag"]), f"step {it}: flags"
+        s.refine(2, slope=True)
+        ctx.refine(2, slope=True)
+        cells, slots, _ = ctx.mesh_download()
+        assert np.array_equal(cells["level"], m.cells()["level"]), f"step {it}: levels"
+        assert np.array_equal(slots["adj"][slots["boundary"] >= 0],
+                              m.slots()["adj"][m.slots()["boundary"] >= 0]), f"step {it}: adjacency"
+        for a, b in zip(ctx.state(), s.state()):
+            np.testing.assert_allclose(a, b, rtol=1e-12, atol=1e-12 * np.abs(b).max())

@@ -28,6 +28,7 @@ EXPORTS = [
     "zfv_wait_upload", "zfv_wait_download", "zfv_grad_download", "zfv_smf_step", "zfv_smf_dt",
     "zfv_smf_compute_grad", "zfv_smf_compute_dt", "zfv_conserved_totals",
     "zfv_limiter_from_name", "zfv_profile_enable", "zfv_profile_read",
+    "zfv_flags_upload", "zfv_flags_download", "zfv_set_flags", "zfv_balance_flags", "zfv_refine",
 ]
 
 
@@ -81,6 +82,9 @@ def lib():
         L.zfv_smf_set_max_dt.argtypes = [vp, d]
         L.zfv_smf_step.argtypes = [vp, C.c_int]
         L.zfv_profile_enable.argtypes = [vp, C.c_int]
+        L.zfv_set_flags.argtypes = [vp, C.c_int, d]
+        L.zfv_balance_flags.argtypes = [vp, C.c_int]
+        L.zfv_refine.argtypes = [vp, C.c_int, C.c_int]
         _lib = L
     return _lib
 
@@ -203,6 +207,30 @@ class Context:
         dx, dy = np.zeros(4 * self.n), np.zeros(4 * self.n)
         _check(lib().zfv_grad_download(self.h, _ptr(dx), _ptr(dy)))
         return dx.reshape(4, self.n), dy.reshape(4, self.n)
+
+    # ---- adaptation
+    def set_flags_array(self, flags):
+        f = np.ascontiguousarray(flags, dtype=np.int32)
+        _check(lib().zfv_flags_upload(self.h, _ptr(f)))
+
+    def flags(self):
+        n = C.c_int32()
+        _check(lib().zfv_mesh_n_cells(self.h, C.byref(n)))
+        f = np.zeros(n.value, np.int32)
+        _check(lib().zfv_flags_download(self.h, _ptr(f)))
+        return f
+
+    def set_flags(self, criterion=0, threshold=0.1):
+        _check(lib().zfv_set_flags(self.h, criterion, threshold))
+
+    def balance_flags(self, max_level):
+        _check(lib().zfv_balance_flags(self.h, max_level))
+
+    def refine(self, max_level, slope=True):
+        _check(lib().zfv_refine(self.h, max_level, 1 if slope else 0))
+        n = C.c_int32()
+        _check(lib().zfv_mesh_n_cells(self.h, C.byref(n)))
+        self.n = n.value
 
     def step(self, n=1):
         _check(lib().zfv_smf_step(self.h, n))

@@ -340,6 +340,7 @@ __global__ void __launch_bounds__(256) k_amr_swapmap(AmrMesh A, int n_short, int
     const int p = n_short + i;
     if (p >= n_large || !act_mark[i]) return;
     const int r = n_swap - act_incl[i];
+    if (r < 0 || r >= n_swap) return;   // unreachable for consistent counts (checked on the host)
     const int h = hole_at[r];
     A.next[p] = h;
     swap_src[r] = p;
@@ -381,7 +382,7 @@ __device__ void retain_cell(const AmrMesh& A, int ic) {
             } else {
                 for (int sub = side; sub < 8; sub += 4) {
                     const int jf = SI(A, ic, sub);
-                    A.adj[jf] = A.next[A.adj[jf]];
+                    if (A.bnd[jf] != B_UNDEF) A.adj[jf] = A.next[A.adj[jf]];
                 }
             }
         } else {
@@ -634,7 +635,7 @@ __device__ __forceinline__ double cube_cr(double x) {
 // Derived per-cell data after an adaptation: slot code words (type and
 // general-normal bit), LSM weights w = area / |drn|^3 (gradient.h:251-256)
 // and incircle diameters (amr_cells.cpp:107-114).
-__global__ void __launch_bounds__(256) k_amr_derive(AmrMesh A) {
+__global__ void __launch_bounds__(256) k_amr_derive(AmrMesh A, int* bad) {
     const int c = cell_id();
     if (c >= A.n) return;
     const double kNX[4] = {-1.0, 1.0, 0.0, 0.0}, kNY[4] = {0.0, 0.0, -1.0, 1.0};
@@ -659,9 +660,14 @@ __global__ void __launch_bounds__(256) k_amr_derive(AmrMesh A) {
         const double nx = A.nx[f], ny = A.ny[f];
         const bool gen = !(nx == kNX[side] && ny == kNY[side]);
         cw |= (t | (gen ? kGeneralNormal : 0)) << (4 * s);
+        const int nb = A.adj[f];
+        if (nb < 0 || nb >= A.n || (t >= SC_WALL && nb != c)) {   // inconsistent mesh: report, keep in bounds
+            A.adj[f] = c;
+            *bad = 1;
+            continue;
+        }
         double px, py;
         if (t == SC_INNER) {
-            const int nb = A.adj[f];
             px = A.cx[nb];
             py = A.cy[nb];
         } else {
@@ -753,8 +759,8 @@ hipError_t amr_move(const AmrMesh& A, const DevState& z, int* wait, int n_swap, 
     return hipGetLastError();
 }
 
-hipError_t amr_derive(const AmrMesh& A, hipStream_t s) {
-    hipLaunchKernelGGL(k_amr_derive, grid256(A.n), dim3(256), 0, s, A);
+hipError_t amr_derive(const AmrMesh& A, int* bad, hipStream_t s) {
+    hipLaunchKernelGGL(k_amr_derive, grid256(A.n), dim3(256), 0, s, A, bad);
     return hipGetLastError();
 }
 

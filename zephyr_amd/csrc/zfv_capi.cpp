@@ -76,8 +76,13 @@ struct zfv_ctx {
 
     // device mesh (capacity `cap` cells; slot-major arrays have stride cap)
     int cap = 0;
-    DBuf<double> cx, cy, vol, h, area, fcx, fcy, w, nx, ny;
-    DBuf<int> code, adj;
+    bool host_stale = false;   // hm out of date after a device-side adaptation
+    DBuf<double> cx, cy, vol, h, area, fcx, fcy, w, nx, ny, vx, vy;
+    DBuf<int> code, adj, bnd, level, flag, flag0, b_idx, z_idx, next, idx;
+    // AMR scratch
+    DBuf<int> s_ref_mark, s_crs_mark, s_hole_mark, s_ref_rank, s_crs_rank, s_hole_rank, s_hole_at;
+    DBuf<int> s_act_mark, s_act_incl, s_swap_src, s_swap_dst, s_counts;
+    DBuf<unsigned char> s_tmp;
     // device state
     DBuf<double> st[3][5];   // init / next / half layers
     DBuf<double> gdx, gdy;
@@ -103,6 +108,18 @@ struct zfv_ctx {
         M.code = code.p; M.adj = adj.p; M.area = area.p; M.fcx = fcx.p; M.fcy = fcy.p; M.w = w.p;
         M.nx = nx.p; M.ny = ny.p;
         return M;
+    }
+    AmrMesh amesh() const {
+        AmrMesh A;
+        A.n = hm.n;
+        A.stride = cap;
+        A.cx = cx.p; A.cy = cy.p; A.vol = vol.p; A.h = h.p;
+        A.code = code.p; A.adj = adj.p; A.bnd = bnd.p;
+        A.area = area.p; A.fcx = fcx.p; A.fcy = fcy.p; A.w = w.p; A.nx = nx.p; A.ny = ny.p;
+        A.vx = vx.p; A.vy = vy.p;
+        A.level = level.p; A.flag = flag.p; A.flag0 = flag0.p; A.b_idx = b_idx.p; A.z_idx = z_idx.p;
+        A.next = next.p; A.idx = idx.p;
+        return A;
     }
     DevState layer(int i) const {
         return DevState{st[i][0].p, st[i][1].p, st[i][2].p, st[i][3].p, st[i][4].p};
@@ -203,6 +220,16 @@ int upload_mesh(zfv_ctx* ctx) {
         h[c] = incircle_2d(&m.vx[9 * (size_t)c], &m.vy[9 * (size_t)c]);
     }
     ctx->cap = n;
+    std::vector<int> bnd_sm(ns), iota(n);
+    std::vector<double> vx_vm(9 * (size_t)n), vy_vm(9 * (size_t)n);
+    for (int c = 0; c < n; ++c) {
+        iota[c] = c;
+        for (int s8 = 0; s8 < 8; ++s8) bnd_sm[(size_t)s8 * n + c] = m.bnd[8 * (size_t)c + s8];
+        for (int k = 0; k < 9; ++k) {
+            vx_vm[(size_t)k * n + c] = m.vx[9 * (size_t)c + k];
+            vy_vm[(size_t)k * n + c] = m.vy[9 * (size_t)c + k];
+        }
+    }
 
     hipStream_t s = ctx->stream;
     auto put = [&](auto& buf, const auto& host) -> hipError_t {
@@ -222,6 +249,16 @@ int upload_mesh(zfv_ctx* ctx) {
     ZFV_HIP(put(ctx->w, w));
     ZFV_HIP(put(ctx->nx, nxs));
     ZFV_HIP(put(ctx->ny, nys));
+    ZFV_HIP(put(ctx->bnd, bnd_sm));
+    ZFV_HIP(put(ctx->vx, vx_vm));
+    ZFV_HIP(put(ctx->vy, vy_vm));
+    ZFV_HIP(put(ctx->level, m.level));
+    ZFV_HIP(put(ctx->flag, m.flag));
+    ZFV_HIP(put(ctx->flag0, m.flag));
+    ZFV_HIP(put(ctx->b_idx, m.b_idx));
+    ZFV_HIP(put(ctx->z_idx, m.z_idx));
+    ZFV_HIP(put(ctx->next, iota));
+    ZFV_HIP(put(ctx->idx, iota));
     for (int l = 0; l < 3; ++l)
         for (int k = 0; k < 5; ++k) ZFV_HIP(ctx->st[l][k].alloc(n));
     ZFV_HIP(ctx->gdx.alloc(4 * (size_t)n));
@@ -237,6 +274,7 @@ int upload_mesh(zfv_ctx* ctx) {
     ZFV_HIP(ctx->partial.alloc(4 * (size_t)((n + kBlock - 1) / kBlock)));
     ZFV_HIP(hipStreamSynchronize(s));
     ctx->have_mesh = true;
+    ctx->host_stale = false;
     ctx->have_state = false;
     ctx->dt_valid = false;
     ctx->cur = 0;
@@ -249,6 +287,97 @@ int read_err(zfv_ctx* ctx) {
     ZFV_HIP(hipMemcpyAsync(&e, ctx->err.p, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
     ZFV_HIP(hipStreamSynchronize(ctx->stream));
     if (e) return fail(ZFV_ERR_NUMERIC, "numeric error: a flux or Riemann solve produced NaN / did not converge");
+    return ZFV_OK;
+}
+
+// Re-allocate a [rows][cap] device array with a larger capacity, keeping
+// the first n_valid entries of every row.
+template <class T>
+hipError_t grow_rows(DBuf<T>& b, int rows, int old_cap, int new_cap, int n_valid, hipStream_t s) {
+    T* np = nullptr;
+    hipError_t e = hipMalloc(&np, (size_t)rows * new_cap * sizeof(T));
+    if (e != hipSuccess) return e;
+    if (b.p && n_valid > 0) {
+        e = hipMemcpy2DAsync(np, (size_t)new_cap * sizeof(T), b.p, (size_t)old_cap * sizeof(T),
+                             (size_t)n_valid * sizeof(T), rows, hipMemcpyDeviceToDevice, s);
+        if (e != hipSuccess) {
+            hipFree(np);
+            return e;
+        }
+        e = hipStreamSynchronize(s);
+        if (e != hipSuccess) return e;
+    }
+    b.release();
+    b.p = np;
+    b.n = (size_t)rows * new_cap;
+    return hipSuccess;
+}
+
+// Grow every capacity-sized device array to hold new_cap cells.
+int grow_capacity(zfv_ctx* ctx, int new_cap) {
+    if (new_cap <= ctx->cap) return ZFV_OK;
+    const int oc = ctx->cap, n = ctx->hm.n;
+    hipStream_t s = ctx->stream;
+    for (auto* b : {&ctx->cx, &ctx->cy, &ctx->vol, &ctx->h}) ZFV_HIP(grow_rows(*b, 1, oc, new_cap, n, s));
+    for (auto* b : {&ctx->code, &ctx->level, &ctx->flag, &ctx->flag0, &ctx->b_idx, &ctx->z_idx, &ctx->next,
+                    &ctx->idx, &ctx->wait[0], &ctx->wait[1]})
+        ZFV_HIP(grow_rows(*b, 1, oc, new_cap, n, s));
+    for (auto* b : {&ctx->area, &ctx->fcx, &ctx->fcy, &ctx->w, &ctx->nx, &ctx->ny})
+        ZFV_HIP(grow_rows(*b, 8, oc, new_cap, n, s));
+    for (auto* b : {&ctx->adj, &ctx->bnd}) ZFV_HIP(grow_rows(*b, 8, oc, new_cap, n, s));
+    for (auto* b : {&ctx->vx, &ctx->vy}) ZFV_HIP(grow_rows(*b, 9, oc, new_cap, n, s));
+    for (auto* b : {&ctx->gdx, &ctx->gdy}) ZFV_HIP(grow_rows(*b, 4, oc, new_cap, n, s));
+    for (int l = 0; l < 3; ++l)
+        for (int k = 0; k < 5; ++k) ZFV_HIP(grow_rows(ctx->st[l][k], 1, oc, new_cap, n, s));
+    ZFV_HIP(ctx->partial.alloc(4 * (size_t)((new_cap + kBlock - 1) / kBlock)));
+    ctx->cap = new_cap;
+    return ZFV_OK;
+}
+
+// Copy the device mesh back into the host AmrCells-layout copy.
+int sync_host_mesh(zfv_ctx* ctx) {
+    if (!ctx->host_stale) return ZFV_OK;
+    HostMesh& m = ctx->hm;
+    const int n = m.n, cap = ctx->cap;
+    hipStream_t s = ctx->stream;
+    auto get1 = [&](auto& host, const auto& buf) -> hipError_t {
+        host.resize(n);
+        return hipMemcpyAsync(host.data(), buf.p, (size_t)n * sizeof(host[0]), hipMemcpyDeviceToHost, s);
+    };
+    auto getr = [&](auto& host, const auto& buf, int rows) -> hipError_t {
+        host.resize((size_t)rows * n);
+        return hipMemcpy2DAsync(host.data(), (size_t)n * sizeof(host[0]), buf.p, (size_t)cap * sizeof(host[0]),
+                                (size_t)n * sizeof(host[0]), rows, hipMemcpyDeviceToHost, s);
+    };
+    ZFV_HIP(get1(m.cx, ctx->cx)); ZFV_HIP(get1(m.cy, ctx->cy)); ZFV_HIP(get1(m.vol, ctx->vol));
+    ZFV_HIP(get1(m.level, ctx->level)); ZFV_HIP(get1(m.flag, ctx->flag));
+    ZFV_HIP(get1(m.b_idx, ctx->b_idx)); ZFV_HIP(get1(m.z_idx, ctx->z_idx));
+    std::vector<int32_t> bnd, adj;
+    std::vector<double> nx, ny, fcx, fcy, area, vx, vy;
+    ZFV_HIP(getr(bnd, ctx->bnd, 8)); ZFV_HIP(getr(adj, ctx->adj, 8));
+    ZFV_HIP(getr(nx, ctx->nx, 8)); ZFV_HIP(getr(ny, ctx->ny, 8));
+    ZFV_HIP(getr(fcx, ctx->fcx, 8)); ZFV_HIP(getr(fcy, ctx->fcy, 8)); ZFV_HIP(getr(area, ctx->area, 8));
+    ZFV_HIP(getr(vx, ctx->vx, 9)); ZFV_HIP(getr(vy, ctx->vy, 9));
+    ZFV_HIP(hipStreamSynchronize(s));
+    const size_t ns = 8 * (size_t)n;
+    m.bnd.resize(ns); m.adj.resize(ns); m.nx.resize(ns); m.ny.resize(ns);
+    m.fcx.resize(ns); m.fcy.resize(ns); m.area.resize(ns);
+    m.vx.resize(9 * (size_t)n); m.vy.resize(9 * (size_t)n);
+    for (int c = 0; c < n; ++c) {
+        for (int k = 0; k < 8; ++k) {
+            size_t a = 8 * (size_t)c + k, b = (size_t)k * n + c;
+            m.bnd[a] = bnd[b]; m.adj[a] = adj[b]; m.nx[a] = nx[b]; m.ny[a] = ny[b];
+            m.fcx[a] = fcx[b]; m.fcy[a] = fcy[b]; m.area[a] = area[b];
+            if (m.bnd[a] == ZFV_UNDEFINED) {   // undefined slots carry no geometry
+                m.adj[a] = -1; m.nx[a] = 0.0; m.ny[a] = 0.0; m.fcx[a] = 0.0; m.fcy[a] = 0.0; m.area[a] = 0.0;
+            }
+        }
+        for (int k = 0; k < 9; ++k) {
+            m.vx[9 * (size_t)c + k] = vx[(size_t)k * n + c];
+            m.vy[9 * (size_t)c + k] = vy[(size_t)k * n + c];
+        }
+    }
+    ctx->host_stale = false;
     return ZFV_OK;
 }
 
@@ -493,6 +622,8 @@ int zfv_mesh_n_cells(zfv_ctx* ctx, int32_t* n) {
 int zfv_mesh_download(zfv_ctx* ctx, const zfv_mesh_soa* o) {
     if (!ctx || !o) return fail(ZFV_ERR_ARG, "null argument");
     if (!ctx->have_mesh) return fail(ZFV_ERR_STATE, "no mesh uploaded");
+    int rs = sync_host_mesh(ctx);
+    if (rs) return rs;
     const HostMesh& m = ctx->hm;
     auto cp = [](const void* dst, const void* src, size_t bytes) {
         if (dst) std::memcpy(const_cast<void*>(dst), src, bytes);
@@ -610,9 +741,9 @@ int zfv_grad_download(zfv_ctx* ctx, double* dx, double* dy) {
     int r = need_ready(ctx);
     if (r) return r;
     if (!dx || !dy) return fail(ZFV_ERR_ARG, "null output");
-    const size_t bytes = 4 * (size_t)ctx->hm.n * sizeof(double);
-    ZFV_HIP(hipMemcpyAsync(dx, ctx->gdx.p, bytes, hipMemcpyDeviceToHost, ctx->stream));
-    ZFV_HIP(hipMemcpyAsync(dy, ctx->gdy.p, bytes, hipMemcpyDeviceToHost, ctx->stream));
+    const size_t row = (size_t)ctx->hm.n * sizeof(double), pitch = (size_t)ctx->cap * sizeof(double);
+    ZFV_HIP(hipMemcpy2DAsync(dx, row, ctx->gdx.p, pitch, row, 4, hipMemcpyDeviceToHost, ctx->stream));
+    ZFV_HIP(hipMemcpy2DAsync(dy, row, ctx->gdy.p, pitch, row, 4, hipMemcpyDeviceToHost, ctx->stream));
     ZFV_HIP(hipStreamSynchronize(ctx->stream));
     return ZFV_OK;
 }
@@ -703,6 +834,138 @@ int zfv_conserved_totals(zfv_ctx* ctx, double* out4) {
     for (int b = 0; b < nb; ++b)
         for (int k = 0; k < 4; ++k) out4[k] += part[4 * (size_t)b + k];
     return read_err(ctx);
+}
+
+// ---- AMR -------------------------------------------------------------------
+int zfv_flags_upload(zfv_ctx* ctx, const int32_t* flags) {
+    int r = need_ready(ctx, false);
+    if (r) return r;
+    if (!flags) return fail(ZFV_ERR_ARG, "null flags");
+    ZFV_HIP(hipMemcpyAsync(ctx->flag.p, flags, ctx->hm.n * sizeof(int), hipMemcpyHostToDevice, ctx->stream));
+    ZFV_HIP(hipStreamSynchronize(ctx->stream));
+    ctx->host_stale = true;
+    return ZFV_OK;
+}
+
+int zfv_flags_download(zfv_ctx* ctx, int32_t* flags) {
+    int r = need_ready(ctx, false);
+    if (r) return r;
+    if (!flags) return fail(ZFV_ERR_ARG, "null flags");
+    ZFV_HIP(hipMemcpyAsync(flags, ctx->flag.p, ctx->hm.n * sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    ZFV_HIP(hipStreamSynchronize(ctx->stream));
+    return ZFV_OK;
+}
+
+// SmFluid::set_flags (sm_fluid.cpp:517-643): gradient, then the criterion.
+int zfv_set_flags(zfv_ctx* ctx, int criterion, double threshold) {
+    int r = need_ready(ctx, true, true);
+    if (r) return r;
+    if (criterion != 0 && criterion != 1) return fail(ZFV_ERR_ARG, "unknown refinement criterion");
+    r = ensure_dt(ctx);
+    if (r) return r;
+    StepCtl ctl = make_ctl(ctx);
+    DevGrad G{ctx->gdx.p, ctx->gdy.p};
+    ZFV_HIP(launch_grad_pred(ctx->prm.limiter, false, ctx->dmesh(), ctx->layer(ctx->cur), G, ctx->layer(2),
+                             ctx->wait[ctx->wcur].p, ctx->eos(), ctl, ctx->stream));
+    ZFV_HIP(amr_criterion(ctx->amesh(), ctx->layer(ctx->cur), G, criterion, threshold, ctx->stream));
+    ctx->host_stale = true;
+    return ZFV_OK;
+}
+
+// amr::balance_flags (mesh/amr/balancing.h:15-35) alone.
+int zfv_balance_flags(zfv_ctx* ctx, int max_level) {
+    int r = need_ready(ctx, false);
+    if (r) return r;
+    if (max_level < 0 || max_level > 15) return fail(ZFV_ERR_ARG, "max_level out of [0, 15]");
+    ZFV_HIP(amr_balance(ctx->amesh(), max_level, ctx->stream));
+    ctx->host_stale = true;
+    return ZFV_OK;
+}
+
+static int scan(zfv_ctx* ctx, const int* in, int* out, int count, bool inclusive) {
+    if (count <= 0) return ZFV_OK;
+    size_t bytes = 0;
+    ZFV_HIP(amr_scan(in, out, count, inclusive, nullptr, bytes, ctx->stream));
+    if (ctx->s_tmp.n < bytes) ZFV_HIP(ctx->s_tmp.alloc(bytes));
+    ZFV_HIP(amr_scan(in, out, count, inclusive, ctx->s_tmp.p, bytes, ctx->stream));
+    return ZFV_OK;
+}
+
+// EuMesh::refine (eu_mesh.cpp:290-359): balance_flags then amr::apply
+// (mesh/amr/apply.h:43-94) with the SmFluid distributor (sm_fluid.cpp:372-470).
+int zfv_refine(zfv_ctx* ctx, int max_level, int distributor) {
+    int r = need_ready(ctx, true, true);
+    if (r) return r;
+    if (max_level < 0 || max_level > 15) return fail(ZFV_ERR_ARG, "max_level out of [0, 15]");
+    if (distributor != 0 && distributor != 1) return fail(ZFV_ERR_ARG, "distributor must be 0 (const) or 1 (slope)");
+    hipStream_t s = ctx->stream;
+    if (max_level == 0) return ZFV_OK;   // EuMesh::adaptive() is false
+    ZFV_HIP(amr_balance(ctx->amesh(), max_level, s));
+    ctx->host_stale = true;
+    ZFV_HIP(ctx->s_counts.alloc(4));
+    ZFV_HIP(hipMemsetAsync(ctx->s_counts.p, 0, 4 * sizeof(int), s));
+    ZFV_HIP(amr_count(ctx->amesh(), ctx->s_counts.p, s));
+    int cnt[4];
+    ZFV_HIP(hipMemcpyAsync(cnt, ctx->s_counts.p, 4 * sizeof(int), hipMemcpyDeviceToHost, s));
+    ZFV_HIP(hipStreamSynchronize(s));
+    const int n = ctx->hm.n;
+    const int n_retain = cnt[0], n_refine = cnt[1], n_coarse = cnt[2];
+    const int n_parents = n_coarse / 4, n_children = 4 * n_refine;
+    const long long n_large_ll = (long long)n + n_parents + n_children;
+    if (n_large_ll > std::numeric_limits<int>::max() / 9) return fail(ZFV_ERR_ARG, "adapted mesh too large");
+    const int n_large = (int)n_large_ll;
+    const int n_short = n_retain + n_children + n_parents;
+    if (n_large <= n) return ZFV_OK;
+    if (n_large > ctx->cap) {
+        r = grow_capacity(ctx, std::max(n_large, ctx->cap + ctx->cap / 2));
+        if (r) return r;
+    }
+    const int na = n_large - n_short;
+    for (auto* b : {&ctx->s_ref_mark, &ctx->s_crs_mark, &ctx->s_hole_mark, &ctx->s_ref_rank, &ctx->s_crs_rank,
+                    &ctx->s_hole_rank, &ctx->s_hole_at})
+        if (b->n < (size_t)n) ZFV_HIP(b->alloc(n));
+    for (auto* b : {&ctx->s_act_mark, &ctx->s_act_incl, &ctx->s_swap_src, &ctx->s_swap_dst})
+        if (b->n < (size_t)std::max(na, 1)) ZFV_HIP(b->alloc(std::max(na, 1)));
+
+    AmrMesh A = ctx->amesh();   // A.n = n (cells before adaptation)
+    ZFV_HIP(amr_mark(A, n_short, ctx->s_ref_mark.p, ctx->s_crs_mark.p, ctx->s_hole_mark.p, s));
+    r = scan(ctx, ctx->s_ref_mark.p, ctx->s_ref_rank.p, n, false); if (r) return r;
+    r = scan(ctx, ctx->s_crs_mark.p, ctx->s_crs_rank.p, n, false); if (r) return r;
+    r = scan(ctx, ctx->s_hole_mark.p, ctx->s_hole_rank.p, n, false); if (r) return r;
+    int tail[2];
+    ZFV_HIP(hipMemcpyAsync(&tail[0], ctx->s_hole_rank.p + (n - 1), sizeof(int), hipMemcpyDeviceToHost, s));
+    ZFV_HIP(hipMemcpyAsync(&tail[1], ctx->s_hole_mark.p + (n - 1), sizeof(int), hipMemcpyDeviceToHost, s));
+    ZFV_HIP(hipStreamSynchronize(s));
+    const int n_swap = tail[0] + tail[1];
+    if (n_swap > na) return fail(ZFV_ERR_STATE, "AMR apply: inconsistent swap lists");
+    ZFV_HIP(amr_positions(A, n_large, n_parents, ctx->s_ref_rank.p, ctx->s_crs_rank.p, ctx->s_hole_mark.p,
+                          ctx->s_hole_rank.p, ctx->s_hole_at.p, s));
+    if (na > 0) {
+        ZFV_HIP(amr_actual_mark(A, n_short, n_large, ctx->s_act_mark.p, s));
+        r = scan(ctx, ctx->s_act_mark.p, ctx->s_act_incl.p, na, true); if (r) return r;
+        int total = 0;
+        ZFV_HIP(hipMemcpyAsync(&total, ctx->s_act_incl.p + (na - 1), sizeof(int), hipMemcpyDeviceToHost, s));
+        ZFV_HIP(hipStreamSynchronize(s));
+        if (total != n_swap) return fail(ZFV_ERR_STATE, "AMR apply: hole / survivor counts differ");
+        ZFV_HIP(amr_swapmap(A, n_short, n_large, n_swap, ctx->s_act_mark.p, ctx->s_act_incl.p, ctx->s_hole_at.p,
+                            ctx->s_swap_src.p, ctx->s_swap_dst.p, s));
+    }
+    DevGrad G{ctx->gdx.p, ctx->gdy.p};
+    ZFV_HIP(amr_geometry(A, ctx->layer(ctx->cur), G, ctx->wait[ctx->wcur].p, ctx->eos(), distributor == 1, s));
+    ZFV_HIP(amr_move(A, ctx->layer(ctx->cur), ctx->wait[ctx->wcur].p, n_swap, ctx->s_swap_src.p,
+                     ctx->s_swap_dst.p, s));
+    ctx->hm.n = n_short;
+    ZFV_HIP(hipMemsetAsync(ctx->s_counts.p + 3, 0, sizeof(int), s));
+    ZFV_HIP(amr_derive(ctx->amesh(), ctx->s_counts.p + 3, s));
+    int bad = 0;
+    ZFV_HIP(hipMemcpyAsync(&bad, ctx->s_counts.p + 3, sizeof(int), hipMemcpyDeviceToHost, s));
+    ZFV_HIP(hipStreamSynchronize(s));
+    ctx->dt_valid = false;
+    if (bad) {
+        ctx->have_state = false;
+        return fail(ZFV_ERR_STATE, "AMR apply produced an inconsistent adjacency; state invalidated");
+    }
+    return ZFV_OK;
 }
 
 // Limiter(const std::string&) name parsing (math/cfd/limiter.cxx:163-252).

@@ -57,6 +57,40 @@ struct StepCtl {
     int* err;                    // sticky numeric-error flag
 };
 
+// Mutable mesh view used by the AMR kernels (zfv_amr.hip): the DevMesh
+// arrays plus the AMR bookkeeping of the reference's AmrCells
+// (mesh/euler/amr_cells.h:74-104): per slot the Boundary id, per cell
+// level / flag / b_idx / z_idx / next / idx (idx < 0: undefined cell) and the
+// 9 SqQuad vertices ([9][stride]).  flag0 is the restricted-flag snapshot of
+// the balancing's sort phase.
+struct AmrMesh {
+    int n;
+    int stride;
+    double *cx, *cy, *vol, *h;
+    int *code, *adj, *bnd;
+    double *area, *fcx, *fcy, *w, *nx, *ny;
+    double *vx, *vy;
+    int *level, *flag, *flag0, *b_idx, *z_idx, *next, *idx;
+};
+
+hipError_t amr_criterion(const AmrMesh& A, const DevState& z, const DevGrad& G, int criterion, double thr,
+                         hipStream_t s);
+hipError_t amr_balance(const AmrMesh& A, int max_level, hipStream_t s);
+hipError_t amr_count(const AmrMesh& A, int* counts, hipStream_t s);
+hipError_t amr_scan(const int* in, int* out, int count, bool inclusive, void* tmp, size_t& tmp_bytes,
+                    hipStream_t s);
+hipError_t amr_mark(const AmrMesh& A, int n_short, int* ref_mark, int* crs_mark, int* hole_mark, hipStream_t s);
+hipError_t amr_positions(const AmrMesh& A, int n_large, int n_parents, const int* ref_rank, const int* crs_rank,
+                         const int* hole_mark, const int* hole_rank, int* hole_at, hipStream_t s);
+hipError_t amr_actual_mark(const AmrMesh& A, int n_short, int n_large, int* act_mark, hipStream_t s);
+hipError_t amr_swapmap(const AmrMesh& A, int n_short, int n_large, int n_swap, const int* act_mark,
+                       const int* act_incl, const int* hole_at, int* swap_src, int* swap_dst, hipStream_t s);
+hipError_t amr_geometry(const AmrMesh& A, const DevState& z, const DevGrad& G, int* wait, const Eos& eos,
+                        bool slope, hipStream_t s);
+hipError_t amr_move(const AmrMesh& A, const DevState& z, int* wait, int n_swap, const int* swap_src,
+                    const int* swap_dst, hipStream_t s);
+hipError_t amr_derive(const AmrMesh& A, int* bad, hipStream_t s);
+
 // Launchers (zfv_kernels.hip).
 hipError_t launch_fill_u64(unsigned long long* p, int count, unsigned long long v, hipStream_t s);
 hipError_t launch_dt(const DevMesh& M, const DevState& z, const Eos& eos, unsigned long long* slot, hipStream_t s);
