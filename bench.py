@@ -46,52 +46,140 @@ def kernel_bytes_per_cell(phi):
 
 
 WORKLOADS = {
-    # name: (nx, ny, acc, limiter, description)
-    "uniform1024_muscl": (1024, 1024, 2, "MC",
-                          "1024x1024 uniform mesh, MUSCL-Hancock + MC + HLLC, FP64, ZOE bounds, "
-                          "seeded Gaussian random density field (mean 1, std 0.1, corr. length 0.05), p = 1"),
-    "quadrants200_first_order": (200, 200, 1, "MC",
-                                 "200x200 mesh, first-order HLLC, CFL 0.5, seeded four-quadrant Riemann data"),
+    "amr256_muscl": dict(
+        kind="amr", nx=256, ny=256, acc=2, limiter="minmod", max_level=3, threshold=0.1, every=4,
+        desc="base 256x256 AMR-2D mesh, up to 3 refinement levels, MUSCL-Hancock + minmod + HLLC, CFL 0.5, FP64, "
+             "ZOE bounds; seeded circle (radius 0.2, rho = p = 1 inside, 0.1 outside, 1% sinusoidal density "
+             "perturbation); flags |grad rho| h / rho > 0.1 (hysteresis 0.4) and slope-distributor "
+             "refine/coarsen every 4 steps, inside the timed region"),
+    "uniform1024_muscl": dict(
+        kind="uniform", nx=1024, ny=1024, acc=2, limiter="MC",
+        desc="1024x1024 uniform mesh, MUSCL-Hancock + MC + HLLC, FP64, ZOE bounds, seeded Gaussian random "
+             "density field (mean 1, std 0.1, corr. length 0.05), p = 1"),
+    "quadrants200_first_order": dict(
+        kind="uniform", nx=200, ny=200, acc=1, limiter="MC",
+        desc="200x200 mesh, first-order HLLC, CFL 0.5, seeded four-quadrant Riemann data"),
 }
 
 
-def initial_state(name, nx, ny, seed):
+def initial_state(name, cx, cy, seed):
     from zephyr_amd import problems
+    w = WORKLOADS[name]
+    if w["kind"] == "amr":
+        return problems.circle_blast(cx, cy, seed)
     if name.startswith("uniform"):
-        return problems.smooth_density_field(nx, ny, seed)
-    i, j = np.divmod(np.arange(nx * ny), ny)
-    cx = (i + 0.5) / nx
-    cy = (j + 0.5) / ny
+        return problems.smooth_density_field(w["nx"], w["ny"], seed)
     return problems.four_quadrant(cx, cy, seed)
 
 
-def cpu_baseline(name, nx, ny, acc, limiter, seed, budget_s):
-    """Oracle (CPU restatement, OpenMP) on the same mesh and scheme for as many
-    steps as fit in budget_s (at least 2)."""
+def rect_centres(nx, ny):
+    i, j = np.divmod(np.arange(nx * ny), ny)
+    return (i + 0.5) / nx, (j + 0.5) / ny
+
+
+def adapt_to_initial_data(solver_like, name, seed, centres, set_state, set_flags, refine, max_level, thr):
+    """Initial adaptation: max_level + 1 rounds of (initial data at the cell
+    centres, flags, refine with the const distributor), then the initial data
+    on the final mesh."""
+    for _ in range(max_level + 1):
+        cx, cy = centres()
+        set_state(*initial_state(name, cx, cy, seed))
+        set_flags(1, thr)
+        refine(max_level, False)
+    cx, cy = centres()
+    set_state(*initial_state(name, cx, cy, seed))
+
+
+def cpu_baseline(name, seed, budget_s):
+    """Oracle (CPU restatement, OpenMP) on the same workload for as many steps
+    as fit in budget_s (at least 2)."""
     from oracle import pyoracle as O
+    w = WORKLOADS[name]
+    nx, ny = w["nx"], w["ny"]
     threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(16, os.cpu_count() or 1)
     m = O.Mesh.rectangle(nx, ny)
-    s = O.SmFluid(m, acc=acc, flux=O.HLLC, limiter=limiter, cfl=0.5)
-    s.set_state(*initial_state(name, nx, ny, seed))
+    s = O.SmFluid(m, acc=w["acc"], flux=O.HLLC, limiter=w["limiter"], cfl=0.5)
+    amr = w["kind"] == "amr"
+    if amr:
+        def centres():
+            c = m.cells()
+            return c["cx"], c["cy"]
+        adapt_to_initial_data(s, name, seed, centres, s.set_state,
+                              lambda crit, thr: s.set_flags(crit, thr, threads),
+                              lambda ml, slope: s.refine(ml, slope), w["max_level"], w["threshold"])
+    else:
+        cx, cy = rect_centres(nx, ny)
+        s.set_state(*initial_state(name, cx, cy, seed))
     s.update(threads)   # untimed warm-up step
-    steps, t0 = 0, time.perf_counter()
+    steps, cells, t0 = 0, 0, time.perf_counter()
     while True:
+        if amr and steps % w["every"] == 0:
+            s.set_flags(1, w["threshold"], threads)
+            s.refine(w["max_level"], True)
+        cells += m.n
         s.update(threads)
         steps += 1
         el = time.perf_counter() - t0
         if steps >= 2 and el >= budget_s:
             break
-    return {"value": nx * ny * steps / el, "unit": "cell-updates/s", "cores": threads, "kind": "port",
+    return {"value": cells / el, "unit": "cell-updates/s", "cores": threads, "kind": "port",
             "sample": f"oracle (C++ -O3 -ffp-contract=off, OpenMP {threads} threads) on the same "
-                      f"{nx}x{ny} workload, {steps} steps in {el:.2f} s"}
+                      f"{name} workload from the same initial data, {steps} steps"
+                      f"{' (adaptation included)' if amr else ''} in {el:.2f} s"}
+
+
+class ChunkProfile:
+    """Per-kernel HIP-event time and algorithmic bytes accumulated over
+    chunks of steps run on one mesh (n cells, phi = faces / cells)."""
+
+    def __init__(self, ctx):
+        self.ctx = ctx
+        self.prev = {}
+        self.ms, self.launches, self.bytes = {}, {}, {}
+
+    def close_chunk(self, n, phi):
+        cur = self.ctx.profile_read()   # cumulative since profile_enable
+        kb = kernel_bytes_per_cell(phi)
+        for k, (ms, cnt) in cur.items():
+            pms, pcnt = self.prev.get(k, (0.0, 0))
+            dms, dcnt = ms - pms, cnt - pcnt
+            if dcnt <= 0:
+                continue
+            self.ms[k] = self.ms.get(k, 0.0) + dms
+            self.launches[k] = self.launches.get(k, 0) + dcnt
+            self.bytes[k] = self.bytes.get(k, 0.0) + kb.get(k, 0.0) * n * dcnt
+        self.prev = cur
+
+
+def run_steps(ctx, w, steps, step0, prof=None):
+    """`steps` steps from global step index step0; AMR workloads adapt
+    (flags + refine) before every step whose index is a multiple of
+    w["every"].  Returns the number of cell-updates."""
+    cells = 0
+    i = 0
+    while i < steps:
+        k = steps - i
+        if w["kind"] == "amr":
+            g = step0 + i
+            if g % w["every"] == 0:
+                ctx.set_flags(1, w["threshold"])
+                ctx.refine(w["max_level"], slope=True)   # synchronises
+                if prof is not None:
+                    prof.close_chunk(prof.n, prof.phi)
+                    prof.n, prof.phi = ctx.n, ctx.n_faces() / ctx.n
+            k = min(k, w["every"] - (g % w["every"]))
+        ctx.step(k)
+        cells += ctx.n * k
+        i += k
+    return cells
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--steps", type=int, default=500)
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--workload", default="uniform1024_muscl", choices=sorted(WORKLOADS))
+    ap.add_argument("--workload", default="amr256_muscl", choices=sorted(WORKLOADS))
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--cpu-budget", type=float, default=10.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -106,14 +194,22 @@ def main():
 
     import zephyr_amd as Z
 
-    nx, ny, acc, limiter, desc = WORKLOADS[args.workload]
-    n = nx * ny
+    w = WORKLOADS[args.workload]
+    nx, ny, acc, limiter = w["nx"], w["ny"], w["acc"], w["limiter"]
     ctx = Z.Context()
     ctx.mesh_rectangle(nx, ny)
     ctx.configure(acc=acc, flux="hllc", limiter=limiter, cfl=0.5, gamma=1.4)
-    ctx.set_state(*initial_state(args.workload, nx, ny, args.seed))
+    if w["kind"] == "amr":
+        def centres():
+            c, _, _ = ctx.mesh_download()
+            return c["cx"], c["cy"]
+        adapt_to_initial_data(ctx, args.workload, args.seed, centres, ctx.set_state, ctx.set_flags,
+                              lambda ml, slope: ctx.refine(ml, slope), w["max_level"], w["threshold"])
+    else:
+        cx, cy = rect_centres(nx, ny)
+        ctx.set_state(*initial_state(args.workload, cx, cy, args.seed))
 
-    ctx.step(args.warmup)
+    run_steps(ctx, w, args.warmup, 0)
     ctx.synchronize()
 
     if args.copy_calibration:
@@ -123,31 +219,34 @@ def main():
         hip = C.CDLL("libamdhip64.so")
         bufs = [C.c_void_p() for _ in range(5)]
         for b in bufs:
-            if hip.hipMalloc(C.byref(b), C.c_size_t(8 * n)) != 0:
+            if hip.hipMalloc(C.byref(b), C.c_size_t(8 * ctx.n)) != 0:
                 raise RuntimeError("hipMalloc failed")
         for _ in range(3):
             Z._check(Z.lib().zfv_state_download_device(ctx.h, *bufs))
+        ctx.synchronize()
         for b in bufs:
             hip.hipFree(b)
+        print(f"copy calibration: n = {ctx.n} doubles per copy", file=sys.stderr, flush=True)
 
+    prof = ChunkProfile(ctx)
+    prof.n, prof.phi = ctx.n, ctx.n_faces() / ctx.n
     ctx.profile_enable(True)
     ctx.synchronize()
     t0 = time.perf_counter()
-    ctx.step(args.steps)
+    cells = run_steps(ctx, w, args.steps, args.warmup, prof)
     ctx.synchronize()
     el = time.perf_counter() - t0
-    prof = ctx.profile_read()
+    prof.close_chunk(prof.n, prof.phi)
     ctx.profile_enable(False)
 
-    value = n * args.steps / el
-    phi = 2.0 + (nx + ny) / float(n)
-    kb = kernel_bytes_per_cell(phi)
-    per_kernel = {k: (ms / max(cnt, 1)) for k, (ms, cnt) in prof.items()}
-    dom = max((k for k in prof if k != "dt"), key=lambda k: prof[k][0])
-    dom_ms = per_kernel[dom]
-    alg_bytes = kb[dom] * n
+    value = cells / el
+    step_kernels = ("flux1",) if acc == 1 else ("grad_pred", "corr")
+    dom = max(step_kernels, key=lambda k: prof.ms.get(k, 0.0))
+    dom_ms = prof.ms[dom] / prof.launches[dom]
+    alg_bytes = prof.bytes[dom] / prof.launches[dom]
     achieved = alg_bytes / (dom_ms * 1e-3) / 1e9
-    step_bytes = sum(kb[k] for k in (("flux1",) if acc == 1 else ("grad_pred", "corr"))) * n
+    step_bytes = sum(prof.bytes.get(k, 0.0) for k in step_kernels)
+    step_ms = sum(prof.ms.get(k, 0.0) for k in step_kernels)
 
     traffic = None
     tpath = os.path.join(HERE, "profiles", "traffic.json")
@@ -156,8 +255,10 @@ def main():
         if t:
             traffic = t
 
+    scheme = "MUSCL+HLLC" if acc == 2 else "first-order HLLC"
     out = {
-        "metric": "FP64 cell-updates/s (HLLC+MUSCL)" if acc == 2 else "FP64 cell-updates/s (HLLC first order)",
+        "metric": f"FP64 cell-updates/s (HLLC+MUSCL{', AMR' if w['kind'] == 'amr' else ''})"
+                  if acc == 2 else "FP64 cell-updates/s (HLLC first order)",
         "value": value,
         "unit": "cell-updates/s",
         "n_gpus": 1,
@@ -169,17 +270,19 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic",
-        "config": {"workload": args.workload, "description": desc, "cells": n, "phi": phi,
-                   "scheme": "MUSCL+HLLC" if acc == 2 else "first-order HLLC", "limiter": limiter,
-                   "step_algorithmic_bytes": step_bytes,
-                   "step_roofline_frac": step_bytes / (el / args.steps) / 1e9 / HBM_PEAK_GBS,
-                   "kernel_ms": per_kernel},
+        "config": {"workload": args.workload, "description": w["desc"], "scheme": scheme,
+                   "limiter": limiter, "cells_final": ctx.n, "mean_cells_per_step": cells / args.steps,
+                   "phi_final": ctx.n_faces() / ctx.n,
+                   "kernel_ms_avg": {k: prof.ms[k] / prof.launches[k] for k in prof.ms},
+                   "kernel_launches": dict(prof.launches),
+                   "step_kernels_hbm_frac": step_bytes / (step_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                   "step_kernels_ms_share_of_wall": step_ms / (el * 1e3)},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "algorithmic_bytes_per_launch": alg_bytes},
+                     "algorithmic_bytes_per_launch": alg_bytes, "avg_launch_ms": dom_ms},
     }
     if not args.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(args.workload, nx, ny, acc, limiter, args.seed, args.cpu_budget)
+        out["cpu_baseline"] = cpu_baseline(args.workload, args.seed, args.cpu_budget)
     print(json.dumps(out), flush=True)
 
 

@@ -108,6 +108,10 @@ int zfv_mesh_rectangle(zfv_ctx* ctx, int nx, int ny, double xmin, double xmax,
                        double ymin, double ymax, const int32_t* bounds);
 /* Number of cells of the current mesh (EuMesh::n_cells). */
 int zfv_mesh_n_cells(zfv_ctx* ctx, int32_t* n);
+/* Number of unique faces of the current mesh (interior face sides / 2 +
+ * boundary face sides; a hanging face counts once per fine side), the phi =
+ * faces / cells of the bandwidth accounting.  No reference counterpart. */
+int zfv_mesh_n_faces(zfv_ctx* ctx, int64_t* n);
 /* Download the current mesh into caller arrays sized as zfv_mesh_soa
  * (pointers of a zfv_mesh_soa used as outputs; NULL members are skipped). */
 int zfv_mesh_download(zfv_ctx* ctx, const zfv_mesh_soa* out);
@@ -150,8 +154,9 @@ int zfv_flags_upload(zfv_ctx* ctx, const int32_t* flags);
 int zfv_flags_download(zfv_ctx* ctx, int32_t* flags);
 /* SmFluid::set_flags (sm_fluid.cpp:517-643): limited gradient, then flags.
  * criterion 0: the reference's relative jumps of density and pressure
- * (xi = 0.05, hysteresis 0.4); criterion 1: |grad rho| h / rho of the limited
- * gradient against `threshold` (+1 above, 0 above 0.4 threshold, else -1). */
+ * (xi = 0.05, hysteresis 0.4); criterion 1: |grad rho| h / rho of the
+ * unlimited LSM gradient against `threshold` (+1 above, 0 above 0.4 threshold,
+ * else -1).  Both leave the limited gradient for the slope distributor. */
 int zfv_set_flags(zfv_ctx* ctx, int criterion, double threshold);
 /* amr::balance_flags (mesh/amr/balancing.h:15-35, balancing_fast.h:412-451). */
 int zfv_balance_flags(zfv_ctx* ctx, int max_level);

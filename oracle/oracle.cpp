@@ -696,7 +696,8 @@ inline void zero(PState& z) { std::memset(&z, 0, sizeof(z)); }
 // gradient::LSM (Storable version, math/cfd/gradient.h:226-282) and
 // gradient::limiting (gradient.h:412-474), as called by
 // SmFluid::compute_grad (sm_fluid.cpp:116-125).
-void grad_cell(orc_smf& S, int c, Grad& out) {
+// raw (optional): the unlimited LSM gradient of rho, {d/dx, d/dy}.
+void grad_cell(orc_smf& S, int c, Grad& out, double* raw = nullptr) {
     const orc_mesh& m = *S.m;
     const double w0 = 1.0;   // cell.dim() < 3
     const PState zc = S.init[c];
@@ -755,6 +756,11 @@ void grad_cell(orc_smf& S, int c, Grad& out) {
         g.x.arr()[k] = (B[0][0] * Fx[k] + B[0][1] * Fy[k]) + B[0][2] * Fz[k];
         g.y.arr()[k] = (B[1][0] * Fx[k] + B[1][1] * Fy[k]) + B[1][2] * Fz[k];
         g.z.arr()[k] = (B[2][0] * Fx[k] + B[2][1] * Fy[k]) + B[2][2] * Fz[k];
+    }
+
+    if (raw) {
+        raw[0] = g.x.rho;
+        raw[1] = g.y.rho;
     }
 
     // limiting: the same A (recomputed identically in the reference), new F.

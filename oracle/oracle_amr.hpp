@@ -547,15 +547,24 @@ extern "C" {
 
 // SmFluid::set_flags (sm_fluid.cpp:517-643), criterion 0: relative jumps of
 // density and pressure (xi = 0.05, hysteresis 0.4).  Criterion 1: the
-// normalised limited density gradient |grad rho| h / rho against `threshold`
-// (+1 above, 0 above 0.4 threshold, -1 otherwise), h the incircle diameter.
+// normalised density gradient |grad rho| h / rho against `threshold`
+// (+1 above, 0 above 0.4 threshold, -1 otherwise), h the incircle diameter,
+// grad rho the unlimited LSM gradient (gradient.h:226-282; a limited one is
+// zero next to a jump under minmod).  Both compute the limited gradient too,
+// for the slope distributor.
 void orc_smf_set_flags(orc_smf* S, int criterion, double threshold, int nthreads) {
     orc_mesh& m = *S->m;
-    orc_smf_compute_grad(S, nthreads);
+    std::vector<double> raw(2 * (size_t)m.n);
+    #pragma omp parallel for num_threads(nthreads) schedule(static)
+    for (int c = 0; c < m.n; ++c) {
+        Grad g;
+        grad_cell(*S, c, g, &raw[2 * (size_t)c]);
+        S->d_dx[c] = g.x; S->d_dy[c] = g.y; S->d_dz[c] = g.z;
+    }
     for (int ic = 0; ic < m.n; ++ic) {
         const PState& z = S->init[ic];
         if (criterion == 1) {
-            double gx = S->d_dx[ic].rho, gy = S->d_dy[ic].rho;
+            double gx = raw[2 * (size_t)ic], gy = raw[2 * (size_t)ic + 1];
             double xi = std::sqrt(gx * gx + gy * gy) * incircle(m, ic) / z.rho;
             m.flag[ic] = xi > threshold ? 1 : (xi > 0.4 * threshold ? 0 : -1);
             continue;

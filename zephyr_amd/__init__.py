@@ -29,6 +29,7 @@ EXPORTS = [
     "zfv_smf_compute_grad", "zfv_smf_compute_dt", "zfv_conserved_totals",
     "zfv_limiter_from_name", "zfv_profile_enable", "zfv_profile_read",
     "zfv_flags_upload", "zfv_flags_download", "zfv_set_flags", "zfv_balance_flags", "zfv_refine",
+    "zfv_mesh_n_faces",
 ]
 
 
@@ -153,6 +154,11 @@ class Context:
         m = MeshSoA(n_cells=n, **{k: v.ctypes.data for k, v in arrs.items()})
         _check(lib().zfv_mesh_upload(self.h, C.byref(m)))
         self.n = n
+
+    def n_faces(self):
+        v = C.c_int64()
+        _check(lib().zfv_mesh_n_faces(self.h, C.byref(v)))
+        return v.value
 
     def mesh_download(self):
         n = C.c_int32()

@@ -36,18 +36,32 @@ FLAGS = [
 ]
 
 
-def build(verbose=False):
-    """Compile libzfv.so; always recompiles (the build check)."""
-    os.makedirs(BUILD_DIR, exist_ok=True)
-    tmp = LIB_PATH + ".tmp"
-    cmd = [HIPCC] + FLAGS + ["-o", tmp] + SOURCES
+EXAMPLES = {
+    # problem files written against the host C++ facade (zephyr_amd/host/zfv.hpp)
+    "smf_quadrants": os.path.join(HERE, "host", "examples", "smf_quadrants.cpp"),
+}
+EXAMPLE_PATHS = {k: os.path.join(BUILD_DIR, k) for k in EXAMPLES}
+
+
+def _run(cmd, what, verbose):
     if verbose:
         print(" ".join(cmd), flush=True)
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:
         sys.stderr.write(res.stdout + res.stderr)
-        raise RuntimeError("hipcc failed building libzfv.so")
+        raise RuntimeError(what)
+
+
+def build(verbose=False):
+    """Compile libzfv.so and the host examples; always recompiles (the build check)."""
+    os.makedirs(BUILD_DIR, exist_ok=True)
+    tmp = LIB_PATH + ".tmp"
+    _run([HIPCC] + FLAGS + ["-o", tmp] + SOURCES, "hipcc failed building libzfv.so", verbose)
     os.replace(tmp, LIB_PATH)
+    for name, src in EXAMPLES.items():
+        _run(["g++", "-O2", "-std=c++17", "-I" + os.path.join(ROOT, "include"), "-I" + os.path.join(HERE, "host"),
+              src, "-o", EXAMPLE_PATHS[name], "-L" + BUILD_DIR, "-lzfv", "-Wl,-rpath,$ORIGIN"],
+             f"g++ failed building {name}", verbose)
     return LIB_PATH
 
 

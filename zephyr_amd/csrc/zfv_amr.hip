@@ -173,7 +173,8 @@ __device__ __forceinline__ int cell_id() { return (int)(blockIdx.x * blockDim.x 
 
 // ---------------------------------------------------------------- flags
 // Refinement criterion.  0: SmFluid::set_flags relative jumps (sm_fluid.cpp:517-643);
-// 1: |grad rho| h / rho of the limited gradient against `threshold`.
+// 1: |grad rho| h / rho against `threshold`, G holding the unlimited LSM
+// gradient of rho (G.dx[c], G.dy[c]).
 __global__ void __launch_bounds__(256) k_amr_criterion(AmrMesh A, DevState z, DevGrad G, int criterion,
                                                        double threshold) {
     const int ic = cell_id();
@@ -635,9 +636,13 @@ __device__ __forceinline__ double cube_cr(double x) {
 // Derived per-cell data after an adaptation: slot code words (type and
 // general-normal bit), LSM weights w = area / |drn|^3 (gradient.h:251-256)
 // and incircle diameters (amr_cells.cpp:107-114).
-__global__ void __launch_bounds__(256) k_amr_derive(AmrMesh A, int* bad) {
+// counts[0]: inconsistency flag; counts[1], counts[2]: defined interior
+// (inner + periodic) and boundary slots, for the unique-face count
+// interior / 2 + boundary.
+__global__ void __launch_bounds__(256) k_amr_derive(AmrMesh A, int* counts) {
     const int c = cell_id();
-    if (c >= A.n) return;
+    int n_int = 0, n_bnd = 0;
+    if (c < A.n) {
     const double kNX[4] = {-1.0, 1.0, 0.0, 0.0}, kNY[4] = {0.0, 0.0, -1.0, 1.0};
     int cw = 0;
     const double cx = A.cx[c], cy = A.cy[c];
@@ -660,10 +665,12 @@ __global__ void __launch_bounds__(256) k_amr_derive(AmrMesh A, int* bad) {
         const double nx = A.nx[f], ny = A.ny[f];
         const bool gen = !(nx == kNX[side] && ny == kNY[side]);
         cw |= (t | (gen ? kGeneralNormal : 0)) << (4 * s);
+        if (t >= SC_WALL) ++n_bnd;
+        else ++n_int;
         const int nb = A.adj[f];
         if (nb < 0 || nb >= A.n || (t >= SC_WALL && nb != c)) {   // inconsistent mesh: report, keep in bounds
             A.adj[f] = c;
-            *bad = 1;
+            counts[0] = 1;
             continue;
         }
         double px, py;
@@ -686,6 +693,15 @@ __global__ void __launch_bounds__(256) k_amr_derive(AmrMesh A, int* bad) {
     const double ax = a.x - b.x, ay = a.y - b.y, bx = p.x - q.x, by = p.y - q.y;
     const double s1 = ax * ax + ay * ay, s2 = bx * bx + by * by;
     A.h[c] = sqrt((s2 < s1) ? s2 : s1);
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        n_int += __shfl_xor(n_int, off, 64);
+        n_bnd += __shfl_xor(n_bnd, off, 64);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        if (n_int) atomicAdd(&counts[1], n_int);
+        if (n_bnd) atomicAdd(&counts[2], n_bnd);
+    }
 }
 
 // ---------------------------------------------------------------- launchers
@@ -759,8 +775,8 @@ hipError_t amr_move(const AmrMesh& A, const DevState& z, int* wait, int n_swap, 
     return hipGetLastError();
 }
 
-hipError_t amr_derive(const AmrMesh& A, int* bad, hipStream_t s) {
-    hipLaunchKernelGGL(k_amr_derive, grid256(A.n), dim3(256), 0, s, A, bad);
+hipError_t amr_derive(const AmrMesh& A, int* counts, hipStream_t s) {
+    hipLaunchKernelGGL(k_amr_derive, grid256(A.n), dim3(256), 0, s, A, counts);
     return hipGetLastError();
 }
 

@@ -86,11 +86,14 @@ struct zfv_ctx {
     // device state
     DBuf<double> st[3][5];   // init / next / half layers
     DBuf<double> gdx, gdy;
+    DBuf<double> graw;       // unlimited LSM gradient of rho [2][cap] (refinement criterion)
     DBuf<int> wait[2];
     int cur = 0, wcur = 0;   // index of the init layer / current wait buffer
     DBuf<unsigned long long> dtmin;
     DBuf<double> dt_out, partial;
     DBuf<int> err;
+
+    int64_t n_faces = 0;   // unique faces: interior slots / 2 + boundary slots
 
     zfv_smf_params prm{};
     bool dt_valid = false;
@@ -170,6 +173,7 @@ int upload_mesh(zfv_ctx* ctx) {
     std::vector<double> area(ns, 0.0), fcx(ns, 0.0), fcy(ns, 0.0), w(ns, 0.0), h(n, 0.0);
     std::vector<double> nxs(ns, 0.0), nys(ns, 0.0);
 
+    int64_t n_int = 0, n_bnd = 0;
     for (int c = 0; c < n; ++c) {
         int cw = 0;
         for (int s = 0; s < 8; ++s) {
@@ -181,6 +185,8 @@ int upload_mesh(zfv_ctx* ctx) {
                 adj[j] = c;
                 continue;
             }
+            if (t >= SC_WALL) ++n_bnd;
+            else ++n_int;
             const int side = s & 3;
             const double nx = m.nx[i], ny = m.ny[i];
             if (!std::isfinite(nx) || !std::isfinite(ny) || !(nx * nx + ny * ny > 0.0))
@@ -220,6 +226,7 @@ int upload_mesh(zfv_ctx* ctx) {
         h[c] = incircle_2d(&m.vx[9 * (size_t)c], &m.vy[9 * (size_t)c]);
     }
     ctx->cap = n;
+    ctx->n_faces = n_int / 2 + n_bnd;
     std::vector<int> bnd_sm(ns), iota(n);
     std::vector<double> vx_vm(9 * (size_t)n), vy_vm(9 * (size_t)n);
     for (int c = 0; c < n; ++c) {
@@ -619,6 +626,12 @@ int zfv_mesh_n_cells(zfv_ctx* ctx, int32_t* n) {
     return ZFV_OK;
 }
 
+int zfv_mesh_n_faces(zfv_ctx* ctx, int64_t* n) {
+    if (!ctx || !n) return fail(ZFV_ERR_ARG, "null argument");
+    *n = ctx->have_mesh ? ctx->n_faces : 0;
+    return ZFV_OK;
+}
+
 int zfv_mesh_download(zfv_ctx* ctx, const zfv_mesh_soa* o) {
     if (!ctx || !o) return fail(ZFV_ERR_ARG, "null argument");
     if (!ctx->have_mesh) return fail(ZFV_ERR_STATE, "no mesh uploaded");
@@ -865,9 +878,11 @@ int zfv_set_flags(zfv_ctx* ctx, int criterion, double threshold) {
     if (r) return r;
     StepCtl ctl = make_ctl(ctx);
     DevGrad G{ctx->gdx.p, ctx->gdy.p};
+    if (ctx->graw.n < 2 * (size_t)ctx->cap) ZFV_HIP(ctx->graw.alloc(2 * (size_t)ctx->cap));
     ZFV_HIP(launch_grad_pred(ctx->prm.limiter, false, ctx->dmesh(), ctx->layer(ctx->cur), G, ctx->layer(2),
-                             ctx->wait[ctx->wcur].p, ctx->eos(), ctl, ctx->stream));
-    ZFV_HIP(amr_criterion(ctx->amesh(), ctx->layer(ctx->cur), G, criterion, threshold, ctx->stream));
+                             ctx->wait[ctx->wcur].p, ctx->eos(), ctl, ctx->stream, ctx->graw.p));
+    const DevGrad R{ctx->graw.p, ctx->graw.p + ctx->cap};   // unlimited LSM gradient of rho
+    ZFV_HIP(amr_criterion(ctx->amesh(), ctx->layer(ctx->cur), R, criterion, threshold, ctx->stream));
     ctx->host_stale = true;
     return ZFV_OK;
 }
@@ -902,7 +917,7 @@ int zfv_refine(zfv_ctx* ctx, int max_level, int distributor) {
     if (max_level == 0) return ZFV_OK;   // EuMesh::adaptive() is false
     ZFV_HIP(amr_balance(ctx->amesh(), max_level, s));
     ctx->host_stale = true;
-    ZFV_HIP(ctx->s_counts.alloc(4));
+    ZFV_HIP(ctx->s_counts.alloc(8));
     ZFV_HIP(hipMemsetAsync(ctx->s_counts.p, 0, 4 * sizeof(int), s));
     ZFV_HIP(amr_count(ctx->amesh(), ctx->s_counts.p, s));
     int cnt[4];
@@ -955,13 +970,14 @@ int zfv_refine(zfv_ctx* ctx, int max_level, int distributor) {
     ZFV_HIP(amr_move(A, ctx->layer(ctx->cur), ctx->wait[ctx->wcur].p, n_swap, ctx->s_swap_src.p,
                      ctx->s_swap_dst.p, s));
     ctx->hm.n = n_short;
-    ZFV_HIP(hipMemsetAsync(ctx->s_counts.p + 3, 0, sizeof(int), s));
-    ZFV_HIP(amr_derive(ctx->amesh(), ctx->s_counts.p + 3, s));
-    int bad = 0;
-    ZFV_HIP(hipMemcpyAsync(&bad, ctx->s_counts.p + 3, sizeof(int), hipMemcpyDeviceToHost, s));
+    ZFV_HIP(hipMemsetAsync(ctx->s_counts.p + 4, 0, 3 * sizeof(int), s));
+    ZFV_HIP(amr_derive(ctx->amesh(), ctx->s_counts.p + 4, s));
+    int dc[3] = {0, 0, 0};
+    ZFV_HIP(hipMemcpyAsync(dc, ctx->s_counts.p + 4, 3 * sizeof(int), hipMemcpyDeviceToHost, s));
     ZFV_HIP(hipStreamSynchronize(s));
     ctx->dt_valid = false;
-    if (bad) {
+    ctx->n_faces = (int64_t)dc[1] / 2 + dc[2];
+    if (dc[0]) {
         ctx->have_state = false;
         return fail(ZFV_ERR_STATE, "AMR apply produced an inconsistent adjacency; state invalidated");
     }

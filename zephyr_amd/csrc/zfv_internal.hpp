@@ -89,7 +89,7 @@ hipError_t amr_geometry(const AmrMesh& A, const DevState& z, const DevGrad& G, i
                         bool slope, hipStream_t s);
 hipError_t amr_move(const AmrMesh& A, const DevState& z, int* wait, int n_swap, const int* swap_src,
                     const int* swap_dst, hipStream_t s);
-hipError_t amr_derive(const AmrMesh& A, int* bad, hipStream_t s);
+hipError_t amr_derive(const AmrMesh& A, int* counts, hipStream_t s);
 
 // Launchers (zfv_kernels.hip).
 hipError_t launch_fill_u64(unsigned long long* p, int count, unsigned long long v, hipStream_t s);
@@ -98,7 +98,7 @@ hipError_t launch_flux1(int flux, const DevMesh& M, const DevState& in, const De
                         const Eos& eos, const StepCtl& ctl, hipStream_t s);
 hipError_t launch_grad_pred(int limiter, bool predictor, const DevMesh& M, const DevState& in, const DevGrad& g,
                             const DevState& half, const int* wait, const Eos& eos, const StepCtl& ctl,
-                            hipStream_t s);
+                            hipStream_t s, double* raw = nullptr);
 hipError_t launch_corr(int flux, const DevMesh& M, const DevState& init, const DevState& half,
                        const DevGrad& g, const int* wait_in, int* wait_out, const DevState& next,
                        const Eos& eos, const StepCtl& ctl, hipStream_t s);

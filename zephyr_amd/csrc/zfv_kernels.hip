@@ -209,9 +209,12 @@ __global__ void __launch_bounds__(kBlock) k_flux1(DevMesh M, DevState in, DevSta
 // 3x3 LSM matrix has the 2x2 block A = sum w drn dr^T, A(2,2) = 1, and
 // Eigen's cofactor inverse reduces to adj(A) * (1/det) with
 // det = A11*A00 + (-A01)*A10.
+// Without the predictor, `raw` (if not null) receives the unlimited LSM
+// gradient of rho ([2][stride]) for the refinement criterion.
 template <int LIM, bool PRED>
 __global__ void __launch_bounds__(kBlock) k_grad_pred(DevMesh M, DevState in, DevGrad G, DevState half,
-                                                      const int* __restrict__ wait, Eos eos, StepCtl ctl) {
+                                                      const int* __restrict__ wait, Eos eos, StepCtl ctl,
+                                                      double* __restrict__ raw) {
     const int n = M.n;
     const int c = cell_of_thread();
     const double dt = step_dt(ctl);
@@ -262,6 +265,10 @@ __global__ void __launch_bounds__(kBlock) k_grad_pred(DevMesh M, DevState in, De
     for (int k = 0; k < 4; ++k) {
         gx[k] = B00 * Fx[k] + B01 * Fy[k];
         gy[k] = B10 * Fx[k] + B11 * Fy[k];
+    }
+    if (!PRED && raw) {
+        raw[c] = gx[0];
+        raw[M.stride + c] = gy[0];
     }
 
     // limiting: limiter(2 du_n - dF_p, dF_p) * dF_p re-solved with the same A
@@ -479,8 +486,8 @@ hipError_t launch_flux1(int flux, const DevMesh& M, const DevState& in, const De
 template <bool PRED>
 static void launch_gp_t(int limiter, dim3 g, dim3 b, hipStream_t s, const DevMesh& M, const DevState& in,
                         const DevGrad& G, const DevState& half, const int* wait, const Eos& eos,
-                        const StepCtl& ctl) {
-#define ZFV_GP(L) hipLaunchKernelGGL((k_grad_pred<L, PRED>), g, b, 0, s, M, in, G, half, wait, eos, ctl)
+                        const StepCtl& ctl, double* raw) {
+#define ZFV_GP(L) hipLaunchKernelGGL((k_grad_pred<L, PRED>), g, b, 0, s, M, in, G, half, wait, eos, ctl, raw)
     switch (limiter) {
         case 0: ZFV_GP(0); break;
         case 1: ZFV_GP(1); break;
@@ -501,10 +508,10 @@ static void launch_gp_t(int limiter, dim3 g, dim3 b, hipStream_t s, const DevMes
 
 hipError_t launch_grad_pred(int limiter, bool predictor, const DevMesh& M, const DevState& in, const DevGrad& G,
                             const DevState& half, const int* wait, const Eos& eos, const StepCtl& ctl,
-                            hipStream_t s) {
+                            hipStream_t s, double* raw) {
     dim3 g(grid_of(M.n)), b(kBlock);
-    if (predictor) launch_gp_t<true>(limiter, g, b, s, M, in, G, half, wait, eos, ctl);
-    else launch_gp_t<false>(limiter, g, b, s, M, in, G, half, wait, eos, ctl);
+    if (predictor) launch_gp_t<true>(limiter, g, b, s, M, in, G, half, wait, eos, ctl, nullptr);
+    else launch_gp_t<false>(limiter, g, b, s, M, in, G, half, wait, eos, ctl, raw);
     return hipGetLastError();
 }
 

@@ -77,3 +77,26 @@ def smooth_density_field(nx, ny, seed, gamma=1.4):
     p = np.ones_like(rho)
     z = np.zeros_like(rho)
     return rho, z, z.copy(), p, energy_rP(rho, p, gamma)
+
+
+def circle_blast_params(seed):
+    """Seeded parameters of the AMR circle workload: centre ~U[0.3,0.7]^2
+    (x then y), perturbation wavenumber k ~ U{4..16}."""
+    g = SplitMix64(seed)
+    x0 = 0.3 + 0.4 * g.uniform()
+    y0 = 0.3 + 0.4 * g.uniform()
+    k = 4 + int(13.0 * g.uniform())
+    return x0, y0, k
+
+
+def circle_blast(cx, cy, seed, gamma=1.4, radius=0.2):
+    """Circular density/pressure jump 10:1 (rho = p = 1 inside radius 0.2,
+    0.1 outside) with rho *= 1 + 0.01 sin(2 pi k (x + y)); u = v = 0."""
+    x0, y0, k = circle_blast_params(seed)
+    cx = np.asarray(cx, dtype=np.float64)
+    cy = np.asarray(cy, dtype=np.float64)
+    inside = (cx - x0) ** 2 + (cy - y0) ** 2 < radius * radius
+    rho = np.where(inside, 1.0, 0.1) * (1.0 + 0.01 * np.sin(2.0 * np.pi * k * (cx + cy)))
+    p = np.where(inside, 1.0, 0.1)
+    z = np.zeros_like(rho)
+    return rho, z, z.copy(), p, energy_rP(rho, p, gamma)
