@@ -59,7 +59,27 @@ WORKLOADS = {
     "quadrants200_first_order": dict(
         kind="uniform", nx=200, ny=200, acc=1, limiter="MC",
         desc="200x200 mesh, first-order HLLC, CFL 0.5, seeded four-quadrant Riemann data"),
+    "ensemble256_muscl": dict(
+        kind="ensemble", problems=256, smin=64, smax=512, acc=2, limiter="MC",
+        desc="256 independent problems of s x s cells, s ~ U{64..512} (seeded), per-problem seeded "
+             "four-quadrant Riemann data (seed + problem index), MUSCL-Hancock + MC + HLLC, FP64, CFL 0.5, "
+             "per-problem dt; one context, one launch per stage for all problems, captured in HIP graphs, "
+             "launch configuration autotuned; compared with the plain per-problem path looped over 256 contexts"),
 }
+
+
+def ensemble_sizes(w, seed):
+    """Side lengths s_i ~ U{smin..smax} from SplitMix64 (draw i -> problem i)."""
+    from zephyr_amd import problems
+    g = problems.SplitMix64(seed)
+    span = w["smax"] - w["smin"] + 1
+    return [w["smin"] + int(span * g.uniform()) for _ in range(w["problems"])]
+
+
+def ensemble_state(s, seed, p):
+    cx, cy = rect_centres(s, s)
+    from zephyr_amd import problems
+    return problems.four_quadrant(cx, cy, seed + p)
 
 
 def initial_state(name, cx, cy, seed):
@@ -174,6 +194,116 @@ def run_steps(ctx, w, steps, step0, prof=None):
     return cells
 
 
+def ensemble_run(args, w, Z):
+    """Batched ensemble vs the looped per-problem path, same problems."""
+    sizes = ensemble_sizes(w, args.seed)
+    cells_per_step = sum(s * s for s in sizes)
+    # batched: one context, graphs, autotuned launches
+    b = Z.Context()
+    b.batch_rectangles([(s, s) for s in sizes])
+    b.configure(acc=w["acc"], flux="hllc", limiter=w["limiter"], cfl=0.5, gamma=1.4)
+    begin, count = b.batch_layout()
+    full = [np.ones(b.n), np.zeros(b.n), np.zeros(b.n), np.ones(b.n), np.full(b.n, 2.5)]
+    for p, s in enumerate(sizes):
+        st = ensemble_state(s, args.seed, p)
+        for k in range(5):
+            full[k][begin[p]:begin[p] + count[p]] = st[k]
+    b.set_state(*full)
+    del full
+    b.step(2)
+    tune = b.autotune(reps=3)
+    b.graph_enable(True)
+    b.step(args.warmup)
+    b.synchronize()
+    t0 = time.perf_counter()
+    b.step(args.steps)
+    b.synchronize()
+    t_batch = time.perf_counter() - t0
+    # roofline of the batched corrector: one profiled step (eager launches)
+    b.graph_enable(False)
+    b.profile_enable(True)
+    b.step(6)
+    b.synchronize()
+    prof = b.profile_read()
+    b.profile_enable(False)
+    # looped: the plain path, one context per problem
+    ctxs = []
+    for p, s in enumerate(sizes):
+        c = Z.Context()
+        c.mesh_rectangle(s, s)
+        c.configure(acc=w["acc"], flux="hllc", limiter=w["limiter"], cfl=0.5, gamma=1.4)
+        c.set_state(*ensemble_state(s, args.seed, p))
+        ctxs.append(c)
+    for c in ctxs:
+        c.step(args.warmup)
+    for c in ctxs:
+        c.synchronize()
+    t0 = time.perf_counter()
+    for c in ctxs:
+        c.step(args.steps)
+    for c in ctxs:
+        c.synchronize()
+    t_loop_prob = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        for c in ctxs:
+            c.step(1)
+    for c in ctxs:
+        c.synchronize()
+    t_loop_step = time.perf_counter() - t0
+    t_loop = min(t_loop_prob, t_loop_step)   # the faster way of looping is the baseline
+    value = cells_per_step * args.steps / t_batch
+    looped = cells_per_step * args.steps / t_loop
+    phi = 2.0 + sum(2.0 * s for s in sizes) / cells_per_step   # unique faces / cells, s x s meshes
+    kb = kernel_bytes_per_cell(phi)
+    dom = max(("grad_pred", "corr"), key=lambda k: prof[k][0])
+    dom_ms = prof[dom][0] / prof[dom][1]
+    alg = kb[dom] * cells_per_step
+    achieved = alg / (dom_ms * 1e-3) / 1e9
+    out = {
+        "metric": "FP64 cell-updates/s (HLLC+MUSCL, batched ensemble)",
+        "value": value, "unit": "cell-updates/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": t_batch * 1e3 / args.steps, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": args.workload, "description": w["desc"], "problems": len(sizes),
+                   "cells_per_step": cells_per_step, "phi": phi,
+                   "looped_value": looped, "batched_over_looped": value / looped,
+                   "looped_ms_per_step": {"per_problem_runs": t_loop_prob * 1e3 / args.steps,
+                                          "interleaved_steps": t_loop_step * 1e3 / args.steps},
+                   "launch": {str(st): b.launch(st) for st in (1, 2)},
+                   "autotune_ms": tune.tolist(),
+                   "kernel_ms_avg": {k: v[0] / v[1] for k, v in prof.items()}},
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "algorithmic_bytes_per_launch": alg,
+                     "avg_launch_ms": dom_ms},
+    }
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = ensemble_cpu_baseline(w, sizes, args.seed, args.cpu_budget)
+    print(json.dumps(out), flush=True)
+
+
+def ensemble_cpu_baseline(w, sizes, seed, budget_s):
+    """Oracle on the ensemble's problems, in order, one step each, until the budget is spent."""
+    from oracle import pyoracle as O
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(16, os.cpu_count() or 1)
+    cells, steps, el = 0, 0, 0.0
+    for p, s in enumerate(sizes):
+        m = O.Mesh.rectangle(s, s)
+        f = O.SmFluid(m, acc=w["acc"], flux=O.HLLC, limiter=w["limiter"], cfl=0.5)
+        f.set_state(*ensemble_state(s, seed, p))
+        t0 = time.perf_counter()
+        f.update(threads)
+        f.update(threads)
+        el += time.perf_counter() - t0
+        cells += 2 * s * s
+        steps += 1
+        if el >= budget_s:
+            break
+    return {"value": cells / el, "unit": "cell-updates/s", "cores": threads, "kind": "port",
+            "sample": f"oracle (C++ -O3 -ffp-contract=off, OpenMP {threads} threads): two steps of each of the "
+                      f"first {steps} ensemble problems ({cells} cell-updates, steps only) in {el:.2f} s"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -195,6 +325,8 @@ def main():
     import zephyr_amd as Z
 
     w = WORKLOADS[args.workload]
+    if w["kind"] == "ensemble":
+        return ensemble_run(args, w, Z)
     nx, ny, acc, limiter = w["nx"], w["ny"], w["acc"], w["limiter"]
     ctx = Z.Context()
     ctx.mesh_rectangle(nx, ny)

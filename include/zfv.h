@@ -173,6 +173,42 @@ int zfv_conserved_totals(zfv_ctx* ctx, double* out4);
  * (the reference throws, limiter.cxx:249-251). */
 int zfv_limiter_from_name(const char* name);
 
+/* ---- batches of independent problems (no reference counterpart; the
+ * reference would loop over separate EuMesh objects) ------------------- */
+/* One context holding n_problems Rectangle meshes nx[p] x ny[p] (box: 4
+ * doubles {xmin, xmax, ymin, ymax} per problem, NULL = unit square; bounds: 4
+ * Boundary ids per problem, NULL = ZOE), concatenated, each starting on a
+ * 512-cell boundary (padding cells have no faces and zero volume).  Each
+ * step kernel runs once per stage over all problems, every problem with its
+ * own CFL time step.  State arrays span the padded length
+ * (zfv_mesh_n_cells); adaptation is not available on a batch. */
+int zfv_batch_rectangles(zfv_ctx* ctx, int n_problems, const int32_t* nx, const int32_t* ny,
+                         const double* box, const int32_t* bounds);
+/* Layout: *n_problems, begin[n_problems + 1] first cells (padded offsets),
+ * count[n_problems] cells of each problem (any pointer may be NULL). */
+int zfv_batch_layout(zfv_ctx* ctx, int32_t* n_problems, int32_t* begin, int32_t* count);
+/* dt of the last step of every problem (n_problems doubles; synchronises). */
+int zfv_batch_dt(zfv_ctx* ctx, double* dt);
+
+/* ---- launch configuration (no reference counterpart) --------------------- */
+/* Step kernels of a stage (0 = first-order update, 1 = gradient + predictor,
+ * 2 = corrector): layout 0 = one thread per cell, 1 = four lanes per cell
+ * (one face each), block = workgroup size 128, 256 or 512 (0 keeps the
+ * current one).  All choices give bit-identical results. */
+int zfv_set_launch(zfv_ctx* ctx, int stage, int layout, int block);
+int zfv_get_launch(zfv_ctx* ctx, int stage, int* layout, int* block);
+/* Autotuner: times every (layout, workgroup size) of the configured scheme's
+ * step kernels on the current state (`reps` launches each; the state is not
+ * advanced) and keeps the fastest per stage.  ms_out (may be NULL): 18
+ * doubles, ms per launch indexed [stage][layout][block 128/256/512]
+ * (0 for stages the scheme does not use). */
+int zfv_autotune(zfv_ctx* ctx, int reps, double* ms_out);
+/* Capture steps into HIP graphs (6 steps per graph, one graph per starting
+ * phase, replayed by zfv_smf_step; re-captured after any change of mesh,
+ * scheme or launch configuration).  Off by default; profiling timers are
+ * not recorded for graph-launched steps. */
+int zfv_graph_enable(zfv_ctx* ctx, int on);
+
 /* Kernel timing: average device time (ms) per launch of each step kernel
  * over the last zfv_smf_step call when enabled (HIP events on the context
  * stream).  names/ms arrays of length *count (in: capacity, out: used). */

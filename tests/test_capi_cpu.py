@@ -85,3 +85,17 @@ def test_synthetic_quadrants_match_oracle(oracle):
     ref = m.quadrant_state(77)
     for a, b in zip(ours, ref):
         assert np.array_equal(a, b)
+
+
+def test_host_example_built_and_fails_cleanly_without_gpu():
+    """The problem file written against the host facade (zephyr_amd/host/zfv.hpp)
+    is built by build(); without a HIP device it reports the C ABI error."""
+    import subprocess
+    from zephyr_amd import build as B
+    exe = B.EXAMPLE_PATHS["smf_quadrants"]
+    assert os.path.exists(exe)
+    import torch  # noqa: F401  (only to learn whether a GPU is present)
+    if torch.cuda.is_available():
+        pytest.skip("a GPU is present: covered by the GPU test")
+    r = subprocess.run([exe, "16", "1"], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 1 and "zfv:" in r.stderr

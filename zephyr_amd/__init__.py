@@ -29,8 +29,10 @@ EXPORTS = [
     "zfv_smf_compute_grad", "zfv_smf_compute_dt", "zfv_conserved_totals",
     "zfv_limiter_from_name", "zfv_profile_enable", "zfv_profile_read",
     "zfv_flags_upload", "zfv_flags_download", "zfv_set_flags", "zfv_balance_flags", "zfv_refine",
-    "zfv_mesh_n_faces",
+    "zfv_mesh_n_faces", "zfv_set_launch", "zfv_get_launch", "zfv_autotune",
+    "zfv_batch_rectangles", "zfv_batch_layout", "zfv_batch_dt", "zfv_graph_enable",
 ]
+CELL, QUAD = 0, 1   # step-kernel layouts
 
 
 class ZfvError(RuntimeError):
@@ -86,6 +88,10 @@ def lib():
         L.zfv_set_flags.argtypes = [vp, C.c_int, d]
         L.zfv_balance_flags.argtypes = [vp, C.c_int]
         L.zfv_refine.argtypes = [vp, C.c_int, C.c_int]
+        L.zfv_set_launch.argtypes = [vp, C.c_int, C.c_int, C.c_int]
+        L.zfv_autotune.argtypes = [vp, C.c_int, vp]
+        L.zfv_batch_rectangles.argtypes = [vp, C.c_int, vp, vp, vp, vp]
+        L.zfv_graph_enable.argtypes = [vp, C.c_int]
         _lib = L
     return _lib
 
@@ -264,6 +270,51 @@ class Context:
 
     def set_stream(self, stream_ptr):
         _check(lib().zfv_set_stream(self.h, C.c_void_p(stream_ptr)))
+
+    def set_launch(self, stage, layout, block=0):
+        _check(lib().zfv_set_launch(self.h, stage, layout, block))
+
+    def launch(self, stage):
+        """(layout, workgroup size) of a stage."""
+        v, b = C.c_int(), C.c_int()
+        _check(lib().zfv_get_launch(self.h, stage, C.byref(v), C.byref(b)))
+        return v.value, b.value
+
+    def autotune(self, reps=5):
+        """ms per launch, [stage][layout][block 128/256/512]."""
+        ms = np.zeros(18)
+        _check(lib().zfv_autotune(self.h, reps, _ptr(ms)))
+        return ms.reshape(3, 2, 3)
+
+    def graph_enable(self, on=True):
+        _check(lib().zfv_graph_enable(self.h, 1 if on else 0))
+
+    # ---- batches
+    def batch_rectangles(self, sizes, boxes=None, bounds=None):
+        """Independent Rectangle problems sizes = [(nx, ny), ...] in one context."""
+        nx = np.ascontiguousarray([a for a, _ in sizes], np.int32)
+        ny = np.ascontiguousarray([b for _, b in sizes], np.int32)
+        bx = None if boxes is None else np.ascontiguousarray(boxes, np.float64).reshape(-1)
+        bd = None if bounds is None else np.ascontiguousarray(bounds, np.int32).reshape(-1)
+        _check(lib().zfv_batch_rectangles(self.h, len(sizes), _ptr(nx), _ptr(ny), _ptr(bx), _ptr(bd)))
+        n = C.c_int32()
+        _check(lib().zfv_mesh_n_cells(self.h, C.byref(n)))
+        self.n = n.value
+
+    def batch_layout(self):
+        """(begin[np + 1], count[np]) cell ranges of the problems."""
+        npb = C.c_int32()
+        _check(lib().zfv_batch_layout(self.h, C.byref(npb), None, None))
+        begin = np.zeros(npb.value + 1, np.int32)
+        count = np.zeros(npb.value, np.int32)
+        _check(lib().zfv_batch_layout(self.h, None, _ptr(begin), _ptr(count)))
+        return begin, count
+
+    def batch_dt(self):
+        begin, _ = self.batch_layout()
+        dt = np.zeros(len(begin) - 1)
+        _check(lib().zfv_batch_dt(self.h, _ptr(dt)))
+        return dt
 
     def profile_enable(self, on=True):
         _check(lib().zfv_profile_enable(self.h, 1 if on else 0))

@@ -7,9 +7,12 @@
 // setup_positions (mesh/amr/setup_positions.h:24-64) are exclusive scans of
 // the refine / coarsen marks, and the SwapLists (swap_lists.h:28-58) pair
 // the k-th hole (ascending) with the k-th surviving cell above n_short
-// (descending), both found by scans.  Per-cell work (retain_cell, refine_cell,
-// coarse_cell) runs one thread per old cell; each writes only its own slots
-// and the cells it creates, as in the reference's parallel setup_geometry.
+// (descending), both found by scans.  The geometry work of the reference's
+// parallel setup_geometry runs as three kernels over compacted lists:
+// retained cells (one thread each), refined cells (one thread per child) and
+// coarsened families (one thread per family); each thread writes only the
+// slots of its own cell and the cells it creates.  The host synchronises once
+// per adaptation, to read the new cell count.
 // Single process, Cartesian base: every adjacency rotation is 0.
 #include "zfv_internal.hpp"
 
@@ -267,81 +270,81 @@ __global__ void __launch_bounds__(256) k_amr_round(AmrMesh A, int lvl, int round
     }
 }
 
-// Statistics (statistics.h:59-81) and the marks scanned for positions.
-__global__ void __launch_bounds__(256) k_amr_count(AmrMesh A, int* counts) {
-    const int ic = cell_id();
-    int ret = 0, ref = 0, crs = 0;
-    if (ic < A.n) {
-        const int f = A.flag[ic];
-        ret = f == 0;
-        ref = f > 0;
-        crs = f < 0;
-    }
-    // wavefront reduction, then one atomic per wavefront
-    for (int off = 32; off > 0; off >>= 1) {
-        ret += __shfl_xor(ret, off, 64);
-        ref += __shfl_xor(ref, off, 64);
-        crs += __shfl_xor(crs, off, 64);
-    }
-    if ((threadIdx.x & 63) == 0) {
-        if (ret) atomicAdd(&counts[0], ret);
-        if (ref) atomicAdd(&counts[1], ref);
-        if (crs) atomicAdd(&counts[2], crs);
-    }
-}
-
-__global__ void __launch_bounds__(256) k_amr_mark(AmrMesh A, int n_short, int* ref_mark, int* crs_mark,
-                                                  int* hole_mark) {
+// Statistics (statistics.h:59-81), setup_positions (setup_positions.h:24-64)
+// and the SwapLists (swap_lists.h:28-58) all come from one exclusive scan of
+// per-cell marks {refine, coarsening-family head, flag != 0}: its tail gives
+// n_refine, n_parents and the number of changed cells; the ranks give the
+// positions of children and parents, the rank of each hole (a changed cell
+// below n_short) and, in closed form, the number of surviving positions
+// above each position >= n_short.
+__global__ void __launch_bounds__(256) k_amr_mark(AmrMesh A, Marks3* marks) {
     const int ic = cell_id();
     if (ic >= A.n) return;
     const int f = A.flag[ic];
-    ref_mark[ic] = f > 0;
-    crs_mark[ic] = f < 0 && (A.z_idx[ic] & 3) == 0;
-    hole_mark[ic] = ic < n_short && f != 0;
+    marks[ic] = Marks3{f > 0 ? 1 : 0, (f < 0 && (A.z_idx[ic] & 3) == 0) ? 1 : 0, f != 0 ? 1 : 0};
 }
 
-// setup_positions (setup_positions.h:24-64) from the scans, plus the hole table.
-__global__ void __launch_bounds__(256) k_amr_positions(AmrMesh A, int n_large, int n_parents, const int* ref_rank,
-                                                       const int* crs_rank, const int* hole_mark,
-                                                       const int* hole_rank, int* hole_at) {
+// Positions: retained cells keep their index, parents go to n + rank,
+// children to n + n_parents + 4 rank.  Also the compacted refine / coarsen
+// lists and the hole table (k-th hole ascending).  Thread 0 clears the
+// face counters and the dt slots the derive pass fills.
+__global__ void __launch_bounds__(256) k_amr_positions(AmrMesh A, int n_large, int n_short, int n_parents,
+                                                       const Marks3* rank, int* hole_at, int* ref_list,
+                                                       int* crs_list, unsigned long long* faces,
+                                                       unsigned long long* dt_rd, unsigned long long* dt_wr) {
     const int ic = cell_id();
     const int n = A.n;
+    if (ic == 0) {
+        faces[0] = 0ULL;
+        faces[1] = 0ULL;
+        *dt_rd = 0x7ff0000000000000ULL;
+        *dt_wr = 0x7ff0000000000000ULL;
+    }
     if (ic >= n_large) return;
     if (ic >= n) {
         A.next[ic] = ic;
         return;
     }
     const int f = A.flag[ic];
-    if (f == 0) A.next[ic] = ic;
-    else if (f > 0) A.next[ic] = n + n_parents + 4 * ref_rank[ic];
-    else if ((A.z_idx[ic] & 3) == 0) {
-        const int pos = n + crs_rank[ic];
+    const Marks3 rk = rank[ic];
+    if (f == 0) {
+        A.next[ic] = ic;
+    } else if (f > 0) {
+        A.next[ic] = n + n_parents + 4 * rk.ref;
+        ref_list[rk.ref] = ic;
+    } else if ((A.z_idx[ic] & 3) == 0) {
+        const int pos = n + rk.head;
         int sibs[3];
         get_siblings(A, ic, sibs);
         A.next[ic] = pos;
         for (int k = 0; k < 3; ++k) A.next[sibs[k]] = pos;
+        crs_list[rk.head] = ic;
     }
-    if (hole_mark[ic]) hole_at[hole_rank[ic]] = ic;
+    if (ic < n_short && f != 0) hole_at[rk.nz] = ic;
 }
 
-// Positions p >= n_short that survive (new cells, or old cells kept as
-// they are) are taken from the top: the k-th of them from above fills the
-// k-th hole (swap_lists.h:33-55).
-__global__ void __launch_bounds__(256) k_amr_actual_mark(AmrMesh A, int n_short, int n_large, int* act_mark) {
-    const int i = cell_id();
-    const int p = n_short + i;
+// SwapLists: the survivor at position p >= n_short (a new cell, or an old
+// cell kept as it is) with r survivors above it fills hole r.  For p >= n
+// every position survives; for p < n the survivors above p are the new
+// cells plus the retained cells in (p, n).  n_swap = number of holes.
+__global__ void __launch_bounds__(256) k_amr_swapmap(AmrMesh A, int n_short, int n_large, int nz_total,
+                                                     const Marks3* rank, const int* hole_at, int* swap_src,
+                                                     int* swap_dst, int* err) {
+    const int p = n_short + cell_id();
     if (p >= n_large) return;
-    act_mark[i] = p >= A.n || A.flag[p] == 0;
-}
-
-__global__ void __launch_bounds__(256) k_amr_swapmap(AmrMesh A, int n_short, int n_large, int n_swap,
-                                                     const int* act_mark, const int* act_incl, const int* hole_at,
-                                                     int* swap_src, int* swap_dst) {
-    const int i = cell_id();
-    const int p = n_short + i;
-    if (p >= n_large || !act_mark[i]) return;
-    const int r = n_swap - act_incl[i];
-    if (r < 0 || r >= n_swap) return;   // unreachable for consistent counts (checked on the host)
+    const int n = A.n;
+    const int n_swap = n_short >= n ? nz_total : rank[n_short].nz;
+    int r;
+    if (p >= n) {
+        r = n_large - 1 - p;
+    } else {
+        if (A.flag[p] != 0) return;   // a changed cell leaves its position
+        r = (n_large - n) + (n - 1 - p) - (nz_total - rank[p].nz);
+    }
+    if (r < 0 || r >= n_swap) {
+        atomicOr(err, 2);
+        return;
+    }
     const int h = hole_at[r];
     A.next[p] = h;
     swap_src[r] = p;
@@ -393,113 +396,49 @@ __device__ void retain_cell(const AmrMesh& A, int ic) {
     }
 }
 
-// SmFluid distributor split "slope" / "const" (sm_fluid.cpp:395-458).
-__device__ void split_data(const AmrMesh& A, const DevState& z, const DevGrad& G, int* wait, const Eos& eos,
-                           int ip, int mc, bool slope) {
-    const Z zp{z.r[ip], z.u[ip], z.v[ip], z.p[ip], z.e[ip]};
-    bool bad = !slope;
-    if (slope) {
-        const int st = A.stride;
-        const double gxr = G.dx[ip], gxu = G.dx[st + ip], gxv = G.dx[2 * st + ip], gxp = G.dx[3 * st + ip];
-        const double gyr = G.dy[ip], gyu = G.dy[st + ip], gyv = G.dy[2 * st + ip], gyp = G.dy[3 * st + ip];
-        const double dR = (eos.g - 1.0) * (zp.e - eos.e0);
-        const double dE = (eos.g - 1.0) * zp.r;
-        const double gex = (gxp - dR * gxr) / dE, gey = (gyp - dR * gyr) / dE;
-        Z ch[4];
-        for (int k = 0; k < 4 && !bad; ++k) {
-            const int c = mc + k;
-            const double drx = A.cx[c] - A.cx[ip], dry = A.cy[c] - A.cy[ip];
-            Z q = zp;
-            q.r = zp.r + gxr * drx + gyr * dry;
-            const double ratio = zp.r / q.r;
-            q.u = zp.u + ratio * (gxu * drx + gyu * dry);
-            q.v = zp.v + ratio * (gxv * drx + gyv * dry);
-            const double du = zp.u - q.u, dv = zp.v - q.v;
-            q.e = zp.e - 0.5 * (du * du + dv * dv) + ratio * (gex * drx + gey * dry);
-            q.p = eos_p(eos, q.r, q.e);
-            if (is_bad(q, eos)) bad = true;
-            ch[k] = q;
-        }
-        if (!bad) {
-            for (int k = 0; k < 4; ++k) {
-                const int c = mc + k;
-                z.r[c] = ch[k].r; z.u[c] = ch[k].u; z.v[c] = ch[k].v; z.p[c] = ch[k].p; z.e[c] = ch[k].e;
-                wait[c] = 0;
-            }
-        }
-    }
-    if (bad) {
-        for (int k = 0; k < 4; ++k) {
-            const int c = mc + k;
-            z.r[c] = zp.r; z.u[c] = zp.u; z.v[c] = zp.v; z.p[c] = zp.p; z.e[c] = zp.e;
-            wait[c] = 0;
-        }
-    }
-}
-
-// refine_cell / make_children / link_siblings (refine.h:36-444).
-__device__ void refine_cell(const AmrMesh& A, const DevState& z, const DevGrad& G, int* wait, const Eos& eos,
-                            int ip, bool slope) {
+// One child of refine_cell / make_children / link_siblings (refine.h:36-444):
+// child i of parent ip, written only into the child's own cell and slots.
+__device__ void refine_child(const AmrMesh& A, int ip, int i) {
     const int mc = A.next[ip];
-    V2 V[9];
-    for (int k = 0; k < 9; ++k) V[k] = vtx(A, ip, k);
-    for (int i = 0; i < 4; ++i)
-        set_cell(A, mc + i, V[cCorners[i][0]], V[cCorners[i][1]], V[cCorners[i][2]], V[cCorners[i][3]]);
-    const int lvl = A.level[ip], zi = A.z_idx[ip], bi = A.b_idx[ip];
-    for (int i = 0; i < 4; ++i) {
-        const int ich = mc + i;
-        A.idx[ich] = ich;
-        A.flag[ich] = 0;
-        A.b_idx[ich] = bi;
-        A.level[ich] = lvl + 1;
-        A.z_idx[ich] = 4 * zi + i;
-    }
+    const int ich = mc + i;
+    set_cell(A, ich, vtx(A, ip, cCorners[i][0]), vtx(A, ip, cCorners[i][1]), vtx(A, ip, cCorners[i][2]),
+             vtx(A, ip, cCorners[i][3]));
+    const int lvl = A.level[ip], zi = A.z_idx[ip];
+    A.idx[ich] = ich;
+    A.flag[ich] = 0;
+    A.b_idx[ich] = A.b_idx[ip];
+    A.level[ich] = lvl + 1;
+    A.z_idx[ich] = 4 * zi + i;
     for (int l = 0; l < 8; ++l) {
-        const int f = SI(A, mc + cLinks[l][0], cLinks[l][1]);
+        if (cLinks[l][0] != i) continue;
+        const int f = SI(A, ich, cLinks[l][1]);
         A.bnd[f] = B_INNER;
         A.adj[f] = A.next[mc + cLinks[l][2]];
     }
     for (int side = 0; side < 4; ++side) {
+        const int sub = cSubfaceBySideChild[side][i];
+        if (sub < 0) continue;   // child i does not touch this side
         const int pf = SI(A, ip, side);
         const int bflag = A.bnd[pf];
         if (bflag == B_UNDEF) continue;
-        for (int k = 0; k < 2; ++k) A.bnd[SI(A, mc + cChildrenBySide[side][k], side)] = bflag;
+        const int chf = SI(A, ich, side);
+        A.bnd[chf] = bflag;
         if (is_bnd(bflag)) {
-            for (int k = 0; k < 2; ++k) {
-                const int i = cChildrenBySide[side][k];
-                A.adj[SI(A, mc + i, side)] = A.next[mc + i];
-            }
+            A.adj[chf] = A.next[ich];
             continue;
         }
         if (simple_face(A, ip, side)) {
             const int jc = A.adj[pf];
-            for (int k = 0; k < 2; ++k) {
-                const int i = cChildrenBySide[side][k];
-                const int chf = SI(A, mc + i, side);
-                if (lvl > A.level[jc]) A.adj[chf] = A.next[A.next[jc] + adjacent_child(side, zi & 3)];
-                else if (A.flag[jc] == 0) A.adj[chf] = A.next[jc];
-                else A.adj[chf] = A.next[A.next[jc] + adjacent_child(side, i)];
-            }
+            if (lvl > A.level[jc]) A.adj[chf] = A.next[A.next[jc] + adjacent_child(side, zi & 3)];
+            else if (A.flag[jc] == 0) A.adj[chf] = A.next[jc];
+            else A.adj[chf] = A.next[A.next[jc] + adjacent_child(side, i)];
         } else {
-            for (int k = 0; k < 2; ++k) {
-                const int i = cChildrenBySide[side][k];
-                const int chf = SI(A, mc + i, side);
-                const int jc = A.adj[SI(A, ip, cSubfaceBySideChild[side][i])];
-                const int fl = A.flag[jc];
-                if (fl == 0) A.adj[chf] = A.next[jc];
-                else if (fl < 0) A.adj[chf] = A.next[A.next[jc]];
-                else A.adj[chf] = jc;
-            }
-        }
-    }
-    for (int side = 0; side < 4; ++side) {
-        const int pf = SI(A, ip, side);
-        if (simple_face(A, ip, side) || A.bnd[pf] == B_UNDEF || is_bnd(A.bnd[pf])) continue;
-        for (int k = 0; k < 2; ++k) {
-            const int i = cChildrenBySide[side][k];
-            const int ich = mc + i;
-            const int jc = A.adj[SI(A, ip, cSubfaceBySideChild[side][i])];
-            if (A.level[jc] + A.flag[jc] > lvl + 1) {
+            const int jc = A.adj[SI(A, ip, sub)];
+            const int fl = A.flag[jc];
+            if (fl == 0) A.adj[chf] = A.next[jc];
+            else if (fl < 0) A.adj[chf] = A.next[A.next[jc]];
+            else A.adj[chf] = jc;
+            if (A.level[jc] + fl > lvl + 1) {
                 split_face(A, ich, side);
                 const int neib_next = A.next[jc];
                 A.adj[SI(A, ich, side)] = A.next[neib_next + cNeibChild0[side]];
@@ -507,17 +446,35 @@ __device__ void refine_cell(const AmrMesh& A, const DevState& z, const DevGrad& 
             }
         }
     }
-    split_data(A, z, G, wait, eos, ip, mc, slope);
-    A.idx[ip] = -1;
+}
+
+// SmFluid distributor split "slope" (sm_fluid.cpp:395-458) for child c of
+// parent ip: returns false if the reconstructed state is bad.
+__device__ bool split_slope(const AmrMesh& A, const DevState& z, const DevGrad& G, const Eos& eos, int ip, int c,
+                            Z& q) {
+    const Z zp{z.r[ip], z.u[ip], z.v[ip], z.p[ip], z.e[ip]};
+    const int st = A.stride;
+    const double gxr = G.dx[ip], gxu = G.dx[st + ip], gxv = G.dx[2 * st + ip], gxp = G.dx[3 * st + ip];
+    const double gyr = G.dy[ip], gyu = G.dy[st + ip], gyv = G.dy[2 * st + ip], gyp = G.dy[3 * st + ip];
+    const double dR = (eos.g - 1.0) * (zp.e - eos.e0);
+    const double dE = (eos.g - 1.0) * zp.r;
+    const double gex = (gxp - dR * gxr) / dE, gey = (gyp - dR * gyr) / dE;
+    const double drx = A.cx[c] - A.cx[ip], dry = A.cy[c] - A.cy[ip];
+    q = zp;
+    q.r = zp.r + gxr * drx + gyr * dry;
+    const double ratio = zp.r / q.r;
+    q.u = zp.u + ratio * (gxu * drx + gyu * dry);
+    q.v = zp.v + ratio * (gxv * drx + gyv * dry);
+    const double du = zp.u - q.u, dv = zp.v - q.v;
+    q.e = zp.e - 0.5 * (du * du + dv * dv) + ratio * (gex * drx + gey * dry);
+    q.p = eos_p(eos, q.r, q.e);
+    return !is_bad(q, eos);
 }
 
 // coarse_cell / select_children / make_parent (coarse.h:95-404) and the
-// conservative merge (sm_fluid.cpp:382-393).
-__device__ void coarse_cell(const AmrMesh& A, const DevState& z, int* wait, const Eos& eos, int ich) {
-    if ((A.z_idx[ich] & 3) != 0) {
-        A.idx[ich] = -1;
-        return;
-    }
+// conservative merge (sm_fluid.cpp:382-393), for the family whose first
+// child (z_idx % 4 == 0) is ich.
+__device__ void coarse_family(const AmrMesh& A, const DevState& z, int* wait, const Eos& eos, int ich) {
     int ch[4];
     ch[0] = ich;
     int sibs[3];
@@ -572,39 +529,65 @@ __device__ void coarse_cell(const AmrMesh& A, const DevState& z, int* wait, cons
     const Z zp = to_z(Q{q[0] / Vp, q[1] / Vp, q[2] / Vp, q[3] / Vp}, eos);
     z.r[ip] = zp.r; z.u[ip] = zp.u; z.v[ip] = zp.v; z.p[ip] = zp.p; z.e[ip] = zp.e;
     wait[ip] = 0;
-    A.idx[ich] = -1;
+    for (int k = 0; k < 4; ++k) A.idx[ch[k]] = -1;
 }
 
 }  // namespace
 
-__global__ void __launch_bounds__(128) k_amr_geometry(AmrMesh A, DevState z, DevGrad G, int* wait, Eos eos,
-                                                      int slope) {
-    const int ic = cell_id();
-    if (ic >= A.n) return;
-    const int f = A.flag[ic];
-    if (f == 0) retain_cell(A, ic);
-    else if (f > 0) refine_cell(A, z, G, wait, eos, ic, slope != 0);
-    else coarse_cell(A, z, wait, eos, ic);
+// One launch for the three kinds of cells, by workgroup: the first
+// blocks_retain workgroups take the retained cells (one thread each, flag 0),
+// the next blocks_refine the refined cells (a lane quad per parent, one lane
+// per child; the slope distributor's all-or-nothing fallback is decided
+// across the quad), the rest the coarsened families (one thread per family).
+__global__ void __launch_bounds__(256) k_amr_geometry(AmrMesh A, DevState z, DevGrad G, int* wait, Eos eos,
+                                                      int slope, const int* ref_list, int n_refine,
+                                                      const int* crs_list, int n_parents, int blocks_retain,
+                                                      int blocks_refine) {
+    const int b = (int)blockIdx.x;
+    if (b < blocks_retain) {
+        const int ic = b * 256 + (int)threadIdx.x;
+        if (ic >= A.n || A.flag[ic] != 0) return;
+        retain_cell(A, ic);
+        return;
+    }
+    if (b < blocks_retain + blocks_refine) {
+        const int t = (b - blocks_retain) * 256 + (int)threadIdx.x;
+        const int pi = t >> 2, i = t & 3;
+        if (pi >= n_refine) return;   // whole quads leave together
+        const int ip = ref_list[pi];
+        refine_child(A, ip, i);
+        const int c = A.next[ip] + i;
+        Z q;
+        bool ok = false;
+        if (slope) ok = split_slope(A, z, G, eos, ip, c, q);
+        const unsigned long long m = __ballot(!ok);
+        const bool any_bad = ((m >> (threadIdx.x & 60)) & 0xFULL) != 0ULL;
+        if (any_bad) q = Z{z.r[ip], z.u[ip], z.v[ip], z.p[ip], z.e[ip]};
+        z.r[c] = q.r; z.u[c] = q.u; z.v[c] = q.v; z.p[c] = q.p; z.e[c] = q.e;
+        wait[c] = 0;
+        if (i == 0) A.idx[ip] = -1;
+        return;
+    }
+    const int t = (b - blocks_retain - blocks_refine) * 256 + (int)threadIdx.x;
+    if (t >= n_parents) return;
+    coarse_family(A, z, wait, eos, crs_list[t]);
 }
 
 // SwapLists::move_elements (swap_lists.h:129-138) + AmrCells::move_item
-// (amr_cells.cpp:428-468) + the data rows (init layer, wait).
-__global__ void __launch_bounds__(256) k_amr_move(AmrMesh A, DevState z, int* wait, int n_swap,
-                                                  const int* swap_src, const int* swap_dst) {
-    const int i = cell_id();
+// (amr_cells.cpp:428-468) + the data rows (init layer, wait).  16 threads
+// per move: lanes 0-7 copy one face slot each, lane 8 the vertices, lane 9
+// the cell arrays and the state.
+__global__ void __launch_bounds__(256) k_amr_move(AmrMesh A, DevState z, int* wait, int n_short, int nz_total,
+                                                  const Marks3* rank, int bound, const int* swap_src,
+                                                  const int* swap_dst) {
+    const int t = cell_id();
+    const int i = t >> 4, k = t & 15;
+    if (i >= bound || k >= 10) return;
+    const int n_swap = n_short >= A.n ? nz_total : rank[n_short].nz;   // the holes
     if (i >= n_swap) return;
     const int from = swap_src[i], to = swap_dst[i];
-    A.next[to] = to;
-    A.idx[to] = to;
-    A.flag[to] = A.flag[from];
-    A.level[to] = A.level[from];
-    A.b_idx[to] = A.b_idx[from];
-    A.z_idx[to] = A.z_idx[from];
-    A.cx[to] = A.cx[from];
-    A.cy[to] = A.cy[from];
-    A.vol[to] = A.vol[from];
-    for (int s = 0; s < 8; ++s) {
-        const int a = SI(A, from, s), b = SI(A, to, s);
+    if (k < 8) {
+        const int a = SI(A, from, k), b = SI(A, to, k);
         A.bnd[b] = A.bnd[a];
         A.adj[b] = A.adj[a];
         A.nx[b] = A.nx[a];
@@ -612,14 +595,25 @@ __global__ void __launch_bounds__(256) k_amr_move(AmrMesh A, DevState z, int* wa
         A.fcx[b] = A.fcx[a];
         A.fcy[b] = A.fcy[a];
         A.area[b] = A.area[a];
+    } else if (k == 8) {
+        for (int v = 0; v < 9; ++v) {
+            A.vx[v * A.stride + to] = A.vx[v * A.stride + from];
+            A.vy[v * A.stride + to] = A.vy[v * A.stride + from];
+        }
+    } else {
+        A.next[to] = to;
+        A.idx[to] = to;
+        A.flag[to] = A.flag[from];
+        A.level[to] = A.level[from];
+        A.b_idx[to] = A.b_idx[from];
+        A.z_idx[to] = A.z_idx[from];
+        A.cx[to] = A.cx[from];
+        A.cy[to] = A.cy[from];
+        A.vol[to] = A.vol[from];
+        A.idx[from] = -1;
+        z.r[to] = z.r[from]; z.u[to] = z.u[from]; z.v[to] = z.v[from]; z.p[to] = z.p[from]; z.e[to] = z.e[from];
+        wait[to] = wait[from];
     }
-    for (int k = 0; k < 9; ++k) {
-        A.vx[k * A.stride + to] = A.vx[k * A.stride + from];
-        A.vy[k * A.stride + to] = A.vy[k * A.stride + from];
-    }
-    A.idx[from] = -1;
-    z.r[to] = z.r[from]; z.u[to] = z.u[from]; z.v[to] = z.v[from]; z.p[to] = z.p[from]; z.e[to] = z.e[from];
-    wait[to] = wait[from];
 }
 
 // Correctly rounded x^3 for x > 0: x^2 = p + e1 and p x = q + e2 exactly
@@ -635,72 +629,99 @@ __device__ __forceinline__ double cube_cr(double x) {
 
 // Derived per-cell data after an adaptation: slot code words (type and
 // general-normal bit), LSM weights w = area / |drn|^3 (gradient.h:251-256)
-// and incircle diameters (amr_cells.cpp:107-114).
-// counts[0]: inconsistency flag; counts[1], counts[2]: defined interior
-// (inner + periodic) and boundary slots, for the unique-face count
-// interior / 2 + boundary.
-__global__ void __launch_bounds__(256) k_amr_derive(AmrMesh A, int* counts) {
+// and incircle diameters (amr_cells.cpp:107-114).  counts[0], counts[1]:
+// defined interior (inner + periodic) and boundary slots, for the unique-face
+// count interior / 2 + boundary (one atomic per workgroup).  An adjacency
+// out of range sets bit 2 of the error word.
+__global__ void __launch_bounds__(256) k_amr_derive(AmrMesh A, DevState z, Eos eos, unsigned long long* counts,
+                                                    unsigned long long* dt_slot, int* err) {
     const int c = cell_id();
     int n_int = 0, n_bnd = 0;
+    double cand = INFINITY;
     if (c < A.n) {
-    const double kNX[4] = {-1.0, 1.0, 0.0, 0.0}, kNY[4] = {0.0, 0.0, -1.0, 1.0};
-    int cw = 0;
-    const double cx = A.cx[c], cy = A.cy[c];
-    for (int s = 0; s < 8; ++s) {
-        const int f = SI(A, c, s);
-        const int b = A.bnd[f];
-        int t;
-        switch (b) {
-            case B_UNDEF: t = SC_UNDEF; break;
-            case B_INNER: t = SC_INNER; break;
-            case B_PERIODIC: t = SC_PERIODIC; break;
-            case 1: t = SC_WALL; break;
-            default: t = SC_COPY; break;
+        const double kNX[4] = {-1.0, 1.0, 0.0, 0.0}, kNY[4] = {0.0, 0.0, -1.0, 1.0};
+        int cw = 0;
+        const double cx = A.cx[c], cy = A.cy[c];
+        for (int s = 0; s < 8; ++s) {
+            const int f = SI(A, c, s);
+            const int b = A.bnd[f];
+            int t;
+            switch (b) {
+                case B_UNDEF: t = SC_UNDEF; break;
+                case B_INNER: t = SC_INNER; break;
+                case B_PERIODIC: t = SC_PERIODIC; break;
+                case 1: t = SC_WALL; break;
+                default: t = SC_COPY; break;
+            }
+            if (t == SC_UNDEF) {
+                A.adj[f] = c;
+                continue;
+            }
+            const int side = s & 3;
+            const double nx = A.nx[f], ny = A.ny[f];
+            const bool gen = !(nx == kNX[side] && ny == kNY[side]);
+            cw |= (t | (gen ? kGeneralNormal : 0)) << (4 * s);
+            if (t >= SC_WALL) ++n_bnd;
+            else ++n_int;
+            const int nb = A.adj[f];
+            if (nb < 0 || nb >= A.n || (t >= SC_WALL && nb != c)) {   // inconsistent mesh: report, keep in bounds
+                A.adj[f] = c;
+                atomicOr(err, 2);
+                continue;
+            }
+            double px, py;
+            if (t == SC_INNER) {
+                px = A.cx[nb];
+                py = A.cy[nb];
+            } else {
+                const double d2 = 2.0 * ((A.fcx[f] - cx) * nx + (A.fcy[f] - cy) * ny);
+                px = cx + d2 * nx;
+                py = cy + d2 * ny;
+            }
+            const double drx = px - cx, dry = py - cy;
+            const double d = drx * nx + dry * ny;
+            const double dnx = d * nx, dny = d * ny;
+            const double len = sqrt(dnx * dnx + dny * dny);
+            A.w[f] = (1.0 * A.area[f]) / cube_cr(len);
         }
-        if (t == SC_UNDEF) {
-            A.adj[f] = c;
-            continue;
-        }
-        const int side = s & 3;
-        const double nx = A.nx[f], ny = A.ny[f];
-        const bool gen = !(nx == kNX[side] && ny == kNY[side]);
-        cw |= (t | (gen ? kGeneralNormal : 0)) << (4 * s);
-        if (t >= SC_WALL) ++n_bnd;
-        else ++n_int;
-        const int nb = A.adj[f];
-        if (nb < 0 || nb >= A.n || (t >= SC_WALL && nb != c)) {   // inconsistent mesh: report, keep in bounds
-            A.adj[f] = c;
-            counts[0] = 1;
-            continue;
-        }
-        double px, py;
-        if (t == SC_INNER) {
-            px = A.cx[nb];
-            py = A.cy[nb];
-        } else {
-            const double d2 = 2.0 * ((A.fcx[f] - cx) * nx + (A.fcy[f] - cy) * ny);
-            px = cx + d2 * nx;
-            py = cy + d2 * ny;
-        }
-        const double drx = px - cx, dry = py - cy;
-        const double d = drx * nx + dry * ny;
-        const double dnx = d * nx, dny = d * ny;
-        const double len = sqrt(dnx * dnx + dny * dny);
-        A.w[f] = (1.0 * A.area[f]) / cube_cr(len);
+        A.code[c] = cw;
+        const V2 a = vtx(A, c, 5), b = vtx(A, c, 3), p = vtx(A, c, 7), q = vtx(A, c, 1);
+        const double ax = a.x - b.x, ay = a.y - b.y, bx = p.x - q.x, by = p.y - q.y;
+        const double s1 = ax * ax + ay * ay, s2 = bx * bx + by * by;
+        const double h = sqrt((s2 < s1) ? s2 : s1);
+        A.h[c] = h;
+        // CFL candidate of the adapted state (sm_fluid.cpp:104-114)
+        const Z zc{z.r[c], z.u[c], z.v[c], z.p[c], z.e[c]};
+        cand = h / (sqrt(vsq(zc)) + eos_c(eos, zc.r, zc.p));
     }
-    A.code[c] = cw;
-    const V2 a = vtx(A, c, 5), b = vtx(A, c, 3), p = vtx(A, c, 7), q = vtx(A, c, 1);
-    const double ax = a.x - b.x, ay = a.y - b.y, bx = p.x - q.x, by = p.y - q.y;
-    const double s1 = ax * ax + ay * ay, s2 = bx * bx + by * by;
-    A.h[c] = sqrt((s2 < s1) ? s2 : s1);
-    }
+    if (!(cand < INFINITY)) cand = INFINITY;   // NaN and +inf never win
     for (int off = 32; off > 0; off >>= 1) {
         n_int += __shfl_xor(n_int, off, 64);
         n_bnd += __shfl_xor(n_bnd, off, 64);
+        const double o = __shfl_xor(cand, off, 64);
+        cand = (o < cand) ? o : cand;
     }
-    if ((threadIdx.x & 63) == 0) {
-        if (n_int) atomicAdd(&counts[1], n_int);
-        if (n_bnd) atomicAdd(&counts[2], n_bnd);
+    __shared__ int red[2][4];
+    __shared__ double dred[4];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    if (lane == 0) {
+        red[0][wid] = n_int;
+        red[1][wid] = n_bnd;
+        dred[wid] = cand;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const int nw = (blockDim.x + 63) >> 6;
+        unsigned long long ti = 0, tb = 0;
+        double m = dred[0];
+        for (int w = 0; w < nw; ++w) {
+            ti += (unsigned long long)red[0][w];
+            tb += (unsigned long long)red[1][w];
+            m = (dred[w] < m) ? dred[w] : m;
+        }
+        if (ti) atomicAdd(&counts[0], ti);
+        if (tb) atomicAdd(&counts[1], tb);
+        if (m < INFINITY) atomicMin(dt_slot, (unsigned long long)__double_as_longlong(m));
     }
 }
 
@@ -725,58 +746,57 @@ hipError_t amr_balance(const AmrMesh& A, int max_level, hipStream_t s) {
     return hipGetLastError();
 }
 
-hipError_t amr_count(const AmrMesh& A, int* counts, hipStream_t s) {
-    hipLaunchKernelGGL(k_amr_count, grid256(A.n), dim3(256), 0, s, A, counts);
+struct Add3 {
+    __host__ __device__ Marks3 operator()(const Marks3& a, const Marks3& b) const {
+        return Marks3{a.ref + b.ref, a.head + b.head, a.nz + b.nz};
+    }
+};
+
+hipError_t amr_scan3(const Marks3* in, Marks3* out, int count, void* tmp, size_t& tmp_bytes, hipStream_t s) {
+    return hipcub::DeviceScan::ExclusiveScan(tmp, tmp_bytes, in, out, Add3(), Marks3{0, 0, 0}, count, s);
+}
+
+hipError_t amr_mark(const AmrMesh& A, Marks3* marks, hipStream_t s) {
+    hipLaunchKernelGGL(k_amr_mark, grid256(A.n), dim3(256), 0, s, A, marks);
     return hipGetLastError();
 }
 
-hipError_t amr_scan(const int* in, int* out, int count, bool inclusive, void* tmp, size_t& tmp_bytes,
-                    hipStream_t s) {
-    if (inclusive) return hipcub::DeviceScan::InclusiveSum(tmp, tmp_bytes, in, out, count, s);
-    return hipcub::DeviceScan::ExclusiveSum(tmp, tmp_bytes, in, out, count, s);
-}
-
-hipError_t amr_mark(const AmrMesh& A, int n_short, int* ref_mark, int* crs_mark, int* hole_mark, hipStream_t s) {
-    hipLaunchKernelGGL(k_amr_mark, grid256(A.n), dim3(256), 0, s, A, n_short, ref_mark, crs_mark, hole_mark);
+hipError_t amr_positions(const AmrMesh& A, int n_large, int n_short, int n_parents, const Marks3* rank,
+                         int* hole_at, int* ref_list, int* crs_list, unsigned long long* faces,
+                         unsigned long long* dt_rd, unsigned long long* dt_wr, hipStream_t s) {
+    hipLaunchKernelGGL(k_amr_positions, grid256(n_large), dim3(256), 0, s, A, n_large, n_short, n_parents, rank,
+                       hole_at, ref_list, crs_list, faces, dt_rd, dt_wr);
     return hipGetLastError();
 }
 
-hipError_t amr_positions(const AmrMesh& A, int n_large, int n_parents, const int* ref_rank, const int* crs_rank,
-                         const int* hole_mark, const int* hole_rank, int* hole_at, hipStream_t s) {
-    hipLaunchKernelGGL(k_amr_positions, grid256(n_large), dim3(256), 0, s, A, n_large, n_parents, ref_rank,
-                       crs_rank, hole_mark, hole_rank, hole_at);
-    return hipGetLastError();
-}
-
-hipError_t amr_actual_mark(const AmrMesh& A, int n_short, int n_large, int* act_mark, hipStream_t s) {
-    hipLaunchKernelGGL(k_amr_actual_mark, grid256(n_large - n_short), dim3(256), 0, s, A, n_short, n_large,
-                       act_mark);
-    return hipGetLastError();
-}
-
-hipError_t amr_swapmap(const AmrMesh& A, int n_short, int n_large, int n_swap, const int* act_mark,
-                       const int* act_incl, const int* hole_at, int* swap_src, int* swap_dst, hipStream_t s) {
-    hipLaunchKernelGGL(k_amr_swapmap, grid256(n_large - n_short), dim3(256), 0, s, A, n_short, n_large, n_swap,
-                       act_mark, act_incl, hole_at, swap_src, swap_dst);
+hipError_t amr_swapmap(const AmrMesh& A, int n_short, int n_large, int nz_total, const Marks3* rank,
+                       const int* hole_at, int* swap_src, int* swap_dst, int* err, hipStream_t s) {
+    if (n_large <= n_short) return hipSuccess;
+    hipLaunchKernelGGL(k_amr_swapmap, grid256(n_large - n_short), dim3(256), 0, s, A, n_short, n_large, nz_total,
+                       rank, hole_at, swap_src, swap_dst, err);
     return hipGetLastError();
 }
 
 hipError_t amr_geometry(const AmrMesh& A, const DevState& z, const DevGrad& G, int* wait, const Eos& eos,
-                        bool slope, hipStream_t s) {
-    hipLaunchKernelGGL(k_amr_geometry, dim3((A.n + 127) / 128), dim3(128), 0, s, A, z, G, wait, eos,
-                       slope ? 1 : 0);
+                        bool slope, const int* ref_list, int n_refine, const int* crs_list, int n_parents,
+                        hipStream_t s) {
+    const int br = (A.n + 255) / 256, bf = (4 * n_refine + 255) / 256, bc = (n_parents + 255) / 256;
+    hipLaunchKernelGGL(k_amr_geometry, dim3(br + bf + bc), dim3(256), 0, s, A, z, G, wait, eos, slope ? 1 : 0,
+                       ref_list, n_refine, crs_list, n_parents, br, bf);
     return hipGetLastError();
 }
 
-hipError_t amr_move(const AmrMesh& A, const DevState& z, int* wait, int n_swap, const int* swap_src,
-                    const int* swap_dst, hipStream_t s) {
-    if (n_swap <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_amr_move, grid256(n_swap), dim3(256), 0, s, A, z, wait, n_swap, swap_src, swap_dst);
+hipError_t amr_move(const AmrMesh& A, const DevState& z, int* wait, int n_short, int nz_total, const Marks3* rank,
+                    int bound, const int* swap_src, const int* swap_dst, hipStream_t s) {
+    if (bound <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_amr_move, grid256(16 * bound), dim3(256), 0, s, A, z, wait, n_short, nz_total, rank, bound,
+                       swap_src, swap_dst);
     return hipGetLastError();
 }
 
-hipError_t amr_derive(const AmrMesh& A, int* counts, hipStream_t s) {
-    hipLaunchKernelGGL(k_amr_derive, grid256(A.n), dim3(256), 0, s, A, counts);
+hipError_t amr_derive(const AmrMesh& A, const DevState& z, const Eos& eos, unsigned long long* counts,
+                      unsigned long long* dt_slot, int* err, hipStream_t s) {
+    hipLaunchKernelGGL(k_amr_derive, grid256(A.n), dim3(256), 0, s, A, z, eos, counts, dt_slot, err);
     return hipGetLastError();
 }
 

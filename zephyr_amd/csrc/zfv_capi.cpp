@@ -80,9 +80,11 @@ struct zfv_ctx {
     DBuf<double> cx, cy, vol, h, area, fcx, fcy, w, nx, ny, vx, vy;
     DBuf<int> code, adj, bnd, level, flag, flag0, b_idx, z_idx, next, idx;
     // AMR scratch
-    DBuf<int> s_ref_mark, s_crs_mark, s_hole_mark, s_ref_rank, s_crs_rank, s_hole_rank, s_hole_at;
-    DBuf<int> s_act_mark, s_act_incl, s_swap_src, s_swap_dst, s_counts;
+    DBuf<Marks3> s_marks, s_rank;
+    DBuf<unsigned long long> s_faces;
+    DBuf<int> s_hole_at, s_ref_list, s_crs_list, s_swap_src, s_swap_dst;
     DBuf<unsigned char> s_tmp;
+    bool faces_pending = false;   // n_faces still on the device (s_faces)
     // device state
     DBuf<double> st[3][5];   // init / next / half layers
     DBuf<double> gdx, gdy;
@@ -91,6 +93,7 @@ struct zfv_ctx {
     int cur = 0, wcur = 0;   // index of the init layer / current wait buffer
     DBuf<unsigned long long> dtmin;
     DBuf<double> dt_out, partial;
+    DBuf<double> dmax_dt;   // set_max_dt value on the device
     DBuf<int> err;
 
     int64_t n_faces = 0;   // unique faces: interior slots / 2 + boundary slots
@@ -99,8 +102,33 @@ struct zfv_ctx {
     bool dt_valid = false;
     long long step_no = 0;
 
+    // step-kernel layout per stage (kLayoutCell / kLayoutQuad): first-order
+    // update, gradient + predictor, corrector
+    int layout[3] = {kLayoutCell, kLayoutCell, kLayoutCell};
+    int block[3] = {kBlock, kBlock, kBlock};   // workgroup size per stage
+
+    // batch of independent problems (nseg > 1): padded cell offsets, valid
+    // cell counts, device tables of the kernels' Tile lookup
+    int nseg = 1;
+    std::vector<int> seg_begin_h, seg_n_h;
+    DBuf<int> chunk_pid, seg_begin, seg_end;
+
+    // captured step graphs: kGraphSteps steps, keyed by the phase they start
+    // in (dt slot rotation, layer and wait buffer parity, scheme)
+    bool use_graphs = false;
+    struct GraphEntry {
+        int key;
+        hipGraphExec_t exec;
+    };
+    std::vector<GraphEntry> graphs;
+    void graphs_clear() {
+        for (auto& g : graphs) (void)hipGraphExecDestroy(g.exec);
+        graphs.clear();
+    }
+
     bool profile = false;
     std::vector<KernelTimer> timers;   // pending event pairs of the last zfv_smf_step
+    std::vector<hipEvent_t> event_pool;
     std::map<std::string, std::pair<double, int>> prof;   // name -> (ms, launches)
 
     DevMesh dmesh() const {
@@ -110,6 +138,9 @@ struct zfv_ctx {
         M.cx = cx.p; M.cy = cy.p; M.vol = vol.p; M.h = h.p;
         M.code = code.p; M.adj = adj.p; M.area = area.p; M.fcx = fcx.p; M.fcy = fcy.p; M.w = w.p;
         M.nx = nx.p; M.ny = ny.p;
+        M.chunk_pid = nseg > 1 ? chunk_pid.p : nullptr;
+        M.seg_begin = nseg > 1 ? seg_begin.p : nullptr;
+        M.seg_end = nseg > 1 ? seg_end.p : nullptr;
         return M;
     }
     AmrMesh amesh() const {
@@ -274,12 +305,13 @@ int upload_mesh(zfv_ctx* ctx) {
         ZFV_HIP(ctx->wait[k].alloc(n));
         ZFV_HIP(hipMemsetAsync(ctx->wait[k].p, 0, n * sizeof(int), s));
     }
-    ZFV_HIP(ctx->dtmin.alloc(3));
-    ZFV_HIP(ctx->dt_out.alloc(1));
+    ZFV_HIP(ctx->dtmin.alloc(3 * (size_t)ctx->nseg));
+    ZFV_HIP(ctx->dt_out.alloc(ctx->nseg));
     ZFV_HIP(ctx->err.alloc(1));
     ZFV_HIP(hipMemsetAsync(ctx->err.p, 0, sizeof(int), s));
     ZFV_HIP(ctx->partial.alloc(4 * (size_t)((n + kBlock - 1) / kBlock)));
     ZFV_HIP(hipStreamSynchronize(s));
+    ctx->graphs_clear();
     ctx->have_mesh = true;
     ctx->host_stale = false;
     ctx->have_state = false;
@@ -289,10 +321,16 @@ int upload_mesh(zfv_ctx* ctx) {
     return ZFV_OK;
 }
 
+// Error word written by the kernels: bit 0 numeric (NaN flux / Riemann
+// solve), bit 1 inconsistent mesh after an adaptation.
 int read_err(zfv_ctx* ctx) {
     int e = 0;
     ZFV_HIP(hipMemcpyAsync(&e, ctx->err.p, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
     ZFV_HIP(hipStreamSynchronize(ctx->stream));
+    if (e & 2) {
+        ctx->have_state = false;
+        return fail(ZFV_ERR_STATE, "AMR apply produced an inconsistent mesh; state invalidated");
+    }
     if (e) return fail(ZFV_ERR_NUMERIC, "numeric error: a flux or Riemann solve produced NaN / did not converge");
     return ZFV_OK;
 }
@@ -323,6 +361,7 @@ hipError_t grow_rows(DBuf<T>& b, int rows, int old_cap, int new_cap, int n_valid
 // Grow every capacity-sized device array to hold new_cap cells.
 int grow_capacity(zfv_ctx* ctx, int new_cap) {
     if (new_cap <= ctx->cap) return ZFV_OK;
+    ctx->graphs_clear();   // buffers move
     const int oc = ctx->cap, n = ctx->hm.n;
     hipStream_t s = ctx->stream;
     for (auto* b : {&ctx->cx, &ctx->cy, &ctx->vol, &ctx->h}) ZFV_HIP(grow_rows(*b, 1, oc, new_cap, n, s));
@@ -391,15 +430,26 @@ int sync_host_mesh(zfv_ctx* ctx) {
 void collect_timers(zfv_ctx* ctx) {
     for (auto& t : ctx->timers) {
         float ms = 0.0f;
-        hipEventSynchronize(t.t1);
-        hipEventElapsedTime(&ms, t.t0, t.t1);
+        (void)hipEventSynchronize(t.t1);
+        (void)hipEventElapsedTime(&ms, t.t0, t.t1);
         auto& slot = ctx->prof[t.name];
         slot.first += ms;
         slot.second += 1;
-        hipEventDestroy(t.t0);
-        hipEventDestroy(t.t1);
+        ctx->event_pool.push_back(t.t0);
+        ctx->event_pool.push_back(t.t1);
     }
     ctx->timers.clear();
+}
+
+hipEvent_t pooled_event(zfv_ctx* ctx) {
+    if (!ctx->event_pool.empty()) {
+        hipEvent_t e = ctx->event_pool.back();
+        ctx->event_pool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    (void)hipEventCreate(&e);
+    return e;
 }
 
 template <class F>
@@ -407,20 +457,19 @@ hipError_t timed(zfv_ctx* ctx, const char* name, F&& launch) {
     if (!ctx->profile) return launch();
     KernelTimer t;
     t.name = name;
-    hipEventCreate(&t.t0);
-    hipEventCreate(&t.t1);
-    hipEventRecord(t.t0, ctx->stream);
+    t.t0 = pooled_event(ctx);
+    t.t1 = pooled_event(ctx);
+    (void)hipEventRecord(t.t0, ctx->stream);
     hipError_t e = launch();
-    hipEventRecord(t.t1, ctx->stream);
+    (void)hipEventRecord(t.t1, ctx->stream);
     ctx->timers.push_back(t);
     return e;
 }
 
 int ensure_dt(zfv_ctx* ctx) {
     if (ctx->dt_valid) return ZFV_OK;
-    int rd = (int)(ctx->step_no % 3), wr = (int)((ctx->step_no + 1) % 3);
-    ZFV_HIP(launch_fill_u64(ctx->dtmin.p + rd, 1, 0x7ff0000000000000ULL, ctx->stream));
-    ZFV_HIP(launch_fill_u64(ctx->dtmin.p + wr, 1, 0x7ff0000000000000ULL, ctx->stream));
+    int rd = (int)(ctx->step_no % 3);
+    ZFV_HIP(launch_fill_u64(ctx->dtmin.p, 3 * ctx->nseg, 0x7ff0000000000000ULL, ctx->stream));
     DevMesh M = ctx->dmesh();
     DevState z = ctx->layer(ctx->cur);
     Eos eos = ctx->eos();
@@ -436,7 +485,7 @@ StepCtl make_ctl(zfv_ctx* ctx) {
     c.wr = (int)((ctx->step_no + 1) % 3);
     c.clr = (int)((ctx->step_no + 2) % 3);
     c.cfl = ctx->prm.cfl;
-    c.max_dt = ctx->prm.max_dt;
+    c.max_dt = ctx->dmax_dt.p;
     c.dt_out = ctx->dt_out.p;
     c.err = ctx->err.p;
     return c;
@@ -450,6 +499,8 @@ int need_ready(zfv_ctx* ctx, bool state = true, bool config = false) {
     if (config && !ctx->configured) return fail(ZFV_ERR_STATE, "solver not configured");
     return ZFV_OK;
 }
+
+int put_max_dt(zfv_ctx* ctx);
 
 }  // namespace
 
@@ -473,6 +524,12 @@ int zfv_create(int device, zfv_ctx** out) {
         return fail(ZFV_ERR_HIP, std::string("hipStreamCreate: ") + hipGetErrorString(e));
     }
     ctx->stream = ctx->own_stream;
+    ctx->prm.max_dt = std::numeric_limits<double>::max();
+    if (ctx->dmax_dt.alloc(1) != hipSuccess || put_max_dt(ctx) != ZFV_OK) {
+        (void)hipStreamDestroy(ctx->own_stream);
+        delete ctx;
+        return fail(ZFV_ERR_HIP, "context allocation failed");
+    }
     *out = ctx;
     return ZFV_OK;
 }
@@ -482,6 +539,8 @@ int zfv_destroy(zfv_ctx* ctx) {
     hipSetDevice(ctx->device);
     hipStreamSynchronize(ctx->stream);
     collect_timers(ctx);
+    ctx->graphs_clear();
+    for (hipEvent_t e : ctx->event_pool) (void)hipEventDestroy(e);
     hipStream_t s = ctx->own_stream;
     delete ctx;
     if (s) hipStreamDestroy(s);
@@ -518,6 +577,7 @@ int zfv_mesh_upload(zfv_ctx* ctx, const zfv_mesh_soa* M) {
         return fail(ZFV_ERR_ARG, "mesh array missing");
     HostMesh& m = ctx->hm;
     const size_t ns = 8 * (size_t)n, nv = 9 * (size_t)n;
+    ctx->nseg = 1;
     m.n = n;
     m.cx.assign(M->cx, M->cx + n);
     m.cy.assign(M->cy, M->cy + n);
@@ -550,10 +610,9 @@ int zfv_mesh_upload(zfv_ctx* ctx, const zfv_mesh_soa* M) {
 // geom/generator/rectangle.cpp:465-...; AmrCells(const Rectangle&),
 // mesh/euler/amr_cells_gen.cpp:128-281): cell ic = i*ny + j, slots L,R,B,T
 // active, boundary slots reference the cell itself, 9 SqQuad vertices.
-int zfv_mesh_rectangle(zfv_ctx* ctx, int nx, int ny, double xmin, double xmax,
-                       double ymin, double ymax, const int32_t* bounds) {
-    int r = check_ctx(ctx);
-    if (r) return r;
+// Written into m at cells [off, off + nx*ny) (arrays already sized).
+static int check_rectangle(int nx, int ny, double xmin, double xmax, double ymin, double ymax,
+                           const int32_t* bounds) {
     if (nx < 1 || ny < 1) return fail(ZFV_ERR_ARG, "Rectangle: nx, ny must be >= 1");
     if (!(xmax > xmin) || !(ymax > ymin)) return fail(ZFV_ERR_ARG, "Rectangle: empty domain");
     if (!bounds) return fail(ZFV_ERR_ARG, "null bounds");
@@ -562,24 +621,31 @@ int zfv_mesh_rectangle(zfv_ctx* ctx, int nx, int ny, double xmin, double xmax,
             return fail(ZFV_ERR_ARG, "Rectangle: bad boundary id");
     if ((long long)nx * ny * 9 > (long long)std::numeric_limits<int>::max())
         return fail(ZFV_ERR_ARG, "Rectangle too large");
-    const bool xper = bounds[0] == ZFV_PERIODIC && bounds[1] == ZFV_PERIODIC;
-    const bool yper = bounds[2] == ZFV_PERIODIC && bounds[3] == ZFV_PERIODIC;
-    const int n = nx * ny;
-    HostMesh& m = ctx->hm;
+    return ZFV_OK;
+}
+
+static void resize_host_mesh(HostMesh& m, int n) {
     m.n = n;
     const size_t ns = 8 * (size_t)n;
-    m.cx.resize(n); m.cy.resize(n); m.vol.resize(n);
-    m.vx.resize(9 * (size_t)n); m.vy.resize(9 * (size_t)n);
+    m.cx.assign(n, 0.0); m.cy.assign(n, 0.0); m.vol.assign(n, 0.0);
+    m.vx.assign(9 * (size_t)n, 0.0); m.vy.assign(9 * (size_t)n, 0.0);
     m.level.assign(n, 0); m.flag.assign(n, 0); m.b_idx.resize(n); m.z_idx.assign(n, 0);
+    for (int c = 0; c < n; ++c) m.b_idx[c] = c;
     m.bnd.assign(ns, ZFV_UNDEFINED); m.adj.assign(ns, -1);
     m.nx.assign(ns, 0.0); m.ny.assign(ns, 0.0); m.fcx.assign(ns, 0.0); m.fcy.assign(ns, 0.0);
     m.area.assign(ns, 0.0);
+}
+
+static void build_rectangle(HostMesh& m, int off, int nx, int ny, double xmin, double xmax, double ymin,
+                            double ymax, const int32_t* bounds) {
+    const bool xper = bounds[0] == ZFV_PERIODIC && bounds[1] == ZFV_PERIODIC;
+    const bool yper = bounds[2] == ZFV_PERIODIC && bounds[3] == ZFV_PERIODIC;
     const double hx = (xmax - xmin) / nx, hy = (ymax - ymin) / ny;
     const double Lx = xmax - xmin, Ly = ymax - ymin;
     for (int i = 0; i < nx; ++i) {
         const double x0 = xmin + (Lx * i) / nx, x1 = xmin + (Lx * (i + 1)) / nx;
         for (int j = 0; j < ny; ++j) {
-            const int c = i * ny + j;
+            const int lc = i * ny + j, c = off + lc;
             const double y0 = ymin + (Ly * j) / ny, y1 = ymin + (Ly * (j + 1)) / ny;
             // SqQuad(v00, v01, v10, v11): rows by y, columns by x (quad.cpp:429-436)
             double* X = &m.vx[9 * (size_t)c];
@@ -597,16 +663,16 @@ int zfv_mesh_rectangle(zfv_ctx* ctx, int nx, int ny, double xmin, double xmax,
             m.cx[c] = X[4];
             m.cy[c] = Y[4];
             m.vol[c] = hx * hy;
-            m.b_idx[c] = c;
+            m.b_idx[c] = lc;
             const size_t f = 8 * (size_t)c;
             m.bnd[f + 0] = i > 0 ? ZFV_INNER : bounds[0];
             m.bnd[f + 1] = i < nx - 1 ? ZFV_INNER : bounds[1];
             m.bnd[f + 2] = j > 0 ? ZFV_INNER : bounds[2];
             m.bnd[f + 3] = j < ny - 1 ? ZFV_INNER : bounds[3];
-            m.adj[f + 0] = (i == 0 && !xper) ? c : ((i - 1 + nx) % nx) * ny + j;
-            m.adj[f + 1] = (i == nx - 1 && !xper) ? c : ((i + 1) % nx) * ny + j;
-            m.adj[f + 2] = (j == 0 && !yper) ? c : i * ny + (j - 1 + ny) % ny;
-            m.adj[f + 3] = (j == ny - 1 && !yper) ? c : i * ny + (j + 1) % ny;
+            m.adj[f + 0] = off + ((i == 0 && !xper) ? lc : ((i - 1 + nx) % nx) * ny + j);
+            m.adj[f + 1] = off + ((i == nx - 1 && !xper) ? lc : ((i + 1) % nx) * ny + j);
+            m.adj[f + 2] = off + ((j == 0 && !yper) ? lc : i * ny + (j - 1 + ny) % ny);
+            m.adj[f + 3] = off + ((j == ny - 1 && !yper) ? lc : i * ny + (j + 1) % ny);
             const int cen[4] = {3, 5, 1, 7};
             for (int s = 0; s < 4; ++s) {
                 m.nx[f + s] = kNX[s];
@@ -617,7 +683,89 @@ int zfv_mesh_rectangle(zfv_ctx* ctx, int nx, int ny, double xmin, double xmax,
             }
         }
     }
+}
+
+int zfv_mesh_rectangle(zfv_ctx* ctx, int nx, int ny, double xmin, double xmax,
+                       double ymin, double ymax, const int32_t* bounds) {
+    int r = check_ctx(ctx);
+    if (r) return r;
+    r = check_rectangle(nx, ny, xmin, xmax, ymin, ymax, bounds);
+    if (r) return r;
+    ctx->nseg = 1;
+    resize_host_mesh(ctx->hm, nx * ny);
+    build_rectangle(ctx->hm, 0, nx, ny, xmin, xmax, ymin, ymax, bounds);
     return upload_mesh(ctx);
+}
+
+int zfv_batch_rectangles(zfv_ctx* ctx, int n_problems, const int32_t* nx, const int32_t* ny, const double* box,
+                         const int32_t* bounds) {
+    int r = check_ctx(ctx);
+    if (r) return r;
+    if (n_problems < 1 || !nx || !ny) return fail(ZFV_ERR_ARG, "batch: need n_problems >= 1 and nx, ny");
+    const double unit[4] = {0.0, 1.0, 0.0, 1.0};
+    const int32_t zoe[4] = {ZFV_ZOE, ZFV_ZOE, ZFV_ZOE, ZFV_ZOE};
+    std::vector<int> begin(n_problems + 1, 0), count(n_problems);
+    long long total = 0;
+    for (int p = 0; p < n_problems; ++p) {
+        const double* b = box ? box + 4 * p : unit;
+        r = check_rectangle(nx[p], ny[p], b[0], b[1], b[2], b[3], bounds ? bounds + 4 * p : zoe);
+        if (r) return r;
+        begin[p] = (int)total;
+        count[p] = nx[p] * ny[p];
+        total += ((long long)count[p] + kSegAlign - 1) / kSegAlign * kSegAlign;
+        if (total * 9 > (long long)std::numeric_limits<int>::max()) return fail(ZFV_ERR_ARG, "batch too large");
+    }
+    begin[n_problems] = (int)total;
+    HostMesh& m = ctx->hm;
+    resize_host_mesh(m, (int)total);   // padding cells: no faces, zero volume
+    for (int p = 0; p < n_problems; ++p) {
+        const double* b = box ? box + 4 * p : unit;
+        build_rectangle(m, begin[p], nx[p], ny[p], b[0], b[1], b[2], b[3], bounds ? bounds + 4 * p : zoe);
+    }
+    ctx->nseg = n_problems;
+    ctx->seg_begin_h = begin;
+    ctx->seg_n_h = count;
+    r = upload_mesh(ctx);
+    if (r) return r;
+    const int nchunk = (int)(total >> kChunkShift);
+    std::vector<int> cp(nchunk), se(n_problems);
+    for (int p = 0; p < n_problems; ++p) {
+        se[p] = begin[p] + count[p];
+        for (int k = begin[p] >> kChunkShift; k < (begin[p + 1] >> kChunkShift); ++k) cp[k] = p;
+    }
+    ZFV_HIP(ctx->chunk_pid.alloc(nchunk));
+    ZFV_HIP(ctx->seg_begin.alloc(n_problems));
+    ZFV_HIP(ctx->seg_end.alloc(n_problems));
+    ZFV_HIP(hipMemcpyAsync(ctx->chunk_pid.p, cp.data(), nchunk * sizeof(int), hipMemcpyHostToDevice, ctx->stream));
+    ZFV_HIP(hipMemcpyAsync(ctx->seg_begin.p, begin.data(), n_problems * sizeof(int), hipMemcpyHostToDevice,
+                           ctx->stream));
+    ZFV_HIP(hipMemcpyAsync(ctx->seg_end.p, se.data(), n_problems * sizeof(int), hipMemcpyHostToDevice, ctx->stream));
+    ZFV_HIP(hipStreamSynchronize(ctx->stream));
+    return ZFV_OK;
+}
+
+int zfv_batch_layout(zfv_ctx* ctx, int32_t* n_problems, int32_t* begin, int32_t* count) {
+    int r = check_ctx(ctx);
+    if (r) return r;
+    if (!ctx->have_mesh) return fail(ZFV_ERR_STATE, "no mesh uploaded");
+    if (n_problems) *n_problems = ctx->nseg;
+    if (ctx->nseg == 1) {
+        if (begin) { begin[0] = 0; begin[1] = ctx->hm.n; }
+        if (count) count[0] = ctx->hm.n;
+        return ZFV_OK;
+    }
+    if (begin) std::memcpy(begin, ctx->seg_begin_h.data(), (ctx->nseg + 1) * sizeof(int32_t));
+    if (count) std::memcpy(count, ctx->seg_n_h.data(), ctx->nseg * sizeof(int32_t));
+    return ZFV_OK;
+}
+
+int zfv_batch_dt(zfv_ctx* ctx, double* dt) {
+    int r = need_ready(ctx, false);
+    if (r) return r;
+    if (!dt) return fail(ZFV_ERR_ARG, "null output");
+    ZFV_HIP(hipMemcpyAsync(dt, ctx->dt_out.p, ctx->nseg * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    ZFV_HIP(hipStreamSynchronize(ctx->stream));
+    return read_err(ctx);
 }
 
 int zfv_mesh_n_cells(zfv_ctx* ctx, int32_t* n) {
@@ -628,6 +776,13 @@ int zfv_mesh_n_cells(zfv_ctx* ctx, int32_t* n) {
 
 int zfv_mesh_n_faces(zfv_ctx* ctx, int64_t* n) {
     if (!ctx || !n) return fail(ZFV_ERR_ARG, "null argument");
+    if (ctx->have_mesh && ctx->faces_pending) {
+        unsigned long long fc[2] = {0, 0};
+        ZFV_HIP(hipMemcpyAsync(fc, ctx->s_faces.p, sizeof(fc), hipMemcpyDeviceToHost, ctx->stream));
+        ZFV_HIP(hipStreamSynchronize(ctx->stream));
+        ctx->n_faces = (int64_t)(fc[0] / 2 + fc[1]);
+        ctx->faces_pending = false;
+    }
     *n = ctx->have_mesh ? ctx->n_faces : 0;
     return ZFV_OK;
 }
@@ -666,15 +821,34 @@ int zfv_smf_configure(zfv_ctx* ctx, const zfv_smf_params* p) {
     ctx->prm.acc = std::min(std::max(1, p->acc), 2);               // set_accuracy
     ctx->prm.cfl = std::max(0.0, std::min(p->cfl, 1.0));            // set_CFL
     if (std::isnan(ctx->prm.max_dt)) ctx->prm.max_dt = std::numeric_limits<double>::max();
+    r = put_max_dt(ctx);
+    if (r) return r;
     ctx->configured = true;
     ctx->dt_valid = false;
+    ctx->graphs_clear();
     return ZFV_OK;
 }
 
-int zfv_smf_set_max_dt(zfv_ctx* ctx, double max_dt) {
-    if (!ctx) return fail(ZFV_ERR_ARG, "null context");
-    ctx->prm.max_dt = max_dt;
+}  // extern "C"
+
+namespace {
+// The value goes to the device in stream order (a one-word fill), so steps
+// enqueued before keep the old bound and captured graphs read the new one.
+int put_max_dt(zfv_ctx* ctx) {
+    unsigned long long bits;
+    std::memcpy(&bits, &ctx->prm.max_dt, 8);
+    ZFV_HIP(launch_fill_u64(reinterpret_cast<unsigned long long*>(ctx->dmax_dt.p), 1, bits, ctx->stream));
     return ZFV_OK;
+}
+}  // namespace
+
+extern "C" {
+
+int zfv_smf_set_max_dt(zfv_ctx* ctx, double max_dt) {
+    int r = check_ctx(ctx);
+    if (r) return r;
+    ctx->prm.max_dt = max_dt;
+    return put_max_dt(ctx);
 }
 
 static int state_copy(zfv_ctx* ctx, const double* const* src, bool to_device, hipMemcpyKind kind) {
@@ -762,37 +936,102 @@ int zfv_grad_download(zfv_ctx* ctx, double* dx, double* dy) {
 }
 
 // SmFluid::update x n_steps (sm_fluid.cpp:81-102).
+// One SmFluid::update (sm_fluid.cpp:81-102) enqueued on the context stream.
+static int step_once(zfv_ctx* ctx) {
+    const DevMesh M = ctx->dmesh();
+    const Eos eos = ctx->eos();
+    const zfv_smf_params& P = ctx->prm;
+    StepCtl ctl = make_ctl(ctx);
+    const int nxt = (ctx->cur + 1) % 2;   // layers 0/1 alternate as init/next, 2 is half
+    DevState in = ctx->layer(ctx->cur), out = ctx->layer(nxt);
+    if (P.acc == 1) {
+        ZFV_HIP(timed(ctx, "flux1", [&] {
+            return launch_flux1(P.flux, ctx->layout[0], ctx->block[0], M, in, out, eos, ctl, ctx->stream);
+        }));
+    } else {
+        DevGrad G{ctx->gdx.p, ctx->gdy.p};
+        DevState half = ctx->layer(2);
+        int* w_in = ctx->wait[ctx->wcur].p;
+        int* w_out = ctx->wait[1 - ctx->wcur].p;
+        ZFV_HIP(timed(ctx, "grad_pred", [&] {
+            return launch_grad_pred(P.limiter, true, ctx->layout[1], ctx->block[1], M, in, G, half, w_in, eos, ctl,
+                                    ctx->stream);
+        }));
+        ZFV_HIP(timed(ctx, "corr", [&] {
+            return launch_corr(P.flux, ctx->layout[2], ctx->block[2], M, in, half, G, w_in, w_out, out, eos, ctl,
+                               ctx->stream);
+        }));
+        ctx->wcur = 1 - ctx->wcur;
+    }
+    ctx->cur = nxt;
+    ctx->step_no += 1;
+    return ZFV_OK;
+}
+
+// Steps per captured graph: the dt slots rotate with period 3 and the
+// layers and wait buffers swap with period 2.
+constexpr int kGraphSteps = 6;
+
+static int graph_for_phase(zfv_ctx* ctx, hipGraphExec_t* out) {
+    const int key = (int)(ctx->step_no % 3) | (ctx->cur << 2) | (ctx->wcur << 3) | (ctx->prm.acc << 4);
+    for (auto& g : ctx->graphs)
+        if (g.key == key) {
+            *out = g.exec;
+            return ZFV_OK;
+        }
+    const int cur = ctx->cur, wcur = ctx->wcur;
+    const long long step_no = ctx->step_no;
+    ZFV_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+    int r = ZFV_OK;
+    for (int k = 0; k < kGraphSteps && r == ZFV_OK; ++k) r = step_once(ctx);
+    hipGraph_t graph = nullptr;
+    const hipError_t ec = hipStreamEndCapture(ctx->stream, &graph);
+    ctx->cur = cur;
+    ctx->wcur = wcur;
+    ctx->step_no = step_no;
+    if (r) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return r;
+    }
+    if (ec != hipSuccess) return fail(ZFV_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(ec));
+    hipGraphExec_t exec = nullptr;
+    const hipError_t ei = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (ei != hipSuccess) return fail(ZFV_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(ei));
+    ctx->graphs.push_back({key, exec});
+    *out = exec;
+    return ZFV_OK;
+}
+
 int zfv_smf_step(zfv_ctx* ctx, int n_steps) {
     int r = need_ready(ctx, true, true);
     if (r) return r;
     if (n_steps < 0) return fail(ZFV_ERR_ARG, "negative step count");
     r = ensure_dt(ctx);
     if (r) return r;
-    const DevMesh M = ctx->dmesh();
-    const Eos eos = ctx->eos();
-    const zfv_smf_params& P = ctx->prm;
-    for (int it = 0; it < n_steps; ++it) {
-        StepCtl ctl = make_ctl(ctx);
-        const int nxt = (ctx->cur + 1) % 2;   // layers 0/1 alternate as init/next, 2 is half
-        DevState in = ctx->layer(ctx->cur), out = ctx->layer(nxt);
-        if (P.acc == 1) {
-            ZFV_HIP(timed(ctx, "flux1", [&] { return launch_flux1(P.flux, M, in, out, eos, ctl, ctx->stream); }));
-        } else {
-            DevGrad G{ctx->gdx.p, ctx->gdy.p};
-            DevState half = ctx->layer(2);
-            int* w_in = ctx->wait[ctx->wcur].p;
-            int* w_out = ctx->wait[1 - ctx->wcur].p;
-            ZFV_HIP(timed(ctx, "grad_pred", [&] {
-                return launch_grad_pred(P.limiter, true, M, in, G, half, w_in, eos, ctl, ctx->stream);
-            }));
-            ZFV_HIP(timed(ctx, "corr", [&] {
-                return launch_corr(P.flux, M, in, half, G, w_in, w_out, out, eos, ctl, ctx->stream);
-            }));
-            ctx->wcur = 1 - ctx->wcur;
+    int it = 0;
+    if (ctx->use_graphs && ctx->stream && !ctx->profile) {
+        while (n_steps - it >= kGraphSteps) {
+            hipGraphExec_t exec = nullptr;
+            r = graph_for_phase(ctx, &exec);
+            if (r) return r;
+            ZFV_HIP(hipGraphLaunch(exec, ctx->stream));
+            ctx->step_no += kGraphSteps;   // cur and wcur return to their values after 6 swaps
+            it += kGraphSteps;
         }
-        ctx->cur = nxt;
-        ctx->step_no += 1;
     }
+    for (; it < n_steps; ++it) {
+        r = step_once(ctx);
+        if (r) return r;
+    }
+    return ZFV_OK;
+}
+
+int zfv_graph_enable(zfv_ctx* ctx, int on) {
+    int r = check_ctx(ctx);
+    if (r) return r;
+    ctx->use_graphs = on != 0;
+    if (!ctx->use_graphs) ctx->graphs_clear();
     return ZFV_OK;
 }
 
@@ -812,7 +1051,8 @@ int zfv_smf_compute_grad(zfv_ctx* ctx) {
     if (r) return r;
     StepCtl ctl = make_ctl(ctx);
     DevGrad G{ctx->gdx.p, ctx->gdy.p};
-    ZFV_HIP(launch_grad_pred(ctx->prm.limiter, false, ctx->dmesh(), ctx->layer(ctx->cur), G, ctx->layer(2),
+    ZFV_HIP(launch_grad_pred(ctx->prm.limiter, false, ctx->layout[1], ctx->block[1], ctx->dmesh(), ctx->layer(ctx->cur), G,
+                             ctx->layer(2),
                              ctx->wait[ctx->wcur].p, ctx->eos(), ctl, ctx->stream));
     ZFV_HIP(hipStreamSynchronize(ctx->stream));
     return ZFV_OK;
@@ -879,7 +1119,8 @@ int zfv_set_flags(zfv_ctx* ctx, int criterion, double threshold) {
     StepCtl ctl = make_ctl(ctx);
     DevGrad G{ctx->gdx.p, ctx->gdy.p};
     if (ctx->graw.n < 2 * (size_t)ctx->cap) ZFV_HIP(ctx->graw.alloc(2 * (size_t)ctx->cap));
-    ZFV_HIP(launch_grad_pred(ctx->prm.limiter, false, ctx->dmesh(), ctx->layer(ctx->cur), G, ctx->layer(2),
+    ZFV_HIP(launch_grad_pred(ctx->prm.limiter, false, ctx->layout[1], ctx->block[1], ctx->dmesh(), ctx->layer(ctx->cur), G,
+                             ctx->layer(2),
                              ctx->wait[ctx->wcur].p, ctx->eos(), ctl, ctx->stream, ctx->graw.p));
     const DevGrad R{ctx->graw.p, ctx->graw.p + ctx->cap};   // unlimited LSM gradient of rho
     ZFV_HIP(amr_criterion(ctx->amesh(), ctx->layer(ctx->cur), R, criterion, threshold, ctx->stream));
@@ -897,94 +1138,80 @@ int zfv_balance_flags(zfv_ctx* ctx, int max_level) {
     return ZFV_OK;
 }
 
-static int scan(zfv_ctx* ctx, const int* in, int* out, int count, bool inclusive) {
-    if (count <= 0) return ZFV_OK;
+static int scan3(zfv_ctx* ctx, const Marks3* in, Marks3* out, int count) {
     size_t bytes = 0;
-    ZFV_HIP(amr_scan(in, out, count, inclusive, nullptr, bytes, ctx->stream));
+    ZFV_HIP(amr_scan3(in, out, count, nullptr, bytes, ctx->stream));
     if (ctx->s_tmp.n < bytes) ZFV_HIP(ctx->s_tmp.alloc(bytes));
-    ZFV_HIP(amr_scan(in, out, count, inclusive, ctx->s_tmp.p, bytes, ctx->stream));
+    ZFV_HIP(amr_scan3(in, out, count, ctx->s_tmp.p, bytes, ctx->stream));
     return ZFV_OK;
 }
 
-// EuMesh::refine (eu_mesh.cpp:290-359): balance_flags then amr::apply
-// (mesh/amr/apply.h:43-94) with the SmFluid distributor (sm_fluid.cpp:372-470).
+// EuMesh::refine (eu_mesh.cpp:290-359) = balance_flags + amr::apply
+// (apply.h:43-94).  One host synchronisation: the scan tail that sizes the
+// new mesh.  The derive pass also seeds the next step's dt, so no further
+// launch is needed before stepping.  Consistency failures are reported by
+// the next synchronising call through the error word.
 int zfv_refine(zfv_ctx* ctx, int max_level, int distributor) {
     int r = need_ready(ctx, true, true);
     if (r) return r;
     if (max_level < 0 || max_level > 15) return fail(ZFV_ERR_ARG, "max_level out of [0, 15]");
     if (distributor != 0 && distributor != 1) return fail(ZFV_ERR_ARG, "distributor must be 0 (const) or 1 (slope)");
+    if (ctx->nseg > 1) return fail(ZFV_ERR_UNSUPPORTED, "adaptation of a batch of problems");
     hipStream_t s = ctx->stream;
     if (max_level == 0) return ZFV_OK;   // EuMesh::adaptive() is false
+    const int n = ctx->hm.n;
     ZFV_HIP(amr_balance(ctx->amesh(), max_level, s));
     ctx->host_stale = true;
-    ZFV_HIP(ctx->s_counts.alloc(8));
-    ZFV_HIP(hipMemsetAsync(ctx->s_counts.p, 0, 4 * sizeof(int), s));
-    ZFV_HIP(amr_count(ctx->amesh(), ctx->s_counts.p, s));
-    int cnt[4];
-    ZFV_HIP(hipMemcpyAsync(cnt, ctx->s_counts.p, 4 * sizeof(int), hipMemcpyDeviceToHost, s));
+    for (auto* b : {&ctx->s_marks, &ctx->s_rank})
+        if (b->n < (size_t)n) ZFV_HIP(b->alloc(n));
+    ZFV_HIP(amr_mark(ctx->amesh(), ctx->s_marks.p, s));
+    r = scan3(ctx, ctx->s_marks.p, ctx->s_rank.p, n);
+    if (r) return r;
+    Marks3 tail[2];
+    ZFV_HIP(hipMemcpyAsync(&tail[0], ctx->s_rank.p + (n - 1), sizeof(Marks3), hipMemcpyDeviceToHost, s));
+    ZFV_HIP(hipMemcpyAsync(&tail[1], ctx->s_marks.p + (n - 1), sizeof(Marks3), hipMemcpyDeviceToHost, s));
     ZFV_HIP(hipStreamSynchronize(s));
-    const int n = ctx->hm.n;
-    const int n_retain = cnt[0], n_refine = cnt[1], n_coarse = cnt[2];
-    const int n_parents = n_coarse / 4, n_children = 4 * n_refine;
+    const int n_refine = tail[0].ref + tail[1].ref;
+    const int n_parents = tail[0].head + tail[1].head;
+    const int nz_total = tail[0].nz + tail[1].nz;   // refined + coarsened cells
+    const int n_children = 4 * n_refine;
     const long long n_large_ll = (long long)n + n_parents + n_children;
     if (n_large_ll > std::numeric_limits<int>::max() / 9) return fail(ZFV_ERR_ARG, "adapted mesh too large");
     const int n_large = (int)n_large_ll;
-    const int n_short = n_retain + n_children + n_parents;
+    // Statistics (statistics.h:77-80): n_short = retained + children + parents
+    const int n_short = (n - nz_total) + n_children + n_parents;
     if (n_large <= n) return ZFV_OK;
+    if (nz_total != n_refine + 4 * n_parents) return fail(ZFV_ERR_STATE, "unbalanced coarsening flags");
     if (n_large > ctx->cap) {
         r = grow_capacity(ctx, std::max(n_large, ctx->cap + ctx->cap / 2));
         if (r) return r;
     }
-    const int na = n_large - n_short;
-    for (auto* b : {&ctx->s_ref_mark, &ctx->s_crs_mark, &ctx->s_hole_mark, &ctx->s_ref_rank, &ctx->s_crs_rank,
-                    &ctx->s_hole_rank, &ctx->s_hole_at})
-        if (b->n < (size_t)n) ZFV_HIP(b->alloc(n));
-    for (auto* b : {&ctx->s_act_mark, &ctx->s_act_incl, &ctx->s_swap_src, &ctx->s_swap_dst})
-        if (b->n < (size_t)std::max(na, 1)) ZFV_HIP(b->alloc(std::max(na, 1)));
+    for (auto* b : {&ctx->s_hole_at, &ctx->s_ref_list, &ctx->s_crs_list, &ctx->s_swap_src, &ctx->s_swap_dst})
+        if (b->n < (size_t)std::max(n, 1)) ZFV_HIP(b->alloc(std::max(n, 1)));
+    if (ctx->s_faces.n < 2) ZFV_HIP(ctx->s_faces.alloc(2));
 
     AmrMesh A = ctx->amesh();   // A.n = n (cells before adaptation)
-    ZFV_HIP(amr_mark(A, n_short, ctx->s_ref_mark.p, ctx->s_crs_mark.p, ctx->s_hole_mark.p, s));
-    r = scan(ctx, ctx->s_ref_mark.p, ctx->s_ref_rank.p, n, false); if (r) return r;
-    r = scan(ctx, ctx->s_crs_mark.p, ctx->s_crs_rank.p, n, false); if (r) return r;
-    r = scan(ctx, ctx->s_hole_mark.p, ctx->s_hole_rank.p, n, false); if (r) return r;
-    int tail[2];
-    ZFV_HIP(hipMemcpyAsync(&tail[0], ctx->s_hole_rank.p + (n - 1), sizeof(int), hipMemcpyDeviceToHost, s));
-    ZFV_HIP(hipMemcpyAsync(&tail[1], ctx->s_hole_mark.p + (n - 1), sizeof(int), hipMemcpyDeviceToHost, s));
-    ZFV_HIP(hipStreamSynchronize(s));
-    const int n_swap = tail[0] + tail[1];
-    if (n_swap > na) return fail(ZFV_ERR_STATE, "AMR apply: inconsistent swap lists");
-    ZFV_HIP(amr_positions(A, n_large, n_parents, ctx->s_ref_rank.p, ctx->s_crs_rank.p, ctx->s_hole_mark.p,
-                          ctx->s_hole_rank.p, ctx->s_hole_at.p, s));
-    if (na > 0) {
-        ZFV_HIP(amr_actual_mark(A, n_short, n_large, ctx->s_act_mark.p, s));
-        r = scan(ctx, ctx->s_act_mark.p, ctx->s_act_incl.p, na, true); if (r) return r;
-        int total = 0;
-        ZFV_HIP(hipMemcpyAsync(&total, ctx->s_act_incl.p + (na - 1), sizeof(int), hipMemcpyDeviceToHost, s));
-        ZFV_HIP(hipStreamSynchronize(s));
-        if (total != n_swap) return fail(ZFV_ERR_STATE, "AMR apply: hole / survivor counts differ");
-        ZFV_HIP(amr_swapmap(A, n_short, n_large, n_swap, ctx->s_act_mark.p, ctx->s_act_incl.p, ctx->s_hole_at.p,
-                            ctx->s_swap_src.p, ctx->s_swap_dst.p, s));
-    }
+    const int rd = (int)(ctx->step_no % 3), wr = (int)((ctx->step_no + 1) % 3);
+    ZFV_HIP(amr_positions(A, n_large, n_short, n_parents, ctx->s_rank.p, ctx->s_hole_at.p, ctx->s_ref_list.p,
+                          ctx->s_crs_list.p, ctx->s_faces.p, ctx->dtmin.p + rd, ctx->dtmin.p + wr, s));
+    ZFV_HIP(amr_swapmap(A, n_short, n_large, nz_total, ctx->s_rank.p, ctx->s_hole_at.p, ctx->s_swap_src.p,
+                        ctx->s_swap_dst.p, ctx->err.p, s));
     DevGrad G{ctx->gdx.p, ctx->gdy.p};
-    ZFV_HIP(amr_geometry(A, ctx->layer(ctx->cur), G, ctx->wait[ctx->wcur].p, ctx->eos(), distributor == 1, s));
-    ZFV_HIP(amr_move(A, ctx->layer(ctx->cur), ctx->wait[ctx->wcur].p, n_swap, ctx->s_swap_src.p,
-                     ctx->s_swap_dst.p, s));
+    ZFV_HIP(amr_geometry(A, ctx->layer(ctx->cur), G, ctx->wait[ctx->wcur].p, ctx->eos(), distributor == 1,
+                         ctx->s_ref_list.p, n_refine, ctx->s_crs_list.p, n_parents, s));
+    // moves: one per hole (the kernel reads the hole count from the ranks);
+    // there are at most min(changed cells, positions above n_short) of them
+    ZFV_HIP(amr_move(A, ctx->layer(ctx->cur), ctx->wait[ctx->wcur].p, n_short, nz_total, ctx->s_rank.p,
+                     std::min(nz_total, n_large - n_short), ctx->s_swap_src.p, ctx->s_swap_dst.p, s));
     ctx->hm.n = n_short;
-    ZFV_HIP(hipMemsetAsync(ctx->s_counts.p + 4, 0, 3 * sizeof(int), s));
-    ZFV_HIP(amr_derive(ctx->amesh(), ctx->s_counts.p + 4, s));
-    int dc[3] = {0, 0, 0};
-    ZFV_HIP(hipMemcpyAsync(dc, ctx->s_counts.p + 4, 3 * sizeof(int), hipMemcpyDeviceToHost, s));
-    ZFV_HIP(hipStreamSynchronize(s));
-    ctx->dt_valid = false;
-    ctx->n_faces = (int64_t)dc[1] / 2 + dc[2];
-    if (dc[0]) {
-        ctx->have_state = false;
-        return fail(ZFV_ERR_STATE, "AMR apply produced an inconsistent adjacency; state invalidated");
-    }
+    ZFV_HIP(amr_derive(ctx->amesh(), ctx->layer(ctx->cur), ctx->eos(), ctx->s_faces.p, ctx->dtmin.p + rd,
+                       ctx->err.p, s));
+    ctx->faces_pending = true;
+    ctx->dt_valid = true;
+    ctx->graphs_clear();   // the cell count changed
     return ZFV_OK;
 }
 
-// Limiter(const std::string&) name parsing (math/cfd/limiter.cxx:163-252).
 int zfv_limiter_from_name(const char* orig) {
     if (!orig) return -1;
     std::string name(orig);
@@ -1013,6 +1240,100 @@ int zfv_limiter_from_name(const char* orig) {
     if (has("superbee")) return ZFV_LIM_SUPERBEE;
     g_last_error = std::string("Unknown limiter '") + orig + "'";
     return -1;
+}
+
+static bool valid_block(int b) { return b == 128 || b == 256 || b == 512; }
+
+int zfv_set_launch(zfv_ctx* ctx, int stage, int layout, int block) {
+    int r = check_ctx(ctx);
+    if (r) return r;
+    if (stage < 0 || stage > 2) return fail(ZFV_ERR_ARG, "stage must be 0 (first order), 1 (gradient), 2 (corrector)");
+    if (layout != kLayoutCell && layout != kLayoutQuad) return fail(ZFV_ERR_ARG, "layout must be 0 (cell) or 1 (quad)");
+    if (block != 0 && !valid_block(block)) return fail(ZFV_ERR_ARG, "block must be 0 (keep), 128, 256 or 512");
+    ctx->layout[stage] = layout;
+    if (block) ctx->block[stage] = block;
+    ctx->graphs_clear();
+    return ZFV_OK;
+}
+
+int zfv_get_launch(zfv_ctx* ctx, int stage, int* layout, int* block) {
+    int r = check_ctx(ctx);
+    if (r) return r;
+    if (stage < 0 || stage > 2) return fail(ZFV_ERR_ARG, "bad stage");
+    if (layout) *layout = ctx->layout[stage];
+    if (block) *block = ctx->block[stage];
+    return ZFV_OK;
+}
+
+// Launch autotuner: times every (layout, workgroup size) of the configured
+// scheme's step kernels on the current state (reps launches each) and keeps
+// the fastest per stage.  The timed launches recompute the same outputs as
+// the next step's first launches (same inputs, same dt slot values), so the
+// state is not advanced.
+int zfv_autotune(zfv_ctx* ctx, int reps, double* ms_out) {
+    int r = need_ready(ctx, true, true);
+    if (r) return r;
+    if (reps <= 0) return fail(ZFV_ERR_ARG, "reps must be positive");
+    r = ensure_dt(ctx);
+    if (r) return r;
+    hipStream_t s = ctx->stream;
+    const DevMesh M = ctx->dmesh();
+    const Eos eos = ctx->eos();
+    const zfv_smf_params& P = ctx->prm;
+    const StepCtl ctl = make_ctl(ctx);
+    const int nxt = (ctx->cur + 1) % 2;
+    const DevState in = ctx->layer(ctx->cur), out = ctx->layer(nxt), half = ctx->layer(2);
+    const DevGrad G{ctx->gdx.p, ctx->gdy.p};
+    int* w_in = ctx->wait[ctx->wcur].p;
+    int* w_out = ctx->wait[1 - ctx->wcur].p;
+    hipEvent_t e0 = pooled_event(ctx), e1 = pooled_event(ctx);
+    auto time_it = [&](auto&& launch, double& ms) -> int {
+        ZFV_HIP(launch());   // warm
+        ZFV_HIP(hipEventRecord(e0, s));
+        for (int i = 0; i < reps; ++i) ZFV_HIP(launch());
+        ZFV_HIP(hipEventRecord(e1, s));
+        ZFV_HIP(hipEventSynchronize(e1));
+        float f = 0.0f;
+        ZFV_HIP(hipEventElapsedTime(&f, e0, e1));
+        ms = f / reps;
+        return ZFV_OK;
+    };
+    const int blocks[3] = {128, 256, 512};
+    double t[3][2][3] = {};   // [stage][layout][block]
+    const int stages[2] = {P.acc == 1 ? 0 : 1, P.acc == 1 ? -1 : 2};
+    for (int st : stages) {
+        if (st < 0) continue;
+        for (int L = 0; L < 2; ++L)
+            for (int bi = 0; bi < 3; ++bi) {
+                const int B = blocks[bi];
+                if (st == 0)
+                    r = time_it([&] { return launch_flux1(P.flux, L, B, M, in, out, eos, ctl, s); }, t[st][L][bi]);
+                else if (st == 1)
+                    r = time_it([&] { return launch_grad_pred(P.limiter, true, L, B, M, in, G, half, w_in, eos, ctl, s); },
+                                t[st][L][bi]);
+                else
+                    r = time_it([&] { return launch_corr(P.flux, L, B, M, in, half, G, w_in, w_out, out, eos, ctl, s); },
+                                t[st][L][bi]);
+                if (r) return r;
+            }
+        int bl = 0, bb = 0;
+        for (int L = 0; L < 2; ++L)
+            for (int bi = 0; bi < 3; ++bi)
+                if (t[st][L][bi] < t[st][bl][bb]) {
+                    bl = L;
+                    bb = bi;
+                }
+        ctx->layout[st] = bl;
+        ctx->block[st] = blocks[bb];
+    }
+    ctx->event_pool.push_back(e0);
+    ctx->event_pool.push_back(e1);
+    if (ms_out)
+        for (int st = 0; st < 3; ++st)
+            for (int L = 0; L < 2; ++L)
+                for (int bi = 0; bi < 3; ++bi) ms_out[(st * 2 + L) * 3 + bi] = t[st][L][bi];
+    ctx->graphs_clear();
+    return read_err(ctx);
 }
 
 int zfv_profile_enable(zfv_ctx* ctx, int on) {

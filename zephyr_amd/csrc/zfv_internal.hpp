@@ -30,7 +30,17 @@ struct DevMesh {
     const double* __restrict__ w;       // [8][stride] LSM weight area / |drn|^3
     const double* __restrict__ nx;      // [8][stride] outward normal
     const double* __restrict__ ny;
+    // Batches of independent problems (null for one problem): problem of each
+    // 64-cell chunk, first cell and end of each problem's cells.  Problems
+    // start on 512-cell boundaries.
+    const int* __restrict__ chunk_pid;
+    const int* __restrict__ seg_begin;
+    const int* __restrict__ seg_end;
 };
+
+constexpr int kChunkShift = 6;     // 64-cell chunks
+constexpr int kSegAlign = 512;     // problem alignment in a batch (cells)
+constexpr int kMaxBlock = 512;     // largest workgroup of the step kernels
 
 constexpr int kGeneralNormal = 8;
 
@@ -48,11 +58,13 @@ struct DevGrad {
 };
 
 // Per-launch step control: dt slots hold min_c h/(|v|+c) as the bit pattern
-// of a positive double (ordered like uint64), reduced with atomicMin.
+// of a positive double (ordered like uint64), reduced with atomicMin; three
+// rotating slots per problem (problem p at 3p), dt_out one per problem.
 struct StepCtl {
-    unsigned long long* dtmin;   // 3 rotating slots
+    unsigned long long* dtmin;   // [3 * problems] rotating slots
     int rd, wr, clr;             // slot read for this step's dt, written for the next, cleared
-    double cfl, max_dt;
+    double cfl;
+    const double* max_dt;        // device value, so captured step graphs follow set_max_dt
     double* dt_out;              // dt of this step
     int* err;                    // sticky numeric-error flag
 };
@@ -76,30 +88,39 @@ struct AmrMesh {
 hipError_t amr_criterion(const AmrMesh& A, const DevState& z, const DevGrad& G, int criterion, double thr,
                          hipStream_t s);
 hipError_t amr_balance(const AmrMesh& A, int max_level, hipStream_t s);
-hipError_t amr_count(const AmrMesh& A, int* counts, hipStream_t s);
-hipError_t amr_scan(const int* in, int* out, int count, bool inclusive, void* tmp, size_t& tmp_bytes,
-                    hipStream_t s);
-hipError_t amr_mark(const AmrMesh& A, int n_short, int* ref_mark, int* crs_mark, int* hole_mark, hipStream_t s);
-hipError_t amr_positions(const AmrMesh& A, int n_large, int n_parents, const int* ref_rank, const int* crs_rank,
-                         const int* hole_mark, const int* hole_rank, int* hole_at, hipStream_t s);
-hipError_t amr_actual_mark(const AmrMesh& A, int n_short, int n_large, int* act_mark, hipStream_t s);
-hipError_t amr_swapmap(const AmrMesh& A, int n_short, int n_large, int n_swap, const int* act_mark,
-                       const int* act_incl, const int* hole_at, int* swap_src, int* swap_dst, hipStream_t s);
+// Per-cell marks scanned by the adaptation: refine, coarsening-family head,
+// flag != 0.
+struct Marks3 {
+    int ref, head, nz;
+};
+hipError_t amr_scan3(const Marks3* in, Marks3* out, int count, void* tmp, size_t& tmp_bytes, hipStream_t s);
+hipError_t amr_mark(const AmrMesh& A, Marks3* marks, hipStream_t s);
+hipError_t amr_positions(const AmrMesh& A, int n_large, int n_short, int n_parents, const Marks3* rank,
+                         int* hole_at, int* ref_list, int* crs_list, unsigned long long* faces,
+                         unsigned long long* dt_rd, unsigned long long* dt_wr, hipStream_t s);
+hipError_t amr_swapmap(const AmrMesh& A, int n_short, int n_large, int nz_total, const Marks3* rank,
+                       const int* hole_at, int* swap_src, int* swap_dst, int* err, hipStream_t s);
 hipError_t amr_geometry(const AmrMesh& A, const DevState& z, const DevGrad& G, int* wait, const Eos& eos,
-                        bool slope, hipStream_t s);
-hipError_t amr_move(const AmrMesh& A, const DevState& z, int* wait, int n_swap, const int* swap_src,
-                    const int* swap_dst, hipStream_t s);
-hipError_t amr_derive(const AmrMesh& A, int* counts, hipStream_t s);
+                        bool slope, const int* ref_list, int n_refine, const int* crs_list, int n_parents,
+                        hipStream_t s);
+hipError_t amr_move(const AmrMesh& A, const DevState& z, int* wait, int n_short, int nz_total, const Marks3* rank,
+                    int bound, const int* swap_src, const int* swap_dst, hipStream_t s);
+hipError_t amr_derive(const AmrMesh& A, const DevState& z, const Eos& eos, unsigned long long* counts,
+                      unsigned long long* dt_slot, int* err, hipStream_t s);
 
 // Launchers (zfv_kernels.hip).
 hipError_t launch_fill_u64(unsigned long long* p, int count, unsigned long long v, hipStream_t s);
 hipError_t launch_dt(const DevMesh& M, const DevState& z, const Eos& eos, unsigned long long* slot, hipStream_t s);
-hipError_t launch_flux1(int flux, const DevMesh& M, const DevState& in, const DevState& out,
+
+// Step-kernel layouts: one thread per cell, or a lane quad per cell.
+constexpr int kLayoutCell = 0, kLayoutQuad = 1;
+
+hipError_t launch_flux1(int flux, int layout, int block, const DevMesh& M, const DevState& in, const DevState& out,
                         const Eos& eos, const StepCtl& ctl, hipStream_t s);
-hipError_t launch_grad_pred(int limiter, bool predictor, const DevMesh& M, const DevState& in, const DevGrad& g,
-                            const DevState& half, const int* wait, const Eos& eos, const StepCtl& ctl,
-                            hipStream_t s, double* raw = nullptr);
-hipError_t launch_corr(int flux, const DevMesh& M, const DevState& init, const DevState& half,
+hipError_t launch_grad_pred(int limiter, bool predictor, int layout, int block, const DevMesh& M, const DevState& in,
+                            const DevGrad& g, const DevState& half, const int* wait, const Eos& eos,
+                            const StepCtl& ctl, hipStream_t s, double* raw = nullptr);
+hipError_t launch_corr(int flux, int layout, int block, const DevMesh& M, const DevState& init, const DevState& half,
                        const DevGrad& g, const int* wait_in, int* wait_out, const DevState& next,
                        const Eos& eos, const StepCtl& ctl, hipStream_t s);
 hipError_t launch_totals(const DevMesh& M, const DevState& z, double* partial, int nblocks, hipStream_t s);

@@ -16,6 +16,27 @@
 // are dropped, as `temp < res` drops them in threads::min (utils/threads.h:292-352).
 #include "zfv_internal.hpp"
 
+// Minimum resident waves per SIMD requested from the register allocator for
+// the step kernels (caps VGPRs at 512 / waves).
+#ifndef ZFV_WAVES_FLUX1
+#define ZFV_WAVES_FLUX1 4
+#endif
+#ifndef ZFV_WAVES_GRAD
+#define ZFV_WAVES_GRAD 4
+#endif
+#ifndef ZFV_WAVES_CORR
+#define ZFV_WAVES_CORR 4
+#endif
+#ifndef ZFV_WAVES_FLUX1Q
+#define ZFV_WAVES_FLUX1Q 6
+#endif
+#ifndef ZFV_WAVES_GRADQ
+#define ZFV_WAVES_GRADQ 2
+#endif
+#ifndef ZFV_WAVES_CORRQ
+#define ZFV_WAVES_CORRQ 4
+#endif
+
 namespace zfv {
 
 namespace {
@@ -27,8 +48,30 @@ __device__ __forceinline__ int xcd_block(int b, int G) {
     return x * q + (x < r ? x : r) + (b >> 3);
 }
 
-__device__ __forceinline__ int cell_of_thread() {
-    return xcd_block((int)blockIdx.x, (int)gridDim.x) * kBlock + (int)threadIdx.x;
+// The cells of one workgroup: tile = XCD-aware block index, cells_per_block
+// cells from c0.  In a batch of problems (chunk_pid set) every problem
+// starts on a 512-cell boundary, so a workgroup lies in one problem `pid`;
+// its cells end at `end`, and `first` marks the problem's first workgroup,
+// which does the per-problem step prologue.
+struct Tile {
+    int c0, end, pid;
+    bool first;
+};
+
+__device__ __forceinline__ Tile tile_of(const DevMesh& M, int cells_per_block) {
+    const int tile = xcd_block((int)blockIdx.x, (int)gridDim.x);
+    Tile t;
+    t.c0 = tile * cells_per_block;
+    if (M.chunk_pid) {
+        t.pid = M.chunk_pid[t.c0 >> kChunkShift];
+        t.end = M.seg_end[t.pid];
+        t.first = t.c0 == M.seg_begin[t.pid];
+    } else {
+        t.pid = 0;
+        t.end = M.n;
+        t.first = tile == 0;
+    }
+    return t;
 }
 
 __device__ __forceinline__ Z load_z(const DevState& s, int c) {
@@ -49,15 +92,16 @@ __device__ __forceinline__ double cell_dt(const Z& z, double h, const Eos& eos) 
     return h / (sqrt(vsq(z)) + c);
 }
 
-__device__ __forceinline__ double step_dt(const StepCtl& ctl) {
-    double dmin = __longlong_as_double((long long)ctl.dtmin[ctl.rd]);
-    return smin(ctl.cfl * dmin, ctl.max_dt);
+// dt of problem pid: three rotating slots per problem.
+__device__ __forceinline__ double step_dt(const StepCtl& ctl, int pid) {
+    double dmin = __longlong_as_double((long long)ctl.dtmin[3 * pid + ctl.rd]);
+    return smin(ctl.cfl * dmin, *ctl.max_dt);
 }
 
-__device__ __forceinline__ void step_prologue(const StepCtl& ctl, double dt) {
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        *ctl.dt_out = dt;
-        ctl.dtmin[ctl.clr] = kInfBits;
+__device__ __forceinline__ void step_prologue(const StepCtl& ctl, const Tile& t, double dt) {
+    if (t.first && threadIdx.x == 0) {
+        ctl.dt_out[t.pid] = dt;
+        ctl.dtmin[3 * t.pid + ctl.clr] = kInfBits;
     }
 }
 
@@ -68,14 +112,14 @@ __device__ __forceinline__ void block_min_atomic(unsigned long long* slot, doubl
         double o = __shfl_xor(v, off, 64);
         v = (o < v) ? o : v;
     }
-    __shared__ double red[kBlock / 64];
+    __shared__ double red[kMaxBlock / 64];
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     if (lane == 0) red[wid] = v;
     __syncthreads();
     if (threadIdx.x == 0) {
         double m = red[0];
-#pragma unroll
-        for (int i = 1; i < kBlock / 64; ++i) m = (red[i] < m) ? red[i] : m;
+        const int nw = (int)(blockDim.x >> 6);
+        for (int i = 1; i < nw; ++i) m = (red[i] < m) ? red[i] : m;
         if (m < INFINITY) atomicMin(slot, (unsigned long long)__double_as_longlong(m));
     }
 }
@@ -92,26 +136,19 @@ __device__ __forceinline__ Frame slot_frame(const DevMesh& M, int c, int s, int 
 }
 
 // Neighbour state through a slot and the LSM offset dr (gradient.h:245-252).
+// All loads are unconditional (adj of a boundary slot is the cell itself),
+// so a face costs one memory round trip; the slot type then selects.
 __device__ __forceinline__ void lsm_neib(const DevMesh& M, const DevState& in, int c, int s,
                                          int t, const Frame& f, const Z& zc, double ccx, double ccy,
                                          Z& zn, double& drx, double& dry) {
-    const int nb = M.adj[s * M.stride + c];
-    if (t == SC_INNER || t == SC_PERIODIC) {
-        zn.r = in.r[nb];
-        zn.u = in.u[nb];
-        zn.v = in.v[nb];
-        zn.p = in.p[nb];
-        zn.e = 0.0;
-    } else {
-        zn = frame_boundary(f, zc, t == SC_WALL);
-    }
-    double px, py;
-    if (t == SC_INNER) {
-        px = M.cx[nb];
-        py = M.cy[nb];
-    } else {
-        frame_symm(f, ccx, ccy, M.fcx[s * M.stride + c], M.fcy[s * M.stride + c], px, py);
-    }
+    const int so = s * M.stride + c;
+    const int nb = M.adj[so];
+    const Z zl{in.r[nb], in.u[nb], in.v[nb], in.p[nb], 0.0};
+    const double ncx = M.cx[nb], ncy = M.cy[nb];
+    const double fx = M.fcx[so], fy = M.fcy[so];
+    zn = (t == SC_INNER || t == SC_PERIODIC) ? zl : frame_boundary(f, zc, t == SC_WALL);
+    double px = ncx, py = ncy;
+    if (t != SC_INNER) frame_symm(f, ccx, ccy, fx, fy, px, py);
     drx = px - ccx;
     dry = py - ccy;
 }
@@ -148,23 +185,25 @@ __global__ void __launch_bounds__(256) k_fill_u64(unsigned long long* p, int cou
     if (i < count) p[i] = v;
 }
 
-__global__ void __launch_bounds__(kBlock) k_dt(DevMesh M, DevState z, Eos eos, unsigned long long* slot) {
-    const int c = cell_of_thread();
+__global__ void __launch_bounds__(kMaxBlock) k_dt(DevMesh M, DevState z, Eos eos, unsigned long long* slot) {
+    const Tile t = tile_of(M, (int)blockDim.x);
+    const int c = t.c0 + (int)threadIdx.x;
     double cand = INFINITY;
-    if (c < M.n) cand = cell_dt(load_z(z, c), M.h[c], eos);
-    block_min_atomic(slot, cand);
+    if (c < t.end) cand = cell_dt(load_z(z, c), M.h[c], eos);
+    block_min_atomic(slot + 3 * t.pid, cand);
 }
 
 // SmFluid::fluxes, first order (sm_fluid.cpp:127-174), plus the next step's
 // CFL candidate.
 template <int FLUX>
-__global__ void __launch_bounds__(kBlock) k_flux1(DevMesh M, DevState in, DevState out, Eos eos, StepCtl ctl) {
-    const int n = M.n;
-    const int c = cell_of_thread();
-    const double dt = step_dt(ctl);
-    step_prologue(ctl, dt);
+__global__ void __launch_bounds__(kMaxBlock) __attribute__((amdgpu_waves_per_eu(ZFV_WAVES_FLUX1)))
+k_flux1(DevMesh M, DevState in, DevState out, Eos eos, StepCtl ctl) {
+    const Tile tl = tile_of(M, (int)blockDim.x);
+    const int c = tl.c0 + (int)threadIdx.x;
+    const double dt = step_dt(ctl, tl.pid);
+    step_prologue(ctl, tl, dt);
     double cand = INFINITY;
-    if (c < n) {
+    if (c < tl.end) {
         const Z zc = load_z(in, c);
         const int code = M.code[c];
         double F0 = 0.0, F1 = 0.0, F2 = 0.0, F3 = 0.0;
@@ -197,7 +236,7 @@ __global__ void __launch_bounds__(kBlock) k_flux1(DevMesh M, DevState in, DevSta
         if (bad) *ctl.err = 1;
         cand = cell_dt(zn, M.h[c], eos);
     }
-    block_min_atomic(&ctl.dtmin[ctl.wr], cand);
+    block_min_atomic(&ctl.dtmin[3 * tl.pid + ctl.wr], cand);
 }
 
 // SmFluid::compute_grad (sm_fluid.cpp:116-125): gradient::LSM
@@ -212,16 +251,17 @@ __global__ void __launch_bounds__(kBlock) k_flux1(DevMesh M, DevState in, DevSta
 // Without the predictor, `raw` (if not null) receives the unlimited LSM
 // gradient of rho ([2][stride]) for the refinement criterion.
 template <int LIM, bool PRED>
-__global__ void __launch_bounds__(kBlock) k_grad_pred(DevMesh M, DevState in, DevGrad G, DevState half,
+__global__ void __launch_bounds__(kMaxBlock) __attribute__((amdgpu_waves_per_eu(ZFV_WAVES_GRAD)))
+k_grad_pred(DevMesh M, DevState in, DevGrad G, DevState half,
                                                       const int* __restrict__ wait, Eos eos, StepCtl ctl,
                                                       double* __restrict__ raw) {
-    const int n = M.n;
-    const int c = cell_of_thread();
-    const double dt = step_dt(ctl);
+    const Tile tl = tile_of(M, (int)blockDim.x);
+    const int c = tl.c0 + (int)threadIdx.x;
+    const double dt = step_dt(ctl, tl.pid);
     if (PRED) {
-        if (blockIdx.x == 0 && threadIdx.x == 0) ctl.dtmin[ctl.clr] = kInfBits;
+        if (tl.first && threadIdx.x == 0) ctl.dtmin[3 * tl.pid + ctl.clr] = kInfBits;
     }
-    if (c >= n) return;
+    if (c >= tl.end) return;
 
     const Z zc = load_z(in, c);
     const double ccx = M.cx[c], ccy = M.cy[c];
@@ -335,20 +375,68 @@ __global__ void __launch_bounds__(kBlock) k_grad_pred(DevMesh M, DevState in, De
     store_z(half, c, zh);
 }
 
+// One face of SmFluid::fluxes_stage2 (sm_fluid.cpp:255-366): flux * area of
+// slot s.  Every load of the face is issued in one batch before any choice
+// is made (on boundary slots adj is the cell itself, so the neighbour loads
+// stay in bounds); the isolation, boundary and bad-state choices are then
+// selects among computed states, which keeps one memory round trip per face.
+template <int FLUX>
+__device__ __forceinline__ Q corr_face(const DevMesh& M, const DevState& half, const DevGrad& G,
+                                       const int* __restrict__ wait_in, int c, int s, int nib, const Z& zch, int wc,
+                                       double ccx, double ccy, const double* gx, const double* gy, const Eos& eos,
+                                       bool& bad) {
+    const int t = nib & 7;
+    const int so = s * M.stride + c;
+    const int nb = M.adj[so];
+    const double fx = M.fcx[so], fy = M.fcy[so], a = M.area[so];
+    const Z hn = load_z(half, nb);
+    const int wnb = wait_in[nb];
+    const double ncx = M.cx[nb], ncy = M.cy[nb];
+    double ngx[4], ngy[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        ngx[k] = G.dx[k * M.stride + nb];
+        ngy[k] = G.dy[k * M.stride + nb];
+    }
+    const Frame fr = slot_frame(M, c, s, nib);
+    const bool bound = t >= SC_WALL;
+    const bool wall = t == SC_WALL;
+    const bool robust = wc > 0 || wnb > 0;
+
+    // face_extra: own side from the cell's half state, neighbour side from
+    // the neighbour's (or, on a periodic slot, from the mirror point)
+    Z zm = extrap(zch, gx, gy, fx - ccx, fy - ccy);
+    zm.e = eos_e(eos, zm.r, zm.p);
+    if (is_bad(zm, eos)) zm = zch;
+    double px = ncx, py = ncy;
+    if (t != SC_INNER) frame_symm(fr, ccx, ccy, fx, fy, px, py);
+    Z zp = extrap(hn, ngx, ngy, fx - px, fy - py);
+    zp.e = eos_e(eos, zp.r, zp.p);
+    if (is_bad(zp, eos)) zp = hn;
+    if (bound) zp = frame_boundary(fr, zm, wall);
+    if (robust) {
+        zm = zch;
+        zp = bound ? frame_boundary(fr, zch, wall) : hn;
+    }
+    const Z lm = frame_local(fr, zm), lp = frame_local(fr, zp);
+    Q f = robust ? flux_hll(lm, lp, eos, bad) : num_flux<FLUX>(lm, lp, eos, bad);
+    f = frame_global(fr, f);
+    return Q{f.r * a, f.mx * a, f.my * a, f.E * a};
+}
+
 // SmFluid::fluxes_stage2, corrector (sm_fluid.cpp:255-366), plus the next
 // step's CFL candidate.  `wait` is double-buffered: neighbours' isolation is
 // read from the pre-pass array.
 template <int FLUX>
-__global__ void __launch_bounds__(kBlock) k_corr(DevMesh M, DevState init, DevState half, DevGrad G,
-                                                 const int* __restrict__ wait_in, int* __restrict__ wait_out,
-                                                 DevState next, Eos eos, StepCtl ctl) {
-    const int n = M.n;
-    const int c = cell_of_thread();
-    const double dt = step_dt(ctl);
-    if (blockIdx.x == 0 && threadIdx.x == 0) *ctl.dt_out = dt;
+__global__ void __launch_bounds__(kMaxBlock) __attribute__((amdgpu_waves_per_eu(ZFV_WAVES_CORR)))
+k_corr(DevMesh M, DevState init, DevState half, DevGrad G, const int* __restrict__ wait_in,
+       int* __restrict__ wait_out, DevState next, Eos eos, StepCtl ctl) {
+    const Tile tl = tile_of(M, (int)blockDim.x);
+    const int c = tl.c0 + (int)threadIdx.x;
+    const double dt = step_dt(ctl, tl.pid);
+    if (tl.first && threadIdx.x == 0) ctl.dt_out[tl.pid] = dt;
     double cand = INFINITY;
-    if (c < n) {
-        const Z zc = load_z(init, c);
+    if (c < tl.end) {
         const Z zch = load_z(half, c);
         const double ccx = M.cx[c], ccy = M.cy[c];
         const int code = M.code[c];
@@ -363,53 +451,14 @@ __global__ void __launch_bounds__(kBlock) k_corr(DevMesh M, DevState init, DevSt
         bool bad = false;
         for (int s = 0; s < 8; ++s) {
             const int nib = (code >> (4 * s)) & 15;
-            const int t = nib & 7;
-            if (t == SC_UNDEF) continue;
-            const Frame fr = slot_frame(M, c, s, nib);
-            const int nb = M.adj[s * M.stride + c];
-            const bool bound = t >= SC_WALL;
-            const bool wall = t == SC_WALL;
-            const Z znh = bound ? frame_boundary(fr, zch, wall) : load_z(half, nb);
-            const bool robust = wc > 0 || wait_in[nb] > 0;
-            Z zm, zp;
-            if (robust) {
-                zm = zch;
-                zp = znh;
-            } else {
-                const double fx = M.fcx[s * M.stride + c], fy = M.fcy[s * M.stride + c];
-                zm = extrap(zch, gx, gy, fx - ccx, fy - ccy);
-                zm.e = eos_e(eos, zm.r, zm.p);
-                if (is_bad(zm, eos)) zm = zch;
-                if (!bound) {
-                    double nx, ny;
-                    if (t == SC_INNER) {
-                        nx = M.cx[nb];
-                        ny = M.cy[nb];
-                    } else {
-                        frame_symm(fr, ccx, ccy, fx, fy, nx, ny);
-                    }
-                    double ngx[4], ngy[4];
-#pragma unroll
-                    for (int k = 0; k < 4; ++k) {
-                        ngx[k] = G.dx[k * M.stride + nb];
-                        ngy[k] = G.dy[k * M.stride + nb];
-                    }
-                    zp = extrap(znh, ngx, ngy, fx - nx, fy - ny);
-                    zp.e = eos_e(eos, zp.r, zp.p);
-                    if (is_bad(zp, eos)) zp = znh;
-                } else {
-                    zp = frame_boundary(fr, zm, wall);
-                }
-            }
-            const Z lm = frame_local(fr, zm), lp = frame_local(fr, zp);
-            Q f = robust ? flux_hll(lm, lp, eos, bad) : num_flux<FLUX>(lm, lp, eos, bad);
-            f = frame_global(fr, f);
-            const double a = M.area[s * M.stride + c];
-            F0 = F0 + f.r * a;
-            F1 = F1 + f.mx * a;
-            F2 = F2 + f.my * a;
-            F3 = F3 + f.E * a;
+            if ((nib & 7) == SC_UNDEF) continue;
+            const Q f = corr_face<FLUX>(M, half, G, wait_in, c, s, nib, zch, wc, ccx, ccy, gx, gy, eos, bad);
+            F0 = F0 + f.r;
+            F1 = F1 + f.mx;
+            F2 = F2 + f.my;
+            F3 = F3 + f.E;
         }
+        const Z zc = load_z(init, c);
         Q q = to_q(zc);
         const double coef = dt / M.vol[c];
         q.r = q.r - coef * F0;
@@ -429,7 +478,384 @@ __global__ void __launch_bounds__(kBlock) k_corr(DevMesh M, DevState init, DevSt
         if (bad) *ctl.err = 1;
         cand = cell_dt(zn, M.h[c], eos);
     }
-    block_min_atomic(&ctl.dtmin[ctl.wr], cand);
+    block_min_atomic(&ctl.dtmin[3 * tl.pid + ctl.wr], cand);
+}
+
+// ---------------------------------------------------------------- quad layout
+// Four lanes per cell (a lane quad): lane j of the quad evaluates slot j
+// (side L, R, B, T) and, when it is defined, slot j + 4.  The per-slot terms
+// are combined in slot order through DPP quad broadcasts, so every sum is the
+// same sequence of additions as in the one-thread-per-cell kernels above and
+// the results are bit-identical; a wavefront holds 16 cells and each lane
+// carries one face, which gives four times the waves per cell count.
+namespace {
+
+template <int J>
+__device__ __forceinline__ double qget(double v) {
+    constexpr int ctrl = J * 0x55;   // quad_perm [J, J, J, J]
+    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), ctrl, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), ctrl, 0xF, 0xF, false);
+    return __hiloint2double(hi, lo);
+}
+
+// Sums in slot order.  qlo: 0.0 plus, for each main slot s = 0..3 set in
+// `mask`, the term held by lane s of the quad.  qhi: continues `acc` with
+// the sub-slots s = 4..7 set in `mask`, the term of slot s held by lane s - 4.
+// qlo then qhi is the sequential slot-order sum over all eight slots.
+__device__ __forceinline__ double qlo(double v, unsigned mask) {
+    double acc = 0.0, x;
+    x = qget<0>(v); if (mask & 1u) acc = acc + x;
+    x = qget<1>(v); if (mask & 2u) acc = acc + x;
+    x = qget<2>(v); if (mask & 4u) acc = acc + x;
+    x = qget<3>(v); if (mask & 8u) acc = acc + x;
+    return acc;
+}
+__device__ __forceinline__ double qhi(double acc, double v, unsigned mask) {
+    double x;
+    x = qget<0>(v); if (mask & 16u) acc = acc + x;
+    x = qget<1>(v); if (mask & 32u) acc = acc + x;
+    x = qget<2>(v); if (mask & 64u) acc = acc + x;
+    x = qget<3>(v); if (mask & 128u) acc = acc + x;
+    return acc;
+}
+
+
+
+// Slot masks of a code word: defined slots, and the slots whose LSM x / y
+// rows receive a term (canonical L/R normals have drn.y = 0, B/T drn.x = 0).
+struct Masks {
+    unsigned def, xm, ym;
+};
+__device__ __forceinline__ Masks slot_masks(int code) {
+    unsigned def = 0u, gen = 0u;
+#pragma unroll
+    for (int s = 0; s < 8; ++s) {
+        const int nib = (code >> (4 * s)) & 15;
+        if (nib & 7) def |= 1u << s;
+        if (nib & kGeneralNormal) gen |= 1u << s;
+    }
+    return Masks{def, def & (gen | 0x33u), def & (gen | 0xCCu)};
+}
+
+template <class T>
+__device__ __forceinline__ T by_side(int side, T a, T b, T c, T d) {
+    return side == 0 ? a : (side == 1 ? b : (side == 2 ? c : d));
+}
+
+// Store a state: lane j of the quad writes rho / u / v / p, lane 0 also e.
+__device__ __forceinline__ void store_z_quad(const DevState& s, int c, int side, const Z& z) {
+    double* dst = by_side(side, s.r, s.u, s.v, s.p);
+    dst[c] = by_side(side, z.r, z.u, z.v, z.p);
+    if (side == 0) s.e[c] = z.e;
+}
+
+// Flux term of slot s (f * area), first order.
+template <int FLUX>
+__device__ __forceinline__ Q flux1_term(const DevMesh& M, const DevState& in, const Z& zc, int c, int s, int nib,
+                                        const Eos& eos, bool& bad) {
+    const int t = nib & 7;
+    const Frame fr = slot_frame(M, c, s, nib);
+    Z zn;
+    if (t <= SC_PERIODIC) zn = load_z(in, M.adj[s * M.stride + c]);
+    else zn = frame_boundary(fr, zc, t == SC_WALL);
+    Q f = num_flux<FLUX>(frame_local(fr, zc), frame_local(fr, zn), eos, bad);
+    f = frame_global(fr, f);
+    const double a = M.area[s * M.stride + c];
+    return Q{f.r * a, f.mx * a, f.my * a, f.E * a};
+}
+
+// Slot-order sums of the four flux components: main slots from t0, then
+// (SUB) the sub-slots from t1.
+template <bool SUB>
+__device__ __forceinline__ void qsum4(const Q& t0, const Q& t1, unsigned mask, double& F0, double& F1, double& F2,
+                                      double& F3) {
+    F0 = qlo(t0.r, mask);
+    F1 = qlo(t0.mx, mask);
+    F2 = qlo(t0.my, mask);
+    F3 = qlo(t0.E, mask);
+    if (SUB) {
+        F0 = qhi(F0, t1.r, mask);
+        F1 = qhi(F1, t1.mx, mask);
+        F2 = qhi(F2, t1.my, mask);
+        F3 = qhi(F3, t1.E, mask);
+    }
+}
+
+}  // namespace
+
+template <int FLUX>
+__global__ void __launch_bounds__(kMaxBlock) __attribute__((amdgpu_waves_per_eu(ZFV_WAVES_FLUX1Q)))
+k_flux1_q(DevMesh M, DevState in, DevState out, Eos eos, StepCtl ctl) {
+    const Tile tl = tile_of(M, (int)blockDim.x >> 2);
+    const int c = tl.c0 + ((int)threadIdx.x >> 2), side = threadIdx.x & 3;
+    const double dt = step_dt(ctl, tl.pid);
+    step_prologue(ctl, tl, dt);
+    double cand = INFINITY;
+    const bool valid = c < tl.end;
+    const int code = valid ? M.code[c] : 0;
+    const bool sub = __any((code >> 16) != 0);
+    if (valid) {
+        const Z zc = load_z(in, c);
+        const Masks mk = slot_masks(code);
+        bool bad = false;
+        Q t0{0.0, 0.0, 0.0, 0.0}, t1{0.0, 0.0, 0.0, 0.0};
+        const int nib0 = (code >> (4 * side)) & 15;
+        if (nib0 & 7) t0 = flux1_term<FLUX>(M, in, zc, c, side, nib0, eos, bad);
+        const int nib1 = (code >> (4 * (side + 4))) & 15;
+        if (nib1 & 7) t1 = flux1_term<FLUX>(M, in, zc, c, side + 4, nib1, eos, bad);
+        double F0, F1, F2, F3;
+        if (sub) qsum4<true>(t0, t1, mk.def, F0, F1, F2, F3);
+        else qsum4<false>(t0, t1, mk.def, F0, F1, F2, F3);
+        Q q = to_q(zc);
+        const double coef = dt / M.vol[c];
+        q.r = q.r - coef * F0;
+        q.mx = q.mx - coef * F1;
+        q.my = q.my - coef * F2;
+        q.E = q.E - coef * F3;
+        const Z zn = to_z(q, eos);
+        store_z_quad(out, c, side, zn);
+        if (bad) *ctl.err = 1;
+        cand = cell_dt(zn, M.h[c], eos);
+    }
+    block_min_atomic(&ctl.dtmin[3 * tl.pid + ctl.wr], cand);
+}
+
+namespace {
+
+// LSM data of one slot (gradient.h:245-256).
+struct LsmSlot {
+    Z zn;
+    double drx, dry, dnx, dny, w;
+};
+
+__device__ __forceinline__ LsmSlot lsm_slot(const DevMesh& M, const DevState& in, int c, int s, int nib,
+                                            const Z& zc, double ccx, double ccy) {
+    LsmSlot L;
+    const Frame fr = slot_frame(M, c, s, nib);
+    lsm_neib(M, in, c, s, nib & 7, fr, zc, ccx, ccy, L.zn, L.drx, L.dry);
+    lsm_drn(fr, L.drx, L.dry, L.dnx, L.dny);
+    L.w = M.w[s * M.stride + c];
+    return L;
+}
+
+// Predictor flux term of slot s (sm_fluid.cpp:176-253).
+__device__ __forceinline__ Q pred_term(const DevMesh& M, int c, int s, int nib, const Z& zc, double ccx, double ccy,
+                                       const double* gx, const double* gy, const Eos& eos) {
+    const Frame fr = slot_frame(M, c, s, nib);
+    const double dx = M.fcx[s * M.stride + c] - ccx;
+    const double dy = M.fcy[s * M.stride + c] - ccy;
+    Z zm = extrap(zc, gx, gy, dx, dy);
+    zm.e = eos_e(eos, zm.r, zm.p);
+    if (is_bad(zm, eos)) zm = zc;
+    Q f = phys_flux(frame_local(fr, zm));
+    f = frame_global(fr, f);
+    const double a = M.area[s * M.stride + c];
+    return Q{f.r * a, f.mx * a, f.my * a, f.E * a};
+}
+
+}  // namespace
+
+// LSM pass-1 terms of one slot added to the running slot-order sums.
+template <bool HI>
+__device__ __forceinline__ void lsm_acc(const LsmSlot& L, const Z& zc, const Masks& mk, double* A, double* Fx,
+                                        double* Fy) {
+    const double wx = L.w * L.dnx, wy = L.w * L.dny;
+    const double d[4] = {L.zn.r - zc.r, L.zn.u - zc.u, L.zn.v - zc.v, L.zn.p - zc.p};
+    if (HI) {
+        A[0] = qhi(A[0], wx * L.drx, mk.xm);
+        A[1] = qhi(A[1], wx * L.dry, mk.xm);
+        A[2] = qhi(A[2], wy * L.drx, mk.ym);
+        A[3] = qhi(A[3], wy * L.dry, mk.ym);
+    } else {
+        A[0] = qlo(wx * L.drx, mk.xm);
+        A[1] = qlo(wx * L.dry, mk.xm);
+        A[2] = qlo(wy * L.drx, mk.ym);
+        A[3] = qlo(wy * L.dry, mk.ym);
+    }
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const double e = L.w * d[k];
+        if (HI) {
+            Fx[k] = qhi(Fx[k], e * L.dnx, mk.xm);
+            Fy[k] = qhi(Fy[k], e * L.dny, mk.ym);
+        } else {
+            Fx[k] = qlo(e * L.dnx, mk.xm);
+            Fy[k] = qlo(e * L.dny, mk.ym);
+        }
+    }
+}
+
+// Limiting pass terms of one slot (gradient.h:412-474) added to the sums.
+template <int LIM, bool HI>
+__device__ __forceinline__ void lim_acc(const LsmSlot& L, const Z& zc, const Masks& mk, const double* gx,
+                                        const double* gy, double* Lx, double* Ly) {
+    const double d[4] = {L.zn.r - zc.r, L.zn.u - zc.u, L.zn.v - zc.v, L.zn.p - zc.p};
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const double du_n = gx[k] * L.drx + gy[k] * L.dry;
+        const double lim = limiter2<LIM>(2.0 * du_n - d[k], d[k]) * d[k];
+        const double e = L.w * lim;
+        if (HI) {
+            Lx[k] = qhi(Lx[k], e * L.dnx, mk.xm);
+            Ly[k] = qhi(Ly[k], e * L.dny, mk.ym);
+        } else {
+            Lx[k] = qlo(e * L.dnx, mk.xm);
+            Ly[k] = qlo(e * L.dny, mk.ym);
+        }
+    }
+}
+
+template <int LIM, bool PRED, bool SUB>
+__device__ __forceinline__ void grad_pred_quad(const DevMesh& M, const DevState& in, const DevGrad& G,
+                                               const DevState& half, const int* __restrict__ wait, const Eos& eos,
+                                               double dt, double* __restrict__ raw, int c, int side, int code) {
+    const Z zc = load_z(in, c);
+    const double ccx = M.cx[c], ccy = M.cy[c];
+    const Masks mk = slot_masks(code);
+    const int nib0 = (code >> (4 * side)) & 15;
+    const int nib1 = (code >> (4 * (side + 4))) & 15;
+    LsmSlot L0{};
+    if (nib0 & 7) L0 = lsm_slot(M, in, c, side, nib0, zc, ccx, ccy);
+
+    // pass 1: A = sum w drn dr^T, F = sum w dU drn, in slot order
+    double A[4], Fx[4], Fy[4];
+    lsm_acc<false>(L0, zc, mk, A, Fx, Fy);
+    if (SUB) {
+        LsmSlot L1{};
+        if (nib1 & 7) L1 = lsm_slot(M, in, c, side + 4, nib1, zc, ccx, ccy);
+        lsm_acc<true>(L1, zc, mk, A, Fx, Fy);
+    }
+    const double A00 = A[0], A01 = A[1], A10 = A[2];
+    double A11 = A[3];
+    if (fabs(A11) < 1e-8) A11 = 1.0;   // A(2,2) = 0 -> 1 always holds in 2D (gradient.h:268-273)
+    const double det = A11 * A00 + (-A01) * A10;
+    const double invdet = 1.0 / det;
+    const double B00 = A11 * invdet, B01 = (-A01) * invdet;
+    const double B10 = (-A10) * invdet, B11 = A00 * invdet;
+    double gx[4], gy[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        gx[k] = B00 * Fx[k] + B01 * Fy[k];
+        gy[k] = B10 * Fx[k] + B11 * Fy[k];
+    }
+    if (!PRED && raw && side == 0) {
+        raw[c] = gx[0];
+        raw[M.stride + c] = gy[0];
+    }
+
+    // pass 2: limited differences, re-solved with the same A
+    double Lx[4], Ly[4];
+    lim_acc<LIM, false>(L0, zc, mk, gx, gy, Lx, Ly);
+    if (SUB) {
+        LsmSlot L1{};
+        if (nib1 & 7) L1 = lsm_slot(M, in, c, side + 4, nib1, zc, ccx, ccy);
+        lim_acc<LIM, true>(L1, zc, mk, gx, gy, Lx, Ly);
+    }
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        gx[k] = B00 * Lx[k] + B01 * Ly[k];
+        gy[k] = B10 * Lx[k] + B11 * Ly[k];
+    }
+    G.dx[side * M.stride + c] = by_side(side, gx[0], gx[1], gx[2], gx[3]);
+    G.dy[side * M.stride + c] = by_side(side, gy[0], gy[1], gy[2], gy[3]);
+    if (!PRED) return;
+
+    // predictor: half = PState(q - (dt/2V) sum Flux(z_c + grad.(x_f - x_c)) S)
+    Z zh = zc;
+    if (wait[c] <= 0) {
+        Q t0{0.0, 0.0, 0.0, 0.0}, t1{0.0, 0.0, 0.0, 0.0};
+        if (nib0 & 7) t0 = pred_term(M, c, side, nib0, zc, ccx, ccy, gx, gy, eos);
+        if (SUB && (nib1 & 7)) t1 = pred_term(M, c, side + 4, nib1, zc, ccx, ccy, gx, gy, eos);
+        double F0, F1, F2, F3;
+        qsum4<SUB>(t0, t1, mk.def, F0, F1, F2, F3);
+        Q q = to_q(zc);
+        const double coef = 0.5 * dt / M.vol[c];
+        q.r = q.r - coef * F0;
+        q.mx = q.mx - coef * F1;
+        q.my = q.my - coef * F2;
+        q.E = q.E - coef * F3;
+        zh = to_z(q, eos);
+        if (is_bad(zh, eos)) zh = zc;
+    }
+    store_z_quad(half, c, side, zh);
+}
+
+template <int LIM, bool PRED>
+__global__ void __launch_bounds__(kMaxBlock) __attribute__((amdgpu_waves_per_eu(ZFV_WAVES_GRADQ)))
+k_grad_pred_q(DevMesh M, DevState in, DevGrad G, DevState half, const int* __restrict__ wait, Eos eos, StepCtl ctl,
+              double* __restrict__ raw) {
+    const Tile tl = tile_of(M, (int)blockDim.x >> 2);
+    const int c = tl.c0 + ((int)threadIdx.x >> 2), side = threadIdx.x & 3;
+    const double dt = step_dt(ctl, tl.pid);
+    if (PRED) {
+        if (tl.first && threadIdx.x == 0) ctl.dtmin[3 * tl.pid + ctl.clr] = kInfBits;
+    }
+    const bool valid = c < tl.end;
+    const int code = valid ? M.code[c] : 0;
+    const bool sub = __any((code >> 16) != 0);
+    if (!valid) return;
+    if (sub) grad_pred_quad<LIM, PRED, true>(M, in, G, half, wait, eos, dt, raw, c, side, code);
+    else grad_pred_quad<LIM, PRED, false>(M, in, G, half, wait, eos, dt, raw, c, side, code);
+}
+
+namespace {
+
+}  // namespace
+
+template <int FLUX>
+__global__ void __launch_bounds__(kMaxBlock) __attribute__((amdgpu_waves_per_eu(ZFV_WAVES_CORRQ)))
+k_corr_q(DevMesh M, DevState init, DevState half, DevGrad G, const int* __restrict__ wait_in,
+         int* __restrict__ wait_out, DevState next, Eos eos, StepCtl ctl) {
+    const Tile tl = tile_of(M, (int)blockDim.x >> 2);
+    const int c = tl.c0 + ((int)threadIdx.x >> 2), side = threadIdx.x & 3;
+    const double dt = step_dt(ctl, tl.pid);
+    if (tl.first && threadIdx.x == 0) ctl.dt_out[tl.pid] = dt;
+    double cand = INFINITY;
+    const bool valid = c < tl.end;
+    const int code = valid ? M.code[c] : 0;
+    const bool sub = __any((code >> 16) != 0);
+    if (valid) {
+        const Z zch = load_z(half, c);
+        const double ccx = M.cx[c], ccy = M.cy[c];
+        const int wc = wait_in[c];
+        double gx[4], gy[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            gx[k] = G.dx[k * M.stride + c];
+            gy[k] = G.dy[k * M.stride + c];
+        }
+        const Masks mk = slot_masks(code);
+        bool bad = false;
+        Q t0{0.0, 0.0, 0.0, 0.0}, t1{0.0, 0.0, 0.0, 0.0};
+        const int nib0 = (code >> (4 * side)) & 15;
+        if (nib0 & 7) t0 = corr_face<FLUX>(M, half, G, wait_in, c, side, nib0, zch, wc, ccx, ccy, gx, gy, eos, bad);
+        const int nib1 = (code >> (4 * (side + 4))) & 15;
+        if (nib1 & 7)
+            t1 = corr_face<FLUX>(M, half, G, wait_in, c, side + 4, nib1, zch, wc, ccx, ccy, gx, gy, eos, bad);
+        double F0, F1, F2, F3;
+        if (sub) qsum4<true>(t0, t1, mk.def, F0, F1, F2, F3);
+        else qsum4<false>(t0, t1, mk.def, F0, F1, F2, F3);
+        const Z zc = load_z(init, c);
+        Q q = to_q(zc);
+        const double coef = dt / M.vol[c];
+        q.r = q.r - coef * F0;
+        q.mx = q.mx - coef * F1;
+        q.my = q.my - coef * F2;
+        q.E = q.E - coef * F3;
+        Z zn = to_z(q, eos);
+        int wn;
+        if (is_bad(zn, eos)) {
+            zn = zc;
+            wn = 3;
+        } else {
+            wn = wc > 0 ? wc - 1 : wc;
+        }
+        store_z_quad(next, c, side, zn);
+        if (side == 0) wait_out[c] = wn;
+        if (bad) *ctl.err = 1;
+        cand = cell_dt(zn, M.h[c], eos);
+    }
+    block_min_atomic(&ctl.dtmin[3 * tl.pid + ctl.wr], cand);
 }
 
 // Per-workgroup partial sums of V*q (conserved totals).
@@ -459,7 +885,9 @@ __global__ void __launch_bounds__(kBlock) k_totals(DevMesh M, DevState z, double
 }
 
 // ---------------------------------------------------------------- launchers
-static inline int grid_of(int n) { return (n + kBlock - 1) / kBlock; }
+// Grids: `block` threads per workgroup, a workgroup covers `block` cells
+// (cell layout) or block / 4 cells (quad layout).
+static inline int grid_cells(int n, int cells) { return (n + cells - 1) / cells; }
 
 hipError_t launch_fill_u64(unsigned long long* p, int count, unsigned long long v, hipStream_t s) {
     hipLaunchKernelGGL(k_fill_u64, dim3((count + 255) / 256), dim3(256), 0, s, p, count, v);
@@ -467,13 +895,24 @@ hipError_t launch_fill_u64(unsigned long long* p, int count, unsigned long long 
 }
 
 hipError_t launch_dt(const DevMesh& M, const DevState& z, const Eos& eos, unsigned long long* slot, hipStream_t s) {
-    hipLaunchKernelGGL(k_dt, dim3(grid_of(M.n)), dim3(kBlock), 0, s, M, z, eos, slot);
+    hipLaunchKernelGGL(k_dt, dim3(grid_cells(M.n, kBlock)), dim3(kBlock), 0, s, M, z, eos, slot);
     return hipGetLastError();
 }
 
-hipError_t launch_flux1(int flux, const DevMesh& M, const DevState& in, const DevState& out,
+hipError_t launch_flux1(int flux, int layout, int block, const DevMesh& M, const DevState& in, const DevState& out,
                         const Eos& eos, const StepCtl& ctl, hipStream_t s) {
-    dim3 g(grid_of(M.n)), b(kBlock);
+    dim3 b(block);
+    if (layout == kLayoutQuad) {
+        dim3 g(grid_cells(M.n, block / 4));
+        switch (flux) {
+            case 0: hipLaunchKernelGGL(k_flux1_q<0>, g, b, 0, s, M, in, out, eos, ctl); break;
+            case 1: hipLaunchKernelGGL(k_flux1_q<1>, g, b, 0, s, M, in, out, eos, ctl); break;
+            case 2: hipLaunchKernelGGL(k_flux1_q<2>, g, b, 0, s, M, in, out, eos, ctl); break;
+            default: hipLaunchKernelGGL(k_flux1_q<3>, g, b, 0, s, M, in, out, eos, ctl); break;
+        }
+        return hipGetLastError();
+    }
+    dim3 g(grid_cells(M.n, block));
     switch (flux) {
         case 0: hipLaunchKernelGGL(k_flux1<0>, g, b, 0, s, M, in, out, eos, ctl); break;
         case 1: hipLaunchKernelGGL(k_flux1<1>, g, b, 0, s, M, in, out, eos, ctl); break;
@@ -483,11 +922,15 @@ hipError_t launch_flux1(int flux, const DevMesh& M, const DevState& in, const De
     return hipGetLastError();
 }
 
-template <bool PRED>
+template <bool PRED, bool QUAD>
 static void launch_gp_t(int limiter, dim3 g, dim3 b, hipStream_t s, const DevMesh& M, const DevState& in,
                         const DevGrad& G, const DevState& half, const int* wait, const Eos& eos,
                         const StepCtl& ctl, double* raw) {
-#define ZFV_GP(L) hipLaunchKernelGGL((k_grad_pred<L, PRED>), g, b, 0, s, M, in, G, half, wait, eos, ctl, raw)
+#define ZFV_GP(L)                                                                                          \
+    do {                                                                                                   \
+        if (QUAD) hipLaunchKernelGGL((k_grad_pred_q<L, PRED>), g, b, 0, s, M, in, G, half, wait, eos, ctl, raw); \
+        else hipLaunchKernelGGL((k_grad_pred<L, PRED>), g, b, 0, s, M, in, G, half, wait, eos, ctl, raw);       \
+    } while (0)
     switch (limiter) {
         case 0: ZFV_GP(0); break;
         case 1: ZFV_GP(1); break;
@@ -506,25 +949,45 @@ static void launch_gp_t(int limiter, dim3 g, dim3 b, hipStream_t s, const DevMes
 #undef ZFV_GP
 }
 
-hipError_t launch_grad_pred(int limiter, bool predictor, const DevMesh& M, const DevState& in, const DevGrad& G,
-                            const DevState& half, const int* wait, const Eos& eos, const StepCtl& ctl,
-                            hipStream_t s, double* raw) {
-    dim3 g(grid_of(M.n)), b(kBlock);
-    if (predictor) launch_gp_t<true>(limiter, g, b, s, M, in, G, half, wait, eos, ctl, nullptr);
-    else launch_gp_t<false>(limiter, g, b, s, M, in, G, half, wait, eos, ctl, raw);
+hipError_t launch_grad_pred(int limiter, bool predictor, int layout, int block, const DevMesh& M, const DevState& in,
+                            const DevGrad& G, const DevState& half, const int* wait, const Eos& eos,
+                            const StepCtl& ctl, hipStream_t s, double* raw) {
+    dim3 b(block);
+    if (layout == kLayoutQuad) {
+        dim3 g(grid_cells(M.n, block / 4));
+        if (predictor) launch_gp_t<true, true>(limiter, g, b, s, M, in, G, half, wait, eos, ctl, nullptr);
+        else launch_gp_t<false, true>(limiter, g, b, s, M, in, G, half, wait, eos, ctl, raw);
+    } else {
+        dim3 g(grid_cells(M.n, block));
+        if (predictor) launch_gp_t<true, false>(limiter, g, b, s, M, in, G, half, wait, eos, ctl, nullptr);
+        else launch_gp_t<false, false>(limiter, g, b, s, M, in, G, half, wait, eos, ctl, raw);
+    }
     return hipGetLastError();
 }
 
-hipError_t launch_corr(int flux, const DevMesh& M, const DevState& init, const DevState& half,
+hipError_t launch_corr(int flux, int layout, int block, const DevMesh& M, const DevState& init, const DevState& half,
                        const DevGrad& G, const int* wait_in, int* wait_out, const DevState& next,
                        const Eos& eos, const StepCtl& ctl, hipStream_t s) {
-    dim3 g(grid_of(M.n)), b(kBlock);
-    switch (flux) {
-        case 0: hipLaunchKernelGGL(k_corr<0>, g, b, 0, s, M, init, half, G, wait_in, wait_out, next, eos, ctl); break;
-        case 1: hipLaunchKernelGGL(k_corr<1>, g, b, 0, s, M, init, half, G, wait_in, wait_out, next, eos, ctl); break;
-        case 2: hipLaunchKernelGGL(k_corr<2>, g, b, 0, s, M, init, half, G, wait_in, wait_out, next, eos, ctl); break;
-        default: hipLaunchKernelGGL(k_corr<3>, g, b, 0, s, M, init, half, G, wait_in, wait_out, next, eos, ctl); break;
+    dim3 b(block);
+#define ZFV_CORR(K, F) hipLaunchKernelGGL(K<F>, g, b, 0, s, M, init, half, G, wait_in, wait_out, next, eos, ctl)
+    if (layout == kLayoutQuad) {
+        dim3 g(grid_cells(M.n, block / 4));
+        switch (flux) {
+            case 0: ZFV_CORR(k_corr_q, 0); break;
+            case 1: ZFV_CORR(k_corr_q, 1); break;
+            case 2: ZFV_CORR(k_corr_q, 2); break;
+            default: ZFV_CORR(k_corr_q, 3); break;
+        }
+    } else {
+        dim3 g(grid_cells(M.n, block));
+        switch (flux) {
+            case 0: ZFV_CORR(k_corr, 0); break;
+            case 1: ZFV_CORR(k_corr, 1); break;
+            case 2: ZFV_CORR(k_corr, 2); break;
+            default: ZFV_CORR(k_corr, 3); break;
+        }
     }
+#undef ZFV_CORR
     return hipGetLastError();
 }
 
