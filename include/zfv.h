@@ -190,6 +190,20 @@ int zfv_batch_layout(zfv_ctx* ctx, int32_t* n_problems, int32_t* begin, int32_t*
 /* dt of the last step of every problem (n_problems doubles; synchronises). */
 int zfv_batch_dt(zfv_ctx* ctx, double* dt);
 
+/* ---- discrete adjoint (no reference counterpart) ---------------------------- */
+/* Gradient of J = integral over the window {xmin, xmax, ymin, ymax} of p^2 at
+ * the final time with respect to the initial primitive state, for the MUSCL
+ * (acc 2) + HLLC step on a Cartesian mesh (limiter none, unit, minmod,
+ * superbee or MC).  zfv_adjoint_forward runs n_steps steps from the current
+ * state, keeping a checkpoint every `every` steps, and returns J.
+ * zfv_adjoint_backward recomputes each segment from its checkpoint and runs
+ * the reverse sweep; it returns dJ/d{rho, u, v, p, e} of the initial state
+ * (n doubles each; e as an independent input, so a caller holding e(rho, p)
+ * adds the chain rule) and leaves the solver at the final state.  Steps that
+ * isolate a cell (bad state) are not differentiated: ZFV_ERR_UNSUPPORTED. */
+int zfv_adjoint_forward(zfv_ctx* ctx, int n_steps, int every, const double* window, double* J);
+int zfv_adjoint_backward(zfv_ctx* ctx, double* d_rho, double* d_u, double* d_v, double* d_p, double* d_e);
+
 /* ---- launch configuration (no reference counterpart) --------------------- */
 /* Step kernels of a stage (0 = first-order update, 1 = gradient + predictor,
  * 2 = corrector): layout 0 = one thread per cell, 1 = four lanes per cell

@@ -31,6 +31,7 @@ EXPORTS = [
     "zfv_flags_upload", "zfv_flags_download", "zfv_set_flags", "zfv_balance_flags", "zfv_refine",
     "zfv_mesh_n_faces", "zfv_set_launch", "zfv_get_launch", "zfv_autotune",
     "zfv_batch_rectangles", "zfv_batch_layout", "zfv_batch_dt", "zfv_graph_enable",
+    "zfv_adjoint_forward", "zfv_adjoint_backward",
 ]
 CELL, QUAD = 0, 1   # step-kernel layouts
 
@@ -92,6 +93,8 @@ def lib():
         L.zfv_autotune.argtypes = [vp, C.c_int, vp]
         L.zfv_batch_rectangles.argtypes = [vp, C.c_int, vp, vp, vp, vp]
         L.zfv_graph_enable.argtypes = [vp, C.c_int]
+        L.zfv_adjoint_forward.argtypes = [vp, C.c_int, C.c_int, vp, vp]
+        L.zfv_adjoint_backward.argtypes = [vp, vp, vp, vp, vp, vp]
         _lib = L
     return _lib
 
@@ -288,6 +291,20 @@ class Context:
 
     def graph_enable(self, on=True):
         _check(lib().zfv_graph_enable(self.h, 1 if on else 0))
+
+    # ---- discrete adjoint
+    def adjoint_forward(self, n_steps, every, window):
+        """n_steps steps with a checkpoint every `every`; returns J = int_window p^2 dA."""
+        win = np.ascontiguousarray(window, np.float64)
+        J = C.c_double()
+        _check(lib().zfv_adjoint_forward(self.h, n_steps, every, _ptr(win), C.byref(J)))
+        return J.value
+
+    def adjoint_backward(self):
+        """dJ/d{rho, u, v, p, e} of the state the last adjoint_forward started from."""
+        arrs = [np.zeros(self.n) for _ in range(5)]
+        _check(lib().zfv_adjoint_backward(self.h, *[_ptr(a) for a in arrs]))
+        return arrs
 
     # ---- batches
     def batch_rectangles(self, sizes, boxes=None, bounds=None):

@@ -16,6 +16,7 @@ LIB_PATH = os.path.join(BUILD_DIR, "libzfv.so")
 SOURCES = [
     os.path.join(HERE, "csrc", "zfv_kernels.hip"),
     os.path.join(HERE, "csrc", "zfv_amr.hip"),
+    os.path.join(HERE, "csrc", "zfv_adjoint.hip"),
     os.path.join(HERE, "csrc", "zfv_capi.cpp"),
 ]
 HEADERS = [

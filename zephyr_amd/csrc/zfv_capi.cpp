@@ -36,7 +36,7 @@ struct DBuf {
     size_t n = 0;
     ~DBuf() { release(); }
     void release() {
-        if (p) hipFree(p);
+        if (p) (void)hipFree(p);
         p = nullptr;
         n = 0;
     }
@@ -125,6 +125,17 @@ struct zfv_ctx {
         for (auto& g : graphs) (void)hipGraphExecDestroy(g.exec);
         graphs.clear();
     }
+
+    // discrete adjoint: checkpoints of the forward run and reverse-sweep buffers
+    struct Adjoint {
+        int n_steps = 0, every = 0;
+        double win[4] = {0, 0, 0, 0};
+        bool ready = false;
+        std::vector<DBuf<double>> ckpt;   // [5][n] state every `every` steps
+        DBuf<double> seg, final_z;         // segment states [every][5][n], final state [5][n]
+        DBuf<double> lam[2], lz, lh, lG, NB, NG, dtb, dts;
+        DBuf<unsigned long long> slot;
+    } adjoint;
 
     bool profile = false;
     std::vector<KernelTimer> timers;   // pending event pairs of the last zfv_smf_step
@@ -346,7 +357,7 @@ hipError_t grow_rows(DBuf<T>& b, int rows, int old_cap, int new_cap, int n_valid
         e = hipMemcpy2DAsync(np, (size_t)new_cap * sizeof(T), b.p, (size_t)old_cap * sizeof(T),
                              (size_t)n_valid * sizeof(T), rows, hipMemcpyDeviceToDevice, s);
         if (e != hipSuccess) {
-            hipFree(np);
+            (void)hipFree(np);
             return e;
         }
         e = hipStreamSynchronize(s);
@@ -536,14 +547,14 @@ int zfv_create(int device, zfv_ctx** out) {
 
 int zfv_destroy(zfv_ctx* ctx) {
     if (!ctx) return ZFV_OK;
-    hipSetDevice(ctx->device);
-    hipStreamSynchronize(ctx->stream);
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
     collect_timers(ctx);
     ctx->graphs_clear();
     for (hipEvent_t e : ctx->event_pool) (void)hipEventDestroy(e);
     hipStream_t s = ctx->own_stream;
     delete ctx;
-    if (s) hipStreamDestroy(s);
+    if (s) (void)hipStreamDestroy(s);
     return ZFV_OK;
 }
 
@@ -1210,6 +1221,165 @@ int zfv_refine(zfv_ctx* ctx, int max_level, int distributor) {
     ctx->dt_valid = true;
     ctx->graphs_clear();   // the cell count changed
     return ZFV_OK;
+}
+
+// ---- discrete adjoint -------------------------------------------------------
+static DevState state_at(double* base, int n) { return DevState{base, base + n, base + 2 * (size_t)n, base + 3 * (size_t)n, base + 4 * (size_t)n}; }
+
+static int copy_state(zfv_ctx* ctx, const DevState& dst, const DevState& src) {
+    const size_t b = (size_t)ctx->hm.n * sizeof(double);
+    double* const d[5] = {dst.r, dst.u, dst.v, dst.p, dst.e};
+    const double* const sr[5] = {src.r, src.u, src.v, src.p, src.e};
+    for (int k = 0; k < 5; ++k) ZFV_HIP(hipMemcpyAsync(d[k], sr[k], b, hipMemcpyDeviceToDevice, ctx->stream));
+    return ZFV_OK;
+}
+
+// The adjoint covers the MUSCL + HLLC step on one Cartesian mesh: no
+// sub-face slots, canonical normals, limiters without a derivative table
+// other than none / unit / minmod (superbee) / MC are refused.
+static int adjoint_supported(zfv_ctx* ctx) {
+    const zfv_smf_params& P = ctx->prm;
+    if (P.acc != 2 || P.flux != ZFV_HLLC) return fail(ZFV_ERR_UNSUPPORTED, "adjoint: MUSCL (acc 2) with HLLC only");
+    const int L = P.limiter;
+    if (!(L == ZFV_LIM_NONE || L == ZFV_LIM_UNIT || L == ZFV_LIM_MINMOD || L == ZFV_LIM_MC || L == ZFV_LIM_SUPERBEE))
+        return fail(ZFV_ERR_UNSUPPORTED, "adjoint: limiter must be none, unit, minmod, superbee or MC");
+    if (ctx->nseg != 1) return fail(ZFV_ERR_UNSUPPORTED, "adjoint of a batch");
+    int r = sync_host_mesh(ctx);
+    if (r) return r;
+    const HostMesh& m = ctx->hm;
+    for (int c = 0; c < m.n; ++c)
+        for (int s8 = 0; s8 < 8; ++s8) {
+            const size_t i = 8 * (size_t)c + s8;
+            if (s8 >= 4 && m.bnd[i] != ZFV_UNDEFINED) return fail(ZFV_ERR_UNSUPPORTED, "adjoint: adapted mesh");
+            if (s8 < 4 && !(m.nx[i] == kNX[s8] && m.ny[i] == kNY[s8]))
+                return fail(ZFV_ERR_UNSUPPORTED, "adjoint: non-Cartesian normals");
+        }
+    return ZFV_OK;
+}
+
+int zfv_adjoint_forward(zfv_ctx* ctx, int n_steps, int every, const double* window, double* J) {
+    int r = need_ready(ctx, true, true);
+    if (r) return r;
+    if (n_steps < 1 || every < 1 || !window || !J) return fail(ZFV_ERR_ARG, "adjoint_forward: bad arguments");
+    r = adjoint_supported(ctx);
+    if (r) return r;
+    auto& A = ctx->adjoint;
+    const int n = ctx->hm.n, cap = ctx->cap;
+    const int nck = (n_steps + every - 1) / every;
+    A.ready = false;
+    A.ckpt.resize(nck);
+    for (auto& b : A.ckpt)
+        if (b.n != 5 * (size_t)n) ZFV_HIP(b.alloc(5 * (size_t)n));
+    if (A.final_z.n != 5 * (size_t)n) ZFV_HIP(A.final_z.alloc(5 * (size_t)n));
+    if (A.seg.n != (size_t)every * 5 * n) ZFV_HIP(A.seg.alloc((size_t)every * 5 * n));
+    for (auto* b : {&A.lam[0], &A.lam[1], &A.lz, &A.lh})
+        if (b->n != 5 * (size_t)n) ZFV_HIP(b->alloc(5 * (size_t)n));
+    if (A.lG.n != 8 * (size_t)cap) ZFV_HIP(A.lG.alloc(8 * (size_t)cap));
+    if (A.NB.n != 52 * (size_t)cap) ZFV_HIP(A.NB.alloc(52 * (size_t)cap));
+    if (A.NG.n != 16 * (size_t)cap) ZFV_HIP(A.NG.alloc(16 * (size_t)cap));
+    if (A.dtb.n != 1) ZFV_HIP(A.dtb.alloc(1));
+    if (A.dts.n != 1) ZFV_HIP(A.dts.alloc(1));
+    if (A.slot.n != 3) ZFV_HIP(A.slot.alloc(3));
+    r = ensure_dt(ctx);
+    if (r) return r;
+    for (int i = 0; i < n_steps; ++i) {
+        if (i % every == 0) {
+            r = copy_state(ctx, state_at(A.ckpt[i / every].p, n), ctx->layer(ctx->cur));
+            if (r) return r;
+        }
+        r = step_once(ctx);
+        if (r) return r;
+    }
+    r = copy_state(ctx, state_at(A.final_z.p, n), ctx->layer(ctx->cur));
+    if (r) return r;
+    // J = sum over the window's cells of p^2 V, in cell order on the host
+    std::vector<double> p(n);
+    ZFV_HIP(hipMemcpyAsync(p.data(), ctx->layer(ctx->cur).p, n * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    ZFV_HIP(hipStreamSynchronize(ctx->stream));
+    r = read_err(ctx);
+    if (r) return r;
+    const HostMesh& m = ctx->hm;
+    double j = 0.0;
+    for (int c = 0; c < n; ++c) {
+        const double x = m.cx[c], y = m.cy[c];
+        if (x >= window[0] && x <= window[1] && y >= window[2] && y <= window[3]) j = j + p[c] * p[c] * m.vol[c];
+    }
+    *J = j;
+    A.n_steps = n_steps;
+    A.every = every;
+    for (int k = 0; k < 4; ++k) A.win[k] = window[k];
+    A.ready = true;
+    return ZFV_OK;
+}
+
+int zfv_adjoint_backward(zfv_ctx* ctx, double* lr, double* lu, double* lv, double* lp, double* le) {
+    int r = need_ready(ctx, true, true);
+    if (r) return r;
+    auto& A = ctx->adjoint;
+    if (!A.ready) return fail(ZFV_ERR_STATE, "adjoint_backward: no forward run to reverse");
+    if (!lr || !lu || !lv || !lp || !le) return fail(ZFV_ERR_ARG, "adjoint_backward: null output");
+    const int n = ctx->hm.n;
+    hipStream_t s = ctx->stream;
+    const DevMesh M = ctx->dmesh();
+    const Eos eos = ctx->eos();
+    const DevGrad G{ctx->gdx.p, ctx->gdy.p};
+    const DevGrad lG{A.lG.p, A.lG.p + 4 * (size_t)ctx->cap};
+    ZFV_HIP(adj_seed(M, state_at(A.final_z.p, n), A.win, state_at(A.lam[0].p, n), s));
+    int cur = 0;
+    const int nck = (int)A.ckpt.size();
+    for (int seg = nck - 1; seg >= 0; --seg) {
+        const int s0 = seg * A.every, len = std::min(A.every, A.n_steps - s0);
+        // recompute the segment's states from its checkpoint
+        r = copy_state(ctx, ctx->layer(ctx->cur), state_at(A.ckpt[seg].p, n));
+        if (r) return r;
+        ctx->dt_valid = false;
+        r = ensure_dt(ctx);
+        if (r) return r;
+        for (int j = 0; j < len; ++j) {
+            r = copy_state(ctx, state_at(A.seg.p + (size_t)j * 5 * n, n), ctx->layer(ctx->cur));
+            if (r) return r;
+            if (j < len - 1) {
+                r = step_once(ctx);
+                if (r) return r;
+            }
+        }
+        // reverse through the segment
+        for (int j = len - 1; j >= 0; --j) {
+            const DevState z = state_at(A.seg.p + (size_t)j * 5 * n, n);
+            ZFV_HIP(launch_fill_u64(A.slot.p, 3, 0x7ff0000000000000ULL, s));
+            ZFV_HIP(launch_dt(M, z, eos, A.slot.p, s));
+            StepCtl ctl;
+            ctl.dtmin = A.slot.p;
+            ctl.rd = 0;
+            ctl.wr = 1;
+            ctl.clr = 2;
+            ctl.cfl = ctx->prm.cfl;
+            ctl.max_dt = ctx->dmax_dt.p;
+            ctl.dt_out = A.dts.p;
+            ctl.err = ctx->err.p;
+            const int* wait = ctx->wait[ctx->wcur].p;
+            ZFV_HIP(launch_grad_pred(ctx->prm.limiter, true, kLayoutCell, kBlock, M, z, G, ctx->layer(2), wait, eos,
+                                     ctl, s));
+            ZFV_HIP(launch_fill_u64(reinterpret_cast<unsigned long long*>(A.dtb.p), 1, 0ULL, s));
+            ZFV_HIP(adj_step(ctx->prm.limiter, M, z, ctx->layer(2), G, wait, eos, A.slot.p, ctx->prm.cfl,
+                             ctx->dmax_dt.p, state_at(A.lam[cur].p, n), state_at(A.lam[1 - cur].p, n),
+                             state_at(A.lz.p, n), state_at(A.lh.p, n), lG, A.NB.p, A.NG.p, A.dtb.p, ctx->err.p, s));
+            cur = 1 - cur;
+        }
+    }
+    const DevState L = state_at(A.lam[cur].p, n);
+    double* const out[5] = {lr, lu, lv, lp, le};
+    const double* const src[5] = {L.r, L.u, L.v, L.p, L.e};
+    for (int k = 0; k < 5; ++k) ZFV_HIP(hipMemcpyAsync(out[k], src[k], n * sizeof(double), hipMemcpyDeviceToHost, s));
+    // leave the solver at the forward run's final state
+    r = copy_state(ctx, ctx->layer(ctx->cur), state_at(A.final_z.p, n));
+    if (r) return r;
+    ctx->dt_valid = false;
+    ZFV_HIP(hipStreamSynchronize(s));
+    int e = 0;
+    ZFV_HIP(hipMemcpy(&e, ctx->err.p, sizeof(int), hipMemcpyDeviceToHost));
+    if (e & 4) return fail(ZFV_ERR_UNSUPPORTED, "adjoint: a cell was isolated (bad state) during the run");
+    return read_err(ctx);
 }
 
 int zfv_limiter_from_name(const char* orig) {

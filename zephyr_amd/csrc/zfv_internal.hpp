@@ -125,6 +125,14 @@ hipError_t launch_corr(int flux, int layout, int block, const DevMesh& M, const 
                        const Eos& eos, const StepCtl& ctl, hipStream_t s);
 hipError_t launch_totals(const DevMesh& M, const DevState& z, double* partial, int nblocks, hipStream_t s);
 
+// Discrete adjoint of the MUSCL + HLLC step (zfv_adjoint.hip).
+hipError_t adj_step(int limiter, const DevMesh& M, const DevState& z, const DevState& half, const DevGrad& G,
+                    const int* wait, const Eos& eos, const unsigned long long* slot, double cfl,
+                    const double* max_dt, const DevState& lam_in, const DevState& lam_out, const DevState& lz,
+                    const DevState& lh, const DevGrad& lG, double* NB, double* NG, double* dtb, int* err,
+                    hipStream_t s);
+hipError_t adj_seed(const DevMesh& M, const DevState& z, const double* win, const DevState& lam, hipStream_t s);
+
 constexpr int kBlock = 256;
 
 }  // namespace zfv

@@ -239,6 +239,79 @@ k_flux1(DevMesh M, DevState in, DevState out, Eos eos, StepCtl ctl) {
     block_min_atomic(&ctl.dtmin[3 * tl.pid + ctl.wr], cand);
 }
 
+// LSM data of one slot for the one-thread-per-cell gradient kernel:
+// differences dF of {rho, u, v, p}, offset dr, its normal part drn, weight,
+// and which rows of the LSM system the slot feeds (a canonical L/R normal
+// has drn.y = 0, B/T drn.x = 0).
+struct SlotL {
+    double dF[4];
+    double drx, dry, dnx, dny, w;
+    bool xp, yp;
+};
+
+__device__ __forceinline__ SlotL slot_load(const DevMesh& M, const DevState& in, int c, int s, int nib, const Z& zc,
+                                           double ccx, double ccy) {
+    SlotL L;
+    const Frame fr = slot_frame(M, c, s, nib);
+    Z zn;
+    lsm_neib(M, in, c, s, nib & 7, fr, zc, ccx, ccy, zn, L.drx, L.dry);
+    lsm_drn(fr, L.drx, L.dry, L.dnx, L.dny);
+    L.w = M.w[s * M.stride + c];
+    L.dF[0] = zn.r - zc.r;
+    L.dF[1] = zn.u - zc.u;
+    L.dF[2] = zn.v - zc.v;
+    L.dF[3] = zn.p - zc.p;
+    L.xp = fr.gen || fr.side < 2;
+    L.yp = fr.gen || fr.side >= 2;
+    return L;
+}
+
+// LSM pass-1 terms: A += w drn dr^T, F += w dU drn (gradient.h:251-266).
+__device__ __forceinline__ void lsm_add(const SlotL& L, double& A00, double& A01, double& A10, double& A11,
+                                        double* Fx, double* Fy) {
+    if (L.xp) {
+        const double wd = L.w * L.dnx;
+        A00 = A00 + wd * L.drx;
+        A01 = A01 + wd * L.dry;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) Fx[k] = Fx[k] + (L.w * L.dF[k]) * L.dnx;
+    }
+    if (L.yp) {
+        const double wd = L.w * L.dny;
+        A10 = A10 + wd * L.drx;
+        A11 = A11 + wd * L.dry;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) Fy[k] = Fy[k] + (L.w * L.dF[k]) * L.dny;
+    }
+}
+
+// Limiting terms (gradient.h:412-474): limiter(2 du_n - dF, dF) dF.
+template <int LIM>
+__device__ __forceinline__ void lim_add(const SlotL& L, const double* gx, const double* gy, double* Lx, double* Ly) {
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const double du_n = gx[k] * L.drx + gy[k] * L.dry;
+        const double dFm = 2.0 * du_n - L.dF[k];
+        const double lim = limiter2<LIM>(dFm, L.dF[k]) * L.dF[k];
+        if (L.xp) Lx[k] = Lx[k] + (L.w * lim) * L.dnx;
+        if (L.yp) Ly[k] = Ly[k] + (L.w * lim) * L.dny;
+    }
+}
+
+// Predictor flux term of slot s, flux * area (sm_fluid.cpp:176-253).
+__device__ __forceinline__ Q pred_face(const DevMesh& M, int c, int s, int nib, const Z& zc, double ccx, double ccy,
+                                       const double* gx, const double* gy, const Eos& eos) {
+    const int so = s * M.stride + c;
+    const double dx = M.fcx[so] - ccx, dy = M.fcy[so] - ccy, a = M.area[so];
+    const Frame fr = slot_frame(M, c, s, nib);
+    Z zm = extrap(zc, gx, gy, dx, dy);
+    zm.e = eos_e(eos, zm.r, zm.p);
+    if (is_bad(zm, eos)) zm = zc;
+    Q f = phys_flux(frame_local(fr, zm));
+    f = frame_global(fr, f);
+    return Q{f.r * a, f.mx * a, f.my * a, f.E * a};
+}
+
 // SmFluid::compute_grad (sm_fluid.cpp:116-125): gradient::LSM
 // (gradient.h:226-282) then gradient::limiting (gradient.h:412-474) on
 // {rho, u, v, p}, fused with SmFluid::fluxes_stage1 (sm_fluid.cpp:176-253),
@@ -266,33 +339,26 @@ k_grad_pred(DevMesh M, DevState in, DevGrad G, DevState half,
     const Z zc = load_z(in, c);
     const double ccx = M.cx[c], ccy = M.cy[c];
     const int code = M.code[c];
+    const bool sub = (code >> 16) != 0;   // some second sub-face is defined
 
+    // The four main slots are loaded once, in one batch, and kept for the
+    // limiting pass; sub-face slots (hanging faces) follow in slot order.
+    SlotL S[4];
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+        const int nib = (code >> (4 * s)) & 15;
+        S[s] = (nib & 7) != SC_UNDEF ? slot_load(M, in, c, s, nib, zc, ccx, ccy) : SlotL{};
+    }
     double A00 = 0.0, A01 = 0.0, A10 = 0.0, A11 = 0.0;
     double Fx[4] = {0.0, 0.0, 0.0, 0.0}, Fy[4] = {0.0, 0.0, 0.0, 0.0};
-    for (int s = 0; s < 8; ++s) {
-        const int nib = (code >> (4 * s)) & 15;
-        const int t = nib & 7;
-        if (t == SC_UNDEF) continue;
-        const Frame fr = slot_frame(M, c, s, nib);
-        Z zn;
-        double drx, dry, dnx, dny;
-        lsm_neib(M, in, c, s, t, fr, zc, ccx, ccy, zn, drx, dry);
-        lsm_drn(fr, drx, dry, dnx, dny);
-        const double w = M.w[s * M.stride + c];
-        const double dF[4] = {zn.r - zc.r, zn.u - zc.u, zn.v - zc.v, zn.p - zc.p};
-        if (fr.gen || fr.side < 2) {
-            const double wd = w * dnx;
-            A00 = A00 + wd * drx;
-            A01 = A01 + wd * dry;
 #pragma unroll
-            for (int k = 0; k < 4; ++k) Fx[k] = Fx[k] + (w * dF[k]) * dnx;
-        }
-        if (fr.gen || fr.side >= 2) {
-            const double wd = w * dny;
-            A10 = A10 + wd * drx;
-            A11 = A11 + wd * dry;
-#pragma unroll
-            for (int k = 0; k < 4; ++k) Fy[k] = Fy[k] + (w * dF[k]) * dny;
+    for (int s = 0; s < 4; ++s)
+        if (((code >> (4 * s)) & 7) != SC_UNDEF) lsm_add(S[s], A00, A01, A10, A11, Fx, Fy);
+    if (sub) {
+        for (int s = 4; s < 8; ++s) {
+            const int nib = (code >> (4 * s)) & 15;
+            if ((nib & 7) == SC_UNDEF) continue;
+            lsm_add(slot_load(M, in, c, s, nib, zc, ccx, ccy), A00, A01, A10, A11, Fx, Fy);
         }
     }
     if (fabs(A11) < 1e-8) A11 = 1.0;   // A(2,2) = 0 -> 1 always holds in 2D (gradient.h:268-273)
@@ -313,24 +379,14 @@ k_grad_pred(DevMesh M, DevState in, DevGrad G, DevState half,
 
     // limiting: limiter(2 du_n - dF_p, dF_p) * dF_p re-solved with the same A
     double Lx[4] = {0.0, 0.0, 0.0, 0.0}, Ly[4] = {0.0, 0.0, 0.0, 0.0};
-    for (int s = 0; s < 8; ++s) {
-        const int nib = (code >> (4 * s)) & 15;
-        const int t = nib & 7;
-        if (t == SC_UNDEF) continue;
-        const Frame fr = slot_frame(M, c, s, nib);
-        Z zn;
-        double drx, dry, dnx, dny;
-        lsm_neib(M, in, c, s, t, fr, zc, ccx, ccy, zn, drx, dry);
-        lsm_drn(fr, drx, dry, dnx, dny);
-        const double w = M.w[s * M.stride + c];
-        const double dFp[4] = {zn.r - zc.r, zn.u - zc.u, zn.v - zc.v, zn.p - zc.p};
 #pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            const double du_n = gx[k] * drx + gy[k] * dry;
-            const double dFm = 2.0 * du_n - dFp[k];
-            const double lim = limiter2<LIM>(dFm, dFp[k]) * dFp[k];
-            if (fr.gen || fr.side < 2) Lx[k] = Lx[k] + (w * lim) * dnx;
-            if (fr.gen || fr.side >= 2) Ly[k] = Ly[k] + (w * lim) * dny;
+    for (int s = 0; s < 4; ++s)
+        if (((code >> (4 * s)) & 7) != SC_UNDEF) lim_add<LIM>(S[s], gx, gy, Lx, Ly);
+    if (sub) {
+        for (int s = 4; s < 8; ++s) {
+            const int nib = (code >> (4 * s)) & 15;
+            if ((nib & 7) == SC_UNDEF) continue;
+            lim_add<LIM>(slot_load(M, in, c, s, nib, zc, ccx, ccy), gx, gy, Lx, Ly);
         }
     }
 #pragma unroll
@@ -349,19 +405,11 @@ k_grad_pred(DevMesh M, DevState in, DevGrad G, DevState half,
         for (int s = 0; s < 8; ++s) {
             const int nib = (code >> (4 * s)) & 15;
             if ((nib & 7) == SC_UNDEF) continue;
-            const Frame fr = slot_frame(M, c, s, nib);
-            const double dx = M.fcx[s * M.stride + c] - ccx;
-            const double dy = M.fcy[s * M.stride + c] - ccy;
-            Z zm = extrap(zc, gx, gy, dx, dy);
-            zm.e = eos_e(eos, zm.r, zm.p);
-            if (is_bad(zm, eos)) zm = zc;
-            Q f = phys_flux(frame_local(fr, zm));
-            f = frame_global(fr, f);
-            const double a = M.area[s * M.stride + c];
-            F0 = F0 + f.r * a;
-            F1 = F1 + f.mx * a;
-            F2 = F2 + f.my * a;
-            F3 = F3 + f.E * a;
+            const Q f = pred_face(M, c, s, nib, zc, ccx, ccy, gx, gy, eos);
+            F0 = F0 + f.r;
+            F1 = F1 + f.mx;
+            F2 = F2 + f.my;
+            F3 = F3 + f.E;
         }
         Q q = to_q(zc);
         const double coef = 0.5 * dt / M.vol[c];
