@@ -59,6 +59,15 @@ WORKLOADS = {
     "quadrants200_first_order": dict(
         kind="uniform", nx=200, ny=200, acc=1, limiter="MC",
         desc="200x200 mesh, first-order HLLC, CFL 0.5, seeded four-quadrant Riemann data"),
+    "adjoint1024_muscl": dict(
+        kind="adjoint", nx=1024, ny=1024, acc=2, limiter="MC", every=20, window=(0.4, 0.6, 0.4, 0.6),
+        fd_dirs=64, fd_eps=1e-7,
+        desc="1024x1024 uniform mesh, MUSCL-Hancock + MC + HLLC, FP64, ZOE bounds; initial rho = 1 + 0.1 g0, "
+             "p = 1 + 0.1 g1, u = 0.1 g2, v = 0.1 g3 (g_k: seeded unit Gaussian random fields, corr. length "
+             "0.05); J = integral of p^2 "
+             "over [0.4,0.6]^2 at the final step; gradient dJ/drho0 by the discrete adjoint (checkpoint every "
+             "20 steps, recompute in between), checked against central differences (eps 1e-7) on 64 seeded "
+             "directions"),
     "ensemble256_muscl": dict(
         kind="ensemble", problems=256, smin=64, smax=512, acc=2, limiter="MC",
         desc="256 independent problems of s x s cells, s ~ U{64..512} (seeded), per-problem seeded "
@@ -194,6 +203,70 @@ def run_steps(ctx, w, steps, step0, prof=None):
     return cells
 
 
+def adjoint_run(args, w, Z):
+    """Forward (with checkpoints) + backward of the discrete adjoint; FD check."""
+    from zephyr_amd import problems
+    nx, ny = w["nx"], w["ny"]
+    ctx = Z.Context()
+    ctx.mesh_rectangle(nx, ny)
+    ctx.configure(acc=2, flux="hllc", limiter=w["limiter"], cfl=0.5, gamma=1.4)
+    rho0, u0, v0, p0, e0 = problems.smooth_flow_field(nx, ny, args.seed)
+
+    def state(rho):
+        return rho, u0, v0, p0, problems.energy_rP(rho, p0, 1.4)
+
+    steps = args.steps
+    ctx.set_state(*state(rho0))
+    ctx.step(2)                       # warm-up of the step kernels
+    ctx.set_state(*state(rho0))
+    ctx.adjoint_forward(4, 2, w["window"])
+    ctx.adjoint_backward()            # reverse buffers allocated, adjoint kernels loaded
+    ctx.set_state(*state(rho0))
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    ctx.step(steps)
+    ctx.synchronize()
+    t_plain = time.perf_counter() - t0
+    ctx.set_state(*state(rho0))
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    J = ctx.adjoint_forward(steps, w["every"], w["window"])
+    t_fwd = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    lr, lu, lv, lp, le = ctx.adjoint_backward()
+    t_bwd = time.perf_counter() - t0
+    grad = lr - le * problems.energy_rP(rho0, p0, 1.4) / rho0
+    # central differences of the same forward map on seeded smooth directions
+    rel = []
+    n_dirs = w["fd_dirs"] if not args.fd_dirs else args.fd_dirs
+    eps = w["fd_eps"]
+    for k in range(n_dirs):
+        d = problems.gaussian_random_field(nx, ny, 10_000 + k, corr_len=0.05)
+        ctx.set_state(*state(rho0 + eps * d))
+        Jp = ctx.adjoint_forward(steps, steps, w["window"])
+        ctx.set_state(*state(rho0 - eps * d))
+        Jm = ctx.adjoint_forward(steps, steps, w["window"])
+        fd = (Jp - Jm) / (2 * eps)
+        ad = float(np.dot(grad, d))
+        rel.append(abs(fd - ad) / max(abs(fd), abs(ad), 1e-300))
+    n = nx * ny
+    out = {
+        "metric": "FP64 cell-updates/s (discrete adjoint of HLLC+MUSCL, reverse sweep incl. recompute)",
+        "value": n * steps / t_bwd, "unit": "cell-updates/s", "n_gpus": 1, "steps": steps, "warmup": 2,
+        "ms_per_step": t_bwd * 1e3 / steps, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic",
+        "config": {"workload": args.workload, "description": w["desc"], "cells": n, "J": J,
+                   "forward_plain_ms_per_step": t_plain * 1e3 / steps,
+                   "forward_with_checkpoints_ms_per_step": t_fwd * 1e3 / steps,
+                   "backward_ms_per_step": t_bwd * 1e3 / steps,
+                   "adjoint_over_forward_step": t_bwd / t_plain,
+                   "fd_directions": n_dirs, "fd_eps": eps,
+                   "fd_max_rel_err": max(rel) if rel else None,
+                   "fd_median_rel_err": float(np.median(rel)) if rel else None},
+    }
+    print(json.dumps(out), flush=True)
+
+
 def ensemble_run(args, w, Z):
     """Batched ensemble vs the looped per-problem path, same problems."""
     sizes = ensemble_sizes(w, args.seed)
@@ -251,7 +324,23 @@ def ensemble_run(args, w, Z):
     for c in ctxs:
         c.synchronize()
     t_loop_step = time.perf_counter() - t0
-    t_loop = min(t_loop_prob, t_loop_step)   # the faster way of looping is the baseline
+    # the same loop with every context on one shared stream (launches serialised)
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so")
+    shared = C.c_void_p()
+    if hip.hipStreamCreate(C.byref(shared)) != 0:
+        raise RuntimeError("hipStreamCreate failed")
+    for c in ctxs:
+        c.set_stream(shared.value)
+    t0 = time.perf_counter()
+    for c in ctxs:
+        c.step(args.steps)
+    ctxs[0].synchronize()
+    t_loop_one = time.perf_counter() - t0
+    for c in ctxs:
+        c.set_stream(0)
+    hip.hipStreamDestroy(shared)
+    t_loop = min(t_loop_prob, t_loop_step)   # the faster multi-stream loop is the baseline
     value = cells_per_step * args.steps / t_batch
     looped = cells_per_step * args.steps / t_loop
     phi = 2.0 + sum(2.0 * s for s in sizes) / cells_per_step   # unique faces / cells, s x s meshes
@@ -269,7 +358,9 @@ def ensemble_run(args, w, Z):
                    "cells_per_step": cells_per_step, "phi": phi,
                    "looped_value": looped, "batched_over_looped": value / looped,
                    "looped_ms_per_step": {"per_problem_runs": t_loop_prob * 1e3 / args.steps,
-                                          "interleaved_steps": t_loop_step * 1e3 / args.steps},
+                                          "interleaved_steps": t_loop_step * 1e3 / args.steps,
+                                          "one_shared_stream": t_loop_one * 1e3 / args.steps},
+                   "batched_over_looped_one_stream": t_loop_one / t_batch,
                    "launch": {str(st): b.launch(st) for st in (1, 2)},
                    "autotune_ms": tune.tolist(),
                    "kernel_ms_avg": {k: v[0] / v[1] for k, v in prof.items()}},
@@ -313,6 +404,7 @@ def main():
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--cpu-budget", type=float, default=10.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--fd-dirs", type=int, default=0, help="adjoint workload: finite-difference directions")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run device-to-device copies of the state (known bytes) for HBM counter scaling")
     args = ap.parse_args()
@@ -327,6 +419,8 @@ def main():
     w = WORKLOADS[args.workload]
     if w["kind"] == "ensemble":
         return ensemble_run(args, w, Z)
+    if w["kind"] == "adjoint":
+        return adjoint_run(args, w, Z)
     nx, ny, acc, limiter = w["nx"], w["ny"], w["acc"], w["limiter"]
     ctx = Z.Context()
     ctx.mesh_rectangle(nx, ny)

@@ -22,6 +22,7 @@ SOURCES = [
 HEADERS = [
     os.path.join(HERE, "csrc", "zfv_physics.hpp"),
     os.path.join(HERE, "csrc", "zfv_internal.hpp"),
+    os.path.join(HERE, "csrc", "zfv_adjoint_ops.hpp"),
     os.path.join(ROOT, "include", "zfv.h"),
 ]
 

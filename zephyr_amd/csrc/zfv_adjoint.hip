@@ -11,16 +11,22 @@
 //   time step   dt = min(CFL min_c h_c / (|v_c| + c_c), max_dt)
 // Branches (supersonic HLLC exits, bad-state fallbacks, limiter regions,
 // min/max selections) take the derivative of the branch the forward step
-// took.  Contributions of a cell's faces to its neighbours' inputs are
-// written per slot and gathered by the neighbour through the opposite slot
-// (on a Cartesian mesh slot s of cell c points to n and slot s ^ 1 of n
-// points back to c), so no floating-point atomics are needed except for the
-// scalar dt adjoint.
+// took.  Each corrector face is reversed once, by the cell that owns it,
+// which writes the adjoints of both face states into per-slot buffers (on a
+// Cartesian mesh slot s of cell c points to n and slot s ^ 1 of n points
+// back to c); the gradient's neighbour contributions are written per slot
+// and gathered through the opposite slot.  No floating-point atomics are
+// needed except for the scalar dt adjoint.
 #include "zfv_internal.hpp"
+#include "zfv_adjoint_ops.hpp"
 
 namespace zfv {
 
+using namespace adj;
+
 namespace {
+
+__device__ __forceinline__ Z load_zs(const DevState& s, int c) { return Z{s.r[c], s.u[c], s.v[c], s.p[c], s.e[c]}; }
 
 __device__ __forceinline__ int cell_id_a() { return (int)(blockIdx.x * blockDim.x + threadIdx.x); }
 
@@ -32,270 +38,6 @@ struct DtRef {
 };
 __device__ __forceinline__ double dt_of(const DtRef& d) {
     return smin(d.cfl * __longlong_as_double((long long)*d.slot), *d.max_dt);
-}
-
-__device__ __forceinline__ void zadd(Z& a, const Z& b) {
-    a.r += b.r;
-    a.u += b.u;
-    a.v += b.v;
-    a.p += b.p;
-    a.e += b.e;
-}
-
-// ---- reverse of the state primitives ----------------------------------------
-// to_q: q = {r, r u, r v, r (e + |v|^2 / 2)}
-__device__ __forceinline__ void to_q_rev(const Z& z, const Q& qb, Z& zb) {
-    const double k = z.e + 0.5 * vsq(z);
-    zb.r += qb.r + qb.mx * z.u + qb.my * z.v + qb.E * k;
-    zb.u += qb.mx * z.r + qb.E * z.r * z.u;
-    zb.v += qb.my * z.r + qb.E * z.r * z.v;
-    zb.e += qb.E * z.r;
-}
-
-// to_z: r = q.r, u = mx / r, v = my / r, e = E / r - |v|^2 / 2, p = p(r, e)
-__device__ __forceinline__ void to_z_rev(const Q& q, const Z& z, const Z& zb, const Eos& eos, Q& qb) {
-    double eb = zb.e + zb.p * (eos.g - 1.0) * z.r;
-    double rb = zb.r + zb.p * (eos.g - 1.0) * (z.e - eos.e0);
-    double ub = zb.u - eb * z.u;
-    double vb = zb.v - eb * z.v;
-    qb.E += eb / z.r;
-    rb -= eb * (q.E / z.r) / z.r;
-    qb.mx += ub / z.r;
-    rb -= ub * z.u / z.r;
-    qb.my += vb / z.r;
-    rb -= vb * z.v / z.r;
-    qb.r += rb;
-}
-
-// e = e0 + (p + g P0) / ((g - 1) r): adds the e adjoint into r and p
-__device__ __forceinline__ void eos_e_rev(const Z& z, double eb, const Eos& eos, Z& zb) {
-    zb.p += eb / ((eos.g - 1.0) * z.r);
-    zb.r -= eb * (z.e - eos.e0) / z.r;
-}
-
-// c = sqrt(g (p + P0) / r)
-__device__ __forceinline__ void eos_c_rev(double r, double p, double c, double cb, const Eos& eos, double& rb,
-                                          double& pb) {
-    if (!(c > 0.0)) return;
-    const double sb = cb / (2.0 * c);
-    pb += sb * eos.g / r;
-    rb -= sb * (c * c) / r;
-}
-
-// phys_flux (local frame): {r u, r u u + p, r u v, (r (e + |v|^2/2) + p) u}
-__device__ __forceinline__ void phys_flux_rev(const Z& z, const Q& fb, Z& zb) {
-    const double k = z.e + 0.5 * vsq(z);
-    const double Ei = z.r * k;
-    zb.r += fb.r * z.u + fb.mx * z.u * z.u + fb.my * z.u * z.v + fb.E * z.u * k;
-    zb.u += fb.r * z.r + fb.mx * 2.0 * z.r * z.u + fb.my * z.r * z.v + fb.E * ((Ei + z.p) + z.u * z.r * z.u);
-    zb.v += fb.my * z.r * z.u + fb.E * z.u * z.r * z.v;
-    zb.p += fb.mx + fb.E * z.u;
-    zb.e += fb.E * z.u * z.r;
-}
-
-// HLLC (flux_hllc) reverse: Lb, Rb += (dF/dL)^T Fb, (dF/dR)^T Fb.
-__device__ void hllc_rev(const Z& L, const Z& R, const Q& Fb, const Eos& eos, Z& Lb, Z& Rb) {
-    const double cL = eos_c(eos, L.r, L.p), cR = eos_c(eos, R.r, R.p);
-    const double sl1 = L.u - cL, sl2 = R.u - cR, sr1 = L.u + cL, sr2 = R.u + cR;
-    const double SL = smin(sl1, sl2), SR = smax(sr1, sr2);
-    double cLb = 0.0, cRb = 0.0;
-    if (SL >= 0.0) {
-        phys_flux_rev(L, Fb, Lb);
-        return;
-    }
-    if (SR <= 0.0) {
-        phys_flux_rev(R, Fb, Rb);
-        return;
-    }
-    const double aL = L.r * (SL - L.u), aR = R.r * (SR - R.u);
-    const double N = L.p - R.p + aR * R.u - aL * L.u, D = aR - aL;
-    const double SC = N / D;
-    const double dL = SL - SC, dR = SR - SC;
-    const double rsL = aL / dL, rsR = aR / dR;
-    const double hL = L.e + 0.5 * vsq(L), hR = R.e + 0.5 * vsq(R);
-    const double mL = SC - L.u, kL = SC + L.p / aL, mR = SC - R.u, kR = SC + R.p / aR;
-    const double EsL = hL + mL * kL, EsR = hR + mR * kR;
-    const double ruL = L.r * L.u, ruR = R.r * R.u;
-    const double EL = L.r * hL, ER = R.r * hR;
-    const double qsL[4] = {rsL, rsL * SC, rsL * L.v, rsL * EsL};
-    const double qsR[4] = {rsR, rsR * SC, rsR * R.v, rsR * EsR};
-    const double qL[4] = {L.r, ruL, L.r * L.v, EL};
-    const double qR[4] = {R.r, ruR, R.r * R.v, ER};
-    const double aSC = fabs(SC);
-    const double tb[4] = {0.5 * Fb.r, 0.5 * Fb.mx, 0.5 * Fb.my, 0.5 * Fb.E};
-    double SLb = 0.0, SRb = 0.0, SCb = 0.0, aSCb = 0.0;
-    double qsLb[4], qsRb[4], qLb[4], qRb[4], fLb[4], fRb[4];
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
-        fLb[k] = tb[k];
-        fRb[k] = tb[k];
-        SLb += tb[k] * (qsL[k] - qL[k]);
-        aSCb += tb[k] * (qsL[k] - qsR[k]);
-        SRb += tb[k] * (qsR[k] - qR[k]);
-        qsLb[k] = tb[k] * (SL + aSC);
-        qsRb[k] = tb[k] * (SR - aSC);
-        qLb[k] = -tb[k] * SL;
-        qRb[k] = -tb[k] * SR;
-    }
-    SCb += (SC < 0.0 ? -aSCb : aSCb);
-    // side fluxes and conserved vectors
-    double ruLb = fLb[0] + fLb[1] * L.u + fLb[2] * L.v + qLb[1];
-    double ruRb = fRb[0] + fRb[1] * R.u + fRb[2] * R.v + qRb[1];
-    double ELb = fLb[3] * L.u + qLb[3], ERb = fRb[3] * R.u + qRb[3];
-    Lb.u += fLb[1] * ruL + fLb[3] * (EL + L.p);
-    Rb.u += fRb[1] * ruR + fRb[3] * (ER + R.p);
-    Lb.v += fLb[2] * ruL + qLb[2] * L.r;
-    Rb.v += fRb[2] * ruR + qRb[2] * R.r;
-    Lb.p += fLb[1] + fLb[3] * L.u;
-    Rb.p += fRb[1] + fRb[3] * R.u;
-    Lb.r += qLb[0] + qLb[2] * L.v;
-    Rb.r += qRb[0] + qRb[2] * R.v;
-    double hLb = ELb * L.r, hRb = ERb * R.r;
-    Lb.r += ELb * hL;
-    Rb.r += ERb * hR;
-    Lb.r += ruLb * L.u;
-    Lb.u += ruLb * L.r;
-    Rb.r += ruRb * R.u;
-    Rb.u += ruRb * R.r;
-    // star states
-    double rsLb = qsLb[0] + qsLb[1] * SC + qsLb[2] * L.v + qsLb[3] * EsL;
-    double rsRb = qsRb[0] + qsRb[1] * SC + qsRb[2] * R.v + qsRb[3] * EsR;
-    SCb += qsLb[1] * rsL + qsRb[1] * rsR;
-    Lb.v += qsLb[2] * rsL;
-    Rb.v += qsRb[2] * rsR;
-    const double EsLb = qsLb[3] * rsL, EsRb = qsRb[3] * rsR;
-    hLb += EsLb;
-    hRb += EsRb;
-    double aLb = 0.0, aRb = 0.0;
-    {
-        const double mb = EsLb * kL, kb = EsLb * mL;
-        SCb += mb + kb;
-        Lb.u -= mb;
-        Lb.p += kb / aL;
-        aLb -= kb * L.p / (aL * aL);
-    }
-    {
-        const double mb = EsRb * kR, kb = EsRb * mR;
-        SCb += mb + kb;
-        Rb.u -= mb;
-        Rb.p += kb / aR;
-        aRb -= kb * R.p / (aR * aR);
-    }
-    Lb.e += hLb;
-    Lb.u += hLb * L.u;
-    Lb.v += hLb * L.v;
-    Rb.e += hRb;
-    Rb.u += hRb * R.u;
-    Rb.v += hRb * R.v;
-    // rs = a / (S - SC)
-    aLb += rsLb / dL;
-    {
-        const double db = -rsLb * rsL / dL;
-        SLb += db;
-        SCb -= db;
-    }
-    aRb += rsRb / dR;
-    {
-        const double db = -rsRb * rsR / dR;
-        SRb += db;
-        SCb -= db;
-    }
-    // SC = N / D
-    {
-        const double Nb = SCb / D, Db = -SCb * SC / D;
-        Lb.p += Nb;
-        Rb.p -= Nb;
-        aRb += Nb * R.u + Db;
-        Rb.u += Nb * aR;
-        aLb += -Nb * L.u - Db;
-        Lb.u -= Nb * aL;
-    }
-    // a = r (S - u)
-    Lb.r += aLb * (SL - L.u);
-    SLb += aLb * L.r;
-    Lb.u -= aLb * L.r;
-    Rb.r += aRb * (SR - R.u);
-    SRb += aRb * R.r;
-    Rb.u -= aRb * R.r;
-    // SL = min(uL - cL, uR - cR), SR = max(uL + cL, uR + cR) (smin / smax choice)
-    if (sl2 < sl1) {
-        Rb.u += SLb;
-        cRb -= SLb;
-    } else {
-        Lb.u += SLb;
-        cLb -= SLb;
-    }
-    if (sr1 < sr2) {
-        Rb.u += SRb;
-        cRb += SRb;
-    } else {
-        Lb.u += SRb;
-        cLb += SRb;
-    }
-    eos_c_rev(L.r, L.p, cL, cLb, eos, Lb.r, Lb.p);
-    eos_c_rev(R.r, R.p, cR, cRb, eos, Rb.r, Rb.p);
-}
-
-// ---- forward pieces recomputed in the reverse sweep --------------------------
-__device__ __forceinline__ Z load_zs(const DevState& s, int c) { return Z{s.r[c], s.u[c], s.v[c], s.p[c], s.e[c]}; }
-
-__device__ __forceinline__ Z extrap_a(const Z& z, const double* gx, const double* gy, double dx, double dy) {
-    Z r;
-    r.r = z.r + (gx[0] * dx + gy[0] * dy);
-    r.u = z.u + (gx[1] * dx + gy[1] * dy);
-    r.v = z.v + (gx[2] * dx + gy[2] * dy);
-    r.p = z.p + (gx[3] * dx + gy[3] * dy);
-    r.e = 0.0;
-    return r;
-}
-
-// Mirror point of a boundary / periodic slot (symm_point, canonical normal).
-__device__ __forceinline__ void mirror(int side, double ccx, double ccy, double fx, double fy, double& px,
-                                       double& py) {
-    const Frame f = frame_of(side, false, 0.0, 0.0);
-    frame_symm(f, ccx, ccy, fx, fy, px, py);
-}
-
-// Reverse of a face extrapolation z_face = extrap(z, G, dx, dy) followed by
-// e = eos_e(r, p) and the bad-state fallback to z.  zb: adjoint of the face
-// state (5); adds into the base state's adjoint (5) and gradient adjoints.
-__device__ __forceinline__ void face_state_rev(const Z& zf, bool bad, const Z& zfb, double dx, double dy,
-                                               const Eos& eos, Z& zbase_b, double* gxb, double* gyb) {
-    if (bad) {
-        zadd(zbase_b, zfb);
-        return;
-    }
-    Z t = zfb;
-    t.e = 0.0;
-    eos_e_rev(zf, zfb.e, eos, t);
-    const double tk[4] = {t.r, t.u, t.v, t.p};
-    zbase_b.r += t.r;
-    zbase_b.u += t.u;
-    zbase_b.v += t.v;
-    zbase_b.p += t.p;
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
-        gxb[k] += tk[k] * dx;
-        gyb[k] += tk[k] * dy;
-    }
-}
-
-__device__ __forceinline__ Z face_state(const Z& base, const double* gx, const double* gy, double dx, double dy,
-                                        const Eos& eos, bool& bad) {
-    Z zf = extrap_a(base, gx, gy, dx, dy);
-    zf.e = eos_e(eos, zf.r, zf.p);
-    bad = is_bad(zf, eos);
-    return bad ? base : zf;
-}
-
-// wall reflection of the normal velocity component (its own transpose)
-__device__ __forceinline__ Z reflect(const Z& z, int side, bool wall) {
-    Z r = z;
-    if (wall) {
-        if (side < 2) r.u = -z.u;
-        else r.v = -z.v;
-    }
-    return r;
 }
 
 __device__ __forceinline__ void block_sum_atomic(double* out, double v) {
@@ -314,27 +56,56 @@ __device__ __forceinline__ void block_sum_atomic(double* out, double v) {
 }  // namespace
 
 // ---- adjoint kernels --------------------------------------------------------
-// Buffers: NB [13][4][stride] per-slot contributions of a cell's corrector
-// faces to the neighbour's half state (5) and gradient (8); NG [4][4][stride]
-// per-slot contributions of a cell's gradient to the neighbour's state.
+// Buffers: W [4][stride] the corrector's flux adjoint per unit dt and area
+// (qb / V); ZB [5][4][stride] per slot, the adjoint of the face state a cell
+// extrapolates to that slot; NG [4][4][stride] per-slot contributions of a
+// cell's gradient to the neighbour's state.
 
-// 1. Corrector reverse, own part.  lam: adjoint of z^{n+1}; writes the
-// adjoint of z through Q(z), of the own half state and gradient, the
-// neighbour contributions, and adds dJ/ddt.
-__global__ void __launch_bounds__(256) k_adj_corr(DevMesh M, DevState init, DevState half, DevGrad G,
-                                                  const int* __restrict__ wait_in, Eos eos, DtRef dref,
-                                                  DevState lam, DevState lz, DevState lh, DevGrad lG,
-                                                  double* __restrict__ NB, double* dtb, int* err) {
+// 1. Corrector reverse through the conservative update, per cell:
+//   z' = PState(q),  q = Q(z) - dt / V sum_s a_s F_s
+// lam (adjoint of z') gives qb; lz = (dQ/dz)^T qb; W = qb / V.  q is rebuilt
+// from the stored next state (q = Q(z') up to rounding).  Clears dtb for the
+// faces' and the predictor's dt-adjoint sums.
+__global__ void __launch_bounds__(256) k_adj_qb(DevMesh M, DevState init, DevState next, const int* __restrict__ wait_in,
+                                                Eos eos, DevState lam, DevState lz, double* __restrict__ W,
+                                                double* dtb, int* err) {
     const int c = cell_id_a();
-    const double dt = dt_of(dref);
+    if (c == 0) *dtb = 0.0;
+    if (c >= M.n) return;
+    if (wait_in[c] > 0) atomicOr(err, kErrRobust);
+    const int st = M.stride;
+    const Z zn = load_zs(next, c), zc = load_zs(init, c), lb = load_zs(lam, c);
+    const Q q = to_q(zn);
+    Q qb{0, 0, 0, 0};
+    to_z_rev(q, zn, lb, eos, qb);
+    Z zcb{0, 0, 0, 0, 0};
+    to_q_rev(zc, qb, zcb);
+    lz.r[c] = zcb.r; lz.u[c] = zcb.u; lz.v[c] = zcb.v; lz.p[c] = zcb.p; lz.e[c] = zcb.e;
+    const double V = M.vol[c];
+    W[c] = qb.r / V;
+    W[st + c] = qb.mx / V;
+    W[2 * st + c] = qb.my / V;
+    W[3 * st + c] = qb.E / V;
+}
+
+// 2. Corrector faces, once per face: a cell owns its boundary faces and the
+// inner / periodic faces of slots 1 and 3 (right, top); the faces of slots 0
+// and 2 belong to the neighbour there.  The flux F of a face enters the
+// left cell with -dt a / V_c and the right cell with +dt a / V_n, so its
+// adjoint is Fb = dt a (W_n - W_c) (boundary: -dt a W_c), and the face adds
+// F . Fb / dt to the time step's adjoint.  The HLLC reverse gives the
+// adjoints of both face states: the own side goes to ZB[s][c], the
+// neighbour's side to ZB[s ^ 1][n] (on a wall / copy face the reflected
+// state's adjoint folds back into the own side).
+__global__ void __launch_bounds__(256) k_adj_face(DevMesh M, DevState half, DevGrad G, Eos eos, DtRef dref,
+                                                  const double* __restrict__ W, double* __restrict__ ZB, double* dtb) {
+    const int c = cell_id_a();
     double dtb_loc = 0.0;
     if (c < M.n) {
+        const double dt = dt_of(dref);
         const int st = M.stride;
-        const Z zc = load_zs(init, c), zch = load_zs(half, c);
-        const Z lb = load_zs(lam, c);
-        Z zcb{0, 0, 0, 0, 0}, zchb{0, 0, 0, 0, 0};
-        double gxb[4] = {0, 0, 0, 0}, gyb[4] = {0, 0, 0, 0};
-        if (wait_in[c] > 0) atomicOr(err, 4);
+        const int code = M.code[c];
+        const Z zch = load_zs(half, c);
         const double ccx = M.cx[c], ccy = M.cy[c];
         double gx[4], gy[4];
 #pragma unroll
@@ -342,20 +113,23 @@ __global__ void __launch_bounds__(256) k_adj_corr(DevMesh M, DevState init, DevS
             gx[k] = G.dx[k * st + c];
             gy[k] = G.dy[k * st + c];
         }
-        // forward face fluxes (needed for q and dt adjoint)
-        const int code = M.code[c];
-        Q Fs[4];
-        double F0 = 0.0, F1 = 0.0, F2 = 0.0, F3 = 0.0;
+        const double wc0 = W[c], wc1 = W[st + c], wc2 = W[2 * st + c], wc3 = W[3 * st + c];
+#pragma unroll 1
         for (int s = 0; s < 4; ++s) {
             const int t = (code >> (4 * s)) & 7;
+            const bool bound = t >= SC_WALL;
+            if (!bound && (s & 1) == 0) continue;
             const int so = s * st + c;
             const int nb = M.adj[so];
             const double fx = M.fcx[so], fy = M.fcy[so], a = M.area[so];
             bool bm, bp = false;
             const Z zm = face_state(zch, gx, gy, fx - ccx, fy - ccy, eos, bm);
             Z zp;
-            if (t >= SC_WALL) {
+            Q Fb;
+            if (bound) {
                 zp = reflect(zm, s, t == SC_WALL);
+                const double k = -dt * a;
+                Fb = Q{k * wc0, k * wc1, k * wc2, k * wc3};
             } else {
                 double px = M.cx[nb], py = M.cy[nb];
                 if (t != SC_INNER) mirror(s, ccx, ccy, fx, fy, px, py);
@@ -366,147 +140,81 @@ __global__ void __launch_bounds__(256) k_adj_corr(DevMesh M, DevState init, DevS
                     ngy[k] = G.dy[k * st + nb];
                 }
                 zp = face_state(load_zs(half, nb), ngx, ngy, fx - px, fy - py, eos, bp);
+                const double k = dt * a;
+                Fb = Q{k * (W[nb] - wc0), k * (W[st + nb] - wc1), k * (W[2 * st + nb] - wc2),
+                       k * (W[3 * st + nb] - wc3)};
             }
-            bool nanf = false;
-            Q f = flux_hllc(to_local(zm, s), to_local(zp, s), eos, nanf);
-            f = flux_to_global(f, s);
-            Fs[s] = Q{f.r * a, f.mx * a, f.my * a, f.E * a};
-            F0 = F0 + Fs[s].r;
-            F1 = F1 + Fs[s].mx;
-            F2 = F2 + Fs[s].my;
-            F3 = F3 + Fs[s].E;
-        }
-        Q q = to_q(zc);
-        const double coef = dt / M.vol[c];
-        q.r = q.r - coef * F0;
-        q.mx = q.mx - coef * F1;
-        q.my = q.my - coef * F2;
-        q.E = q.E - coef * F3;
-        const Z zn = to_z(q, eos);
-        if (is_bad(zn, eos)) {   // the cell is isolated from here on: not differentiated
-            atomicOr(err, 4);
-            zadd(zcb, lb);
-            for (int s = 0; s < 4; ++s)
-                for (int k = 0; k < 13; ++k) NB[(k * 4 + s) * st + c] = 0.0;
-        } else {
-            Q qb{0, 0, 0, 0};
-            to_z_rev(q, zn, lb, eos, qb);
-            to_q_rev(zc, qb, zcb);
-            dtb_loc = -(qb.r * F0 + qb.mx * F1 + qb.my * F2 + qb.E * F3) / M.vol[c];
-            const Q Fb{-coef * qb.r, -coef * qb.mx, -coef * qb.my, -coef * qb.E};
-            for (int s = 0; s < 4; ++s) {
-                const int t = (code >> (4 * s)) & 7;
-                const int so = s * st + c;
-                const int nb = M.adj[so];
-                const double fx = M.fcx[so], fy = M.fcy[so], a = M.area[so];
-                bool bm, bp = false;
-                const Z zm = face_state(zch, gx, gy, fx - ccx, fy - ccy, eos, bm);
-                Z zp, hn{0, 0, 0, 0, 0};
-                double px = 0.0, py = 0.0;
-                if (t >= SC_WALL) {
-                    zp = reflect(zm, s, t == SC_WALL);
-                } else {
-                    px = M.cx[nb];
-                    py = M.cy[nb];
-                    if (t != SC_INNER) mirror(s, ccx, ccy, fx, fy, px, py);
-                    hn = load_zs(half, nb);
-                    double ngx[4], ngy[4];
-#pragma unroll
-                    for (int k = 0; k < 4; ++k) {
-                        ngx[k] = G.dx[k * st + nb];
-                        ngy[k] = G.dy[k * st + nb];
-                    }
-                    zp = face_state(hn, ngx, ngy, fx - px, fy - py, eos, bp);
-                }
-                // flux adjoint: global -> local (transpose of flux_to_global)
-                Q fb{Fb.r * a, Fb.mx * a, Fb.my * a, Fb.E * a};
-                Q lfb = fb;
-                vel_local(s, fb.mx, fb.my, lfb.mx, lfb.my);
-                Z lmb{0, 0, 0, 0, 0}, lpb{0, 0, 0, 0, 0};
-                hllc_rev(to_local(zm, s), to_local(zp, s), lfb, eos, lmb, lpb);
-                Z zmb = lmb, zpb = lpb;
-                vel_global(s, lmb.u, lmb.v, zmb.u, zmb.v);
-                vel_global(s, lpb.u, lpb.v, zpb.u, zpb.v);
-                double nbh[13];
-#pragma unroll
-                for (int k = 0; k < 13; ++k) nbh[k] = 0.0;
-                if (t >= SC_WALL) {
-                    zadd(zmb, reflect(zpb, s, t == SC_WALL));
-                } else {
-                    Z hnb{0, 0, 0, 0, 0};
-                    double ngxb[4] = {0, 0, 0, 0}, ngyb[4] = {0, 0, 0, 0};
-                    face_state_rev(zp, bp, zpb, fx - px, fy - py, eos, hnb, ngxb, ngyb);
-                    nbh[0] = hnb.r; nbh[1] = hnb.u; nbh[2] = hnb.v; nbh[3] = hnb.p; nbh[4] = hnb.e;
-#pragma unroll
-                    for (int k = 0; k < 4; ++k) {
-                        nbh[5 + k] = ngxb[k];
-                        nbh[9 + k] = ngyb[k];
-                    }
-                }
-#pragma unroll
-                for (int k = 0; k < 13; ++k) NB[(k * 4 + s) * st + c] = nbh[k];
-                face_state_rev(zm, bm, zmb, fx - ccx, fy - ccy, eos, zchb, gxb, gyb);
+            Q lfb = Fb;
+            vel_local(s, Fb.mx, Fb.my, lfb.mx, lfb.my);
+            Z lmb{0, 0, 0, 0, 0}, lpb{0, 0, 0, 0, 0};
+            double fdot = 0.0;
+            hllc_rev(to_local(zm, s), to_local(zp, s), lfb, eos, lmb, lpb, &fdot);
+            dtb_loc += fdot / dt;
+            Z zmb = lmb, zpb = lpb;
+            vel_global(s, lmb.u, lmb.v, zmb.u, zmb.v);
+            vel_global(s, lpb.u, lpb.v, zpb.u, zpb.v);
+            if (bound) {
+                zadd(zmb, reflect(zpb, s, t == SC_WALL));
+            } else {
+                const int o = s ^ 1;
+                ZB[(0 * 4 + o) * st + nb] = zpb.r;
+                ZB[(1 * 4 + o) * st + nb] = zpb.u;
+                ZB[(2 * 4 + o) * st + nb] = zpb.v;
+                ZB[(3 * 4 + o) * st + nb] = zpb.p;
+                ZB[(4 * 4 + o) * st + nb] = zpb.e;
             }
-        }
-        lz.r[c] = zcb.r; lz.u[c] = zcb.u; lz.v[c] = zcb.v; lz.p[c] = zcb.p; lz.e[c] = zcb.e;
-        lh.r[c] = zchb.r; lh.u[c] = zchb.u; lh.v[c] = zchb.v; lh.p[c] = zchb.p; lh.e[c] = zchb.e;
-#pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            lG.dx[k * st + c] = gxb[k];
-            lG.dy[k * st + c] = gyb[k];
+            ZB[(0 * 4 + s) * st + c] = zmb.r;
+            ZB[(1 * 4 + s) * st + c] = zmb.u;
+            ZB[(2 * 4 + s) * st + c] = zmb.v;
+            ZB[(3 * 4 + s) * st + c] = zmb.p;
+            ZB[(4 * 4 + s) * st + c] = zmb.e;
         }
     }
     block_sum_atomic(dtb, dtb_loc);
 }
 
-// 2. Gather of the corrector's neighbour contributions, then the predictor
-// reverse (own cell).  Adds into lz and lG.
+// 3. Per cell: the corrector face states' reverse (half state and gradient
+// adjoints from ZB), then the predictor reverse
+//   h = PState(Q(z) - dt / (2V) sum_s a_s Flux(zm_s(z, G)))
+// adding into lz and writing lG.
 __global__ void __launch_bounds__(256) k_adj_pred(DevMesh M, DevState init, DevState half, DevGrad G,
                                                   const int* __restrict__ wait_in, Eos eos, DtRef dref,
-                                                  DevState lz, DevState lh, DevGrad lG, const double* __restrict__ NB,
+                                                  const double* __restrict__ ZB, DevState lz, DevGrad lG,
                                                   double* dtb) {
     const int c = cell_id_a();
-    const double dt = dt_of(dref);
     double dtb_loc = 0.0;
     if (c < M.n) {
+        const double dt = dt_of(dref);
         const int st = M.stride;
-        const int code = M.code[c];
-        Z hb = load_zs(lh, c);
-        double gxb[4], gyb[4];
+        const double ccx = M.cx[c], ccy = M.cy[c];
+        double gx[4], gy[4];
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
-            gxb[k] = lG.dx[k * st + c];
-            gyb[k] = lG.dy[k * st + c];
+            gx[k] = G.dx[k * st + c];
+            gy[k] = G.dy[k * st + c];
         }
-        for (int s = 0; s < 4; ++s) {
-            const int t = (code >> (4 * s)) & 7;
-            if (t != SC_INNER && t != SC_PERIODIC) continue;
-            const int nb = M.adj[s * st + c], o = s ^ 1;
-            hb.r += NB[(0 * 4 + o) * st + nb];
-            hb.u += NB[(1 * 4 + o) * st + nb];
-            hb.v += NB[(2 * 4 + o) * st + nb];
-            hb.p += NB[(3 * 4 + o) * st + nb];
-            hb.e += NB[(4 * 4 + o) * st + nb];
-#pragma unroll
-            for (int k = 0; k < 4; ++k) {
-                gxb[k] += NB[((5 + k) * 4 + o) * st + nb];
-                gyb[k] += NB[((9 + k) * 4 + o) * st + nb];
+        Z hb{0, 0, 0, 0, 0};
+        double gxb[4] = {0, 0, 0, 0}, gyb[4] = {0, 0, 0, 0};
+        {
+            const Z zch = load_zs(half, c);
+#pragma unroll 1
+            for (int s = 0; s < 4; ++s) {
+                const int so = s * st + c;
+                const double dx = M.fcx[so] - ccx, dy = M.fcy[so] - ccy;
+                bool bm;
+                const Z zm = face_state(zch, gx, gy, dx, dy, eos, bm);
+                const Z zmb{ZB[(0 * 4 + s) * st + c], ZB[(1 * 4 + s) * st + c], ZB[(2 * 4 + s) * st + c],
+                            ZB[(3 * 4 + s) * st + c], ZB[(4 * 4 + s) * st + c]};
+                face_state_rev(zm, bm, zmb, dx, dy, eos, hb, gxb, gyb);
             }
         }
-        // predictor reverse: h = PState(Q(z) - dt/(2V) sum Flux(zm) S)
         const Z zc = load_zs(init, c);
         Z zcb = load_zs(lz, c);
         if (wait_in[c] > 0) {
             zadd(zcb, hb);
         } else {
-            const double ccx = M.cx[c], ccy = M.cy[c];
-            double gx[4], gy[4];
-#pragma unroll
-            for (int k = 0; k < 4; ++k) {
-                gx[k] = G.dx[k * st + c];
-                gy[k] = G.dy[k * st + c];
-            }
             double F0 = 0.0, F1 = 0.0, F2 = 0.0, F3 = 0.0;
+#pragma unroll 1
             for (int s = 0; s < 4; ++s) {
                 const int so = s * st + c;
                 bool bm;
@@ -532,6 +240,7 @@ __global__ void __launch_bounds__(256) k_adj_pred(DevMesh M, DevState init, DevS
                 to_z_rev(q, zh, hb, eos, qb);
                 to_q_rev(zc, qb, zcb);
                 dtb_loc = -0.5 * (qb.r * F0 + qb.mx * F1 + qb.my * F2 + qb.E * F3) / M.vol[c];
+#pragma unroll 1
                 for (int s = 0; s < 4; ++s) {
                     const int so = s * st + c;
                     const double dx = M.fcx[so] - ccx, dy = M.fcy[so] - ccy, a = M.area[so];
@@ -559,55 +268,7 @@ __global__ void __launch_bounds__(256) k_adj_pred(DevMesh M, DevState init, DevS
     block_sum_atomic(dtb, dtb_loc);
 }
 
-// Limiter psi(a, b) = phi(a, b) b and its partial derivatives (MC, minmod).
-template <int LIM>
-__device__ __forceinline__ void psi_d(double a, double b, double& pa, double& pb) {
-    if (LIM == 0) {
-        pa = 0.0;
-        pb = 0.0;
-        return;
-    }
-    if (LIM == 1) {
-        pa = 0.0;
-        pb = 1.0;
-        return;
-    }
-    double phi, dphi, r;
-    if (LIM == 2) {   // minmod: ab <= 0 -> 0, else smin(a / b, 1)
-        if (a * b <= 0.0) {
-            pa = 0.0;
-            pb = 0.0;
-            return;
-        }
-        r = a / b;
-        const bool one = 1.0 < r;
-        phi = one ? 1.0 : r;
-        dphi = one ? 0.0 : 1.0;
-    } else {   // MC: isnan(r) ? 0 : smax(0, smin(2 r, smin((1 + r) / 2, 2)))
-        r = a / b;
-        if (isnan(r)) {
-            pa = 0.0;
-            pb = 0.0;
-            return;
-        }
-        const double half = 0.5 * (1.0 + r);
-        const bool sat = 2.0 < half;
-        const double inner = sat ? 2.0 : half;
-        const bool lin = !(inner < 2.0 * r);
-        const double mid = lin ? 2.0 * r : inner;
-        if (!(0.0 < mid)) {
-            pa = 0.0;
-            pb = 0.0;
-            return;
-        }
-        phi = mid;
-        dphi = lin ? 2.0 : (sat ? 0.0 : 0.5);
-    }
-    pa = dphi;
-    pb = (dphi == 0.0) ? phi : phi - dphi * r;
-}
-
-// 3. Gradient reverse (LSM + limiting, gradient.h:226-282, 412-474), own part:
+// 4. Gradient reverse (LSM + limiting, gradient.h:226-282, 412-474), own part:
 // adds the own-state adjoint into lz and writes the neighbour contributions.
 template <int LIM>
 __global__ void __launch_bounds__(256) k_adj_grad(DevMesh M, DevState init, DevGrad lG, DevState lz,
@@ -728,7 +389,7 @@ __global__ void __launch_bounds__(256) k_adj_grad(DevMesh M, DevState init, DevG
     lz.p[c] += zcb[3];
 }
 
-// 4. Gather of the gradient's neighbour contributions and the dt reverse:
+// 5. Gather of the gradient's neighbour contributions and the dt reverse:
 // dt = min(CFL tau_min, max_dt), tau_c = h_c / (|v_c| + c_c); the cells
 // whose tau equals the minimum receive CFL * dJ/ddt d tau / dz.
 __global__ void __launch_bounds__(256) k_adj_final(DevMesh M, DevState init, Eos eos, const double* __restrict__ NG,
@@ -790,15 +451,14 @@ __global__ void __launch_bounds__(256) k_adj_seed(DevMesh M, DevState z, double 
 // ---- launchers ----------------------------------------------------------------
 static inline dim3 g256(int n) { return dim3((n + 255) / 256); }
 
-hipError_t adj_step(int limiter, const DevMesh& M, const DevState& z, const DevState& half, const DevGrad& G,
-                    const int* wait, const Eos& eos, const unsigned long long* slot, double cfl,
+hipError_t adj_step(int limiter, const DevMesh& M, const DevState& z, const DevState& znext, const DevState& half,
+                    const DevGrad& G, const int* wait, const Eos& eos, const unsigned long long* slot, double cfl,
                     const double* max_dt, const DevState& lam_in, const DevState& lam_out, const DevState& lz,
-                    const DevState& lh, const DevGrad& lG, double* NB, double* NG, double* dtb, int* err,
-                    hipStream_t s) {
+                    double* W, const DevGrad& lG, double* ZB, double* NG, double* dtb, int* err, hipStream_t s) {
     const DtRef d{slot, cfl, max_dt};
-    hipLaunchKernelGGL(k_adj_corr, g256(M.n), dim3(256), 0, s, M, z, half, G, wait, eos, d, lam_in, lz, lh, lG, NB,
-                       dtb, err);
-    hipLaunchKernelGGL(k_adj_pred, g256(M.n), dim3(256), 0, s, M, z, half, G, wait, eos, d, lz, lh, lG, NB, dtb);
+    hipLaunchKernelGGL(k_adj_qb, g256(M.n), dim3(256), 0, s, M, z, znext, wait, eos, lam_in, lz, W, dtb, err);
+    hipLaunchKernelGGL(k_adj_face, g256(M.n), dim3(256), 0, s, M, half, G, eos, d, W, ZB, dtb);
+    hipLaunchKernelGGL(k_adj_pred, g256(M.n), dim3(256), 0, s, M, z, half, G, wait, eos, d, ZB, lz, lG, dtb);
     switch (limiter) {
         case 0: hipLaunchKernelGGL(k_adj_grad<0>, g256(M.n), dim3(256), 0, s, M, z, lG, lz, NG); break;
         case 1: hipLaunchKernelGGL(k_adj_grad<1>, g256(M.n), dim3(256), 0, s, M, z, lG, lz, NG); break;

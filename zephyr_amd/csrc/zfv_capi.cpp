@@ -34,6 +34,25 @@ template <class T>
 struct DBuf {
     T* p = nullptr;
     size_t n = 0;
+    DBuf() = default;
+    // Owns its allocation: movable (containers of buffers may reallocate),
+    // never copied.
+    DBuf(const DBuf&) = delete;
+    DBuf& operator=(const DBuf&) = delete;
+    DBuf(DBuf&& o) noexcept : p(o.p), n(o.n) {
+        o.p = nullptr;
+        o.n = 0;
+    }
+    DBuf& operator=(DBuf&& o) noexcept {
+        if (this != &o) {
+            release();
+            p = o.p;
+            n = o.n;
+            o.p = nullptr;
+            o.n = 0;
+        }
+        return *this;
+    }
     ~DBuf() { release(); }
     void release() {
         if (p) (void)hipFree(p);
@@ -132,9 +151,10 @@ struct zfv_ctx {
         double win[4] = {0, 0, 0, 0};
         bool ready = false;
         std::vector<DBuf<double>> ckpt;   // [5][n] state every `every` steps
-        DBuf<double> seg, final_z;         // segment states [every][5][n], final state [5][n]
-        DBuf<double> lam[2], lz, lh, lG, NB, NG, dtb, dts;
-        DBuf<unsigned long long> slot;
+        DBuf<double> seg, final_z;         // segment states [every][5][n] (entry 0 unused), final state [5][n]
+        DBuf<double> seg_g, seg_h;         // the steps' gradients [every][8][cap] and half states [every][5][n]
+        DBuf<unsigned long long> seg_dt;   // the steps' dt slots [every]
+        DBuf<double> lam[2], lz, W, lG, ZB, NG, dtb;
     } adjoint;
 
     bool profile = false;
@@ -333,7 +353,9 @@ int upload_mesh(zfv_ctx* ctx) {
 }
 
 // Error word written by the kernels: bit 0 numeric (NaN flux / Riemann
-// solve), bit 1 inconsistent mesh after an adaptation.
+// solve), bit 1 inconsistent mesh after an adaptation; bits 2 and 3 are
+// notices for the adjoint (a cell in the robust isolation mode, a cell
+// isolated by the corrector), not errors of the step.
 int read_err(zfv_ctx* ctx) {
     int e = 0;
     ZFV_HIP(hipMemcpyAsync(&e, ctx->err.p, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
@@ -342,7 +364,7 @@ int read_err(zfv_ctx* ctx) {
         ctx->have_state = false;
         return fail(ZFV_ERR_STATE, "AMR apply produced an inconsistent mesh; state invalidated");
     }
-    if (e) return fail(ZFV_ERR_NUMERIC, "numeric error: a flux or Riemann solve produced NaN / did not converge");
+    if (e & 1) return fail(ZFV_ERR_NUMERIC, "numeric error: a flux or Riemann solve produced NaN / did not converge");
     return ZFV_OK;
 }
 
@@ -979,6 +1001,26 @@ static int step_once(zfv_ctx* ctx) {
     return ZFV_OK;
 }
 
+// One MUSCL step (acc 2) of the adjoint's recompute: the step's kernels and
+// dt-slot rotation on explicit buffers (input, half state, gradient,
+// output), so the recomputed segment lands where the reverse sweep reads it.
+static int recompute_step(zfv_ctx* ctx, const DevState& in, const DevState& half, const DevGrad& G,
+                          const DevState& out) {
+    const DevMesh M = ctx->dmesh();
+    const Eos eos = ctx->eos();
+    const zfv_smf_params& P = ctx->prm;
+    StepCtl ctl = make_ctl(ctx);
+    int* w_in = ctx->wait[ctx->wcur].p;
+    int* w_out = ctx->wait[1 - ctx->wcur].p;
+    ZFV_HIP(launch_grad_pred(P.limiter, true, ctx->layout[1], ctx->block[1], M, in, G, half, w_in, eos, ctl,
+                             ctx->stream));
+    ZFV_HIP(launch_corr(P.flux, ctx->layout[2], ctx->block[2], M, in, half, G, w_in, w_out, out, eos, ctl,
+                        ctx->stream));
+    ctx->wcur = 1 - ctx->wcur;
+    ctx->step_no += 1;
+    return ZFV_OK;
+}
+
 // Steps per captured graph: the dt slots rotate with period 3 and the
 // layers and wait buffers swap with period 2.
 constexpr int kGraphSteps = 6;
@@ -1253,6 +1295,15 @@ static int adjoint_supported(zfv_ctx* ctx) {
             if (s8 >= 4 && m.bnd[i] != ZFV_UNDEFINED) return fail(ZFV_ERR_UNSUPPORTED, "adjoint: adapted mesh");
             if (s8 < 4 && !(m.nx[i] == kNX[s8] && m.ny[i] == kNY[s8]))
                 return fail(ZFV_ERR_UNSUPPORTED, "adjoint: non-Cartesian normals");
+            // The reverse sweep gathers a face's contributions to the neighbour
+            // through the neighbour's opposite slot: every inner or periodic
+            // face must be seen from both sides (a periodic side needs its
+            // periodic partner side).
+            if (s8 < 4 && (m.bnd[i] == ZFV_INNER || m.bnd[i] == ZFV_PERIODIC)) {
+                const size_t o = 8 * (size_t)m.adj[i] + (s8 ^ 1);
+                if (m.adj[o] != c || m.bnd[o] != m.bnd[i])
+                    return fail(ZFV_ERR_UNSUPPORTED, "adjoint: unpaired periodic boundary");
+            }
         }
     return ZFV_OK;
 }
@@ -1264,22 +1315,24 @@ int zfv_adjoint_forward(zfv_ctx* ctx, int n_steps, int every, const double* wind
     r = adjoint_supported(ctx);
     if (r) return r;
     auto& A = ctx->adjoint;
-    const int n = ctx->hm.n, cap = ctx->cap;
+    const int n = ctx->hm.n;
     const int nck = (n_steps + every - 1) / every;
     A.ready = false;
+    // the derivative is that of the plain (non-isolating) step: no cell may
+    // start in the robust isolation mode, and none may be isolated on the way
+    r = read_err(ctx);
+    if (r) return r;
+    {
+        std::vector<int> w(n);
+        ZFV_HIP(hipMemcpy(w.data(), ctx->wait[ctx->wcur].p, n * sizeof(int), hipMemcpyDeviceToHost));
+        for (int c = 0; c < n; ++c)
+            if (w[c] > 0) return fail(ZFV_ERR_UNSUPPORTED, "adjoint: cells in the isolation mode");
+    }
+    ZFV_HIP(hipMemsetAsync(ctx->err.p, 0, sizeof(int), ctx->stream));
     A.ckpt.resize(nck);
     for (auto& b : A.ckpt)
         if (b.n != 5 * (size_t)n) ZFV_HIP(b.alloc(5 * (size_t)n));
     if (A.final_z.n != 5 * (size_t)n) ZFV_HIP(A.final_z.alloc(5 * (size_t)n));
-    if (A.seg.n != (size_t)every * 5 * n) ZFV_HIP(A.seg.alloc((size_t)every * 5 * n));
-    for (auto* b : {&A.lam[0], &A.lam[1], &A.lz, &A.lh})
-        if (b->n != 5 * (size_t)n) ZFV_HIP(b->alloc(5 * (size_t)n));
-    if (A.lG.n != 8 * (size_t)cap) ZFV_HIP(A.lG.alloc(8 * (size_t)cap));
-    if (A.NB.n != 52 * (size_t)cap) ZFV_HIP(A.NB.alloc(52 * (size_t)cap));
-    if (A.NG.n != 16 * (size_t)cap) ZFV_HIP(A.NG.alloc(16 * (size_t)cap));
-    if (A.dtb.n != 1) ZFV_HIP(A.dtb.alloc(1));
-    if (A.dts.n != 1) ZFV_HIP(A.dts.alloc(1));
-    if (A.slot.n != 3) ZFV_HIP(A.slot.alloc(3));
     r = ensure_dt(ctx);
     if (r) return r;
     for (int i = 0; i < n_steps; ++i) {
@@ -1298,6 +1351,11 @@ int zfv_adjoint_forward(zfv_ctx* ctx, int n_steps, int every, const double* wind
     ZFV_HIP(hipStreamSynchronize(ctx->stream));
     r = read_err(ctx);
     if (r) return r;
+    {
+        int e = 0;
+        ZFV_HIP(hipMemcpy(&e, ctx->err.p, sizeof(int), hipMemcpyDeviceToHost));
+        if (e & kErrIsolated) return fail(ZFV_ERR_UNSUPPORTED, "adjoint: a cell was isolated (bad state) during the run");
+    }
     const HostMesh& m = ctx->hm;
     double j = 0.0;
     for (int c = 0; c < n; ++c) {
@@ -1318,52 +1376,58 @@ int zfv_adjoint_backward(zfv_ctx* ctx, double* lr, double* lu, double* lv, doubl
     auto& A = ctx->adjoint;
     if (!A.ready) return fail(ZFV_ERR_STATE, "adjoint_backward: no forward run to reverse");
     if (!lr || !lu || !lv || !lp || !le) return fail(ZFV_ERR_ARG, "adjoint_backward: null output");
-    const int n = ctx->hm.n;
+    const int n = ctx->hm.n, cap = ctx->cap;
+    // reverse-sweep buffers: segment states, adjoint states, per-slot face-state adjoints
+    if (A.seg.n < (size_t)A.every * 5 * n) ZFV_HIP(A.seg.alloc((size_t)A.every * 5 * n));
+    if (A.seg_h.n < (size_t)A.every * 5 * n) ZFV_HIP(A.seg_h.alloc((size_t)A.every * 5 * n));
+    if (A.seg_g.n < (size_t)A.every * 8 * cap) ZFV_HIP(A.seg_g.alloc((size_t)A.every * 8 * cap));
+    if (A.seg_dt.n < (size_t)A.every) ZFV_HIP(A.seg_dt.alloc(A.every));
+    for (auto* b : {&A.lam[0], &A.lam[1], &A.lz})
+        if (b->n != 5 * (size_t)n) ZFV_HIP(b->alloc(5 * (size_t)n));
+    if (A.W.n != 4 * (size_t)cap) ZFV_HIP(A.W.alloc(4 * (size_t)cap));
+    if (A.lG.n != 8 * (size_t)cap) ZFV_HIP(A.lG.alloc(8 * (size_t)cap));
+    if (A.ZB.n != 20 * (size_t)cap) ZFV_HIP(A.ZB.alloc(20 * (size_t)cap));
+    if (A.NG.n != 16 * (size_t)cap) ZFV_HIP(A.NG.alloc(16 * (size_t)cap));
+    if (A.dtb.n != 1) ZFV_HIP(A.dtb.alloc(1));
     hipStream_t s = ctx->stream;
     const DevMesh M = ctx->dmesh();
     const Eos eos = ctx->eos();
-    const DevGrad G{ctx->gdx.p, ctx->gdy.p};
-    const DevGrad lG{A.lG.p, A.lG.p + 4 * (size_t)ctx->cap};
+    const DevGrad lG{A.lG.p, A.lG.p + 4 * (size_t)cap};
     ZFV_HIP(adj_seed(M, state_at(A.final_z.p, n), A.win, state_at(A.lam[0].p, n), s));
     int cur = 0;
     const int nck = (int)A.ckpt.size();
+    const DevState start = ctx->layer(ctx->cur), scratch = ctx->layer(1 - ctx->cur);
+    // state entering step j of the segment (j = 0: the checkpoint, copied to the current layer)
+    auto seg_state = [&](int j) { return j == 0 ? start : state_at(A.seg.p + (size_t)j * 5 * n, n); };
     for (int seg = nck - 1; seg >= 0; --seg) {
         const int s0 = seg * A.every, len = std::min(A.every, A.n_steps - s0);
-        // recompute the segment's states from its checkpoint
-        r = copy_state(ctx, ctx->layer(ctx->cur), state_at(A.ckpt[seg].p, n));
+        r = copy_state(ctx, start, state_at(A.ckpt[seg].p, n));
         if (r) return r;
         ctx->dt_valid = false;
         r = ensure_dt(ctx);
         if (r) return r;
+        // recompute the segment: each step writes its half state, gradient and
+        // output straight into the segment buffers (the last output, which is
+        // the next checkpoint, into scratch), and its dt slot is kept
         for (int j = 0; j < len; ++j) {
-            r = copy_state(ctx, state_at(A.seg.p + (size_t)j * 5 * n, n), ctx->layer(ctx->cur));
+            const int rd = (int)(ctx->step_no % 3);
+            ZFV_HIP(hipMemcpyAsync(A.seg_dt.p + j, ctx->dtmin.p + rd, 8, hipMemcpyDeviceToDevice, s));
+            double* gj = A.seg_g.p + (size_t)j * 8 * cap;
+            r = recompute_step(ctx, seg_state(j), state_at(A.seg_h.p + (size_t)j * 5 * n, n),
+                               DevGrad{gj, gj + 4 * (size_t)cap}, j + 1 < len ? seg_state(j + 1) : scratch);
             if (r) return r;
-            if (j < len - 1) {
-                r = step_once(ctx);
-                if (r) return r;
-            }
         }
         // reverse through the segment
+        const int* wait = ctx->wait[ctx->wcur].p;   // zero: isolation is reported, not differentiated
+        const DevState seg_out = seg + 1 < nck ? state_at(A.ckpt[seg + 1].p, n) : state_at(A.final_z.p, n);
         for (int j = len - 1; j >= 0; --j) {
-            const DevState z = state_at(A.seg.p + (size_t)j * 5 * n, n);
-            ZFV_HIP(launch_fill_u64(A.slot.p, 3, 0x7ff0000000000000ULL, s));
-            ZFV_HIP(launch_dt(M, z, eos, A.slot.p, s));
-            StepCtl ctl;
-            ctl.dtmin = A.slot.p;
-            ctl.rd = 0;
-            ctl.wr = 1;
-            ctl.clr = 2;
-            ctl.cfl = ctx->prm.cfl;
-            ctl.max_dt = ctx->dmax_dt.p;
-            ctl.dt_out = A.dts.p;
-            ctl.err = ctx->err.p;
-            const int* wait = ctx->wait[ctx->wcur].p;
-            ZFV_HIP(launch_grad_pred(ctx->prm.limiter, true, kLayoutCell, kBlock, M, z, G, ctx->layer(2), wait, eos,
-                                     ctl, s));
-            ZFV_HIP(launch_fill_u64(reinterpret_cast<unsigned long long*>(A.dtb.p), 1, 0ULL, s));
-            ZFV_HIP(adj_step(ctx->prm.limiter, M, z, ctx->layer(2), G, wait, eos, A.slot.p, ctx->prm.cfl,
-                             ctx->dmax_dt.p, state_at(A.lam[cur].p, n), state_at(A.lam[1 - cur].p, n),
-                             state_at(A.lz.p, n), state_at(A.lh.p, n), lG, A.NB.p, A.NG.p, A.dtb.p, ctx->err.p, s));
+            const DevState h = state_at(A.seg_h.p + (size_t)j * 5 * n, n);
+            double* gj = A.seg_g.p + (size_t)j * 8 * cap;
+            const DevGrad Gj{gj, gj + 4 * (size_t)cap};
+            ZFV_HIP(adj_step(ctx->prm.limiter, M, seg_state(j), j + 1 < len ? seg_state(j + 1) : seg_out, h, Gj, wait,
+                             eos, A.seg_dt.p + j, ctx->prm.cfl, ctx->dmax_dt.p, state_at(A.lam[cur].p, n),
+                             state_at(A.lam[1 - cur].p, n), state_at(A.lz.p, n), A.W.p, lG, A.ZB.p, A.NG.p,
+                             A.dtb.p, ctx->err.p, s));
             cur = 1 - cur;
         }
     }
@@ -1378,7 +1442,8 @@ int zfv_adjoint_backward(zfv_ctx* ctx, double* lr, double* lu, double* lv, doubl
     ZFV_HIP(hipStreamSynchronize(s));
     int e = 0;
     ZFV_HIP(hipMemcpy(&e, ctx->err.p, sizeof(int), hipMemcpyDeviceToHost));
-    if (e & 4) return fail(ZFV_ERR_UNSUPPORTED, "adjoint: a cell was isolated (bad state) during the run");
+    if (e & (kErrRobust | kErrIsolated))
+        return fail(ZFV_ERR_UNSUPPORTED, "adjoint: a cell was isolated (bad state) during the run");
     return read_err(ctx);
 }
 

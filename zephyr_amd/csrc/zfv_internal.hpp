@@ -126,13 +126,22 @@ hipError_t launch_corr(int flux, int layout, int block, const DevMesh& M, const 
 hipError_t launch_totals(const DevMesh& M, const DevState& z, double* partial, int nblocks, hipStream_t s);
 
 // Discrete adjoint of the MUSCL + HLLC step (zfv_adjoint.hip).
-hipError_t adj_step(int limiter, const DevMesh& M, const DevState& z, const DevState& half, const DevGrad& G,
-                    const int* wait, const Eos& eos, const unsigned long long* slot, double cfl,
+// One reverse step: z the step's input, znext its output, half / G its half
+// state and gradient, slot its dt-minimum slot; lam_in = dJ/dznext ->
+// lam_out = dJ/dz.  W [4][stride], ZB [20][stride], NG [16][stride], lz and
+// lG are work buffers.
+hipError_t adj_step(int limiter, const DevMesh& M, const DevState& z, const DevState& znext, const DevState& half,
+                    const DevGrad& G, const int* wait, const Eos& eos, const unsigned long long* slot, double cfl,
                     const double* max_dt, const DevState& lam_in, const DevState& lam_out, const DevState& lz,
-                    const DevState& lh, const DevGrad& lG, double* NB, double* NG, double* dtb, int* err,
-                    hipStream_t s);
+                    double* W, const DevGrad& lG, double* ZB, double* NG, double* dtb, int* err, hipStream_t s);
 hipError_t adj_seed(const DevMesh& M, const DevState& z, const double* win, const DevState& lam, hipStream_t s);
 
 constexpr int kBlock = 256;
+
+// Error-word bits: 1 numeric, 2 inconsistent mesh (errors); kErrRobust and
+// kErrIsolated are notices for the adjoint (a cell in the robust isolation
+// mode, a cell isolated by the corrector's bad-state fallback).
+constexpr int kErrRobust = 4;
+constexpr int kErrIsolated = 8;
 
 }  // namespace zfv

@@ -22,7 +22,7 @@
 #define ZFV_WAVES_FLUX1 4
 #endif
 #ifndef ZFV_WAVES_GRAD
-#define ZFV_WAVES_GRAD 4
+#define ZFV_WAVES_GRAD 3
 #endif
 #ifndef ZFV_WAVES_CORR
 #define ZFV_WAVES_CORR 4
@@ -518,6 +518,7 @@ k_corr(DevMesh M, DevState init, DevState half, DevGrad G, const int* __restrict
         if (is_bad(zn, eos)) {
             zn = zc;
             wn = 3;
+            atomicOr(ctl.err, kErrIsolated);
         } else {
             wn = wc > 0 ? wc - 1 : wc;
         }
@@ -895,6 +896,7 @@ k_corr_q(DevMesh M, DevState init, DevState half, DevGrad G, const int* __restri
         if (is_bad(zn, eos)) {
             zn = zc;
             wn = 3;
+            atomicOr(ctl.err, kErrIsolated);
         } else {
             wn = wc > 0 ? wc - 1 : wc;
         }

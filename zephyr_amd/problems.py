@@ -79,6 +79,18 @@ def smooth_density_field(nx, ny, seed, gamma=1.4):
     return rho, z, z.copy(), p, energy_rP(rho, p, gamma)
 
 
+def smooth_flow_field(nx, ny, seed, gamma=1.4, corr_len=0.05):
+    """Adjoint workload: rho = 1 + 0.1 g0, p = 1 + 0.1 g1, u = 0.1 g2,
+    v = 0.1 g3, with g_k the seeded unit Gaussian random fields of seeds
+    seed + k.  Pressure and velocity vary, so the final pressure depends on
+    the initial density through the acoustics (with p and u constant it
+    would not: a resting contact keeps p = const)."""
+    g = [gaussian_random_field(nx, ny, seed + k, corr_len=corr_len) for k in range(4)]
+    rho = 1.0 + 0.1 * g[0]
+    p = 1.0 + 0.1 * g[1]
+    return rho, 0.1 * g[2], 0.1 * g[3], p, energy_rP(rho, p, gamma)
+
+
 def circle_blast_params(seed):
     """Seeded parameters of the AMR circle workload: centre ~U[0.3,0.7]^2
     (x then y), perturbation wavenumber k ~ U{4..16}."""
