@@ -164,7 +164,7 @@ class ChunkProfile:
     def __init__(self, ctx):
         self.ctx = ctx
         self.prev = {}
-        self.ms, self.launches, self.bytes = {}, {}, {}
+        self.ms, self.launches, self.bytes, self.cells = {}, {}, {}, {}
 
     def close_chunk(self, n, phi):
         cur = self.ctx.profile_read()   # cumulative since profile_enable
@@ -177,6 +177,7 @@ class ChunkProfile:
             self.ms[k] = self.ms.get(k, 0.0) + dms
             self.launches[k] = self.launches.get(k, 0) + dcnt
             self.bytes[k] = self.bytes.get(k, 0.0) + kb.get(k, 0.0) * n * dcnt
+            self.cells[k] = self.cells.get(k, 0) + n * dcnt
         self.prev = cur
 
 
@@ -474,12 +475,18 @@ def main():
     step_bytes = sum(prof.bytes.get(k, 0.0) for k in step_kernels)
     step_ms = sum(prof.ms.get(k, 0.0) for k in step_kernels)
 
-    traffic = None
+    # HBM bytes per launch of the dominant kernel, from the FETCH_SIZE /
+    # WRITE_SIZE counter runs summarised in profiles/traffic.json (counter
+    # KiB per dispatch, scaled by the copy-kernel calibration; per cell for
+    # the adaptive workload, whose launches vary in size)
+    traffic, traffic_detail = None, None
     tpath = os.path.join(HERE, "profiles", "traffic.json")
     if os.path.exists(tpath):
         t = json.load(open(tpath)).get(args.workload, {}).get(dom)
         if t:
-            traffic = t
+            per_launch_cells = prof.cells[dom] / prof.launches[dom]
+            traffic = t["bytes_per_cell"] * per_launch_cells
+            traffic_detail = t
 
     scheme = "MUSCL+HLLC" if acc == 2 else "first-order HLLC"
     out = {
@@ -505,7 +512,8 @@ def main():
                    "step_kernels_ms_share_of_wall": step_ms / (el * 1e3)},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "algorithmic_bytes_per_launch": alg_bytes, "avg_launch_ms": dom_ms},
+                     "algorithmic_bytes_per_launch": alg_bytes, "avg_launch_ms": dom_ms,
+                     "traffic_detail": traffic_detail},
     }
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(args.workload, args.seed, args.cpu_budget)

@@ -96,6 +96,28 @@ def test_graph_replays_equal_eager(zfv, acc):
     assert np.all(graph.batch_dt() <= 1e-4)
 
 
+def test_padding_cells_keep_uploaded_values(zfv):
+    """The step kernels never write a batch's padding cells; downloads
+    return the uploaded padding values after odd and even step counts (the
+    two state layers alternate).  A context that filled and freed device
+    memory first makes stale values likely in fresh allocations."""
+    junk = zfv.Context()
+    junk.batch_rectangles(SIZES, bounds=BOUNDS)
+    junk.configure(acc=2, flux="hllc", limiter="MC", cfl=0.5, gamma=GAMMA)
+    junk.set_state(*[np.full(junk.n, 7.5) for _ in range(5)])
+    del junk
+    ctx, begin, count = batch_context(zfv, 2, "MC")
+    pad = np.ones(ctx.n, bool)
+    for b, c in zip(begin[:-1], count):
+        pad[b:b + c] = False
+    assert pad.any()
+    up = ctx.state()
+    for k in (1, 1, 3):
+        ctx.step(k)
+        for a, b in zip(ctx.state(), up):
+            assert np.array_equal(a[pad], b[pad])
+
+
 def test_graph_on_single_problem_with_adaptation(zfv):
     """Graphs are re-captured after the mesh changes."""
     from zephyr_amd import problems

@@ -893,6 +893,13 @@ static int state_copy(zfv_ctx* ctx, const double* const* src, bool to_device, hi
         if (to_device) ZFV_HIP(hipMemcpyAsync(d, src[k], n * sizeof(double), kind, ctx->stream));
         else ZFV_HIP(hipMemcpyAsync(const_cast<double*>(src[k]), d, n * sizeof(double), kind, ctx->stream));
     }
+    // A batch's padding cells are never written by the step kernels: give
+    // both alternating layers the uploaded values, so a download returns
+    // them whatever the parity of the step count.
+    if (to_device && ctx->nseg > 1)
+        for (int k = 0; k < 5; ++k)
+            ZFV_HIP(hipMemcpyAsync(ctx->st[1 - ctx->cur][k].p, ctx->st[ctx->cur][k].p, n * sizeof(double),
+                                   hipMemcpyDeviceToDevice, ctx->stream));
     ZFV_HIP(hipStreamSynchronize(ctx->stream));
     return ZFV_OK;
 }
