@@ -61,15 +61,15 @@ WORKLOADS = {
         desc="200x200 mesh, first-order HLLC, CFL 0.5, seeded four-quadrant Riemann data"),
     "adjoint1024_muscl": dict(
         kind="adjoint", nx=1024, ny=1024, acc=2, limiter="MC", every=20, window=(0.4, 0.6, 0.4, 0.6),
-        fd_dirs=64, fd_eps=1e-7,
+        steps=200, fd_dirs=64, fd_eps=(1e-6, 1e-7),
         desc="1024x1024 uniform mesh, MUSCL-Hancock + MC + HLLC, FP64, ZOE bounds; initial rho = 1 + 0.1 g0, "
              "p = 1 + 0.1 g1, u = 0.1 g2, v = 0.1 g3 (g_k: seeded unit Gaussian random fields, corr. length "
              "0.05); J = integral of p^2 "
              "over [0.4,0.6]^2 at the final step; gradient dJ/drho0 by the discrete adjoint (checkpoint every "
-             "20 steps, recompute in between), checked against central differences (eps 1e-7) on 64 seeded "
+             "20 steps, recompute in between), checked against central differences (eps 1e-6 and 1e-7) on 64 seeded "
              "directions"),
     "ensemble256_muscl": dict(
-        kind="ensemble", problems=256, smin=64, smax=512, acc=2, limiter="MC",
+        kind="ensemble", problems=256, smin=64, smax=512, acc=2, limiter="MC", steps=200,
         desc="256 independent problems of s x s cells, s ~ U{64..512} (seeded), per-problem seeded "
              "four-quadrant Riemann data (seed + problem index), MUSCL-Hancock + MC + HLLC, FP64, CFL 0.5, "
              "per-problem dt; one context, one launch per stage for all problems, captured in HIP graphs, "
@@ -237,19 +237,25 @@ def adjoint_run(args, w, Z):
     lr, lu, lv, lp, le = ctx.adjoint_backward()
     t_bwd = time.perf_counter() - t0
     grad = lr - le * problems.energy_rP(rho0, p0, 1.4) / rho0
-    # central differences of the same forward map on seeded smooth directions
-    rel = []
+    # central differences of the same forward map on seeded smooth directions,
+    # at the configured eps and at eps / 10 (fewer limiter / wave-speed
+    # switches crossed inside +-eps)
     n_dirs = w["fd_dirs"] if not args.fd_dirs else args.fd_dirs
-    eps = w["fd_eps"]
+    fds = {e: [] for e in w["fd_eps"]}
+    ads = []
     for k in range(n_dirs):
         d = problems.gaussian_random_field(nx, ny, 10_000 + k, corr_len=0.05)
-        ctx.set_state(*state(rho0 + eps * d))
-        Jp = ctx.adjoint_forward(steps, steps, w["window"])
-        ctx.set_state(*state(rho0 - eps * d))
-        Jm = ctx.adjoint_forward(steps, steps, w["window"])
-        fd = (Jp - Jm) / (2 * eps)
-        ad = float(np.dot(grad, d))
-        rel.append(abs(fd - ad) / max(abs(fd), abs(ad), 1e-300))
+        ads.append(float(np.dot(grad, d)))
+        for eps in w["fd_eps"]:
+            ctx.set_state(*state(rho0 + eps * d))
+            Jp = ctx.adjoint_forward(steps, steps, w["window"])
+            ctx.set_state(*state(rho0 - eps * d))
+            Jm = ctx.adjoint_forward(steps, steps, w["window"])
+            fds[eps].append((Jp - Jm) / (2 * eps))
+    ads = np.array(ads)
+    scale = float(np.sqrt(np.mean(ads ** 2))) if n_dirs else 1.0
+    rel = {e: [abs(f - a) / max(abs(f), abs(a), 1e-300) for f, a in zip(v, ads)] for e, v in fds.items()}
+    nrm = {e: [abs(f - a) / scale for f, a in zip(v, ads)] for e, v in fds.items()}
     n = nx * ny
     out = {
         "metric": "FP64 cell-updates/s (discrete adjoint of HLLC+MUSCL, reverse sweep incl. recompute)",
@@ -261,9 +267,12 @@ def adjoint_run(args, w, Z):
                    "forward_with_checkpoints_ms_per_step": t_fwd * 1e3 / steps,
                    "backward_ms_per_step": t_bwd * 1e3 / steps,
                    "adjoint_over_forward_step": t_bwd / t_plain,
-                   "fd_directions": n_dirs, "fd_eps": eps,
-                   "fd_max_rel_err": max(rel) if rel else None,
-                   "fd_median_rel_err": float(np.median(rel)) if rel else None},
+                   "fd_directions": n_dirs,
+                   "fd_rel_err": {f"{e:g}": {"max": max(r), "median": float(np.median(r)),
+                                             "within_1e-5": int(sum(x <= 1e-5 for x in r))}
+                                  for e, r in rel.items() if r},
+                   "fd_err_over_rms_derivative": {f"{e:g}": {"max": max(r), "median": float(np.median(r))}
+                                                  for e, r in nrm.items() if r}},
     }
     print(json.dumps(out), flush=True)
 
@@ -399,13 +408,16 @@ def ensemble_cpu_baseline(w, sizes, seed, budget_s):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=500)
+    ap.add_argument("--steps", type=int, default=None,
+                    help="timed steps (default: the workload's own, 500 or 200)")
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--workload", default="amr256_muscl", choices=sorted(WORKLOADS))
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--cpu-budget", type=float, default=10.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--fd-dirs", type=int, default=0, help="adjoint workload: finite-difference directions")
+    ap.add_argument("--prof-steps", type=int, default=100,
+                    help="steps of the event-timed kernel pass after the timed region")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run device-to-device copies of the state (known bytes) for HBM counter scaling")
     args = ap.parse_args()
@@ -418,6 +430,8 @@ def main():
     import zephyr_amd as Z
 
     w = WORKLOADS[args.workload]
+    if args.steps is None:
+        args.steps = w.get("steps", 500)
     if w["kind"] == "ensemble":
         return ensemble_run(args, w, Z)
     if w["kind"] == "adjoint":
@@ -438,6 +452,10 @@ def main():
 
     run_steps(ctx, w, args.warmup, 0)
     ctx.synchronize()
+    # launch configuration (layout, workgroup size per stage) picked by the
+    # library's autotuner on the warmed-up state; the state does not advance
+    tune = ctx.autotune(reps=3)
+    ctx.synchronize()
 
     if args.copy_calibration:
         # 5 device-to-device copies of n doubles (the init layer) into fresh
@@ -455,14 +473,22 @@ def main():
             hip.hipFree(b)
         print(f"copy calibration: n = {ctx.n} doubles per copy", file=sys.stderr, flush=True)
 
+    # timed region: exactly --steps steps, no per-kernel events (an event pair
+    # around every launch adds an idle gap between dependent kernels)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    cells = run_steps(ctx, w, args.steps, args.warmup)
+    ctx.synchronize()
+    el = time.perf_counter() - t0
+
+    # kernel timing pass: the workload continues for --prof-steps steps with
+    # HIP events around every step-kernel launch (on the context's stream)
     prof = ChunkProfile(ctx)
     prof.n, prof.phi = ctx.n, ctx.n_faces() / ctx.n
     ctx.profile_enable(True)
     ctx.synchronize()
-    t0 = time.perf_counter()
-    cells = run_steps(ctx, w, args.steps, args.warmup, prof)
+    run_steps(ctx, w, args.prof_steps, args.warmup + args.steps, prof)
     ctx.synchronize()
-    el = time.perf_counter() - t0
     prof.close_chunk(prof.n, prof.phi)
     ctx.profile_enable(False)
 
@@ -508,8 +534,11 @@ def main():
                    "phi_final": ctx.n_faces() / ctx.n,
                    "kernel_ms_avg": {k: prof.ms[k] / prof.launches[k] for k in prof.ms},
                    "kernel_launches": dict(prof.launches),
+                   "kernel_timing": f"HIP events, {args.prof_steps} steps following the timed region",
+                   "launch": {str(st): ctx.launch(st) for st in ((0,) if acc == 1 else (1, 2))},
+                   "autotune_ms": tune.tolist(),
                    "step_kernels_hbm_frac": step_bytes / (step_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
-                   "step_kernels_ms_share_of_wall": step_ms / (el * 1e3)},
+                   "step_kernels_ms_share_of_wall": (step_ms / max(args.prof_steps, 1)) / (el * 1e3 / args.steps)},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "algorithmic_bytes_per_launch": alg_bytes, "avg_launch_ms": dom_ms,
