@@ -68,6 +68,13 @@ WORKLOADS = {
              "over [0.4,0.6]^2 at the final step; gradient dJ/drho0 by the discrete adjoint (checkpoint every "
              "20 steps, recompute in between), checked against central differences (eps 1e-6 and 1e-7) on 64 seeded "
              "directions"),
+    "mm512_bubble_first_order": dict(
+        kind="mm", nx=512, ny=512, steps=300, cfl=0.5,
+        desc="two-material shock-bubble on a 512x512 mesh (the C2 base mesh, without AMR): stiffened-gas water "
+             "(gamma 4.4, P0 6e8) with an air bubble (gamma 1.4) of radius 0.15 at the centre, a planar shock of "
+             "seeded Mach number 1.5-3 at x = 0.2 moving into the water; MmFluid first order, material-aware "
+             "HLLC on every face (no composite Riemann problems), PT-equilibrium closure, CFL 0.5, FP64, "
+             "ZOE bounds"),
     "ensemble256_muscl": dict(
         kind="ensemble", problems=256, smin=64, smax=512, acc=2, limiter="MC", steps=200,
         desc="256 independent problems of s x s cells, s ~ U{64..512} (seeded), per-problem seeded "
@@ -277,6 +284,78 @@ def adjoint_run(args, w, Z):
     print(json.dumps(out), flush=True)
 
 
+# Algorithmic bytes per cell of the multi-material first-order update with
+# n_mat materials: own state read and write (7 + 2 n_mat doubles each),
+# code, volume, and per face slot adjacency, area and normal.
+def mm_bytes_per_cell(n_mat, faces_per_cell=4):
+    state = 8 * (7 + 2 * n_mat)
+    return 2 * state + 4 + 8 + faces_per_cell * (4 + 8 + 16)
+
+
+def mm_run(args, w, Z):
+    """Multi-material first-order step (MmFluid, no CRP) on the shock-bubble."""
+    from zephyr_amd import problems
+    nx, ny = w["nx"], w["ny"]
+    ctx = Z.Context()
+    ctx.mesh_rectangle(nx, ny)
+    mats = [problems.WATER, problems.AIR]
+    ctx.mm_configure(mats, w["cfl"])
+    cx, cy = rect_centres(nx, ny)
+    st = problems.mm_bubble_shock(cx, cy, 1.0 / nx, args.seed)
+    ctx.mm_set_state(*st)
+    ctx.mm_step(args.warmup)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    ctx.mm_step(args.steps)
+    ctx.synchronize()
+    el = time.perf_counter() - t0
+    ctx.profile_enable(True)
+    ctx.mm_step(args.prof_steps)
+    ctx.synchronize()
+    prof = ctx.profile_read()
+    ctx.profile_enable(False)
+    ms, launches = prof["mm_flux1"]
+    avg_ms = ms / launches
+    z = ctx.mm_state()
+    mixed = int(np.sum(np.sum(z["beta"] > 0.0, axis=0) > 1))
+    n = nx * ny
+    alg = mm_bytes_per_cell(2) * n
+    achieved = alg / (avg_ms * 1e-3) / 1e9
+    out = {
+        "metric": "FP64 cell-updates/s (multi-material first order, HLLC + PT closure)",
+        "value": n * args.steps / el, "unit": "cell-updates/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": el * 1e3 / args.steps, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": args.workload, "description": w["desc"], "cells": n,
+                   "mach": problems.mm_bubble_shock_params(args.seed), "mixed_cells_final": mixed,
+                   "dt_final": ctx.mm_dt(), "kernel_ms_avg": {"mm_flux1": avg_ms},
+                   "kernel_timing": f"HIP events, {args.prof_steps} steps following the timed region"},
+        "roofline": {"bound": "hbm", "kernel": "mm_flux1", "achieved": achieved, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "algorithmic_bytes_per_launch": alg, "avg_launch_ms": avg_ms},
+    }
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = mm_cpu_baseline(w, st, mats, args.cpu_budget)
+    print(json.dumps(out), flush=True)
+
+
+def mm_cpu_baseline(w, st, mats, budget_s):
+    """Oracle MmFluid (serial C++) on the same initial data, whole steps until the budget is spent."""
+    from oracle import pyoracle as O
+    O.build()
+    m = O.Mesh.rectangle(w["nx"], w["ny"])
+    o = O.MmFluid(m, mats, w["cfl"])
+    o.set_state(*st)
+    steps, t0 = 0, time.perf_counter()
+    while time.perf_counter() - t0 < budget_s:
+        o.update()
+        steps += 1
+    el = time.perf_counter() - t0
+    return {"value": m.n * steps / el, "unit": "cell-updates/s", "cores": 1, "kind": "port",
+            "sample": f"oracle MmFluid (C++ -O3 -ffp-contract=off, serial) on the same workload from the same "
+                      f"initial data, {steps} steps in {el:.2f} s"}
+
+
 def ensemble_run(args, w, Z):
     """Batched ensemble vs the looped per-problem path, same problems."""
     sizes = ensemble_sizes(w, args.seed)
@@ -436,6 +515,8 @@ def main():
         return ensemble_run(args, w, Z)
     if w["kind"] == "adjoint":
         return adjoint_run(args, w, Z)
+    if w["kind"] == "mm":
+        return mm_run(args, w, Z)
     nx, ny, acc, limiter = w["nx"], w["ny"], w["acc"], w["limiter"]
     ctx = Z.Context()
     ctx.mesh_rectangle(nx, ny)

@@ -204,6 +204,38 @@ int zfv_batch_dt(zfv_ctx* ctx, double* dt);
 int zfv_adjoint_forward(zfv_ctx* ctx, int n_steps, int every, const double* window, double* J);
 int zfv_adjoint_backward(zfv_ctx* ctx, double* d_rho, double* d_u, double* d_v, double* d_p, double* d_e);
 
+/* ---- multi-material solver (MmFluid, mm_fluid.h:45-163) ------------------- */
+/* First-order MmFluid step without composite Riemann problems (the
+ * CrpMode::NONE branch of MmFluid::calc_flux, mm_fluid.cpp:351-356, with the
+ * default HLLC: the mmf HLLC flux on every face), on the context's mesh
+ * (one problem, uniform or adapted).  Materials are stiffened gases with a
+ * temperature; mats = n_mat x {gamma, P0, e0, Cv, T0, rho_ref} (StiffenedGas
+ * parameters, stiffened_gas.cpp:11-12, and the reference density that seeds
+ * the closure, eos.h:152); the mixture is in pressure and temperature
+ * equilibrium (MixturePT, mixture_pt.cpp:199-533).  cfl is clamped to [0, 1]
+ * (MmFluid::set_CFL, mm_fluid.cpp:41-43; the reference default is 0.7).
+ * Per-material arrays are [n_mat][n]. */
+int zfv_mm_configure(zfv_ctx* ctx, int n_mat, const double* mats, double cfl);
+/* MmFluid::set_max_dt (mm_fluid.cpp:94-96). */
+int zfv_mm_set_max_dt(zfv_ctx* ctx, double max_dt);
+/* Cells from {rho, u, v, P, beta}: mmf::PState(rho, v, P, Fractions, mixture)
+ * (models.cpp:176-185), beta normalised as Fractions does. */
+int zfv_mm_set_state(zfv_ctx* ctx, const double* rho, const double* u, const double* v, const double* P,
+                     const double* beta);
+/* rho, u, v, P, e, T, beta [n_mat][n] and true densities rhos [n_mat][n]
+ * (NaN for an absent material). */
+int zfv_mm_state(zfv_ctx* ctx, double* rho, double* u, double* v, double* P, double* e, double* T, double* beta,
+                 double* rhos);
+/* n_steps MmFluid::update steps (compute_dt + fluxes + swap,
+ * mm_fluid.cpp:110-196, 373-445, 682-684). */
+int zfv_mm_step(zfv_ctx* ctx, int n_steps);
+/* dt of the last step (MmFluid::dt). */
+int zfv_mm_dt(zfv_ctx* ctx, double* dt);
+/* MmFluid::interface_recovery (mm_fluid.cpp:484-519) of the current state:
+ * per material the PLIC normal (nx, ny) and section position p,
+ * [n_mat][n] each; zero for pure cells and absent materials. */
+int zfv_mm_interface(zfv_ctx* ctx, double* nx, double* ny, double* p);
+
 /* ---- launch configuration (no reference counterpart) --------------------- */
 /* Step kernels of a stage (0 = first-order update, 1 = gradient + predictor,
  * 2 = corrector): layout 0 = one thread per cell, 1 = four lanes per cell

@@ -1136,3 +1136,4 @@ void orc_init_quadrants(const orc_mesh* m, uint64_t seed, double gamma,
 }  // extern "C"
 
 #include "oracle_amr.hpp"
+#include "oracle_mm.hpp"

@@ -117,6 +117,30 @@ double orc_splitmix_uniform(uint64_t* state);
 void orc_init_quadrants(const orc_mesh* m, uint64_t seed, double gamma,
                         double* rho, double* u, double* v, double* p, double* e);
 
+/* Multi-material solver (MmFluid, first order, CrpMode::NONE) with the
+ * PT-equilibrium closure of stiffened gases; mats: 6 doubles per material
+ * {gamma, P0, e0, Cv, T0, rho_ref}; per-material arrays are [mat][n]. */
+typedef struct orc_mm orc_mm;
+orc_mm* orc_mm_create(orc_mesh* m, int n_mat, const double* mats, double cfl);
+void orc_mm_free(orc_mm* s);
+void orc_mm_set_max_dt(orc_mm* s, double v);
+void orc_mm_set_state(orc_mm* s, const double* rho, const double* u, const double* v, const double* P,
+                      const double* beta);
+void orc_mm_state(const orc_mm* s, double* rho, double* u, double* v, double* P, double* e, double* T,
+                  double* beta, double* rhos);
+double orc_mm_compute_dt(orc_mm* s);
+int orc_mm_update(orc_mm* s);
+double orc_mm_dt(const orc_mm* s);
+void orc_mm_interface(orc_mm* s, double* nx, double* ny, double* p);
+void orc_mm_closure(int n_mat, const double* mats, double rho, double e, const double* beta, double P0,
+                    double T0, double* out);
+void orc_mm_closure_rP(int n_mat, const double* mats, double rho, double P, const double* beta, double* out);
+void orc_mm_face_flux(const orc_mm* s, int cL, int cR, double nx, double ny, double* F);
+double orc_quad_volume_fraction(double p, double nx, double ny, double A, double B);
+double orc_quad_find_section(double alpha, double nx, double ny, double A, double B);
+double orc_face_fraction(int kind, double a1, double a2);
+double orc_average_flux(double alpha, double cs, double CFL);
+
 #ifdef __cplusplus
 }
 #endif

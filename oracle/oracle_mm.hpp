@@ -1,0 +1,752 @@
+// ORACLE — test infrastructure only (included at the end of oracle.cpp).
+// CPU restatement of zephyr's multi-material solver MmFluid in its
+// first-order form without composite Riemann problems (CrpMode::NONE: the
+// material-aware HLLC flux on every face), with the PT-equilibrium mixture
+// closure of stiffened gases and the PLIC interface reconstruction.  Each
+// function cites the reference file:line it restates.  Operation order
+// follows the reference's expressions under -ffp-contract=off; std::pow(x, 2)
+// is written x * x (glibc's pow(x, 2.0) equals x * x bit for bit: checked on
+// 5e7 random arguments over 200 binades).
+
+#include <limits>
+
+namespace orcmm {
+
+constexpr int kMaxMat = 4;   // Fractions::max_size (phys/fractions.h:14)
+
+// StiffenedGas (phys/matter/eos/stiffened_gas.cpp:11-175), one material.
+struct SgMat {
+    double g, P0, e0, Cv, T0, rho_ref;
+};
+
+struct DPDT { double val, dP, dT; };   // dPdT (phys/matter/eos/eos.h:37-47)
+
+inline double sg_pressure_re(const SgMat& m, double rho, double e) {   // :104-111
+    return (m.g - 1.0) * rho * (e - m.e0) - m.g * m.P0;
+}
+inline double sg_energy_rP(const SgMat& m, double rho, double P) {   // :122-129
+    return m.e0 + (P + m.g * m.P0) / ((m.g - 1.0) * rho);
+}
+inline double sg_energy_rT(const SgMat& m, double rho, double T) {   // :131-138
+    return m.e0 + m.Cv * (T - m.T0) + m.P0 / rho;
+}
+inline double sg_temperature_rP(const SgMat& m, double rho, double P) {   // :140-142
+    return m.T0 + (P + m.P0) / ((m.g - 1.0) * m.Cv * rho);
+}
+inline double sg_sound_speed_rP(const SgMat& m, double rho, double P) {   // :148-150
+    return std::sqrt(m.g * (P + m.P0) / rho);
+}
+inline DPDT sg_volume_PT(const SgMat& m, double P, double T) {   // :152-159
+    DPDT r;
+    r.val = ((m.g - 1.0) * m.Cv * (T - m.T0)) / (P + m.P0);
+    r.dP = -r.val / (P + m.P0);
+    r.dT = +r.val / (T - m.T0);
+    return r;
+}
+inline DPDT sg_energy_PT(const SgMat& m, double P, double T) {   // :161-168
+    DPDT r;
+    r.val = m.e0 + ((P + m.g * m.P0) / (P + m.P0)) * m.Cv * (T - m.T0);
+    r.dP = (1.0 - m.g) * m.P0 * m.Cv * (T - m.T0) / ((P + m.P0) * (P + m.P0));
+    r.dT = ((P + m.g * m.P0) / (P + m.P0)) * m.Cv;
+    return r;
+}
+
+// Fractions (phys/fractions.h:12-105, fractions.cpp:29-110).
+struct Frac {
+    double a[kMaxMat] = {0.0, 0.0, 0.0, 0.0};
+    bool has(int i) const { return a[i] > 0.0; }
+    int index() const {   // fractions.cpp:49-61
+        int idx = -1;
+        for (int i = 0; i < kMaxMat; ++i)
+            if (has(i)) {
+                if (idx < 0) idx = i;
+                else return -1;
+            }
+        return idx;
+    }
+    bool is_pure() const {   // fractions.cpp:29-41
+        bool pure = false;
+        for (int i = 0; i < kMaxMat; ++i)
+            if (has(i)) {
+                if (!pure) pure = true;
+                else return false;
+            }
+        return pure;
+    }
+    void cutoff(double eps) {   // fractions.cpp:67-80
+        double sum = 0.0;
+        for (int i = 0; i < kMaxMat; ++i) {
+            if (a[i] < eps) a[i] = 0.0;
+            else if (a[i] > 1.0) a[i] = 1.0;
+            sum += a[i];
+        }
+        for (int i = 0; i < kMaxMat; ++i) a[i] /= sum;
+    }
+    void normalize() { cutoff(std::numeric_limits<double>::epsilon() / 4); }   // :63-65
+};
+
+// MixturePT (phys/matter/mixture_pt.{h,cpp}), old-style iterations (the
+// default, mixture_pt.h:33).
+struct Mixture {
+    int n = 0;
+    SgMat m[kMaxMat];
+};
+
+struct MixOpt {   // MixOptions (mixture_pt.h:12-27)
+    bool deriv = false;
+    double P0 = NAN, T0 = NAN;
+    const double* rhos = nullptr;   // empty span = nullptr
+};
+
+constexpr int kMaxIter = 30;      // mixture_pt.cpp:298
+constexpr double kEps = 1.0e-12;  // epsilon_v, epsilon_e
+
+// init_densities (mixture_pt.cpp:240-256).
+inline void init_densities(const Mixture& mx, const Frac& b, const MixOpt& o, double* rhos) {
+    for (int i = 0; i < kMaxMat; ++i) rhos[i] = NAN;
+    for (int i = 0; i < mx.n; ++i)
+        if (b.has(i)) rhos[i] = o.rhos ? o.rhos[i] : mx.m[i].rho_ref;
+}
+
+inline double min_pressure(const Mixture& mx, const Frac& b) {   // mixture_pt.cpp:171-179
+    double P_min = -std::numeric_limits<double>::infinity();
+    for (int i = 0; i < mx.n; ++i)
+        if (b.has(i)) P_min = smax(P_min, -mx.m[i].P0);
+    return P_min;
+}
+
+// find_rT_rP_old (mixture_pt.cpp:383-432): T from v_mix = sum beta_i v_i(P, T).
+inline void find_rT_rP(const Mixture& mx, double rho, double P, const Frac& b, const MixOpt& o, double* rhos,
+                       double& T, double& TdR, double& TdP) {
+    const double v_mix = 1.0 / rho;
+    T = std::isnan(o.T0) ? 300.0 : o.T0;
+    init_densities(mx, b, o, rhos);
+    DPDT v{NAN, NAN, NAN};
+    int counter = 0;
+    while (counter < kMaxIter) {
+        double err = 0.0;
+        v = {0.0, 0.0, 0.0};
+        for (int i = 0; i < mx.n; ++i) {
+            if (b.has(i)) {
+                DPDT vi = sg_volume_PT(mx.m[i], P, T);
+                v.val += b.a[i] * vi.val;
+                v.dP += b.a[i] * vi.dP;
+                v.dT += b.a[i] * vi.dT;
+                err = smax(err, std::abs(1.0 - rhos[i] * vi.val));
+                rhos[i] = 1.0 / vi.val;
+            }
+        }
+        err = smax(err, std::abs(1.0 - rho * v.val));
+        if (err < kEps) break;
+        T += (v_mix - v.val) / v.dT;
+        ++counter;
+    }
+    TdR = NAN;
+    TdP = NAN;
+    if (o.deriv) {
+        TdR = -(v.val * v.val) / v.dT;
+        TdP = -v.dP / v.dT;
+    }
+}
+
+// energy_PT of the mixture (mixture_pt.cpp:134-150).
+inline DPDT mix_energy_PT(const Mixture& mx, double P, double T, const Frac& b) {
+    DPDT e{0.0, 0.0, 0.0};
+    for (int i = 0; i < mx.n; ++i)
+        if (b.has(i)) {
+            DPDT ei = sg_energy_PT(mx.m[i], P, T);
+            e.val += b.a[i] * ei.val;
+            e.dP += b.a[i] * ei.dP;
+            e.dT += b.a[i] * ei.dT;
+        }
+    return e;
+}
+
+// find_reT_rP_old (mixture_pt.cpp:434-446).
+inline void find_reT_rP(const Mixture& mx, double rho, double P, const Frac& b, const MixOpt& o, double* rhos,
+                        double& e, double& edR, double& edP, double& T) {
+    double TdR, TdP;
+    find_rT_rP(mx, rho, P, b, o, rhos, T, TdR, TdP);
+    DPDT e1 = mix_energy_PT(mx, P, T, b);
+    e = e1.val;
+    edR = NAN;
+    edP = NAN;
+    if (o.deriv) {
+        edR = e1.dT * TdR;
+        edP = e1.dT * TdP + e1.dP;
+    }
+}
+
+// find_rPT_old (mixture_pt.cpp:448-533): Newton on (P, T) for
+// v_mix = sum beta_i v_i(P, T), e_mix = sum beta_i e_i(P, T).
+inline void find_rPT(const Mixture& mx, double rho, double e_mix, const Frac& b, const MixOpt& o, double* rhos,
+                     double& P, double& T) {
+    const double v_mix = 1.0 / rho;
+    P = std::isnan(o.P0) ? 1.0e5 : o.P0;
+    T = std::isnan(o.T0) ? 300.0 : o.T0;
+    const double P_min = min_pressure(mx, b);
+    if (P < P_min) P = P_min + 1;
+    init_densities(mx, b, o, rhos);
+    int counter = 0;
+    DPDT v{NAN, NAN, NAN}, e{NAN, NAN, NAN};
+    while (counter < kMaxIter) {
+        double err = 0.0;
+        v = {0.0, 0.0, 0.0};
+        e = {0.0, 0.0, 0.0};
+        for (int i = 0; i < mx.n; ++i) {
+            if (b.has(i)) {
+                DPDT vi = sg_volume_PT(mx.m[i], P, T);
+                v.val += b.a[i] * vi.val;
+                v.dP += b.a[i] * vi.dP;
+                v.dT += b.a[i] * vi.dT;
+                err = smax(err, std::abs(1.0 - rhos[i] * vi.val));
+                rhos[i] = 1.0 / vi.val;
+                DPDT ei = sg_energy_PT(mx.m[i], P, T);
+                e.val += b.a[i] * ei.val;
+                e.dP += b.a[i] * ei.dP;
+                e.dT += b.a[i] * ei.dT;
+            }
+        }
+        err = smax(err, std::abs(1.0 - rho * v.val));
+        const double err_e = std::abs(1.0 - e.val / e_mix);
+        if (err < kEps && err_e < kEps) break;
+        const double inv_D = 1.0 / (v.dT * e.dP - v.dP * e.dT);   // inv_J (mixture_pt.cpp:22-24)
+        const double dP = inv_D * ((e_mix - e.val) * v.dT - (v_mix - v.val) * e.dT);
+        const double dT = inv_D * ((v_mix - v.val) * e.dP - (e_mix - e.val) * v.dP);
+        double dP_lim = dP;
+        if (P + dP < P_min) {
+            const double a = 0.1;
+            dP_lim = (1.0 - a) * P_min + (a - 1.0) * P;
+        }
+        const double dT_lim = dP != 0.0 ? (dP_lim / dP) * dT : dT;
+        P += dP_lim;
+        T += dT_lim;
+        ++counter;
+    }
+}
+
+// get_rPT (mixture_pt.cpp:199-212): pure material closed form, else find_rPT.
+inline void get_rPT(const Mixture& mx, double rho, double e, const Frac& b, const MixOpt& o, double* rhos,
+                    double& P, double& T) {
+    const int idx = b.index();
+    if (idx >= 0) {
+        for (int i = 0; i < kMaxMat; ++i) rhos[i] = NAN;
+        rhos[idx] = rho;
+        P = sg_pressure_re(mx.m[idx], rho, e);
+        T = sg_temperature_rP(mx.m[idx], rho, P);
+        return;
+    }
+    find_rPT(mx, rho, e, b, o, rhos, P, T);
+}
+
+// get_reT (mixture_pt.cpp:214-227).
+inline void get_reT(const Mixture& mx, double rho, double P, const Frac& b, const MixOpt& o, double* rhos,
+                    double& e, double& T) {
+    const int idx = b.index();
+    if (idx >= 0) {
+        for (int i = 0; i < kMaxMat; ++i) rhos[i] = NAN;
+        rhos[idx] = rho;
+        e = sg_energy_rP(mx.m[idx], rho, P);
+        T = sg_temperature_rP(mx.m[idx], rho, P);
+        return;
+    }
+    double edR, edP;
+    find_reT_rP(mx, rho, P, b, o, rhos, e, edR, edP, T);
+}
+
+// sound_speed_rP (mixture_pt.cpp:101-114).
+inline double mix_sound_speed_rP(const Mixture& mx, double rho, double P, const Frac& b, const MixOpt& o) {
+    const int idx = b.index();
+    if (idx >= 0) return sg_sound_speed_rP(mx.m[idx], rho, P);
+    MixOpt od = o;
+    od.deriv = true;
+    double rhos[kMaxMat], e, edR, edP, T;
+    find_reT_rP(mx, rho, P, b, od, rhos, e, edR, edP, T);
+    const double c2 = (P / (rho * rho) - edR) / edP;
+    return std::sqrt(c2);
+}
+
+// mmf::PState (math/cfd/models.h:164-283): 2D, velocity z = 0.
+struct MState {
+    double rho;
+    Vec3 v;
+    double P, e, T;
+    Frac beta;
+    double rhos[kMaxMat];
+};
+
+inline double m_E(const MState& z) { return z.e + 0.5 * sqnorm(z.v); }   // models.h:237
+
+// PState::alpha (models.cpp:224-227).
+inline double m_alpha(const MState& z, int i) {
+    if (std::isnan(z.rhos[i])) return 0.0;
+    const double a = z.beta.a[i] * z.rho / z.rhos[i];
+    const double lo = (1.0 < a) ? 1.0 : a;   // std::min(a, 1.0)
+    return (0.0 < lo) ? lo : 0.0;            // std::max(0.0, lo)
+}
+
+// PState::volume_fractions (models.cpp:229-239): unclamped, no normalisation.
+inline Frac m_volume_fractions(const MState& z) {
+    Frac a;
+    for (int i = 0; i < kMaxMat; ++i)
+        if (z.beta.has(i)) a.a[i] = z.beta.a[i] * z.rho / z.rhos[i];
+    return a;
+}
+
+// mmf::QState / mmf::Flux as 9 doubles {rho, m.x, m.y, m.z, E, mass_frac[4]}.
+struct MArr { double a[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}; };
+
+inline MArr m_qstate(const MState& z) {   // QState(const PState&) (models.cpp:413-420)
+    MArr q;
+    q.a[0] = z.rho;
+    q.a[1] = z.rho * z.v.x;
+    q.a[2] = z.rho * z.v.y;
+    q.a[3] = z.rho * z.v.z;
+    q.a[4] = z.rho * m_E(z);
+    for (int i = 0; i < kMaxMat; ++i) q.a[5 + i] = z.rho * z.beta.a[i];
+    return q;
+}
+
+inline MArr m_flux(const MState& z) {   // Flux(const PState&) (models.cpp:459-469)
+    MArr f;
+    f.a[0] = z.rho * z.v.x;
+    f.a[1] = z.rho * z.v.x * z.v.x + z.P;
+    f.a[2] = z.rho * z.v.x * z.v.y;
+    f.a[3] = z.rho * z.v.x * z.v.z;
+    f.a[4] = (z.rho * m_E(z) + z.P) * z.v.x;
+    for (int i = 0; i < kMaxMat; ++i) f.a[5 + i] = z.rho * z.v.x * z.beta.a[i];
+    return f;
+}
+
+// PState(const QState&, mixture, P0, T0, rhos) (models.cpp:187-201).
+inline MState m_from_q(const MArr& q, const Mixture& mx, double P0, double T0, const double* rhos0) {
+    MState z;
+    z.rho = q.a[0];
+    z.v = {q.a[1] / z.rho, q.a[2] / z.rho, q.a[3] / z.rho};
+    z.e = q.a[4] / z.rho - 0.5 * sqnorm(z.v);
+    for (int i = 0; i < mx.n; ++i) z.beta.a[i] = q.a[5 + i] / z.rho;
+    z.beta.cutoff(1.0e-12);
+    z.beta.normalize();
+    MixOpt o;
+    o.P0 = P0;
+    o.T0 = T0;
+    o.rhos = rhos0;
+    get_rPT(mx, z.rho, z.e, z.beta, o, z.rhos, z.P, z.T);
+    return z;
+}
+
+// PState(rho, v, P, beta, mixture) (models.cpp:176-185).
+inline MState m_from_rvp(const Mixture& mx, double rho, double u, double v, double P, const Frac& beta) {
+    MState z;
+    z.rho = rho;
+    z.v = {u, v, 0.0};
+    z.P = P;
+    z.beta = beta;
+    MixOpt o;
+    get_reT(mx, rho, P, beta, o, z.rhos, z.e, z.T);
+    return z;
+}
+
+// HLLC::calc_flux, mmf (math/cfd/flux/hllc.cpp:339-421); states in the face frame.
+inline MArr m_hllc(const MState& zL, const MState& zR, const Mixture& mx) {
+    MixOpt oL, oR;
+    oL.T0 = zL.T;
+    oL.rhos = zL.rhos;
+    oR.T0 = zR.T;
+    oR.rhos = zR.rhos;
+    const double c_L = mix_sound_speed_rP(mx, zL.rho, zL.P, zL.beta, oL);
+    const double c_R = mix_sound_speed_rP(mx, zR.rho, zR.P, zR.beta, oR);
+    double S_L = smin(zL.v.x - c_L, zR.v.x - c_L);
+    double S_R = smax(zL.v.x + c_R, zR.v.x + c_R);
+    if (S_L >= S_R || zL.beta.index() == zR.beta.index()) {
+        S_L = smin(zL.v.x - c_L, zR.v.x - c_R);
+        S_R = smax(zL.v.x + c_L, zR.v.x + c_R);
+    }
+    if (S_L >= 0.0) return m_flux(zL);
+    if (S_R <= 0.0) return m_flux(zR);
+    const double a_L = zL.rho * (S_L - zL.v.x);
+    const double a_R = zR.rho * (S_R - zR.v.x);
+    double S_C = (zL.P - zR.P + a_R * zR.v.x - a_L * zL.v.x) / (a_R - a_L);
+    if (S_C < S_L) S_C = S_L + 1.0e-9 * smax(c_L, c_R);
+    if (S_C > S_R) S_C = S_R - 1.0e-9 * smax(c_L, c_R);
+    const double rho_sL = a_L / (S_L - S_C);
+    const double rho_sR = a_R / (S_R - S_C);
+    const double E_sL = m_E(zL) + (S_C - zL.v.x) * (S_C + zL.P / a_L);
+    const double E_sR = m_E(zR) + (S_C - zR.v.x) * (S_C + zR.P / a_R);
+    MArr QsL, QsR;
+    QsL.a[0] = rho_sL;
+    QsR.a[0] = rho_sR;
+    QsL.a[1] = rho_sL * S_C; QsL.a[2] = rho_sL * zL.v.y; QsL.a[3] = rho_sL * zL.v.z;
+    QsR.a[1] = rho_sR * S_C; QsR.a[2] = rho_sR * zR.v.y; QsR.a[3] = rho_sR * zR.v.z;
+    QsL.a[4] = rho_sL * E_sL;
+    QsR.a[4] = rho_sR * E_sR;
+    for (int i = 0; i < mx.n; ++i) {
+        QsL.a[5 + i] = rho_sL * zL.beta.a[i];
+        QsR.a[5 + i] = rho_sR * zR.beta.a[i];
+    }
+    const MArr QL = m_qstate(zL), QR = m_qstate(zR), FL = m_flux(zL), FR = m_flux(zR);
+    const double aSC = std::abs(S_C);
+    MArr F;
+    for (int k = 0; k < 9; ++k) {
+        double t = FL.a[k] + FR.a[k];
+        t = t + S_L * (QsL.a[k] - QL.a[k]);
+        t = t + aSC * (QsL.a[k] - QsR.a[k]);
+        t = t + S_R * (QsR.a[k] - QR.a[k]);
+        F.a[k] = 0.5 * t;
+        if (std::isnan(F.a[k])) g_numeric_error = 1;   // hllc.cpp:407-417 throws
+    }
+    return F;
+}
+
+// boundary_value, mmf (math/solver/mm_fluid.cpp:98-108).
+inline MState m_boundary_value(const MState& zc, const Vec3& n, int flag) {
+    if (flag != ORC_WALL) return zc;
+    MState zn = zc;
+    const Vec3 Vn = scale(dot(zc.v, n), n);
+    zn.v = {zc.v.x - 2.0 * Vn.x, zc.v.y - 2.0 * Vn.y, zc.v.z - 2.0 * Vn.z};
+    return zn;
+}
+
+inline MState m_in_local(const MState& z, const Vec3& n) {
+    MState r = z;
+    r.v = rot_local(z.v, n);
+    return r;
+}
+
+// ---- PLIC formulas (geom/sections.cpp) and math/funcs.h helpers ----------
+inline double f_sign(double x) { return x > 0.0 ? 1.0 : (x < 0.0 ? -1.0 : 0.0); }   // funcs.h:21-23
+inline double f_heav(double x) { return x > 0.0 ? 1.0 : 0.0; }                        // funcs.h:27-29
+inline void f_sorted(double a, double b, double& lo, double& hi) {                    // funcs.h:32-35
+    if (a < b) { lo = a; hi = b; }
+    else { lo = b; hi = a; }
+}
+inline double f_between(double x, double x1, double x2) {   // funcs.h:47-50
+    double lo, hi;
+    f_sorted(x1, x2, lo, hi);
+    return smax(lo, smin(x, hi));
+}
+
+double quad_volume_fraction(double p, double nx, double ny, double A, double B) {   // sections.cpp:13-23
+    double a, b;
+    f_sorted(A * std::abs(nx), B * std::abs(ny), a, b);
+    if (std::abs(p) <= 0.5 * (b - a)) return 0.5 + p / b;
+    if (std::abs(p) < 0.5 * (b + a)) {
+        const double t = std::abs(p) - 0.5 * (a + b);
+        return f_heav(p) - 0.5 * f_sign(p) * (t * t) / (a * b);
+    }
+    return f_heav(p);
+}
+
+double quad_find_section(double alpha, double nx, double ny, double A, double B) {   // sections.cpp:25-37
+    double a, b;
+    f_sorted(A * std::abs(nx), B * std::abs(ny), a, b);
+    if (alpha <= 0.5 * a / b) return -0.5 * (a + b) + std::sqrt(2.0 * alpha * a * b);
+    if (alpha < 1.0 - 0.5 * a / b) return (alpha - 0.5) * b;
+    return +0.5 * (a + b) - std::sqrt(2.0 * (1.0 - alpha) * a * b);
+}
+
+inline void minmax_unit(double a1, double a2, double& lo, double& hi) {   // sections.cpp:42-48
+    if (a1 < a2) { lo = smax(0.0, a1); hi = smin(a2, 1.0); }
+    else { lo = smax(0.0, a2); hi = smin(a1, 1.0); }
+}
+
+double face_fraction_v3(double a1, double a2) {   // sections.cpp:52-62
+    double lo, hi;
+    minmax_unit(a1, a2, lo, hi);
+    if (lo == 0.0) return 0.0;
+    if (hi == 1.0) return 1.0;
+    if (a1 + a2 < 1.0) return hi;
+    if (a1 + a2 > 1.0) return lo;
+    return 0.5 * (a1 + a2);
+}
+
+double face_fraction_v5(double a1, double a2) {   // sections.cpp:64-71
+    double lo, hi;
+    minmax_unit(a1, a2, lo, hi);
+    if (lo == 0.0) return 0.0;
+    if (hi == 1.0) return 1.0;
+    return 0.5 * (a1 + a2);
+}
+
+double face_fraction_s(double a1, double a2) {   // sections.cpp:73-84
+    double lo, hi;
+    minmax_unit(a1, a2, lo, hi);
+    double a_sig = lo / (1.0 - (hi - lo));
+    if (std::isnan(a_sig)) a_sig = 0.5;
+    return f_between(a_sig, lo, hi);
+}
+
+double face_fraction(double a1, double a2) {   // sections.cpp:86-104
+    double lo, hi;
+    minmax_unit(a1, a2, lo, hi);
+    if (3.0 * lo >= hi && lo >= 3.0 * hi - 2.0) return 0.5 * (lo + hi);
+    if (hi <= 0.5 || lo * (1.0 - hi) >= (hi - 0.5) * (hi - 0.5))
+        return -2.0 * lo + 2.0 * std::sqrt(lo * (lo + hi));
+    if (lo >= 0.5 || lo * (1.0 - hi) >= (lo - 0.5) * (lo - 0.5))
+        return 3.0 - 2.0 * hi - 2.0 * std::sqrt((1.0 - hi) * (2.0 - lo - hi));
+    if (lo == 0.0 && hi == 1.0) return 0.5;
+    return std::sqrt(lo) / (std::sqrt(1.0 - hi) + std::sqrt(lo));
+}
+
+double average_flux(double alpha, double cs, double CFL) {   // sections.cpp:125-148
+    double a1, b1, a2, b2;
+    f_sorted(std::abs(cs), std::sqrt(1.0 - cs * cs), a1, b1);
+    f_sorted(CFL * std::abs(cs), std::sqrt(1.0 - cs * cs), a2, b2);
+    double P1;
+    if (std::abs(2 * alpha - 1) <= 1.0 - a1 / b1) P1 = (alpha - 0.5) * b1;
+    else P1 = f_sign(alpha - 0.5) * (0.5 * (a1 + b1) - std::sqrt((1.0 - std::abs(2.0 * alpha - 1.0)) * a1 * b1));
+    const double P2 = P1 + 0.5 * (CFL - 1.0) * cs;
+    if (std::abs(P2) <= 0.5 * (b2 - a2)) return 0.5 + P2 / b2;
+    if (std::abs(P2) < 0.5 * (b2 + a2)) {
+        const double t = std::abs(P2) - 0.5 * (a2 + b2);
+        return f_heav(P2) - 0.5 * f_sign(P2) * (t * t) / (a2 * b2);
+    }
+    return f_heav(P2);
+}
+
+}  // namespace orcmm
+
+using namespace orcmm;
+
+// =============================================================== MmFluid
+struct orc_mm {
+    orc_mesh* m;
+    Mixture mix;
+    double cfl, dt, max_dt;
+    std::vector<MState> init, next;   // MmFluid::Parts init / next (mm_fluid.h:53-67)
+    std::vector<double> nrm_x, nrm_y, pos;   // interface normals / positions [c][mat]
+};
+
+namespace {
+
+// AmrCells::hx / hy (mesh/euler/amr_cells.cpp:91-99): |vs<+1,0> - vs<-1,0>|, |vs<0,+1> - vs<0,-1>|.
+inline double cell_hx(const orc_mesh& m, int c) {
+    const double* X = &m.vx[9 * (size_t)c];
+    const double* Y = &m.vy[9 * (size_t)c];
+    Vec3 d{X[5] - X[3], Y[5] - Y[3], 0.0};
+    return std::sqrt(sqnorm(d));
+}
+inline double cell_hy(const orc_mesh& m, int c) {
+    const double* X = &m.vx[9 * (size_t)c];
+    const double* Y = &m.vy[9 * (size_t)c];
+    Vec3 d{X[7] - X[1], Y[7] - Y[1], 0.0};
+    return std::sqrt(sqnorm(d));
+}
+
+}  // namespace
+
+extern "C" {
+
+orc_mm* orc_mm_create(orc_mesh* m, int n_mat, const double* mats, double cfl) {
+    orc_mm* s = new orc_mm();
+    s->m = m;
+    s->mix.n = n_mat;
+    for (int i = 0; i < n_mat; ++i)
+        s->mix.m[i] = SgMat{mats[6 * i], mats[6 * i + 1], mats[6 * i + 2], mats[6 * i + 3], mats[6 * i + 4],
+                            mats[6 * i + 5]};
+    s->cfl = std::max(0.0, std::min(cfl, 1.0));   // set_CFL (mm_fluid.cpp:41-43)
+    s->dt = 1.0e+300;
+    s->max_dt = 1.0e+300;                          // mm_fluid.cpp:16-26
+    s->init.resize(m->n);
+    s->next.resize(m->n);
+    return s;
+}
+
+void orc_mm_free(orc_mm* s) { delete s; }
+
+void orc_mm_set_max_dt(orc_mm* s, double v) { s->max_dt = v; }
+
+// Cells from {rho, u, v, P, beta[mat][n]}: the problem files' PState(rho, v,
+// P, Fractions{...}, mixture) (models.cpp:176-185; the Fractions constructor
+// normalises, fractions.cpp:6-27).
+void orc_mm_set_state(orc_mm* s, const double* rho, const double* u, const double* v, const double* P,
+                      const double* beta) {
+    const int n = s->m->n;
+    for (int c = 0; c < n; ++c) {
+        Frac b;
+        for (int i = 0; i < s->mix.n; ++i) b.a[i] = beta[(size_t)i * n + c];
+        b.normalize();
+        s->init[c] = m_from_rvp(s->mix, rho[c], u[c], v[c], P[c], b);
+    }
+}
+
+void orc_mm_state(const orc_mm* s, double* rho, double* u, double* v, double* P, double* e, double* T,
+                  double* beta, double* rhos) {
+    const int n = s->m->n;
+    for (int c = 0; c < n; ++c) {
+        const MState& z = s->init[c];
+        rho[c] = z.rho; u[c] = z.v.x; v[c] = z.v.y; P[c] = z.P; e[c] = z.e; T[c] = z.T;
+        for (int i = 0; i < s->mix.n; ++i) {
+            beta[(size_t)i * n + c] = z.beta.a[i];
+            rhos[(size_t)i * n + c] = z.rhos[i];
+        }
+    }
+}
+
+// MmFluid::compute_dt (mm_fluid.cpp:166-194).
+double orc_mm_compute_dt(orc_mm* s) {
+    const orc_mesh& m = *s->m;
+    double dtmin = std::numeric_limits<double>::max();
+    for (int c = 0; c < m.n; ++c) {
+        double dt = std::numeric_limits<double>::max();
+        const MState& z = s->init[c];
+        MixOpt o;
+        o.T0 = z.T;
+        o.rhos = z.rhos;
+        const double cs = mix_sound_speed_rP(s->mix, z.rho, z.P, z.beta, o);
+        for (int k = 0; k < 8; ++k) {
+            const size_t f = 8 * (size_t)c + k;
+            if (m.bnd[f] == ORC_UNDEFINED) continue;
+            const Vec3 n{m.fnx[f], m.fny[f], 0.0};
+            const double vn = dot(z.v, n);
+            const double lambda = std::abs(vn) + cs;
+            dt = smin(dt, m.vol[c] / (m.area[f] * lambda));
+        }
+        dtmin = smin(dtmin, dt);   // EuMesh::min
+    }
+    s->dt = smin(s->cfl * dtmin, s->max_dt);
+    return s->dt;
+}
+
+// MmFluid::fluxes, CrpMode::NONE (mm_fluid.cpp:373-445 with calc_flux :351-356)
+// + swap; returns 0, or 1 on a NaN flux / negative density (the reference throws).
+int orc_mm_update(orc_mm* s) {
+    g_numeric_error = 0;
+    orc_mm_compute_dt(s);
+    const orc_mesh& m = *s->m;
+    const double dt = s->dt;
+    int neg = 0;
+    for (int c = 0; c < m.n; ++c) {
+        const double V_c = m.vol[c];
+        const MState z_c = s->init[c];
+        MArr flux;
+        for (int k = 0; k < 8; ++k) {
+            const size_t f = 8 * (size_t)c + k;
+            const int b = m.bnd[f];
+            if (b == ORC_UNDEFINED) continue;
+            const Vec3 normal{m.fnx[f], m.fny[f], 0.0};
+            MState z_n;
+            if (!slot_is_boundary(b)) z_n = s->init[m.adj[f]];
+            else z_n = m_boundary_value(z_c, normal, b);
+            const MState z_m = m_in_local(z_c, normal);
+            const MState z_p = m_in_local(z_n, normal);
+            const double S = m.area[f];
+            MArr loc = m_hllc(z_m, z_p, s->mix);
+            const Vec3 mg = rot_global(Vec3{loc.a[1], loc.a[2], loc.a[3]}, normal);   // Flux::to_global
+            loc.a[1] = mg.x; loc.a[2] = mg.y; loc.a[3] = mg.z;
+            for (int q = 0; q < 9; ++q) flux.a[q] += loc.a[q] * S;
+        }
+        MArr q_c = m_qstate(z_c);
+        const double coef = dt / V_c;
+        for (int q = 0; q < 9; ++q) q_c.a[q] -= coef * flux.a[q];
+        if (q_c.a[0] < 0.0) neg = 1;
+        s->next[c] = m_from_q(q_c, s->mix, z_c.P, z_c.T, z_c.rhos);
+    }
+    std::swap(s->init, s->next);   // MmFluid::swap (mm_fluid.cpp:682-684)
+    return (g_numeric_error || neg) ? 1 : 0;
+}
+
+double orc_mm_dt(const orc_mm* s) { return s->dt; }
+
+// MmFluid::interface_recovery (mm_fluid.cpp:484-519): normals nx/ny and
+// section positions p per material [mat][n] (zero for pure cells and
+// absent materials).
+void orc_mm_interface(orc_mm* s, double* nx, double* ny, double* p) {
+    const orc_mesh& m = *s->m;
+    const int n = m.n, nm = s->mix.n;
+    for (int c = 0; c < n; ++c) {
+        Vec3 ns[kMaxMat];
+        double ps[kMaxMat];
+        for (int i = 0; i < nm; ++i) {
+            ns[i] = {0.0, 0.0, 0.0};
+            ps[i] = 0.0;
+        }
+        const MState& z = s->init[c];
+        if (!z.beta.is_pure()) {
+            const Frac a_c = m_volume_fractions(z);
+            for (int k = 0; k < 8; ++k) {
+                const size_t f = 8 * (size_t)c + k;
+                const int b = m.bnd[f];
+                if (b == ORC_UNDEFINED) continue;
+                const Vec3 S = scale(m.area[f], Vec3{m.fnx[f], m.fny[f], 0.0});
+                const MState& zn = slot_is_boundary(b) ? z : s->init[m.adj[f]];   // neib() = the cell at boundaries
+                for (int i = 0; i < nm; ++i)
+                    if (a_c.has(i)) {
+                        const double a_n = m_alpha(zn, i);
+                        const Vec3 t = scale(face_fraction(a_c.a[i], a_n), S);
+                        ns[i] = {ns[i].x - t.x, ns[i].y - t.y, ns[i].z - t.z};
+                    }
+            }
+            for (int i = 0; i < nm; ++i)
+                if (a_c.has(i)) {
+                    const double zz = sqnorm(ns[i]);   // Vector3d::normalize
+                    if (zz > 0.0) {
+                        const double r = std::sqrt(zz);
+                        ns[i] = {ns[i].x / r, ns[i].y / r, ns[i].z / r};
+                    }
+                    ps[i] = quad_find_section(a_c.a[i], ns[i].x, ns[i].y, cell_hx(m, c), cell_hy(m, c));
+                }
+        }
+        for (int i = 0; i < nm; ++i) {
+            nx[(size_t)i * n + c] = ns[i].x;
+            ny[(size_t)i * n + c] = ns[i].y;
+            p[(size_t)i * n + c] = ps[i];
+        }
+    }
+}
+
+// Scalar entry points for the known-answer tests.
+void orc_mm_closure(int n_mat, const double* mats, double rho, double e, const double* beta, double P0, double T0,
+                    double* out /* P, T, rhos[4] */) {
+    Mixture mx;
+    mx.n = n_mat;
+    for (int i = 0; i < n_mat; ++i)
+        mx.m[i] = SgMat{mats[6 * i], mats[6 * i + 1], mats[6 * i + 2], mats[6 * i + 3], mats[6 * i + 4], mats[6 * i + 5]};
+    Frac b;
+    for (int i = 0; i < n_mat; ++i) b.a[i] = beta[i];
+    MixOpt o;
+    o.P0 = P0;
+    o.T0 = T0;
+    get_rPT(mx, rho, e, b, o, out + 2, out[0], out[1]);
+}
+
+void orc_mm_closure_rP(int n_mat, const double* mats, double rho, double P, const double* beta,
+                       double* out /* e, T, c, rhos[4] */) {
+    Mixture mx;
+    mx.n = n_mat;
+    for (int i = 0; i < n_mat; ++i)
+        mx.m[i] = SgMat{mats[6 * i], mats[6 * i + 1], mats[6 * i + 2], mats[6 * i + 3], mats[6 * i + 4], mats[6 * i + 5]};
+    Frac b;
+    for (int i = 0; i < n_mat; ++i) b.a[i] = beta[i];
+    MixOpt o;
+    get_reT(mx, rho, P, b, o, out + 3, out[0], out[1]);
+    MixOpt oc;
+    oc.T0 = out[1];
+    oc.rhos = out + 3;
+    out[2] = mix_sound_speed_rP(mx, rho, P, b, oc);
+}
+
+// mmf HLLC between two cells of an orc_mm state (face frame = normal (nx, ny)).
+void orc_mm_face_flux(const orc_mm* s, int cL, int cR, double nx, double ny, double* F) {
+    const Vec3 n{nx, ny, 0.0};
+    MArr f = m_hllc(m_in_local(s->init[cL], n), m_in_local(s->init[cR], n), s->mix);
+    for (int k = 0; k < 9; ++k) F[k] = f.a[k];
+}
+
+double orc_quad_volume_fraction(double p, double nx, double ny, double A, double B) {
+    return quad_volume_fraction(p, nx, ny, A, B);
+}
+double orc_quad_find_section(double alpha, double nx, double ny, double A, double B) {
+    return quad_find_section(alpha, nx, ny, A, B);
+}
+double orc_face_fraction(int kind, double a1, double a2) {
+    switch (kind) {
+        case 3: return face_fraction_v3(a1, a2);
+        case 5: return face_fraction_v5(a1, a2);
+        case 1: return face_fraction_s(a1, a2);
+        default: return face_fraction(a1, a2);
+    }
+}
+double orc_average_flux(double alpha, double cs, double CFL) { return average_flux(alpha, cs, CFL); }
+
+}  // extern "C"

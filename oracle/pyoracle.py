@@ -72,6 +72,22 @@ def lib():
             "orc_mesh_get_next": (None, [_P, _ip]),
             "orc_splitmix_uniform": (_D, [C.POINTER(C.c_uint64)]),
             "orc_init_quadrants": (None, [_P, C.c_uint64, _D, _dp, _dp, _dp, _dp, _dp]),
+            "orc_mm_create": (_P, [_P, _I, _dp, _D]),
+            "orc_mm_free": (None, [_P]),
+            "orc_mm_set_max_dt": (None, [_P, _D]),
+            "orc_mm_set_state": (None, [_P, _dp, _dp, _dp, _dp, _dp]),
+            "orc_mm_state": (None, [_P] + [_dp] * 8),
+            "orc_mm_compute_dt": (_D, [_P]),
+            "orc_mm_update": (_I, [_P]),
+            "orc_mm_dt": (_D, [_P]),
+            "orc_mm_interface": (None, [_P, _dp, _dp, _dp]),
+            "orc_mm_closure": (None, [_I, _dp, _D, _D, _dp, _D, _D, _dp]),
+            "orc_mm_closure_rP": (None, [_I, _dp, _D, _D, _dp, _dp]),
+            "orc_mm_face_flux": (None, [_P, _I, _I, _D, _D, _dp]),
+            "orc_quad_volume_fraction": (_D, [_D] * 5),
+            "orc_quad_find_section": (_D, [_D] * 5),
+            "orc_face_fraction": (_D, [_I, _D, _D]),
+            "orc_average_flux": (_D, [_D] * 3),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -267,3 +283,100 @@ class SmFluid:
         out = np.zeros(4)
         lib().orc_smf_totals(self.h, _d(out))
         return out
+
+
+# ---- multi-material solver (MmFluid, first order, CrpMode::NONE) -----------
+def _mats(materials):
+    return np.ascontiguousarray(np.asarray(materials, dtype=np.float64).reshape(-1, 6))
+
+
+class MmFluid:
+    """Oracle MmFluid (math/solver/mm_fluid.h:45-163 restated, first order,
+    no composite Riemann problems); materials: (gamma, P0, e0, Cv, T0, rho_ref)."""
+
+    def __init__(self, mesh, materials, cfl=0.7):
+        self.mesh = mesh
+        self.mats = _mats(materials)
+        self.nm = self.mats.shape[0]
+        self.h = lib().orc_mm_create(mesh.h, self.nm, _d(self.mats), cfl)
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.orc_mm_free(self.h)
+            self.h = None
+
+    def set_max_dt(self, v):
+        lib().orc_mm_set_max_dt(self.h, v)
+
+    def set_state(self, rho, u, v, p, beta):
+        arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (rho, u, v, p)]
+        b = np.ascontiguousarray(np.asarray(beta, dtype=np.float64).reshape(self.nm, -1))
+        lib().orc_mm_set_state(self.h, *[_d(a) for a in arrs], _d(b))
+
+    def state(self):
+        n, nm = self.mesh.n, self.nm
+        out = {k: np.zeros(n) for k in ("rho", "u", "v", "P", "e", "T")}
+        out["beta"] = np.zeros((nm, n))
+        out["rhos"] = np.zeros((nm, n))
+        lib().orc_mm_state(self.h, *[_d(out[k]) for k in ("rho", "u", "v", "P", "e", "T", "beta", "rhos")])
+        return out
+
+    def compute_dt(self):
+        return lib().orc_mm_compute_dt(self.h)
+
+    def update(self):
+        """One MmFluid::update; returns 1 on a NaN flux or negative density."""
+        return lib().orc_mm_update(self.h)
+
+    def dt(self):
+        return lib().orc_mm_dt(self.h)
+
+    def interface(self):
+        nm, n = self.nm, self.mesh.n
+        nx, ny, p = np.zeros((nm, n)), np.zeros((nm, n)), np.zeros((nm, n))
+        lib().orc_mm_interface(self.h, _d(nx), _d(ny), _d(p))
+        return nx, ny, p
+
+    def face_flux(self, cL, cR, nx, ny):
+        F = np.zeros(9)
+        lib().orc_mm_face_flux(self.h, cL, cR, nx, ny, _d(F))
+        return F
+
+
+def mm_closure(materials, rho, e, beta, P0=float("nan"), T0=float("nan")):
+    """MixturePT::get_rPT: (P, T, rhos[4])."""
+    m = _mats(materials)
+    b = np.zeros(4)
+    b[:m.shape[0]] = beta
+    out = np.zeros(6)
+    lib().orc_mm_closure(m.shape[0], _d(m), rho, e, _d(b), P0, T0, _d(out))
+    return out[0], out[1], out[2:]
+
+
+def mm_closure_rP(materials, rho, P, beta):
+    """MixturePT::get_reT and sound_speed_rP: (e, T, c, rhos[4])."""
+    m = _mats(materials)
+    b = np.zeros(4)
+    b[:m.shape[0]] = beta
+    out = np.zeros(7)
+    lib().orc_mm_closure_rP(m.shape[0], _d(m), rho, P, _d(b), _d(out))
+    return out[0], out[1], out[2], out[3:]
+
+
+def quad_volume_fraction(p, nx, ny, A, B):
+    return lib().orc_quad_volume_fraction(p, nx, ny, A, B)
+
+
+def quad_find_section(alpha, nx, ny, A, B):
+    return lib().orc_quad_find_section(alpha, nx, ny, A, B)
+
+
+FACE_FRACTION = {"plic": 0, "s": 1, "v3": 3, "v5": 5}
+
+
+def face_fraction(a1, a2, kind="plic"):
+    return lib().orc_face_fraction(FACE_FRACTION[kind], a1, a2)
+
+
+def average_flux(alpha, cos, cfl):
+    return lib().orc_average_flux(alpha, cos, cfl)

@@ -32,6 +32,8 @@ EXPORTS = [
     "zfv_mesh_n_faces", "zfv_set_launch", "zfv_get_launch", "zfv_autotune",
     "zfv_batch_rectangles", "zfv_batch_layout", "zfv_batch_dt", "zfv_graph_enable",
     "zfv_adjoint_forward", "zfv_adjoint_backward",
+    "zfv_mm_configure", "zfv_mm_set_max_dt", "zfv_mm_set_state", "zfv_mm_state", "zfv_mm_step", "zfv_mm_dt",
+    "zfv_mm_interface",
 ]
 CELL, QUAD = 0, 1   # step-kernel layouts
 
@@ -93,6 +95,13 @@ def lib():
         L.zfv_autotune.argtypes = [vp, C.c_int, vp]
         L.zfv_batch_rectangles.argtypes = [vp, C.c_int, vp, vp, vp, vp]
         L.zfv_graph_enable.argtypes = [vp, C.c_int]
+        L.zfv_mm_configure.argtypes = [vp, C.c_int, vp, C.c_double]
+        L.zfv_mm_set_max_dt.argtypes = [vp, C.c_double]
+        L.zfv_mm_set_state.argtypes = [vp] * 6
+        L.zfv_mm_state.argtypes = [vp] * 9
+        L.zfv_mm_step.argtypes = [vp, C.c_int]
+        L.zfv_mm_dt.argtypes = [vp, vp]
+        L.zfv_mm_interface.argtypes = [vp] * 4
         L.zfv_adjoint_forward.argtypes = [vp, C.c_int, C.c_int, vp, vp]
         L.zfv_adjoint_backward.argtypes = [vp, vp, vp, vp, vp, vp]
         _lib = L
@@ -291,6 +300,46 @@ class Context:
 
     def graph_enable(self, on=True):
         _check(lib().zfv_graph_enable(self.h, 1 if on else 0))
+
+    # ---- multi-material solver (first order, no CRP)
+    def mm_configure(self, materials, cfl=0.7):
+        """materials: sequence of (gamma, P0, e0, Cv, T0, rho_ref)."""
+        mats = np.ascontiguousarray(np.asarray(materials, dtype=np.float64).reshape(-1, 6))
+        self._mm_nmat = mats.shape[0]
+        _check(lib().zfv_mm_configure(self.h, mats.shape[0], _ptr(mats), float(cfl)))
+
+    def mm_set_max_dt(self, v):
+        _check(lib().zfv_mm_set_max_dt(self.h, float(v)))
+
+    def mm_set_state(self, rho, u, v, p, beta):
+        """beta: (n_mat, n) mass fractions."""
+        arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (rho, u, v, p)]
+        b = np.ascontiguousarray(np.asarray(beta, dtype=np.float64).reshape(self._mm_nmat, -1))
+        _check(lib().zfv_mm_set_state(self.h, *[_ptr(a) for a in arrs], _ptr(b)))
+
+    def mm_state(self):
+        """dict of rho, u, v, P, e, T (n) and beta, rhos (n_mat, n)."""
+        n, nm = self.n, self._mm_nmat
+        out = {k: np.zeros(n) for k in ("rho", "u", "v", "P", "e", "T")}
+        out["beta"] = np.zeros((nm, n))
+        out["rhos"] = np.zeros((nm, n))
+        _check(lib().zfv_mm_state(self.h, *[_ptr(out[k]) for k in ("rho", "u", "v", "P", "e", "T", "beta", "rhos")]))
+        return out
+
+    def mm_step(self, n=1):
+        _check(lib().zfv_mm_step(self.h, n))
+
+    def mm_dt(self):
+        d = C.c_double()
+        _check(lib().zfv_mm_dt(self.h, C.byref(d)))
+        return d.value
+
+    def mm_interface(self):
+        """PLIC normals and section positions, (n_mat, n) each."""
+        nm, n = self._mm_nmat, self.n
+        nx, ny, p = np.zeros((nm, n)), np.zeros((nm, n)), np.zeros((nm, n))
+        _check(lib().zfv_mm_interface(self.h, _ptr(nx), _ptr(ny), _ptr(p)))
+        return nx, ny, p
 
     # ---- discrete adjoint
     def adjoint_forward(self, n_steps, every, window):

@@ -17,12 +17,15 @@ SOURCES = [
     os.path.join(HERE, "csrc", "zfv_kernels.hip"),
     os.path.join(HERE, "csrc", "zfv_amr.hip"),
     os.path.join(HERE, "csrc", "zfv_adjoint.hip"),
+    os.path.join(HERE, "csrc", "zfv_mm.hip"),
     os.path.join(HERE, "csrc", "zfv_capi.cpp"),
 ]
 HEADERS = [
     os.path.join(HERE, "csrc", "zfv_physics.hpp"),
     os.path.join(HERE, "csrc", "zfv_internal.hpp"),
     os.path.join(HERE, "csrc", "zfv_adjoint_ops.hpp"),
+    os.path.join(HERE, "csrc", "zfv_step_common.hpp"),
+    os.path.join(HERE, "csrc", "zfv_mm.hpp"),
     os.path.join(ROOT, "include", "zfv.h"),
 ]
 

@@ -157,6 +157,20 @@ struct zfv_ctx {
         DBuf<double> lam[2], lz, W, lG, ZB, NG, dtb;
     } adjoint;
 
+    // multi-material solver (MmFluid, first order, no CRP): two state layers
+    // of (7 + 2 n_mat) rows of cap doubles, its own dt slots and step count
+    struct MmSolver {
+        bool configured = false, have_state = false;
+        long long epoch = -1;   // mesh epoch of the state
+        mm::Mix mix{};
+        double cfl = 0.7, max_dt = 1.0e300;   // MmFluid defaults (mm_fluid.cpp:16-26)
+        int cur = 0;
+        long long step_no = 0;
+        DBuf<double> st[2], dmax_dt, dt_out, inter;
+        DBuf<unsigned long long> dtmin;
+    } mm;
+    long long mesh_epoch = 0;   // incremented whenever the mesh changes
+
     bool profile = false;
     std::vector<KernelTimer> timers;   // pending event pairs of the last zfv_smf_step
     std::vector<hipEvent_t> event_pool;
@@ -228,6 +242,7 @@ double incircle_2d(const double* X, const double* Y) {
 
 // Allocate device buffers and copy the validated host mesh.
 int upload_mesh(zfv_ctx* ctx) {
+    ctx->mesh_epoch += 1;
     const HostMesh& m = ctx->hm;
     const int n = m.n;
     const size_t ns = 8 * (size_t)n;
@@ -1269,6 +1284,7 @@ int zfv_refine(zfv_ctx* ctx, int max_level, int distributor) {
     ctx->faces_pending = true;
     ctx->dt_valid = true;
     ctx->graphs_clear();   // the cell count changed
+    ctx->mesh_epoch += 1;
     return ZFV_OK;
 }
 
@@ -1451,6 +1467,171 @@ int zfv_adjoint_backward(zfv_ctx* ctx, double* lr, double* lu, double* lv, doubl
     ZFV_HIP(hipMemcpy(&e, ctx->err.p, sizeof(int), hipMemcpyDeviceToHost));
     if (e & (kErrRobust | kErrIsolated))
         return fail(ZFV_ERR_UNSUPPORTED, "adjoint: a cell was isolated (bad state) during the run");
+    return read_err(ctx);
+}
+
+// ---- multi-material solver ----------------------------------------------------
+static MmLayer mm_layer(zfv_ctx* ctx, int i) {
+    const size_t cap = (size_t)ctx->cap;
+    double* p = ctx->mm.st[i].p;
+    const int nm = ctx->mm.mix.n;
+    return MmLayer{p, p + cap, p + 2 * cap, p + 3 * cap, p + 4 * cap, p + 5 * cap, p + 6 * cap, p + 7 * cap,
+                   p + (7 + (size_t)nm) * cap};
+}
+
+static int mm_ready(zfv_ctx* ctx, bool state) {
+    int r = need_ready(ctx, false, false);
+    if (r) return r;
+    if (!ctx->mm.configured) return fail(ZFV_ERR_STATE, "multi-material solver not configured");
+    if (ctx->nseg != 1) return fail(ZFV_ERR_UNSUPPORTED, "multi-material solver on a batch");
+    if (state && (!ctx->mm.have_state || ctx->mm.epoch != ctx->mesh_epoch))
+        return fail(ZFV_ERR_STATE, "no multi-material state for the current mesh");
+    return ZFV_OK;
+}
+
+static int mm_alloc(zfv_ctx* ctx) {
+    auto& S = ctx->mm;
+    const size_t rows = 7 + 2 * (size_t)S.mix.n;
+    for (auto& b : S.st) ZFV_HIP(b.alloc(rows * (size_t)ctx->cap));
+    ZFV_HIP(S.dtmin.alloc(3));
+    ZFV_HIP(S.dt_out.alloc(1));
+    ZFV_HIP(S.dmax_dt.alloc(1));
+    unsigned long long bits;
+    std::memcpy(&bits, &S.max_dt, 8);
+    ZFV_HIP(launch_fill_u64(reinterpret_cast<unsigned long long*>(S.dmax_dt.p), 1, bits, ctx->stream));
+    return ZFV_OK;
+}
+
+int zfv_mm_configure(zfv_ctx* ctx, int n_mat, const double* mats, double cfl) {
+    int r = need_ready(ctx, false, false);
+    if (r) return r;
+    if (n_mat < 1 || n_mat > mm::kMaxMat || !mats) return fail(ZFV_ERR_ARG, "mm_configure: 1..4 materials");
+    auto& S = ctx->mm;
+    S.mix = mm::Mix{};
+    S.mix.n = n_mat;
+    for (int i = 0; i < n_mat; ++i) {
+        const double* m = mats + 6 * i;
+        if (!(m[0] > 1.0) || !(m[3] > 0.0) || !(m[5] > 0.0))
+            return fail(ZFV_ERR_ARG, "mm_configure: material needs gamma > 1, Cv > 0, reference density > 0");
+        S.mix.m[i] = mm::Mat{m[0], m[1], m[2], m[3], m[4], m[5]};
+    }
+    S.cfl = std::max(0.0, std::min(cfl, 1.0));   // set_CFL (mm_fluid.cpp:41-43)
+    S.have_state = false;
+    S.configured = true;
+    return mm_alloc(ctx);
+}
+
+int zfv_mm_set_max_dt(zfv_ctx* ctx, double max_dt) {
+    int r = mm_ready(ctx, false);
+    if (r) return r;
+    ctx->mm.max_dt = max_dt;
+    unsigned long long bits;
+    std::memcpy(&bits, &max_dt, 8);
+    ZFV_HIP(launch_fill_u64(reinterpret_cast<unsigned long long*>(ctx->mm.dmax_dt.p), 1, bits, ctx->stream));
+    return ZFV_OK;
+}
+
+int zfv_mm_set_state(zfv_ctx* ctx, const double* rho, const double* u, const double* v, const double* P,
+                     const double* beta) {
+    int r = mm_ready(ctx, false);
+    if (r) return r;
+    if (!rho || !u || !v || !P || !beta) return fail(ZFV_ERR_ARG, "mm_set_state: null array");
+    auto& S = ctx->mm;
+    if (S.st[0].n != (7 + 2 * (size_t)S.mix.n) * (size_t)ctx->cap) {
+        r = mm_alloc(ctx);   // the mesh grew since configure
+        if (r) return r;
+    }
+    const int n = ctx->hm.n;
+    const size_t b = (size_t)n * sizeof(double);
+    const MmLayer L = mm_layer(ctx, S.cur);
+    hipStream_t s = ctx->stream;
+    ZFV_HIP(hipMemcpyAsync(L.r, rho, b, hipMemcpyHostToDevice, s));
+    ZFV_HIP(hipMemcpyAsync(L.u, u, b, hipMemcpyHostToDevice, s));
+    ZFV_HIP(hipMemcpyAsync(L.v, v, b, hipMemcpyHostToDevice, s));
+    ZFV_HIP(hipMemcpyAsync(L.P, P, b, hipMemcpyHostToDevice, s));
+    for (int i = 0; i < S.mix.n; ++i)
+        ZFV_HIP(hipMemcpyAsync(L.b + (size_t)i * ctx->cap, beta + (size_t)i * n, b, hipMemcpyHostToDevice, s));
+    ZFV_HIP(launch_fill_u64(S.dtmin.p, 3, 0x7ff0000000000000ULL, s));
+    ZFV_HIP(mm_prepare(ctx->dmesh(), L, S.mix, S.dtmin.p + (S.step_no % 3), s));
+    ZFV_HIP(hipStreamSynchronize(s));
+    S.have_state = true;
+    S.epoch = ctx->mesh_epoch;
+    return read_err(ctx);
+}
+
+int zfv_mm_state(zfv_ctx* ctx, double* rho, double* u, double* v, double* P, double* e, double* T, double* beta,
+                 double* rhos) {
+    int r = mm_ready(ctx, true);
+    if (r) return r;
+    if (!rho || !u || !v || !P || !e || !T || !beta || !rhos) return fail(ZFV_ERR_ARG, "mm_state: null array");
+    auto& S = ctx->mm;
+    const int n = ctx->hm.n;
+    const size_t b = (size_t)n * sizeof(double);
+    const MmLayer L = mm_layer(ctx, S.cur);
+    hipStream_t s = ctx->stream;
+    double* const dst[6] = {rho, u, v, P, e, T};
+    const double* const src[6] = {L.r, L.u, L.v, L.P, L.e, L.T};
+    for (int k = 0; k < 6; ++k) ZFV_HIP(hipMemcpyAsync(dst[k], src[k], b, hipMemcpyDeviceToHost, s));
+    for (int i = 0; i < S.mix.n; ++i) {
+        ZFV_HIP(hipMemcpyAsync(beta + (size_t)i * n, L.b + (size_t)i * ctx->cap, b, hipMemcpyDeviceToHost, s));
+        ZFV_HIP(hipMemcpyAsync(rhos + (size_t)i * n, L.rh + (size_t)i * ctx->cap, b, hipMemcpyDeviceToHost, s));
+    }
+    ZFV_HIP(hipStreamSynchronize(s));
+    return read_err(ctx);
+}
+
+int zfv_mm_step(zfv_ctx* ctx, int n_steps) {
+    int r = mm_ready(ctx, true);
+    if (r) return r;
+    if (n_steps < 0) return fail(ZFV_ERR_ARG, "negative step count");
+    auto& S = ctx->mm;
+    const DevMesh M = ctx->dmesh();
+    for (int k = 0; k < n_steps; ++k) {
+        StepCtl c;
+        c.dtmin = S.dtmin.p;
+        c.rd = (int)(S.step_no % 3);
+        c.wr = (int)((S.step_no + 1) % 3);
+        c.clr = (int)((S.step_no + 2) % 3);
+        c.cfl = S.cfl;
+        c.max_dt = S.dmax_dt.p;
+        c.dt_out = S.dt_out.p;
+        c.err = ctx->err.p;
+        const MmLayer in = mm_layer(ctx, S.cur), out = mm_layer(ctx, 1 - S.cur);
+        ZFV_HIP(timed(ctx, "mm_flux1", [&] { return mm_flux1(M, in, out, S.mix, c, ctx->stream); }));
+        S.cur = 1 - S.cur;
+        S.step_no += 1;
+    }
+    return ZFV_OK;
+}
+
+int zfv_mm_dt(zfv_ctx* ctx, double* dt) {
+    int r = mm_ready(ctx, true);
+    if (r) return r;
+    if (!dt) return fail(ZFV_ERR_ARG, "null output");
+    ZFV_HIP(hipMemcpyAsync(dt, ctx->mm.dt_out.p, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    ZFV_HIP(hipStreamSynchronize(ctx->stream));
+    return read_err(ctx);
+}
+
+int zfv_mm_interface(zfv_ctx* ctx, double* nx, double* ny, double* p) {
+    int r = mm_ready(ctx, true);
+    if (r) return r;
+    if (!nx || !ny || !p) return fail(ZFV_ERR_ARG, "mm_interface: null array");
+    auto& S = ctx->mm;
+    const size_t cap = (size_t)ctx->cap, nm = (size_t)S.mix.n;
+    if (S.inter.n != 3 * nm * cap) ZFV_HIP(S.inter.alloc(3 * nm * cap));
+    double* o = S.inter.p;
+    hipStream_t s = ctx->stream;
+    ZFV_HIP(mm_interface(ctx->dmesh(), mm_layer(ctx, S.cur), S.mix, ctx->vx.p, ctx->vy.p, o, o + nm * cap,
+                         o + 2 * nm * cap, s));
+    const int n = ctx->hm.n;
+    const size_t b = (size_t)n * sizeof(double);
+    for (size_t i = 0; i < nm; ++i) {
+        ZFV_HIP(hipMemcpyAsync(nx + i * n, o + i * cap, b, hipMemcpyDeviceToHost, s));
+        ZFV_HIP(hipMemcpyAsync(ny + i * n, o + (nm + i) * cap, b, hipMemcpyDeviceToHost, s));
+        ZFV_HIP(hipMemcpyAsync(p + i * n, o + (2 * nm + i) * cap, b, hipMemcpyDeviceToHost, s));
+    }
+    ZFV_HIP(hipStreamSynchronize(s));
     return read_err(ctx);
 }
 

@@ -5,6 +5,7 @@
 #include <cstdint>
 
 #include "zfv_physics.hpp"
+#include "zfv_mm.hpp"
 
 namespace zfv {
 
@@ -137,6 +138,19 @@ hipError_t adj_step(int limiter, const DevMesh& M, const DevState& z, const DevS
 hipError_t adj_seed(const DevMesh& M, const DevState& z, const double* win, const DevState& lam, hipStream_t s);
 
 constexpr int kBlock = 256;
+
+// Multi-material state layer (zfv_mm.hip): per cell rho, u, v, P, e, T, the
+// cached sound speed c, and per material [mat][stride] mass fractions b and
+// true densities rh (NaN for an absent material).
+struct MmLayer {
+    double *r, *u, *v, *P, *e, *T, *c, *b, *rh;
+};
+hipError_t mm_prepare(const DevMesh& M, const MmLayer& L, const mm::Mix& mx, unsigned long long* slot,
+                      hipStream_t s);
+hipError_t mm_flux1(const DevMesh& M, const MmLayer& in, const MmLayer& out, const mm::Mix& mx,
+                    const StepCtl& ctl, hipStream_t s);
+hipError_t mm_interface(const DevMesh& M, const MmLayer& L, const mm::Mix& mx, const double* vx, const double* vy,
+                        double* nx, double* ny, double* p, hipStream_t s);
 
 // Error-word bits: 1 numeric, 2 inconsistent mesh (errors); kErrRobust and
 // kErrIsolated are notices for the adjoint (a cell in the robust isolation

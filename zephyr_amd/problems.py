@@ -112,3 +112,47 @@ def circle_blast(cx, cy, seed, gamma=1.4, radius=0.2):
     p = np.where(inside, 1.0, 0.1)
     z = np.zeros_like(rho)
     return rho, z, z.copy(), p, energy_rP(rho, p, gamma)
+
+
+# ---- two-material shock-bubble (stiffened-gas water with an air bubble) ----
+WATER = (4.4, 6.0e8, 0.0, 4160.0, 230.7, 1000.0)   # StiffenedGas("Water") (stiffened_gas.cpp:33-40)
+AIR = (1.4, 0.0, 0.0, 718.0, 0.0, 1.25)            # StiffenedGas("Air") (stiffened_gas.cpp:22-25)
+
+
+def mm_bubble_shock_params(seed):
+    """Seeded shock Mach number M ~ U[1.5, 3]."""
+    g = SplitMix64(seed)
+    return 1.5 + 1.5 * g.uniform()
+
+
+def mm_bubble_shock(cx, cy, h, seed, radius=0.15, centre=(0.5, 0.5), x_shock=0.2, p1=1.0e5, sub=4):
+    """Water at rest (rho 1000, P = p1) with an air bubble (rho 1.25) of the
+    given radius; a planar shock of seeded Mach number at x = x_shock moves
+    into the water along +x, the post-shock water state from the
+    stiffened-gas Rankine-Hugoniot relations (P + P0 behaves as an ideal-gas
+    pressure).  Cells cut by the bubble get the volume fraction of a sub x sub
+    sampling, mass fractions beta_i = alpha_i rho_i / rho.  Returns rho, u, v,
+    P and beta (2, n): material 0 water, 1 air."""
+    M = mm_bubble_shock_params(seed)
+    gw, P0w = WATER[0], WATER[1]
+    rw, ra = WATER[5], AIR[5]
+    pt1 = p1 + P0w
+    c1 = np.sqrt(gw * pt1 / rw)
+    D = M * c1
+    r2 = rw * (gw + 1.0) * M * M / ((gw - 1.0) * M * M + 2.0)
+    p2 = pt1 * (2.0 * gw * M * M - (gw - 1.0)) / (gw + 1.0) - P0w
+    u2 = D * (1.0 - rw / r2)
+    n = len(cx)
+    offs = (np.arange(sub) + 0.5) / sub - 0.5
+    alpha = np.zeros(n)
+    for ox in offs:
+        for oy in offs:
+            alpha += ((cx + ox * h - centre[0]) ** 2 + (cy + oy * h - centre[1]) ** 2 < radius * radius)
+    alpha /= sub * sub
+    shocked = cx < x_shock
+    rho_w = np.where(shocked, r2, rw)
+    rho = alpha * ra + (1.0 - alpha) * rho_w
+    b_air = alpha * ra / rho
+    u = np.where(shocked, u2, 0.0)
+    P = np.where(shocked, p2, p1)
+    return rho, u, np.zeros(n), P, np.stack([1.0 - b_air, b_air])
