@@ -205,10 +205,11 @@ int zfv_adjoint_forward(zfv_ctx* ctx, int n_steps, int every, const double* wind
 int zfv_adjoint_backward(zfv_ctx* ctx, double* d_rho, double* d_u, double* d_v, double* d_p, double* d_e);
 
 /* ---- multi-material solver (MmFluid, mm_fluid.h:45-163) ------------------- */
-/* First-order MmFluid step without composite Riemann problems (the
- * CrpMode::NONE branch of MmFluid::calc_flux, mm_fluid.cpp:351-356, with the
- * default HLLC: the mmf HLLC flux on every face), on the context's mesh
- * (one problem, uniform or adapted).  Materials are stiffened gases with a
+/* First-order MmFluid step on the context's mesh (one problem, uniform or
+ * adapted): the mmf HLLC flux between pure states and, with composite
+ * Riemann problems on (CrpMode::PLIC, the MmFluid default), the CRP flux of
+ * faces next to mixed cells (MmFluid::calc_flux, mm_fluid.cpp:339-371, with
+ * interface_recovery each step).  Materials are stiffened gases with a
  * temperature; mats = n_mat x {gamma, P0, e0, Cv, T0, rho_ref} (StiffenedGas
  * parameters, stiffened_gas.cpp:11-12, and the reference density that seeds
  * the closure, eos.h:152); the mixture is in pressure and temperature
@@ -216,6 +217,10 @@ int zfv_adjoint_backward(zfv_ctx* ctx, double* d_rho, double* d_u, double* d_v, 
  * (MmFluid::set_CFL, mm_fluid.cpp:41-43; the reference default is 0.7).
  * Per-material arrays are [n_mat][n]. */
 int zfv_mm_configure(zfv_ctx* ctx, int n_mat, const double* mats, double cfl);
+/* Composite Riemann problems at mixed faces: 1 = CrpMode::PLIC (default),
+ * 0 = CrpMode::NONE (HLLC everywhere; MmFluid::set_crp_mode / set_method,
+ * mm_fluid.cpp:45-70). */
+int zfv_mm_set_crp(zfv_ctx* ctx, int mode);
 /* MmFluid::set_max_dt (mm_fluid.cpp:94-96). */
 int zfv_mm_set_max_dt(zfv_ctx* ctx, double max_dt);
 /* Cells from {rho, u, v, P, beta}: mmf::PState(rho, v, P, Fractions, mixture)
@@ -231,6 +236,9 @@ int zfv_mm_state(zfv_ctx* ctx, double* rho, double* u, double* v, double* P, dou
 int zfv_mm_step(zfv_ctx* ctx, int n_steps);
 /* dt of the last step (MmFluid::dt). */
 int zfv_mm_dt(zfv_ctx* ctx, double* dt);
+/* Faces (cell sides) that took a composite Riemann flux in the last step
+ * (0 without composite Riemann problems). */
+int zfv_mm_crp_faces(zfv_ctx* ctx, int64_t* n);
 /* MmFluid::interface_recovery (mm_fluid.cpp:484-519) of the current state:
  * per material the PLIC normal (nx, ny) and section position p,
  * [n_mat][n] each; zero for pure cells and absent materials. */

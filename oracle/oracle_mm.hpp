@@ -180,7 +180,7 @@ inline void find_reT_rP(const Mixture& mx, double rho, double P, const Frac& b, 
 // find_rPT_old (mixture_pt.cpp:448-533): Newton on (P, T) for
 // v_mix = sum beta_i v_i(P, T), e_mix = sum beta_i e_i(P, T).
 inline void find_rPT(const Mixture& mx, double rho, double e_mix, const Frac& b, const MixOpt& o, double* rhos,
-                     double& P, double& T) {
+                     double& P, double& T, DPDT* v_last = nullptr, DPDT* e_last = nullptr) {
     const double v_mix = 1.0 / rho;
     P = std::isnan(o.P0) ? 1.0e5 : o.P0;
     T = std::isnan(o.T0) ? 300.0 : o.T0;
@@ -223,6 +223,8 @@ inline void find_rPT(const Mixture& mx, double rho, double e_mix, const Frac& b,
         T += dT_lim;
         ++counter;
     }
+    if (v_last) *v_last = v;
+    if (e_last) *e_last = e;
 }
 
 // get_rPT (mixture_pt.cpp:199-212): pure material closed form, else find_rPT.
@@ -504,6 +506,331 @@ double average_flux(double alpha, double cs, double CFL) {   // sections.cpp:125
     return f_heav(P2);
 }
 
+// sound_speed_re of the mixture (mixture_pt.cpp:86-99): pure material
+// sqrt(g (g - 1) ((e - e0) - P0 / rho)) (stiffened_gas.cpp:144-146), else
+// find_rPT with derivatives (P.dE = inv_D v.dT, P.dR = inv_D e.dT v^2,
+// mixture_pt.cpp:525-529) and c^2 = P.dR + P P.dE / rho^2.
+inline double mix_sound_speed_re(const Mixture& mx, double rho, double e, const Frac& b) {
+    const int idx = b.index();
+    if (idx >= 0) {
+        const SgMat& m = mx.m[idx];
+        return std::sqrt(m.g * (m.g - 1.0) * ((e - m.e0) - m.P0 / rho));
+    }
+    MixOpt o;
+    o.deriv = true;
+    double rhos[kMaxMat], P, T;
+    DPDT v, ee;
+    find_rPT(mx, rho, e, b, o, rhos, P, T, &v, &ee);
+    const double inv_D = 1.0 / (v.dT * ee.dP - v.dP * ee.dT);
+    const double PdE = inv_D * v.dT;
+    const double PdR = inv_D * ee.dT * (v.val * v.val);
+    const double c2 = PdR + P * PdE / (rho * rho);
+    return std::sqrt(c2);
+}
+
+// mmf::WaveConfig3 (hllc.h:21-27).
+struct Wave {
+    double SL, SC, SR;
+    MArr QsL, FsL, QsR, FsR;
+};
+
+// HLLC::wave_config from conserved vectors and fluxes (hllc.cpp:256-335).
+inline Wave wave_config(const Mixture& mx, const MArr& QL, const MArr& FL, const MArr& QR, const MArr& FR) {
+    const Vec3 vL{QL.a[1] / QL.a[0], QL.a[2] / QL.a[0], QL.a[3] / QL.a[0]};
+    const Vec3 vR{QR.a[1] / QR.a[0], QR.a[2] / QR.a[0], QR.a[3] / QR.a[0]};
+    const double eL = QL.a[4] / QL.a[0] - 0.5 * sqnorm(vL);
+    const double eR = QR.a[4] / QR.a[0] - 0.5 * sqnorm(vR);
+    Frac bL, bR;
+    for (int i = 0; i < mx.n; ++i) {
+        bL.a[i] = QL.a[5 + i] / QL.a[0];
+        bR.a[i] = QR.a[5 + i] / QR.a[0];
+    }
+    const double cL = mix_sound_speed_re(mx, QL.a[0], eL, bL);
+    const double cR = mix_sound_speed_re(mx, QR.a[0], eR, bR);
+    double SL = smin(vL.x - cL, vR.x - cL);
+    double SR = smax(vL.x + cR, vR.x + cR);
+    if (SL >= SR || bL.index() == bR.index()) {
+        SL = smin(vL.x - cL, vR.x - cR);
+        SR = smax(vL.x + cL, vR.x + cR);
+    }
+    MArr aL, aR;
+    for (int k = 0; k < 9; ++k) {
+        aL.a[k] = FL.a[k] - SL * QL.a[k];
+        aR.a[k] = FR.a[k] - SR * QR.a[k];
+    }
+    double SC = (aR.a[1] - aL.a[1]) / (aR.a[0] - aL.a[0]);
+    if (SC < SL) SC = SL + 1.0e-9 * smax(cL, cR);
+    if (SC > SR) SC = SR - 1.0e-9 * smax(cL, cR);
+    const double rsL = aL.a[0] / (SC - SL);
+    const double rsR = aR.a[0] / (SC - SR);
+    const double EsL = SC * SC + (aL.a[4] - SC * aL.a[1]) / aL.a[0];
+    const double EsR = SC * SC + (aR.a[4] - SC * aR.a[1]) / aR.a[0];
+    Wave w;
+    w.SL = SL;
+    w.SC = SC;
+    w.SR = SR;
+    w.QsL.a[0] = rsL;
+    w.QsR.a[0] = rsR;
+    w.QsL.a[1] = rsL * SC; w.QsL.a[2] = rsL * vL.y; w.QsL.a[3] = rsL * vL.z;
+    w.QsR.a[1] = rsR * SC; w.QsR.a[2] = rsR * vR.y; w.QsR.a[3] = rsR * vR.z;
+    w.QsL.a[4] = rsL * EsL;
+    w.QsR.a[4] = rsR * EsR;
+    for (int i = 0; i < mx.n; ++i) {
+        w.QsL.a[5 + i] = rsL * bL.a[i];
+        w.QsR.a[5 + i] = rsR * bR.a[i];
+    }
+    for (int k = 0; k < 9; ++k) {
+        w.FsL.a[k] = aL.a[k] + SL * w.QsL.a[k];
+        w.FsR.a[k] = aR.a[k] + SR * w.QsR.a[k];
+    }
+    return w;
+}
+
+// Characteristics in the (x, t) plane (crp_flux.cpp:13-55).
+struct XT { double x, t; };
+struct Chr {
+    double x, t, S;
+    double edge_t() const {
+        const double ts = (S * t - x) / S;
+        return ts < t ? std::numeric_limits<double>::infinity() : ts;
+    }
+    XT cross(const Chr& c) const {
+        constexpr double inf = std::numeric_limits<double>::infinity();
+        if (S == c.S) return {0.5 * (x + c.x), inf};
+        XT p;
+        p.t = (c.x - x + S * t - c.S * c.t) / (S - c.S);
+        if (p.t <= smax(t, c.t)) return {0.5 * (x + c.x), inf};
+        p.x = 0.5 * (x + c.x + S * (p.t - t) + c.S * (p.t - c.t));
+        return p;
+    }
+};
+
+// F (t1 - t0) - Q (x1 - x0) for the segment from point 0 to point 1.
+inline MArr seg(const MArr& F, const MArr& Q, double dtv, double dxv) {
+    MArr r;
+    for (int k = 0; k < 9; ++k) r.a[k] = F.a[k] * dtv - Q.a[k] * dxv;
+    return r;
+}
+
+// CrpFlux::classic, ZPP version (crp_flux.cpp:234-489): a pure material A
+// at distance delta behind the face inside the left cell's mixture B, and
+// the right state RB; the face flux averaged over dt from the wave pattern.
+// err: set for the reference's "very strange case" (it throws).
+inline MArr crp_classic(const MState& zLA, const MState& zLB, const MState& zRB, const Mixture& mx, double delta,
+                        double dt, int& err) {
+    const MArr qLB = m_qstate(zLB), fLB = m_flux(zLB), Q_R = m_qstate(zRB), F_R = m_flux(zRB);
+    const Wave w0 = wave_config(mx, qLB, fLB, Q_R, F_R);
+    const Chr C_0{-delta, 0.0, zLA.v.x};
+    const Chr C_0L{0.0, 0.0, w0.SL};
+    const XT O1 = C_0.cross(C_0L);
+    const XT O0{0.0, 0.0}, Odt{0.0, dt};
+    auto total2 = [&](const MArr& G1, const MArr& G2) {
+        MArr r;
+        for (int k = 0; k < 9; ++k) r.a[k] = (G1.a[k] + G2.a[k]) / (0.0 - dt);
+        return r;
+    };
+    if (O1.x >= 0.0) {
+        const double t2u = C_0.edge_t();
+        if (dt <= t2u) return fLB;
+        const MArr qLA = m_qstate(zLA), fLA = m_flux(zLA);
+        const Wave w1 = wave_config(mx, qLA, fLA, w0.QsL, w0.FsL);
+        const double lb = -O1.x / (dt - O1.t);
+        MArr F = fLA, Q = qLA;
+        if (dt > O1.t) {
+            if (lb < w1.SL) { F = fLA; Q = qLA; }
+            else if (lb < w1.SC) { F = w1.FsL; Q = w1.QsL; }
+            else if (lb < w1.SR) { F = w1.FsR; Q = w1.QsR; }
+            else { F = w0.FsL; Q = w0.QsL; }
+        }
+        const MArr G_OdtO1 = seg(F, Q, O1.t - Odt.t, O1.x - Odt.x);
+        const MArr G_O1O0 = seg(fLB, qLB, O0.t - O1.t, O0.x - O1.x);
+        return total2(G_OdtO1, G_O1O0);
+    }
+    if (w0.SC >= 0.0) {   // mix to clear
+        if (dt <= O1.t && O1.x < 0.0) return w0.FsL;
+        const MArr Q_L = m_qstate(zLA), F_L = m_flux(zLA);
+        const Wave w1 = wave_config(mx, Q_L, F_L, w0.QsL, w0.FsL);
+        const Chr C_1R{O1.x, O1.t, w1.SR};
+        const double t1 = C_1R.edge_t();
+        if (dt <= t1 && O1.x < 0.0) return w0.FsL;
+        const double lb = -O1.x / (dt - O1.t);
+        const MArr& F = lb < w1.SL ? F_L : (lb < w1.SC ? w1.FsL : w1.FsR);
+        const MArr& Q = lb < w1.SL ? Q_L : (lb < w1.SC ? w1.QsL : w1.QsR);
+        const MArr G_OdtO1 = seg(F, Q, O1.t - Odt.t, O1.x - Odt.x);
+        const MArr G_O1O = seg(w0.FsL, w0.QsL, O0.t - O1.t, O0.x - O1.x);
+        return total2(G_OdtO1, G_O1O);
+    }
+    // clear to mix
+    if (O1.t >= dt) return w0.FsR;
+    const MArr Q_L = m_qstate(zLA), F_L = m_flux(zLA);
+    const Wave w1 = wave_config(mx, Q_L, F_L, w0.QsL, w0.FsL);
+    const Chr C_0C{0.0, 0.0, w0.SC};
+    const Chr C_1R{O1.x, O1.t, w1.SR};
+    const XT O2 = C_1R.cross(C_0C);
+    if (O2.t > dt || O2.x > 0.0) return w0.FsR;
+    const Wave w2 = wave_config(mx, w1.QsR, w1.FsR, w0.QsR, w0.FsR);
+    const Chr C_2L{O2.x, O2.t, w2.SL};
+    const Chr C_1C{O1.x, O1.t, w1.SC};
+    const XT O3 = C_1C.cross(C_2L);
+    if (O3.x >= 0.0) {
+        err = 1;   // crp_flux.cpp:419-421 throws
+        return w0.FsR;
+    }
+    const Wave w3 = wave_config(mx, w1.QsL, w1.FsL, w2.QsL, w2.FsL);
+    const Chr C_2R{O2.x, O2.t, w2.SR};
+    const Chr C_3R{O3.x, O3.t, w3.SR};
+    const double t1 = C_2R.edge_t();
+    const double t4 = C_3R.edge_t();
+    if (dt <= t4) {
+        if (dt <= t1) return w0.FsR;
+        const double lb = -O2.x / (dt - O2.t);
+        const MArr& F = lb < w2.SC ? w2.FsL : (lb < w2.SR ? w2.FsR : w0.FsR);
+        const MArr& Q = lb < w2.SC ? w2.QsL : (lb < w2.SR ? w2.QsR : w0.QsR);
+        const MArr G_OdtO2 = seg(F, Q, O2.t - Odt.t, O2.x - Odt.x);
+        const MArr G_O2O0 = seg(w0.FsR, w0.QsR, O0.t - O2.t, O0.x - O2.x);
+        return total2(G_OdtO2, G_O2O0);
+    }
+    const double lb = -O3.x / (dt - O3.t);
+    const MArr& F = lb < w3.SL ? w1.FsL : (lb < w3.SC ? w3.FsL : w3.FsR);
+    const MArr& Q = lb < w3.SL ? w1.QsL : (lb < w3.SC ? w3.QsL : w3.QsR);
+    const MArr G_OdtO3 = seg(F, Q, O3.t - Odt.t, O3.x - Odt.x);
+    const MArr G_O3O2 = seg(w2.FsL, w2.QsL, O2.t - O3.t, O2.x - O3.x);
+    const MArr G_O2O0 = seg(w0.FsR, w0.QsR, O0.t - O2.t, O0.x - O2.x);
+    MArr r;
+    for (int k = 0; k < 9; ++k) r.a[k] = (G_OdtO3.a[k] + G_O3O2.a[k] + G_O2O0.a[k]) / (0.0 - dt);
+    return r;
+}
+
+// PState::inverse / Flux::inverse (models.cpp:219-221, 506-515).
+inline MState m_inverse(MState z) {
+    z.v.x = -z.v.x;
+    return z;
+}
+inline MArr f_inverse(MArr f) {
+    f.a[0] = -f.a[0];
+    f.a[2] = -f.a[2];
+    f.a[3] = -f.a[3];
+    f.a[4] = -f.a[4];
+    for (int i = 0; i < kMaxMat; ++i) f.a[5 + i] = -f.a[5 + i];
+    return f;
+}
+
+// CrpFlux::inverse (crp_flux.cpp:493-512): the mirrored classic problem.
+inline MArr crp_inverse(const MState& zLA, const MState& zRA, const MState& zRB, const Mixture& mx, double delta,
+                        double dt, int& err) {
+    return f_inverse(crp_classic(m_inverse(zRB), m_inverse(zRA), m_inverse(zLA), mx, delta, dt, err));
+}
+
+// PState::split, ZPP version (models.cpp:261-335): pure material iA and the
+// mixture of the others at the same velocity, pressure and temperature.
+inline void m_split(const MState& z, const Mixture& mx, int iA, MState& zA, MState& zB) {
+    const double energy_A = sg_energy_rT(mx.m[iA], z.rhos[iA], z.T);
+    zA.rho = z.rhos[iA];
+    zA.v = z.v;
+    zA.P = z.P;
+    zA.e = energy_A;
+    zA.T = z.T;
+    zA.beta = Frac();
+    zA.beta.a[iA] = 1.0;
+    for (int i = 0; i < kMaxMat; ++i) zA.rhos[i] = NAN;
+    zA.rhos[iA] = z.rhos[iA];
+
+    zB.rho = NAN;
+    zB.v = z.v;
+    zB.P = z.P;
+    zB.e = NAN;
+    zB.T = z.T;
+    zB.beta = Frac();
+    for (int i = 0; i < kMaxMat; ++i) zB.rhos[i] = NAN;
+
+    const double beta_B = 1.0 - z.beta.a[iA];
+    Frac alpha;
+    for (int i = 0; i < kMaxMat; ++i) {
+        if (z.beta.has(i) && i != iA) {
+            zB.rhos[i] = z.rhos[i];
+            zB.beta.a[i] = z.beta.a[i] / beta_B;
+        }
+        if (z.beta.has(i)) alpha.a[i] = m_alpha(z, i);
+    }
+    zB.beta.normalize();
+    const double alpha_B = 1.0 - alpha.a[iA];
+    alpha.a[iA] = 0.0;
+    for (int i = 0; i < kMaxMat; ++i)
+        if (z.beta.has(i) && i != iA) alpha.a[i] = alpha.a[i] / alpha_B;
+    alpha.normalize();
+    zB.rho = 0.0;
+    for (int i = 0; i < kMaxMat; ++i)
+        if (z.beta.has(i) && i != iA) zB.rho = zB.rho + z.rhos[i] * alpha.a[i];
+    zB.e = 0.0;
+    for (int i = 0; i < kMaxMat; ++i)
+        if (z.beta.has(i) && i != iA) {
+            zB.beta.a[i] = alpha.a[i] * z.rhos[i] / zB.rho;
+            zB.e += zB.beta.a[i] * sg_energy_rT(mx.m[i], z.rhos[i], z.T);
+        }
+}
+
+// master_component (mm_fluid.cpp:201-226): largest volume fraction of the upwind side.
+inline int master_component(const MState& zm, const MState& zp) {
+    const double vi = 0.5 * (zm.v.x + zp.v.x);
+    const MState& z = vi > 0.0 ? zm : zp;
+    int master = 0;
+    double alpha_max = 0.0;
+    for (int i = 0; i < kMaxMat; ++i)
+        if (z.beta.has(i)) {
+            const double a = m_alpha(z, i);
+            if (a > alpha_max) {
+                alpha_max = a;
+                master = i;
+            }
+        }
+    return master;
+}
+
+// MmFluid::calc_crp_flux (mm_fluid.cpp:259-337).
+inline MArr calc_crp_flux(const MState& zL, const MState& zR, double hL, double hR, int iA, double a_sig, double dt,
+                          const Mixture& mx, int& err) {
+    const double aL = m_alpha(zL, iA), aR = m_alpha(zR, iA);
+    if ((aL == 0.0 || aL == 1.0) && (aR == 0.0 || aR == 1.0)) return m_hllc(zL, zR, mx);
+    const double delta_L = (aL < a_sig ? aL / a_sig : (1.0 - aL) / (1.0 - a_sig)) * hL;
+    const double delta_R = (aR < a_sig ? aR / a_sig : (1.0 - aR) / (1.0 - a_sig)) * hR;
+    MArr fA, fB;
+    if (aL == 0.0) {
+        MState zRA, zRB;
+        m_split(zR, mx, iA, zRA, zRB);
+        fA = m_hllc(zL, zRA, mx);
+        fB = crp_inverse(zL, zRB, zRA, mx, delta_R, dt, err);
+    } else if (aL == 1.0) {
+        MState zRA, zRB;
+        m_split(zR, mx, iA, zRA, zRB);
+        fA = crp_inverse(zL, zRA, zRB, mx, delta_R, dt, err);
+        fB = m_hllc(zL, zRB, mx);
+    } else if (aR == 0.0) {
+        MState zLA, zLB;
+        m_split(zL, mx, iA, zLA, zLB);
+        fA = m_hllc(zLA, zR, mx);
+        fB = crp_classic(zLA, zLB, zR, mx, delta_L, dt, err);
+    } else if (aR == 1.0) {
+        MState zLA, zLB;
+        m_split(zL, mx, iA, zLA, zLB);
+        fA = crp_classic(zLB, zLA, zR, mx, delta_L, dt, err);
+        fB = m_hllc(zLB, zR, mx);
+    } else {
+        MState zLA, zLB, zRA, zRB;
+        m_split(zL, mx, iA, zLA, zLB);
+        m_split(zR, mx, iA, zRA, zRB);
+        if (aL < aR) {
+            fA = crp_classic(zLB, zLA, zRA, mx, delta_L, dt, err);
+            fB = crp_inverse(zLB, zRB, zRA, mx, delta_R, dt, err);
+        } else {
+            fA = crp_inverse(zLA, zRA, zRB, mx, delta_R, dt, err);
+            fB = crp_classic(zLA, zLB, zRB, mx, delta_L, dt, err);
+        }
+    }
+    MArr r;
+    for (int k = 0; k < 9; ++k) r.a[k] = a_sig * fA.a[k] + (1.0 - a_sig) * fB.a[k];
+    return r;
+}
+
 }  // namespace orcmm
 
 using namespace orcmm;
@@ -512,6 +839,7 @@ using namespace orcmm;
 struct orc_mm {
     orc_mesh* m;
     Mixture mix;
+    int crp = 1;   // CrpMode::PLIC, the MmFluid default (mm_fluid.cpp:18); 0 = CrpMode::NONE
     double cfl, dt, max_dt;
     std::vector<MState> init, next;   // MmFluid::Parts init / next (mm_fluid.h:53-67)
     std::vector<double> nrm_x, nrm_y, pos;   // interface normals / positions [c][mat]
@@ -608,14 +936,28 @@ double orc_mm_compute_dt(orc_mm* s) {
     return s->dt;
 }
 
-// MmFluid::fluxes, CrpMode::NONE (mm_fluid.cpp:373-445 with calc_flux :351-356)
-// + swap; returns 0, or 1 on a NaN flux / negative density (the reference throws).
+void orc_mm_set_crp(orc_mm* s, int mode) { s->crp = mode; }
+
+// MmFluid::update, first order (mm_fluid.cpp:110-145): compute_dt, for
+// CrpMode::PLIC interface_recovery, fluxes (:373-445) with calc_flux
+// (:339-371: HLLC between two pure states or without CRP, else the
+// composite Riemann flux of the master material with the PLIC face
+// fraction, alpha_sigma :228-256), swap; returns 0, or 1 on a NaN flux, a
+// negative density or a CRP case the reference rejects (it throws).
 int orc_mm_update(orc_mm* s) {
     g_numeric_error = 0;
     orc_mm_compute_dt(s);
     const orc_mesh& m = *s->m;
     const double dt = s->dt;
-    int neg = 0;
+    const int nm = s->mix.n;
+    std::vector<double> inx, iny, ip;
+    if (s->crp) {
+        inx.resize((size_t)nm * m.n);
+        iny.resize((size_t)nm * m.n);
+        ip.resize((size_t)nm * m.n);
+        orc_mm_interface(s, inx.data(), iny.data(), ip.data());
+    }
+    int neg = 0, crp_err = 0;
     for (int c = 0; c < m.n; ++c) {
         const double V_c = m.vol[c];
         const MState z_c = s->init[c];
@@ -631,7 +973,26 @@ int orc_mm_update(orc_mm* s) {
             const MState z_m = m_in_local(z_c, normal);
             const MState z_p = m_in_local(z_n, normal);
             const double S = m.area[f];
-            MArr loc = m_hllc(z_m, z_p, s->mix);
+            const double h_L = V_c / S;
+            const double h_R = (slot_is_boundary(b) ? V_c : m.vol[m.adj[f]]) / S;
+            MArr loc;
+            if (!s->crp || (z_m.beta.index() >= 0 && z_p.beta.index() >= 0)) {
+                loc = m_hllc(z_m, z_p, s->mix);
+            } else {
+                const int iA = master_component(z_m, z_p);
+                // alpha_sigma, CrpMode::PLIC (mm_fluid.cpp:241-254)
+                const double a_L = m_alpha(z_m, iA), a_R = m_alpha(z_p, iA);
+                const double vi = 0.5 * (z_m.v.x + z_p.v.x);
+                const bool upwind = vi > 0.0;
+                const double CFL = dt * std::abs(vi) / (upwind ? h_L : h_R);
+                const int cn = slot_is_boundary(b) ? c : m.adj[f];   // face.neib() is the cell at a boundary
+                const size_t ki = (size_t)iA * m.n;
+                const Vec3 nc = upwind ? Vec3{inx[ki + c], iny[ki + c], 0.0}
+                                       : Vec3{-inx[ki + cn], -iny[ki + cn], -0.0};
+                const double cs = dot(normal, nc);
+                const double a_sig = f_between(average_flux(upwind ? a_L : a_R, cs, CFL), a_L, a_R);
+                loc = calc_crp_flux(z_m, z_p, h_L, h_R, iA, a_sig, dt, s->mix, crp_err);
+            }
             const Vec3 mg = rot_global(Vec3{loc.a[1], loc.a[2], loc.a[3]}, normal);   // Flux::to_global
             loc.a[1] = mg.x; loc.a[2] = mg.y; loc.a[3] = mg.z;
             for (int q = 0; q < 9; ++q) flux.a[q] += loc.a[q] * S;
@@ -643,7 +1004,7 @@ int orc_mm_update(orc_mm* s) {
         s->next[c] = m_from_q(q_c, s->mix, z_c.P, z_c.T, z_c.rhos);
     }
     std::swap(s->init, s->next);   // MmFluid::swap (mm_fluid.cpp:682-684)
-    return (g_numeric_error || neg) ? 1 : 0;
+    return (g_numeric_error || neg || crp_err) ? 1 : 0;
 }
 
 double orc_mm_dt(const orc_mm* s) { return s->dt; }

@@ -75,6 +75,7 @@ def lib():
             "orc_mm_create": (_P, [_P, _I, _dp, _D]),
             "orc_mm_free": (None, [_P]),
             "orc_mm_set_max_dt": (None, [_P, _D]),
+            "orc_mm_set_crp": (None, [_P, _I]),
             "orc_mm_set_state": (None, [_P, _dp, _dp, _dp, _dp, _dp]),
             "orc_mm_state": (None, [_P] + [_dp] * 8),
             "orc_mm_compute_dt": (_D, [_P]),
@@ -294,11 +295,12 @@ class MmFluid:
     """Oracle MmFluid (math/solver/mm_fluid.h:45-163 restated, first order,
     no composite Riemann problems); materials: (gamma, P0, e0, Cv, T0, rho_ref)."""
 
-    def __init__(self, mesh, materials, cfl=0.7):
+    def __init__(self, mesh, materials, cfl=0.7, crp=True):
         self.mesh = mesh
         self.mats = _mats(materials)
         self.nm = self.mats.shape[0]
         self.h = lib().orc_mm_create(mesh.h, self.nm, _d(self.mats), cfl)
+        lib().orc_mm_set_crp(self.h, 1 if crp else 0)
 
     def __del__(self):
         if getattr(self, "h", None) and _lib is not None:

@@ -1,6 +1,7 @@
-"""Device multi-material step (zfv_mm_*: MmFluid first order, CrpMode::NONE,
-PT-equilibrium closure of stiffened gases, PLIC interface reconstruction)
-against the oracle on the same inputs.
+"""Device multi-material step (zfv_mm_*: MmFluid first order with and
+without composite Riemann problems at mixed faces, PT-equilibrium closure
+of stiffened gases, PLIC interface reconstruction) against the oracle on the
+same inputs.
 
 Bar: bit-exact (numpy equality, NaN where a material is absent) for the
 state after the closure, every field and dt after steps on uniform and
@@ -48,12 +49,12 @@ def assert_same(zg, zo, what):
             raise AssertionError(f"{what}: {k} differs at {bad[:5]}: {a.reshape(-1)[bad[:3]]} vs {b.reshape(-1)[bad[:3]]}")
 
 
-def pair(oracle, zfv, nx, ny, bounds, mats, cfl=0.5):
+def pair(oracle, zfv, nx, ny, bounds, mats, crp, cfl=0.5):
     m = oracle.Mesh.rectangle(nx, ny, bounds=bounds)
     ctx = zfv.Context()
     ctx.mesh_rectangle(nx, ny, bounds=bounds)
-    ctx.mm_configure(mats, cfl)
-    o = oracle.MmFluid(m, mats, cfl)
+    ctx.mm_configure(mats, cfl, crp=crp)
+    o = oracle.MmFluid(m, mats, cfl, crp=crp)
     return m, o, ctx
 
 
@@ -65,9 +66,10 @@ CASES = [
 ]
 
 
+@pytest.mark.parametrize("crp", [False, True])
 @pytest.mark.parametrize("mats,bounds,sp", CASES)
-def test_steps_match_oracle(oracle, zfv, mats, bounds, sp):
-    m, o, ctx = pair(oracle, zfv, 30, 22, bounds, mats)
+def test_steps_match_oracle(oracle, zfv, mats, bounds, sp, crp):
+    m, o, ctx = pair(oracle, zfv, 30, 22, bounds, mats, crp)
     cl = m.cells()
     st = initial(cl["cx"], cl["cy"], mats, 4, sp)
     o.set_state(*st)
@@ -85,7 +87,8 @@ def test_steps_match_oracle(oracle, zfv, mats, bounds, sp):
         assert np.any(gp != 0.0), "no mixed cell"
 
 
-def test_steps_on_an_adapted_mesh(oracle, zfv):
+@pytest.mark.parametrize("crp", [False, True])
+def test_steps_on_an_adapted_mesh(oracle, zfv, crp):
     """Hanging faces (second sub-face slots): refine both meshes with the same
     random flags through the single-material solvers, then run the
     multi-material step on the adapted meshes."""
@@ -109,8 +112,8 @@ def test_steps_on_an_adapted_mesh(oracle, zfv):
     _, slots, _ = ctx.mesh_download()
     assert np.any(slots["boundary"].reshape(-1, 8)[:, 4:] >= 0), "no sub-face slots"
     mats = [GAS, HEAVY]
-    o = oracle.MmFluid(m, mats, 0.5)
-    ctx.mm_configure(mats, 0.5)
+    o = oracle.MmFluid(m, mats, 0.5, crp=crp)
+    ctx.mm_configure(mats, 0.5, crp=crp)
     cl = m.cells()
     st = initial(cl["cx"], cl["cy"], mats, 6)
     o.set_state(*st)
@@ -132,7 +135,7 @@ def test_one_material_equals_single_fluid_step(zfv):
     nx, ny, bounds = 40, 30, (1, 2, 3, 3)
     a = zfv.Context()
     a.mesh_rectangle(nx, ny, bounds=bounds)
-    a.mm_configure([GAS], 0.5)
+    a.mm_configure([GAS], 0.5, crp=True)
     b = zfv.Context()
     b.mesh_rectangle(nx, ny, bounds=bounds)
     b.configure(acc=1, flux="hllc", limiter="MC", cfl=0.5, gamma=1.4)
@@ -161,3 +164,22 @@ def test_state_is_tied_to_the_mesh(zfv):
     ctx.mesh_rectangle(10, 8)
     with pytest.raises(zfv.ZfvError, match="no multi-material state"):
         ctx.mm_step(1)
+
+
+def test_composite_fluxes_keep_an_advected_interface_sharp(zfv):
+    """The device step with composite Riemann fluxes keeps an advected
+    interface a few cells wide; plain HLLC smears it (as in the oracle)."""
+    counts = {}
+    for crp in (False, True):
+        ctx = zfv.Context()
+        ctx.mesh_rectangle(40, 40, bounds=(3, 3, 3, 3))
+        ctx.mm_configure([GAS, HEAVY], 0.5, crp=crp)
+        cells, _, _ = ctx.mesh_download()
+        inside = ((cells["cx"] - 0.5) ** 2 + (cells["cy"] - 0.5) ** 2 < 0.2 ** 2).astype(float)
+        n = ctx.n
+        ctx.mm_set_state(np.where(inside > 0, 4.0, 1.0), np.full(n, 1.0), np.full(n, 0.5), np.ones(n),
+                         np.stack([1.0 - inside, inside]))
+        ctx.mm_step(60)
+        a = ctx.mm_state()["beta"][1]
+        counts[crp] = int(np.sum((a > 1e-6) & (a < 1.0 - 1e-6)))
+    assert counts[True] < 0.2 * counts[False], counts

@@ -32,7 +32,7 @@ EXPORTS = [
     "zfv_mesh_n_faces", "zfv_set_launch", "zfv_get_launch", "zfv_autotune",
     "zfv_batch_rectangles", "zfv_batch_layout", "zfv_batch_dt", "zfv_graph_enable",
     "zfv_adjoint_forward", "zfv_adjoint_backward",
-    "zfv_mm_configure", "zfv_mm_set_max_dt", "zfv_mm_set_state", "zfv_mm_state", "zfv_mm_step", "zfv_mm_dt",
+    "zfv_mm_configure", "zfv_mm_set_crp", "zfv_mm_set_max_dt", "zfv_mm_set_state", "zfv_mm_state", "zfv_mm_step", "zfv_mm_dt", "zfv_mm_crp_faces",
     "zfv_mm_interface",
 ]
 CELL, QUAD = 0, 1   # step-kernel layouts
@@ -97,6 +97,7 @@ def lib():
         L.zfv_graph_enable.argtypes = [vp, C.c_int]
         L.zfv_mm_configure.argtypes = [vp, C.c_int, vp, C.c_double]
         L.zfv_mm_set_max_dt.argtypes = [vp, C.c_double]
+        L.zfv_mm_set_crp.argtypes = [vp, C.c_int]
         L.zfv_mm_set_state.argtypes = [vp] * 6
         L.zfv_mm_state.argtypes = [vp] * 9
         L.zfv_mm_step.argtypes = [vp, C.c_int]
@@ -302,11 +303,13 @@ class Context:
         _check(lib().zfv_graph_enable(self.h, 1 if on else 0))
 
     # ---- multi-material solver (first order, no CRP)
-    def mm_configure(self, materials, cfl=0.7):
-        """materials: sequence of (gamma, P0, e0, Cv, T0, rho_ref)."""
+    def mm_configure(self, materials, cfl=0.7, crp=True):
+        """materials: sequence of (gamma, P0, e0, Cv, T0, rho_ref); crp:
+        composite Riemann problems at mixed faces (the MmFluid default)."""
         mats = np.ascontiguousarray(np.asarray(materials, dtype=np.float64).reshape(-1, 6))
         self._mm_nmat = mats.shape[0]
         _check(lib().zfv_mm_configure(self.h, mats.shape[0], _ptr(mats), float(cfl)))
+        _check(lib().zfv_mm_set_crp(self.h, 1 if crp else 0))
 
     def mm_set_max_dt(self, v):
         _check(lib().zfv_mm_set_max_dt(self.h, float(v)))
@@ -333,6 +336,12 @@ class Context:
         d = C.c_double()
         _check(lib().zfv_mm_dt(self.h, C.byref(d)))
         return d.value
+
+    def mm_crp_faces(self):
+        """Cell sides that took a composite Riemann flux in the last step."""
+        k = C.c_int64()
+        _check(lib().zfv_mm_crp_faces(self.h, C.byref(k)))
+        return k.value
 
     def mm_interface(self):
         """PLIC normals and section positions, (n_mat, n) each."""

@@ -163,10 +163,13 @@ struct zfv_ctx {
         bool configured = false, have_state = false;
         long long epoch = -1;   // mesh epoch of the state
         mm::Mix mix{};
+        int crp = 1;   // CrpMode::PLIC (mm_fluid.cpp:18); 0 = CrpMode::NONE
         double cfl = 0.7, max_dt = 1.0e300;   // MmFluid defaults (mm_fluid.cpp:16-26)
         int cur = 0;
         long long step_no = 0;
-        DBuf<double> st[2], dmax_dt, dt_out, inter;
+        DBuf<double> st[2], dmax_dt, dt_out, inter, cf;
+        DBuf<int> crp_list;
+        DBuf<unsigned int> crp_count;
         DBuf<unsigned long long> dtmin;
     } mm;
     long long mesh_epoch = 0;   // incremented whenever the mesh changes
@@ -1531,6 +1534,14 @@ int zfv_mm_set_max_dt(zfv_ctx* ctx, double max_dt) {
     return ZFV_OK;
 }
 
+int zfv_mm_set_crp(zfv_ctx* ctx, int mode) {
+    int r = mm_ready(ctx, false);
+    if (r) return r;
+    if (mode != 0 && mode != 1) return fail(ZFV_ERR_ARG, "mm_set_crp: 0 (NONE) or 1 (PLIC)");
+    ctx->mm.crp = mode;
+    return ZFV_OK;
+}
+
 int zfv_mm_set_state(zfv_ctx* ctx, const double* rho, const double* u, const double* v, const double* P,
                      const double* beta) {
     int r = mm_ready(ctx, false);
@@ -1586,6 +1597,13 @@ int zfv_mm_step(zfv_ctx* ctx, int n_steps) {
     if (n_steps < 0) return fail(ZFV_ERR_ARG, "negative step count");
     auto& S = ctx->mm;
     const DevMesh M = ctx->dmesh();
+    const size_t cap = (size_t)ctx->cap, nm = (size_t)S.mix.n;
+    if (S.crp) {
+        if (S.inter.n != 3 * nm * cap) ZFV_HIP(S.inter.alloc(3 * nm * cap));
+        if (S.cf.n != 8 * (4 + nm) * cap) ZFV_HIP(S.cf.alloc(8 * (4 + nm) * cap));
+        if (S.crp_list.n != 8 * cap) ZFV_HIP(S.crp_list.alloc(8 * cap));
+        if (S.crp_count.n != 1) ZFV_HIP(S.crp_count.alloc(1));
+    }
     for (int k = 0; k < n_steps; ++k) {
         StepCtl c;
         c.dtmin = S.dtmin.p;
@@ -1597,10 +1615,34 @@ int zfv_mm_step(zfv_ctx* ctx, int n_steps) {
         c.dt_out = S.dt_out.p;
         c.err = ctx->err.p;
         const MmLayer in = mm_layer(ctx, S.cur), out = mm_layer(ctx, 1 - S.cur);
-        ZFV_HIP(timed(ctx, "mm_flux1", [&] { return mm_flux1(M, in, out, S.mix, c, ctx->stream); }));
+        const double* CF = nullptr;
+        if (S.crp) {   // interface_recovery, then the composite fluxes (mm_fluid.cpp:130-145)
+            double* o = S.inter.p;
+            ZFV_HIP(timed(ctx, "mm_interface", [&] {
+                return mm_interface(M, in, S.mix, ctx->vx.p, ctx->vy.p, o, o + nm * cap, o + 2 * nm * cap, ctx->stream);
+            }));
+            ZFV_HIP(timed(ctx, "mm_crp", [&] {
+                return mm_crp(M, in, S.mix, c, o, o + nm * cap, S.crp_list.p, S.crp_count.p, S.cf.p, ctx->stream);
+            }));
+            CF = S.cf.p;
+        }
+        ZFV_HIP(timed(ctx, "mm_flux1", [&] { return mm_flux1(M, in, out, S.mix, c, CF, ctx->stream); }));
         S.cur = 1 - S.cur;
         S.step_no += 1;
     }
+    return ZFV_OK;
+}
+
+int zfv_mm_crp_faces(zfv_ctx* ctx, int64_t* n) {
+    int r = mm_ready(ctx, true);
+    if (r) return r;
+    if (!n) return fail(ZFV_ERR_ARG, "null output");
+    unsigned int k = 0;
+    if (ctx->mm.crp && ctx->mm.crp_count.p) {
+        ZFV_HIP(hipMemcpyAsync(&k, ctx->mm.crp_count.p, sizeof(k), hipMemcpyDeviceToHost, ctx->stream));
+        ZFV_HIP(hipStreamSynchronize(ctx->stream));
+    }
+    *n = k;
     return ZFV_OK;
 }
 

@@ -147,8 +147,13 @@ struct MmLayer {
 };
 hipError_t mm_prepare(const DevMesh& M, const MmLayer& L, const mm::Mix& mx, unsigned long long* slot,
                       hipStream_t s);
+// CF: composite Riemann fluxes per [slot][4 + n_mat][stride] from mm_crp,
+// or null without composite Riemann problems.
 hipError_t mm_flux1(const DevMesh& M, const MmLayer& in, const MmLayer& out, const mm::Mix& mx,
-                    const StepCtl& ctl, hipStream_t s);
+                    const StepCtl& ctl, const double* CF, hipStream_t s);
+// list: 8 * stride ints of work space, count: one counter.
+hipError_t mm_crp(const DevMesh& M, const MmLayer& in, const mm::Mix& mx, const StepCtl& ctl, const double* nx,
+                  const double* ny, int* list, unsigned int* count, double* CF, hipStream_t s);
 hipError_t mm_interface(const DevMesh& M, const MmLayer& L, const mm::Mix& mx, const double* vx, const double* vy,
                         double* nx, double* ny, double* p, hipStream_t s);
 

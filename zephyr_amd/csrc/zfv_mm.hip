@@ -1,6 +1,7 @@
 // HIP kernels of the multi-material first-order step (MmFluid::update with
-// acc 1 and CrpMode::NONE, math/solver/mm_fluid.cpp:110-145, 166-194,
-// 373-445) and the PLIC interface reconstruction (:484-519).
+// acc 1, math/solver/mm_fluid.cpp:110-145, 166-194, 373-445): the PLIC
+// interface reconstruction (:484-519), the composite Riemann fluxes of the
+// faces between mixed cells (CrpMode::PLIC, the default) and the update.
 //
 // Same structure as the single-material step: one thread per cell gathering
 // its faces in slot order, writing only itself; the next step's CFL
@@ -107,13 +108,118 @@ __global__ void __launch_bounds__(kBlock) k_mm_prepare(DevMesh M, MmLayer L, Mix
     block_min_atomic(slot + 3 * tl.pid, cand);
 }
 
-// MmFluid::fluxes, CrpMode::NONE (mm_fluid.cpp:373-445): per face the mmf
-// HLLC flux of the face-frame states (own, neighbour or boundary value,
-// :98-108), q -= dt / V sum F S, the new state by the PT closure from the
-// cell's previous P, T and true densities (models.cpp:187-201), its sound
-// speed and the next step's CFL candidate.
+// Whether a face's flux is a composite Riemann flux: CRP on, and not both
+// face states pure (MmFluid::calc_flux, mm_fluid.cpp:339-371).
 template <int NM>
-__global__ void __launch_bounds__(kBlock) k_mm_flux1(DevMesh M, MmLayer in, MmLayer out, Mix mx, StepCtl ctl) {
+__device__ __forceinline__ bool crp_face(const double* CF, const MZ<NM>& zm, const MZ<NM>& zp) {
+    return CF != nullptr && !(frac_index<NM>(zm.b) >= 0 && frac_index<NM>(zp.b) >= 0);
+}
+
+// Faces that need a composite Riemann flux (CrpMode::PLIC): a mixed cell on
+// either side (MmFluid::calc_flux, mm_fluid.cpp:339-371; a boundary face's
+// other side is the cell's own boundary value).  Appends slot-major pairs
+// s * stride + c to `list`; `count` starts at zero.
+template <int NM>
+__global__ void __launch_bounds__(kBlock) k_mm_crp_mark(DevMesh M, MmLayer in, int* __restrict__ list,
+                                                        unsigned int* count) {
+    const Tile tl = tile_of(M, (int)blockDim.x);
+    const int c = tl.c0 + (int)threadIdx.x;
+    if (c >= tl.end) return;
+    const int st = M.stride;
+    double b[NM];
+#pragma unroll
+    for (int i = 0; i < NM; ++i) b[i] = in.b[i * st + c];
+    const bool own_pure = frac_index<NM>(b) >= 0;
+    const int code = M.code[c];
+    for (int s = 0; s < 8; ++s) {
+        const int t = (code >> (4 * s)) & 7;
+        if (t == SC_UNDEF) continue;
+        bool nb_pure = own_pure;
+        if (t <= SC_PERIODIC) {
+            const int nb = M.adj[s * st + c];
+            double bn[NM];
+#pragma unroll
+            for (int i = 0; i < NM; ++i) bn[i] = in.b[i * st + nb];
+            nb_pure = frac_index<NM>(bn) >= 0;
+        }
+        if (!(own_pure && nb_pure)) list[atomicAdd(count, 1u)] = s * st + c;
+    }
+}
+
+// The composite Riemann flux of each listed face, one face per thread
+// (grid-stride): the face-frame states, the master material
+// (master_component, mm_fluid.cpp:201-226), the PLIC face fraction of the
+// upwind cell's interface (alpha_sigma, :241-254, with the normals of
+// interface_recovery) and calc_crp_flux (:259-337), rotated to the global
+// frame, into CF[slot][k][stride].  These faces are few and need many more
+// registers than the plain HLLC faces of k_mm_flux1, hence the compaction.
+template <int NM>
+__global__ void __launch_bounds__(kBlock) k_mm_crp(DevMesh M, MmLayer in, Mix mx, StepCtl ctl,
+                                                   const double* __restrict__ inx, const double* __restrict__ iny,
+                                                   const int* __restrict__ list, const unsigned int* count,
+                                                   double* __restrict__ CF) {
+    constexpr int K = 4 + NM;
+    const int st = M.stride;
+    const int n_faces = (int)*count;
+    bool bad = false;
+    for (int e = (int)(blockIdx.x * blockDim.x + threadIdx.x); e < n_faces; e += (int)(gridDim.x * blockDim.x)) {
+        const int so = list[e];
+        const int s = so / st, c = so - s * st;
+        const double dt = step_dt(ctl, 0);
+        const MZ<NM> zc = load_mz<NM>(in, c, st);
+        const int nib = (M.code[c] >> (4 * s)) & 15;
+        const int t = nib & 7;
+        const bool bound = t > SC_PERIODIC;
+        const int nb = bound ? c : M.adj[so];
+        const Frame fr = slot_frame(M, c, s, nib);
+        MZ<NM> zp;
+        if (!bound) {
+            zp = load_mz<NM>(in, nb, st);
+        } else {
+            zp = zc;
+            const Z w = frame_boundary(fr, Z{zc.r, zc.u, zc.v, zc.P, zc.e}, t == SC_WALL);
+            zp.u = w.u;
+            zp.v = w.v;
+        }
+        MZ<NM> zm = zc;
+        rotate_uv<NM>(fr, zm);
+        rotate_uv<NM>(fr, zp);
+        const double S = M.area[so];
+        const double h_L = M.vol[c] / S;
+        const double h_R = (bound ? M.vol[c] : M.vol[nb]) / S;
+        const int iA = master_component<NM>(zm, zp);
+        const double a_L = alpha_of<NM>(zm, iA), a_R = alpha_of<NM>(zp, iA);
+        const double vi = 0.5 * (zm.u + zp.u);
+        const bool upwind = vi > 0.0;
+        const double CFL = dt * fabs(vi) / (upwind ? h_L : h_R);
+        double ncx = 0.0, ncy = 0.0;
+#pragma unroll
+        for (int i = 0; i < NM; ++i)
+            if (i == iA) {
+                ncx = upwind ? inx[i * st + c] : -inx[i * st + nb];
+                ncy = upwind ? iny[i * st + c] : -iny[i * st + nb];
+            }
+        const double cs = M.nx[so] * ncx + M.ny[so] * ncy;
+        const double a_sig = f_between(average_flux(upwind ? a_L : a_R, cs, CFL), a_L, a_R);
+        double F[K];
+        calc_crp_flux<NM>(zm, zp, h_L, h_R, iA, a_sig, dt, mx, F, bad);
+        const Q g = frame_global(fr, Q{F[0], F[1], F[2], F[3]});
+        F[1] = g.mx;
+        F[2] = g.my;
+#pragma unroll
+        for (int k = 0; k < K; ++k) CF[(s * K + k) * st + c] = F[k];
+    }
+    if (bad) atomicOr(ctl.err, 1);
+}
+
+// MmFluid::fluxes (mm_fluid.cpp:373-445): per face the mmf HLLC flux of the
+// face-frame states (own, neighbour or boundary value, :98-108) or the
+// composite flux from CF, q -= dt / V sum F S, the new state by the PT
+// closure from the cell's previous P, T and true densities
+// (models.cpp:187-201), its sound speed and the next step's CFL candidate.
+template <int NM>
+__global__ void __launch_bounds__(kBlock) k_mm_flux1(DevMesh M, MmLayer in, MmLayer out, Mix mx, StepCtl ctl,
+                                                     const double* __restrict__ CF) {
     const Tile tl = tile_of(M, (int)blockDim.x);
     const int c = tl.c0 + (int)threadIdx.x;
     const double dt = step_dt(ctl, tl.pid);
@@ -147,10 +253,15 @@ __global__ void __launch_bounds__(kBlock) k_mm_flux1(DevMesh M, MmLayer in, MmLa
             rotate_uv<NM>(fr, zm);
             rotate_uv<NM>(fr, zp);
             double F[4 + NM];
-            hllc_mm<NM>(zm, zp, F, bad);
-            const Q g = frame_global(fr, Q{F[0], F[1], F[2], F[3]});
-            F[1] = g.mx;
-            F[2] = g.my;
+            if (crp_face<NM>(CF, zm, zp)) {
+#pragma unroll
+                for (int k = 0; k < 4 + NM; ++k) F[k] = CF[(s * (4 + NM) + k) * st + c];
+            } else {
+                hllc_mm<NM>(zm, zp, F, bad);
+                const Q g = frame_global(fr, Q{F[0], F[1], F[2], F[3]});
+                F[1] = g.mx;
+                F[2] = g.my;
+            }
             const double S = M.area[so];
 #pragma unroll
             for (int k = 0; k < 4 + NM; ++k) acc[k] = acc[k] + F[k] * S;
@@ -268,8 +379,18 @@ hipError_t mm_prepare(const DevMesh& M, const MmLayer& L, const Mix& mx, unsigne
 }
 
 hipError_t mm_flux1(const DevMesh& M, const MmLayer& in, const MmLayer& out, const Mix& mx, const StepCtl& ctl,
-                    hipStream_t s) {
-    ZFV_MM_DISPATCH(mx.n, k_mm_flux1, mm_grid(M.n), dim3(kBlock), 0, s, M, in, out, mx, ctl);
+                    const double* CF, hipStream_t s) {
+    ZFV_MM_DISPATCH(mx.n, k_mm_flux1, mm_grid(M.n), dim3(kBlock), 0, s, M, in, out, mx, ctl, CF);
+    return hipGetLastError();
+}
+
+hipError_t mm_crp(const DevMesh& M, const MmLayer& in, const Mix& mx, const StepCtl& ctl, const double* nx,
+                  const double* ny, int* list, unsigned int* count, double* CF, hipStream_t s) {
+    hipError_t e = hipMemsetAsync(count, 0, sizeof(unsigned int), s);
+    if (e != hipSuccess) return e;
+    ZFV_MM_DISPATCH(mx.n, k_mm_crp_mark, mm_grid(M.n), dim3(kBlock), 0, s, M, in, list, count);
+    // one face per thread over the list; 1024 workgroups cover any list length
+    ZFV_MM_DISPATCH(mx.n, k_mm_crp, dim3(1024), dim3(kBlock), 0, s, M, in, mx, ctl, nx, ny, list, count, CF);
     return hipGetLastError();
 }
 

@@ -200,6 +200,10 @@ __device__ double sound_speed(const Mix& mx, double rho, double P, const double*
     return sqrt(c2);
 }
 
+template <int NM>
+__device__ void find_rPT(const Mix& mx, double rho, double e_mix, const double* b, double P0, double T0,
+                         const double* rh0, double* rh, double& P, double& T, DPDT& v, DPDT& e);
+
 // get_rPT (:199-212): P, T and true densities from (rho, e); a mixture by
 // find_rPT_old (:448-533), Newton on (P, T) with the P > P_min guard.
 template <int NM>
@@ -213,6 +217,15 @@ __device__ void get_rPT(const Mix& mx, double rho, double e_mix, const double* b
         T = sg_temperature_rP(m, rho, P);
         return;
     }
+    DPDT vl, el;
+    find_rPT<NM>(mx, rho, e_mix, b, P0, T0, rh0, rh, P, T, vl, el);
+}
+
+// find_rPT_old (mixture_pt.cpp:448-533); v, e: the last iteration's sums
+// (for the derivatives of sound_speed_re).
+template <int NM>
+__device__ void find_rPT(const Mix& mx, double rho, double e_mix, const double* b, double P0, double T0,
+                         const double* rh0, double* rh, double& P, double& T, DPDT& v, DPDT& e) {
     const double v_mix = 1.0 / rho;
     P = isnan(P0) ? 1.0e5 : P0;
     T = isnan(T0) ? 300.0 : T0;
@@ -222,9 +235,12 @@ __device__ void get_rPT(const Mix& mx, double rho, double e_mix, const double* b
         if (b[i] > 0.0) P_min = smax(P_min, -mx.m[i].P0);
     if (P < P_min) P = P_min + 1;
     init_rhos<NM>(mx, b, rh0, rh);
+    v = DPDT{NAN, NAN, NAN};
+    e = DPDT{NAN, NAN, NAN};
     for (int counter = 0; counter < kMaxIter; ++counter) {
         double err = 0.0;
-        DPDT v{0.0, 0.0, 0.0}, e{0.0, 0.0, 0.0};
+        v = DPDT{0.0, 0.0, 0.0};
+        e = DPDT{0.0, 0.0, 0.0};
 #pragma unroll
         for (int i = 0; i < NM; ++i) {
             if (b[i] > 0.0) {
@@ -376,6 +392,439 @@ __device__ __forceinline__ double face_fraction(double a1, double a2) {
         return 3.0 - 2.0 * hi - 2.0 * sqrt((1.0 - hi) * (2.0 - lo - hi));
     if (lo == 0.0 && hi == 1.0) return 0.5;
     return sqrt(lo) / (sqrt(1.0 - hi) + sqrt(lo));
+}
+
+// ---- composite Riemann problems (CrpMode::PLIC, the MmFluid default) --------
+// Vectors of K = 4 + NM components {rho, m_x, m_y, E, rho beta_i} in the
+// face frame; the reference's m_z = 0 component only carries zeros.
+
+template <int NM>
+__device__ __forceinline__ double alpha_of(const MZ<NM>& z, int i) {   // PState::alpha (models.cpp:224-227)
+    double a = 0.0;
+#pragma unroll
+    for (int k = 0; k < NM; ++k)
+        if (k == i && !isnan(z.rh[k])) {
+            const double x = z.b[k] * z.r / z.rh[k];
+            const double lo = (1.0 < x) ? 1.0 : x;
+            a = (0.0 < lo) ? lo : 0.0;
+        }
+    return a;
+}
+
+// sound_speed_re of the mixture (mixture_pt.cpp:86-99; pure material
+// stiffened_gas.cpp:144-146): c^2 = P.dR + P P.dE / rho^2 with the
+// derivatives of find_rPT_old (:525-529), default initial guesses.
+template <int NM>
+__device__ double sound_speed_re(const Mix& mx, double rho, double e, const double* b) {
+    const int idx = frac_index<NM>(b);
+    if (idx >= 0) {
+        const Mat m = pick<NM>(mx, idx);
+        return sqrt(m.g * (m.g - 1.0) * ((e - m.e0) - m.P0 / rho));
+    }
+    double rh[NM], P, T;
+    DPDT v, ee;
+    find_rPT<NM>(mx, rho, e, b, NAN, NAN, nullptr, rh, P, T, v, ee);
+    const double inv_D = 1.0 / (v.dT * ee.dP - v.dP * ee.dT);
+    const double PdE = inv_D * v.dT;
+    const double PdR = inv_D * ee.dT * (v.val * v.val);
+    return sqrt(PdR + P * PdE / (rho * rho));
+}
+
+template <int NM>
+struct Wv {   // mmf::WaveConfig3 (hllc.h:21-27)
+    double SL, SC, SR;
+    double QsL[4 + NM], FsL[4 + NM], QsR[4 + NM], FsR[4 + NM];
+};
+
+// HLLC::wave_config from conserved vectors and fluxes (hllc.cpp:256-335).
+template <int NM>
+__device__ void wave_config(const Mix& mx, const double* QL, const double* FL, const double* QR, const double* FR,
+                            Wv<NM>& w) {
+    constexpr int K = 4 + NM;
+    const double uL = QL[1] / QL[0], vL = QL[2] / QL[0];
+    const double uR = QR[1] / QR[0], vR = QR[2] / QR[0];
+    const double eL = QL[3] / QL[0] - 0.5 * (uL * uL + vL * vL);
+    const double eR = QR[3] / QR[0] - 0.5 * (uR * uR + vR * vR);
+    double bL[NM], bR[NM];
+#pragma unroll
+    for (int i = 0; i < NM; ++i) {
+        bL[i] = QL[4 + i] / QL[0];
+        bR[i] = QR[4 + i] / QR[0];
+    }
+    const double cL = sound_speed_re<NM>(mx, QL[0], eL, bL);
+    const double cR = sound_speed_re<NM>(mx, QR[0], eR, bR);
+    double SL = smin(uL - cL, uR - cL);
+    double SR = smax(uL + cR, uR + cR);
+    if (SL >= SR || frac_index<NM>(bL) == frac_index<NM>(bR)) {
+        SL = smin(uL - cL, uR - cR);
+        SR = smax(uL + cL, uR + cR);
+    }
+    double aL[K], aR[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        aL[k] = FL[k] - SL * QL[k];
+        aR[k] = FR[k] - SR * QR[k];
+    }
+    double SC = (aR[1] - aL[1]) / (aR[0] - aL[0]);
+    if (SC < SL) SC = SL + 1.0e-9 * smax(cL, cR);
+    if (SC > SR) SC = SR - 1.0e-9 * smax(cL, cR);
+    const double rsL = aL[0] / (SC - SL);
+    const double rsR = aR[0] / (SC - SR);
+    const double EsL = SC * SC + (aL[3] - SC * aL[1]) / aL[0];
+    const double EsR = SC * SC + (aR[3] - SC * aR[1]) / aR[0];
+    w.SL = SL;
+    w.SC = SC;
+    w.SR = SR;
+    w.QsL[0] = rsL;
+    w.QsR[0] = rsR;
+    w.QsL[1] = rsL * SC;
+    w.QsL[2] = rsL * vL;
+    w.QsR[1] = rsR * SC;
+    w.QsR[2] = rsR * vR;
+    w.QsL[3] = rsL * EsL;
+    w.QsR[3] = rsR * EsR;
+#pragma unroll
+    for (int i = 0; i < NM; ++i) {
+        w.QsL[4 + i] = rsL * bL[i];
+        w.QsR[4 + i] = rsR * bR[i];
+    }
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        w.FsL[k] = aL[k] + SL * w.QsL[k];
+        w.FsR[k] = aR[k] + SR * w.QsR[k];
+    }
+}
+
+// Characteristics in the (x, t) plane (crp_flux.cpp:13-55).
+struct XT {
+    double x, t;
+};
+struct Chr {
+    double x, t, S;
+    __device__ double edge_t() const {
+        const double ts = (S * t - x) / S;
+        return ts < t ? INFINITY : ts;
+    }
+    __device__ XT cross(const Chr& c) const {
+        if (S == c.S) return XT{0.5 * (x + c.x), INFINITY};
+        XT p;
+        p.t = (c.x - x + S * t - c.S * c.t) / (S - c.S);
+        if (p.t <= smax(t, c.t)) return XT{0.5 * (x + c.x), INFINITY};
+        p.x = 0.5 * (x + c.x + S * (p.t - t) + c.S * (p.t - c.t));
+        return p;
+    }
+};
+
+template <int NM>
+__device__ __forceinline__ void copyk(double* d, const double* s) {
+#pragma unroll
+    for (int k = 0; k < 4 + NM; ++k) d[k] = s[k];
+}
+
+// (F1 (t1 - t0) - Q1 (x1 - x0) + F2 (...) - ...) / (0 - dt) for two segments.
+template <int NM>
+__device__ __forceinline__ void total2(const double* F1, const double* Q1, double dt1, double dx1, const double* F2,
+                                       const double* Q2, double dt2, double dx2, double dt, double* out) {
+#pragma unroll
+    for (int k = 0; k < 4 + NM; ++k) out[k] = ((F1[k] * dt1 - Q1[k] * dx1) + (F2[k] * dt2 - Q2[k] * dx2)) / (0.0 - dt);
+}
+
+// CrpFlux::classic, ZPP version (crp_flux.cpp:234-489).
+template <int NM>
+__device__ void crp_classic(const MZ<NM>& zLA, const MZ<NM>& zLB, const MZ<NM>& zRB, const Mix& mx, double delta,
+                            double dt, double* out, bool& err) {
+    constexpr int K = 4 + NM;
+    double qLB[K], fLB[K], Q_R[K], F_R[K];
+    m_qstate(zLB, qLB);
+    m_flux(zLB, fLB);
+    m_qstate(zRB, Q_R);
+    m_flux(zRB, F_R);
+    Wv<NM> w0;
+    wave_config<NM>(mx, qLB, fLB, Q_R, F_R, w0);
+    const Chr C_0{-delta, 0.0, zLA.u};
+    const Chr C_0L{0.0, 0.0, w0.SL};
+    const XT O1 = C_0.cross(C_0L);
+    double qA[K], fA[K];
+    m_qstate(zLA, qA);
+    m_flux(zLA, fA);
+    if (O1.x >= 0.0) {
+        const double t2u = C_0.edge_t();
+        if (dt <= t2u) {
+            copyk<NM>(out, fLB);
+            return;
+        }
+        Wv<NM> w1;
+        wave_config<NM>(mx, qA, fA, w0.QsL, w0.FsL, w1);
+        const double lb = -O1.x / (dt - O1.t);
+        const double* F = fA;
+        const double* Q = qA;
+        if (dt > O1.t) {
+            if (lb < w1.SL) { F = fA; Q = qA; }
+            else if (lb < w1.SC) { F = w1.FsL; Q = w1.QsL; }
+            else if (lb < w1.SR) { F = w1.FsR; Q = w1.QsR; }
+            else { F = w0.FsL; Q = w0.QsL; }
+        }
+        total2<NM>(F, Q, O1.t - dt, O1.x - 0.0, fLB, qLB, 0.0 - O1.t, 0.0 - O1.x, dt, out);
+        return;
+    }
+    if (w0.SC >= 0.0) {   // mix to clear
+        if (dt <= O1.t && O1.x < 0.0) {
+            copyk<NM>(out, w0.FsL);
+            return;
+        }
+        Wv<NM> w1;
+        wave_config<NM>(mx, qA, fA, w0.QsL, w0.FsL, w1);
+        const Chr C_1R{O1.x, O1.t, w1.SR};
+        const double t1 = C_1R.edge_t();
+        if (dt <= t1 && O1.x < 0.0) {
+            copyk<NM>(out, w0.FsL);
+            return;
+        }
+        const double lb = -O1.x / (dt - O1.t);
+        const double* F = lb < w1.SL ? fA : (lb < w1.SC ? w1.FsL : w1.FsR);
+        const double* Q = lb < w1.SL ? qA : (lb < w1.SC ? w1.QsL : w1.QsR);
+        total2<NM>(F, Q, O1.t - dt, O1.x - 0.0, w0.FsL, w0.QsL, 0.0 - O1.t, 0.0 - O1.x, dt, out);
+        return;
+    }
+    // clear to mix
+    if (O1.t >= dt) {
+        copyk<NM>(out, w0.FsR);
+        return;
+    }
+    Wv<NM> w1;
+    wave_config<NM>(mx, qA, fA, w0.QsL, w0.FsL, w1);
+    const Chr C_0C{0.0, 0.0, w0.SC};
+    const Chr C_1R{O1.x, O1.t, w1.SR};
+    const XT O2 = C_1R.cross(C_0C);
+    if (O2.t > dt || O2.x > 0.0) {
+        copyk<NM>(out, w0.FsR);
+        return;
+    }
+    Wv<NM> w2;
+    wave_config<NM>(mx, w1.QsR, w1.FsR, w0.QsR, w0.FsR, w2);
+    const Chr C_2L{O2.x, O2.t, w2.SL};
+    const Chr C_1C{O1.x, O1.t, w1.SC};
+    const XT O3 = C_1C.cross(C_2L);
+    if (O3.x >= 0.0) {   // crp_flux.cpp:419-421 throws
+        err = true;
+        copyk<NM>(out, w0.FsR);
+        return;
+    }
+    const Chr C_2R{O2.x, O2.t, w2.SR};
+    const double t1 = C_2R.edge_t();
+    Wv<NM> w3;
+    wave_config<NM>(mx, w1.QsL, w1.FsL, w2.QsL, w2.FsL, w3);
+    const Chr C_3R{O3.x, O3.t, w3.SR};
+    const double t4 = C_3R.edge_t();
+    if (dt <= t4) {
+        if (dt <= t1) {
+            copyk<NM>(out, w0.FsR);
+            return;
+        }
+        const double lb = -O2.x / (dt - O2.t);
+        const double* F = lb < w2.SC ? w2.FsL : (lb < w2.SR ? w2.FsR : w0.FsR);
+        const double* Q = lb < w2.SC ? w2.QsL : (lb < w2.SR ? w2.QsR : w0.QsR);
+        total2<NM>(F, Q, O2.t - dt, O2.x - 0.0, w0.FsR, w0.QsR, 0.0 - O2.t, 0.0 - O2.x, dt, out);
+        return;
+    }
+    const double lb = -O3.x / (dt - O3.t);
+    const double* F = lb < w3.SL ? w1.FsL : (lb < w3.SC ? w3.FsL : w3.FsR);
+    const double* Q = lb < w3.SL ? w1.QsL : (lb < w3.SC ? w3.QsL : w3.QsR);
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        const double g1 = F[k] * (O3.t - dt) - Q[k] * (O3.x - 0.0);
+        const double g2 = w2.FsL[k] * (O2.t - O3.t) - w2.QsL[k] * (O2.x - O3.x);
+        const double g3 = w0.FsR[k] * (0.0 - O2.t) - w0.QsR[k] * (0.0 - O2.x);
+        out[k] = ((g1 + g2) + g3) / (0.0 - dt);
+    }
+}
+
+// CrpFlux::inverse (crp_flux.cpp:493-512): the classic problem mirrored in x
+// (PState::inverse flips u, Flux::inverse negates all but m_x, models.cpp:219-221, 506-515).
+template <int NM>
+__device__ void crp_inverse(const MZ<NM>& zLA, const MZ<NM>& zRA, const MZ<NM>& zRB, const Mix& mx, double delta,
+                            double dt, double* out, bool& err) {
+    MZ<NM> a = zRB, b = zRA, c = zLA;
+    a.u = -a.u;
+    b.u = -b.u;
+    c.u = -c.u;
+    crp_classic<NM>(a, b, c, mx, delta, dt, out, err);
+    out[0] = -out[0];
+    out[2] = -out[2];
+    out[3] = -out[3];
+#pragma unroll
+    for (int i = 0; i < NM; ++i) out[4 + i] = -out[4 + i];
+}
+
+__device__ __forceinline__ double sg_energy_rT(const Mat& m, double rho, double T) {   // stiffened_gas.cpp:131-138
+    return m.e0 + m.Cv * (T - m.T0) + m.P0 / rho;
+}
+
+// PState::split, ZPP version (models.cpp:261-335): pure material iA and
+// the mixture of the others at the same velocity, pressure and temperature;
+// both with their sound speeds (sound_speed_rP, as the HLLC of the split
+// states computes them).
+template <int NM>
+__device__ void m_split(const MZ<NM>& z, const Mix& mx, int iA, MZ<NM>& zA, MZ<NM>& zB) {
+    double rhoA = 0.0;
+#pragma unroll
+    for (int i = 0; i < NM; ++i)
+        if (i == iA) rhoA = z.rh[i];
+    const Mat mA = pick<NM>(mx, iA);
+    zA.r = rhoA;
+    zA.u = z.u;
+    zA.v = z.v;
+    zA.P = z.P;
+    zA.e = sg_energy_rT(mA, rhoA, z.T);
+    zA.T = z.T;
+#pragma unroll
+    for (int i = 0; i < NM; ++i) {
+        zA.b[i] = (i == iA) ? 1.0 : 0.0;
+        zA.rh[i] = (i == iA) ? rhoA : NAN;
+    }
+    zA.c = sg_sound_speed_rP(mA, zA.r, zA.P);
+
+    double bA = 0.0;
+#pragma unroll
+    for (int i = 0; i < NM; ++i)
+        if (i == iA) bA = z.b[i];
+    const double beta_B = 1.0 - bA;
+    double alpha[NM];
+    zB.u = z.u;
+    zB.v = z.v;
+    zB.P = z.P;
+    zB.T = z.T;
+#pragma unroll
+    for (int i = 0; i < NM; ++i) {
+        const bool has = z.b[i] > 0.0;
+        zB.rh[i] = (has && i != iA) ? z.rh[i] : NAN;
+        zB.b[i] = (has && i != iA) ? z.b[i] / beta_B : 0.0;
+        alpha[i] = has ? alpha_of<NM>(z, i) : 0.0;
+    }
+    frac_cutoff<NM>(zB.b, kFracMinimal);
+    double aA = 0.0;
+#pragma unroll
+    for (int i = 0; i < NM; ++i)
+        if (i == iA) aA = alpha[i];
+    const double alpha_B = 1.0 - aA;
+#pragma unroll
+    for (int i = 0; i < NM; ++i) {
+        if (i == iA) alpha[i] = 0.0;
+        else if (z.b[i] > 0.0) alpha[i] = alpha[i] / alpha_B;
+    }
+    frac_cutoff<NM>(alpha, kFracMinimal);
+    double rB = 0.0;
+#pragma unroll
+    for (int i = 0; i < NM; ++i)
+        if (z.b[i] > 0.0 && i != iA) rB = rB + z.rh[i] * alpha[i];
+    zB.r = rB;
+    double eB = 0.0;
+#pragma unroll
+    for (int i = 0; i < NM; ++i)
+        if (z.b[i] > 0.0 && i != iA) {
+            zB.b[i] = alpha[i] * z.rh[i] / rB;
+            eB += zB.b[i] * sg_energy_rT(mx.m[i], z.rh[i], z.T);
+        }
+    zB.e = eB;
+    zB.c = sound_speed<NM>(mx, zB.r, zB.P, zB.b, zB.T, zB.rh);
+}
+
+// master_component (mm_fluid.cpp:201-226).
+template <int NM>
+__device__ __forceinline__ int master_component(const MZ<NM>& zm, const MZ<NM>& zp) {
+    const double vi = 0.5 * (zm.u + zp.u);
+    const MZ<NM>& z = vi > 0.0 ? zm : zp;
+    int master = 0;
+    double amax = 0.0;
+#pragma unroll
+    for (int i = 0; i < NM; ++i)
+        if (z.b[i] > 0.0) {
+            const double a = alpha_of<NM>(z, i);
+            if (a > amax) {
+                amax = a;
+                master = i;
+            }
+        }
+    return master;
+}
+
+// MmFluid::calc_crp_flux (mm_fluid.cpp:259-337).
+template <int NM>
+__device__ void calc_crp_flux(const MZ<NM>& zL, const MZ<NM>& zR, double hL, double hR, int iA, double a_sig,
+                              double dt, const Mix& mx, double* F, bool& err) {
+    constexpr int K = 4 + NM;
+    const double aL = alpha_of<NM>(zL, iA), aR = alpha_of<NM>(zR, iA);
+    if ((aL == 0.0 || aL == 1.0) && (aR == 0.0 || aR == 1.0)) {
+        hllc_mm<NM>(zL, zR, F, err);
+        return;
+    }
+    const double delta_L = (aL < a_sig ? aL / a_sig : (1.0 - aL) / (1.0 - a_sig)) * hL;
+    const double delta_R = (aR < a_sig ? aR / a_sig : (1.0 - aR) / (1.0 - a_sig)) * hR;
+    double fA[K], fB[K];
+    MZ<NM> xA, xB;
+    if (aL == 0.0) {
+        m_split<NM>(zR, mx, iA, xA, xB);
+        hllc_mm<NM>(zL, xA, fA, err);
+        crp_inverse<NM>(zL, xB, xA, mx, delta_R, dt, fB, err);
+    } else if (aL == 1.0) {
+        m_split<NM>(zR, mx, iA, xA, xB);
+        crp_inverse<NM>(zL, xA, xB, mx, delta_R, dt, fA, err);
+        hllc_mm<NM>(zL, xB, fB, err);
+    } else if (aR == 0.0) {
+        m_split<NM>(zL, mx, iA, xA, xB);
+        hllc_mm<NM>(xA, zR, fA, err);
+        crp_classic<NM>(xA, xB, zR, mx, delta_L, dt, fB, err);
+    } else if (aR == 1.0) {
+        m_split<NM>(zL, mx, iA, xA, xB);
+        crp_classic<NM>(xB, xA, zR, mx, delta_L, dt, fA, err);
+        hllc_mm<NM>(xB, zR, fB, err);
+    } else {
+        MZ<NM> yA, yB;
+        m_split<NM>(zL, mx, iA, xA, xB);
+        m_split<NM>(zR, mx, iA, yA, yB);
+        if (aL < aR) {
+            crp_classic<NM>(xB, xA, yA, mx, delta_L, dt, fA, err);
+            crp_inverse<NM>(xB, yB, yA, mx, delta_R, dt, fB, err);
+        } else {
+            crp_inverse<NM>(xA, yA, yB, mx, delta_R, dt, fA, err);
+            crp_classic<NM>(xA, xB, yB, mx, delta_L, dt, fB, err);
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < K; ++k) F[k] = a_sig * fA[k] + (1.0 - a_sig) * fB[k];
+}
+
+// average_flux (geom/sections.cpp:125-148) and between (math/funcs.h:47-50).
+__device__ __forceinline__ double f_sign(double x) { return x > 0.0 ? 1.0 : (x < 0.0 ? -1.0 : 0.0); }
+__device__ __forceinline__ double f_heav(double x) { return x > 0.0 ? 1.0 : 0.0; }
+__device__ __forceinline__ void f_sorted(double a, double b, double& lo, double& hi) {
+    if (a < b) {
+        lo = a;
+        hi = b;
+    } else {
+        lo = b;
+        hi = a;
+    }
+}
+__device__ __forceinline__ double f_between(double x, double x1, double x2) {
+    double lo, hi;
+    f_sorted(x1, x2, lo, hi);
+    return smax(lo, smin(x, hi));
+}
+__device__ __forceinline__ double average_flux(double alpha, double cs, double CFL) {
+    double a1, b1, a2, b2;
+    f_sorted(fabs(cs), sqrt(1.0 - cs * cs), a1, b1);
+    f_sorted(CFL * fabs(cs), sqrt(1.0 - cs * cs), a2, b2);
+    double P1;
+    if (fabs(2 * alpha - 1) <= 1.0 - a1 / b1) P1 = (alpha - 0.5) * b1;
+    else P1 = f_sign(alpha - 0.5) * (0.5 * (a1 + b1) - sqrt((1.0 - fabs(2.0 * alpha - 1.0)) * a1 * b1));
+    const double P2 = P1 + 0.5 * (CFL - 1.0) * cs;
+    if (fabs(P2) <= 0.5 * (b2 - a2)) return 0.5 + P2 / b2;
+    if (fabs(P2) < 0.5 * (b2 + a2)) {
+        const double t = fabs(P2) - 0.5 * (a2 + b2);
+        return f_heav(P2) - 0.5 * f_sign(P2) * (t * t) / (a2 * b2);
+    }
+    return f_heav(P2);
 }
 
 }  // namespace mm
