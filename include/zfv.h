@@ -247,13 +247,15 @@ int zfv_mm_interface(zfv_ctx* ctx, double* nx, double* ny, double* p);
 /* ---- launch configuration (no reference counterpart) --------------------- */
 /* Step kernels of a stage (0 = first-order update, 1 = gradient + predictor,
  * 2 = corrector): layout 0 = one thread per cell, 1 = four lanes per cell
- * (one face each), block = workgroup size 128, 256 or 512 (0 keeps the
- * current one).  All choices give bit-identical results. */
+ * (one face each), 2 = one thread per cell with the main slots' loads
+ * issued together (more registers per thread; for meshes that do not fill
+ * the GPU), block = workgroup size 128, 256 or 512 (0 keeps the current
+ * one).  All choices give bit-identical results. */
 int zfv_set_launch(zfv_ctx* ctx, int stage, int layout, int block);
 int zfv_get_launch(zfv_ctx* ctx, int stage, int* layout, int* block);
 /* Autotuner: times every (layout, workgroup size) of the configured scheme's
  * step kernels on the current state (`reps` launches each; the state is not
- * advanced) and keeps the fastest per stage.  ms_out (may be NULL): 18
+ * advanced) and keeps the fastest per stage.  ms_out (may be NULL): 27
  * doubles, ms per launch indexed [stage][layout][block 128/256/512]
  * (0 for stages the scheme does not use). */
 int zfv_autotune(zfv_ctx* ctx, int reps, double* ms_out);
@@ -262,6 +264,16 @@ int zfv_autotune(zfv_ctx* ctx, int reps, double* ms_out);
  * scheme or launch configuration).  Off by default; profiling timers are
  * not recorded for graph-launched steps. */
 int zfv_graph_enable(zfv_ctx* ctx, int on);
+/* Cell order of the step kernels.  mode 1 (default): on a mesh with refined
+ * cells the steps run on a copy of the mesh sorted along a space-filling
+ * curve (base cell, then Morton index inside it), rebuilt after each
+ * adaptation; the state is gathered into that order before steps and back
+ * into the mesh's order by every entry point that reads or writes per-cell
+ * data, so the order is never observable and the results are bit-identical
+ * to mode 0 (the mesh's own order throughout).  active: 1 while the state
+ * is held in curve order. */
+int zfv_set_cell_order(zfv_ctx* ctx, int mode);
+int zfv_get_cell_order(zfv_ctx* ctx, int* mode, int* active);
 
 /* Kernel timing: average device time (ms) per launch of each step kernel
  * over the last zfv_smf_step call when enabled (HIP events on the context

@@ -65,9 +65,13 @@ def test_batch_autotune_and_layouts(zfv):
     ref, begin, count = batch_context(zfv, 2, "MC")
     tun, _, _ = batch_context(zfv, 2, "MC")
     tun.set_launch(1, zfv.QUAD, 128)
-    tun.set_launch(2, zfv.QUAD, 512)
+    tun.set_launch(2, zfv.BATCH, 512)
     ref.step(3)
     tun.step(3)
+    tun.set_launch(1, zfv.BATCH, 256)
+    tun.set_launch(2, zfv.QUAD, 512)
+    ref.step(2)
+    tun.step(2)
     tun.autotune(reps=2)
     ref.step(5)
     tun.step(5)

@@ -1,9 +1,11 @@
-"""The two step-kernel layouts (one thread per cell, four lanes per cell) and
-the launch autotuner give bit-identical results, on uniform meshes and on
-adapted meshes whose hanging faces use the second sub-face slots.
+"""The three step-kernel layouts (one thread per cell, four lanes per cell,
+one thread per cell with the main slots' loads batched) and the launch
+autotuner give bit-identical results, on uniform meshes and on adapted
+meshes whose hanging faces use the second sub-face slots.
 
 The quad layout combines per-face terms with DPP quad broadcasts in slot
-order; any lane-layout or ordering mistake shows up as a bit difference.
+order, the batched layout sums the main slots before the sub-face slots;
+any lane-layout or ordering mistake shows up as a bit difference.
 """
 import numpy as np
 import pytest
@@ -43,21 +45,22 @@ def assert_same(a, b, what):
     (1, 3, "MC", 256), (1, 2, "MC", 128), (1, 1, "MC", 512),
     (2, 3, "MC", 256), (2, 3, "minmod", 512), (2, 3, "vanLeer", 128), (2, 2, "superbee", 256), (2, 1, "koren", 512),
 ])
-def test_quad_equals_cell_on_adapted_mesh(zfv, acc, flux, limiter, block):
+@pytest.mark.parametrize("layout", ["QUAD", "BATCH"])
+def test_layout_equals_cell_on_adapted_mesh(zfv, layout, acc, flux, limiter, block):
     cell = adapted_context(zfv, zfv.CELL, acc, flux, limiter, seed=11)
-    quad = adapted_context(zfv, zfv.QUAD, acc, flux, limiter, seed=11, block=block)
-    assert cell.n == quad.n
+    other = adapted_context(zfv, getattr(zfv, layout), acc, flux, limiter, seed=11, block=block)
+    assert cell.n == other.n
     _, slots, _ = cell.mesh_download()
     n = cell.n
     assert np.any(slots["boundary"].reshape(n, 8)[:, 4:] >= 0), "mesh has no sub-face slots"
     for step in range(6):
         cell.step(1)
-        quad.step(1)
-        assert cell.dt() == quad.dt(), f"dt differs at step {step}"
-        assert_same(cell.state(), quad.state(), f"state after step {step + 1}")
-        assert np.array_equal(cell.wait(), quad.wait())
+        other.step(1)
+        assert cell.dt() == other.dt(), f"dt differs at step {step}"
+        assert_same(cell.state(), other.state(), f"state after step {step + 1}")
+        assert np.array_equal(cell.wait(), other.wait())
     if acc == 2:
-        assert_same(cell.grad(), quad.grad(), "gradients")
+        assert_same(cell.grad(), other.grad(), "gradients")
 
 
 def test_quad_equals_cell_set_flags(zfv):
@@ -77,11 +80,11 @@ def test_autotune_keeps_state_and_results(zfv):
     tun.step(2)
     before = tun.state()
     ms = tun.autotune(reps=3)
-    assert ms.shape == (3, 2, 3) and np.all(ms[1:] > 0.0) and np.all(ms[0] == 0.0)
+    assert ms.shape == (3, 3, 3) and np.all(ms[1:] > 0.0) and np.all(ms[0] == 0.0)
     assert_same(before, tun.state(), "autotune changed the state")
     for st in (1, 2):
         layout, block = tun.launch(st)
-        assert layout in (zfv.CELL, zfv.QUAD) and block in (128, 256, 512)
+        assert layout in (zfv.CELL, zfv.QUAD, zfv.BATCH) and block in (128, 256, 512)
         assert ms[st, layout, (128, 256, 512).index(block)] == ms[st].min()
     ref.step(4)
     tun.step(4)

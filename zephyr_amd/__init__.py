@@ -31,11 +31,12 @@ EXPORTS = [
     "zfv_flags_upload", "zfv_flags_download", "zfv_set_flags", "zfv_balance_flags", "zfv_refine",
     "zfv_mesh_n_faces", "zfv_set_launch", "zfv_get_launch", "zfv_autotune",
     "zfv_batch_rectangles", "zfv_batch_layout", "zfv_batch_dt", "zfv_graph_enable",
+    "zfv_set_cell_order", "zfv_get_cell_order",
     "zfv_adjoint_forward", "zfv_adjoint_backward",
     "zfv_mm_configure", "zfv_mm_set_crp", "zfv_mm_set_max_dt", "zfv_mm_set_state", "zfv_mm_state", "zfv_mm_step", "zfv_mm_dt", "zfv_mm_crp_faces",
     "zfv_mm_interface",
 ]
-CELL, QUAD = 0, 1   # step-kernel layouts
+CELL, QUAD, BATCH = 0, 1, 2   # step-kernel layouts
 
 
 class ZfvError(RuntimeError):
@@ -95,6 +96,8 @@ def lib():
         L.zfv_autotune.argtypes = [vp, C.c_int, vp]
         L.zfv_batch_rectangles.argtypes = [vp, C.c_int, vp, vp, vp, vp]
         L.zfv_graph_enable.argtypes = [vp, C.c_int]
+        L.zfv_set_cell_order.argtypes = [vp, C.c_int]
+        L.zfv_get_cell_order.argtypes = [vp, vp, vp]
         L.zfv_mm_configure.argtypes = [vp, C.c_int, vp, C.c_double]
         L.zfv_mm_set_max_dt.argtypes = [vp, C.c_double]
         L.zfv_mm_set_crp.argtypes = [vp, C.c_int]
@@ -295,12 +298,24 @@ class Context:
 
     def autotune(self, reps=5):
         """ms per launch, [stage][layout][block 128/256/512]."""
-        ms = np.zeros(18)
+        ms = np.zeros(27)
         _check(lib().zfv_autotune(self.h, reps, _ptr(ms)))
-        return ms.reshape(3, 2, 3)
+        return ms.reshape(3, 3, 3)
 
     def graph_enable(self, on=True):
         _check(lib().zfv_graph_enable(self.h, 1 if on else 0))
+
+    def set_cell_order(self, curve=True):
+        """Curve order for the step kernels of refined meshes (default) or
+        the mesh's own order throughout; results are identical."""
+        _check(lib().zfv_set_cell_order(self.h, 1 if curve else 0))
+
+    def cell_order(self):
+        """(mode, active): mode 1 = curve order for steps; active = the
+        state is currently held in curve order."""
+        m, a = C.c_int(0), C.c_int(0)
+        _check(lib().zfv_get_cell_order(self.h, C.byref(m), C.byref(a)))
+        return m.value, a.value
 
     # ---- multi-material solver (first order, no CRP)
     def mm_configure(self, materials, cfl=0.7, crp=True):

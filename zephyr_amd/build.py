@@ -18,6 +18,7 @@ SOURCES = [
     os.path.join(HERE, "csrc", "zfv_amr.hip"),
     os.path.join(HERE, "csrc", "zfv_adjoint.hip"),
     os.path.join(HERE, "csrc", "zfv_mm.hip"),
+    os.path.join(HERE, "csrc", "zfv_order.hip"),
     os.path.join(HERE, "csrc", "zfv_capi.cpp"),
 ]
 HEADERS = [

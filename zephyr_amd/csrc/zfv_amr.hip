@@ -16,7 +16,7 @@
 // Single process, Cartesian base: every adjacency rotation is 0.
 #include "zfv_internal.hpp"
 
-#include <hipcub/hipcub.hpp>
+#include "zfv_scan.hpp"
 
 namespace zfv {
 
@@ -747,13 +747,18 @@ hipError_t amr_balance(const AmrMesh& A, int max_level, hipStream_t s) {
 }
 
 struct Add3 {
-    __host__ __device__ Marks3 operator()(const Marks3& a, const Marks3& b) const {
+    __device__ __forceinline__ Marks3 operator()(const Marks3& a, const Marks3& b) const {
         return Marks3{a.ref + b.ref, a.head + b.head, a.nz + b.nz};
+    }
+    __device__ static __forceinline__ Marks3 shfl(const Marks3& v, int d) {
+        return Marks3{__shfl_up(v.ref, d, 64), __shfl_up(v.head, d, 64), __shfl_up(v.nz, d, 64)};
     }
 };
 
-hipError_t amr_scan3(const Marks3* in, Marks3* out, int count, void* tmp, size_t& tmp_bytes, hipStream_t s) {
-    return hipcub::DeviceScan::ExclusiveScan(tmp, tmp_bytes, in, out, Add3(), Marks3{0, 0, 0}, count, s);
+int amr_scan3_tiles(int count) { return scan::tiles_of(count); }
+
+hipError_t amr_scan3(const Marks3* in, Marks3* out, Marks3* tile_sums, int count, Marks3* total, hipStream_t s) {
+    return scan::exclusive(in, out, tile_sums, count, Add3(), Marks3{0, 0, 0}, total, s);
 }
 
 hipError_t amr_mark(const AmrMesh& A, Marks3* marks, hipStream_t s) {

@@ -102,7 +102,8 @@ struct zfv_ctx {
     DBuf<Marks3> s_marks, s_rank;
     DBuf<unsigned long long> s_faces;
     DBuf<int> s_hole_at, s_ref_list, s_crs_list, s_swap_src, s_swap_dst;
-    DBuf<unsigned char> s_tmp;
+    DBuf<Marks3> s_tiles;             // tile sums of the marks scan
+    Marks3* marks_total = nullptr;    // pinned host: totals of the marks scan
     bool faces_pending = false;   // n_faces still on the device (s_faces)
     // device state
     DBuf<double> st[3][5];   // init / next / half layers
@@ -121,7 +122,7 @@ struct zfv_ctx {
     bool dt_valid = false;
     long long step_no = 0;
 
-    // step-kernel layout per stage (kLayoutCell / kLayoutQuad): first-order
+    // step-kernel layout per stage (kLayoutCell / kLayoutQuad / kLayoutBatch): first-order
     // update, gradient + predictor, corrector
     int layout[3] = {kLayoutCell, kLayoutCell, kLayoutCell};
     int block[3] = {kBlock, kBlock, kBlock};   // workgroup size per stage
@@ -174,6 +175,31 @@ struct zfv_ctx {
     } mm;
     long long mesh_epoch = 0;   // incremented whenever the mesh changes
 
+    // Curve order of the step kernels (zfv_order.hip): after a device
+    // adaptation (or the upload of a mesh with refined cells) the steps run
+    // on a copy of the mesh sorted along the base-cell / Morton curve; the
+    // state layer `cur`, the current wait buffer and the last step's
+    // gradients are then held in that order until an entry point that
+    // observes or changes per-cell data (downloads, uploads, flags,
+    // adaptation, totals, the adjoint) puts them back in the mesh's order.
+    struct CurveOrder {
+        int mode = 1;             // 0: always the mesh's order; 1: curve order for steps of refined meshes
+        bool refined = false;     // the mesh has refined cells (curve order differs from row order)
+        bool keys_ok = false;     // levels, z and base indices of the uploaded mesh form valid curve keys
+        int lmax = 0;             // finest level the keys resolve
+        int base_bits = 0;        // bits of the largest base-cell index
+        long long n_base = 0;     // base cells (largest b_idx + 1)
+        bool slots_clean = false; // the dense slot array holds -1 everywhere
+        long long epoch = -1;     // mesh epoch of the sorted copy
+        int cap = 0;              // stride of the copy (= the mesh's capacity when built)
+        bool state = false;       // state layer cur and wait[wcur] in curve order
+        bool grad = false;        // gdx / gdy last written in curve order
+        DBuf<double> cx, cy, vol, h, area, fcx, fcy, w, nx, ny, gtmp;
+        DBuf<int> code, adj, perm, iperm, vals, slots, counts, offsets, tiles;
+        DBuf<unsigned long long> keys[2];
+        DBuf<unsigned char> tmp;
+    } co;
+
     bool profile = false;
     std::vector<KernelTimer> timers;   // pending event pairs of the last zfv_smf_step
     std::vector<hipEvent_t> event_pool;
@@ -189,6 +215,20 @@ struct zfv_ctx {
         M.chunk_pid = nseg > 1 ? chunk_pid.p : nullptr;
         M.seg_begin = nseg > 1 ? seg_begin.p : nullptr;
         M.seg_end = nseg > 1 ? seg_end.p : nullptr;
+        return M;
+    }
+    // The mesh in the order the state is currently held in.
+    DevMesh smesh() const {
+        if (!co.state) return dmesh();
+        DevMesh M;
+        M.n = hm.n;
+        M.stride = co.cap;
+        M.cx = co.cx.p; M.cy = co.cy.p; M.vol = co.vol.p; M.h = co.h.p;
+        M.code = co.code.p; M.adj = co.adj.p; M.area = co.area.p; M.fcx = co.fcx.p; M.fcy = co.fcy.p;
+        M.w = co.w.p; M.nx = co.nx.p; M.ny = co.ny.p;
+        M.chunk_pid = nullptr;
+        M.seg_begin = nullptr;
+        M.seg_end = nullptr;
         return M;
     }
     AmrMesh amesh() const {
@@ -367,6 +407,25 @@ int upload_mesh(zfv_ctx* ctx) {
     ctx->dt_valid = false;
     ctx->cur = 0;
     ctx->wcur = 0;
+    // curve keys need 0 <= level <= 15, 0 <= z_idx < 4^level, b_idx >= 0
+    int lmax = 0, bmax = 0;
+    bool keys_ok = true;
+    for (int c = 0; c < n; ++c) {
+        const int L = m.level[c], z = m.z_idx[c], b = m.b_idx[c];
+        keys_ok = keys_ok && L >= 0 && L <= 15 && z >= 0 && (long long)z < (1LL << (2 * L)) && b >= 0;
+        lmax = std::max(lmax, L);
+        bmax = std::max(bmax, b);
+    }
+    ctx->co.keys_ok = keys_ok && ctx->nseg == 1;
+    ctx->co.refined = ctx->co.keys_ok && lmax > 0;
+    ctx->co.n_base = (long long)bmax + 1;
+    ctx->co.lmax = std::min(lmax, 15);
+    int bits = 0;
+    while (bits < 31 && (1LL << bits) <= bmax) ++bits;
+    ctx->co.base_bits = bits;
+    ctx->co.state = false;
+    ctx->co.grad = false;
+    ctx->co.epoch = -1;
     return ZFV_OK;
 }
 
@@ -521,7 +580,7 @@ int ensure_dt(zfv_ctx* ctx) {
     if (ctx->dt_valid) return ZFV_OK;
     int rd = (int)(ctx->step_no % 3);
     ZFV_HIP(launch_fill_u64(ctx->dtmin.p, 3 * ctx->nseg, 0x7ff0000000000000ULL, ctx->stream));
-    DevMesh M = ctx->dmesh();
+    DevMesh M = ctx->smesh();
     DevState z = ctx->layer(ctx->cur);
     Eos eos = ctx->eos();
     ZFV_HIP(timed(ctx, "dt", [&] { return launch_dt(M, z, eos, ctx->dtmin.p + rd, ctx->stream); }));
@@ -552,6 +611,115 @@ int need_ready(zfv_ctx* ctx, bool state = true, bool config = false) {
 }
 
 int put_max_dt(zfv_ctx* ctx);
+
+// Sorted copy of the mesh for the step kernels (state in the mesh's order).
+int build_curve(zfv_ctx* ctx) {
+    auto& co = ctx->co;
+    const int n = ctx->hm.n, cap = ctx->cap;
+    hipStream_t s = ctx->stream;
+    if (co.cap != cap) {
+        for (auto* b : {&co.cx, &co.cy, &co.vol, &co.h}) ZFV_HIP(b->alloc(cap));
+        for (auto* b : {&co.area, &co.fcx, &co.fcy, &co.w, &co.nx, &co.ny}) ZFV_HIP(b->alloc(8 * (size_t)cap));
+        ZFV_HIP(co.code.alloc(cap));
+        ZFV_HIP(co.adj.alloc(8 * (size_t)cap));
+        for (auto* b : {&co.perm, &co.iperm, &co.vals}) ZFV_HIP(b->alloc(cap));
+        for (auto& k : co.keys) ZFV_HIP(k.alloc(cap));
+        co.cap = cap;
+    }
+    // The order: dense slots (one per finest-level Morton cell of every base
+    // cell) compacted in order when they are at most 64 per cell, else a
+    // radix sort of the same keys.
+    const long long n_slots = co.n_base << (2 * co.lmax);
+    const long long n_chunks = (n_slots + 63) / 64;
+    size_t bytes = 0;
+    if (n_slots <= 64LL * std::max(n, 1) && n_chunks * 64 < (1LL << 31)) {
+        if (co.slots.n < (size_t)(n_chunks * 64)) {
+            ZFV_HIP(co.slots.alloc((size_t)(n_chunks * 64)));
+            co.slots_clean = false;
+        }
+        if (co.counts.n < (size_t)n_chunks) {
+            ZFV_HIP(co.counts.alloc((size_t)n_chunks));
+            co.slots_clean = false;
+        }
+        if (co.offsets.n < (size_t)n_chunks) ZFV_HIP(co.offsets.alloc((size_t)n_chunks));
+        const size_t tiles = (size_t)std::max(curve_scan_tiles((int)n_chunks), 1);
+        if (co.tiles.n < tiles) ZFV_HIP(co.tiles.alloc(tiles));
+        if (!co.slots_clean) {   // a completed build leaves every slot at -1 and every count at 0
+            ZFV_HIP(hipMemsetAsync(co.slots.p, 0xff, co.slots.n * sizeof(int), s));
+            ZFV_HIP(hipMemsetAsync(co.counts.p, 0, co.counts.n * sizeof(int), s));
+        }
+        co.slots_clean = false;
+        ZFV_HIP(curve_slots(ctx->b_idx.p, ctx->z_idx.p, ctx->level.p, n, co.lmax, co.slots.p, co.counts.p, n_slots,
+                            s));
+        ZFV_HIP(curve_compact(co.slots.p, (int)n_chunks, co.counts.p, co.offsets.p, co.tiles.p, co.perm.p,
+                              co.iperm.p, s));
+        co.slots_clean = true;
+    } else {
+        ZFV_HIP(curve_keys(ctx->b_idx.p, ctx->z_idx.p, ctx->level.p, n, co.lmax, co.keys[0].p, co.vals.p, s));
+        const int end_bit = std::min(64, 2 * co.lmax + co.base_bits);
+        ZFV_HIP(curve_sort(co.keys[0].p, co.keys[1].p, co.vals.p, co.perm.p, n, end_bit, nullptr, bytes, s));
+        if (co.tmp.n < bytes) ZFV_HIP(co.tmp.alloc(bytes));
+        ZFV_HIP(curve_sort(co.keys[0].p, co.keys[1].p, co.vals.p, co.perm.p, n, end_bit, co.tmp.p, bytes, s));
+        ZFV_HIP(curve_inverse(co.perm.p, co.iperm.p, n, s));
+    }
+    const CurveMesh C{co.cap, co.cx.p, co.cy.p, co.vol.p, co.h.p, co.code.p, co.adj.p,
+                      co.area.p, co.fcx.p, co.fcy.p, co.w.p, co.nx.p, co.ny.p};
+    ZFV_HIP(curve_mesh(ctx->dmesh(), C, co.perm.p, co.iperm.p, s));
+    co.epoch = ctx->mesh_epoch;
+    ctx->graphs_clear();   // captured steps read the previous copy
+    return ZFV_OK;
+}
+
+// State layer cur and wait[wcur] gathered through idx into the other layer
+// and wait buffer, which become the current ones.
+int gather_current(zfv_ctx* ctx, const int* idx) {
+    const int nxt = 1 - ctx->cur;
+    ZFV_HIP(gather_state(ctx->layer(ctx->cur), ctx->layer(nxt), ctx->wait[ctx->wcur].p, ctx->wait[1 - ctx->wcur].p,
+                         idx, ctx->hm.n, ctx->stream));
+    ctx->cur = nxt;
+    ctx->wcur = 1 - ctx->wcur;
+    return ZFV_OK;
+}
+
+// Before steps: the state in curve order when the mesh has refined cells.
+int to_curve_order(zfv_ctx* ctx) {
+    auto& co = ctx->co;
+    if (co.state || co.mode == 0 || !co.refined || ctx->nseg > 1) return ZFV_OK;
+    if (co.epoch != ctx->mesh_epoch || co.cap != ctx->cap) {
+        int r = build_curve(ctx);
+        if (r) return r;
+    }
+    int r = gather_current(ctx, co.perm.p);
+    if (r) return r;
+    co.state = true;
+    return ZFV_OK;
+}
+
+// The state (and wait counters) back in the mesh's order.
+int to_mesh_order(zfv_ctx* ctx) {
+    auto& co = ctx->co;
+    if (!co.state) return ZFV_OK;
+    int r = gather_current(ctx, co.iperm.p);
+    if (r) return r;
+    co.state = false;
+    return ZFV_OK;
+}
+
+// The last gradients back in the mesh's order.
+int grad_to_mesh_order(zfv_ctx* ctx) {
+    auto& co = ctx->co;
+    if (!co.grad) return ZFV_OK;
+    const int n = ctx->hm.n, cap = ctx->cap;
+    if (co.epoch != ctx->mesh_epoch || co.cap != cap) return fail(ZFV_ERR_STATE, "gradients of a previous mesh");
+    if (co.gtmp.n < 4 * (size_t)cap) ZFV_HIP(co.gtmp.alloc(4 * (size_t)cap));
+    for (double* g : {ctx->gdx.p, ctx->gdy.p}) {
+        ZFV_HIP(gather_rows(g, co.gtmp.p, co.iperm.p, n, 4, cap, ctx->stream));
+        ZFV_HIP(hipMemcpy2DAsync(g, (size_t)cap * 8, co.gtmp.p, (size_t)cap * 8, (size_t)n * 8, 4,
+                                 hipMemcpyDeviceToDevice, ctx->stream));
+    }
+    co.grad = false;
+    return ZFV_OK;
+}
 
 }  // namespace
 
@@ -592,6 +760,7 @@ int zfv_destroy(zfv_ctx* ctx) {
     collect_timers(ctx);
     ctx->graphs_clear();
     for (hipEvent_t e : ctx->event_pool) (void)hipEventDestroy(e);
+    if (ctx->marks_total) (void)hipHostFree(ctx->marks_total);
     hipStream_t s = ctx->own_stream;
     delete ctx;
     if (s) (void)hipStreamDestroy(s);
@@ -926,6 +1095,8 @@ int zfv_state_upload(zfv_ctx* ctx, const double* rho, const double* u, const dou
                      const double* e) {
     int r = need_ready(ctx, false);
     if (r) return r;
+    r = to_mesh_order(ctx);
+    if (r) return r;
     const double* a[5] = {rho, u, v, p, e};
     r = state_copy(ctx, a, true, hipMemcpyHostToDevice);
     if (r) return r;
@@ -938,6 +1109,8 @@ int zfv_state_upload_device(zfv_ctx* ctx, const double* rho, const double* u, co
                             const double* p, const double* e) {
     int r = need_ready(ctx, false);
     if (r) return r;
+    r = to_mesh_order(ctx);
+    if (r) return r;
     const double* a[5] = {rho, u, v, p, e};
     r = state_copy(ctx, a, true, hipMemcpyDeviceToDevice);
     if (r) return r;
@@ -949,6 +1122,8 @@ int zfv_state_upload_device(zfv_ctx* ctx, const double* rho, const double* u, co
 int zfv_state_download(zfv_ctx* ctx, double* rho, double* u, double* v, double* p, double* e) {
     int r = need_ready(ctx);
     if (r) return r;
+    r = to_mesh_order(ctx);
+    if (r) return r;
     const double* a[5] = {rho, u, v, p, e};
     r = state_copy(ctx, a, false, hipMemcpyDeviceToHost);
     if (r) return r;
@@ -957,6 +1132,8 @@ int zfv_state_download(zfv_ctx* ctx, double* rho, double* u, double* v, double* 
 
 int zfv_state_download_device(zfv_ctx* ctx, double* rho, double* u, double* v, double* p, double* e) {
     int r = need_ready(ctx);
+    if (r) return r;
+    r = to_mesh_order(ctx);
     if (r) return r;
     const double* a[5] = {rho, u, v, p, e};
     r = state_copy(ctx, a, false, hipMemcpyDeviceToDevice);
@@ -967,6 +1144,8 @@ int zfv_state_download_device(zfv_ctx* ctx, double* rho, double* u, double* v, d
 int zfv_wait_upload(zfv_ctx* ctx, const int32_t* wait) {
     int r = need_ready(ctx, false);
     if (r) return r;
+    r = to_mesh_order(ctx);
+    if (r) return r;
     if (!wait) return fail(ZFV_ERR_ARG, "null wait");
     ZFV_HIP(hipMemcpyAsync(ctx->wait[ctx->wcur].p, wait, ctx->hm.n * sizeof(int), hipMemcpyHostToDevice, ctx->stream));
     ZFV_HIP(hipStreamSynchronize(ctx->stream));
@@ -976,6 +1155,8 @@ int zfv_wait_upload(zfv_ctx* ctx, const int32_t* wait) {
 int zfv_wait_download(zfv_ctx* ctx, int32_t* wait) {
     int r = need_ready(ctx, false);
     if (r) return r;
+    r = to_mesh_order(ctx);
+    if (r) return r;
     if (!wait) return fail(ZFV_ERR_ARG, "null wait");
     ZFV_HIP(hipMemcpyAsync(wait, ctx->wait[ctx->wcur].p, ctx->hm.n * sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
     ZFV_HIP(hipStreamSynchronize(ctx->stream));
@@ -984,6 +1165,8 @@ int zfv_wait_download(zfv_ctx* ctx, int32_t* wait) {
 
 int zfv_grad_download(zfv_ctx* ctx, double* dx, double* dy) {
     int r = need_ready(ctx);
+    if (r) return r;
+    r = grad_to_mesh_order(ctx);
     if (r) return r;
     if (!dx || !dy) return fail(ZFV_ERR_ARG, "null output");
     const size_t row = (size_t)ctx->hm.n * sizeof(double), pitch = (size_t)ctx->cap * sizeof(double);
@@ -996,7 +1179,7 @@ int zfv_grad_download(zfv_ctx* ctx, double* dx, double* dy) {
 // SmFluid::update x n_steps (sm_fluid.cpp:81-102).
 // One SmFluid::update (sm_fluid.cpp:81-102) enqueued on the context stream.
 static int step_once(zfv_ctx* ctx) {
-    const DevMesh M = ctx->dmesh();
+    const DevMesh M = ctx->smesh();
     const Eos eos = ctx->eos();
     const zfv_smf_params& P = ctx->prm;
     StepCtl ctl = make_ctl(ctx);
@@ -1020,6 +1203,7 @@ static int step_once(zfv_ctx* ctx) {
                                ctx->stream);
         }));
         ctx->wcur = 1 - ctx->wcur;
+        ctx->co.grad = ctx->co.state;
     }
     ctx->cur = nxt;
     ctx->step_no += 1;
@@ -1031,7 +1215,7 @@ static int step_once(zfv_ctx* ctx) {
 // output), so the recomputed segment lands where the reverse sweep reads it.
 static int recompute_step(zfv_ctx* ctx, const DevState& in, const DevState& half, const DevGrad& G,
                           const DevState& out) {
-    const DevMesh M = ctx->dmesh();
+    const DevMesh M = ctx->smesh();
     const Eos eos = ctx->eos();
     const zfv_smf_params& P = ctx->prm;
     StepCtl ctl = make_ctl(ctx);
@@ -1051,7 +1235,8 @@ static int recompute_step(zfv_ctx* ctx, const DevState& in, const DevState& half
 constexpr int kGraphSteps = 6;
 
 static int graph_for_phase(zfv_ctx* ctx, hipGraphExec_t* out) {
-    const int key = (int)(ctx->step_no % 3) | (ctx->cur << 2) | (ctx->wcur << 3) | (ctx->prm.acc << 4);
+    const int key = (int)(ctx->step_no % 3) | (ctx->cur << 2) | (ctx->wcur << 3) | (ctx->prm.acc << 4) |
+                    ((ctx->co.state ? 1 : 0) << 7);
     for (auto& g : ctx->graphs)
         if (g.key == key) {
             *out = g.exec;
@@ -1085,6 +1270,8 @@ int zfv_smf_step(zfv_ctx* ctx, int n_steps) {
     int r = need_ready(ctx, true, true);
     if (r) return r;
     if (n_steps < 0) return fail(ZFV_ERR_ARG, "negative step count");
+    r = to_curve_order(ctx);
+    if (r) return r;
     r = ensure_dt(ctx);
     if (r) return r;
     int it = 0;
@@ -1113,6 +1300,29 @@ int zfv_graph_enable(zfv_ctx* ctx, int on) {
     return ZFV_OK;
 }
 
+int zfv_set_cell_order(zfv_ctx* ctx, int mode) {
+    int r = check_ctx(ctx);
+    if (r) return r;
+    if (mode != 0 && mode != 1) return fail(ZFV_ERR_ARG, "cell order must be 0 (mesh) or 1 (curve for steps)");
+    if (mode == 0 && ctx->have_mesh) {
+        r = to_mesh_order(ctx);
+        if (r) return r;
+        r = grad_to_mesh_order(ctx);
+        if (r) return r;
+    }
+    ctx->co.mode = mode;
+    ctx->graphs_clear();
+    return ZFV_OK;
+}
+
+int zfv_get_cell_order(zfv_ctx* ctx, int* mode, int* active) {
+    int r = check_ctx(ctx);
+    if (r) return r;
+    if (mode) *mode = ctx->co.mode;
+    if (active) *active = ctx->co.state ? 1 : 0;
+    return ZFV_OK;
+}
+
 int zfv_smf_dt(zfv_ctx* ctx, double* dt) {
     int r = need_ready(ctx, false);
     if (r) return r;
@@ -1125,6 +1335,9 @@ int zfv_smf_dt(zfv_ctx* ctx, double* dt) {
 int zfv_smf_compute_grad(zfv_ctx* ctx) {
     int r = need_ready(ctx, true, true);
     if (r) return r;
+    r = to_mesh_order(ctx);
+    if (r) return r;
+    ctx->co.grad = false;
     r = ensure_dt(ctx);
     if (r) return r;
     StepCtl ctl = make_ctl(ctx);
@@ -1154,6 +1367,8 @@ int zfv_smf_compute_dt(zfv_ctx* ctx, double* dt) {
 
 int zfv_conserved_totals(zfv_ctx* ctx, double* out4) {
     int r = need_ready(ctx, true, true);
+    if (r) return r;
+    r = to_mesh_order(ctx);   // the partial sums run in the mesh's order
     if (r) return r;
     if (!out4) return fail(ZFV_ERR_ARG, "null output");
     const int nb = (ctx->hm.n + kBlock - 1) / kBlock;
@@ -1192,6 +1407,9 @@ int zfv_set_flags(zfv_ctx* ctx, int criterion, double threshold) {
     int r = need_ready(ctx, true, true);
     if (r) return r;
     if (criterion != 0 && criterion != 1) return fail(ZFV_ERR_ARG, "unknown refinement criterion");
+    r = to_mesh_order(ctx);
+    if (r) return r;
+    ctx->co.grad = false;
     r = ensure_dt(ctx);
     if (r) return r;
     StepCtl ctl = make_ctl(ctx);
@@ -1216,11 +1434,13 @@ int zfv_balance_flags(zfv_ctx* ctx, int max_level) {
     return ZFV_OK;
 }
 
+// Exclusive scan of the marks; the totals land in pinned host memory
+// (ctx->marks_total), valid after the next synchronisation.
 static int scan3(zfv_ctx* ctx, const Marks3* in, Marks3* out, int count) {
-    size_t bytes = 0;
-    ZFV_HIP(amr_scan3(in, out, count, nullptr, bytes, ctx->stream));
-    if (ctx->s_tmp.n < bytes) ZFV_HIP(ctx->s_tmp.alloc(bytes));
-    ZFV_HIP(amr_scan3(in, out, count, ctx->s_tmp.p, bytes, ctx->stream));
+    const int tiles = amr_scan3_tiles(count);
+    if (ctx->s_tiles.n < (size_t)std::max(tiles, 1)) ZFV_HIP(ctx->s_tiles.alloc(std::max(tiles, 1)));
+    if (!ctx->marks_total) ZFV_HIP(hipHostMalloc((void**)&ctx->marks_total, sizeof(Marks3), hipHostMallocDefault));
+    ZFV_HIP(amr_scan3(in, out, ctx->s_tiles.p, count, ctx->marks_total, ctx->stream));
     return ZFV_OK;
 }
 
@@ -1237,6 +1457,10 @@ int zfv_refine(zfv_ctx* ctx, int max_level, int distributor) {
     if (ctx->nseg > 1) return fail(ZFV_ERR_UNSUPPORTED, "adaptation of a batch of problems");
     hipStream_t s = ctx->stream;
     if (max_level == 0) return ZFV_OK;   // EuMesh::adaptive() is false
+    r = to_mesh_order(ctx);
+    if (r) return r;
+    r = grad_to_mesh_order(ctx);   // the slope distributor reads them
+    if (r) return r;
     const int n = ctx->hm.n;
     ZFV_HIP(amr_balance(ctx->amesh(), max_level, s));
     ctx->host_stale = true;
@@ -1245,13 +1469,11 @@ int zfv_refine(zfv_ctx* ctx, int max_level, int distributor) {
     ZFV_HIP(amr_mark(ctx->amesh(), ctx->s_marks.p, s));
     r = scan3(ctx, ctx->s_marks.p, ctx->s_rank.p, n);
     if (r) return r;
-    Marks3 tail[2];
-    ZFV_HIP(hipMemcpyAsync(&tail[0], ctx->s_rank.p + (n - 1), sizeof(Marks3), hipMemcpyDeviceToHost, s));
-    ZFV_HIP(hipMemcpyAsync(&tail[1], ctx->s_marks.p + (n - 1), sizeof(Marks3), hipMemcpyDeviceToHost, s));
     ZFV_HIP(hipStreamSynchronize(s));
-    const int n_refine = tail[0].ref + tail[1].ref;
-    const int n_parents = tail[0].head + tail[1].head;
-    const int nz_total = tail[0].nz + tail[1].nz;   // refined + coarsened cells
+    const volatile Marks3* tot = ctx->marks_total;
+    const int n_refine = tot->ref;
+    const int n_parents = tot->head;
+    const int nz_total = tot->nz;   // refined + coarsened cells
     const int n_children = 4 * n_refine;
     const long long n_large_ll = (long long)n + n_parents + n_children;
     if (n_large_ll > std::numeric_limits<int>::max() / 9) return fail(ZFV_ERR_ARG, "adapted mesh too large");
@@ -1288,6 +1510,8 @@ int zfv_refine(zfv_ctx* ctx, int max_level, int distributor) {
     ctx->dt_valid = true;
     ctx->graphs_clear();   // the cell count changed
     ctx->mesh_epoch += 1;
+    ctx->co.refined = ctx->co.keys_ok;
+    ctx->co.lmax = std::max(ctx->co.lmax, max_level);
     return ZFV_OK;
 }
 
@@ -1338,6 +1562,10 @@ int zfv_adjoint_forward(zfv_ctx* ctx, int n_steps, int every, const double* wind
     int r = need_ready(ctx, true, true);
     if (r) return r;
     if (n_steps < 1 || every < 1 || !window || !J) return fail(ZFV_ERR_ARG, "adjoint_forward: bad arguments");
+    r = to_mesh_order(ctx);   // checkpoints and the reverse sweep use the mesh's order
+    if (r) return r;
+    r = grad_to_mesh_order(ctx);
+    if (r) return r;
     r = adjoint_supported(ctx);
     if (r) return r;
     auto& A = ctx->adjoint;
@@ -1401,6 +1629,8 @@ int zfv_adjoint_backward(zfv_ctx* ctx, double* lr, double* lu, double* lv, doubl
     if (r) return r;
     auto& A = ctx->adjoint;
     if (!A.ready) return fail(ZFV_ERR_STATE, "adjoint_backward: no forward run to reverse");
+    r = to_mesh_order(ctx);
+    if (r) return r;
     if (!lr || !lu || !lv || !lp || !le) return fail(ZFV_ERR_ARG, "adjoint_backward: null output");
     const int n = ctx->hm.n, cap = ctx->cap;
     // reverse-sweep buffers: segment states, adjoint states, per-slot face-state adjoints
@@ -1713,7 +1943,7 @@ int zfv_set_launch(zfv_ctx* ctx, int stage, int layout, int block) {
     int r = check_ctx(ctx);
     if (r) return r;
     if (stage < 0 || stage > 2) return fail(ZFV_ERR_ARG, "stage must be 0 (first order), 1 (gradient), 2 (corrector)");
-    if (layout != kLayoutCell && layout != kLayoutQuad) return fail(ZFV_ERR_ARG, "layout must be 0 (cell) or 1 (quad)");
+    if (layout < 0 || layout >= kLayouts) return fail(ZFV_ERR_ARG, "layout must be 0 (cell), 1 (quad) or 2 (batched)");
     if (block != 0 && !valid_block(block)) return fail(ZFV_ERR_ARG, "block must be 0 (keep), 128, 256 or 512");
     ctx->layout[stage] = layout;
     if (block) ctx->block[stage] = block;
@@ -1739,10 +1969,12 @@ int zfv_autotune(zfv_ctx* ctx, int reps, double* ms_out) {
     int r = need_ready(ctx, true, true);
     if (r) return r;
     if (reps <= 0) return fail(ZFV_ERR_ARG, "reps must be positive");
+    r = to_curve_order(ctx);   // tune on the order the steps will run in
+    if (r) return r;
     r = ensure_dt(ctx);
     if (r) return r;
     hipStream_t s = ctx->stream;
-    const DevMesh M = ctx->dmesh();
+    const DevMesh M = ctx->smesh();
     const Eos eos = ctx->eos();
     const zfv_smf_params& P = ctx->prm;
     const StepCtl ctl = make_ctl(ctx);
@@ -1764,13 +1996,18 @@ int zfv_autotune(zfv_ctx* ctx, int reps, double* ms_out) {
         return ZFV_OK;
     };
     const int blocks[3] = {128, 256, 512};
-    double t[3][2][3] = {};   // [stage][layout][block]
+    double t[3][kLayouts][3] = {};   // [stage][layout][block]
     const int stages[2] = {P.acc == 1 ? 0 : 1, P.acc == 1 ? -1 : 2};
     for (int st : stages) {
         if (st < 0) continue;
-        for (int L = 0; L < 2; ++L)
+        for (int L = 0; L < kLayouts; ++L)
             for (int bi = 0; bi < 3; ++bi) {
                 const int B = blocks[bi];
+                if (st == 0 && L == kLayoutBatch) {
+                    // the first-order kernel is the same in both cell layouts
+                    t[st][L][bi] = t[st][kLayoutCell][bi];
+                    continue;
+                }
                 if (st == 0)
                     r = time_it([&] { return launch_flux1(P.flux, L, B, M, in, out, eos, ctl, s); }, t[st][L][bi]);
                 else if (st == 1)
@@ -1782,7 +2019,7 @@ int zfv_autotune(zfv_ctx* ctx, int reps, double* ms_out) {
                 if (r) return r;
             }
         int bl = 0, bb = 0;
-        for (int L = 0; L < 2; ++L)
+        for (int L = 0; L < kLayouts; ++L)
             for (int bi = 0; bi < 3; ++bi)
                 if (t[st][L][bi] < t[st][bl][bb]) {
                     bl = L;
@@ -1795,8 +2032,9 @@ int zfv_autotune(zfv_ctx* ctx, int reps, double* ms_out) {
     ctx->event_pool.push_back(e1);
     if (ms_out)
         for (int st = 0; st < 3; ++st)
-            for (int L = 0; L < 2; ++L)
-                for (int bi = 0; bi < 3; ++bi) ms_out[(st * 2 + L) * 3 + bi] = t[st][L][bi];
+            for (int L = 0; L < kLayouts; ++L)
+                for (int bi = 0; bi < 3; ++bi) ms_out[(st * kLayouts + L) * 3 + bi] = t[st][L][bi];
+    ctx->co.grad = ctx->co.state;
     ctx->graphs_clear();
     return read_err(ctx);
 }

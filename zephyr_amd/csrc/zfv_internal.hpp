@@ -94,7 +94,10 @@ hipError_t amr_balance(const AmrMesh& A, int max_level, hipStream_t s);
 struct Marks3 {
     int ref, head, nz;
 };
-hipError_t amr_scan3(const Marks3* in, Marks3* out, int count, void* tmp, size_t& tmp_bytes, hipStream_t s);
+// Exclusive scan of the marks (tile_sums: amr_scan3_tiles(count) entries);
+// the total of all marks goes to `total` (device or pinned host memory).
+int amr_scan3_tiles(int count);
+hipError_t amr_scan3(const Marks3* in, Marks3* out, Marks3* tile_sums, int count, Marks3* total, hipStream_t s);
 hipError_t amr_mark(const AmrMesh& A, Marks3* marks, hipStream_t s);
 hipError_t amr_positions(const AmrMesh& A, int n_large, int n_short, int n_parents, const Marks3* rank,
                          int* hole_at, int* ref_list, int* crs_list, unsigned long long* faces,
@@ -109,12 +112,50 @@ hipError_t amr_move(const AmrMesh& A, const DevState& z, int* wait, int n_short,
 hipError_t amr_derive(const AmrMesh& A, const DevState& z, const Eos& eos, unsigned long long* counts,
                       unsigned long long* dt_slot, int* err, hipStream_t s);
 
+// Curve-ordered copy of the step kernels' mesh arrays (zfv_order.hip).
+struct CurveMesh {
+    int stride;
+    double *cx, *cy, *vol, *h;
+    int *code, *adj;
+    double *area, *fcx, *fcy, *w, *nx, *ny;
+};
+
+// Keys of the space-filling-curve order (base cell, then Morton index at
+// level lmax inside it) and the identity as sort values.
+hipError_t curve_keys(const int* b_idx, const int* z_idx, const int* level, int n, int lmax,
+                      unsigned long long* keys, int* vals, hipStream_t s);
+// Dense-slot form of the same order (when n_base 4^lmax slots fit): every
+// cell writes its index into its slot of an array that holds -1 elsewhere
+// and counts itself in its 64-slot chunk (counts hold 0 before); the chunks
+// are compacted in order into perm and iperm, and slots and counts are
+// reset.  tile_sums: curve_scan_tiles(n_chunks) ints.
+hipError_t curve_slots(const int* b_idx, const int* z_idx, const int* level, int n, int lmax, int* slots,
+                       int* counts, long long n_slots, hipStream_t s);
+int curve_scan_tiles(int n_chunks);
+hipError_t curve_compact(int* slots, int n_chunks, int* counts, int* offsets, int* tile_sums, int* perm, int* iperm,
+                         hipStream_t s);
+hipError_t curve_inverse(const int* perm, int* iperm, int n, hipStream_t s);
+// Radix sort of (key, value) pairs on bits [0, end_bit); tmp == nullptr
+// sizes the scratch.
+hipError_t curve_sort(const unsigned long long* keys_in, unsigned long long* keys_out, const int* vals_in,
+                      int* vals_out, int n, int end_bit, void* tmp, size_t& tmp_bytes, hipStream_t s);
+// The curve-ordered mesh copy (perm: curve -> mesh index, iperm: inverse).
+hipError_t curve_mesh(const DevMesh& src, const CurveMesh& dst, const int* perm, const int* iperm, hipStream_t s);
+// dst[i] = src[idx[i]]: the five state rows and the wait counters, or
+// `rows` rows of a [rows][stride] array.
+hipError_t gather_state(const DevState& src, const DevState& dst, const int* wsrc, int* wdst, const int* idx, int n,
+                        hipStream_t s);
+hipError_t gather_rows(const double* src, double* dst, const int* idx, int n, int rows, int stride, hipStream_t s);
+
 // Launchers (zfv_kernels.hip).
 hipError_t launch_fill_u64(unsigned long long* p, int count, unsigned long long v, hipStream_t s);
 hipError_t launch_dt(const DevMesh& M, const DevState& z, const Eos& eos, unsigned long long* slot, hipStream_t s);
 
-// Step-kernel layouts: one thread per cell, or a lane quad per cell.
-constexpr int kLayoutCell = 0, kLayoutQuad = 1;
+// Step-kernel layouts: one thread per cell, a lane quad per cell, or one
+// thread per cell with the main slots' loads batched (more registers, fewer
+// resident waves, fewer dependent memory round trips per cell).
+constexpr int kLayoutCell = 0, kLayoutQuad = 1, kLayoutBatch = 2;
+constexpr int kLayouts = 3;
 
 hipError_t launch_flux1(int flux, int layout, int block, const DevMesh& M, const DevState& in, const DevState& out,
                         const Eos& eos, const StepCtl& ctl, hipStream_t s);

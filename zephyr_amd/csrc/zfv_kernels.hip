@@ -28,6 +28,12 @@
 #ifndef ZFV_WAVES_CORR
 #define ZFV_WAVES_CORR 4
 #endif
+#ifndef ZFV_WAVES_CORRB
+#define ZFV_WAVES_CORRB 2
+#endif
+#ifndef ZFV_WAVES_GRADB
+#define ZFV_WAVES_GRADB 2
+#endif
 #ifndef ZFV_WAVES_FLUX1Q
 #define ZFV_WAVES_FLUX1Q 6
 #endif
@@ -206,18 +212,22 @@ __device__ __forceinline__ void lim_add(const SlotL& L, const double* gx, const 
     }
 }
 
-// Predictor flux term of slot s, flux * area (sm_fluid.cpp:176-253).
-__device__ __forceinline__ Q pred_face(const DevMesh& M, int c, int s, int nib, const Z& zc, double ccx, double ccy,
+// Predictor flux term of a slot, flux * area (sm_fluid.cpp:176-253), from
+// the face centre offset (dx, dy), the area and the frame.
+__device__ __forceinline__ Q pred_eval(const Frame& fr, double dx, double dy, double a, const Z& zc,
                                        const double* gx, const double* gy, const Eos& eos) {
-    const int so = s * M.stride + c;
-    const double dx = M.fcx[so] - ccx, dy = M.fcy[so] - ccy, a = M.area[so];
-    const Frame fr = slot_frame(M, c, s, nib);
     Z zm = extrap(zc, gx, gy, dx, dy);
     zm.e = eos_e(eos, zm.r, zm.p);
     if (is_bad(zm, eos)) zm = zc;
     Q f = phys_flux(frame_local(fr, zm));
     f = frame_global(fr, f);
     return Q{f.r * a, f.mx * a, f.my * a, f.E * a};
+}
+
+__device__ __forceinline__ Q pred_face(const DevMesh& M, int c, int s, int nib, const Z& zc, double ccx, double ccy,
+                                       const double* gx, const double* gy, const Eos& eos) {
+    const int so = s * M.stride + c;
+    return pred_eval(slot_frame(M, c, s, nib), M.fcx[so] - ccx, M.fcy[so] - ccy, M.area[so], zc, gx, gy, eos);
 }
 
 // SmFluid::compute_grad (sm_fluid.cpp:116-125): gradient::LSM
@@ -230,9 +240,13 @@ __device__ __forceinline__ Q pred_face(const DevMesh& M, int c, int s, int nib, 
 // Eigen's cofactor inverse reduces to adj(A) * (1/det) with
 // det = A11*A00 + (-A01)*A10.
 // Without the predictor, `raw` (if not null) receives the unlimited LSM
-// gradient of rho ([2][stride]) for the refinement criterion.
-template <int LIM, bool PRED>
-__global__ void __launch_bounds__(kMaxBlock) __attribute__((amdgpu_waves_per_eu(ZFV_WAVES_GRAD)))
+// gradient of rho ([2][stride]) for the refinement criterion.  BATCH (the
+// batched layout): the predictor's face geometry of the main slots is
+// loaded with the LSM loads at the start, so the predictor makes no further
+// memory round trips on cells without hanging faces.
+template <int LIM, bool PRED, bool BATCH>
+__global__ void __launch_bounds__(kMaxBlock)
+__attribute__((amdgpu_waves_per_eu(BATCH ? ZFV_WAVES_GRADB : ZFV_WAVES_GRAD)))
 k_grad_pred(DevMesh M, DevState in, DevGrad G, DevState half,
                                                       const int* __restrict__ wait, Eos eos, StepCtl ctl,
                                                       double* __restrict__ raw) {
@@ -256,6 +270,17 @@ k_grad_pred(DevMesh M, DevState in, DevGrad G, DevState half,
     for (int s = 0; s < 4; ++s) {
         const int nib = (code >> (4 * s)) & 15;
         S[s] = (nib & 7) != SC_UNDEF ? slot_load(M, in, c, s, nib, zc, ccx, ccy) : SlotL{};
+    }
+    double pdx[4], pdy[4], pa[4];
+    if (BATCH && PRED) {
+#pragma unroll
+        for (int s = 0; s < 4; ++s) {
+            const int so = s * M.stride + c;
+            const bool def = ((code >> (4 * s)) & 7) != SC_UNDEF;
+            pdx[s] = def ? M.fcx[so] - ccx : 0.0;
+            pdy[s] = def ? M.fcy[so] - ccy : 0.0;
+            pa[s] = def ? M.area[so] : 0.0;
+        }
     }
     double A00 = 0.0, A01 = 0.0, A10 = 0.0, A11 = 0.0;
     double Fx[4] = {0.0, 0.0, 0.0, 0.0}, Fy[4] = {0.0, 0.0, 0.0, 0.0};
@@ -310,7 +335,19 @@ k_grad_pred(DevMesh M, DevState in, DevGrad G, DevState half,
     Z zh = zc;
     if (wait[c] <= 0) {
         double F0 = 0.0, F1 = 0.0, F2 = 0.0, F3 = 0.0;
-        for (int s = 0; s < 8; ++s) {
+        if (BATCH) {
+#pragma unroll
+            for (int s = 0; s < 4; ++s) {
+                const int nib = (code >> (4 * s)) & 15;
+                if ((nib & 7) == SC_UNDEF) continue;
+                const Q f = pred_eval(slot_frame(M, c, s, nib), pdx[s], pdy[s], pa[s], zc, gx, gy, eos);
+                F0 = F0 + f.r;
+                F1 = F1 + f.mx;
+                F2 = F2 + f.my;
+                F3 = F3 + f.E;
+            }
+        }
+        for (int s = BATCH ? 4 : 0; s < 8; ++s) {
             const int nib = (code >> (4 * s)) & 15;
             if ((nib & 7) == SC_UNDEF) continue;
             const Q f = pred_face(M, c, s, nib, zc, ccx, ccy, gx, gy, eos);
@@ -331,30 +368,52 @@ k_grad_pred(DevMesh M, DevState in, DevGrad G, DevState half,
     store_z(half, c, zh);
 }
 
-// One face of SmFluid::fluxes_stage2 (sm_fluid.cpp:255-366): flux * area of
-// slot s.  Every load of the face is issued in one batch before any choice
-// is made (on boundary slots adj is the cell itself, so the neighbour loads
-// stay in bounds); the isolation, boundary and bad-state choices are then
-// selects among computed states, which keeps one memory round trip per face.
-template <int FLUX>
-__device__ __forceinline__ Q corr_face(const DevMesh& M, const DevState& half, const DevGrad& G,
-                                       const int* __restrict__ wait_in, int c, int s, int nib, const Z& zch, int wc,
-                                       double ccx, double ccy, const double* gx, const double* gy, const Eos& eos,
-                                       bool& bad) {
-    const int t = nib & 7;
+// Everything one corrector face reads: the slot's geometry and frame, and
+// the neighbour's half state, gradient, centre and isolation counter.  On
+// boundary slots adj is the cell itself, so the neighbour loads stay in
+// bounds and can be issued before any choice is made.
+struct FaceC {
+    Z hn;
+    double ngx[4], ngy[4];
+    double ncx, ncy, fx, fy, a;
+    Frame fr;
+    int wnb;
+};
+
+__device__ __forceinline__ FaceC corr_load(const DevMesh& M, const DevState& half, const DevGrad& G,
+                                           const int* __restrict__ wait_in, int c, int s, int nib) {
+    FaceC F;
     const int so = s * M.stride + c;
     const int nb = M.adj[so];
-    const double fx = M.fcx[so], fy = M.fcy[so], a = M.area[so];
-    const Z hn = load_z(half, nb);
-    const int wnb = wait_in[nb];
-    const double ncx = M.cx[nb], ncy = M.cy[nb];
-    double ngx[4], ngy[4];
+    F.fx = M.fcx[so];
+    F.fy = M.fcy[so];
+    F.a = M.area[so];
+    F.hn = load_z(half, nb);
+    F.wnb = wait_in[nb];
+    F.ncx = M.cx[nb];
+    F.ncy = M.cy[nb];
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
-        ngx[k] = G.dx[k * M.stride + nb];
-        ngy[k] = G.dy[k * M.stride + nb];
+        F.ngx[k] = G.dx[k * M.stride + nb];
+        F.ngy[k] = G.dy[k * M.stride + nb];
     }
-    const Frame fr = slot_frame(M, c, s, nib);
+    F.fr = slot_frame(M, c, s, nib);
+    return F;
+}
+
+// One face of SmFluid::fluxes_stage2 (sm_fluid.cpp:255-366): flux * area of
+// a slot from its loads.  The isolation, boundary and bad-state choices are
+// selects among computed states.
+template <int FLUX>
+__device__ __forceinline__ Q corr_eval(const FaceC& F, int nib, const Z& zch, int wc, double ccx, double ccy,
+                                       const double* gx, const double* gy, const Eos& eos, bool& bad) {
+    const int t = nib & 7;
+    const Z& hn = F.hn;
+    const int wnb = F.wnb;
+    const double ncx = F.ncx, ncy = F.ncy, fx = F.fx, fy = F.fy, a = F.a;
+    const double* ngx = F.ngx;
+    const double* ngy = F.ngy;
+    const Frame& fr = F.fr;
     const bool bound = t >= SC_WALL;
     const bool wall = t == SC_WALL;
     const bool robust = wc > 0 || wnb > 0;
@@ -380,11 +439,26 @@ __device__ __forceinline__ Q corr_face(const DevMesh& M, const DevState& half, c
     return Q{f.r * a, f.mx * a, f.my * a, f.E * a};
 }
 
+template <int FLUX>
+__device__ __forceinline__ Q corr_face(const DevMesh& M, const DevState& half, const DevGrad& G,
+                                       const int* __restrict__ wait_in, int c, int s, int nib, const Z& zch, int wc,
+                                       double ccx, double ccy, const double* gx, const double* gy, const Eos& eos,
+                                       bool& bad) {
+    return corr_eval<FLUX>(corr_load(M, half, G, wait_in, c, s, nib), nib, zch, wc, ccx, ccy, gx, gy, eos, bad);
+}
+
 // SmFluid::fluxes_stage2, corrector (sm_fluid.cpp:255-366), plus the next
 // step's CFL candidate.  `wait` is double-buffered: neighbours' isolation is
-// read from the pre-pass array.
-template <int FLUX>
-__global__ void __launch_bounds__(kMaxBlock) __attribute__((amdgpu_waves_per_eu(ZFV_WAVES_CORR)))
+// read from the pre-pass array.  BATCH (the batched layout): the loads of
+// the four main slots are issued together before any face is evaluated, two
+// dependent memory round trips per cell instead of two per face, at twice
+// the registers (fewer resident waves; for meshes too small to fill the
+// GPU); otherwise one face at a time.  Sub-face slots of hanging faces
+// follow the main ones, and the flux terms are summed in slot order either
+// way.
+template <int FLUX, bool BATCH>
+__global__ void __launch_bounds__(kMaxBlock)
+__attribute__((amdgpu_waves_per_eu(BATCH ? ZFV_WAVES_CORRB : ZFV_WAVES_CORR)))
 k_corr(DevMesh M, DevState init, DevState half, DevGrad G, const int* __restrict__ wait_in,
        int* __restrict__ wait_out, DevState next, Eos eos, StepCtl ctl) {
     const Tile tl = tile_of(M, (int)blockDim.x);
@@ -405,14 +479,34 @@ k_corr(DevMesh M, DevState init, DevState half, DevGrad G, const int* __restrict
         }
         double F0 = 0.0, F1 = 0.0, F2 = 0.0, F3 = 0.0;
         bool bad = false;
-        for (int s = 0; s < 8; ++s) {
-            const int nib = (code >> (4 * s)) & 15;
-            if ((nib & 7) == SC_UNDEF) continue;
-            const Q f = corr_face<FLUX>(M, half, G, wait_in, c, s, nib, zch, wc, ccx, ccy, gx, gy, eos, bad);
-            F0 = F0 + f.r;
-            F1 = F1 + f.mx;
-            F2 = F2 + f.my;
-            F3 = F3 + f.E;
+        if (BATCH) {
+            FaceC D[4];
+#pragma unroll
+            for (int s = 0; s < 4; ++s) {
+                const int nib = (code >> (4 * s)) & 15;
+                if ((nib & 7) != SC_UNDEF) D[s] = corr_load(M, half, G, wait_in, c, s, nib);
+            }
+#pragma unroll
+            for (int s = 0; s < 4; ++s) {
+                const int nib = (code >> (4 * s)) & 15;
+                if ((nib & 7) == SC_UNDEF) continue;
+                const Q f = corr_eval<FLUX>(D[s], nib, zch, wc, ccx, ccy, gx, gy, eos, bad);
+                F0 = F0 + f.r;
+                F1 = F1 + f.mx;
+                F2 = F2 + f.my;
+                F3 = F3 + f.E;
+            }
+        }
+        if (!BATCH || (code >> 16) != 0) {
+            for (int s = BATCH ? 4 : 0; s < 8; ++s) {
+                const int nib = (code >> (4 * s)) & 15;
+                if ((nib & 7) == SC_UNDEF) continue;
+                const Q f = corr_face<FLUX>(M, half, G, wait_in, c, s, nib, zch, wc, ccx, ccy, gx, gy, eos, bad);
+                F0 = F0 + f.r;
+                F1 = F1 + f.mx;
+                F2 = F2 + f.my;
+                F3 = F3 + f.E;
+            }
         }
         const Z zc = load_z(init, c);
         Q q = to_q(zc);
@@ -880,14 +974,17 @@ hipError_t launch_flux1(int flux, int layout, int block, const DevMesh& M, const
     return hipGetLastError();
 }
 
-template <bool PRED, bool QUAD>
+template <bool PRED, int LAYOUT>
 static void launch_gp_t(int limiter, dim3 g, dim3 b, hipStream_t s, const DevMesh& M, const DevState& in,
                         const DevGrad& G, const DevState& half, const int* wait, const Eos& eos,
                         const StepCtl& ctl, double* raw) {
-#define ZFV_GP(L)                                                                                          \
-    do {                                                                                                   \
-        if (QUAD) hipLaunchKernelGGL((k_grad_pred_q<L, PRED>), g, b, 0, s, M, in, G, half, wait, eos, ctl, raw); \
-        else hipLaunchKernelGGL((k_grad_pred<L, PRED>), g, b, 0, s, M, in, G, half, wait, eos, ctl, raw);       \
+#define ZFV_GP(L)                                                                                             \
+    do {                                                                                                      \
+        if (LAYOUT == kLayoutQuad)                                                                            \
+            hipLaunchKernelGGL((k_grad_pred_q<L, PRED>), g, b, 0, s, M, in, G, half, wait, eos, ctl, raw);    \
+        else                                                                                                  \
+            hipLaunchKernelGGL((k_grad_pred<L, PRED, LAYOUT == kLayoutBatch>), g, b, 0, s, M, in, G, half, wait, \
+                               eos, ctl, raw);                                                                \
     } while (0)
     switch (limiter) {
         case 0: ZFV_GP(0); break;
@@ -913,12 +1010,16 @@ hipError_t launch_grad_pred(int limiter, bool predictor, int layout, int block, 
     dim3 b(block);
     if (layout == kLayoutQuad) {
         dim3 g(grid_cells(M.n, block / 4));
-        if (predictor) launch_gp_t<true, true>(limiter, g, b, s, M, in, G, half, wait, eos, ctl, nullptr);
-        else launch_gp_t<false, true>(limiter, g, b, s, M, in, G, half, wait, eos, ctl, raw);
-    } else {
+        if (predictor) launch_gp_t<true, kLayoutQuad>(limiter, g, b, s, M, in, G, half, wait, eos, ctl, nullptr);
+        else launch_gp_t<false, kLayoutQuad>(limiter, g, b, s, M, in, G, half, wait, eos, ctl, raw);
+    } else if (layout == kLayoutBatch && predictor) {
         dim3 g(grid_cells(M.n, block));
-        if (predictor) launch_gp_t<true, false>(limiter, g, b, s, M, in, G, half, wait, eos, ctl, nullptr);
-        else launch_gp_t<false, false>(limiter, g, b, s, M, in, G, half, wait, eos, ctl, raw);
+        launch_gp_t<true, kLayoutBatch>(limiter, g, b, s, M, in, G, half, wait, eos, ctl, nullptr);
+    } else {
+        // without the predictor the batched layout has nothing to batch
+        dim3 g(grid_cells(M.n, block));
+        if (predictor) launch_gp_t<true, kLayoutCell>(limiter, g, b, s, M, in, G, half, wait, eos, ctl, nullptr);
+        else launch_gp_t<false, kLayoutCell>(limiter, g, b, s, M, in, G, half, wait, eos, ctl, raw);
     }
     return hipGetLastError();
 }
@@ -927,7 +1028,7 @@ hipError_t launch_corr(int flux, int layout, int block, const DevMesh& M, const 
                        const DevGrad& G, const int* wait_in, int* wait_out, const DevState& next,
                        const Eos& eos, const StepCtl& ctl, hipStream_t s) {
     dim3 b(block);
-#define ZFV_CORR(K, F) hipLaunchKernelGGL(K<F>, g, b, 0, s, M, init, half, G, wait_in, wait_out, next, eos, ctl)
+#define ZFV_CORR(K, ...) hipLaunchKernelGGL((K<__VA_ARGS__>), g, b, 0, s, M, init, half, G, wait_in, wait_out, next, eos, ctl)
     if (layout == kLayoutQuad) {
         dim3 g(grid_cells(M.n, block / 4));
         switch (flux) {
@@ -936,13 +1037,21 @@ hipError_t launch_corr(int flux, int layout, int block, const DevMesh& M, const 
             case 2: ZFV_CORR(k_corr_q, 2); break;
             default: ZFV_CORR(k_corr_q, 3); break;
         }
+    } else if (layout == kLayoutBatch) {
+        dim3 g(grid_cells(M.n, block));
+        switch (flux) {
+            case 0: ZFV_CORR(k_corr, 0, true); break;
+            case 1: ZFV_CORR(k_corr, 1, true); break;
+            case 2: ZFV_CORR(k_corr, 2, true); break;
+            default: ZFV_CORR(k_corr, 3, true); break;
+        }
     } else {
         dim3 g(grid_cells(M.n, block));
         switch (flux) {
-            case 0: ZFV_CORR(k_corr, 0); break;
-            case 1: ZFV_CORR(k_corr, 1); break;
-            case 2: ZFV_CORR(k_corr, 2); break;
-            default: ZFV_CORR(k_corr, 3); break;
+            case 0: ZFV_CORR(k_corr, 0, false); break;
+            case 1: ZFV_CORR(k_corr, 1, false); break;
+            case 2: ZFV_CORR(k_corr, 2, false); break;
+            default: ZFV_CORR(k_corr, 3, false); break;
         }
     }
 #undef ZFV_CORR
