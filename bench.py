@@ -299,6 +299,12 @@ def mm_crp_bytes_per_face(n_mat):
     return 2 * 8 * (7 + 2 * n_mat) + 4 + 4 + 8 + 16 + 16 + 16 + 4 + 8 * (4 + n_mat)
 
 
+# ... per cell of the composite-face marking: own mass fractions, code and
+# four adjacency entries (neighbours' fractions through the cache).
+def mm_crp_mark_bytes_per_cell(n_mat, faces_per_cell=4):
+    return 8 * n_mat + 4 + faces_per_cell * 4
+
+
 # ... per cell of the interface reconstruction: own volume fractions'
 # inputs (rho, beta, rhos), neighbours' through the cache, vertices for
 # hx / hy, and n_mat (nx, ny, p) written.
@@ -333,7 +339,7 @@ def mm_run(args, w, Z):
     mixed = int(np.sum(np.sum(z["beta"] > 0.0, axis=0) > 1))
     n = nx * ny
     alg_of = {"mm_flux1": mm_bytes_per_cell(2) * n, "mm_crp": mm_crp_bytes_per_face(2) * n_crp,
-              "mm_interface": mm_interface_bytes_per_cell(2) * n}
+              "mm_crp_mark": mm_crp_mark_bytes_per_cell(2) * n, "mm_interface": mm_interface_bytes_per_cell(2) * n}
     dom = max(prof, key=lambda k: prof[k][0])
     avg_ms = prof[dom][0] / prof[dom][1]
     alg = alg_of[dom]

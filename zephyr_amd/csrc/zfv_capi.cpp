@@ -1851,6 +1851,9 @@ int zfv_mm_step(zfv_ctx* ctx, int n_steps) {
             ZFV_HIP(timed(ctx, "mm_interface", [&] {
                 return mm_interface(M, in, S.mix, ctx->vx.p, ctx->vy.p, o, o + nm * cap, o + 2 * nm * cap, ctx->stream);
             }));
+            ZFV_HIP(timed(ctx, "mm_crp_mark", [&] {
+                return mm_crp_mark(M, in, S.mix, S.crp_list.p, S.crp_count.p, ctx->stream);
+            }));
             ZFV_HIP(timed(ctx, "mm_crp", [&] {
                 return mm_crp(M, in, S.mix, c, o, o + nm * cap, S.crp_list.p, S.crp_count.p, S.cf.p, ctx->stream);
             }));

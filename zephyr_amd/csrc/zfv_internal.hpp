@@ -192,9 +192,12 @@ hipError_t mm_prepare(const DevMesh& M, const MmLayer& L, const mm::Mix& mx, uns
 // or null without composite Riemann problems.
 hipError_t mm_flux1(const DevMesh& M, const MmLayer& in, const MmLayer& out, const mm::Mix& mx,
                     const StepCtl& ctl, const double* CF, hipStream_t s);
-// list: 8 * stride ints of work space, count: one counter.
+// The faces that take composite Riemann fluxes: list (8 * stride ints of
+// work space) and their number (one counter), then the fluxes into CF.
+hipError_t mm_crp_mark(const DevMesh& M, const MmLayer& in, const mm::Mix& mx, int* list, unsigned int* count,
+                       hipStream_t s);
 hipError_t mm_crp(const DevMesh& M, const MmLayer& in, const mm::Mix& mx, const StepCtl& ctl, const double* nx,
-                  const double* ny, int* list, unsigned int* count, double* CF, hipStream_t s);
+                  const double* ny, const int* list, const unsigned int* count, double* CF, hipStream_t s);
 hipError_t mm_interface(const DevMesh& M, const MmLayer& L, const mm::Mix& mx, const double* vx, const double* vy,
                         double* nx, double* ny, double* p, hipStream_t s);
 

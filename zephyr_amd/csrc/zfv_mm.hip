@@ -146,23 +146,30 @@ __global__ void __launch_bounds__(kBlock) k_mm_crp_mark(DevMesh M, MmLayer in, i
     }
 }
 
-// The composite Riemann flux of each listed face, one face per thread
-// (grid-stride): the face-frame states, the master material
+// The composite Riemann flux of each listed face, one face per thread: the
+// face-frame states, the master material
 // (master_component, mm_fluid.cpp:201-226), the PLIC face fraction of the
 // upwind cell's interface (alpha_sigma, :241-254, with the normals of
 // interface_recovery) and calc_crp_flux (:259-337), rotated to the global
 // frame, into CF[slot][k][stride].  These faces are few and need many more
 // registers than the plain HLLC faces of k_mm_flux1, hence the compaction.
 template <int NM>
-__global__ void __launch_bounds__(kBlock) k_mm_crp(DevMesh M, MmLayer in, Mix mx, StepCtl ctl,
-                                                   const double* __restrict__ inx, const double* __restrict__ iny,
-                                                   const int* __restrict__ list, const unsigned int* count,
-                                                   double* __restrict__ CF) {
+__global__ void __launch_bounds__(64) k_mm_crp(DevMesh M, MmLayer in, Mix mx, StepCtl ctl,
+                                            const double* __restrict__ inx, const double* __restrict__ iny,
+                                            const int* __restrict__ list, const unsigned int* count,
+                                            double* __restrict__ CF) {
     constexpr int K = 4 + NM;
     const int st = M.stride;
     const int n_faces = (int)*count;
+    // L active lanes per one-wavefront workgroup: the faces' branchy paths
+    // (closures, wave configurations) diverge, and a wavefront executes the
+    // union of its lanes' paths, so few faces are spread over many
+    // wavefronts; a long list fills whole wavefronts.
+    int L = 4;
+    while (L < 64 && (long long)L * gridDim.x < n_faces) L <<= 1;
+    const int lane = (int)threadIdx.x;
     bool bad = false;
-    for (int e = (int)(blockIdx.x * blockDim.x + threadIdx.x); e < n_faces; e += (int)(gridDim.x * blockDim.x)) {
+    for (int e = (int)blockIdx.x * L + lane; lane < L && e < n_faces; e += (int)gridDim.x * L) {
         const int so = list[e];
         const int s = so / st, c = so - s * st;
         const double dt = step_dt(ctl, 0);
@@ -384,13 +391,22 @@ hipError_t mm_flux1(const DevMesh& M, const MmLayer& in, const MmLayer& out, con
     return hipGetLastError();
 }
 
-hipError_t mm_crp(const DevMesh& M, const MmLayer& in, const Mix& mx, const StepCtl& ctl, const double* nx,
-                  const double* ny, int* list, unsigned int* count, double* CF, hipStream_t s) {
-    hipError_t e = hipMemsetAsync(count, 0, sizeof(unsigned int), s);
-    if (e != hipSuccess) return e;
+__global__ void __launch_bounds__(64) k_zero_u32(unsigned int* p) {
+    if (threadIdx.x == 0) *p = 0u;
+}
+
+hipError_t mm_crp_mark(const DevMesh& M, const MmLayer& in, const Mix& mx, int* list, unsigned int* count,
+                       hipStream_t s) {
+    // the counter is cleared by a kernel, not a memset node, so the step can be captured
+    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, s, count);
     ZFV_MM_DISPATCH(mx.n, k_mm_crp_mark, mm_grid(M.n), dim3(kBlock), 0, s, M, in, list, count);
-    // one face per thread over the list; 1024 workgroups cover any list length
-    ZFV_MM_DISPATCH(mx.n, k_mm_crp, dim3(1024), dim3(kBlock), 0, s, M, in, mx, ctl, nx, ny, list, count, CF);
+    return hipGetLastError();
+}
+
+hipError_t mm_crp(const DevMesh& M, const MmLayer& in, const Mix& mx, const StepCtl& ctl, const double* nx,
+                  const double* ny, const int* list, const unsigned int* count, double* CF, hipStream_t s) {
+    // 2048 one-wavefront workgroups, grid-stride over any list length
+    ZFV_MM_DISPATCH(mx.n, k_mm_crp, dim3(2048), dim3(64), 0, s, M, in, mx, ctl, nx, ny, list, count, CF);
     return hipGetLastError();
 }
 
