@@ -243,6 +243,24 @@ int zfv_mm_crp_faces(zfv_ctx* ctx, int64_t* n);
  * per material the PLIC normal (nx, ny) and section position p,
  * [n_mat][n] each; zero for pure cells and absent materials. */
 int zfv_mm_interface(zfv_ctx* ctx, double* nx, double* ny, double* p);
+/* MmFluid::set_limiter (mm_fluid.cpp:45-47): limiter number as
+ * zfv_limiter_from_name gives it (default MC). */
+int zfv_mm_set_limiter(zfv_ctx* ctx, int limiter);
+/* MmFluid::set_flags (mm_fluid.cpp:836-885): compute_grad (zero gradients
+ * in and next to mixed cells with composite Riemann problems on, else the
+ * limited LSM gradient; kept for zfv_mm_refine), then per cell +1 for a
+ * mixed cell or a pure cell next to another material, else -1, into the
+ * mesh's refinement flags (zfv_flags_download reads them). */
+int zfv_mm_set_flags(zfv_ctx* ctx);
+/* EuMesh::refine (eu_mesh.cpp:290-359) with MmFluid::distributor
+ * (mm_fluid.cpp:765-830): balance_flags, then amr::apply with the
+ * multi-material split (from the gradients of zfv_mm_set_flags, which must
+ * have run on the current mesh) and the conservative merge.  The
+ * single-material state, if any, does not follow the mesh. */
+int zfv_mm_refine(zfv_ctx* ctx, int max_level);
+/* The gradients of the last zfv_mm_set_flags: rows rho, u, v, e,
+ * beta[n_mat], [4 + n_mat][n] each for x and y. */
+int zfv_mm_grad(zfv_ctx* ctx, double* gx, double* gy);
 
 /* ---- launch configuration (no reference counterpart) --------------------- */
 /* Step kernels of a stage (0 = first-order update, 1 = gradient + predictor,

@@ -51,9 +51,20 @@ inline void line_geom(const Vec3& v0, const Vec3& v1, const Vec3& view,
     ny = n.y;
 }
 
+// The solver's cell data through an adaptation: the Distributor's split and
+// merge callbacks (mesh/euler/distributor.h:13-39) plus the storage moves
+// and resizes the reference applies to every storable (amr_cells.cpp:428-468).
+struct AmrData {
+    virtual ~AmrData() = default;
+    virtual void resize(int n) = 0;
+    virtual void split(const orc_mesh& m, int ip, const int* children) = 0;
+    virtual void merge(const orc_mesh& m, const int* children, int ip) = 0;
+    virtual void move(int from, int to) = 0;
+};
+
 struct AmrWork {
     orc_mesh& m;
-    orc_smf* S;
+    AmrData* D;
     int n;   // cells before adaptation
 };
 
@@ -61,7 +72,7 @@ inline Vec3 vert(const orc_mesh& m, int c, int k) {
     return {m.vx[9 * (size_t)c + k], m.vy[9 * (size_t)c + k], 0.0};
 }
 
-void resize_cells(orc_mesh& m, orc_smf* S, int n) {
+void resize_cells(orc_mesh& m, AmrData* D, int n) {
     m.cx.resize(n); m.cy.resize(n); m.vol.resize(n);
     m.level.resize(n, 0); m.flag.resize(n, 0); m.b_idx.resize(n, 0); m.z_idx.resize(n, 0);
     m.next.resize(n, 0); m.idx.resize(n, -1);
@@ -71,12 +82,7 @@ void resize_cells(orc_mesh& m, orc_smf* S, int n) {
     m.fnx.resize(ns, 0.0); m.fny.resize(ns, 0.0); m.fcx.resize(ns, 0.0); m.fcy.resize(ns, 0.0);
     m.area.resize(ns, 0.0);
     m.n = n;
-    if (S) {
-        PState zz; std::memset(&zz, 0, sizeof(zz));
-        S->init.resize(n, zz); S->half.resize(n, zz); S->next.resize(n, zz);
-        S->d_dx.resize(n, zz); S->d_dy.resize(n, zz); S->d_dz.resize(n, zz);
-        S->wait.resize(n, 0); S->wait_next.resize(n, 0);
-    }
+    if (D) D->resize(n);
 }
 
 inline void set_slot_undefined(orc_mesh& m, size_t s) {   // AmrFaces::set_undefined
@@ -293,6 +299,25 @@ void merge_data(orc_smf* S, const int* children, int ip) {
     S->wait[ip] = 0;
 }
 
+// SmFluid's data rows through an adaptation ("const" or "slope" split).
+struct SmfData : AmrData {
+    orc_smf* S;
+    bool slope;
+    SmfData(orc_smf* s, bool sl) : S(s), slope(sl) {}
+    void resize(int n) override {
+        PState zz; std::memset(&zz, 0, sizeof(zz));
+        S->init.resize(n, zz); S->half.resize(n, zz); S->next.resize(n, zz);
+        S->d_dx.resize(n, zz); S->d_dy.resize(n, zz); S->d_dz.resize(n, zz);
+        S->wait.resize(n, 0); S->wait_next.resize(n, 0);
+    }
+    void split(const orc_mesh&, int ip, const int* ch) override { split_data(S, ip, ch, slope); }
+    void merge(const orc_mesh&, const int* ch, int ip) override { merge_data(S, ch, ip); }
+    void move(int from, int to) override {
+        S->init[to] = S->init[from]; S->wait[to] = S->wait[from];
+        S->d_dx[to] = S->d_dx[from]; S->d_dy[to] = S->d_dy[from]; S->d_dz[to] = S->d_dz[from];
+    }
+};
+
 // ---- apply ------------------------------------------------------------------
 // retain_cell (retain.h:17-216).
 void retain_cell(orc_mesh& m, int ic) {
@@ -336,7 +361,7 @@ void retain_cell(orc_mesh& m, int ic) {
 }
 
 // refine_cell / make_children / link_siblings (refine.h:36-444).
-void refine_cell(AmrWork& W, int ip, bool slope) {
+void refine_cell(AmrWork& W, int ip) {
     orc_mesh& m = W.m;
     const int mc = m.next[ip];
     Vec3 V[9];
@@ -408,9 +433,9 @@ void refine_cell(AmrWork& W, int ip, bool slope) {
             }
         }
     }
-    if (W.S) {
+    if (W.D) {
         int ch[4] = {mc, mc + 1, mc + 2, mc + 3};
-        split_data(W.S, ip, ch, slope);
+        W.D->split(m, ip, ch);
     }
     m.idx[ip] = -1;
 }
@@ -456,12 +481,12 @@ void coarse_cell(AmrWork& W, int ich) {
             else m.adj[sf] = m.next[m.next[jc]];
         }
     }
-    if (W.S) merge_data(W.S, ch, ip);
+    if (W.D) W.D->merge(m, ch, ip);
     m.idx[ich] = -1;
 }
 
 // AmrCells::move_item (amr_cells.cpp:428-468) plus the data rows.
-void move_cell(orc_mesh& m, orc_smf* S, int from, int to) {
+void move_cell(orc_mesh& m, AmrData* D, int from, int to) {
     m.next[to] = to; m.idx[to] = to;
     m.flag[to] = m.flag[from]; m.level[to] = m.level[from];
     m.b_idx[to] = m.b_idx[from]; m.z_idx[to] = m.z_idx[from];
@@ -476,14 +501,11 @@ void move_cell(orc_mesh& m, orc_smf* S, int from, int to) {
         m.vy[9 * (size_t)to + k] = m.vy[9 * (size_t)from + k];
     }
     m.idx[from] = -1;
-    if (S) {
-        S->init[to] = S->init[from]; S->wait[to] = S->wait[from];
-        S->d_dx[to] = S->d_dx[from]; S->d_dy[to] = S->d_dy[from]; S->d_dz[to] = S->d_dz[from];
-    }
+    if (D) D->move(from, to);
 }
 
 // apply_impl<2> (apply.h:43-94).  Returns false when nothing changes.
-bool apply_flags(orc_mesh& m, orc_smf* S, bool slope) {
+bool apply_flags(orc_mesh& m, AmrData* D) {
     const int n = m.n;
     int n_coarse = 0, n_retain = 0, n_refine = 0;
     for (int ic = 0; ic < n; ++ic) {
@@ -512,7 +534,7 @@ bool apply_flags(orc_mesh& m, orc_smf* S, bool slope) {
             --jc;
         }
     }
-    resize_cells(m, S, n_large);
+    resize_cells(m, D, n_large);
     // setup_positions (setup_positions.h:24-64)
     int cc = n, rc = n + n_parents;
     for (int ic = 0; ic < n; ++ic) {
@@ -529,15 +551,15 @@ bool apply_flags(orc_mesh& m, orc_smf* S, bool slope) {
     for (int ic = n; ic < n_large; ++ic) m.next[ic] = ic;
     for (size_t i = 0; i < actual.size(); ++i) m.next[actual[i]] = undef[i];
     // setup_geometry (setup_geometry.h:19-41)
-    AmrWork W{m, S, n};
+    AmrWork W{m, D, n};
     for (int ic = 0; ic < n; ++ic) {
         if (m.flag[ic] == 0) retain_cell(m, ic);
-        else if (m.flag[ic] > 0) refine_cell(W, ic, slope);
+        else if (m.flag[ic] > 0) refine_cell(W, ic);
         else coarse_cell(W, ic);
     }
     // move_elements (swap_lists.h:129-138)
-    for (size_t i = 0; i < actual.size(); ++i) move_cell(m, S, actual[i], undef[i]);
-    resize_cells(m, S, n_short);
+    for (size_t i = 0; i < actual.size(); ++i) move_cell(m, D, actual[i], undef[i]);
+    resize_cells(m, D, n_short);
     return true;
 }
 
@@ -590,7 +612,9 @@ void orc_mesh_balance_flags(orc_mesh* m, int max_level) { balance_flags(*m, max_
 // distributor (slope = 1: "slope", 0: "const").  S may be NULL (mesh only).
 int orc_mesh_refine(orc_mesh* m, orc_smf* S, int max_level, int slope) {
     balance_flags(*m, max_level);
-    return apply_flags(*m, S, slope != 0) ? 1 : 0;
+    if (!S) return apply_flags(*m, nullptr) ? 1 : 0;
+    SmfData D(S, slope != 0);
+    return apply_flags(*m, &D) ? 1 : 0;
 }
 
 void orc_mesh_get_next(const orc_mesh* m, int* next) { std::memcpy(next, m->next.data(), sizeof(int) * m->n); }

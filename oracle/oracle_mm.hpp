@@ -836,16 +836,219 @@ inline MArr calc_crp_flux(const MState& zL, const MState& zR, double hL, double 
 using namespace orcmm;
 
 // =============================================================== MmFluid
+// Gradient of the PState components the split distributor reads: rho,
+// v.x, v.y, e, beta[i] (the reference differentiates all of them component
+// by component, gradient.h:226-282, 412-474, so a subset is the same).
+constexpr int kMG = 4 + kMaxMat;
+struct MG { double a[kMG] = {0, 0, 0, 0, 0, 0, 0, 0}; };
+
 struct orc_mm {
     orc_mesh* m;
     Mixture mix;
     int crp = 1;   // CrpMode::PLIC, the MmFluid default (mm_fluid.cpp:18); 0 = CrpMode::NONE
+    int limiter = ORC_LIM_MC;   // m_limiter = "MC" (mm_fluid.cpp:20)
     double cfl, dt, max_dt;
     std::vector<MState> init, next;   // MmFluid::Parts init / next (mm_fluid.h:53-67)
+    std::vector<MG> d_dx, d_dy, d_dz; // Parts d_dx / d_dy / d_dz (components of MG)
     std::vector<double> nrm_x, nrm_y, pos;   // interface normals / positions [c][mat]
 };
 
 namespace {
+
+inline double mg_comp(const MState& z, int k) {
+    switch (k) {
+        case 0: return z.rho;
+        case 1: return z.v.x;
+        case 2: return z.v.y;
+        case 3: return z.e;
+        default: return z.beta.a[k - 4];
+    }
+}
+
+// MmFluid::compute_grad (mm_fluid.cpp:439-466): with a CrpMode other than
+// NONE the gradient is zero in a mixed cell and in a cell with a mixed
+// face neighbour (a boundary face's neighbour is the cell itself); else
+// gradient::LSM (gradient.h:226-282) and gradient::limiting (:412-474) with
+// the mm boundary_value (mm_fluid.cpp:98-108), as grad_cell does for SmFluid.
+void mm_grad_cell(const orc_mm& S, int c, MG& gx, MG& gy, MG& gz) {
+    const orc_mesh& m = *S.m;
+    const MState zc = S.init[c];
+    gx = MG{};
+    gy = MG{};
+    gz = MG{};
+    if (S.crp != 0) {
+        bool zero = zc.beta.index() < 0;
+        for (int s8 = 0; s8 < 8 && !zero; ++s8) {
+            const size_t f = 8 * (size_t)c + s8;
+            const int b = m.bnd[f];
+            if (b == ORC_UNDEFINED || slot_is_boundary(b)) continue;
+            if (S.init[m.adj[f]].beta.index() < 0) zero = true;
+        }
+        if (zero) return;   // PState::Zero()
+    }
+    const double w0 = 1.0;
+    const Vec3 cell_c{m.cx[c], m.cy[c], 0.0};
+    double A[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+    double Fx[kMG] = {0}, Fy[kMG] = {0}, Fz[kMG] = {0};
+    struct FaceGeo { Vec3 dr, drn; double w; MState zn; };
+    FaceGeo fg[8];
+    int nf = 0;
+    for (int s8 = 0; s8 < 8; ++s8) {
+        const size_t f = 8 * (size_t)c + s8;
+        const int b = m.bnd[f];
+        if (b == ORC_UNDEFINED) continue;
+        const Vec3 normal{m.fnx[f], m.fny[f], 0.0};
+        const MState zn = !slot_is_boundary(b) ? S.init[m.adj[f]] : m_boundary_value(zc, normal, b);
+        const Vec3 nc = (b == ORC_INNER) ? Vec3{m.cx[m.adj[f]], m.cy[m.adj[f]], 0.0} : symm_point(m, (int)f, cell_c);
+        const Vec3 dr = nc - cell_c;
+        const Vec3 drn = scale(dot(dr, normal), normal);
+        const double w = w0 * m.area[f] / std::pow(norm(drn), 3);
+        const double wd[3] = {w * drn.x, w * drn.y, w * drn.z};
+        const double drv[3] = {dr.x, dr.y, dr.z};
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j) A[i][j] = A[i][j] + wd[i] * drv[j];
+        for (int k = 0; k < kMG; ++k) {
+            const double dF = mg_comp(zn, k) - mg_comp(zc, k);
+            Fx[k] = Fx[k] + (w * dF) * drn.x;
+            Fy[k] = Fy[k] + (w * dF) * drn.y;
+            Fz[k] = Fz[k] + (w * dF) * drn.z;
+        }
+        fg[nf++] = {dr, drn, w, zn};
+    }
+    if (std::abs(A[2][2]) < 1e-8) {
+        A[2][2] = 1.0;
+        if (std::abs(A[1][1]) < 1e-8) A[1][1] = 1.0;
+    }
+    auto cof = [&](int i, int j) {
+        const int i1 = (i + 1) % 3, i2 = (i + 2) % 3, j1 = (j + 1) % 3, j2 = (j + 2) % 3;
+        return A[i1][j1] * A[i2][j2] - A[i1][j2] * A[i2][j1];
+    };
+    const double c00 = cof(0, 0), c10 = cof(1, 0), c20 = cof(2, 0);
+    const double det = (c00 * A[0][0] + c10 * A[1][0]) + c20 * A[2][0];
+    const double invdet = 1.0 / det;
+    double B[3][3];
+    B[0][0] = c00 * invdet; B[0][1] = c10 * invdet; B[0][2] = c20 * invdet;
+    B[1][0] = cof(0, 1) * invdet; B[1][1] = cof(1, 1) * invdet; B[1][2] = cof(2, 1) * invdet;
+    B[2][0] = cof(0, 2) * invdet; B[2][1] = cof(1, 2) * invdet; B[2][2] = cof(2, 2) * invdet;
+    MG ux, uy, uz;
+    for (int k = 0; k < kMG; ++k) {
+        ux.a[k] = (B[0][0] * Fx[k] + B[0][1] * Fy[k]) + B[0][2] * Fz[k];
+        uy.a[k] = (B[1][0] * Fx[k] + B[1][1] * Fy[k]) + B[1][2] * Fz[k];
+        uz.a[k] = (B[2][0] * Fx[k] + B[2][1] * Fy[k]) + B[2][2] * Fz[k];
+    }
+    double Lx[kMG] = {0}, Ly[kMG] = {0}, Lz[kMG] = {0};
+    for (int q = 0; q < nf; ++q) {
+        const FaceGeo& g = fg[q];
+        for (int k = 0; k < kMG; ++k) {
+            const double dF_p = mg_comp(g.zn, k) - mg_comp(zc, k);
+            const double du_n = (ux.a[k] * g.dr.x + uy.a[k] * g.dr.y) + uz.a[k] * g.dr.z;
+            const double dF_m = 2.0 * du_n - dF_p;
+            const double dF_lim = lim_binary(S.limiter, dF_m, dF_p) * dF_p;
+            Lx[k] = Lx[k] + (g.w * dF_lim) * g.drn.x;
+            Ly[k] = Ly[k] + (g.w * dF_lim) * g.drn.y;
+            Lz[k] = Lz[k] + (g.w * dF_lim) * g.drn.z;
+        }
+    }
+    for (int k = 0; k < kMG; ++k) {
+        gx.a[k] = (B[0][0] * Lx[k] + B[0][1] * Ly[k]) + B[0][2] * Lz[k];
+        gy.a[k] = (B[1][0] * Lx[k] + B[1][1] * Ly[k]) + B[1][2] * Lz[k];
+        gz.a[k] = (B[2][0] * Lx[k] + B[2][1] * Ly[k]) + B[2][2] * Lz[k];
+    }
+}
+
+// PState::is_bad, mmf (models.cpp:393-403); velocity z is zero.
+inline bool m_is_bad(const MState& z) {
+    bool empty = true;   // Fractions::empty (fractions.cpp:91-99)
+    for (int i = 0; i < kMaxMat; ++i)
+        if (z.beta.a[i] > std::numeric_limits<double>::epsilon() / 4) empty = false;
+    auto bad = [](double x) { return std::isinf(x) || std::isnan(x); };
+    return bad(z.rho) || bad(z.v.x) || bad(z.v.y) || bad(z.v.z) || bad(z.P) || bad(z.e) || bad(z.T) || empty ||
+           z.rho < 0;
+}
+
+// MmFluid::distributor (mm_fluid.cpp:765-830) with the storage moves of
+// an adaptation.  `bad`: a child or parent state the reference rejects (it
+// throws "bad cell").
+struct MmData : AmrData {
+    orc_mm* S;
+    bool bad = false;
+    explicit MmData(orc_mm* s) : S(s) {}
+    void resize(int n) override {
+        S->init.resize(n);
+        S->next.resize(n);
+        S->d_dx.resize(n);
+        S->d_dy.resize(n);
+        S->d_dz.resize(n);
+    }
+    // split (:768-800): child = parent + grad . dr with momentum- and
+    // energy-consistent corrections, beta normalised, P from
+    // pressure_re(rho, e, beta, {P0, T0} of the parent), T from
+    // temperature_rP(rho, P, beta, {T0}); the densities stay the
+    // default-constructed ScalarSet (zeros).
+    void split(const orc_mesh& m, int ip, const int* ch) override {
+        const MState zp = S->init[ip];
+        const MG gx = S->d_dx[ip], gy = S->d_dy[ip], gz = S->d_dz[ip];
+        for (int k = 0; k < 4; ++k) {
+            const int c = ch[k];
+            const Vec3 dr = Vec3{m.cx[c], m.cy[c], 0.0} - Vec3{m.cx[ip], m.cy[ip], 0.0};
+            double delta[kMG];
+            for (int j = 0; j < kMG; ++j) delta[j] = (gx.a[j] * dr.x + gy.a[j] * dr.y) + gz.a[j] * dr.z;
+            MState zc;
+            zc.rho = zp.rho + delta[0];
+            const double ratio = zp.rho / zc.rho;
+            zc.v = {zp.v.x + ratio * delta[1], zp.v.y + ratio * delta[2], zp.v.z + ratio * 0.0};
+            for (int i = 0; i < S->mix.n; ++i) zc.beta.a[i] = zp.beta.a[i] + (zp.rho / zc.rho) * delta[4 + i];
+            zc.beta.normalize();
+            const Vec3 dv{zp.v.x - zc.v.x, zp.v.y - zc.v.y, zp.v.z - zc.v.z};
+            zc.e = (zp.e + 0.5 * sqnorm(dv)) + (zp.rho / zc.rho) * delta[3];
+            double rh[kMaxMat];
+            MixOpt op;
+            op.P0 = zp.P;
+            op.T0 = zp.T;
+            const int idx = zc.beta.index();
+            double Tt;
+            if (idx >= 0) zc.P = sg_pressure_re(S->mix.m[idx], zc.rho, zc.e);   // MixturePT::pressure_re (:26-36)
+            else find_rPT(S->mix, zc.rho, zc.e, zc.beta, op, rh, zc.P, Tt);
+            MixOpt ot;
+            ot.T0 = zp.T;
+            if (idx >= 0) zc.T = sg_temperature_rP(S->mix.m[idx], zc.rho, zc.P);   // temperature_rP (:74-84)
+            else {
+                double TdR, TdP;
+                find_rT_rP(S->mix, zc.rho, zc.P, zc.beta, ot, rh, zc.T, TdR, TdP);
+            }
+            for (int i = 0; i < kMaxMat; ++i) zc.rhos[i] = 0.0;
+            if (m_is_bad(zc)) bad = true;
+            S->init[c] = zc;
+        }
+    }
+    // merge (:802-830): volume-weighted conserved sum, mean P and T as the
+    // closure's starting point, negative partial densities clipped.
+    void merge(const orc_mesh& m, const int* ch, int ip) override {
+        MArr sum;
+        double mean_p = 0.0, mean_t = 0.0;
+        for (int k = 0; k < 4; ++k) {
+            const MState& zc = S->init[ch[k]];
+            const MArr qc = m_qstate(zc);
+            const double V = m.vol[ch[k]];
+            for (int j = 0; j < 9; ++j) sum.a[j] = sum.a[j] + qc.a[j] * V;
+            mean_p += zc.P * V;
+            mean_t += zc.T * V;
+        }
+        for (int j = 0; j < 9; ++j) sum.a[j] = sum.a[j] / m.vol[ip];
+        mean_p /= m.vol[ip];
+        mean_t /= m.vol[ip];
+        for (int i = 0; i < kMaxMat; ++i) sum.a[5 + i] = std::max(0.0, sum.a[5 + i]);
+        const MState z = m_from_q(sum, S->mix, mean_p, mean_t, nullptr);
+        if (m_is_bad(z)) bad = true;
+        S->init[ip] = z;
+    }
+    void move(int from, int to) override {
+        S->init[to] = S->init[from];
+        S->d_dx[to] = S->d_dx[from];
+        S->d_dy[to] = S->d_dy[from];
+        S->d_dz[to] = S->d_dz[from];
+    }
+};
 
 // AmrCells::hx / hy (mesh/euler/amr_cells.cpp:91-99): |vs<+1,0> - vs<-1,0>|, |vs<0,+1> - vs<0,-1>|.
 inline double cell_hx(const orc_mesh& m, int c) {
@@ -877,7 +1080,63 @@ orc_mm* orc_mm_create(orc_mesh* m, int n_mat, const double* mats, double cfl) {
     s->max_dt = 1.0e+300;                          // mm_fluid.cpp:16-26
     s->init.resize(m->n);
     s->next.resize(m->n);
+    s->d_dx.resize(m->n);
+    s->d_dy.resize(m->n);
+    s->d_dz.resize(m->n);
     return s;
+}
+
+void orc_mm_set_limiter(orc_mm* s, int limiter) { s->limiter = limiter; }   // set_limiter (mm_fluid.cpp:45-47)
+
+// MmFluid::compute_grad (mm_fluid.cpp:439-466) of the init layer.
+void orc_mm_compute_grad(orc_mm* s) {
+    for (int c = 0; c < s->m->n; ++c) mm_grad_cell(*s, c, s->d_dx[c], s->d_dy[c], s->d_dz[c]);
+}
+
+// The stored gradients: rho, u, v, e, beta[n_mat] rows of n values each,
+// x then y.
+void orc_mm_grad(const orc_mm* s, double* gx, double* gy) {
+    const int n = s->m->n, K = 4 + s->mix.n;
+    for (int c = 0; c < n; ++c)
+        for (int k = 0; k < K; ++k) {
+            gx[(size_t)k * n + c] = s->d_dx[c].a[k];
+            gy[(size_t)k * n + c] = s->d_dy[c].a[k];
+        }
+}
+
+// MmFluid::set_flags (mm_fluid.cpp:836-885): compute_grad, then +1 for a
+// mixed cell or a pure cell with a face neighbour of another material (or
+// mixed; boundary faces skipped), else -1.
+void orc_mm_set_flags(orc_mm* s) {
+    orc_mm_compute_grad(s);
+    orc_mesh& m = *s->m;
+    for (int c = 0; c < m.n; ++c) {
+        const MState& z = s->init[c];
+        if (!z.beta.is_pure()) {
+            m.flag[c] = 1;
+            continue;
+        }
+        const int idx = z.beta.index();
+        bool need_split = false;
+        for (int s8 = 0; s8 < 8 && !need_split; ++s8) {
+            const size_t f = 8 * (size_t)c + s8;
+            const int b = m.bnd[f];
+            if (b == ORC_UNDEFINED || slot_is_boundary(b)) continue;
+            if (s->init[m.adj[f]].beta.index() != idx) need_split = true;
+        }
+        m.flag[c] = need_split ? 1 : -1;
+    }
+}
+
+// EuMesh::refine (eu_mesh.cpp:290-359) with MmFluid::distributor: 1 when
+// the mesh changed, 0 when not, -1 when a split or merge gave a state the
+// reference rejects (it throws).
+int orc_mm_refine(orc_mm* s, int max_level) {
+    balance_flags(*s->m, max_level);
+    MmData D(s);
+    const bool changed = apply_flags(*s->m, &D);
+    if (D.bad) return -1;
+    return changed ? 1 : 0;
 }
 
 void orc_mm_free(orc_mm* s) { delete s; }

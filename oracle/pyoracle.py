@@ -85,6 +85,11 @@ def lib():
             "orc_mm_closure": (None, [_I, _dp, _D, _D, _dp, _D, _D, _dp]),
             "orc_mm_closure_rP": (None, [_I, _dp, _D, _D, _dp, _dp]),
             "orc_mm_face_flux": (None, [_P, _I, _I, _D, _D, _dp]),
+            "orc_mm_set_limiter": (None, [_P, _I]),
+            "orc_mm_compute_grad": (None, [_P]),
+            "orc_mm_grad": (None, [_P, _dp, _dp]),
+            "orc_mm_set_flags": (None, [_P]),
+            "orc_mm_refine": (_I, [_P, _I]),
             "orc_quad_volume_fraction": (_D, [_D] * 5),
             "orc_quad_find_section": (_D, [_D] * 5),
             "orc_face_fraction": (_D, [_I, _D, _D]),
@@ -343,6 +348,28 @@ class MmFluid:
         F = np.zeros(9)
         lib().orc_mm_face_flux(self.h, cL, cR, nx, ny, _d(F))
         return F
+
+    def set_limiter(self, limiter):
+        lib().orc_mm_set_limiter(self.h, LIMITERS.index(limiter) if isinstance(limiter, str) else int(limiter))
+
+    def compute_grad(self):
+        lib().orc_mm_compute_grad(self.h)
+
+    def grad(self):
+        """Stored gradients (gx, gy), rows rho, u, v, e, beta[n_mat]."""
+        K, n = 4 + self.nm, self.mesh.n
+        gx, gy = np.zeros((K, n)), np.zeros((K, n))
+        lib().orc_mm_grad(self.h, _d(gx), _d(gy))
+        return gx, gy
+
+    def set_flags(self):
+        """MmFluid::set_flags into the mesh's flags (computes the gradients)."""
+        lib().orc_mm_set_flags(self.h)
+
+    def refine(self, max_level):
+        """EuMesh::refine with MmFluid's distributor: 1 changed, 0 not, -1 a
+        state the reference rejects."""
+        return lib().orc_mm_refine(self.h, max_level)
 
 
 def mm_closure(materials, rho, e, beta, P0=float("nan"), T0=float("nan")):

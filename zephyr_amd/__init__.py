@@ -32,6 +32,7 @@ EXPORTS = [
     "zfv_mesh_n_faces", "zfv_set_launch", "zfv_get_launch", "zfv_autotune",
     "zfv_batch_rectangles", "zfv_batch_layout", "zfv_batch_dt", "zfv_graph_enable",
     "zfv_set_cell_order", "zfv_get_cell_order",
+    "zfv_mm_set_limiter", "zfv_mm_set_flags", "zfv_mm_refine", "zfv_mm_grad",
     "zfv_adjoint_forward", "zfv_adjoint_backward",
     "zfv_mm_configure", "zfv_mm_set_crp", "zfv_mm_set_max_dt", "zfv_mm_set_state", "zfv_mm_state", "zfv_mm_step", "zfv_mm_dt", "zfv_mm_crp_faces",
     "zfv_mm_interface",
@@ -106,6 +107,10 @@ def lib():
         L.zfv_mm_step.argtypes = [vp, C.c_int]
         L.zfv_mm_dt.argtypes = [vp, vp]
         L.zfv_mm_interface.argtypes = [vp] * 4
+        L.zfv_mm_set_limiter.argtypes = [vp, C.c_int]
+        L.zfv_mm_set_flags.argtypes = [vp]
+        L.zfv_mm_refine.argtypes = [vp, C.c_int]
+        L.zfv_mm_grad.argtypes = [vp, vp, vp]
         L.zfv_adjoint_forward.argtypes = [vp, C.c_int, C.c_int, vp, vp]
         L.zfv_adjoint_backward.argtypes = [vp, vp, vp, vp, vp, vp]
         _lib = L
@@ -365,7 +370,30 @@ class Context:
         _check(lib().zfv_mm_interface(self.h, _ptr(nx), _ptr(ny), _ptr(p)))
         return nx, ny, p
 
+    def mm_set_limiter(self, limiter):
+        lim = limiter if isinstance(limiter, int) else lib().zfv_limiter_from_name(limiter.encode())
+        _check(lib().zfv_mm_set_limiter(self.h, lim))
+
+    def mm_set_flags(self):
+        """MmFluid::set_flags into the mesh's flags (and the gradients kept for mm_refine)."""
+        _check(lib().zfv_mm_set_flags(self.h))
+
+    def mm_refine(self, max_level):
+        """EuMesh::refine with MmFluid's distributor (after mm_set_flags)."""
+        _check(lib().zfv_mm_refine(self.h, max_level))
+        n = C.c_int32()
+        _check(lib().zfv_mesh_n_cells(self.h, C.byref(n)))
+        self.n = n.value
+
+    def mm_grad(self):
+        """Gradients of the last mm_set_flags: (gx, gy), rows rho, u, v, e, beta[n_mat]."""
+        K, n = 4 + self._mm_nmat, self.n
+        gx, gy = np.zeros((K, n)), np.zeros((K, n))
+        _check(lib().zfv_mm_grad(self.h, _ptr(gx), _ptr(gy)))
+        return gx, gy
+
     # ---- discrete adjoint
+
     def adjoint_forward(self, n_steps, every, window):
         """n_steps steps with a checkpoint every `every`; returns J = int_window p^2 dA."""
         win = np.ascontiguousarray(window, np.float64)

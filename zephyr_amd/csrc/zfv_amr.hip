@@ -474,12 +474,14 @@ __device__ bool split_slope(const AmrMesh& A, const DevState& z, const DevGrad& 
 // coarse_cell / select_children / make_parent (coarse.h:95-404) and the
 // conservative merge (sm_fluid.cpp:382-393), for the family whose first
 // child (z_idx % 4 == 0) is ich.
-__device__ void coarse_family(const AmrMesh& A, const DevState& z, int* wait, const Eos& eos, int ich) {
+__device__ void coarse_family(const AmrMesh& A, const DevState& z, int* wait, const Eos& eos, int ich, int* fam) {
     int ch[4];
     ch[0] = ich;
     int sibs[3];
     get_siblings(A, ich, sibs);
     for (int k = 0; k < 3; ++k) ch[A.z_idx[sibs[k]] & 3] = sibs[k];
+    if (fam)
+        for (int k = 0; k < 4; ++k) fam[k] = ch[k];
     const int ip = A.next[ich];
     set_cell(A, ip, vtx(A, ch[0], 0), vtx(A, ch[1], 2), vtx(A, ch[2], 6), vtx(A, ch[3], 8));
     const int ip_next = A.next[ip];
@@ -514,6 +516,10 @@ __device__ void coarse_family(const AmrMesh& A, const DevState& z, int* wait, co
             else A.adj[sf] = A.next[A.next[jc]];
         }
     }
+    if (!z.r) {   // mesh only: the solver's data are merged by its own kernel
+        for (int k = 0; k < 4; ++k) A.idx[ch[k]] = -1;
+        return;
+    }
     // merge: q_p = sum_k q(child_k) V_k / V_p
     double q[4] = {0.0, 0.0, 0.0, 0.0};
     for (int k = 0; k < 4; ++k) {
@@ -539,10 +545,12 @@ __device__ void coarse_family(const AmrMesh& A, const DevState& z, int* wait, co
 // the next blocks_refine the refined cells (a lane quad per parent, one lane
 // per child; the slope distributor's all-or-nothing fallback is decided
 // across the quad), the rest the coarsened families (one thread per family).
+// With z.r null the kernel does the mesh only, and `fam` receives the four
+// children of each coarsened family for the solver's own merge.
 __global__ void __launch_bounds__(256) k_amr_geometry(AmrMesh A, DevState z, DevGrad G, int* wait, Eos eos,
                                                       int slope, const int* ref_list, int n_refine,
                                                       const int* crs_list, int n_parents, int blocks_retain,
-                                                      int blocks_refine) {
+                                                      int blocks_refine, int* fam) {
     const int b = (int)blockIdx.x;
     if (b < blocks_retain) {
         const int ic = b * 256 + (int)threadIdx.x;
@@ -557,6 +565,10 @@ __global__ void __launch_bounds__(256) k_amr_geometry(AmrMesh A, DevState z, Dev
         const int ip = ref_list[pi];
         refine_child(A, ip, i);
         const int c = A.next[ip] + i;
+        if (!z.r) {
+            if (i == 0) A.idx[ip] = -1;
+            return;
+        }
         Z q;
         bool ok = false;
         if (slope) ok = split_slope(A, z, G, eos, ip, c, q);
@@ -570,7 +582,7 @@ __global__ void __launch_bounds__(256) k_amr_geometry(AmrMesh A, DevState z, Dev
     }
     const int t = (b - blocks_retain - blocks_refine) * 256 + (int)threadIdx.x;
     if (t >= n_parents) return;
-    coarse_family(A, z, wait, eos, crs_list[t]);
+    coarse_family(A, z, wait, eos, crs_list[t], fam ? fam + 4 * t : nullptr);
 }
 
 // SwapLists::move_elements (swap_lists.h:129-138) + AmrCells::move_item
@@ -611,8 +623,10 @@ __global__ void __launch_bounds__(256) k_amr_move(AmrMesh A, DevState z, int* wa
         A.cy[to] = A.cy[from];
         A.vol[to] = A.vol[from];
         A.idx[from] = -1;
-        z.r[to] = z.r[from]; z.u[to] = z.u[from]; z.v[to] = z.v[from]; z.p[to] = z.p[from]; z.e[to] = z.e[from];
-        wait[to] = wait[from];
+        if (z.r) {   // mesh only: the solver moves its own rows
+            z.r[to] = z.r[from]; z.u[to] = z.u[from]; z.v[to] = z.v[from]; z.p[to] = z.p[from]; z.e[to] = z.e[from];
+            wait[to] = wait[from];
+        }
     }
 }
 
@@ -691,8 +705,10 @@ __global__ void __launch_bounds__(256) k_amr_derive(AmrMesh A, DevState z, Eos e
         const double h = sqrt((s2 < s1) ? s2 : s1);
         A.h[c] = h;
         // CFL candidate of the adapted state (sm_fluid.cpp:104-114)
-        const Z zc{z.r[c], z.u[c], z.v[c], z.p[c], z.e[c]};
-        cand = h / (sqrt(vsq(zc)) + eos_c(eos, zc.r, zc.p));
+        if (z.r) {
+            const Z zc{z.r[c], z.u[c], z.v[c], z.p[c], z.e[c]};
+            cand = h / (sqrt(vsq(zc)) + eos_c(eos, zc.r, zc.p));
+        }
     }
     if (!(cand < INFINITY)) cand = INFINITY;   // NaN and +inf never win
     for (int off = 32; off > 0; off >>= 1) {
@@ -784,10 +800,10 @@ hipError_t amr_swapmap(const AmrMesh& A, int n_short, int n_large, int nz_total,
 
 hipError_t amr_geometry(const AmrMesh& A, const DevState& z, const DevGrad& G, int* wait, const Eos& eos,
                         bool slope, const int* ref_list, int n_refine, const int* crs_list, int n_parents,
-                        hipStream_t s) {
+                        int* fam, hipStream_t s) {
     const int br = (A.n + 255) / 256, bf = (4 * n_refine + 255) / 256, bc = (n_parents + 255) / 256;
     hipLaunchKernelGGL(k_amr_geometry, dim3(br + bf + bc), dim3(256), 0, s, A, z, G, wait, eos, slope ? 1 : 0,
-                       ref_list, n_refine, crs_list, n_parents, br, bf);
+                       ref_list, n_refine, crs_list, n_parents, br, bf, fam);
     return hipGetLastError();
 }
 

@@ -103,6 +103,7 @@ struct zfv_ctx {
     DBuf<unsigned long long> s_faces;
     DBuf<int> s_hole_at, s_ref_list, s_crs_list, s_swap_src, s_swap_dst;
     DBuf<Marks3> s_tiles;             // tile sums of the marks scan
+    DBuf<int> s_fam;                  // children of each coarsened family (multi-material merge)
     Marks3* marks_total = nullptr;    // pinned host: totals of the marks scan
     bool faces_pending = false;   // n_faces still on the device (s_faces)
     // device state
@@ -169,6 +170,9 @@ struct zfv_ctx {
         int cur = 0;
         long long step_no = 0;
         DBuf<double> st[2], dmax_dt, dt_out, inter, cf;
+        DBuf<double> gx, gy;          // limited gradients of zfv_mm_set_flags, [4 + n_mat][cap]
+        long long grad_epoch = -1;    // mesh epoch of gx / gy
+        int limiter = 3;              // MC (mm_fluid.cpp:20)
         DBuf<int> crp_list;
         DBuf<unsigned int> crp_count;
         DBuf<unsigned long long> dtmin;
@@ -485,6 +489,14 @@ int grow_capacity(zfv_ctx* ctx, int new_cap) {
     for (auto* b : {&ctx->gdx, &ctx->gdy}) ZFV_HIP(grow_rows(*b, 4, oc, new_cap, n, s));
     for (int l = 0; l < 3; ++l)
         for (int k = 0; k < 5; ++k) ZFV_HIP(grow_rows(ctx->st[l][k], 1, oc, new_cap, n, s));
+    auto& M = ctx->mm;   // the multi-material rows keep their [rows][cap] layout
+    if (M.configured) {
+        const int nm = M.mix.n;
+        for (auto& b : M.st)
+            if (b.p) ZFV_HIP(grow_rows(b, 7 + 2 * nm, oc, new_cap, n, s));
+        for (auto* b : {&M.gx, &M.gy})
+            if (b->p) ZFV_HIP(grow_rows(*b, 4 + nm, oc, new_cap, n, s));
+    }
     ZFV_HIP(ctx->partial.alloc(4 * (size_t)((new_cap + kBlock - 1) / kBlock)));
     ctx->cap = new_cap;
     return ZFV_OK;
@@ -1449,18 +1461,37 @@ static int scan3(zfv_ctx* ctx, const Marks3* in, Marks3* out, int count) {
 // new mesh.  The derive pass also seeds the next step's dt, so no further
 // launch is needed before stepping.  Consistency failures are reported by
 // the next synchronising call through the error word.
-int zfv_refine(zfv_ctx* ctx, int max_level, int distributor) {
-    int r = need_ready(ctx, true, true);
+// EuMesh::refine (eu_mesh.cpp:290-359) = balance_flags + amr::apply
+// (apply.h:43-94) with the solver's distributor: kRefineConst / kRefineSlope
+// (SmFluid, sm_fluid.cpp:372-470) or kRefineMm (MmFluid, mm_fluid.cpp:765-830,
+// which reads the gradients of zfv_mm_set_flags).  One host
+// synchronisation: the scan totals that size the new mesh.  The derive pass
+// (SmFluid) or the cache pass (MmFluid) also seeds the next step's dt.  The
+// other solver's state does not follow the mesh and is invalidated.
+// Consistency failures are reported by the next synchronising call through
+// the error word.
+enum { kRefineConst = 0, kRefineSlope = 1, kRefineMm = 2 };
+
+static MmLayer mm_layer(zfv_ctx* ctx, int i);
+static int mm_ready(zfv_ctx* ctx, bool state);
+
+static int refine_impl(zfv_ctx* ctx, int max_level, int mode) {
+    const bool mm = mode == kRefineMm;
+    auto& S = ctx->mm;
+    int r = mm ? mm_ready(ctx, true) : need_ready(ctx, true, true);
     if (r) return r;
     if (max_level < 0 || max_level > 15) return fail(ZFV_ERR_ARG, "max_level out of [0, 15]");
-    if (distributor != 0 && distributor != 1) return fail(ZFV_ERR_ARG, "distributor must be 0 (const) or 1 (slope)");
     if (ctx->nseg > 1) return fail(ZFV_ERR_UNSUPPORTED, "adaptation of a batch of problems");
+    if (mm && S.grad_epoch != ctx->mesh_epoch)
+        return fail(ZFV_ERR_STATE, "multi-material refine needs the gradients of zfv_mm_set_flags on this mesh");
     hipStream_t s = ctx->stream;
     if (max_level == 0) return ZFV_OK;   // EuMesh::adaptive() is false
-    r = to_mesh_order(ctx);
-    if (r) return r;
-    r = grad_to_mesh_order(ctx);   // the slope distributor reads them
-    if (r) return r;
+    if (!mm) {
+        r = to_mesh_order(ctx);
+        if (r) return r;
+        r = grad_to_mesh_order(ctx);   // the slope distributor reads them
+        if (r) return r;
+    }
     const int n = ctx->hm.n;
     ZFV_HIP(amr_balance(ctx->amesh(), max_level, s));
     ctx->host_stale = true;
@@ -1489,6 +1520,7 @@ int zfv_refine(zfv_ctx* ctx, int max_level, int distributor) {
     for (auto* b : {&ctx->s_hole_at, &ctx->s_ref_list, &ctx->s_crs_list, &ctx->s_swap_src, &ctx->s_swap_dst})
         if (b->n < (size_t)std::max(n, 1)) ZFV_HIP(b->alloc(std::max(n, 1)));
     if (ctx->s_faces.n < 2) ZFV_HIP(ctx->s_faces.alloc(2));
+    if (mm && ctx->s_fam.n < 4 * (size_t)std::max(n_parents, 1)) ZFV_HIP(ctx->s_fam.alloc(4 * (size_t)std::max(n_parents, 1)));
 
     AmrMesh A = ctx->amesh();   // A.n = n (cells before adaptation)
     const int rd = (int)(ctx->step_no % 3), wr = (int)((ctx->step_no + 1) % 3);
@@ -1496,23 +1528,47 @@ int zfv_refine(zfv_ctx* ctx, int max_level, int distributor) {
                           ctx->s_crs_list.p, ctx->s_faces.p, ctx->dtmin.p + rd, ctx->dtmin.p + wr, s));
     ZFV_HIP(amr_swapmap(A, n_short, n_large, nz_total, ctx->s_rank.p, ctx->s_hole_at.p, ctx->s_swap_src.p,
                         ctx->s_swap_dst.p, ctx->err.p, s));
+    const DevState z = mm ? DevState{} : ctx->layer(ctx->cur);
+    int* wait = mm ? nullptr : ctx->wait[ctx->wcur].p;
     DevGrad G{ctx->gdx.p, ctx->gdy.p};
-    ZFV_HIP(amr_geometry(A, ctx->layer(ctx->cur), G, ctx->wait[ctx->wcur].p, ctx->eos(), distributor == 1,
-                         ctx->s_ref_list.p, n_refine, ctx->s_crs_list.p, n_parents, s));
+    ZFV_HIP(amr_geometry(A, z, G, wait, ctx->eos(), mode == kRefineSlope, ctx->s_ref_list.p, n_refine,
+                         ctx->s_crs_list.p, n_parents, mm ? ctx->s_fam.p : nullptr, s));
+    const size_t cap = (size_t)ctx->cap;
+    if (mm)
+        ZFV_HIP(mm_distribute(A, mm_layer(ctx, S.cur), S.mix, S.gx.p, S.gy.p, ctx->s_ref_list.p, n_refine,
+                              ctx->s_fam.p, n_parents, ctx->err.p, s));
     // moves: one per hole (the kernel reads the hole count from the ranks);
     // there are at most min(changed cells, positions above n_short) of them
-    ZFV_HIP(amr_move(A, ctx->layer(ctx->cur), ctx->wait[ctx->wcur].p, n_short, nz_total, ctx->s_rank.p,
-                     std::min(nz_total, n_large - n_short), ctx->s_swap_src.p, ctx->s_swap_dst.p, s));
+    const int bound = std::min(nz_total, n_large - n_short);
+    ZFV_HIP(amr_move(A, z, wait, n_short, nz_total, ctx->s_rank.p, bound, ctx->s_swap_src.p, ctx->s_swap_dst.p, s));
+    if (mm)
+        ZFV_HIP(mm_move(mm_layer(ctx, S.cur), S.mix, (int)cap, n, n_short, nz_total, ctx->s_rank.p, bound,
+                        ctx->s_swap_src.p, ctx->s_swap_dst.p, s));
     ctx->hm.n = n_short;
-    ZFV_HIP(amr_derive(ctx->amesh(), ctx->layer(ctx->cur), ctx->eos(), ctx->s_faces.p, ctx->dtmin.p + rd,
-                       ctx->err.p, s));
+    ZFV_HIP(amr_derive(ctx->amesh(), z, ctx->eos(), ctx->s_faces.p, ctx->dtmin.p + rd, ctx->err.p, s));
     ctx->faces_pending = true;
-    ctx->dt_valid = true;
     ctx->graphs_clear();   // the cell count changed
     ctx->mesh_epoch += 1;
     ctx->co.refined = ctx->co.keys_ok;
     ctx->co.lmax = std::max(ctx->co.lmax, max_level);
+    if (mm) {
+        const int slot = (int)(S.step_no % 3);
+        ZFV_HIP(launch_fill_u64(S.dtmin.p + slot, 1, 0x7ff0000000000000ULL, s));
+        ZFV_HIP(mm_cache(ctx->dmesh(), mm_layer(ctx, S.cur), S.mix, S.dtmin.p + slot, s));
+        S.epoch = ctx->mesh_epoch;
+        ctx->have_state = false;   // the single-material state did not follow
+        ctx->co.state = false;
+        ctx->co.grad = false;
+        ctx->dt_valid = false;
+    } else {
+        ctx->dt_valid = true;
+    }
     return ZFV_OK;
+}
+
+int zfv_refine(zfv_ctx* ctx, int max_level, int distributor) {
+    if (distributor != 0 && distributor != 1) return fail(ZFV_ERR_ARG, "distributor must be 0 (const) or 1 (slope)");
+    return refine_impl(ctx, max_level, distributor == 1 ? kRefineSlope : kRefineConst);
 }
 
 // ---- discrete adjoint -------------------------------------------------------
@@ -1908,6 +1964,49 @@ int zfv_mm_interface(zfv_ctx* ctx, double* nx, double* ny, double* p) {
     }
     ZFV_HIP(hipStreamSynchronize(s));
     return read_err(ctx);
+}
+
+int zfv_mm_set_limiter(zfv_ctx* ctx, int limiter) {
+    int r = check_ctx(ctx);
+    if (r) return r;
+    if (limiter < 0 || limiter > 12) return fail(ZFV_ERR_ARG, "unknown limiter");
+    ctx->mm.limiter = limiter;
+    ctx->mm.grad_epoch = -1;
+    return ZFV_OK;
+}
+
+// MmFluid::set_flags (mm_fluid.cpp:836-885): compute_grad (kept for the
+// split distributor of zfv_mm_refine), then the flags.
+int zfv_mm_set_flags(zfv_ctx* ctx) {
+    int r = mm_ready(ctx, true);
+    if (r) return r;
+    auto& S = ctx->mm;
+    const size_t rows = 4 + (size_t)S.mix.n, cap = (size_t)ctx->cap;
+    for (auto* b : {&S.gx, &S.gy})
+        if (b->n != rows * cap) ZFV_HIP(b->alloc(rows * cap));
+    const DevMesh M = ctx->dmesh();
+    const MmLayer L = mm_layer(ctx, S.cur);
+    ZFV_HIP(mm_grad(M, L, S.mix, S.crp, S.limiter, S.gx.p, S.gy.p, ctx->stream));
+    ZFV_HIP(mm_flags(M, L, S.mix, ctx->flag.p, ctx->stream));
+    ctx->host_stale = true;
+    S.grad_epoch = ctx->mesh_epoch;
+    return ZFV_OK;
+}
+
+int zfv_mm_refine(zfv_ctx* ctx, int max_level) { return refine_impl(ctx, max_level, kRefineMm); }
+
+int zfv_mm_grad(zfv_ctx* ctx, double* gx, double* gy) {
+    int r = mm_ready(ctx, true);
+    if (r) return r;
+    if (!gx || !gy) return fail(ZFV_ERR_ARG, "mm_grad: null array");
+    auto& S = ctx->mm;
+    if (S.grad_epoch != ctx->mesh_epoch) return fail(ZFV_ERR_STATE, "no multi-material gradients for this mesh");
+    const int n = ctx->hm.n, rows = 4 + S.mix.n;
+    const size_t row = (size_t)n * sizeof(double), pitch = (size_t)ctx->cap * sizeof(double);
+    ZFV_HIP(hipMemcpy2DAsync(gx, row, S.gx.p, pitch, row, rows, hipMemcpyDeviceToHost, ctx->stream));
+    ZFV_HIP(hipMemcpy2DAsync(gy, row, S.gy.p, pitch, row, rows, hipMemcpyDeviceToHost, ctx->stream));
+    ZFV_HIP(hipStreamSynchronize(ctx->stream));
+    return ZFV_OK;
 }
 
 int zfv_limiter_from_name(const char* orig) {

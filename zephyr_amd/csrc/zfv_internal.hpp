@@ -104,9 +104,12 @@ hipError_t amr_positions(const AmrMesh& A, int n_large, int n_short, int n_paren
                          unsigned long long* dt_rd, unsigned long long* dt_wr, hipStream_t s);
 hipError_t amr_swapmap(const AmrMesh& A, int n_short, int n_large, int nz_total, const Marks3* rank,
                        const int* hole_at, int* swap_src, int* swap_dst, int* err, hipStream_t s);
+// z.r null: mesh only (the solver's data are distributed and moved by its
+// own kernels); fam (may be null) receives the four children of each
+// coarsened family.
 hipError_t amr_geometry(const AmrMesh& A, const DevState& z, const DevGrad& G, int* wait, const Eos& eos,
                         bool slope, const int* ref_list, int n_refine, const int* crs_list, int n_parents,
-                        hipStream_t s);
+                        int* fam, hipStream_t s);
 hipError_t amr_move(const AmrMesh& A, const DevState& z, int* wait, int n_short, int nz_total, const Marks3* rank,
                     int bound, const int* swap_src, const int* swap_dst, hipStream_t s);
 hipError_t amr_derive(const AmrMesh& A, const DevState& z, const Eos& eos, unsigned long long* counts,
@@ -200,6 +203,20 @@ hipError_t mm_crp(const DevMesh& M, const MmLayer& in, const mm::Mix& mx, const 
                   const double* ny, const int* list, const unsigned int* count, double* CF, hipStream_t s);
 hipError_t mm_interface(const DevMesh& M, const MmLayer& L, const mm::Mix& mx, const double* vx, const double* vy,
                         double* nx, double* ny, double* p, hipStream_t s);
+// Adaptation of the multi-material state: the limited gradient (rows rho,
+// u, v, e, beta[n_mat] of gx / gy, [4 + n_mat][stride]), the refinement
+// flags, the split / merge distributor (fam: the four children of each
+// coarsened family), the moves of the state rows, and the cached sound
+// speed with the next step's dt candidate.
+hipError_t mm_grad(const DevMesh& M, const MmLayer& L, const mm::Mix& mx, int crp, int limiter, double* gx,
+                   double* gy, hipStream_t s);
+hipError_t mm_flags(const DevMesh& M, const MmLayer& L, const mm::Mix& mx, int* flag, hipStream_t s);
+hipError_t mm_distribute(const AmrMesh& A, const MmLayer& L, const mm::Mix& mx, const double* gx, const double* gy,
+                         const int* ref_list, int n_refine, const int* fam, int n_parents, int* err, hipStream_t s);
+hipError_t mm_move(const MmLayer& L, const mm::Mix& mx, int stride, int n, int n_short, int nz_total,
+                   const Marks3* rank, int bound, const int* swap_src, const int* swap_dst, hipStream_t s);
+hipError_t mm_cache(const DevMesh& M, const MmLayer& L, const mm::Mix& mx, unsigned long long* slot,
+                    hipStream_t s);
 
 // Error-word bits: 1 numeric, 2 inconsistent mesh (errors); kErrRobust and
 // kErrIsolated are notices for the adjoint (a cell in the robust isolation

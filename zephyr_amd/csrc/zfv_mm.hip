@@ -369,6 +369,349 @@ __global__ void __launch_bounds__(kBlock) k_mm_interface(DevMesh M, MmLayer L, c
     }
 }
 
+// ---- adaptation: MmFluid::compute_grad, set_flags, distributor --------------
+
+// The limiters by number (the gradient runs once per adaptation).
+__device__ double limiter_rt(int kind, double a, double b) {
+    switch (kind) {
+        case 0: return limiter2<0>(a, b);
+        case 1: return limiter2<1>(a, b);
+        case 2: return limiter2<2>(a, b);
+        case 3: return limiter2<3>(a, b);
+        case 4: return limiter2<4>(a, b);
+        case 5: return limiter2<5>(a, b);
+        case 6: return limiter2<6>(a, b);
+        case 7: return limiter2<7>(a, b);
+        case 8: return limiter2<8>(a, b);
+        case 9: return limiter2<9>(a, b);
+        case 10: return limiter2<10>(a, b);
+        case 11: return limiter2<11>(a, b);
+        default: return limiter2<12>(a, b);
+    }
+}
+
+// Components of the gradient the split reads: rho, u, v, e, beta[i].
+template <int NM>
+__device__ __forceinline__ void mg_comps(const MmLayer& L, int c, int st, double* q) {
+    q[0] = L.r[c];
+    q[1] = L.u[c];
+    q[2] = L.v[c];
+    q[3] = L.e[c];
+#pragma unroll
+    for (int i = 0; i < NM; ++i) q[4 + i] = L.b[i * st + c];
+}
+
+// MmFluid::compute_grad (mm_fluid.cpp:439-466): with composite Riemann
+// problems on (crp != 0) zero in a mixed cell and in a cell with a mixed
+// face neighbour; else gradient::LSM (gradient.h:226-282) and
+// gradient::limiting (:412-474) with the mm boundary_value
+// (mm_fluid.cpp:98-108: walls reflect the velocity), component by
+// component as k_grad_pred does for SmFluid (the 2 x 2 block of the LSM
+// system).  gx, gy: [4 + NM][stride].
+template <int NM>
+__global__ void __launch_bounds__(kBlock) k_mm_grad(DevMesh M, MmLayer L, int crp, int limiter,
+                                                    double* __restrict__ gx, double* __restrict__ gy) {
+    constexpr int K = 4 + NM;
+    const int c = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (c >= M.n) return;
+    const int st = M.stride;
+    const int code = M.code[c];
+    double zc[K];
+    mg_comps<NM>(L, c, st, zc);
+    bool zero = false;
+    if (crp) {
+        double b[NM];
+#pragma unroll
+        for (int i = 0; i < NM; ++i) b[i] = zc[4 + i];
+        zero = frac_index<NM>(b) < 0;
+        for (int s = 0; s < 8 && !zero; ++s) {
+            const int t = (code >> (4 * s)) & 7;
+            if (t != SC_INNER && t != SC_PERIODIC) continue;
+            const int nb = M.adj[s * st + c];
+#pragma unroll
+            for (int i = 0; i < NM; ++i) b[i] = L.b[i * st + nb];
+            if (frac_index<NM>(b) < 0) zero = true;
+        }
+    }
+    double Gx[K], Gy[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        Gx[k] = 0.0;
+        Gy[k] = 0.0;
+    }
+    if (!zero) {
+        const double ccx = M.cx[c], ccy = M.cy[c];
+        double A00 = 0.0, A01 = 0.0, A10 = 0.0, A11 = 0.0;
+        double Fx[K], Fy[K];
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            Fx[k] = 0.0;
+            Fy[k] = 0.0;
+        }
+        // pass 1: the LSM system; pass 2: the limited right-hand side
+        for (int pass = 0; pass < 2; ++pass) {
+            double B00 = 0.0, B01 = 0.0, B10 = 0.0, B11 = 0.0;
+            if (pass == 1) {
+                if (fabs(A11) < 1e-8) A11 = 1.0;
+                const double det = A11 * A00 + (-A01) * A10;
+                const double invdet = 1.0 / det;
+                B00 = A11 * invdet;
+                B01 = (-A01) * invdet;
+                B10 = (-A10) * invdet;
+                B11 = A00 * invdet;
+#pragma unroll
+                for (int k = 0; k < K; ++k) {
+                    Gx[k] = B00 * Fx[k] + B01 * Fy[k];
+                    Gy[k] = B10 * Fx[k] + B11 * Fy[k];
+                    Fx[k] = 0.0;
+                    Fy[k] = 0.0;
+                }
+            }
+#pragma unroll 1
+            for (int s = 0; s < 8; ++s) {
+                const int nib = (code >> (4 * s)) & 15;
+                const int t = nib & 7;
+                if (t == SC_UNDEF) continue;
+                const int so = s * st + c;
+                const Frame fr = slot_frame(M, c, s, nib);
+                const int nb = M.adj[so];
+                double zn[K];
+                if (t == SC_INNER || t == SC_PERIODIC) {
+                    mg_comps<NM>(L, nb, st, zn);
+                } else {
+#pragma unroll
+                    for (int k = 0; k < K; ++k) zn[k] = zc[k];
+                    const Z w = frame_boundary(fr, Z{zc[0], zc[1], zc[2], 0.0, zc[3]}, t == SC_WALL);
+                    zn[1] = w.u;
+                    zn[2] = w.v;
+                }
+                double px = M.cx[nb], py = M.cy[nb];
+                if (t != SC_INNER) frame_symm(fr, ccx, ccy, M.fcx[so], M.fcy[so], px, py);
+                const double drx = px - ccx, dry = py - ccy;
+                double dnx, dny;
+                if (!fr.gen) {
+                    dnx = fr.side < 2 ? drx : 0.0;
+                    dny = fr.side < 2 ? 0.0 : dry;
+                } else {
+                    const double d = drx * fr.nx + dry * fr.ny;
+                    dnx = d * fr.nx;
+                    dny = d * fr.ny;
+                }
+                const bool xp = fr.gen || fr.side < 2, yp = fr.gen || fr.side >= 2;
+                const double w = M.w[so];
+                if (pass == 0) {
+                    if (xp) {
+                        const double wd = w * dnx;
+                        A00 = A00 + wd * drx;
+                        A01 = A01 + wd * dry;
+                    }
+                    if (yp) {
+                        const double wd = w * dny;
+                        A10 = A10 + wd * drx;
+                        A11 = A11 + wd * dry;
+                    }
+                }
+#pragma unroll
+                for (int k = 0; k < K; ++k) {
+                    const double dF = zn[k] - zc[k];
+                    double term = dF;
+                    if (pass == 1) {
+                        const double du_n = Gx[k] * drx + Gy[k] * dry;
+                        term = limiter_rt(limiter, 2.0 * du_n - dF, dF) * dF;
+                    }
+                    if (xp) Fx[k] = Fx[k] + (w * term) * dnx;
+                    if (yp) Fy[k] = Fy[k] + (w * term) * dny;
+                }
+            }
+            if (pass == 1) {
+#pragma unroll
+                for (int k = 0; k < K; ++k) {
+                    Gx[k] = B00 * Fx[k] + B01 * Fy[k];
+                    Gy[k] = B10 * Fx[k] + B11 * Fy[k];
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        gx[k * st + c] = Gx[k];
+        gy[k * st + c] = Gy[k];
+    }
+}
+
+// MmFluid::set_flags (mm_fluid.cpp:836-885) after its compute_grad: +1 for
+// a mixed cell or a pure cell with a face neighbour of another material
+// (or mixed; boundary faces skipped), else -1.
+template <int NM>
+__global__ void __launch_bounds__(kBlock) k_mm_flags(DevMesh M, MmLayer L, int* __restrict__ flag) {
+    const int c = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (c >= M.n) return;
+    const int st = M.stride;
+    double b[NM];
+#pragma unroll
+    for (int i = 0; i < NM; ++i) b[i] = L.b[i * st + c];
+    if (!frac_pure<NM>(b)) {
+        flag[c] = 1;
+        return;
+    }
+    const int idx = frac_index<NM>(b);
+    const int code = M.code[c];
+    bool split = false;
+    for (int s = 0; s < 8 && !split; ++s) {
+        const int t = (code >> (4 * s)) & 7;
+        if (t != SC_INNER && t != SC_PERIODIC) continue;
+        const int nb = M.adj[s * st + c];
+#pragma unroll
+        for (int i = 0; i < NM; ++i) b[i] = L.b[i * st + nb];
+        if (frac_index<NM>(b) != idx) split = true;
+    }
+    flag[c] = split ? 1 : -1;
+}
+
+// PState::is_bad, mmf (models.cpp:393-403).
+template <int NM>
+__device__ __forceinline__ bool mm_is_bad(const MZ<NM>& z) {
+    bool empty = true;   // Fractions::empty (fractions.cpp:91-99)
+#pragma unroll
+    for (int i = 0; i < NM; ++i)
+        if (z.b[i] > kFracMinimal) empty = false;
+    auto bad = [](double x) { return isinf(x) || isnan(x); };
+    return bad(z.r) || bad(z.u) || bad(z.v) || bad(z.P) || bad(z.e) || bad(z.T) || empty || z.r < 0.0;
+}
+
+// MmFluid::distributor split (mm_fluid.cpp:768-800), one thread per child
+// (ref_list: the refined cells; child i of parent ip at next[ip] + i, its
+// geometry already written): child = parent + grad . dr with momentum- and
+// energy-consistent corrections, beta normalised, P from pressure_re(rho,
+// e, beta, {P0, T0} of the parent), T from temperature_rP(rho, P, beta,
+// {T0}); the true densities are the default-constructed ScalarSet, zeros.
+// The sound speed is left to k_mm_cache.  A state the reference rejects
+// (it throws) sets bit 0 of the error word.
+template <int NM>
+__global__ void __launch_bounds__(256) k_mm_split(AmrMesh A, MmLayer L, const double* __restrict__ gx,
+                                                  const double* __restrict__ gy, Mix mx,
+                                                  const int* __restrict__ ref_list, int n_refine, int* err) {
+    const int t = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    const int pi = t >> 2, i = t & 3;
+    if (pi >= n_refine) return;
+    const int st = A.stride;
+    const int ip = ref_list[pi];
+    const int c = A.next[ip] + i;
+    const MZ<NM> zp = load_mz<NM>(L, ip, st);
+    const double drx = A.cx[c] - A.cx[ip], dry = A.cy[c] - A.cy[ip];
+    double d[4 + NM];
+#pragma unroll
+    for (int k = 0; k < 4 + NM; ++k) d[k] = gx[k * st + ip] * drx + gy[k * st + ip] * dry;
+    MZ<NM> z;
+    z.r = zp.r + d[0];
+    const double ratio = zp.r / z.r;
+    z.u = zp.u + ratio * d[1];
+    z.v = zp.v + ratio * d[2];
+#pragma unroll
+    for (int k = 0; k < NM; ++k) z.b[k] = zp.b[k] + (zp.r / z.r) * d[4 + k];
+    frac_cutoff<NM>(z.b, kFracMinimal);
+    const double du = zp.u - z.u, dv = zp.v - z.v;
+    z.e = (zp.e + 0.5 * (du * du + dv * dv)) + (zp.r / z.r) * d[3];
+    const int idx = frac_index<NM>(z.b);
+    double rh[NM];
+    if (idx >= 0) {
+        const Mat m = pick<NM>(mx, idx);
+        z.P = sg_pressure_re(m, z.r, z.e);
+        z.T = sg_temperature_rP(m, z.r, z.P);
+    } else {
+        double Tt;
+        DPDT vl, el;
+        find_rPT<NM>(mx, z.r, z.e, z.b, zp.P, zp.T, nullptr, rh, z.P, Tt, vl, el);
+        DPDT v;
+        find_rT_rP<NM>(mx, z.r, z.P, z.b, zp.T, nullptr, rh, z.T, v);
+    }
+#pragma unroll
+    for (int k = 0; k < NM; ++k) z.rh[k] = 0.0;
+    z.c = 0.0;
+    if (mm_is_bad<NM>(z)) atomicOr(err, 1);
+    store_mz<NM>(L, c, st, z);
+}
+
+// MmFluid::distributor merge (mm_fluid.cpp:802-830), one thread per
+// coarsened family (fam: its four children, parent at next[child]):
+// volume-weighted conserved sum, negative partial densities clipped, the
+// closure started from the volume-weighted mean P and T (reference
+// densities as the true-density guesses).
+template <int NM>
+__global__ void __launch_bounds__(256) k_mm_merge(AmrMesh A, MmLayer L, Mix mx, const int* __restrict__ fam,
+                                                  int n_parents, int* err) {
+    const int t = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (t >= n_parents) return;
+    const int st = A.stride;
+    double q[4 + NM];
+#pragma unroll
+    for (int k = 0; k < 4 + NM; ++k) q[k] = 0.0;
+    double mean_p = 0.0, mean_t = 0.0;
+    const int ip = A.next[fam[4 * t]];
+    for (int k = 0; k < 4; ++k) {
+        const int c = fam[4 * t + k];
+        const MZ<NM> zc = load_mz<NM>(L, c, st);
+        double qc[4 + NM];
+        m_qstate<NM>(zc, qc);
+        const double V = A.vol[c];
+#pragma unroll
+        for (int j = 0; j < 4 + NM; ++j) q[j] = q[j] + qc[j] * V;
+        mean_p = mean_p + zc.P * V;
+        mean_t = mean_t + zc.T * V;
+    }
+    const double Vp = A.vol[ip];
+#pragma unroll
+    for (int j = 0; j < 4 + NM; ++j) q[j] = q[j] / Vp;
+    mean_p = mean_p / Vp;
+    mean_t = mean_t / Vp;
+#pragma unroll
+    for (int k = 0; k < NM; ++k) q[4 + k] = (0.0 < q[4 + k]) ? q[4 + k] : 0.0;
+    MZ<NM> z;   // PState(QState, mixture, P0, T0) (models.cpp:187-201)
+    z.r = q[0];
+    z.u = q[1] / z.r;
+    z.v = q[2] / z.r;
+    z.e = q[3] / z.r - 0.5 * (z.u * z.u + z.v * z.v);
+#pragma unroll
+    for (int k = 0; k < NM; ++k) z.b[k] = q[4 + k] / z.r;
+    frac_cutoff<NM>(z.b, 1.0e-12);
+    frac_cutoff<NM>(z.b, kFracMinimal);
+    get_rPT<NM>(mx, z.r, z.e, z.b, mean_p, mean_t, nullptr, z.rh, z.P, z.T);
+    z.c = 0.0;
+    if (mm_is_bad<NM>(z)) atomicOr(err, 1);
+    store_mz<NM>(L, ip, st, z);
+}
+
+// SwapLists::move_elements (swap_lists.h:129-138) for the state rows.
+template <int NM>
+__global__ void __launch_bounds__(256) k_mm_move(MmLayer L, int st, int n_short, int nz_total, int n,
+                                                 const Marks3* __restrict__ rank, int bound,
+                                                 const int* __restrict__ swap_src, const int* __restrict__ swap_dst) {
+    const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (i >= bound) return;
+    const int n_swap = n_short >= n ? nz_total : rank[n_short].nz;   // the holes
+    if (i >= n_swap) return;
+    const int from = swap_src[i], to = swap_dst[i];
+    store_mz<NM>(L, to, st, load_mz<NM>(L, from, st));
+}
+
+// The cached sound speed (sound_speed_rP with the state's T and true
+// densities as guesses, as compute_dt and the fluxes take it) and the
+// compute_dt candidate of every cell after an adaptation.
+template <int NM>
+__global__ void __launch_bounds__(kBlock) k_mm_cache(DevMesh M, MmLayer L, Mix mx, unsigned long long* slot) {
+    const Tile tl = tile_of(M, (int)blockDim.x);
+    const int c = tl.c0 + (int)threadIdx.x;
+    double cand = INFINITY;
+    if (c < tl.end) {
+        const int st = M.stride;
+        MZ<NM> z = load_mz<NM>(L, c, st);
+        z.c = sound_speed<NM>(mx, z.r, z.P, z.b, z.T, z.rh);
+        L.c[c] = z.c;
+        cand = mm_dt_cand<NM>(M, c, z);
+    }
+    block_min_atomic(slot + 3 * tl.pid, cand);
+}
+
 // ---- launchers --------------------------------------------------------------
 static inline dim3 mm_grid(int n) { return dim3((n + kBlock - 1) / kBlock); }
 
@@ -413,6 +756,42 @@ hipError_t mm_crp(const DevMesh& M, const MmLayer& in, const Mix& mx, const Step
 hipError_t mm_interface(const DevMesh& M, const MmLayer& L, const Mix& mx, const double* vx, const double* vy,
                         double* nx, double* ny, double* p, hipStream_t s) {
     ZFV_MM_DISPATCH(mx.n, k_mm_interface, mm_grid(M.n), dim3(kBlock), 0, s, M, L, vx, vy, nx, ny, p);
+    return hipGetLastError();
+}
+
+
+hipError_t mm_grad(const DevMesh& M, const MmLayer& L, const Mix& mx, int crp, int limiter, double* gx, double* gy,
+                   hipStream_t s) {
+    ZFV_MM_DISPATCH(mx.n, k_mm_grad, mm_grid(M.n), dim3(kBlock), 0, s, M, L, crp, limiter, gx, gy);
+    return hipGetLastError();
+}
+
+hipError_t mm_flags(const DevMesh& M, const MmLayer& L, const Mix& mx, int* flag, hipStream_t s) {
+    ZFV_MM_DISPATCH(mx.n, k_mm_flags, mm_grid(M.n), dim3(kBlock), 0, s, M, L, flag);
+    return hipGetLastError();
+}
+
+hipError_t mm_distribute(const AmrMesh& A, const MmLayer& L, const Mix& mx, const double* gx, const double* gy,
+                         const int* ref_list, int n_refine, const int* fam, int n_parents, int* err, hipStream_t s) {
+    if (n_refine > 0)
+        ZFV_MM_DISPATCH(mx.n, k_mm_split, dim3((4 * n_refine + 255) / 256), dim3(256), 0, s, A, L, gx, gy, mx,
+                        ref_list, n_refine, err);
+    if (n_parents > 0)
+        ZFV_MM_DISPATCH(mx.n, k_mm_merge, dim3((n_parents + 255) / 256), dim3(256), 0, s, A, L, mx, fam, n_parents,
+                        err);
+    return hipGetLastError();
+}
+
+hipError_t mm_move(const MmLayer& L, const Mix& mx, int stride, int n, int n_short, int nz_total, const Marks3* rank,
+                   int bound, const int* swap_src, const int* swap_dst, hipStream_t s) {
+    if (bound <= 0) return hipSuccess;
+    ZFV_MM_DISPATCH(mx.n, k_mm_move, dim3((bound + 255) / 256), dim3(256), 0, s, L, stride, n_short, nz_total, n,
+                    rank, bound, swap_src, swap_dst);
+    return hipGetLastError();
+}
+
+hipError_t mm_cache(const DevMesh& M, const MmLayer& L, const Mix& mx, unsigned long long* slot, hipStream_t s) {
+    ZFV_MM_DISPATCH(mx.n, k_mm_cache, mm_grid(M.n), dim3(kBlock), 0, s, M, L, mx, slot);
     return hipGetLastError();
 }
 
