@@ -72,9 +72,15 @@ WORKLOADS = {
         kind="mm", nx=512, ny=512, steps=300, cfl=0.5,
         desc="two-material shock-bubble on a 512x512 mesh (the C2 base mesh, without AMR): stiffened-gas water "
              "(gamma 4.4, P0 6e8) with an air bubble (gamma 1.4) of radius 0.15 at the centre, a planar shock of "
-             "seeded Mach number 1.5-3 at x = 0.2 moving into the water; MmFluid first order with its defaults: "
+             "seeded Mach number 1.5-3 at x = 0.1 moving into the water; MmFluid first order with its defaults: "
              "material-aware HLLC, composite Riemann fluxes at faces next to mixed cells with the PLIC face "
              "fraction (interface reconstruction every step), PT-equilibrium closure; CFL 0.5, FP64, ZOE bounds"),
+    "mm512_bubble_amr2": dict(
+        kind="mm", nx=512, ny=512, steps=300, cfl=0.5, max_level=2, every=4,
+        desc="the two-material shock-bubble of mm512_bubble_first_order on the C2 configuration: 512x512 base, up "
+             "to 2 refinement levels by MmFluid::set_flags (mixed cells and cells next to another material) and "
+             "the MmFluid split / merge distributor every 4 steps inside the timed region; initial adaptation in "
+             "max_level + 1 rounds with the initial data re-sampled on each mesh"),
     "ensemble256_muscl": dict(
         kind="ensemble", problems=256, smin=64, smax=512, acc=2, limiter="MC", steps=200,
         desc="256 independent problems of s x s cells, s ~ U{64..512} (seeded), per-problem seeded "
@@ -312,32 +318,74 @@ def mm_interface_bytes_per_cell(n_mat, faces_per_cell=4):
     return 8 * (1 + 2 * n_mat) + 4 + faces_per_cell * (4 + 8 + 16) + 8 * 8 + 3 * 8 * n_mat
 
 
+def mm_adapt_initial(set_state, centres, set_flags, refine, seed, max_level):
+    """Initial adaptation: max_level + 1 rounds of (the shock-bubble data at
+    the cell centres, sub-sampled over each cell's width, flags, refine),
+    then the data on the final mesh."""
+    from zephyr_amd import problems
+    for _ in range(max_level + 1):
+        cx, cy, h = centres()
+        set_state(*problems.mm_bubble_shock(cx, cy, h, seed))
+        set_flags()
+        refine(max_level)
+    cx, cy, h = centres()
+    set_state(*problems.mm_bubble_shock(cx, cy, h, seed))
+
+
+def mm_steps(ctx, w, steps, step0):
+    """`steps` multi-material steps from global step index step0, adapting
+    before every step whose index is a multiple of w["every"] (AMR
+    workloads).  Returns the number of cell-updates."""
+    cells, i = 0, 0
+    every = w.get("every")
+    while i < steps:
+        k = steps - i
+        if every:
+            g = step0 + i
+            if g % every == 0:
+                ctx.mm_set_flags()
+                ctx.mm_refine(w["max_level"])   # synchronises
+            k = min(k, every - (g % every))
+        ctx.mm_step(k)
+        cells += ctx.n * k
+        i += k
+    return cells
+
+
 def mm_run(args, w, Z):
-    """Multi-material first-order step (MmFluid, no CRP) on the shock-bubble."""
+    """Multi-material first-order step (MmFluid with composite Riemann
+    fluxes) on the shock-bubble, with or without adaptation."""
     from zephyr_amd import problems
     nx, ny = w["nx"], w["ny"]
     ctx = Z.Context()
     ctx.mesh_rectangle(nx, ny)
     mats = [problems.WATER, problems.AIR]
     ctx.mm_configure(mats, w["cfl"])
-    cx, cy = rect_centres(nx, ny)
-    st = problems.mm_bubble_shock(cx, cy, 1.0 / nx, args.seed)
-    ctx.mm_set_state(*st)
-    ctx.mm_step(args.warmup)
+    amr = "max_level" in w
+    st = None
+    if amr:
+        def centres():
+            c, _, _ = ctx.mesh_download()
+            return c["cx"], c["cy"], np.sqrt(c["volume"])
+        mm_adapt_initial(ctx.mm_set_state, centres, ctx.mm_set_flags, ctx.mm_refine, args.seed, w["max_level"])
+    else:
+        cx, cy = rect_centres(nx, ny)
+        st = problems.mm_bubble_shock(cx, cy, 1.0 / nx, args.seed)
+        ctx.mm_set_state(*st)
+    mm_steps(ctx, w, args.warmup, 0)
     ctx.synchronize()
     t0 = time.perf_counter()
-    ctx.mm_step(args.steps)
+    cells = mm_steps(ctx, w, args.steps, args.warmup)
     ctx.synchronize()
     el = time.perf_counter() - t0
     ctx.profile_enable(True)
-    ctx.mm_step(args.prof_steps)
+    n = mm_steps(ctx, w, args.prof_steps, args.warmup + args.steps) / max(args.prof_steps, 1)
     ctx.synchronize()
     prof = ctx.profile_read()
     ctx.profile_enable(False)
     n_crp = ctx.mm_crp_faces()
     z = ctx.mm_state()
     mixed = int(np.sum(np.sum(z["beta"] > 0.0, axis=0) > 1))
-    n = nx * ny
     alg_of = {"mm_flux1": mm_bytes_per_cell(2) * n, "mm_crp": mm_crp_bytes_per_face(2) * n_crp,
               "mm_crp_mark": mm_crp_mark_bytes_per_cell(2) * n, "mm_interface": mm_interface_bytes_per_cell(2) * n}
     dom = max(prof, key=lambda k: prof[k][0])
@@ -345,11 +393,14 @@ def mm_run(args, w, Z):
     alg = alg_of[dom]
     achieved = alg / (avg_ms * 1e-3) / 1e9
     out = {
-        "metric": "FP64 cell-updates/s (multi-material first order, HLLC + PLIC composite fluxes + PT closure)",
-        "value": n * args.steps / el, "unit": "cell-updates/s", "n_gpus": 1, "steps": args.steps,
+        "metric": "FP64 cell-updates/s (multi-material first order, HLLC + PLIC composite fluxes + PT closure"
+                  + (", AMR)" if amr else ")"),
+        "value": cells / el, "unit": "cell-updates/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": el * 1e3 / args.steps, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f64", "data": "synthetic",
-        "config": {"workload": args.workload, "description": w["desc"], "cells": n,
+        "config": {"workload": args.workload, "description": w["desc"], "cells": ctx.n,
+                   "mean_cells_per_step": cells / args.steps,
+                   "levels": np.bincount(ctx.mesh_download()[0]["level"]).tolist(),
                    "mach": problems.mm_bubble_shock_params(args.seed), "mixed_cells_final": mixed,
                    "crp_faces_last_step": n_crp,
                    "kernel_hbm_frac": {k: alg_of[k] / (prof[k][0] / prof[k][1] * 1e-3) / 1e9 / HBM_PEAK_GBS
@@ -362,25 +413,37 @@ def mm_run(args, w, Z):
                      "algorithmic_bytes_per_launch": alg, "avg_launch_ms": avg_ms},
     }
     if not args.no_cpu_baseline:
-        out["cpu_baseline"] = mm_cpu_baseline(w, st, mats, args.cpu_budget)
+        out["cpu_baseline"] = mm_cpu_baseline(w, st, mats, args.cpu_budget, args.seed)
     print(json.dumps(out), flush=True)
 
 
-def mm_cpu_baseline(w, st, mats, budget_s):
-    """Oracle MmFluid (serial C++) on the same initial data, whole steps until the budget is spent."""
+def mm_cpu_baseline(w, st, mats, budget_s, seed):
+    """Oracle MmFluid (serial C++) on the same workload (same initial data,
+    same adaptation), whole steps until the budget is spent."""
     from oracle import pyoracle as O
     O.build()
     m = O.Mesh.rectangle(w["nx"], w["ny"])
     o = O.MmFluid(m, mats, w["cfl"])
-    o.set_state(*st)
-    steps, t0 = 0, time.perf_counter()
+    amr = "max_level" in w
+    if amr:
+        def centres():
+            c = m.cells()
+            return c["cx"], c["cy"], np.sqrt(c["volume"])
+        mm_adapt_initial(o.set_state, centres, o.set_flags, o.refine, seed, w["max_level"])
+    else:
+        o.set_state(*st)
+    steps, cells, t0 = 0, 0, time.perf_counter()
     while time.perf_counter() - t0 < budget_s:
+        if amr and steps % w["every"] == 0:
+            o.set_flags()
+            o.refine(w["max_level"])
+        cells += m.n
         o.update()
         steps += 1
     el = time.perf_counter() - t0
-    return {"value": m.n * steps / el, "unit": "cell-updates/s", "cores": 1, "kind": "port",
+    return {"value": cells / el, "unit": "cell-updates/s", "cores": 1, "kind": "port",
             "sample": f"oracle MmFluid (C++ -O3 -ffp-contract=off, serial) on the same workload from the same "
-                      f"initial data, {steps} steps in {el:.2f} s"}
+                      f"initial data, {steps} steps{' (adaptation included)' if amr else ''} in {el:.2f} s"}
 
 
 def ensemble_run(args, w, Z):

@@ -129,3 +129,43 @@ def test_refine_needs_current_gradients_and_drops_the_single_material_state(zfv)
     ctx.mm_step(2)
     with pytest.raises(zfv.ZfvError, match="gradients"):
         ctx.mm_refine(2)
+
+
+def test_shock_bubble_workload_with_adaptation_matches_oracle(oracle, zfv):
+    """The benchmark's shock-bubble (water / air, a seeded Mach 1.5-3 shock)
+    on a 64 x 64 base with two levels, adapting every 4 steps after the
+    initial adaptation: bit-exact state and dt after every cycle."""
+    from zephyr_amd import problems
+    mats = [problems.WATER, problems.AIR]
+    nx = 64
+    m = oracle.Mesh.rectangle(nx, nx)
+    o = oracle.MmFluid(m, mats, 0.5)
+    ctx = zfv.Context()
+    ctx.mesh_rectangle(nx, nx)
+    ctx.mm_configure(mats, 0.5)
+    for rnd in range(4):
+        c = m.cells()
+        st = problems.mm_bubble_shock(c["cx"], c["cy"], np.sqrt(c["volume"]), 3, x_shock=0.35)
+        o.set_state(*st)
+        ctx.mm_set_state(*st)
+        assert_same(ctx.mm_state(), o.state(), f"initial round {rnd}")
+        if rnd == 3:
+            break
+        o.set_flags()
+        ctx.mm_set_flags()
+        assert o.refine(2) >= 0
+        ctx.mm_refine(2)
+        compare_mesh(ctx, m, f"initial round {rnd}")
+    for cycle in range(12):
+        o.set_flags()
+        ctx.mm_set_flags()
+        assert np.array_equal(ctx.flags(), m.cells()["flag"]), f"cycle {cycle}: flags"
+        assert o.refine(2) >= 0, f"cycle {cycle}: the oracle rejected a state"
+        ctx.mm_refine(2)
+        compare_mesh(ctx, m, f"cycle {cycle}")
+        assert_same(ctx.mm_state(), o.state(), f"cycle {cycle}: after refine")
+        for step in range(4):
+            assert o.update() == 0, f"cycle {cycle} step {step}: oracle update failed"
+            ctx.mm_step(1)
+            assert ctx.mm_dt() == o.dt(), f"cycle {cycle} step {step}: dt"
+        assert_same(ctx.mm_state(), o.state(), f"cycle {cycle}: after steps")

@@ -125,7 +125,7 @@ def mm_bubble_shock_params(seed):
     return 1.5 + 1.5 * g.uniform()
 
 
-def mm_bubble_shock(cx, cy, h, seed, radius=0.15, centre=(0.5, 0.5), x_shock=0.2, p1=1.0e5, sub=4):
+def mm_bubble_shock(cx, cy, h, seed, radius=0.15, centre=(0.5, 0.5), x_shock=0.1, p1=1.0e5, sub=4):
     """Water at rest (rho 1000, P = p1) with an air bubble (rho 1.25) of the
     given radius; a planar shock of seeded Mach number at x = x_shock moves
     into the water along +x, the post-shock water state from the
