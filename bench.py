@@ -81,6 +81,13 @@ WORKLOADS = {
              "to 2 refinement levels by MmFluid::set_flags (mixed cells and cells next to another material) and "
              "the MmFluid split / merge distributor every 4 steps inside the timed region; initial adaptation in "
              "max_level + 1 rounds with the initial data re-sampled on each mesh"),
+    "mm512_bubble_second_order": dict(
+        kind="mm", nx=512, ny=512, steps=300, cfl=0.5, acc=2,
+        desc="the two-material shock-bubble of mm512_bubble_first_order with MmFluid's second order "
+             "(set_accuracy(2)): limited LSM gradients of every PState component (MC, zero in and next to mixed "
+             "cells), the half-step predictor on pure cells, and the corrector on face states extrapolated from "
+             "the half step and re-closed by the PT closure (interpolation_update); composite fluxes wherever an "
+             "extrapolated face state is mixed"),
     "ensemble256_muscl": dict(
         kind="ensemble", problems=256, smin=64, smax=512, acc=2, limiter="MC", steps=200,
         desc="256 independent problems of s x s cells, s ~ U{64..512} (seeded), per-problem seeded "
@@ -298,6 +305,26 @@ def mm_bytes_per_cell(n_mat, faces_per_cell=4):
     return 2 * state + 4 + 8 + faces_per_cell * (4 + 8 + 16)
 
 
+# ... second order, per cell: the gradient (own components in, both
+# gradient rows out, adjacency and LSM weights of four slots), the
+# predictor (own state and gradients in, face centres and areas, the half
+# state out) and the corrector (half state and gradients of the cell, the
+# old state, face geometry, the new state out); neighbours through the cache.
+def mm_grad_bytes_per_cell(n_mat, faces_per_cell=4):
+    k = 6 + 2 * n_mat
+    return 8 * k + 2 * 8 * k + 4 + 16 + faces_per_cell * (4 + 8)
+
+
+def mm_stage1_bytes_per_cell(n_mat, faces_per_cell=4):
+    state, k = 8 * (7 + 2 * n_mat), 6 + 2 * n_mat
+    return 2 * state + 2 * 8 * k + 4 + 24 + faces_per_cell * 24
+
+
+def mm_flux2_bytes_per_cell(n_mat, faces_per_cell=4):
+    state, k = 8 * (7 + 2 * n_mat), 6 + 2 * n_mat
+    return 3 * state + 2 * 8 * k + 4 + 24 + faces_per_cell * (4 + 24)
+
+
 # ... per composite-Riemann face (cell side): both states, code, adjacency,
 # area, normal, both volumes, the master material's interface normal of one
 # cell, the list entry, and the 4 + n_mat flux components written.
@@ -361,6 +388,8 @@ def mm_run(args, w, Z):
     ctx.mesh_rectangle(nx, ny)
     mats = [problems.WATER, problems.AIR]
     ctx.mm_configure(mats, w["cfl"])
+    acc = w.get("acc", 1)
+    ctx.mm_set_accuracy(acc)
     amr = "max_level" in w
     st = None
     if amr:
@@ -387,14 +416,16 @@ def mm_run(args, w, Z):
     z = ctx.mm_state()
     mixed = int(np.sum(np.sum(z["beta"] > 0.0, axis=0) > 1))
     alg_of = {"mm_flux1": mm_bytes_per_cell(2) * n, "mm_crp": mm_crp_bytes_per_face(2) * n_crp,
-              "mm_crp_mark": mm_crp_mark_bytes_per_cell(2) * n, "mm_interface": mm_interface_bytes_per_cell(2) * n}
+              "mm_crp_mark": mm_crp_mark_bytes_per_cell(2) * n, "mm_interface": mm_interface_bytes_per_cell(2) * n,
+              "mm_grad": mm_grad_bytes_per_cell(2) * n, "mm_stage1": mm_stage1_bytes_per_cell(2) * n,
+              "mm_flux2": mm_flux2_bytes_per_cell(2) * n}
     dom = max(prof, key=lambda k: prof[k][0])
     avg_ms = prof[dom][0] / prof[dom][1]
     alg = alg_of[dom]
     achieved = alg / (avg_ms * 1e-3) / 1e9
     out = {
-        "metric": "FP64 cell-updates/s (multi-material first order, HLLC + PLIC composite fluxes + PT closure"
-                  + (", AMR)" if amr else ")"),
+        "metric": f"FP64 cell-updates/s (multi-material {'first' if acc == 1 else 'second'} order, HLLC + PLIC "
+                  "composite fluxes + PT closure" + (", AMR)" if amr else ")"),
         "value": cells / el, "unit": "cell-updates/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": el * 1e3 / args.steps, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f64", "data": "synthetic",
@@ -424,6 +455,7 @@ def mm_cpu_baseline(w, st, mats, budget_s, seed):
     O.build()
     m = O.Mesh.rectangle(w["nx"], w["ny"])
     o = O.MmFluid(m, mats, w["cfl"])
+    o.set_accuracy(w.get("acc", 1))
     amr = "max_level" in w
     if amr:
         def centres():

@@ -246,6 +246,11 @@ int zfv_mm_interface(zfv_ctx* ctx, double* nx, double* ny, double* p);
 /* MmFluid::set_limiter (mm_fluid.cpp:45-47): limiter number as
  * zfv_limiter_from_name gives it (default MC). */
 int zfv_mm_set_limiter(zfv_ctx* ctx, int limiter);
+/* MmFluid::set_accuracy (mm_fluid.cpp:46-48): 1 (default) the first-order
+ * update; 2 compute_grad, fluxes_stage1 and fluxes_stage2 (mm_fluid.cpp:
+ * 153-168, 521-680) with PState::interpolation_update (models.cpp:381-391).
+ * Values outside [1, 2] are clamped. */
+int zfv_mm_set_accuracy(zfv_ctx* ctx, int acc);
 /* MmFluid::set_flags (mm_fluid.cpp:836-885): compute_grad (zero gradients
  * in and next to mixed cells with composite Riemann problems on, else the
  * limited LSM gradient; kept for zfv_mm_refine), then per cell +1 for a
@@ -254,12 +259,15 @@ int zfv_mm_set_limiter(zfv_ctx* ctx, int limiter);
 int zfv_mm_set_flags(zfv_ctx* ctx);
 /* EuMesh::refine (eu_mesh.cpp:290-359) with MmFluid::distributor
  * (mm_fluid.cpp:765-830): balance_flags, then amr::apply with the
- * multi-material split (from the gradients of zfv_mm_set_flags, which must
- * have run on the current mesh) and the conservative merge.  The
+ * multi-material split (from the gradients of the last compute_grad, which
+ * must have run on the current mesh) and the conservative merge.  The
  * single-material state, if any, does not follow the mesh. */
 int zfv_mm_refine(zfv_ctx* ctx, int max_level);
-/* The gradients of the last zfv_mm_set_flags: rows rho, u, v, e,
- * beta[n_mat], [4 + n_mat][n] each for x and y. */
+/* The gradients of the last compute_grad (zfv_mm_set_flags, or a
+ * second-order step: those of its old layer, as the reference's d_dx /
+ * d_dy): rows rho, u, v, e, beta[n_mat], P, T, rhos[n_mat], [6 + 2 n_mat][n]
+ * each for x and y (NaN where a true density's stencil meets a cell without
+ * that material). */
 int zfv_mm_grad(zfv_ctx* ctx, double* gx, double* gy);
 
 /* ---- launch configuration (no reference counterpart) --------------------- */

@@ -138,8 +138,9 @@ void orc_mm_closure(int n_mat, const double* mats, double rho, double e, const d
 void orc_mm_closure_rP(int n_mat, const double* mats, double rho, double P, const double* beta, double* out);
 void orc_mm_face_flux(const orc_mm* s, int cL, int cR, double nx, double ny, double* F);
 void orc_mm_set_limiter(orc_mm* s, int limiter);
+void orc_mm_set_accuracy(orc_mm* s, int acc);   /* 1 or 2 (clamped) */
 void orc_mm_compute_grad(orc_mm* s);
-void orc_mm_grad(const orc_mm* s, double* gx, double* gy);   /* [4 + n_mat][n]: rho, u, v, e, beta */
+void orc_mm_grad(const orc_mm* s, double* gx, double* gy);   /* [6 + 2 n_mat][n]: rho, u, v, e, beta, P, T, rhos */
 void orc_mm_set_flags(orc_mm* s);
 int orc_mm_refine(orc_mm* s, int max_level);   /* 1 changed, 0 unchanged, -1 a rejected state */
 double orc_quad_volume_fraction(double p, double nx, double ny, double A, double B);

@@ -836,19 +836,24 @@ inline MArr calc_crp_flux(const MState& zL, const MState& zR, double hL, double 
 using namespace orcmm;
 
 // =============================================================== MmFluid
-// Gradient of the PState components the split distributor reads: rho,
-// v.x, v.y, e, beta[i] (the reference differentiates all of them component
-// by component, gradient.h:226-282, 412-474, so a subset is the same).
-constexpr int kMG = 4 + kMaxMat;
-struct MG { double a[kMG] = {0, 0, 0, 0, 0, 0, 0, 0}; };
+// Gradient of the PState components (gradient.h:226-282, 412-474 work
+// component by component): rho, v.x, v.y, e, beta[i] (what the split
+// distributor reads), then P, T and the true densities rhos[i] (which the
+// second-order face extrapolation also carries: FaceExtra::Direct,
+// face_extra.h:122-184, moves every component).  v.z is zero throughout,
+// and so is its gradient.  A true density is NaN where its material is
+// absent, so its gradient is NaN wherever the stencil meets such a cell.
+constexpr int kMG = 6 + 2 * kMaxMat;
+struct MG { double a[kMG] = {}; };
 
 struct orc_mm {
     orc_mesh* m;
     Mixture mix;
     int crp = 1;   // CrpMode::PLIC, the MmFluid default (mm_fluid.cpp:18); 0 = CrpMode::NONE
     int limiter = ORC_LIM_MC;   // m_limiter = "MC" (mm_fluid.cpp:20)
+    int acc = 1;                // m_acc (mm_fluid.h:157), set_accuracy (mm_fluid.cpp:46-48)
     double cfl, dt, max_dt;
-    std::vector<MState> init, next;   // MmFluid::Parts init / next (mm_fluid.h:53-67)
+    std::vector<MState> init, half, next;   // MmFluid::Parts init / half / next (mm_fluid.h:53-67)
     std::vector<MG> d_dx, d_dy, d_dz; // Parts d_dx / d_dy / d_dz (components of MG)
     std::vector<double> nrm_x, nrm_y, pos;   // interface normals / positions [c][mat]
 };
@@ -861,7 +866,9 @@ inline double mg_comp(const MState& z, int k) {
         case 1: return z.v.x;
         case 2: return z.v.y;
         case 3: return z.e;
-        default: return z.beta.a[k - 4];
+        case 8: return z.P;
+        case 9: return z.T;
+        default: return k < 8 ? z.beta.a[k - 4] : z.rhos[k - 10];
     }
 }
 
@@ -975,6 +982,7 @@ struct MmData : AmrData {
     explicit MmData(orc_mm* s) : S(s) {}
     void resize(int n) override {
         S->init.resize(n);
+        S->half.resize(n);
         S->next.resize(n);
         S->d_dx.resize(n);
         S->d_dy.resize(n);
@@ -1064,6 +1072,66 @@ inline double cell_hy(const orc_mesh& m, int c) {
     return std::sqrt(sqnorm(d));
 }
 
+// FaceExtra::Direct(...).m / .p (face_extra.h:122-184): every component
+// moved by du = (g_x dr.x + g_y dr.y) + g_z dr.z of its own gradient.
+MState m_extrap(const MState& z, const MG& gx, const MG& gy, const MG& gz, const Vec3& dr) {
+    double d[kMG];
+    for (int k = 0; k < kMG; ++k) d[k] = (gx.a[k] * dr.x + gy.a[k] * dr.y) + gz.a[k] * dr.z;
+    MState r = z;
+    r.rho = z.rho + d[0];
+    r.v = {z.v.x + d[1], z.v.y + d[2], z.v.z};
+    r.e = z.e + d[3];
+    for (int i = 0; i < kMaxMat; ++i) r.beta.a[i] = z.beta.a[i] + d[4 + i];
+    r.P = z.P + d[8];
+    r.T = z.T + d[9];
+    for (int i = 0; i < kMaxMat; ++i) r.rhos[i] = z.rhos[i] + d[10 + i];
+    return r;
+}
+
+// PState::interpolation_update (models.cpp:381-391): normalise the mass
+// fractions, then e, T and the true densities from (rho, P) with the
+// extrapolated T and densities as the closure's starting point.
+void m_interpolation_update(MState& z, const Mixture& mx) {
+    z.beta.normalize();
+    MixOpt o;
+    o.T0 = z.T;
+    o.rhos = z.rhos;
+    double rh[kMaxMat], e, T;
+    get_reT(mx, z.rho, z.P, z.beta, o, rh, e, T);
+    z.e = e;
+    z.T = T;
+    for (int i = 0; i < kMaxMat; ++i) z.rhos[i] = rh[i];
+}
+
+// MmFluid::calc_flux (mm_fluid.cpp:347-371) with alpha_sigma for
+// CrpMode::PLIC (:228-256): face-frame states z_m / z_p of slot f of cell
+// c; the face-frame flux.  inx / iny: the interface normals of
+// interface_recovery, [mat][n].
+MArr mm_calc_flux(const orc_mm& S, int c, size_t f, const MState& z_m, const MState& z_p, double h_L, double h_R,
+                  double dt, const double* inx, const double* iny, int& crp_err) {
+    const orc_mesh& m = *S.m;
+    if (!S.crp || (z_m.beta.index() >= 0 && z_p.beta.index() >= 0)) return m_hllc(z_m, z_p, S.mix);
+    const int iA = master_component(z_m, z_p);
+    const double a_L = m_alpha(z_m, iA), a_R = m_alpha(z_p, iA);
+    const double vi = 0.5 * (z_m.v.x + z_p.v.x);
+    const bool upwind = vi > 0.0;
+    const double CFL = dt * std::abs(vi) / (upwind ? h_L : h_R);
+    const int cn = slot_is_boundary(m.bnd[f]) ? c : m.adj[f];   // face.neib() is the cell at a boundary
+    const size_t ki = (size_t)iA * m.n;
+    const Vec3 normal{m.fnx[f], m.fny[f], 0.0};
+    const Vec3 nc = upwind ? Vec3{inx[ki + c], iny[ki + c], 0.0} : Vec3{-inx[ki + cn], -iny[ki + cn], -0.0};
+    const double cs = dot(normal, nc);
+    const double a_sig = f_between(average_flux(upwind ? a_L : a_R, cs, CFL), a_L, a_R);
+    return calc_crp_flux(z_m, z_p, h_L, h_R, iA, a_sig, dt, S.mix, crp_err);
+}
+
+inline void m_to_global(MArr& loc, const Vec3& normal) {   // Flux::to_global
+    const Vec3 mg = rot_global(Vec3{loc.a[1], loc.a[2], loc.a[3]}, normal);
+    loc.a[1] = mg.x;
+    loc.a[2] = mg.y;
+    loc.a[3] = mg.z;
+}
+
 }  // namespace
 
 extern "C" {
@@ -1079,6 +1147,7 @@ orc_mm* orc_mm_create(orc_mesh* m, int n_mat, const double* mats, double cfl) {
     s->dt = 1.0e+300;
     s->max_dt = 1.0e+300;                          // mm_fluid.cpp:16-26
     s->init.resize(m->n);
+    s->half.resize(m->n);
     s->next.resize(m->n);
     s->d_dx.resize(m->n);
     s->d_dy.resize(m->n);
@@ -1087,20 +1156,22 @@ orc_mm* orc_mm_create(orc_mesh* m, int n_mat, const double* mats, double cfl) {
 }
 
 void orc_mm_set_limiter(orc_mm* s, int limiter) { s->limiter = limiter; }   // set_limiter (mm_fluid.cpp:45-47)
+void orc_mm_set_accuracy(orc_mm* s, int acc) { s->acc = std::min(std::max(1, acc), 2); }   // mm_fluid.cpp:46-48
 
 // MmFluid::compute_grad (mm_fluid.cpp:439-466) of the init layer.
 void orc_mm_compute_grad(orc_mm* s) {
     for (int c = 0; c < s->m->n; ++c) mm_grad_cell(*s, c, s->d_dx[c], s->d_dy[c], s->d_dz[c]);
 }
 
-// The stored gradients: rho, u, v, e, beta[n_mat] rows of n values each,
-// x then y.
+// The stored gradients: rho, u, v, e, beta[n_mat], P, T, rhos[n_mat]
+// rows of n values each, x then y.
 void orc_mm_grad(const orc_mm* s, double* gx, double* gy) {
-    const int n = s->m->n, K = 4 + s->mix.n;
+    const int n = s->m->n, nm = s->mix.n, K = 6 + 2 * nm;
     for (int c = 0; c < n; ++c)
         for (int k = 0; k < K; ++k) {
-            gx[(size_t)k * n + c] = s->d_dx[c].a[k];
-            gy[(size_t)k * n + c] = s->d_dy[c].a[k];
+            const int j = k < 4 + nm ? k : (k < 6 + nm ? 8 + (k - 4 - nm) : 10 + (k - 6 - nm));
+            gx[(size_t)k * n + c] = s->d_dx[c].a[j];
+            gy[(size_t)k * n + c] = s->d_dy[c].a[j];
         }
 }
 
@@ -1203,9 +1274,12 @@ void orc_mm_set_crp(orc_mm* s, int mode) { s->crp = mode; }
 // composite Riemann flux of the master material with the PLIC face
 // fraction, alpha_sigma :228-256), swap; returns 0, or 1 on a NaN flux, a
 // negative density or a CRP case the reference rejects (it throws).
+static int mm_update2(orc_mm* s);
+
 int orc_mm_update(orc_mm* s) {
     g_numeric_error = 0;
     orc_mm_compute_dt(s);
+    if (s->acc == 2) return mm_update2(s);
     const orc_mesh& m = *s->m;
     const double dt = s->dt;
     const int nm = s->mix.n;
@@ -1234,26 +1308,8 @@ int orc_mm_update(orc_mm* s) {
             const double S = m.area[f];
             const double h_L = V_c / S;
             const double h_R = (slot_is_boundary(b) ? V_c : m.vol[m.adj[f]]) / S;
-            MArr loc;
-            if (!s->crp || (z_m.beta.index() >= 0 && z_p.beta.index() >= 0)) {
-                loc = m_hllc(z_m, z_p, s->mix);
-            } else {
-                const int iA = master_component(z_m, z_p);
-                // alpha_sigma, CrpMode::PLIC (mm_fluid.cpp:241-254)
-                const double a_L = m_alpha(z_m, iA), a_R = m_alpha(z_p, iA);
-                const double vi = 0.5 * (z_m.v.x + z_p.v.x);
-                const bool upwind = vi > 0.0;
-                const double CFL = dt * std::abs(vi) / (upwind ? h_L : h_R);
-                const int cn = slot_is_boundary(b) ? c : m.adj[f];   // face.neib() is the cell at a boundary
-                const size_t ki = (size_t)iA * m.n;
-                const Vec3 nc = upwind ? Vec3{inx[ki + c], iny[ki + c], 0.0}
-                                       : Vec3{-inx[ki + cn], -iny[ki + cn], -0.0};
-                const double cs = dot(normal, nc);
-                const double a_sig = f_between(average_flux(upwind ? a_L : a_R, cs, CFL), a_L, a_R);
-                loc = calc_crp_flux(z_m, z_p, h_L, h_R, iA, a_sig, dt, s->mix, crp_err);
-            }
-            const Vec3 mg = rot_global(Vec3{loc.a[1], loc.a[2], loc.a[3]}, normal);   // Flux::to_global
-            loc.a[1] = mg.x; loc.a[2] = mg.y; loc.a[3] = mg.z;
+            MArr loc = mm_calc_flux(*s, c, f, z_m, z_p, h_L, h_R, dt, inx.data(), iny.data(), crp_err);
+            m_to_global(loc, normal);
             for (int q = 0; q < 9; ++q) flux.a[q] += loc.a[q] * S;
         }
         MArr q_c = m_qstate(z_c);
@@ -1263,6 +1319,93 @@ int orc_mm_update(orc_mm* s) {
         s->next[c] = m_from_q(q_c, s->mix, z_c.P, z_c.T, z_c.rhos);
     }
     std::swap(s->init, s->next);   // MmFluid::swap (mm_fluid.cpp:682-684)
+    return (g_numeric_error || neg || crp_err) ? 1 : 0;
+}
+
+// MmFluid::integrate with acc 2 (mm_fluid.cpp:153-168): compute_grad of
+// init, interface_recovery (PLIC), fluxes_stage1 (:521-578) into half and
+// fluxes_stage2 (:580-680) into next, then swap.
+static int mm_update2(orc_mm* s) {
+    const orc_mesh& m = *s->m;
+    const double dt = s->dt;
+    const int nm = s->mix.n;
+    orc_mm_compute_grad(s);
+    std::vector<double> inx, iny, ip;
+    if (s->crp) {
+        inx.resize((size_t)nm * m.n);
+        iny.resize((size_t)nm * m.n);
+        ip.resize((size_t)nm * m.n);
+        orc_mm_interface(s, inx.data(), iny.data(), ip.data());
+    }
+    // stage 1: mixed cells keep their state; a pure cell advances half a
+    // step with the physical flux of its own extrapolated face states
+    for (int c = 0; c < m.n; ++c) {
+        const MState z_c = s->init[c];
+        if (z_c.beta.index() < 0) {
+            s->half[c] = z_c;
+            continue;
+        }
+        const Vec3 cell_c{m.cx[c], m.cy[c], 0.0};
+        MArr flux;
+        for (int k = 0; k < 8; ++k) {
+            const size_t f = 8 * (size_t)c + k;
+            if (m.bnd[f] == ORC_UNDEFINED) continue;
+            const Vec3 normal{m.fnx[f], m.fny[f], 0.0};
+            const Vec3 face_c{m.fcx[f], m.fcy[f], 0.0};
+            MState z_m = m_extrap(z_c, s->d_dx[c], s->d_dy[c], s->d_dz[c], face_c - cell_c);
+            m_interpolation_update(z_m, s->mix);
+            MArr loc = m_flux(m_in_local(z_m, normal));
+            m_to_global(loc, normal);
+            for (int q = 0; q < 9; ++q) flux.a[q] += loc.a[q] * m.area[f];
+        }
+        MArr q_c = m_qstate(z_c);
+        const double coef = 0.5 * dt / m.vol[c];
+        for (int q = 0; q < 9; ++q) q_c.a[q] -= coef * flux.a[q];
+        s->half[c] = m_from_q(q_c, s->mix, z_c.P, z_c.T, z_c.rhos);
+    }
+    // stage 2: face states extrapolated from the half states with the
+    // gradients of init; a periodic face's neighbour centre is the
+    // neighbour's own (face.is_boundary() is false there, :599-604)
+    int neg = 0, crp_err = 0;
+    for (int c = 0; c < m.n; ++c) {
+        const Vec3 cell_c{m.cx[c], m.cy[c], 0.0};
+        const double V_c = m.vol[c];
+        const MState z_c = s->init[c];
+        const MState z_ch = s->half[c];
+        MArr flux;
+        for (int k = 0; k < 8; ++k) {
+            const size_t f = 8 * (size_t)c + k;
+            const int b = m.bnd[f];
+            if (b == ORC_UNDEFINED) continue;
+            const Vec3 normal{m.fnx[f], m.fny[f], 0.0};
+            const Vec3 face_c{m.fcx[f], m.fcy[f], 0.0};
+            const bool bound = slot_is_boundary(b);
+            const int nb = m.adj[f];   // the cell itself on boundary slots
+            MState z_m = m_extrap(z_ch, s->d_dx[c], s->d_dy[c], s->d_dz[c], face_c - cell_c);
+            m_interpolation_update(z_m, s->mix);
+            MState z_p;
+            double V_n = V_c;
+            if (!bound) {
+                const Vec3 neib_c{m.cx[nb], m.cy[nb], 0.0};
+                V_n = m.vol[nb];
+                z_p = m_extrap(s->half[nb], s->d_dx[nb], s->d_dy[nb], s->d_dz[nb], face_c - neib_c);
+                m_interpolation_update(z_p, s->mix);
+            } else {
+                z_p = m_boundary_value(z_m, normal, b);
+            }
+            const double S = m.area[f];
+            MArr loc = mm_calc_flux(*s, c, f, m_in_local(z_m, normal), m_in_local(z_p, normal), V_c / S, V_n / S, dt,
+                                    inx.data(), iny.data(), crp_err);
+            m_to_global(loc, normal);
+            for (int q = 0; q < 9; ++q) flux.a[q] += loc.a[q] * S;
+        }
+        MArr q_c = m_qstate(z_c);
+        const double coef = dt / V_c;
+        for (int q = 0; q < 9; ++q) q_c.a[q] -= coef * flux.a[q];
+        if (q_c.a[0] < 0.0) neg = 1;   // :671-673 throws
+        s->next[c] = m_from_q(q_c, s->mix, z_c.P, z_c.T, z_c.rhos);
+    }
+    std::swap(s->init, s->next);
     return (g_numeric_error || neg || crp_err) ? 1 : 0;
 }
 

@@ -86,6 +86,7 @@ def lib():
             "orc_mm_closure_rP": (None, [_I, _dp, _D, _D, _dp, _dp]),
             "orc_mm_face_flux": (None, [_P, _I, _I, _D, _D, _dp]),
             "orc_mm_set_limiter": (None, [_P, _I]),
+            "orc_mm_set_accuracy": (None, [_P, _I]),
             "orc_mm_compute_grad": (None, [_P]),
             "orc_mm_grad": (None, [_P, _dp, _dp]),
             "orc_mm_set_flags": (None, [_P]),
@@ -297,8 +298,9 @@ def _mats(materials):
 
 
 class MmFluid:
-    """Oracle MmFluid (math/solver/mm_fluid.h:45-163 restated, first order,
-    no composite Riemann problems); materials: (gamma, P0, e0, Cv, T0, rho_ref)."""
+    """Oracle MmFluid (math/solver/mm_fluid.h:45-163 restated: first or
+    second order, CrpMode PLIC or NONE); materials: (gamma, P0, e0, Cv, T0,
+    rho_ref)."""
 
     def __init__(self, mesh, materials, cfl=0.7, crp=True):
         self.mesh = mesh
@@ -355,9 +357,14 @@ class MmFluid:
     def compute_grad(self):
         lib().orc_mm_compute_grad(self.h)
 
+    def set_accuracy(self, acc):
+        """1: the first-order update; 2: compute_grad + fluxes_stage1/2."""
+        lib().orc_mm_set_accuracy(self.h, int(acc))
+
     def grad(self):
-        """Stored gradients (gx, gy), rows rho, u, v, e, beta[n_mat]."""
-        K, n = 4 + self.nm, self.mesh.n
+        """Stored gradients (gx, gy), rows rho, u, v, e, beta[n_mat], P, T,
+        rhos[n_mat] (NaN where the stencil meets an absent material)."""
+        K, n = 6 + 2 * self.nm, self.mesh.n
         gx, gy = np.zeros((K, n)), np.zeros((K, n))
         lib().orc_mm_grad(self.h, _d(gx), _d(gy))
         return gx, gy

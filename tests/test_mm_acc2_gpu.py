@@ -1,0 +1,118 @@
+"""Device second-order multi-material step (zfv_mm_set_accuracy(2):
+MmFluid::integrate with acc 2 — compute_grad, interface_recovery,
+fluxes_stage1 / fluxes_stage2 with PState::interpolation_update) against
+the oracle on the same inputs.
+
+Bar: bit-exact state (NaN where a material is absent), dt and gradients
+after every step, with and without composite Riemann problems, on uniform
+and adapted meshes.  Power-of-two bases on the unit square keep the LSM
+weights exact on both sides (see test_mm_amr_gpu).
+"""
+import numpy as np
+import pytest
+
+from test_mm_amr_gpu import bubble, compare_mesh
+from test_mm_gpu import AIR, GAS, HEAVY, WATER, assert_same
+
+pytestmark = pytest.mark.gpu
+
+THIRD = (1.67, 0.0, 0.0, 1.5, 0.0, 0.5)
+
+
+def acc2_pair(oracle, zfv, nx, mats, crp, bounds, limiter="MC", seed=5):
+    m = oracle.Mesh.rectangle(nx, nx, bounds=bounds)
+    o = oracle.MmFluid(m, mats, 0.5, crp=crp)
+    o.set_accuracy(2)
+    o.set_limiter(limiter)
+    ctx = zfv.Context()
+    ctx.mesh_rectangle(nx, nx, bounds=bounds)
+    ctx.mm_configure(mats, 0.5, crp=crp)
+    ctx.mm_set_accuracy(2)
+    ctx.mm_set_limiter(limiter)
+    cl = m.cells()
+    st = list(bubble(cl["cx"], cl["cy"], mats[:2], seed))
+    if len(mats) == 3:   # a third material in a band
+        b = st[4]
+        band = np.clip(1.0 - np.abs(cl["cy"] - 0.2) / 0.08, 0.0, 1.0)
+        st[4] = np.stack([b[0] * (1.0 - band), b[1] * (1.0 - band), band])
+    o.set_state(*st)
+    ctx.mm_set_state(*st)
+    return m, o, ctx
+
+
+def compare_grad(ctx, o, what):
+    gx, gy = ctx.mm_grad()
+    ox, oy = o.grad()
+    for k in range(gx.shape[0]):
+        assert np.array_equal(gx[k], ox[k], equal_nan=True), f"{what}: gradient x, row {k}"
+        assert np.array_equal(gy[k], oy[k], equal_nan=True), f"{what}: gradient y, row {k}"
+
+
+@pytest.mark.parametrize("mats,bounds,crp,limiter", [
+    ([GAS, HEAVY], (2, 2, 2, 2), True, "MC"),
+    ([GAS, HEAVY], (2, 2, 2, 2), False, "MC"),
+    ([GAS, HEAVY], (1, 1, 1, 1), True, "minmod"),
+    ([WATER, AIR], (1, 2, 2, 1), True, "MC"),
+    ([GAS, HEAVY, THIRD], (2, 1, 2, 1), True, "vanLeer"),
+])
+def test_second_order_steps_match_oracle(oracle, zfv, mats, bounds, crp, limiter):
+    m, o, ctx = acc2_pair(oracle, zfv, 32, mats, crp, bounds, limiter)
+    crp_faces = 0
+    for step in range(8):
+        assert o.update() == 0, f"step {step}: the oracle rejected a state"
+        ctx.mm_step(1)
+        assert ctx.mm_dt() == o.dt(), f"step {step}: dt"
+        assert_same(ctx.mm_state(), o.state(), f"step {step}")
+        compare_grad(ctx, o, f"step {step}")
+        crp_faces += ctx.mm_crp_faces()
+    assert (crp_faces > 0) == crp
+
+
+def test_second_order_differs_from_first_order(zfv):
+    res = []
+    for acc in (1, 2):
+        ctx = zfv.Context()
+        ctx.mesh_rectangle(32, 32)
+        ctx.mm_configure([GAS, HEAVY], 0.5)
+        ctx.mm_set_accuracy(acc)
+        cells, _, _ = ctx.mesh_download()
+        ctx.mm_set_state(*bubble(cells["cx"], cells["cy"], [GAS, HEAVY], 5))
+        ctx.mm_step(5)
+        res.append(ctx.mm_state()["rho"])
+    assert not np.allclose(res[0], res[1], rtol=1e-6, atol=0)
+
+
+@pytest.mark.parametrize("mats,crp", [([GAS, HEAVY], True), ([WATER, AIR], True), ([GAS, HEAVY], False)])
+def test_second_order_with_adaptation_matches_oracle(oracle, zfv, mats, crp):
+    m, o, ctx = acc2_pair(oracle, zfv, 32, mats, crp, (2, 2, 2, 2))
+    for cycle in range(4):
+        o.set_flags()
+        ctx.mm_set_flags()
+        assert np.array_equal(ctx.flags(), m.cells()["flag"]), f"cycle {cycle}: flags"
+        assert o.refine(2) >= 0, "the oracle rejected a state"
+        ctx.mm_refine(2)
+        compare_mesh(ctx, m, f"cycle {cycle}")
+        assert_same(ctx.mm_state(), o.state(), f"cycle {cycle}: after refine")
+        for step in range(3):
+            assert o.update() == 0, f"cycle {cycle} step {step}: the oracle rejected a state"
+            ctx.mm_step(1)
+            assert ctx.mm_dt() == o.dt(), f"cycle {cycle} step {step}: dt"
+        assert_same(ctx.mm_state(), o.state(), f"cycle {cycle}: after steps")
+        compare_grad(ctx, o, f"cycle {cycle}")
+    assert np.max(m.cells()["level"]) == 2
+
+
+def test_refine_after_second_order_steps_uses_their_gradients(oracle, zfv):
+    """Without set_flags in between, the split reads the gradients of the
+    last step's old layer on both sides (the reference's d_dx / d_dy)."""
+    m, o, ctx = acc2_pair(oracle, zfv, 32, [GAS, HEAVY], True, (2, 2, 2, 2))
+    for _ in range(2):
+        assert o.update() == 0
+        ctx.mm_step(1)
+    flags = np.where(np.arange(m.n) % 5 == 0, 1, 0).astype(np.int32)
+    m.set_flags(flags)
+    ctx.set_flags_array(flags)
+    assert o.refine(1) >= 0
+    ctx.mm_refine(1)
+    compare_mesh(ctx, m, "after refine")
+    assert_same(ctx.mm_state(), o.state(), "after refine")

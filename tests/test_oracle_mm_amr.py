@@ -83,7 +83,14 @@ def test_gradient_of_a_linear_field_and_the_zero_rule(oracle):
     np.testing.assert_allclose(gy[0][inner], -0.125, rtol=0, atol=1e-13)
     np.testing.assert_allclose(gy[1][inner], 0.05, rtol=0, atol=1e-13)
     np.testing.assert_allclose(gx[2][inner], -0.05, rtol=0, atol=1e-13)
-    assert np.all(gx[4:] == 0.0) and np.all(gy[4:] == 0.0)   # pure: beta is constant
+    assert np.all(gx[4:6] == 0.0) and np.all(gy[4:6] == 0.0)   # pure: beta is constant
+    # rows 6, 7: P = 1 + 0.1 x, T (nonlinear); rows 8, 9: the true density of
+    # material 0 equals rho, material 1 is absent (NaN)
+    np.testing.assert_allclose(gx[6][inner], 0.1, rtol=0, atol=1e-13)
+    np.testing.assert_allclose(gy[6][inner], 0.0, rtol=0, atol=1e-13)
+    np.testing.assert_array_equal(gx[8], gx[0])
+    np.testing.assert_array_equal(gy[8], gy[0])
+    assert np.all(np.isnan(gx[9])) and np.all(np.isnan(gy[9]))
     # two materials: zero in and next to mixed cells only with CRP on
     for crp in (True, False):
         m2, s2 = _setup(oracle, 24, [GAS, HEAVY], crp=crp)

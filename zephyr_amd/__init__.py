@@ -32,7 +32,7 @@ EXPORTS = [
     "zfv_mesh_n_faces", "zfv_set_launch", "zfv_get_launch", "zfv_autotune",
     "zfv_batch_rectangles", "zfv_batch_layout", "zfv_batch_dt", "zfv_graph_enable",
     "zfv_set_cell_order", "zfv_get_cell_order",
-    "zfv_mm_set_limiter", "zfv_mm_set_flags", "zfv_mm_refine", "zfv_mm_grad",
+    "zfv_mm_set_limiter", "zfv_mm_set_accuracy", "zfv_mm_set_flags", "zfv_mm_refine", "zfv_mm_grad",
     "zfv_adjoint_forward", "zfv_adjoint_backward",
     "zfv_mm_configure", "zfv_mm_set_crp", "zfv_mm_set_max_dt", "zfv_mm_set_state", "zfv_mm_state", "zfv_mm_step", "zfv_mm_dt", "zfv_mm_crp_faces",
     "zfv_mm_interface",
@@ -108,6 +108,7 @@ def lib():
         L.zfv_mm_dt.argtypes = [vp, vp]
         L.zfv_mm_interface.argtypes = [vp] * 4
         L.zfv_mm_set_limiter.argtypes = [vp, C.c_int]
+        L.zfv_mm_set_accuracy.argtypes = [vp, C.c_int]
         L.zfv_mm_set_flags.argtypes = [vp]
         L.zfv_mm_refine.argtypes = [vp, C.c_int]
         L.zfv_mm_grad.argtypes = [vp, vp, vp]
@@ -374,6 +375,11 @@ class Context:
         lim = limiter if isinstance(limiter, int) else lib().zfv_limiter_from_name(limiter.encode())
         _check(lib().zfv_mm_set_limiter(self.h, lim))
 
+    def mm_set_accuracy(self, acc):
+        """1: the first-order update (default); 2: MmFluid's second order
+        (compute_grad, fluxes_stage1 / fluxes_stage2)."""
+        _check(lib().zfv_mm_set_accuracy(self.h, int(acc)))
+
     def mm_set_flags(self):
         """MmFluid::set_flags into the mesh's flags (and the gradients kept for mm_refine)."""
         _check(lib().zfv_mm_set_flags(self.h))
@@ -386,8 +392,9 @@ class Context:
         self.n = n.value
 
     def mm_grad(self):
-        """Gradients of the last mm_set_flags: (gx, gy), rows rho, u, v, e, beta[n_mat]."""
-        K, n = 4 + self._mm_nmat, self.n
+        """Gradients of the last compute_grad (mm_set_flags or a second-order
+        step): (gx, gy), rows rho, u, v, e, beta[n_mat], P, T, rhos[n_mat]."""
+        K, n = 6 + 2 * self._mm_nmat, self.n
         gx, gy = np.zeros((K, n)), np.zeros((K, n))
         _check(lib().zfv_mm_grad(self.h, _ptr(gx), _ptr(gy)))
         return gx, gy

@@ -159,8 +159,9 @@ struct zfv_ctx {
         DBuf<double> lam[2], lz, W, lG, ZB, NG, dtb;
     } adjoint;
 
-    // multi-material solver (MmFluid, first order, no CRP): two state layers
-    // of (7 + 2 n_mat) rows of cap doubles, its own dt slots and step count
+    // multi-material solver (MmFluid, first or second order, CrpMode PLIC or
+    // NONE): two state layers of (7 + 2 n_mat) rows of cap doubles (a third,
+    // the half step, for acc 2), its own dt slots and step count
     struct MmSolver {
         bool configured = false, have_state = false;
         long long epoch = -1;   // mesh epoch of the state
@@ -169,10 +170,11 @@ struct zfv_ctx {
         double cfl = 0.7, max_dt = 1.0e300;   // MmFluid defaults (mm_fluid.cpp:16-26)
         int cur = 0;
         long long step_no = 0;
-        DBuf<double> st[2], dmax_dt, dt_out, inter, cf;
-        DBuf<double> gx, gy;          // limited gradients of zfv_mm_set_flags, [4 + n_mat][cap]
+        DBuf<double> st[2], half, dmax_dt, dt_out, inter, cf;
+        DBuf<double> gx, gy;          // limited gradients (compute_grad), [6 + 2 n_mat][cap]
         long long grad_epoch = -1;    // mesh epoch of gx / gy
         int limiter = 3;              // MC (mm_fluid.cpp:20)
+        int acc = 1;                  // m_acc (mm_fluid.h:157)
         DBuf<int> crp_list;
         DBuf<unsigned int> crp_count;
         DBuf<unsigned long long> dtmin;
@@ -492,10 +494,10 @@ int grow_capacity(zfv_ctx* ctx, int new_cap) {
     auto& M = ctx->mm;   // the multi-material rows keep their [rows][cap] layout
     if (M.configured) {
         const int nm = M.mix.n;
-        for (auto& b : M.st)
-            if (b.p) ZFV_HIP(grow_rows(b, 7 + 2 * nm, oc, new_cap, n, s));
+        for (auto* b : {&M.st[0], &M.st[1], &M.half})
+            if (b->p) ZFV_HIP(grow_rows(*b, 7 + 2 * nm, oc, new_cap, n, s));
         for (auto* b : {&M.gx, &M.gy})
-            if (b->p) ZFV_HIP(grow_rows(*b, 4 + nm, oc, new_cap, n, s));
+            if (b->p) ZFV_HIP(grow_rows(*b, 6 + 2 * nm, oc, new_cap, n, s));
     }
     ZFV_HIP(ctx->partial.alloc(4 * (size_t)((new_cap + kBlock - 1) / kBlock)));
     ctx->cap = new_cap;
@@ -1760,13 +1762,14 @@ int zfv_adjoint_backward(zfv_ctx* ctx, double* lr, double* lu, double* lv, doubl
 }
 
 // ---- multi-material solver ----------------------------------------------------
-static MmLayer mm_layer(zfv_ctx* ctx, int i) {
+static MmLayer mm_rows(zfv_ctx* ctx, double* p) {
     const size_t cap = (size_t)ctx->cap;
-    double* p = ctx->mm.st[i].p;
     const int nm = ctx->mm.mix.n;
     return MmLayer{p, p + cap, p + 2 * cap, p + 3 * cap, p + 4 * cap, p + 5 * cap, p + 6 * cap, p + 7 * cap,
                    p + (7 + (size_t)nm) * cap};
 }
+
+static MmLayer mm_layer(zfv_ctx* ctx, int i) { return mm_rows(ctx, ctx->mm.st[i].p); }
 
 static int mm_ready(zfv_ctx* ctx, bool state) {
     int r = need_ready(ctx, false, false);
@@ -1877,6 +1880,14 @@ int zfv_mm_state(zfv_ctx* ctx, double* rho, double* u, double* v, double* P, dou
     return read_err(ctx);
 }
 
+static int mm_grad_alloc(zfv_ctx* ctx) {
+    auto& S = ctx->mm;
+    const size_t rows = 6 + 2 * (size_t)S.mix.n, cap = (size_t)ctx->cap;
+    for (auto* b : {&S.gx, &S.gy})
+        if (b->n != rows * cap) ZFV_HIP(b->alloc(rows * cap));
+    return ZFV_OK;
+}
+
 int zfv_mm_step(zfv_ctx* ctx, int n_steps) {
     int r = mm_ready(ctx, true);
     if (r) return r;
@@ -1884,6 +1895,13 @@ int zfv_mm_step(zfv_ctx* ctx, int n_steps) {
     auto& S = ctx->mm;
     const DevMesh M = ctx->dmesh();
     const size_t cap = (size_t)ctx->cap, nm = (size_t)S.mix.n;
+    const int acc = S.acc;
+    if (acc == 2) {
+        r = mm_grad_alloc(ctx);
+        if (r) return r;
+        const size_t rows = 7 + 2 * nm;
+        if (S.half.n != rows * cap) ZFV_HIP(S.half.alloc(rows * cap));
+    }
     if (S.crp) {
         if (S.inter.n != 3 * nm * cap) ZFV_HIP(S.inter.alloc(3 * nm * cap));
         if (S.cf.n != 8 * (4 + nm) * cap) ZFV_HIP(S.cf.alloc(8 * (4 + nm) * cap));
@@ -1901,24 +1919,46 @@ int zfv_mm_step(zfv_ctx* ctx, int n_steps) {
         c.dt_out = S.dt_out.p;
         c.err = ctx->err.p;
         const MmLayer in = mm_layer(ctx, S.cur), out = mm_layer(ctx, 1 - S.cur);
+        const MmLayer half = acc == 2 ? mm_rows(ctx, S.half.p) : in;
         const double* CF = nullptr;
-        if (S.crp) {   // interface_recovery, then the composite fluxes (mm_fluid.cpp:130-145)
-            double* o = S.inter.p;
+        if (acc == 2) {   // compute_grad of init (mm_fluid.cpp:154)
+            ZFV_HIP(timed(ctx, "mm_grad", [&] {
+                return mm_grad(M, in, S.mix, S.crp, S.limiter, S.gx.p, S.gy.p, ctx->stream);
+            }));
+        }
+        double* o = S.inter.p;
+        if (S.crp) {   // interface_recovery (mm_fluid.cpp:130-137, 160-163)
             ZFV_HIP(timed(ctx, "mm_interface", [&] {
                 return mm_interface(M, in, S.mix, ctx->vx.p, ctx->vy.p, o, o + nm * cap, o + 2 * nm * cap, ctx->stream);
             }));
+        }
+        if (acc == 2) {   // fluxes_stage1 (:165)
+            ZFV_HIP(timed(ctx, "mm_stage1", [&] {
+                return mm_stage1(M, in, half, S.mix, c, S.gx.p, S.gy.p, ctx->stream);
+            }));
+        }
+        if (S.crp) {   // the composite fluxes of the step's faces
             ZFV_HIP(timed(ctx, "mm_crp_mark", [&] {
-                return mm_crp_mark(M, in, S.mix, S.crp_list.p, S.crp_count.p, ctx->stream);
+                return mm_crp_mark(M, in, half, S.mix, acc, S.gx.p, S.gy.p, S.crp_list.p, S.crp_count.p, ctx->stream);
             }));
             ZFV_HIP(timed(ctx, "mm_crp", [&] {
-                return mm_crp(M, in, S.mix, c, o, o + nm * cap, S.crp_list.p, S.crp_count.p, S.cf.p, ctx->stream);
+                return mm_crp(M, in, half, S.mix, acc, S.gx.p, S.gy.p, c, o, o + nm * cap, S.crp_list.p, S.crp_count.p,
+                              S.cf.p, ctx->stream);
             }));
             CF = S.cf.p;
         }
-        ZFV_HIP(timed(ctx, "mm_flux1", [&] { return mm_flux1(M, in, out, S.mix, c, CF, ctx->stream); }));
+        if (acc == 2)   // fluxes_stage2 (:167)
+            ZFV_HIP(timed(ctx, "mm_flux2", [&] {
+                return mm_flux2(M, in, half, out, S.mix, c, S.gx.p, S.gy.p, CF, ctx->stream);
+            }));
+        else
+            ZFV_HIP(timed(ctx, "mm_flux1", [&] { return mm_flux1(M, in, out, S.mix, c, CF, ctx->stream); }));
         S.cur = 1 - S.cur;
         S.step_no += 1;
     }
+    // as the reference's d_dx / d_dy, the gradients stay those of the last
+    // compute_grad: of the last step's old layer with acc 2
+    if (acc == 2 && n_steps > 0) S.grad_epoch = ctx->mesh_epoch;
     return ZFV_OK;
 }
 
@@ -1966,6 +2006,13 @@ int zfv_mm_interface(zfv_ctx* ctx, double* nx, double* ny, double* p) {
     return read_err(ctx);
 }
 
+int zfv_mm_set_accuracy(zfv_ctx* ctx, int acc) {
+    int r = check_ctx(ctx);
+    if (r) return r;
+    ctx->mm.acc = std::min(std::max(1, acc), 2);   // set_accuracy (mm_fluid.cpp:46-48)
+    return ZFV_OK;
+}
+
 int zfv_mm_set_limiter(zfv_ctx* ctx, int limiter) {
     int r = check_ctx(ctx);
     if (r) return r;
@@ -1981,9 +2028,8 @@ int zfv_mm_set_flags(zfv_ctx* ctx) {
     int r = mm_ready(ctx, true);
     if (r) return r;
     auto& S = ctx->mm;
-    const size_t rows = 4 + (size_t)S.mix.n, cap = (size_t)ctx->cap;
-    for (auto* b : {&S.gx, &S.gy})
-        if (b->n != rows * cap) ZFV_HIP(b->alloc(rows * cap));
+    r = mm_grad_alloc(ctx);
+    if (r) return r;
     const DevMesh M = ctx->dmesh();
     const MmLayer L = mm_layer(ctx, S.cur);
     ZFV_HIP(mm_grad(M, L, S.mix, S.crp, S.limiter, S.gx.p, S.gy.p, ctx->stream));
@@ -2001,7 +2047,7 @@ int zfv_mm_grad(zfv_ctx* ctx, double* gx, double* gy) {
     if (!gx || !gy) return fail(ZFV_ERR_ARG, "mm_grad: null array");
     auto& S = ctx->mm;
     if (S.grad_epoch != ctx->mesh_epoch) return fail(ZFV_ERR_STATE, "no multi-material gradients for this mesh");
-    const int n = ctx->hm.n, rows = 4 + S.mix.n;
+    const int n = ctx->hm.n, rows = 6 + 2 * S.mix.n;
     const size_t row = (size_t)n * sizeof(double), pitch = (size_t)ctx->cap * sizeof(double);
     ZFV_HIP(hipMemcpy2DAsync(gx, row, S.gx.p, pitch, row, rows, hipMemcpyDeviceToHost, ctx->stream));
     ZFV_HIP(hipMemcpy2DAsync(gy, row, S.gy.p, pitch, row, rows, hipMemcpyDeviceToHost, ctx->stream));

@@ -197,10 +197,18 @@ hipError_t mm_flux1(const DevMesh& M, const MmLayer& in, const MmLayer& out, con
                     const StepCtl& ctl, const double* CF, hipStream_t s);
 // The faces that take composite Riemann fluxes: list (8 * stride ints of
 // work space) and their number (one counter), then the fluxes into CF.
-hipError_t mm_crp_mark(const DevMesh& M, const MmLayer& in, const mm::Mix& mx, int* list, unsigned int* count,
-                       hipStream_t s);
-hipError_t mm_crp(const DevMesh& M, const MmLayer& in, const mm::Mix& mx, const StepCtl& ctl, const double* nx,
-                  const double* ny, const int* list, const unsigned int* count, double* CF, hipStream_t s);
+// acc 1 reads the cells of `in`; acc 2 the face states extrapolated from
+// `half` with the gradients gx / gy ([6 + 2 n_mat][stride], k_mm_grad).
+hipError_t mm_crp_mark(const DevMesh& M, const MmLayer& in, const MmLayer& half, const mm::Mix& mx, int acc,
+                       const double* gx, const double* gy, int* list, unsigned int* count, hipStream_t s);
+hipError_t mm_crp(const DevMesh& M, const MmLayer& in, const MmLayer& half, const mm::Mix& mx, int acc,
+                  const double* gx, const double* gy, const StepCtl& ctl, const double* nx, const double* ny,
+                  const int* list, const unsigned int* count, double* CF, hipStream_t s);
+// Second order: the predictor into `half`, then the corrector into `out`.
+hipError_t mm_stage1(const DevMesh& M, const MmLayer& in, const MmLayer& half, const mm::Mix& mx, const StepCtl& ctl,
+                     const double* gx, const double* gy, hipStream_t s);
+hipError_t mm_flux2(const DevMesh& M, const MmLayer& in, const MmLayer& half, const MmLayer& out, const mm::Mix& mx,
+                    const StepCtl& ctl, const double* gx, const double* gy, const double* CF, hipStream_t s);
 hipError_t mm_interface(const DevMesh& M, const MmLayer& L, const mm::Mix& mx, const double* vx, const double* vy,
                         double* nx, double* ny, double* p, hipStream_t s);
 // Adaptation of the multi-material state: the limited gradient (rows rho,

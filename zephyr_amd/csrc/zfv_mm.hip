@@ -79,6 +79,131 @@ __device__ __forceinline__ void rotate_uv(const Frame& f, MZ<NM>& z) {
     z.v = w.v;
 }
 
+// The new state from the conserved vector q: PState(q, mixture, P0, T0,
+// rhos0) (models.cpp:187-201) with the cell's previous P, T and true
+// densities as the closure's starting point.  The sound speed is left to
+// the caller.
+template <int NM>
+__device__ __forceinline__ MZ<NM> mm_from_q(const Mix& mx, const double* q, const MZ<NM>& z0) {
+    MZ<NM> zn;
+    zn.r = q[0];
+    zn.u = q[1] / zn.r;
+    zn.v = q[2] / zn.r;
+    zn.e = q[3] / zn.r - 0.5 * (zn.u * zn.u + zn.v * zn.v);
+#pragma unroll
+    for (int i = 0; i < NM; ++i) zn.b[i] = q[4 + i] / zn.r;
+    frac_cutoff<NM>(zn.b, 1.0e-12);
+    frac_cutoff<NM>(zn.b, kFracMinimal);
+    get_rPT<NM>(mx, zn.r, zn.e, zn.b, z0.P, z0.T, z0.rh, zn.rh, zn.P, zn.T);
+    zn.c = 0.0;
+    return zn;
+}
+
+// ---- second order (acc 2) ----------------------------------------------------
+// Gradient rows of k_mm_grad: rho, u, v, e, beta[NM], P, T, rhos[NM].
+template <int NM>
+__device__ __forceinline__ double mg_du(const double* gx, const double* gy, int k, int c, int st, double dx,
+                                        double dy) {
+    return gx[k * st + c] * dx + gy[k * st + c] * dy;
+}
+
+// FaceExtra::Direct m() / p() (face_extra.h:122-184) of the half state of
+// cell c: every component moved by its gradient along (dx, dy), then
+// PState::interpolation_update (models.cpp:381-391): mass fractions
+// normalised, e, T and the true densities re-closed from (rho, P) starting
+// from the extrapolated T and densities.  The extrapolated e is overwritten
+// there, so it is not formed.  (The reference's g_z dr.z term is a signed
+// zero here: dr.z = 0.)  With `sound`, also the sound speed the fluxes use
+// (sound_speed_rP from the re-closed T and densities).
+template <int NM>
+__device__ __forceinline__ MZ<NM> mm_face_state(const MmLayer& H, const Mix& mx, const double* gx, const double* gy,
+                                                int c, int st, double dx, double dy, bool sound) {
+    MZ<NM> z;
+    z.r = H.r[c] + mg_du<NM>(gx, gy, 0, c, st, dx, dy);
+    z.u = H.u[c] + mg_du<NM>(gx, gy, 1, c, st, dx, dy);
+    z.v = H.v[c] + mg_du<NM>(gx, gy, 2, c, st, dx, dy);
+    z.P = H.P[c] + mg_du<NM>(gx, gy, 4 + NM, c, st, dx, dy);
+    const double T0 = H.T[c] + mg_du<NM>(gx, gy, 5 + NM, c, st, dx, dy);
+    double rh0[NM];
+#pragma unroll
+    for (int i = 0; i < NM; ++i) {
+        z.b[i] = H.b[i * st + c] + mg_du<NM>(gx, gy, 4 + i, c, st, dx, dy);
+        rh0[i] = H.rh[i * st + c] + mg_du<NM>(gx, gy, 6 + NM + i, c, st, dx, dy);
+    }
+    frac_cutoff<NM>(z.b, kFracMinimal);
+    get_reT<NM>(mx, z.r, z.P, z.b, T0, rh0, z.rh, z.e, z.T);
+    z.c = sound ? sound_speed<NM>(mx, z.r, z.P, z.b, z.T, z.rh) : 0.0;
+    return z;
+}
+
+// Material index of the face state of mm_face_state (only its mass
+// fractions: the same sums and normalisation).
+template <int NM>
+__device__ __forceinline__ int mm_face_index(const MmLayer& H, const double* gx, const double* gy, int c, int st,
+                                             double dx, double dy) {
+    double b[NM];
+#pragma unroll
+    for (int i = 0; i < NM; ++i) b[i] = H.b[i * st + c] + mg_du<NM>(gx, gy, 4 + i, c, st, dx, dy);
+    frac_cutoff<NM>(b, kFracMinimal);
+    return frac_index<NM>(b);
+}
+
+// The reference's boundary_value (mm_fluid.cpp:98-108) of a face state:
+// the state with the velocity reflected at walls.
+template <int NM>
+__device__ __forceinline__ MZ<NM> mm_boundary(const Frame& fr, const MZ<NM>& z, bool wall) {
+    MZ<NM> r = z;
+    const Z w = frame_boundary(fr, Z{z.r, z.u, z.v, z.P, z.e}, wall);
+    r.u = w.u;
+    r.v = w.v;
+    return r;
+}
+
+// The two states of slot s of cell c in the global frame: acc 1 the cells'
+// own (MmFluid::fluxes, :373-445), acc 2 the extrapolated half states of
+// fluxes_stage2 (:580-680; a periodic face's neighbour centre is the
+// neighbour's own, since face.is_boundary() is false there).
+template <int NM, int ACC>
+__device__ __forceinline__ void mm_face_pair(const DevMesh& M, const MmLayer& in, const MmLayer& half, const Mix& mx,
+                                             const double* gx, const double* gy, int c, int so, int t,
+                                             const Frame& fr, MZ<NM>& zm, MZ<NM>& zp) {
+    const int st = M.stride;
+    const bool inner = t <= SC_PERIODIC;
+    const int nb = inner ? M.adj[so] : c;
+    if (ACC == 1) {
+        zm = load_mz<NM>(in, c, st);
+        zp = inner ? load_mz<NM>(in, nb, st) : mm_boundary<NM>(fr, zm, t == SC_WALL);
+    } else {
+        const double fx = M.fcx[so], fy = M.fcy[so];
+        zm = mm_face_state<NM>(half, mx, gx, gy, c, st, fx - M.cx[c], fy - M.cy[c], true);
+        zp = inner ? mm_face_state<NM>(half, mx, gx, gy, nb, st, fx - M.cx[nb], fy - M.cy[nb], true)
+                   : mm_boundary<NM>(fr, zm, t == SC_WALL);
+    }
+}
+
+// Whether slot s of cell c is a composite-flux face: not both face states
+// pure (MmFluid::calc_flux, :347-371).
+template <int NM, int ACC>
+__device__ __forceinline__ bool mm_crp_slot(const DevMesh& M, const MmLayer& in, const MmLayer& half,
+                                            const double* gx, const double* gy, int c, int so, int t) {
+    const int st = M.stride;
+    const bool inner = t <= SC_PERIODIC;
+    const int nb = inner ? M.adj[so] : c;
+    if (ACC == 1) {
+        double b[NM];
+#pragma unroll
+        for (int i = 0; i < NM; ++i) b[i] = in.b[i * st + c];
+        if (frac_index<NM>(b) < 0) return true;
+        if (!inner) return false;
+#pragma unroll
+        for (int i = 0; i < NM; ++i) b[i] = in.b[i * st + nb];
+        return frac_index<NM>(b) < 0;
+    }
+    const double fx = M.fcx[so], fy = M.fcy[so];
+    if (mm_face_index<NM>(half, gx, gy, c, st, fx - M.cx[c], fy - M.cy[c]) < 0) return true;
+    return inner && mm_face_index<NM>(half, gx, gy, nb, st, fx - M.cx[nb], fy - M.cy[nb]) < 0;
+}
+
 }  // namespace
 
 // Cells from {rho, u, v, P, beta} in place: the problem files'
@@ -115,34 +240,24 @@ __device__ __forceinline__ bool crp_face(const double* CF, const MZ<NM>& zm, con
     return CF != nullptr && !(frac_index<NM>(zm.b) >= 0 && frac_index<NM>(zp.b) >= 0);
 }
 
-// Faces that need a composite Riemann flux (CrpMode::PLIC): a mixed cell on
-// either side (MmFluid::calc_flux, mm_fluid.cpp:339-371; a boundary face's
-// other side is the cell's own boundary value).  Appends slot-major pairs
-// s * stride + c to `list`; `count` starts at zero.
-template <int NM>
-__global__ void __launch_bounds__(kBlock) k_mm_crp_mark(DevMesh M, MmLayer in, int* __restrict__ list,
-                                                        unsigned int* count) {
+// Faces that need a composite Riemann flux (CrpMode::PLIC): a mixed face
+// state on either side (MmFluid::calc_flux, mm_fluid.cpp:347-371; a
+// boundary face's other side is the boundary value of the cell's own).  acc
+// 1: the cells' states; acc 2: the extrapolated, normalised half states.
+// Appends slot-major pairs s * stride + c to `list`; `count` starts at zero.
+template <int NM, int ACC>
+__global__ void __launch_bounds__(kBlock) k_mm_crp_mark(DevMesh M, MmLayer in, MmLayer half,
+                                                        const double* __restrict__ gx, const double* __restrict__ gy,
+                                                        int* __restrict__ list, unsigned int* count) {
     const Tile tl = tile_of(M, (int)blockDim.x);
     const int c = tl.c0 + (int)threadIdx.x;
     if (c >= tl.end) return;
     const int st = M.stride;
-    double b[NM];
-#pragma unroll
-    for (int i = 0; i < NM; ++i) b[i] = in.b[i * st + c];
-    const bool own_pure = frac_index<NM>(b) >= 0;
     const int code = M.code[c];
     for (int s = 0; s < 8; ++s) {
         const int t = (code >> (4 * s)) & 7;
         if (t == SC_UNDEF) continue;
-        bool nb_pure = own_pure;
-        if (t <= SC_PERIODIC) {
-            const int nb = M.adj[s * st + c];
-            double bn[NM];
-#pragma unroll
-            for (int i = 0; i < NM; ++i) bn[i] = in.b[i * st + nb];
-            nb_pure = frac_index<NM>(bn) >= 0;
-        }
-        if (!(own_pure && nb_pure)) list[atomicAdd(count, 1u)] = s * st + c;
+        if (mm_crp_slot<NM, ACC>(M, in, half, gx, gy, c, s * st + c, t)) list[atomicAdd(count, 1u)] = s * st + c;
     }
 }
 
@@ -153,8 +268,9 @@ __global__ void __launch_bounds__(kBlock) k_mm_crp_mark(DevMesh M, MmLayer in, i
 // interface_recovery) and calc_crp_flux (:259-337), rotated to the global
 // frame, into CF[slot][k][stride].  These faces are few and need many more
 // registers than the plain HLLC faces of k_mm_flux1, hence the compaction.
-template <int NM>
-__global__ void __launch_bounds__(64) k_mm_crp(DevMesh M, MmLayer in, Mix mx, StepCtl ctl,
+template <int NM, int ACC>
+__global__ void __launch_bounds__(64) k_mm_crp(DevMesh M, MmLayer in, MmLayer half, const double* __restrict__ gx,
+                                            const double* __restrict__ gy, Mix mx, StepCtl ctl,
                                             const double* __restrict__ inx, const double* __restrict__ iny,
                                             const int* __restrict__ list, const unsigned int* count,
                                             double* __restrict__ CF) {
@@ -173,22 +289,13 @@ __global__ void __launch_bounds__(64) k_mm_crp(DevMesh M, MmLayer in, Mix mx, St
         const int so = list[e];
         const int s = so / st, c = so - s * st;
         const double dt = step_dt(ctl, 0);
-        const MZ<NM> zc = load_mz<NM>(in, c, st);
         const int nib = (M.code[c] >> (4 * s)) & 15;
         const int t = nib & 7;
         const bool bound = t > SC_PERIODIC;
         const int nb = bound ? c : M.adj[so];
         const Frame fr = slot_frame(M, c, s, nib);
-        MZ<NM> zp;
-        if (!bound) {
-            zp = load_mz<NM>(in, nb, st);
-        } else {
-            zp = zc;
-            const Z w = frame_boundary(fr, Z{zc.r, zc.u, zc.v, zc.P, zc.e}, t == SC_WALL);
-            zp.u = w.u;
-            zp.v = w.v;
-        }
-        MZ<NM> zm = zc;
+        MZ<NM> zm, zp;
+        mm_face_pair<NM, ACC>(M, in, half, mx, gx, gy, c, so, t, fr, zm, zp);
         rotate_uv<NM>(fr, zm);
         rotate_uv<NM>(fr, zp);
         const double S = M.area[so];
@@ -279,16 +386,116 @@ __global__ void __launch_bounds__(kBlock) k_mm_flux1(DevMesh M, MmLayer in, MmLa
 #pragma unroll
         for (int k = 0; k < 4 + NM; ++k) q[k] = q[k] - coef * acc[k];
         if (q[0] < 0.0) bad = true;   // mm_fluid.cpp:440-442 throws
-        MZ<NM> zn;
-        zn.r = q[0];
-        zn.u = q[1] / zn.r;
-        zn.v = q[2] / zn.r;
-        zn.e = q[3] / zn.r - 0.5 * (zn.u * zn.u + zn.v * zn.v);
+        MZ<NM> zn = mm_from_q<NM>(mx, q, zc);
+        zn.c = sound_speed<NM>(mx, zn.r, zn.P, zn.b, zn.T, zn.rh);
+        store_mz<NM>(out, c, st, zn);
+        if (bad) atomicOr(ctl.err, 1);
+        cand = mm_dt_cand<NM>(M, c, zn);
+    }
+    block_min_atomic(&ctl.dtmin[3 * tl.pid + ctl.wr], cand);
+}
+
+// MmFluid::fluxes_stage1 (mm_fluid.cpp:521-578): a mixed cell keeps its
+// state; a pure cell advances half a step, q - 0.5 dt / V sum_faces
+// Flux(z_m) S with the physical flux of its own extrapolated, re-closed
+// face states, and re-closes from its previous P, T and true densities.
+template <int NM>
+__global__ void __launch_bounds__(kBlock) k_mm_stage1(DevMesh M, MmLayer in, MmLayer half, Mix mx, StepCtl ctl,
+                                                      const double* __restrict__ gx, const double* __restrict__ gy) {
+    const Tile tl = tile_of(M, (int)blockDim.x);
+    const int c = tl.c0 + (int)threadIdx.x;
+    if (c >= tl.end) return;
+    const int st = M.stride;
+    const MZ<NM> zc = load_mz<NM>(in, c, st);
+    if (frac_index<NM>(zc.b) < 0) {
+        store_mz<NM>(half, c, st, zc);
+        return;
+    }
+    const double dt = step_dt(ctl, tl.pid);
+    const double ccx = M.cx[c], ccy = M.cy[c];
+    const int code = M.code[c];
+    double acc[4 + NM];
 #pragma unroll
-        for (int i = 0; i < NM; ++i) zn.b[i] = q[4 + i] / zn.r;
-        frac_cutoff<NM>(zn.b, 1.0e-12);
-        frac_cutoff<NM>(zn.b, kFracMinimal);
-        get_rPT<NM>(mx, zn.r, zn.e, zn.b, zc.P, zc.T, zc.rh, zn.rh, zn.P, zn.T);
+    for (int k = 0; k < 4 + NM; ++k) acc[k] = 0.0;
+#pragma unroll 1
+    for (int s = 0; s < 8; ++s) {
+        const int nib = (code >> (4 * s)) & 15;
+        if ((nib & 7) == SC_UNDEF) continue;
+        const int so = s * st + c;
+        const Frame fr = slot_frame(M, c, s, nib);
+        MZ<NM> zm = mm_face_state<NM>(in, mx, gx, gy, c, st, M.fcx[so] - ccx, M.fcy[so] - ccy, false);
+        rotate_uv<NM>(fr, zm);
+        double F[4 + NM];
+        m_flux<NM>(zm, F);
+        const Q g = frame_global(fr, Q{F[0], F[1], F[2], F[3]});
+        F[1] = g.mx;
+        F[2] = g.my;
+        const double S = M.area[so];
+#pragma unroll
+        for (int k = 0; k < 4 + NM; ++k) acc[k] = acc[k] + F[k] * S;
+    }
+    double q[4 + NM];
+    m_qstate<NM>(zc, q);
+    const double coef = 0.5 * dt / M.vol[c];
+#pragma unroll
+    for (int k = 0; k < 4 + NM; ++k) q[k] = q[k] - coef * acc[k];
+    store_mz<NM>(half, c, st, mm_from_q<NM>(mx, q, zc));
+}
+
+// MmFluid::fluxes_stage2 (mm_fluid.cpp:580-680): per face the states
+// extrapolated from the half layer with the gradients of the old one and
+// re-closed, the mmf HLLC flux or the composite flux from CF, q -= dt / V
+// sum F S on the old conserved vector, the closure from the old P, T and
+// true densities, the sound speed and the next step's CFL candidate.
+template <int NM>
+__global__ void __launch_bounds__(kBlock) k_mm_flux2(DevMesh M, MmLayer in, MmLayer half, MmLayer out, Mix mx,
+                                                     StepCtl ctl, const double* __restrict__ gx,
+                                                     const double* __restrict__ gy, const double* __restrict__ CF) {
+    const Tile tl = tile_of(M, (int)blockDim.x);
+    const int c = tl.c0 + (int)threadIdx.x;
+    const double dt = step_dt(ctl, tl.pid);
+    step_prologue(ctl, tl, dt);
+    double cand = INFINITY;
+    if (c < tl.end) {
+        const int st = M.stride;
+        const int code = M.code[c];
+        double acc[4 + NM];
+#pragma unroll
+        for (int k = 0; k < 4 + NM; ++k) acc[k] = 0.0;
+        bool bad = false;
+#pragma unroll 1
+        for (int s = 0; s < 8; ++s) {
+            const int nib = (code >> (4 * s)) & 15;
+            const int t = nib & 7;
+            if (t == SC_UNDEF) continue;
+            const int so = s * st + c;
+            double F[4 + NM];
+            if (CF != nullptr && mm_crp_slot<NM, 2>(M, in, half, gx, gy, c, so, t)) {
+#pragma unroll
+                for (int k = 0; k < 4 + NM; ++k) F[k] = CF[(s * (4 + NM) + k) * st + c];
+            } else {
+                const Frame fr = slot_frame(M, c, s, nib);
+                MZ<NM> zm, zp;
+                mm_face_pair<NM, 2>(M, in, half, mx, gx, gy, c, so, t, fr, zm, zp);
+                rotate_uv<NM>(fr, zm);
+                rotate_uv<NM>(fr, zp);
+                hllc_mm<NM>(zm, zp, F, bad);
+                const Q g = frame_global(fr, Q{F[0], F[1], F[2], F[3]});
+                F[1] = g.mx;
+                F[2] = g.my;
+            }
+            const double S = M.area[so];
+#pragma unroll
+            for (int k = 0; k < 4 + NM; ++k) acc[k] = acc[k] + F[k] * S;
+        }
+        const MZ<NM> zc = load_mz<NM>(in, c, st);
+        double q[4 + NM];
+        m_qstate<NM>(zc, q);
+        const double coef = dt / M.vol[c];
+#pragma unroll
+        for (int k = 0; k < 4 + NM; ++k) q[k] = q[k] - coef * acc[k];
+        if (q[0] < 0.0) bad = true;   // mm_fluid.cpp:671-673 throws
+        MZ<NM> zn = mm_from_q<NM>(mx, q, zc);
         zn.c = sound_speed<NM>(mx, zn.r, zn.P, zn.b, zn.T, zn.rh);
         store_mz<NM>(out, c, st, zn);
         if (bad) atomicOr(ctl.err, 1);
@@ -390,7 +597,11 @@ __device__ double limiter_rt(int kind, double a, double b) {
     }
 }
 
-// Components of the gradient the split reads: rho, u, v, e, beta[i].
+// Components of the gradient: rho, u, v, e, beta[i] (what the split
+// reads), then P, T, rhos[i] (the second-order face states carry every
+// PState component, face_extra.h:122-184).  A true density is NaN where its
+// material is absent, and so is its gradient wherever the stencil meets
+// such a cell.
 template <int NM>
 __device__ __forceinline__ void mg_comps(const MmLayer& L, int c, int st, double* q) {
     q[0] = L.r[c];
@@ -399,6 +610,10 @@ __device__ __forceinline__ void mg_comps(const MmLayer& L, int c, int st, double
     q[3] = L.e[c];
 #pragma unroll
     for (int i = 0; i < NM; ++i) q[4 + i] = L.b[i * st + c];
+    q[4 + NM] = L.P[c];
+    q[5 + NM] = L.T[c];
+#pragma unroll
+    for (int i = 0; i < NM; ++i) q[6 + NM + i] = L.rh[i * st + c];
 }
 
 // MmFluid::compute_grad (mm_fluid.cpp:439-466): with composite Riemann
@@ -407,11 +622,11 @@ __device__ __forceinline__ void mg_comps(const MmLayer& L, int c, int st, double
 // gradient::limiting (:412-474) with the mm boundary_value
 // (mm_fluid.cpp:98-108: walls reflect the velocity), component by
 // component as k_grad_pred does for SmFluid (the 2 x 2 block of the LSM
-// system).  gx, gy: [4 + NM][stride].
+// system).  gx, gy: [6 + 2 NM][stride].
 template <int NM>
 __global__ void __launch_bounds__(kBlock) k_mm_grad(DevMesh M, MmLayer L, int crp, int limiter,
                                                     double* __restrict__ gx, double* __restrict__ gy) {
-    constexpr int K = 4 + NM;
+    constexpr int K = 6 + 2 * NM;
     const int c = (int)(blockIdx.x * blockDim.x + threadIdx.x);
     if (c >= M.n) return;
     const int st = M.stride;
@@ -738,18 +953,45 @@ __global__ void __launch_bounds__(64) k_zero_u32(unsigned int* p) {
     if (threadIdx.x == 0) *p = 0u;
 }
 
-hipError_t mm_crp_mark(const DevMesh& M, const MmLayer& in, const Mix& mx, int* list, unsigned int* count,
-                       hipStream_t s) {
+#define ZFV_MM_DISPATCH_ACC(NMAT, ACC, KERNEL, ...)                                                     \
+    switch ((NMAT) * 2 + (ACC) - 1) {                                                                   \
+        case 2: hipLaunchKernelGGL((KERNEL<1, 1>), __VA_ARGS__); break;                                  \
+        case 3: hipLaunchKernelGGL((KERNEL<1, 2>), __VA_ARGS__); break;                                  \
+        case 4: hipLaunchKernelGGL((KERNEL<2, 1>), __VA_ARGS__); break;                                  \
+        case 5: hipLaunchKernelGGL((KERNEL<2, 2>), __VA_ARGS__); break;                                  \
+        case 6: hipLaunchKernelGGL((KERNEL<3, 1>), __VA_ARGS__); break;                                  \
+        case 7: hipLaunchKernelGGL((KERNEL<3, 2>), __VA_ARGS__); break;                                  \
+        case 8: hipLaunchKernelGGL((KERNEL<4, 1>), __VA_ARGS__); break;                                  \
+        default: hipLaunchKernelGGL((KERNEL<4, 2>), __VA_ARGS__); break;                                 \
+    }
+
+hipError_t mm_crp_mark(const DevMesh& M, const MmLayer& in, const MmLayer& half, const Mix& mx, int acc,
+                       const double* gx, const double* gy, int* list, unsigned int* count, hipStream_t s) {
     // the counter is cleared by a kernel, not a memset node, so the step can be captured
     hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, s, count);
-    ZFV_MM_DISPATCH(mx.n, k_mm_crp_mark, mm_grid(M.n), dim3(kBlock), 0, s, M, in, list, count);
+    ZFV_MM_DISPATCH_ACC(mx.n, acc, k_mm_crp_mark, mm_grid(M.n), dim3(kBlock), 0, s, M, in, half, gx, gy, list,
+                        count);
     return hipGetLastError();
 }
 
-hipError_t mm_crp(const DevMesh& M, const MmLayer& in, const Mix& mx, const StepCtl& ctl, const double* nx,
-                  const double* ny, const int* list, const unsigned int* count, double* CF, hipStream_t s) {
+hipError_t mm_crp(const DevMesh& M, const MmLayer& in, const MmLayer& half, const Mix& mx, int acc,
+                  const double* gx, const double* gy, const StepCtl& ctl, const double* nx, const double* ny,
+                  const int* list, const unsigned int* count, double* CF, hipStream_t s) {
     // 2048 one-wavefront workgroups, grid-stride over any list length
-    ZFV_MM_DISPATCH(mx.n, k_mm_crp, dim3(2048), dim3(64), 0, s, M, in, mx, ctl, nx, ny, list, count, CF);
+    ZFV_MM_DISPATCH_ACC(mx.n, acc, k_mm_crp, dim3(2048), dim3(64), 0, s, M, in, half, gx, gy, mx, ctl, nx, ny, list,
+                        count, CF);
+    return hipGetLastError();
+}
+
+hipError_t mm_stage1(const DevMesh& M, const MmLayer& in, const MmLayer& half, const Mix& mx, const StepCtl& ctl,
+                     const double* gx, const double* gy, hipStream_t s) {
+    ZFV_MM_DISPATCH(mx.n, k_mm_stage1, mm_grid(M.n), dim3(kBlock), 0, s, M, in, half, mx, ctl, gx, gy);
+    return hipGetLastError();
+}
+
+hipError_t mm_flux2(const DevMesh& M, const MmLayer& in, const MmLayer& half, const MmLayer& out, const Mix& mx,
+                    const StepCtl& ctl, const double* gx, const double* gy, const double* CF, hipStream_t s) {
+    ZFV_MM_DISPATCH(mx.n, k_mm_flux2, mm_grid(M.n), dim3(kBlock), 0, s, M, in, half, out, mx, ctl, gx, gy, CF);
     return hipGetLastError();
 }
 
