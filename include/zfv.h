@@ -30,7 +30,11 @@ enum {
 };
 
 /* Numerical flux ids = zephyr::math::Fluxes values (math/cfd/flux/num_flux.h:11-21). */
-enum { ZFV_GODUNOV = 0, ZFV_RUSANOV = 1, ZFV_HLL = 2, ZFV_HLLC = 3 };
+enum { ZFV_GODUNOV = 0, ZFV_RUSANOV = 1, ZFV_HLL = 2, ZFV_HLLC = 3, ZFV_CRP = 8 };
+
+/* MmFluid's CrpMode (math/solver/mm_fluid.h:36-42) and DirSplit (:29-33). */
+enum { ZFV_CRP_NONE = 0, ZFV_CRP_V3 = 1, ZFV_CRP_V5 = 2, ZFV_CRP_VS = 3, ZFV_CRP_PLIC = 4, ZFV_CRP_MUSCL = 5 };
+enum { ZFV_SPLIT_NONE = 0, ZFV_SPLIT_SIMPLE = 1, ZFV_SPLIT_STRANG = 2 };
 
 /* Limiter ids; names parsed as Limiter(const std::string&) (math/cfd/limiter.cxx:163-252). */
 enum {
@@ -217,10 +221,28 @@ int zfv_adjoint_backward(zfv_ctx* ctx, double* d_rho, double* d_u, double* d_v, 
  * (MmFluid::set_CFL, mm_fluid.cpp:41-43; the reference default is 0.7).
  * Per-material arrays are [n_mat][n]. */
 int zfv_mm_configure(zfv_ctx* ctx, int n_mat, const double* mats, double cfl);
-/* Composite Riemann problems at mixed faces: 1 = CrpMode::PLIC (default),
- * 0 = CrpMode::NONE (HLLC everywhere; MmFluid::set_crp_mode / set_method,
- * mm_fluid.cpp:45-70). */
+/* Composite Riemann problems at mixed faces: 1 = CrpMode::PLIC with HLLC
+ * (the default, MmFluid::set_method(Fluxes::CRP)), 0 = CrpMode::NONE with
+ * HLLC everywhere (set_method(Fluxes::HLLC)) (mm_fluid.cpp:50-61). */
 int zfv_mm_set_crp(zfv_ctx* ctx, int mode);
+/* MmFluid::set_method (mm_fluid.cpp:50-61): ZFV_CRP = CrpMode::PLIC with
+ * HLLC; ZFV_HLLC, ZFV_HLL or ZFV_RUSANOV = CrpMode::NONE with that mmf flux
+ * (hllc.cpp:339-421, hll.cpp:125-160, rusanov.cpp:46-83).  The mmf Godunov
+ * flux and the other Fluxes are not built (ZFV_ERR_UNSUPPORTED). */
+int zfv_mm_set_method(zfv_ctx* ctx, int flux);
+/* MmFluid::set_crp_mode (mm_fluid.cpp:63-70): ZFV_CRP_V3 / V5 / VS (the
+ * heuristic face fractions, sections.cpp:52-84) or ZFV_CRP_PLIC, each with
+ * HLLC; ZFV_CRP_NONE leaves the mode unchanged, as in the reference;
+ * ZFV_CRP_MUSCL is ZFV_ERR_UNSUPPORTED (the reference's alpha_sigma returns
+ * NaN for it, mm_fluid.cpp:244-249). */
+int zfv_mm_set_crp_mode(zfv_ctx* ctx, int mode);
+/* MmFluid::set_splitting (mm_fluid.cpp:76-78): ZFV_SPLIT_NONE (default),
+ * ZFV_SPLIT_SIMPLE (an X sweep then a Y sweep, each with dt) or
+ * ZFV_SPLIT_STRANG (X with dt / 2, Y with dt, X with dt / 2); a sweep takes
+ * the faces of its direction (EuCell::faces(dir), |n| component >= 0.7,
+ * amr_faces.cpp:186-197).  MmFluid::set_gravity has no counterpart: the
+ * reference stores g and never reads it. */
+int zfv_mm_set_splitting(zfv_ctx* ctx, int split);
 /* MmFluid::set_max_dt (mm_fluid.cpp:94-96). */
 int zfv_mm_set_max_dt(zfv_ctx* ctx, double max_dt);
 /* Cells from {rho, u, v, P, beta}: mmf::PState(rho, v, P, Fractions, mixture)

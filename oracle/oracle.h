@@ -124,7 +124,9 @@ typedef struct orc_mm orc_mm;
 orc_mm* orc_mm_create(orc_mesh* m, int n_mat, const double* mats, double cfl);
 void orc_mm_free(orc_mm* s);
 void orc_mm_set_max_dt(orc_mm* s, double v);
-void orc_mm_set_crp(orc_mm* s, int mode);   /* 1 = CrpMode::PLIC (default), 0 = NONE */
+void orc_mm_set_crp(orc_mm* s, int mode);   /* CrpMode: 0 NONE, 1 V3, 2 V5, 3 VS, 4 PLIC (default) */
+void orc_mm_set_flux(orc_mm* s, int flux);   /* Fluxes: 1 Rusanov, 2 HLL, 3 HLLC (default) */
+void orc_mm_set_splitting(orc_mm* s, int split);   /* DirSplit: 0 NONE, 1 SIMPLE, 2 STRANG */
 void orc_mm_set_state(orc_mm* s, const double* rho, const double* u, const double* v, const double* P,
                       const double* beta);
 void orc_mm_state(const orc_mm* s, double* rho, double* u, double* v, double* P, double* e, double* T,

@@ -400,6 +400,52 @@ inline MArr m_hllc(const MState& zL, const MState& zR, const Mixture& mx) {
     return F;
 }
 
+// Rusanov::calc_flux, mmf (math/cfd/flux/rusanov.cpp:46-83).
+inline MArr m_rusanov(const MState& zL, const MState& zR, const Mixture& mx) {
+    MixOpt oL, oR;
+    oL.T0 = zL.T;
+    oL.rhos = zL.rhos;
+    oR.T0 = zR.T;
+    oR.rhos = zR.rhos;
+    const double c_L = mix_sound_speed_rP(mx, zL.rho, zL.P, zL.beta, oL);
+    const double c_R = mix_sound_speed_rP(mx, zR.rho, zR.P, zR.beta, oR);
+    if (std::isnan(c_L) || std::isnan(c_R)) g_numeric_error = 1;   // :53-59 throws
+    const double S_L = smin(smin(zL.v.x - c_L, zR.v.x - c_R), 0.0);   // std::min({...})
+    const double S_R = smax(smax(zL.v.x + c_L, zR.v.x + c_R), 0.0);
+    const double S = smax(std::abs(S_L), S_R);
+    const MArr QL = m_qstate(zL), QR = m_qstate(zR), FL = m_flux(zL), FR = m_flux(zR);
+    MArr F;
+    for (int k = 0; k < 9; ++k) {
+        F.a[k] = 0.5 * (FL.a[k] + FR.a[k]) + 0.5 * S * (QL.a[k] - QR.a[k]);
+        if (k < 5 && !std::isfinite(F.a[k])) g_numeric_error = 1;   // Flux::is_bad (models.cpp:535-541)
+    }
+    return F;
+}
+
+// HLL::calc_flux, mmf (math/cfd/flux/hll.cpp:125-160).
+inline MArr m_hll(const MState& zL, const MState& zR, const Mixture& mx) {
+    MixOpt oL, oR;
+    oL.T0 = zL.T;
+    oL.rhos = zL.rhos;
+    oR.T0 = zR.T;
+    oR.rhos = zR.rhos;
+    const double c_L = mix_sound_speed_rP(mx, zL.rho, zL.P, zL.beta, oL);
+    const double c_R = mix_sound_speed_rP(mx, zR.rho, zR.P, zR.beta, oR);
+    double S_L = smin(zL.v.x - c_L, zR.v.x - c_L);
+    double S_R = smax(zL.v.x + c_R, zR.v.x + c_R);
+    if (S_L >= S_R) {
+        S_L = smin(zL.v.x - c_L, zR.v.x - c_R);
+        S_R = smax(zL.v.x + c_L, zR.v.x + c_R);
+    }
+    const MArr QL = m_qstate(zL), QR = m_qstate(zR), FL = m_flux(zL), FR = m_flux(zR);
+    MArr F;
+    for (int k = 0; k < 9; ++k) {
+        F.a[k] = (S_R * FL.a[k] - S_L * FR.a[k] + S_L * S_R * (QR.a[k] - QL.a[k])) / (S_R - S_L);
+        if (std::isnan(F.a[k])) g_numeric_error = 1;   // :151-158 throws
+    }
+    return F;
+}
+
 // boundary_value, mmf (math/solver/mm_fluid.cpp:98-108).
 inline MState m_boundary_value(const MState& zc, const Vec3& n, int flag) {
     if (flag != ORC_WALL) return zc;
@@ -844,12 +890,18 @@ using namespace orcmm;
 // and so is its gradient.  A true density is NaN where its material is
 // absent, so its gradient is NaN wherever the stencil meets such a cell.
 constexpr int kMG = 6 + 2 * kMaxMat;
+// CrpMode (mm_fluid.h:36-42) and Fluxes (num_flux.h:11-21) by their
+// positions in the reference's enums.
+constexpr int kCrpNone = 0, kCrpV3 = 1, kCrpV5 = 2, kCrpVS = 3, kCrpPlic = 4;
+constexpr int kFluxRusanov = 1, kFluxHll = 2, kFluxHllc = 3;
 struct MG { double a[kMG] = {}; };
 
 struct orc_mm {
     orc_mesh* m;
     Mixture mix;
-    int crp = 1;   // CrpMode::PLIC, the MmFluid default (mm_fluid.cpp:18); 0 = CrpMode::NONE
+    int crp = kCrpPlic;   // CrpMode (mm_fluid.h:36-42): the MmFluid default PLIC (mm_fluid.cpp:18)
+    int flux = kFluxHllc; // m_nf: HLLC, or HLL / Rusanov with CrpMode::NONE (set_method, mm_fluid.cpp:50-61)
+    int split = 0;        // DirSplit (mm_fluid.h:29-33): NONE, SIMPLE, STRANG
     int limiter = ORC_LIM_MC;   // m_limiter = "MC" (mm_fluid.cpp:20)
     int acc = 1;                // m_acc (mm_fluid.h:157), set_accuracy (mm_fluid.cpp:46-48)
     double cfl, dt, max_dt;
@@ -1110,9 +1162,18 @@ void m_interpolation_update(MState& z, const Mixture& mx) {
 MArr mm_calc_flux(const orc_mm& S, int c, size_t f, const MState& z_m, const MState& z_p, double h_L, double h_R,
                   double dt, const double* inx, const double* iny, int& crp_err) {
     const orc_mesh& m = *S.m;
-    if (!S.crp || (z_m.beta.index() >= 0 && z_p.beta.index() >= 0)) return m_hllc(z_m, z_p, S.mix);
+    if (!S.crp || (z_m.beta.index() >= 0 && z_p.beta.index() >= 0)) {
+        if (S.flux == kFluxRusanov) return m_rusanov(z_m, z_p, S.mix);
+        if (S.flux == kFluxHll) return m_hll(z_m, z_p, S.mix);
+        return m_hllc(z_m, z_p, S.mix);
+    }
     const int iA = master_component(z_m, z_p);
     const double a_L = m_alpha(z_m, iA), a_R = m_alpha(z_p, iA);
+    if (S.crp != kCrpPlic) {   // the heuristic templates (alpha_sigma, :235-243)
+        const double a_sig = S.crp == kCrpV3 ? face_fraction_v3(a_L, a_R)
+                           : S.crp == kCrpV5 ? face_fraction_v5(a_L, a_R) : face_fraction_s(a_L, a_R);
+        return calc_crp_flux(z_m, z_p, h_L, h_R, iA, a_sig, dt, S.mix, crp_err);
+    }
     const double vi = 0.5 * (z_m.v.x + z_p.v.x);
     const bool upwind = vi > 0.0;
     const double CFL = dt * std::abs(vi) / (upwind ? h_L : h_R);
@@ -1123,6 +1184,14 @@ MArr mm_calc_flux(const orc_mm& S, int c, size_t f, const MState& z_m, const MSt
     const double cs = dot(normal, nc);
     const double a_sig = f_between(average_flux(upwind ? a_L : a_R, cs, CFL), a_L, a_R);
     return calc_crp_flux(z_m, z_p, h_L, h_R, iA, a_sig, dt, S.mix, crp_err);
+}
+
+// EuCell::faces(dir) (amr_faces.cpp:186-197): 1 = X keeps faces with
+// |n.x| >= 0.7, 2 = Y those with |n.y| >= 0.7, 0 = all.
+inline bool mm_skip(const orc_mesh& m, size_t f, int dir) {
+    if (dir == 1) return std::abs(m.fnx[f]) < 0.7;
+    if (dir == 2) return std::abs(m.fny[f]) < 0.7;
+    return false;
 }
 
 inline void m_to_global(MArr& loc, const Vec3& normal) {   // Flux::to_global
@@ -1266,7 +1335,9 @@ double orc_mm_compute_dt(orc_mm* s) {
     return s->dt;
 }
 
-void orc_mm_set_crp(orc_mm* s, int mode) { s->crp = mode; }
+void orc_mm_set_crp(orc_mm* s, int mode) { s->crp = mode; }   // CrpMode value (0 NONE .. 4 PLIC)
+void orc_mm_set_flux(orc_mm* s, int flux) { s->flux = flux; }  // Fluxes value of the non-composite faces
+void orc_mm_set_splitting(orc_mm* s, int split) { s->split = split; }
 
 // MmFluid::update, first order (mm_fluid.cpp:110-145): compute_dt, for
 // CrpMode::PLIC interface_recovery, fluxes (:373-445) with calc_flux
@@ -1274,17 +1345,13 @@ void orc_mm_set_crp(orc_mm* s, int mode) { s->crp = mode; }
 // composite Riemann flux of the master material with the PLIC face
 // fraction, alpha_sigma :228-256), swap; returns 0, or 1 on a NaN flux, a
 // negative density or a CRP case the reference rejects (it throws).
-static int mm_update2(orc_mm* s);
-
-int orc_mm_update(orc_mm* s) {
-    g_numeric_error = 0;
-    orc_mm_compute_dt(s);
-    if (s->acc == 2) return mm_update2(s);
+// MmFluid::integrate with acc 1 (mm_fluid.cpp:139-150): interface_recovery
+// (PLIC), fluxes (:373-445) over the faces of direction dir, swap.
+static int mm_integrate1(orc_mm* s, double dt, int dir) {
     const orc_mesh& m = *s->m;
-    const double dt = s->dt;
     const int nm = s->mix.n;
     std::vector<double> inx, iny, ip;
-    if (s->crp) {
+    if (s->crp == kCrpPlic) {
         inx.resize((size_t)nm * m.n);
         iny.resize((size_t)nm * m.n);
         ip.resize((size_t)nm * m.n);
@@ -1298,7 +1365,7 @@ int orc_mm_update(orc_mm* s) {
         for (int k = 0; k < 8; ++k) {
             const size_t f = 8 * (size_t)c + k;
             const int b = m.bnd[f];
-            if (b == ORC_UNDEFINED) continue;
+            if (b == ORC_UNDEFINED || mm_skip(m, f, dir)) continue;
             const Vec3 normal{m.fnx[f], m.fny[f], 0.0};
             MState z_n;
             if (!slot_is_boundary(b)) z_n = s->init[m.adj[f]];
@@ -1319,19 +1386,18 @@ int orc_mm_update(orc_mm* s) {
         s->next[c] = m_from_q(q_c, s->mix, z_c.P, z_c.T, z_c.rhos);
     }
     std::swap(s->init, s->next);   // MmFluid::swap (mm_fluid.cpp:682-684)
-    return (g_numeric_error || neg || crp_err) ? 1 : 0;
+    return (neg || crp_err) ? 1 : 0;
 }
 
-// MmFluid::integrate with acc 2 (mm_fluid.cpp:153-168): compute_grad of
+// MmFluid::integrate with acc 2 (mm_fluid.cpp:153-170): compute_grad of
 // init, interface_recovery (PLIC), fluxes_stage1 (:521-578) into half and
-// fluxes_stage2 (:580-680) into next, then swap.
-static int mm_update2(orc_mm* s) {
+// fluxes_stage2 (:580-680) into next over the faces of direction dir, swap.
+static int mm_integrate2(orc_mm* s, double dt, int dir) {
     const orc_mesh& m = *s->m;
-    const double dt = s->dt;
     const int nm = s->mix.n;
     orc_mm_compute_grad(s);
     std::vector<double> inx, iny, ip;
-    if (s->crp) {
+    if (s->crp == kCrpPlic) {
         inx.resize((size_t)nm * m.n);
         iny.resize((size_t)nm * m.n);
         ip.resize((size_t)nm * m.n);
@@ -1349,7 +1415,7 @@ static int mm_update2(orc_mm* s) {
         MArr flux;
         for (int k = 0; k < 8; ++k) {
             const size_t f = 8 * (size_t)c + k;
-            if (m.bnd[f] == ORC_UNDEFINED) continue;
+            if (m.bnd[f] == ORC_UNDEFINED || mm_skip(m, f, dir)) continue;
             const Vec3 normal{m.fnx[f], m.fny[f], 0.0};
             const Vec3 face_c{m.fcx[f], m.fcy[f], 0.0};
             MState z_m = m_extrap(z_c, s->d_dx[c], s->d_dy[c], s->d_dz[c], face_c - cell_c);
@@ -1376,7 +1442,7 @@ static int mm_update2(orc_mm* s) {
         for (int k = 0; k < 8; ++k) {
             const size_t f = 8 * (size_t)c + k;
             const int b = m.bnd[f];
-            if (b == ORC_UNDEFINED) continue;
+            if (b == ORC_UNDEFINED || mm_skip(m, f, dir)) continue;
             const Vec3 normal{m.fnx[f], m.fny[f], 0.0};
             const Vec3 face_c{m.fcx[f], m.fcy[f], 0.0};
             const bool bound = slot_is_boundary(b);
@@ -1406,7 +1472,29 @@ static int mm_update2(orc_mm* s) {
         s->next[c] = m_from_q(q_c, s->mix, z_c.P, z_c.T, z_c.rhos);
     }
     std::swap(s->init, s->next);
-    return (g_numeric_error || neg || crp_err) ? 1 : 0;
+    return (neg || crp_err) ? 1 : 0;
+}
+
+// MmFluid::update (mm_fluid.cpp:112-137): compute_dt, then integrate over
+// all faces, or X then Y (SIMPLE), or X for dt / 2, Y for dt, X for dt / 2
+// (STRANG).
+int orc_mm_update(orc_mm* s) {
+    g_numeric_error = 0;
+    orc_mm_compute_dt(s);
+    const double dt = s->dt;
+    auto integrate = [s](double h, int dir) { return s->acc == 2 ? mm_integrate2(s, h, dir) : mm_integrate1(s, h, dir); };
+    int r = 0;
+    if (s->split == 1) {
+        r |= integrate(dt, 1);
+        r |= integrate(dt, 2);
+    } else if (s->split == 2) {
+        r |= integrate(0.5 * dt, 1);
+        r |= integrate(dt, 2);
+        r |= integrate(0.5 * dt, 1);
+    } else {
+        r = integrate(dt, 0);
+    }
+    return (g_numeric_error || r) ? 1 : 0;
 }
 
 double orc_mm_dt(const orc_mm* s) { return s->dt; }

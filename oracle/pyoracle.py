@@ -76,6 +76,8 @@ def lib():
             "orc_mm_free": (None, [_P]),
             "orc_mm_set_max_dt": (None, [_P, _D]),
             "orc_mm_set_crp": (None, [_P, _I]),
+            "orc_mm_set_flux": (None, [_P, _I]),
+            "orc_mm_set_splitting": (None, [_P, _I]),
             "orc_mm_set_state": (None, [_P, _dp, _dp, _dp, _dp, _dp]),
             "orc_mm_state": (None, [_P] + [_dp] * 8),
             "orc_mm_compute_dt": (_D, [_P]),
@@ -299,15 +301,40 @@ def _mats(materials):
 
 class MmFluid:
     """Oracle MmFluid (math/solver/mm_fluid.h:45-163 restated: first or
-    second order, CrpMode PLIC or NONE); materials: (gamma, P0, e0, Cv, T0,
-    rho_ref)."""
+    second order; CrpMode NONE, V3, V5, VS or PLIC; HLLC, HLL or Rusanov on
+    the plain faces; direction splitting NONE, SIMPLE or STRANG); materials:
+    (gamma, P0, e0, Cv, T0, rho_ref)."""
+
+    CRP_MODES = ("NONE", "V3", "V5", "VS", "PLIC")   # CrpMode (mm_fluid.h:36-42)
+    SPLITS = ("NONE", "SIMPLE", "STRANG")            # DirSplit (mm_fluid.h:29-33)
 
     def __init__(self, mesh, materials, cfl=0.7, crp=True):
         self.mesh = mesh
         self.mats = _mats(materials)
         self.nm = self.mats.shape[0]
         self.h = lib().orc_mm_create(mesh.h, self.nm, _d(self.mats), cfl)
-        lib().orc_mm_set_crp(self.h, 1 if crp else 0)
+        lib().orc_mm_set_crp(self.h, 4 if crp else 0)
+
+    def set_crp_mode(self, mode):
+        """CrpMode by name; composite faces always use HLLC (set_crp_mode,
+        mm_fluid.cpp:63-70; NONE leaves the mode as it is there)."""
+        k = self.CRP_MODES.index(mode)
+        if k:
+            lib().orc_mm_set_crp(self.h, k)
+            lib().orc_mm_set_flux(self.h, 3)
+
+    def set_method(self, flux):
+        """set_method (mm_fluid.cpp:50-61): "CRP" = PLIC with HLLC, else
+        CrpMode NONE with the named mmf flux ("HLLC", "HLL", "Rusanov")."""
+        if flux == "CRP":
+            lib().orc_mm_set_crp(self.h, 4)
+            lib().orc_mm_set_flux(self.h, 3)
+        else:
+            lib().orc_mm_set_crp(self.h, 0)
+            lib().orc_mm_set_flux(self.h, {"Rusanov": 1, "HLL": 2, "HLLC": 3}[flux])
+
+    def set_splitting(self, split):
+        lib().orc_mm_set_splitting(self.h, self.SPLITS.index(split))
 
     def __del__(self):
         if getattr(self, "h", None) and _lib is not None:

@@ -32,7 +32,8 @@ EXPORTS = [
     "zfv_mesh_n_faces", "zfv_set_launch", "zfv_get_launch", "zfv_autotune",
     "zfv_batch_rectangles", "zfv_batch_layout", "zfv_batch_dt", "zfv_graph_enable",
     "zfv_set_cell_order", "zfv_get_cell_order",
-    "zfv_mm_set_limiter", "zfv_mm_set_accuracy", "zfv_mm_set_flags", "zfv_mm_refine", "zfv_mm_grad",
+    "zfv_mm_set_limiter", "zfv_mm_set_accuracy", "zfv_mm_set_method", "zfv_mm_set_crp_mode",
+    "zfv_mm_set_splitting", "zfv_mm_set_flags", "zfv_mm_refine", "zfv_mm_grad",
     "zfv_adjoint_forward", "zfv_adjoint_backward",
     "zfv_mm_configure", "zfv_mm_set_crp", "zfv_mm_set_max_dt", "zfv_mm_set_state", "zfv_mm_state", "zfv_mm_step", "zfv_mm_dt", "zfv_mm_crp_faces",
     "zfv_mm_interface",
@@ -109,6 +110,9 @@ def lib():
         L.zfv_mm_interface.argtypes = [vp] * 4
         L.zfv_mm_set_limiter.argtypes = [vp, C.c_int]
         L.zfv_mm_set_accuracy.argtypes = [vp, C.c_int]
+        L.zfv_mm_set_method.argtypes = [vp, C.c_int]
+        L.zfv_mm_set_crp_mode.argtypes = [vp, C.c_int]
+        L.zfv_mm_set_splitting.argtypes = [vp, C.c_int]
         L.zfv_mm_set_flags.argtypes = [vp]
         L.zfv_mm_refine.argtypes = [vp, C.c_int]
         L.zfv_mm_grad.argtypes = [vp, vp, vp]
@@ -379,6 +383,24 @@ class Context:
         """1: the first-order update (default); 2: MmFluid's second order
         (compute_grad, fluxes_stage1 / fluxes_stage2)."""
         _check(lib().zfv_mm_set_accuracy(self.h, int(acc)))
+
+    MM_FLUXES = {"Godunov": 0, "Rusanov": 1, "HLL": 2, "HLLC": 3, "CRP": 8}   # Fluxes (num_flux.h:11-21)
+    CRP_MODES = ("NONE", "V3", "V5", "VS", "PLIC", "MUSCL")                  # CrpMode (mm_fluid.h:36-42)
+    SPLITS = ("NONE", "SIMPLE", "STRANG")                                    # DirSplit (mm_fluid.h:29-33)
+
+    def mm_set_method(self, flux):
+        """MmFluid::set_method: "CRP" (PLIC + HLLC) or a plain mmf flux
+        ("HLLC", "HLL", "Rusanov") without composite fluxes."""
+        _check(lib().zfv_mm_set_method(self.h, self.MM_FLUXES[flux]))
+
+    def mm_set_crp_mode(self, mode):
+        """MmFluid::set_crp_mode by name ("V3", "V5", "VS", "PLIC"; "NONE" is
+        ignored, as in the reference)."""
+        _check(lib().zfv_mm_set_crp_mode(self.h, self.CRP_MODES.index(mode)))
+
+    def mm_set_splitting(self, split):
+        """MmFluid::set_splitting: "NONE", "SIMPLE" or "STRANG"."""
+        _check(lib().zfv_mm_set_splitting(self.h, self.SPLITS.index(split)))
 
     def mm_set_flags(self):
         """MmFluid::set_flags into the mesh's flags (and the gradients kept for mm_refine)."""

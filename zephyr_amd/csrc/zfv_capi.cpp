@@ -166,7 +166,9 @@ struct zfv_ctx {
         bool configured = false, have_state = false;
         long long epoch = -1;   // mesh epoch of the state
         mm::Mix mix{};
-        int crp = 1;   // CrpMode::PLIC (mm_fluid.cpp:18); 0 = CrpMode::NONE
+        int crp = ZFV_CRP_PLIC;   // CrpMode (mm_fluid.cpp:18)
+        int flux = ZFV_HLLC;      // m_nf of the plain faces
+        int split = ZFV_SPLIT_NONE;
         double cfl = 0.7, max_dt = 1.0e300;   // MmFluid defaults (mm_fluid.cpp:16-26)
         int cur = 0;
         long long step_no = 0;
@@ -1785,7 +1787,7 @@ static int mm_alloc(zfv_ctx* ctx) {
     auto& S = ctx->mm;
     const size_t rows = 7 + 2 * (size_t)S.mix.n;
     for (auto& b : S.st) ZFV_HIP(b.alloc(rows * (size_t)ctx->cap));
-    ZFV_HIP(S.dtmin.alloc(3));
+    ZFV_HIP(S.dtmin.alloc(4));   // three rotating slots and the sink of the non-final sweeps' candidates
     ZFV_HIP(S.dt_out.alloc(1));
     ZFV_HIP(S.dmax_dt.alloc(1));
     unsigned long long bits;
@@ -1827,7 +1829,46 @@ int zfv_mm_set_crp(zfv_ctx* ctx, int mode) {
     int r = mm_ready(ctx, false);
     if (r) return r;
     if (mode != 0 && mode != 1) return fail(ZFV_ERR_ARG, "mm_set_crp: 0 (NONE) or 1 (PLIC)");
-    ctx->mm.crp = mode;
+    ctx->mm.crp = mode ? ZFV_CRP_PLIC : ZFV_CRP_NONE;
+    ctx->mm.flux = ZFV_HLLC;
+    return ZFV_OK;
+}
+
+int zfv_mm_set_method(zfv_ctx* ctx, int flux) {
+    int r = mm_ready(ctx, false);
+    if (r) return r;
+    if (flux == ZFV_CRP) {
+        ctx->mm.crp = ZFV_CRP_PLIC;
+        ctx->mm.flux = ZFV_HLLC;
+    } else if (flux == ZFV_HLLC || flux == ZFV_HLL || flux == ZFV_RUSANOV) {
+        ctx->mm.crp = ZFV_CRP_NONE;
+        ctx->mm.flux = flux;
+    } else if (flux == ZFV_GODUNOV || (flux > ZFV_HLLC && flux < ZFV_CRP)) {
+        return fail(ZFV_ERR_UNSUPPORTED, "mm_set_method: only CRP, HLLC, HLL and Rusanov are built");
+    } else {
+        return fail(ZFV_ERR_ARG, "mm_set_method: unknown flux");
+    }
+    return ZFV_OK;
+}
+
+int zfv_mm_set_crp_mode(zfv_ctx* ctx, int mode) {
+    int r = mm_ready(ctx, false);
+    if (r) return r;
+    if (mode == ZFV_CRP_MUSCL)
+        return fail(ZFV_ERR_UNSUPPORTED, "mm_set_crp_mode: MUSCL has no face fraction in the reference");
+    if (mode < ZFV_CRP_NONE || mode > ZFV_CRP_PLIC) return fail(ZFV_ERR_ARG, "mm_set_crp_mode: unknown CrpMode");
+    if (mode != ZFV_CRP_NONE) {   // set_crp_mode ignores NONE (mm_fluid.cpp:63-70)
+        ctx->mm.crp = mode;
+        ctx->mm.flux = ZFV_HLLC;
+    }
+    return ZFV_OK;
+}
+
+int zfv_mm_set_splitting(zfv_ctx* ctx, int split) {
+    int r = mm_ready(ctx, false);
+    if (r) return r;
+    if (split < ZFV_SPLIT_NONE || split > ZFV_SPLIT_STRANG) return fail(ZFV_ERR_ARG, "mm_set_splitting: 0, 1 or 2");
+    ctx->mm.split = split;
     return ZFV_OK;
 }
 
@@ -1908,52 +1949,69 @@ int zfv_mm_step(zfv_ctx* ctx, int n_steps) {
         if (S.crp_list.n != 8 * cap) ZFV_HIP(S.crp_list.alloc(8 * cap));
         if (S.crp_count.n != 1) ZFV_HIP(S.crp_count.alloc(1));
     }
+    // MmFluid::update (mm_fluid.cpp:112-137): the sweeps of one step
+    struct Sweep { int dir; double scale; };
+    const Sweep none[] = {{0, 1.0}}, simple[] = {{1, 1.0}, {2, 1.0}}, strang[] = {{1, 0.5}, {2, 1.0}, {1, 0.5}};
+    const Sweep* sweeps = S.split == ZFV_SPLIT_SIMPLE ? simple : S.split == ZFV_SPLIT_STRANG ? strang : none;
+    const int n_sweeps = S.split == ZFV_SPLIT_SIMPLE ? 2 : S.split == ZFV_SPLIT_STRANG ? 3 : 1;
+    const bool plic = S.crp == ZFV_CRP_PLIC;
     for (int k = 0; k < n_steps; ++k) {
-        StepCtl c;
-        c.dtmin = S.dtmin.p;
-        c.rd = (int)(S.step_no % 3);
-        c.wr = (int)((S.step_no + 1) % 3);
-        c.clr = (int)((S.step_no + 2) % 3);
-        c.cfl = S.cfl;
-        c.max_dt = S.dmax_dt.p;
-        c.dt_out = S.dt_out.p;
-        c.err = ctx->err.p;
-        const MmLayer in = mm_layer(ctx, S.cur), out = mm_layer(ctx, 1 - S.cur);
-        const MmLayer half = acc == 2 ? mm_rows(ctx, S.half.p) : in;
-        const double* CF = nullptr;
-        if (acc == 2) {   // compute_grad of init (mm_fluid.cpp:154)
-            ZFV_HIP(timed(ctx, "mm_grad", [&] {
-                return mm_grad(M, in, S.mix, S.crp, S.limiter, S.gx.p, S.gy.p, ctx->stream);
-            }));
+        for (int j = 0; j < n_sweeps; ++j) {
+            StepCtl c;
+            c.dtmin = S.dtmin.p;
+            c.rd = (int)(S.step_no % 3);
+            // only the last sweep's state is the next step's: earlier sweeps
+            // send their CFL candidates to the unread fourth slot
+            c.wr = j == n_sweeps - 1 ? (int)((S.step_no + 1) % 3) : 3;
+            c.clr = (int)((S.step_no + 2) % 3);
+            c.cfl = S.cfl;
+            c.max_dt = S.dmax_dt.p;
+            c.dt_out = S.dt_out.p;
+            c.err = ctx->err.p;
+            MmOpts o;
+            o.flux = S.flux;
+            o.crp = S.crp;
+            o.dir = sweeps[j].dir;
+            o.dt_scale = sweeps[j].scale;
+            const MmLayer in = mm_layer(ctx, S.cur), out = mm_layer(ctx, 1 - S.cur);
+            const MmLayer half = acc == 2 ? mm_rows(ctx, S.half.p) : in;
+            const double* CF = nullptr;
+            if (acc == 2) {   // compute_grad of init (mm_fluid.cpp:154)
+                ZFV_HIP(timed(ctx, "mm_grad", [&] {
+                    return mm_grad(M, in, S.mix, S.crp, S.limiter, S.gx.p, S.gy.p, ctx->stream);
+                }));
+            }
+            double* ir = S.inter.p;
+            if (plic) {   // interface_recovery (mm_fluid.cpp:143-146, 160-163)
+                ZFV_HIP(timed(ctx, "mm_interface", [&] {
+                    return mm_interface(M, in, S.mix, ctx->vx.p, ctx->vy.p, ir, ir + nm * cap, ir + 2 * nm * cap,
+                                        ctx->stream);
+                }));
+            }
+            if (acc == 2) {   // fluxes_stage1 (:165)
+                ZFV_HIP(timed(ctx, "mm_stage1", [&] {
+                    return mm_stage1(M, in, half, S.mix, c, o, S.gx.p, S.gy.p, ctx->stream);
+                }));
+            }
+            if (S.crp) {   // the composite fluxes of the sweep's faces
+                ZFV_HIP(timed(ctx, "mm_crp_mark", [&] {
+                    return mm_crp_mark(M, in, half, S.mix, acc, o, S.gx.p, S.gy.p, S.crp_list.p, S.crp_count.p,
+                                       ctx->stream);
+                }));
+                ZFV_HIP(timed(ctx, "mm_crp", [&] {
+                    return mm_crp(M, in, half, S.mix, acc, o, S.gx.p, S.gy.p, c, ir, ir + nm * cap, S.crp_list.p,
+                                  S.crp_count.p, S.cf.p, ctx->stream);
+                }));
+                CF = S.cf.p;
+            }
+            if (acc == 2)   // fluxes_stage2 (:167)
+                ZFV_HIP(timed(ctx, "mm_flux2", [&] {
+                    return mm_flux2(M, in, half, out, S.mix, c, o, S.gx.p, S.gy.p, CF, ctx->stream);
+                }));
+            else
+                ZFV_HIP(timed(ctx, "mm_flux1", [&] { return mm_flux1(M, in, out, S.mix, c, o, CF, ctx->stream); }));
+            S.cur = 1 - S.cur;   // MmFluid::swap, after every integrate
         }
-        double* o = S.inter.p;
-        if (S.crp) {   // interface_recovery (mm_fluid.cpp:130-137, 160-163)
-            ZFV_HIP(timed(ctx, "mm_interface", [&] {
-                return mm_interface(M, in, S.mix, ctx->vx.p, ctx->vy.p, o, o + nm * cap, o + 2 * nm * cap, ctx->stream);
-            }));
-        }
-        if (acc == 2) {   // fluxes_stage1 (:165)
-            ZFV_HIP(timed(ctx, "mm_stage1", [&] {
-                return mm_stage1(M, in, half, S.mix, c, S.gx.p, S.gy.p, ctx->stream);
-            }));
-        }
-        if (S.crp) {   // the composite fluxes of the step's faces
-            ZFV_HIP(timed(ctx, "mm_crp_mark", [&] {
-                return mm_crp_mark(M, in, half, S.mix, acc, S.gx.p, S.gy.p, S.crp_list.p, S.crp_count.p, ctx->stream);
-            }));
-            ZFV_HIP(timed(ctx, "mm_crp", [&] {
-                return mm_crp(M, in, half, S.mix, acc, S.gx.p, S.gy.p, c, o, o + nm * cap, S.crp_list.p, S.crp_count.p,
-                              S.cf.p, ctx->stream);
-            }));
-            CF = S.cf.p;
-        }
-        if (acc == 2)   // fluxes_stage2 (:167)
-            ZFV_HIP(timed(ctx, "mm_flux2", [&] {
-                return mm_flux2(M, in, half, out, S.mix, c, S.gx.p, S.gy.p, CF, ctx->stream);
-            }));
-        else
-            ZFV_HIP(timed(ctx, "mm_flux1", [&] { return mm_flux1(M, in, out, S.mix, c, CF, ctx->stream); }));
-        S.cur = 1 - S.cur;
         S.step_no += 1;
     }
     // as the reference's d_dx / d_dy, the gradients stay those of the last

@@ -189,26 +189,37 @@ constexpr int kBlock = 256;
 struct MmLayer {
     double *r, *u, *v, *P, *e, *T, *c, *b, *rh;
 };
+// Options of one multi-material sweep: the flux of the plain faces (the
+// reference's Fluxes numbering: 1 Rusanov, 2 HLL, 3 HLLC), the CrpMode (0
+// NONE, 1 V3, 2 V5, 3 VS, 4 PLIC), the faces swept (EuCell::faces(dir):
+// 0 all, 1 |n.x| >= 0.7, 2 |n.y| >= 0.7) and the factor on the step's dt
+// (0.5 for STRANG's X sweeps).
+struct MmOpts {
+    int flux = 3, crp = 4, dir = 0;
+    double dt_scale = 1.0;
+};
 hipError_t mm_prepare(const DevMesh& M, const MmLayer& L, const mm::Mix& mx, unsigned long long* slot,
                       hipStream_t s);
 // CF: composite Riemann fluxes per [slot][4 + n_mat][stride] from mm_crp,
 // or null without composite Riemann problems.
 hipError_t mm_flux1(const DevMesh& M, const MmLayer& in, const MmLayer& out, const mm::Mix& mx,
-                    const StepCtl& ctl, const double* CF, hipStream_t s);
+                    const StepCtl& ctl, const MmOpts& o, const double* CF, hipStream_t s);
 // The faces that take composite Riemann fluxes: list (8 * stride ints of
 // work space) and their number (one counter), then the fluxes into CF.
 // acc 1 reads the cells of `in`; acc 2 the face states extrapolated from
 // `half` with the gradients gx / gy ([6 + 2 n_mat][stride], k_mm_grad).
 hipError_t mm_crp_mark(const DevMesh& M, const MmLayer& in, const MmLayer& half, const mm::Mix& mx, int acc,
-                       const double* gx, const double* gy, int* list, unsigned int* count, hipStream_t s);
+                       const MmOpts& o, const double* gx, const double* gy, int* list, unsigned int* count,
+                       hipStream_t s);
 hipError_t mm_crp(const DevMesh& M, const MmLayer& in, const MmLayer& half, const mm::Mix& mx, int acc,
-                  const double* gx, const double* gy, const StepCtl& ctl, const double* nx, const double* ny,
-                  const int* list, const unsigned int* count, double* CF, hipStream_t s);
+                  const MmOpts& o, const double* gx, const double* gy, const StepCtl& ctl, const double* nx,
+                  const double* ny, const int* list, const unsigned int* count, double* CF, hipStream_t s);
 // Second order: the predictor into `half`, then the corrector into `out`.
 hipError_t mm_stage1(const DevMesh& M, const MmLayer& in, const MmLayer& half, const mm::Mix& mx, const StepCtl& ctl,
-                     const double* gx, const double* gy, hipStream_t s);
+                     const MmOpts& o, const double* gx, const double* gy, hipStream_t s);
 hipError_t mm_flux2(const DevMesh& M, const MmLayer& in, const MmLayer& half, const MmLayer& out, const mm::Mix& mx,
-                    const StepCtl& ctl, const double* gx, const double* gy, const double* CF, hipStream_t s);
+                    const StepCtl& ctl, const MmOpts& o, const double* gx, const double* gy, const double* CF,
+                    hipStream_t s);
 hipError_t mm_interface(const DevMesh& M, const MmLayer& L, const mm::Mix& mx, const double* vx, const double* vy,
                         double* nx, double* ny, double* p, hipStream_t s);
 // Adaptation of the multi-material state: the limited gradient (rows rho,

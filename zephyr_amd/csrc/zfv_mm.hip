@@ -79,6 +79,13 @@ __device__ __forceinline__ void rotate_uv(const Frame& f, MZ<NM>& z) {
     z.v = w.v;
 }
 
+// EuCell::faces(dir) (amr_faces.cpp:186-197): whether slot so is swept.
+__device__ __forceinline__ bool mm_swept(const DevMesh& M, int so, int dir) {
+    if (dir == 1) return !(fabs(M.nx[so]) < 0.7);
+    if (dir == 2) return !(fabs(M.ny[so]) < 0.7);
+    return true;
+}
+
 // The new state from the conserved vector q: PState(q, mixture, P0, T0,
 // rhos0) (models.cpp:187-201) with the cell's previous P, T and true
 // densities as the closure's starting point.  The sound speed is left to
@@ -248,7 +255,7 @@ __device__ __forceinline__ bool crp_face(const double* CF, const MZ<NM>& zm, con
 template <int NM, int ACC>
 __global__ void __launch_bounds__(kBlock) k_mm_crp_mark(DevMesh M, MmLayer in, MmLayer half,
                                                         const double* __restrict__ gx, const double* __restrict__ gy,
-                                                        int* __restrict__ list, unsigned int* count) {
+                                                        int dir, int* __restrict__ list, unsigned int* count) {
     const Tile tl = tile_of(M, (int)blockDim.x);
     const int c = tl.c0 + (int)threadIdx.x;
     if (c >= tl.end) return;
@@ -256,7 +263,7 @@ __global__ void __launch_bounds__(kBlock) k_mm_crp_mark(DevMesh M, MmLayer in, M
     const int code = M.code[c];
     for (int s = 0; s < 8; ++s) {
         const int t = (code >> (4 * s)) & 7;
-        if (t == SC_UNDEF) continue;
+        if (t == SC_UNDEF || !mm_swept(M, s * st + c, dir)) continue;
         if (mm_crp_slot<NM, ACC>(M, in, half, gx, gy, c, s * st + c, t)) list[atomicAdd(count, 1u)] = s * st + c;
     }
 }
@@ -270,7 +277,7 @@ __global__ void __launch_bounds__(kBlock) k_mm_crp_mark(DevMesh M, MmLayer in, M
 // registers than the plain HLLC faces of k_mm_flux1, hence the compaction.
 template <int NM, int ACC>
 __global__ void __launch_bounds__(64) k_mm_crp(DevMesh M, MmLayer in, MmLayer half, const double* __restrict__ gx,
-                                            const double* __restrict__ gy, Mix mx, StepCtl ctl,
+                                            const double* __restrict__ gy, Mix mx, StepCtl ctl, MmOpts opt,
                                             const double* __restrict__ inx, const double* __restrict__ iny,
                                             const int* __restrict__ list, const unsigned int* count,
                                             double* __restrict__ CF) {
@@ -288,7 +295,7 @@ __global__ void __launch_bounds__(64) k_mm_crp(DevMesh M, MmLayer in, MmLayer ha
     for (int e = (int)blockIdx.x * L + lane; lane < L && e < n_faces; e += (int)gridDim.x * L) {
         const int so = list[e];
         const int s = so / st, c = so - s * st;
-        const double dt = step_dt(ctl, 0);
+        const double dt = opt.dt_scale * step_dt(ctl, 0);
         const int nib = (M.code[c] >> (4 * s)) & 15;
         const int t = nib & 7;
         const bool bound = t > SC_PERIODIC;
@@ -303,18 +310,24 @@ __global__ void __launch_bounds__(64) k_mm_crp(DevMesh M, MmLayer in, MmLayer ha
         const double h_R = (bound ? M.vol[c] : M.vol[nb]) / S;
         const int iA = master_component<NM>(zm, zp);
         const double a_L = alpha_of<NM>(zm, iA), a_R = alpha_of<NM>(zp, iA);
-        const double vi = 0.5 * (zm.u + zp.u);
-        const bool upwind = vi > 0.0;
-        const double CFL = dt * fabs(vi) / (upwind ? h_L : h_R);
-        double ncx = 0.0, ncy = 0.0;
+        double a_sig;
+        if (opt.crp == 4) {   // PLIC
+            const double vi = 0.5 * (zm.u + zp.u);
+            const bool upwind = vi > 0.0;
+            const double CFL = dt * fabs(vi) / (upwind ? h_L : h_R);
+            double ncx = 0.0, ncy = 0.0;
 #pragma unroll
-        for (int i = 0; i < NM; ++i)
-            if (i == iA) {
-                ncx = upwind ? inx[i * st + c] : -inx[i * st + nb];
-                ncy = upwind ? iny[i * st + c] : -iny[i * st + nb];
-            }
-        const double cs = M.nx[so] * ncx + M.ny[so] * ncy;
-        const double a_sig = f_between(average_flux(upwind ? a_L : a_R, cs, CFL), a_L, a_R);
+            for (int i = 0; i < NM; ++i)
+                if (i == iA) {
+                    ncx = upwind ? inx[i * st + c] : -inx[i * st + nb];
+                    ncy = upwind ? iny[i * st + c] : -iny[i * st + nb];
+                }
+            const double cs = M.nx[so] * ncx + M.ny[so] * ncy;
+            a_sig = f_between(average_flux(upwind ? a_L : a_R, cs, CFL), a_L, a_R);
+        } else {   // the heuristic templates V3 / V5 / VS
+            a_sig = opt.crp == 1 ? face_fraction_v3(a_L, a_R)
+                  : opt.crp == 2 ? face_fraction_v5(a_L, a_R) : face_fraction_s(a_L, a_R);
+        }
         double F[K];
         calc_crp_flux<NM>(zm, zp, h_L, h_R, iA, a_sig, dt, mx, F, bad);
         const Q g = frame_global(fr, Q{F[0], F[1], F[2], F[3]});
@@ -333,11 +346,12 @@ __global__ void __launch_bounds__(64) k_mm_crp(DevMesh M, MmLayer in, MmLayer ha
 // (models.cpp:187-201), its sound speed and the next step's CFL candidate.
 template <int NM>
 __global__ void __launch_bounds__(kBlock) k_mm_flux1(DevMesh M, MmLayer in, MmLayer out, Mix mx, StepCtl ctl,
-                                                     const double* __restrict__ CF) {
+                                                     MmOpts opt, const double* __restrict__ CF) {
     const Tile tl = tile_of(M, (int)blockDim.x);
     const int c = tl.c0 + (int)threadIdx.x;
-    const double dt = step_dt(ctl, tl.pid);
-    step_prologue(ctl, tl, dt);
+    const double step = step_dt(ctl, tl.pid);
+    step_prologue(ctl, tl, step);
+    const double dt = opt.dt_scale * step;
     double cand = INFINITY;
     if (c < tl.end) {
         const int st = M.stride;
@@ -351,7 +365,7 @@ __global__ void __launch_bounds__(kBlock) k_mm_flux1(DevMesh M, MmLayer in, MmLa
         for (int s = 0; s < 8; ++s) {
             const int nib = (code >> (4 * s)) & 15;
             const int t = nib & 7;
-            if (t == SC_UNDEF) continue;
+            if (t == SC_UNDEF || !mm_swept(M, s * st + c, opt.dir)) continue;
             const int so = s * st + c;
             const Frame fr = slot_frame(M, c, s, nib);
             MZ<NM> zp;
@@ -371,7 +385,7 @@ __global__ void __launch_bounds__(kBlock) k_mm_flux1(DevMesh M, MmLayer in, MmLa
 #pragma unroll
                 for (int k = 0; k < 4 + NM; ++k) F[k] = CF[(s * (4 + NM) + k) * st + c];
             } else {
-                hllc_mm<NM>(zm, zp, F, bad);
+                plain_flux_mm<NM>(opt.flux, zm, zp, F, bad);
                 const Q g = frame_global(fr, Q{F[0], F[1], F[2], F[3]});
                 F[1] = g.mx;
                 F[2] = g.my;
@@ -401,7 +415,8 @@ __global__ void __launch_bounds__(kBlock) k_mm_flux1(DevMesh M, MmLayer in, MmLa
 // face states, and re-closes from its previous P, T and true densities.
 template <int NM>
 __global__ void __launch_bounds__(kBlock) k_mm_stage1(DevMesh M, MmLayer in, MmLayer half, Mix mx, StepCtl ctl,
-                                                      const double* __restrict__ gx, const double* __restrict__ gy) {
+                                                      MmOpts opt, const double* __restrict__ gx,
+                                                      const double* __restrict__ gy) {
     const Tile tl = tile_of(M, (int)blockDim.x);
     const int c = tl.c0 + (int)threadIdx.x;
     if (c >= tl.end) return;
@@ -411,7 +426,7 @@ __global__ void __launch_bounds__(kBlock) k_mm_stage1(DevMesh M, MmLayer in, MmL
         store_mz<NM>(half, c, st, zc);
         return;
     }
-    const double dt = step_dt(ctl, tl.pid);
+    const double dt = opt.dt_scale * step_dt(ctl, tl.pid);
     const double ccx = M.cx[c], ccy = M.cy[c];
     const int code = M.code[c];
     double acc[4 + NM];
@@ -420,7 +435,7 @@ __global__ void __launch_bounds__(kBlock) k_mm_stage1(DevMesh M, MmLayer in, MmL
 #pragma unroll 1
     for (int s = 0; s < 8; ++s) {
         const int nib = (code >> (4 * s)) & 15;
-        if ((nib & 7) == SC_UNDEF) continue;
+        if ((nib & 7) == SC_UNDEF || !mm_swept(M, s * st + c, opt.dir)) continue;
         const int so = s * st + c;
         const Frame fr = slot_frame(M, c, s, nib);
         MZ<NM> zm = mm_face_state<NM>(in, mx, gx, gy, c, st, M.fcx[so] - ccx, M.fcy[so] - ccy, false);
@@ -449,12 +464,13 @@ __global__ void __launch_bounds__(kBlock) k_mm_stage1(DevMesh M, MmLayer in, MmL
 // true densities, the sound speed and the next step's CFL candidate.
 template <int NM>
 __global__ void __launch_bounds__(kBlock) k_mm_flux2(DevMesh M, MmLayer in, MmLayer half, MmLayer out, Mix mx,
-                                                     StepCtl ctl, const double* __restrict__ gx,
+                                                     StepCtl ctl, MmOpts opt, const double* __restrict__ gx,
                                                      const double* __restrict__ gy, const double* __restrict__ CF) {
     const Tile tl = tile_of(M, (int)blockDim.x);
     const int c = tl.c0 + (int)threadIdx.x;
-    const double dt = step_dt(ctl, tl.pid);
-    step_prologue(ctl, tl, dt);
+    const double step = step_dt(ctl, tl.pid);
+    step_prologue(ctl, tl, step);
+    const double dt = opt.dt_scale * step;
     double cand = INFINITY;
     if (c < tl.end) {
         const int st = M.stride;
@@ -467,7 +483,7 @@ __global__ void __launch_bounds__(kBlock) k_mm_flux2(DevMesh M, MmLayer in, MmLa
         for (int s = 0; s < 8; ++s) {
             const int nib = (code >> (4 * s)) & 15;
             const int t = nib & 7;
-            if (t == SC_UNDEF) continue;
+            if (t == SC_UNDEF || !mm_swept(M, s * st + c, opt.dir)) continue;
             const int so = s * st + c;
             double F[4 + NM];
             if (CF != nullptr && mm_crp_slot<NM, 2>(M, in, half, gx, gy, c, so, t)) {
@@ -479,7 +495,7 @@ __global__ void __launch_bounds__(kBlock) k_mm_flux2(DevMesh M, MmLayer in, MmLa
                 mm_face_pair<NM, 2>(M, in, half, mx, gx, gy, c, so, t, fr, zm, zp);
                 rotate_uv<NM>(fr, zm);
                 rotate_uv<NM>(fr, zp);
-                hllc_mm<NM>(zm, zp, F, bad);
+                plain_flux_mm<NM>(opt.flux, zm, zp, F, bad);
                 const Q g = frame_global(fr, Q{F[0], F[1], F[2], F[3]});
                 F[1] = g.mx;
                 F[2] = g.my;
@@ -944,8 +960,8 @@ hipError_t mm_prepare(const DevMesh& M, const MmLayer& L, const Mix& mx, unsigne
 }
 
 hipError_t mm_flux1(const DevMesh& M, const MmLayer& in, const MmLayer& out, const Mix& mx, const StepCtl& ctl,
-                    const double* CF, hipStream_t s) {
-    ZFV_MM_DISPATCH(mx.n, k_mm_flux1, mm_grid(M.n), dim3(kBlock), 0, s, M, in, out, mx, ctl, CF);
+                    const MmOpts& o, const double* CF, hipStream_t s) {
+    ZFV_MM_DISPATCH(mx.n, k_mm_flux1, mm_grid(M.n), dim3(kBlock), 0, s, M, in, out, mx, ctl, o, CF);
     return hipGetLastError();
 }
 
@@ -966,32 +982,34 @@ __global__ void __launch_bounds__(64) k_zero_u32(unsigned int* p) {
     }
 
 hipError_t mm_crp_mark(const DevMesh& M, const MmLayer& in, const MmLayer& half, const Mix& mx, int acc,
-                       const double* gx, const double* gy, int* list, unsigned int* count, hipStream_t s) {
+                       const MmOpts& o, const double* gx, const double* gy, int* list, unsigned int* count,
+                       hipStream_t s) {
     // the counter is cleared by a kernel, not a memset node, so the step can be captured
     hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, s, count);
-    ZFV_MM_DISPATCH_ACC(mx.n, acc, k_mm_crp_mark, mm_grid(M.n), dim3(kBlock), 0, s, M, in, half, gx, gy, list,
-                        count);
+    ZFV_MM_DISPATCH_ACC(mx.n, acc, k_mm_crp_mark, mm_grid(M.n), dim3(kBlock), 0, s, M, in, half, gx, gy, o.dir,
+                        list, count);
     return hipGetLastError();
 }
 
 hipError_t mm_crp(const DevMesh& M, const MmLayer& in, const MmLayer& half, const Mix& mx, int acc,
-                  const double* gx, const double* gy, const StepCtl& ctl, const double* nx, const double* ny,
-                  const int* list, const unsigned int* count, double* CF, hipStream_t s) {
+                  const MmOpts& o, const double* gx, const double* gy, const StepCtl& ctl, const double* nx,
+                  const double* ny, const int* list, const unsigned int* count, double* CF, hipStream_t s) {
     // 2048 one-wavefront workgroups, grid-stride over any list length
-    ZFV_MM_DISPATCH_ACC(mx.n, acc, k_mm_crp, dim3(2048), dim3(64), 0, s, M, in, half, gx, gy, mx, ctl, nx, ny, list,
-                        count, CF);
+    ZFV_MM_DISPATCH_ACC(mx.n, acc, k_mm_crp, dim3(2048), dim3(64), 0, s, M, in, half, gx, gy, mx, ctl, o, nx, ny,
+                        list, count, CF);
     return hipGetLastError();
 }
 
 hipError_t mm_stage1(const DevMesh& M, const MmLayer& in, const MmLayer& half, const Mix& mx, const StepCtl& ctl,
-                     const double* gx, const double* gy, hipStream_t s) {
-    ZFV_MM_DISPATCH(mx.n, k_mm_stage1, mm_grid(M.n), dim3(kBlock), 0, s, M, in, half, mx, ctl, gx, gy);
+                     const MmOpts& o, const double* gx, const double* gy, hipStream_t s) {
+    ZFV_MM_DISPATCH(mx.n, k_mm_stage1, mm_grid(M.n), dim3(kBlock), 0, s, M, in, half, mx, ctl, o, gx, gy);
     return hipGetLastError();
 }
 
 hipError_t mm_flux2(const DevMesh& M, const MmLayer& in, const MmLayer& half, const MmLayer& out, const Mix& mx,
-                    const StepCtl& ctl, const double* gx, const double* gy, const double* CF, hipStream_t s) {
-    ZFV_MM_DISPATCH(mx.n, k_mm_flux2, mm_grid(M.n), dim3(kBlock), 0, s, M, in, half, out, mx, ctl, gx, gy, CF);
+                    const StepCtl& ctl, const MmOpts& o, const double* gx, const double* gy, const double* CF,
+                    hipStream_t s) {
+    ZFV_MM_DISPATCH(mx.n, k_mm_flux2, mm_grid(M.n), dim3(kBlock), 0, s, M, in, half, out, mx, ctl, o, gx, gy, CF);
     return hipGetLastError();
 }
 

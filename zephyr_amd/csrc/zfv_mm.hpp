@@ -364,6 +364,61 @@ __device__ void hllc_mm(const MZ<NM>& zL, const MZ<NM>& zR, double* F, bool& bad
     }
 }
 
+// Rusanov::calc_flux, mmf (rusanov.cpp:46-83): the largest wave speed
+// bound, zero included.
+template <int NM>
+__device__ void rusanov_mm(const MZ<NM>& zL, const MZ<NM>& zR, double* F, bool& bad) {
+    const double c_L = zL.c, c_R = zR.c;
+    bad |= isnan(c_L) || isnan(c_R);
+    const double S_L = smin(smin(zL.u - c_L, zR.u - c_R), 0.0);
+    const double S_R = smax(smax(zL.u + c_L, zR.u + c_R), 0.0);
+    const double S = smax(fabs(S_L), S_R);
+    double QL[4 + NM], QR[4 + NM], FL[4 + NM], FR[4 + NM];
+    m_qstate(zL, QL);
+    m_qstate(zR, QR);
+    m_flux(zL, FL);
+    m_flux(zR, FR);
+#pragma unroll
+    for (int k = 0; k < 4 + NM; ++k) {
+        F[k] = 0.5 * (FL[k] + FR[k]) + 0.5 * S * (QL[k] - QR[k]);
+        if (k < 4) bad |= !isfinite(F[k]);   // Flux::is_bad (models.cpp:535-541)
+    }
+}
+
+// HLL::calc_flux, mmf (hll.cpp:125-160): S_L bounded with c_L on both
+// sides, S_R with c_R, the Davis bounds if they cross; no upwind branches.
+template <int NM>
+__device__ void hll_mm(const MZ<NM>& zL, const MZ<NM>& zR, double* F, bool& bad) {
+    const double c_L = zL.c, c_R = zR.c;
+    double S_L = smin(zL.u - c_L, zR.u - c_L);
+    double S_R = smax(zL.u + c_R, zR.u + c_R);
+    if (S_L >= S_R) {
+        S_L = smin(zL.u - c_L, zR.u - c_R);
+        S_R = smax(zL.u + c_L, zR.u + c_R);
+    }
+    double QL[4 + NM], QR[4 + NM], FL[4 + NM], FR[4 + NM];
+    m_qstate(zL, QL);
+    m_qstate(zR, QR);
+    m_flux(zL, FL);
+    m_flux(zR, FR);
+#pragma unroll
+    for (int k = 0; k < 4 + NM; ++k) {
+        F[k] = (S_R * FL[k] - S_L * FR[k] + S_L * S_R * (QR[k] - QL[k])) / (S_R - S_L);
+        bad |= isnan(F[k]);
+    }
+}
+
+// The flux of the faces without a composite Riemann problem: MmFluid's m_nf
+// (HLLC by default; HLL or Rusanov after set_method with CrpMode::NONE),
+// numbered as the reference's Fluxes (num_flux.h:11-21).
+constexpr int kFluxRusanov = 1, kFluxHll = 2, kFluxHllc = 3;
+template <int NM>
+__device__ __forceinline__ void plain_flux_mm(int kind, const MZ<NM>& zL, const MZ<NM>& zR, double* F, bool& bad) {
+    if (kind == kFluxRusanov) rusanov_mm<NM>(zL, zR, F, bad);
+    else if (kind == kFluxHll) hll_mm<NM>(zL, zR, F, bad);
+    else hllc_mm<NM>(zL, zR, F, bad);
+}
+
 // ---- PLIC formulas (geom/sections.cpp:25-37, 86-104) ---------------------------
 __device__ __forceinline__ double quad_find_section(double alpha, double nx, double ny, double A, double B) {
     double a = A * fabs(nx), b = B * fabs(ny);
@@ -377,8 +432,7 @@ __device__ __forceinline__ double quad_find_section(double alpha, double nx, dou
     return +0.5 * (a + b) - sqrt(2.0 * (1.0 - alpha) * a * b);
 }
 
-__device__ __forceinline__ double face_fraction(double a1, double a2) {
-    double lo, hi;   // minmax_unit (sections.cpp:42-48)
+__device__ __forceinline__ void minmax_unit(double a1, double a2, double& lo, double& hi) {   // sections.cpp:42-48
     if (a1 < a2) {
         lo = smax(0.0, a1);
         hi = smin(a2, 1.0);
@@ -386,6 +440,37 @@ __device__ __forceinline__ double face_fraction(double a1, double a2) {
         lo = smax(0.0, a2);
         hi = smin(a1, 1.0);
     }
+}
+
+// The heuristic face fractions of CrpMode V3 / V5 / VS (sections.cpp:52-84).
+__device__ __forceinline__ double face_fraction_v3(double a1, double a2) {
+    double lo, hi;
+    minmax_unit(a1, a2, lo, hi);
+    if (lo == 0.0) return 0.0;
+    if (hi == 1.0) return 1.0;
+    if (a1 + a2 < 1.0) return hi;
+    if (a1 + a2 > 1.0) return lo;
+    return 0.5 * (a1 + a2);
+}
+__device__ __forceinline__ double face_fraction_v5(double a1, double a2) {
+    double lo, hi;
+    minmax_unit(a1, a2, lo, hi);
+    if (lo == 0.0) return 0.0;
+    if (hi == 1.0) return 1.0;
+    return 0.5 * (a1 + a2);
+}
+__device__ __forceinline__ double f_between(double x, double x1, double x2);
+__device__ __forceinline__ double face_fraction_s(double a1, double a2) {
+    double lo, hi;
+    minmax_unit(a1, a2, lo, hi);
+    double a_sig = lo / (1.0 - (hi - lo));
+    if (isnan(a_sig)) a_sig = 0.5;
+    return f_between(a_sig, lo, hi);
+}
+
+__device__ __forceinline__ double face_fraction(double a1, double a2) {
+    double lo, hi;
+    minmax_unit(a1, a2, lo, hi);
     if (3.0 * lo >= hi && lo >= 3.0 * hi - 2.0) return 0.5 * (lo + hi);
     if (hi <= 0.5 || lo * (1.0 - hi) >= (hi - 0.5) * (hi - 0.5)) return -2.0 * lo + 2.0 * sqrt(lo * (lo + hi));
     if (lo >= 0.5 || lo * (1.0 - hi) >= (lo - 0.5) * (lo - 0.5))
