@@ -289,7 +289,8 @@ int zfv_mm_refine(zfv_ctx* ctx, int max_level);
  * second-order step: those of its old layer, as the reference's d_dx /
  * d_dy): rows rho, u, v, e, beta[n_mat], P, T, rhos[n_mat], [6 + 2 n_mat][n]
  * each for x and y (NaN where a true density's stencil meets a cell without
- * that material). */
+ * that material).  zfv_mm_set_flags computes only the rows its distributor
+ * reads (rho, u, v, e, beta); the others come back as NaN after it. */
 int zfv_mm_grad(zfv_ctx* ctx, double* gx, double* gy);
 
 /* ---- launch configuration (no reference counterpart) --------------------- */

@@ -175,6 +175,7 @@ struct zfv_ctx {
         DBuf<double> st[2], half, dmax_dt, dt_out, inter, cf;
         DBuf<double> gx, gy;          // limited gradients (compute_grad), [6 + 2 n_mat][cap]
         long long grad_epoch = -1;    // mesh epoch of gx / gy
+        int grad_rows = 0;            // rows of gx / gy the last compute_grad filled
         int limiter = 3;              // MC (mm_fluid.cpp:20)
         int acc = 1;                  // m_acc (mm_fluid.h:157)
         DBuf<int> crp_list;
@@ -1978,7 +1979,7 @@ int zfv_mm_step(zfv_ctx* ctx, int n_steps) {
             const double* CF = nullptr;
             if (acc == 2) {   // compute_grad of init (mm_fluid.cpp:154)
                 ZFV_HIP(timed(ctx, "mm_grad", [&] {
-                    return mm_grad(M, in, S.mix, S.crp, S.limiter, S.gx.p, S.gy.p, ctx->stream);
+                    return mm_grad(M, in, S.mix, S.crp, S.limiter, 6 + 2 * (int)nm, S.gx.p, S.gy.p, ctx->stream);
                 }));
             }
             double* ir = S.inter.p;
@@ -2016,7 +2017,10 @@ int zfv_mm_step(zfv_ctx* ctx, int n_steps) {
     }
     // as the reference's d_dx / d_dy, the gradients stay those of the last
     // compute_grad: of the last step's old layer with acc 2
-    if (acc == 2 && n_steps > 0) S.grad_epoch = ctx->mesh_epoch;
+    if (acc == 2 && n_steps > 0) {
+        S.grad_epoch = ctx->mesh_epoch;
+        S.grad_rows = 6 + 2 * (int)nm;
+    }
     return ZFV_OK;
 }
 
@@ -2090,7 +2094,11 @@ int zfv_mm_set_flags(zfv_ctx* ctx) {
     if (r) return r;
     const DevMesh M = ctx->dmesh();
     const MmLayer L = mm_layer(ctx, S.cur);
-    ZFV_HIP(mm_grad(M, L, S.mix, S.crp, S.limiter, S.gx.p, S.gy.p, ctx->stream));
+    // the rows the split distributor reads: the flags read none, and
+    // P, T and the true densities' rows would only feed a second-order
+    // step, which computes its own
+    S.grad_rows = 4 + S.mix.n;
+    ZFV_HIP(mm_grad(M, L, S.mix, S.crp, S.limiter, S.grad_rows, S.gx.p, S.gy.p, ctx->stream));
     ZFV_HIP(mm_flags(M, L, S.mix, ctx->flag.p, ctx->stream));
     ctx->host_stale = true;
     S.grad_epoch = ctx->mesh_epoch;
@@ -2105,11 +2113,12 @@ int zfv_mm_grad(zfv_ctx* ctx, double* gx, double* gy) {
     if (!gx || !gy) return fail(ZFV_ERR_ARG, "mm_grad: null array");
     auto& S = ctx->mm;
     if (S.grad_epoch != ctx->mesh_epoch) return fail(ZFV_ERR_STATE, "no multi-material gradients for this mesh");
-    const int n = ctx->hm.n, rows = 6 + 2 * S.mix.n;
+    const int n = ctx->hm.n, rows = S.grad_rows, all = 6 + 2 * S.mix.n;
     const size_t row = (size_t)n * sizeof(double), pitch = (size_t)ctx->cap * sizeof(double);
     ZFV_HIP(hipMemcpy2DAsync(gx, row, S.gx.p, pitch, row, rows, hipMemcpyDeviceToHost, ctx->stream));
     ZFV_HIP(hipMemcpy2DAsync(gy, row, S.gy.p, pitch, row, rows, hipMemcpyDeviceToHost, ctx->stream));
     ZFV_HIP(hipStreamSynchronize(ctx->stream));
+    for (size_t k = (size_t)n * rows; k < (size_t)n * all; ++k) gx[k] = gy[k] = NAN;   // rows not computed
     return ZFV_OK;
 }
 

@@ -227,7 +227,9 @@ hipError_t mm_interface(const DevMesh& M, const MmLayer& L, const mm::Mix& mx, c
 // flags, the split / merge distributor (fam: the four children of each
 // coarsened family), the moves of the state rows, and the cached sound
 // speed with the next step's dt candidate.
-hipError_t mm_grad(const DevMesh& M, const MmLayer& L, const mm::Mix& mx, int crp, int limiter, double* gx,
+// compute_grad of the first `rows` components (rho, u, v, e, beta[n_mat],
+// P, T, rhos[n_mat]) into gx / gy ([6 + 2 n_mat][stride]).
+hipError_t mm_grad(const DevMesh& M, const MmLayer& L, const mm::Mix& mx, int crp, int limiter, int rows, double* gx,
                    double* gy, hipStream_t s);
 hipError_t mm_flags(const DevMesh& M, const MmLayer& L, const mm::Mix& mx, int* flag, hipStream_t s);
 hipError_t mm_distribute(const AmrMesh& A, const MmLayer& L, const mm::Mix& mx, const double* gx, const double* gy,

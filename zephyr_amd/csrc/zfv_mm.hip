@@ -344,7 +344,9 @@ __global__ void __launch_bounds__(64) k_mm_crp(DevMesh M, MmLayer in, MmLayer ha
 // composite flux from CF, q -= dt / V sum F S, the new state by the PT
 // closure from the cell's previous P, T and true densities
 // (models.cpp:187-201), its sound speed and the next step's CFL candidate.
-template <int NM>
+// PLAIN: 1 = the plain faces take HLLC (the default, compiled alone), 2 =
+// the flux of opt.flux.
+template <int NM, int PLAIN>
 __global__ void __launch_bounds__(kBlock) k_mm_flux1(DevMesh M, MmLayer in, MmLayer out, Mix mx, StepCtl ctl,
                                                      MmOpts opt, const double* __restrict__ CF) {
     const Tile tl = tile_of(M, (int)blockDim.x);
@@ -385,7 +387,8 @@ __global__ void __launch_bounds__(kBlock) k_mm_flux1(DevMesh M, MmLayer in, MmLa
 #pragma unroll
                 for (int k = 0; k < 4 + NM; ++k) F[k] = CF[(s * (4 + NM) + k) * st + c];
             } else {
-                plain_flux_mm<NM>(opt.flux, zm, zp, F, bad);
+                if (PLAIN == 1) hllc_mm<NM>(zm, zp, F, bad);
+                else plain_flux_mm<NM>(opt.flux, zm, zp, F, bad);
                 const Q g = frame_global(fr, Q{F[0], F[1], F[2], F[3]});
                 F[1] = g.mx;
                 F[2] = g.my;
@@ -462,7 +465,7 @@ __global__ void __launch_bounds__(kBlock) k_mm_stage1(DevMesh M, MmLayer in, MmL
 // re-closed, the mmf HLLC flux or the composite flux from CF, q -= dt / V
 // sum F S on the old conserved vector, the closure from the old P, T and
 // true densities, the sound speed and the next step's CFL candidate.
-template <int NM>
+template <int NM, int PLAIN>
 __global__ void __launch_bounds__(kBlock) k_mm_flux2(DevMesh M, MmLayer in, MmLayer half, MmLayer out, Mix mx,
                                                      StepCtl ctl, MmOpts opt, const double* __restrict__ gx,
                                                      const double* __restrict__ gy, const double* __restrict__ CF) {
@@ -495,7 +498,8 @@ __global__ void __launch_bounds__(kBlock) k_mm_flux2(DevMesh M, MmLayer in, MmLa
                 mm_face_pair<NM, 2>(M, in, half, mx, gx, gy, c, so, t, fr, zm, zp);
                 rotate_uv<NM>(fr, zm);
                 rotate_uv<NM>(fr, zp);
-                plain_flux_mm<NM>(opt.flux, zm, zp, F, bad);
+                if (PLAIN == 1) hllc_mm<NM>(zm, zp, F, bad);
+                else plain_flux_mm<NM>(opt.flux, zm, zp, F, bad);
                 const Q g = frame_global(fr, Q{F[0], F[1], F[2], F[3]});
                 F[1] = g.mx;
                 F[2] = g.my;
@@ -617,19 +621,14 @@ __device__ double limiter_rt(int kind, double a, double b) {
 // reads), then P, T, rhos[i] (the second-order face states carry every
 // PState component, face_extra.h:122-184).  A true density is NaN where its
 // material is absent, and so is its gradient wherever the stencil meets
-// such a cell.
+// such a cell.  Row k of a layer (k is uniform across the wavefront).
 template <int NM>
-__device__ __forceinline__ void mg_comps(const MmLayer& L, int c, int st, double* q) {
-    q[0] = L.r[c];
-    q[1] = L.u[c];
-    q[2] = L.v[c];
-    q[3] = L.e[c];
-#pragma unroll
-    for (int i = 0; i < NM; ++i) q[4 + i] = L.b[i * st + c];
-    q[4 + NM] = L.P[c];
-    q[5 + NM] = L.T[c];
-#pragma unroll
-    for (int i = 0; i < NM; ++i) q[6 + NM + i] = L.rh[i * st + c];
+__device__ __forceinline__ const double* mg_row(const MmLayer& L, int st, int k) {
+    if (k < 4) return k == 0 ? L.r : k == 1 ? L.u : k == 2 ? L.v : L.e;
+    if (k < 4 + NM) return L.b + (size_t)(k - 4) * st;
+    if (k == 4 + NM) return L.P;
+    if (k == 5 + NM) return L.T;
+    return L.rh + (size_t)(k - 6 - NM) * st;
 }
 
 // MmFluid::compute_grad (mm_fluid.cpp:439-466): with composite Riemann
@@ -638,22 +637,26 @@ __device__ __forceinline__ void mg_comps(const MmLayer& L, int c, int st, double
 // gradient::limiting (:412-474) with the mm boundary_value
 // (mm_fluid.cpp:98-108: walls reflect the velocity), component by
 // component as k_grad_pred does for SmFluid (the 2 x 2 block of the LSM
-// system).  gx, gy: [6 + 2 NM][stride].
-template <int NM>
-__global__ void __launch_bounds__(kBlock) k_mm_grad(DevMesh M, MmLayer L, int crp, int limiter,
+// system).  The first `rows` of the 6 + 2 NM components (all of them for a
+// second-order step, the 4 + NM the split distributor reads for
+// set_flags) go in chunks of KC through the two passes (the unlimited and
+// the limited right-hand side), so a thread holds KC components' sums
+// rather than all of them; the LSM matrix is built in the first chunk's
+// first pass.  Each component's sums run over the slots in the same order
+// as unchunked, so the result does not depend on KC.  gx, gy:
+// [6 + 2 NM][stride].
+template <int NM, int KC>
+__global__ void __launch_bounds__(kBlock) k_mm_grad(DevMesh M, MmLayer L, int crp, int limiter, int rows,
                                                     double* __restrict__ gx, double* __restrict__ gy) {
-    constexpr int K = 6 + 2 * NM;
     const int c = (int)(blockIdx.x * blockDim.x + threadIdx.x);
     if (c >= M.n) return;
     const int st = M.stride;
     const int code = M.code[c];
-    double zc[K];
-    mg_comps<NM>(L, c, st, zc);
     bool zero = false;
     if (crp) {
         double b[NM];
 #pragma unroll
-        for (int i = 0; i < NM; ++i) b[i] = zc[4 + i];
+        for (int i = 0; i < NM; ++i) b[i] = L.b[i * st + c];
         zero = frac_index<NM>(b) < 0;
         for (int s = 0; s < 8 && !zero; ++s) {
             const int t = (code >> (4 * s)) & 7;
@@ -664,39 +667,33 @@ __global__ void __launch_bounds__(kBlock) k_mm_grad(DevMesh M, MmLayer L, int cr
             if (frac_index<NM>(b) < 0) zero = true;
         }
     }
-    double Gx[K], Gy[K];
-#pragma unroll
-    for (int k = 0; k < K; ++k) {
-        Gx[k] = 0.0;
-        Gy[k] = 0.0;
-    }
-    if (!zero) {
-        const double ccx = M.cx[c], ccy = M.cy[c];
-        double A00 = 0.0, A01 = 0.0, A10 = 0.0, A11 = 0.0;
-        double Fx[K], Fy[K];
-#pragma unroll
-        for (int k = 0; k < K; ++k) {
-            Fx[k] = 0.0;
-            Fy[k] = 0.0;
+    if (zero) {
+        for (int k = 0; k < rows; ++k) {
+            gx[k * st + c] = 0.0;
+            gy[k * st + c] = 0.0;
         }
-        // pass 1: the LSM system; pass 2: the limited right-hand side
-        for (int pass = 0; pass < 2; ++pass) {
-            double B00 = 0.0, B01 = 0.0, B10 = 0.0, B11 = 0.0;
-            if (pass == 1) {
-                if (fabs(A11) < 1e-8) A11 = 1.0;
-                const double det = A11 * A00 + (-A01) * A10;
-                const double invdet = 1.0 / det;
-                B00 = A11 * invdet;
-                B01 = (-A01) * invdet;
-                B10 = (-A10) * invdet;
-                B11 = A00 * invdet;
+        return;
+    }
+    const double ccx = M.cx[c], ccy = M.cy[c];
+    double A00 = 0.0, A01 = 0.0, A10 = 0.0, A11 = 0.0;
+    double B00 = 0.0, B01 = 0.0, B10 = 0.0, B11 = 0.0;
+#pragma unroll 1
+    for (int k0 = 0; k0 < rows; k0 += KC) {
+        double zc[KC], Gx[KC], Gy[KC];
 #pragma unroll
-                for (int k = 0; k < K; ++k) {
-                    Gx[k] = B00 * Fx[k] + B01 * Fy[k];
-                    Gy[k] = B10 * Fx[k] + B11 * Fy[k];
-                    Fx[k] = 0.0;
-                    Fy[k] = 0.0;
-                }
+        for (int j = 0; j < KC; ++j) {
+            zc[j] = k0 + j < rows ? mg_row<NM>(L, st, k0 + j)[c] : 0.0;
+            Gx[j] = 0.0;
+            Gy[j] = 0.0;
+        }
+        // pass 0: the unlimited right-hand side; pass 1: the limited one
+#pragma unroll 1
+        for (int pass = 0; pass < 2; ++pass) {
+            double Fx[KC], Fy[KC];
+#pragma unroll
+            for (int j = 0; j < KC; ++j) {
+                Fx[j] = 0.0;
+                Fy[j] = 0.0;
             }
 #pragma unroll 1
             for (int s = 0; s < 8; ++s) {
@@ -706,12 +703,11 @@ __global__ void __launch_bounds__(kBlock) k_mm_grad(DevMesh M, MmLayer L, int cr
                 const int so = s * st + c;
                 const Frame fr = slot_frame(M, c, s, nib);
                 const int nb = M.adj[so];
-                double zn[K];
-                if (t == SC_INNER || t == SC_PERIODIC) {
-                    mg_comps<NM>(L, nb, st, zn);
-                } else {
+                const bool inner = t == SC_INNER || t == SC_PERIODIC;
+                double zn[KC];
 #pragma unroll
-                    for (int k = 0; k < K; ++k) zn[k] = zc[k];
+                for (int j = 0; j < KC; ++j) zn[j] = (inner && k0 + j < rows) ? mg_row<NM>(L, st, k0 + j)[nb] : zc[j];
+                if (!inner && k0 == 0) {   // the wall's reflected velocity (rows 1, 2 lie in the first chunk)
                     const Z w = frame_boundary(fr, Z{zc[0], zc[1], zc[2], 0.0, zc[3]}, t == SC_WALL);
                     zn[1] = w.u;
                     zn[2] = w.v;
@@ -730,7 +726,7 @@ __global__ void __launch_bounds__(kBlock) k_mm_grad(DevMesh M, MmLayer L, int cr
                 }
                 const bool xp = fr.gen || fr.side < 2, yp = fr.gen || fr.side >= 2;
                 const double w = M.w[so];
-                if (pass == 0) {
+                if (k0 == 0 && pass == 0) {   // the LSM matrix
                     if (xp) {
                         const double wd = w * dnx;
                         A00 = A00 + wd * drx;
@@ -743,30 +739,38 @@ __global__ void __launch_bounds__(kBlock) k_mm_grad(DevMesh M, MmLayer L, int cr
                     }
                 }
 #pragma unroll
-                for (int k = 0; k < K; ++k) {
-                    const double dF = zn[k] - zc[k];
+                for (int j = 0; j < KC; ++j) {
+                    const double dF = zn[j] - zc[j];
                     double term = dF;
                     if (pass == 1) {
-                        const double du_n = Gx[k] * drx + Gy[k] * dry;
+                        const double du_n = Gx[j] * drx + Gy[j] * dry;
                         term = limiter_rt(limiter, 2.0 * du_n - dF, dF) * dF;
                     }
-                    if (xp) Fx[k] = Fx[k] + (w * term) * dnx;
-                    if (yp) Fy[k] = Fy[k] + (w * term) * dny;
+                    if (xp) Fx[j] = Fx[j] + (w * term) * dnx;
+                    if (yp) Fy[j] = Fy[j] + (w * term) * dny;
                 }
             }
-            if (pass == 1) {
+            if (k0 == 0 && pass == 0) {   // its inverse
+                if (fabs(A11) < 1e-8) A11 = 1.0;
+                const double det = A11 * A00 + (-A01) * A10;
+                const double invdet = 1.0 / det;
+                B00 = A11 * invdet;
+                B01 = (-A01) * invdet;
+                B10 = (-A10) * invdet;
+                B11 = A00 * invdet;
+            }
 #pragma unroll
-                for (int k = 0; k < K; ++k) {
-                    Gx[k] = B00 * Fx[k] + B01 * Fy[k];
-                    Gy[k] = B10 * Fx[k] + B11 * Fy[k];
-                }
+            for (int j = 0; j < KC; ++j) {
+                Gx[j] = B00 * Fx[j] + B01 * Fy[j];
+                Gy[j] = B10 * Fx[j] + B11 * Fy[j];
             }
         }
-    }
 #pragma unroll
-    for (int k = 0; k < K; ++k) {
-        gx[k * st + c] = Gx[k];
-        gy[k * st + c] = Gy[k];
+        for (int j = 0; j < KC; ++j)
+            if (k0 + j < rows) {
+                gx[(k0 + j) * st + c] = Gx[j];
+                gy[(k0 + j) * st + c] = Gy[j];
+            }
     }
 }
 
@@ -954,21 +958,8 @@ static inline dim3 mm_grid(int n) { return dim3((n + kBlock - 1) / kBlock); }
         default: hipLaunchKernelGGL(KERNEL<4>, __VA_ARGS__); break;                                      \
     }
 
-hipError_t mm_prepare(const DevMesh& M, const MmLayer& L, const Mix& mx, unsigned long long* slot, hipStream_t s) {
-    ZFV_MM_DISPATCH(mx.n, k_mm_prepare, mm_grid(M.n), dim3(kBlock), 0, s, M, L, mx, slot);
-    return hipGetLastError();
-}
-
-hipError_t mm_flux1(const DevMesh& M, const MmLayer& in, const MmLayer& out, const Mix& mx, const StepCtl& ctl,
-                    const MmOpts& o, const double* CF, hipStream_t s) {
-    ZFV_MM_DISPATCH(mx.n, k_mm_flux1, mm_grid(M.n), dim3(kBlock), 0, s, M, in, out, mx, ctl, o, CF);
-    return hipGetLastError();
-}
-
-__global__ void __launch_bounds__(64) k_zero_u32(unsigned int* p) {
-    if (threadIdx.x == 0) *p = 0u;
-}
-
+// KERNEL<NM, V>, V = 1 or 2 (the order of accuracy, or the plain-face flux
+// variant of the update kernels).
 #define ZFV_MM_DISPATCH_ACC(NMAT, ACC, KERNEL, ...)                                                     \
     switch ((NMAT) * 2 + (ACC) - 1) {                                                                   \
         case 2: hipLaunchKernelGGL((KERNEL<1, 1>), __VA_ARGS__); break;                                  \
@@ -980,6 +971,22 @@ __global__ void __launch_bounds__(64) k_zero_u32(unsigned int* p) {
         case 8: hipLaunchKernelGGL((KERNEL<4, 1>), __VA_ARGS__); break;                                  \
         default: hipLaunchKernelGGL((KERNEL<4, 2>), __VA_ARGS__); break;                                 \
     }
+
+hipError_t mm_prepare(const DevMesh& M, const MmLayer& L, const Mix& mx, unsigned long long* slot, hipStream_t s) {
+    ZFV_MM_DISPATCH(mx.n, k_mm_prepare, mm_grid(M.n), dim3(kBlock), 0, s, M, L, mx, slot);
+    return hipGetLastError();
+}
+
+hipError_t mm_flux1(const DevMesh& M, const MmLayer& in, const MmLayer& out, const Mix& mx, const StepCtl& ctl,
+                    const MmOpts& o, const double* CF, hipStream_t s) {
+    ZFV_MM_DISPATCH_ACC(mx.n, o.flux == kFluxHllc ? 1 : 2, k_mm_flux1, mm_grid(M.n), dim3(kBlock), 0, s, M, in, out,
+                        mx, ctl, o, CF);
+    return hipGetLastError();
+}
+
+__global__ void __launch_bounds__(64) k_zero_u32(unsigned int* p) {
+    if (threadIdx.x == 0) *p = 0u;
+}
 
 hipError_t mm_crp_mark(const DevMesh& M, const MmLayer& in, const MmLayer& half, const Mix& mx, int acc,
                        const MmOpts& o, const double* gx, const double* gy, int* list, unsigned int* count,
@@ -1009,7 +1016,8 @@ hipError_t mm_stage1(const DevMesh& M, const MmLayer& in, const MmLayer& half, c
 hipError_t mm_flux2(const DevMesh& M, const MmLayer& in, const MmLayer& half, const MmLayer& out, const Mix& mx,
                     const StepCtl& ctl, const MmOpts& o, const double* gx, const double* gy, const double* CF,
                     hipStream_t s) {
-    ZFV_MM_DISPATCH(mx.n, k_mm_flux2, mm_grid(M.n), dim3(kBlock), 0, s, M, in, half, out, mx, ctl, o, gx, gy, CF);
+    ZFV_MM_DISPATCH_ACC(mx.n, o.flux == kFluxHllc ? 1 : 2, k_mm_flux2, mm_grid(M.n), dim3(kBlock), 0, s, M, in, half,
+                        out, mx, ctl, o, gx, gy, CF);
     return hipGetLastError();
 }
 
@@ -1020,9 +1028,21 @@ hipError_t mm_interface(const DevMesh& M, const MmLayer& L, const Mix& mx, const
 }
 
 
-hipError_t mm_grad(const DevMesh& M, const MmLayer& L, const Mix& mx, int crp, int limiter, double* gx, double* gy,
-                   hipStream_t s) {
-    ZFV_MM_DISPATCH(mx.n, k_mm_grad, mm_grid(M.n), dim3(kBlock), 0, s, M, L, crp, limiter, gx, gy);
+hipError_t mm_grad(const DevMesh& M, const MmLayer& L, const Mix& mx, int crp, int limiter, int rows, double* gx,
+                   double* gy, hipStream_t s) {
+    // the rows of set_flags (4 + NM) in one chunk; all 6 + 2 NM of a
+    // second-order step in chunks of 4, which keeps more wavefronts resident
+    const bool split = rows <= 4 + mx.n;
+#define ZFV_MM_GRAD(NMC)                                                                                     \
+    hipLaunchKernelGGL((split ? k_mm_grad<NMC, 4 + NMC> : k_mm_grad<NMC, 4>), mm_grid(M.n), dim3(kBlock), 0, s, \
+                       M, L, crp, limiter, rows, gx, gy)
+    switch (mx.n) {
+        case 1: ZFV_MM_GRAD(1); break;
+        case 2: ZFV_MM_GRAD(2); break;
+        case 3: ZFV_MM_GRAD(3); break;
+        default: ZFV_MM_GRAD(4); break;
+    }
+#undef ZFV_MM_GRAD
     return hipGetLastError();
 }
 
