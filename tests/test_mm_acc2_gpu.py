@@ -116,3 +116,29 @@ def test_refine_after_second_order_steps_uses_their_gradients(oracle, zfv):
     ctx.mm_refine(1)
     compare_mesh(ctx, m, "after refine")
     assert_same(ctx.mm_state(), o.state(), "after refine")
+
+
+def test_second_order_shock_bubble_workload_matches_oracle(oracle, zfv):
+    """The benchmark's second-order workload at its full size: the water /
+    air shock-bubble on the 512 x 512 mesh, bit-exact state and dt for
+    four steps."""
+    from zephyr_amd import problems
+    nx = 512
+    mats = [problems.WATER, problems.AIR]
+    m = oracle.Mesh.rectangle(nx, nx)
+    o = oracle.MmFluid(m, mats, 0.5)
+    o.set_accuracy(2)
+    ctx = zfv.Context()
+    ctx.mesh_rectangle(nx, nx)
+    ctx.mm_configure(mats, 0.5)
+    ctx.mm_set_accuracy(2)
+    c = m.cells()
+    st = problems.mm_bubble_shock(c["cx"], c["cy"], np.full(m.n, 1.0 / nx), 2024)
+    o.set_state(*st)
+    ctx.mm_set_state(*st)
+    for step in range(4):
+        assert o.update() == 0
+        ctx.mm_step(1)
+        assert ctx.mm_dt() == o.dt(), f"step {step}: dt"
+        assert_same(ctx.mm_state(), o.state(), f"step {step}")
+    assert ctx.mm_crp_faces() > 1000
