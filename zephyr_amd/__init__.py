@@ -327,7 +327,7 @@ class Context:
         _check(lib().zfv_get_cell_order(self.h, C.byref(m), C.byref(a)))
         return m.value, a.value
 
-    # ---- multi-material solver (first order, no CRP)
+    # ---- multi-material solver
     def mm_configure(self, materials, cfl=0.7, crp=True):
         """materials: sequence of (gamma, P0, e0, Cv, T0, rho_ref); crp:
         composite Riemann problems at mixed faces (the MmFluid default)."""
