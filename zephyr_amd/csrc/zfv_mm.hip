@@ -276,7 +276,7 @@ __global__ void __launch_bounds__(kBlock) k_mm_crp_mark(DevMesh M, MmLayer in, M
 // frame, into CF[slot][k][stride].  These faces are few and need many more
 // registers than the plain HLLC faces of k_mm_flux1, hence the compaction.
 template <int NM, int ACC>
-__global__ void __launch_bounds__(64) k_mm_crp(DevMesh M, MmLayer in, MmLayer half, const double* __restrict__ gx,
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 1))) k_mm_crp(DevMesh M, MmLayer in, MmLayer half, const double* __restrict__ gx,
                                             const double* __restrict__ gy, Mix mx, StepCtl ctl, MmOpts opt,
                                             const double* __restrict__ inx, const double* __restrict__ iny,
                                             const int* __restrict__ list, const unsigned int* count,

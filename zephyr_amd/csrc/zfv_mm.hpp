@@ -124,7 +124,7 @@ __device__ __forceinline__ void init_rhos(const Mix& mx, const double* b, const 
 
 // find_rT_rP_old (:383-432): T from 1/rho = sum_i beta_i v_i(P, T).
 template <int NM>
-__device__ void find_rT_rP(const Mix& mx, double rho, double P, const double* b, double T0, const double* rh0,
+__device__ __forceinline__ void find_rT_rP(const Mix& mx, double rho, double P, const double* b, double T0, const double* rh0,
                            double* rh, double& T, DPDT& v) {
     const double v_mix = 1.0 / rho;
     T = isnan(T0) ? 300.0 : T0;
@@ -167,7 +167,7 @@ __device__ __forceinline__ DPDT mix_energy_PT(const Mix& mx, double P, double T,
 // get_reT (:214-227): e, T and true densities from (rho, P); pure material
 // in closed form, a mixture through find_reT_rP_old (:434-446).
 template <int NM>
-__device__ void get_reT(const Mix& mx, double rho, double P, const double* b, double T0, const double* rh0,
+__device__ __forceinline__ void get_reT(const Mix& mx, double rho, double P, const double* b, double T0, const double* rh0,
                         double* rh, double& e, double& T) {
     const int idx = frac_index<NM>(b);
     if (idx >= 0) {
@@ -185,7 +185,7 @@ __device__ void get_reT(const Mix& mx, double rho, double P, const double* b, do
 // sound_speed_rP (:101-114): c^2 = (P / rho^2 - de/drho) / (de/dP) at
 // constant (rho, P) derivatives of the equilibrium energy.
 template <int NM>
-__device__ double sound_speed(const Mix& mx, double rho, double P, const double* b, double T0, const double* rh0) {
+__device__ __forceinline__ double sound_speed(const Mix& mx, double rho, double P, const double* b, double T0, const double* rh0) {
     const int idx = frac_index<NM>(b);
     if (idx >= 0) return sg_sound_speed_rP(pick<NM>(mx, idx), rho, P);
     double rh[NM], T;
@@ -201,13 +201,13 @@ __device__ double sound_speed(const Mix& mx, double rho, double P, const double*
 }
 
 template <int NM>
-__device__ void find_rPT(const Mix& mx, double rho, double e_mix, const double* b, double P0, double T0,
+__device__ __forceinline__ void find_rPT(const Mix& mx, double rho, double e_mix, const double* b, double P0, double T0,
                          const double* rh0, double* rh, double& P, double& T, DPDT& v, DPDT& e);
 
 // get_rPT (:199-212): P, T and true densities from (rho, e); a mixture by
 // find_rPT_old (:448-533), Newton on (P, T) with the P > P_min guard.
 template <int NM>
-__device__ void get_rPT(const Mix& mx, double rho, double e_mix, const double* b, double P0, double T0,
+__device__ __forceinline__ void get_rPT(const Mix& mx, double rho, double e_mix, const double* b, double P0, double T0,
                         const double* rh0, double* rh, double& P, double& T) {
     const int idx = frac_index<NM>(b);
     if (idx >= 0) {
@@ -224,7 +224,7 @@ __device__ void get_rPT(const Mix& mx, double rho, double e_mix, const double* b
 // find_rPT_old (mixture_pt.cpp:448-533); v, e: the last iteration's sums
 // (for the derivatives of sound_speed_re).
 template <int NM>
-__device__ void find_rPT(const Mix& mx, double rho, double e_mix, const double* b, double P0, double T0,
+__device__ __forceinline__ void find_rPT(const Mix& mx, double rho, double e_mix, const double* b, double P0, double T0,
                          const double* rh0, double* rh, double& P, double& T, DPDT& v, DPDT& e) {
     const double v_mix = 1.0 / rho;
     P = isnan(P0) ? 1.0e5 : P0;
@@ -309,7 +309,7 @@ __device__ __forceinline__ void m_flux(const MZ<NM>& z, double* f) {
 
 // HLLC::calc_flux, mmf (hllc.cpp:339-421), in the face frame.
 template <int NM>
-__device__ void hllc_mm(const MZ<NM>& zL, const MZ<NM>& zR, double* F, bool& bad) {
+__device__ __forceinline__ void hllc_mm(const MZ<NM>& zL, const MZ<NM>& zR, double* F, bool& bad) {
     const double c_L = zL.c, c_R = zR.c;
     double S_L = smin(zL.u - c_L, zR.u - c_L);
     double S_R = smax(zL.u + c_R, zR.u + c_R);
@@ -500,7 +500,7 @@ __device__ __forceinline__ double alpha_of(const MZ<NM>& z, int i) {   // PState
 // stiffened_gas.cpp:144-146): c^2 = P.dR + P P.dE / rho^2 with the
 // derivatives of find_rPT_old (:525-529), default initial guesses.
 template <int NM>
-__device__ double sound_speed_re(const Mix& mx, double rho, double e, const double* b) {
+__device__ __forceinline__ double sound_speed_re(const Mix& mx, double rho, double e, const double* b) {
     const int idx = frac_index<NM>(b);
     if (idx >= 0) {
         const Mat m = pick<NM>(mx, idx);
@@ -523,7 +523,7 @@ struct Wv {   // mmf::WaveConfig3 (hllc.h:21-27)
 
 // HLLC::wave_config from conserved vectors and fluxes (hllc.cpp:256-335).
 template <int NM>
-__device__ void wave_config(const Mix& mx, const double* QL, const double* FL, const double* QR, const double* FR,
+__device__ __forceinline__ void wave_config(const Mix& mx, const double* QL, const double* FL, const double* QR, const double* FR,
                             Wv<NM>& w) {
     constexpr int K = 4 + NM;
     const double uL = QL[1] / QL[0], vL = QL[2] / QL[0];
@@ -616,7 +616,7 @@ __device__ __forceinline__ void total2(const double* F1, const double* Q1, doubl
 
 // CrpFlux::classic, ZPP version (crp_flux.cpp:234-489).
 template <int NM>
-__device__ void crp_classic(const MZ<NM>& zLA, const MZ<NM>& zLB, const MZ<NM>& zRB, const Mix& mx, double delta,
+__device__ __forceinline__ void crp_classic(const MZ<NM>& zLA, const MZ<NM>& zLB, const MZ<NM>& zRB, const Mix& mx, double delta,
                             double dt, double* out, bool& err) {
     constexpr int K = 4 + NM;
     double qLB[K], fLB[K], Q_R[K], F_R[K];
@@ -727,7 +727,7 @@ __device__ void crp_classic(const MZ<NM>& zLA, const MZ<NM>& zLB, const MZ<NM>& 
 // CrpFlux::inverse (crp_flux.cpp:493-512): the classic problem mirrored in x
 // (PState::inverse flips u, Flux::inverse negates all but m_x, models.cpp:219-221, 506-515).
 template <int NM>
-__device__ void crp_inverse(const MZ<NM>& zLA, const MZ<NM>& zRA, const MZ<NM>& zRB, const Mix& mx, double delta,
+__device__ __forceinline__ void crp_inverse(const MZ<NM>& zLA, const MZ<NM>& zRA, const MZ<NM>& zRB, const Mix& mx, double delta,
                             double dt, double* out, bool& err) {
     MZ<NM> a = zRB, b = zRA, c = zLA;
     a.u = -a.u;
@@ -750,7 +750,7 @@ __device__ __forceinline__ double sg_energy_rT(const Mat& m, double rho, double 
 // both with their sound speeds (sound_speed_rP, as the HLLC of the split
 // states computes them).
 template <int NM>
-__device__ void m_split(const MZ<NM>& z, const Mix& mx, int iA, MZ<NM>& zA, MZ<NM>& zB) {
+__device__ __forceinline__ void m_split(const MZ<NM>& z, const Mix& mx, int iA, MZ<NM>& zA, MZ<NM>& zB) {
     double rhoA = 0.0;
 #pragma unroll
     for (int i = 0; i < NM; ++i)
@@ -835,7 +835,7 @@ __device__ __forceinline__ int master_component(const MZ<NM>& zm, const MZ<NM>& 
 
 // MmFluid::calc_crp_flux (mm_fluid.cpp:259-337).
 template <int NM>
-__device__ void calc_crp_flux(const MZ<NM>& zL, const MZ<NM>& zR, double hL, double hR, int iA, double a_sig,
+__device__ __forceinline__ void calc_crp_flux(const MZ<NM>& zL, const MZ<NM>& zR, double hL, double hR, int iA, double a_sig,
                               double dt, const Mix& mx, double* F, bool& err) {
     constexpr int K = 4 + NM;
     const double aL = alpha_of<NM>(zL, iA), aR = alpha_of<NM>(zR, iA);
