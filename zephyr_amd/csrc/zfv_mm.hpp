@@ -606,6 +606,16 @@ __device__ __forceinline__ void copyk(double* d, const double* s) {
     for (int k = 0; k < 4 + NM; ++k) d[k] = s[k];
 }
 
+// out = c1 ? a : (c2 ? b : c), element by element: a select per component
+// keeps the candidate vectors in registers, where a selected pointer would
+// make them addressable (stack) memory.
+template <int NM>
+__device__ __forceinline__ void sel3(bool c1, const double* a, bool c2, const double* b, const double* c,
+                                     double* out) {
+#pragma unroll
+    for (int k = 0; k < 4 + NM; ++k) out[k] = c1 ? a[k] : (c2 ? b[k] : c[k]);
+}
+
 // (F1 (t1 - t0) - Q1 (x1 - x0) + F2 (...) - ...) / (0 - dt) for two segments.
 template <int NM>
 __device__ __forceinline__ void total2(const double* F1, const double* Q1, double dt1, double dx1, const double* F2,
@@ -641,14 +651,13 @@ __device__ __forceinline__ void crp_classic(const MZ<NM>& zLA, const MZ<NM>& zLB
         Wv<NM> w1;
         wave_config<NM>(mx, qA, fA, w0.QsL, w0.FsL, w1);
         const double lb = -O1.x / (dt - O1.t);
-        const double* F = fA;
-        const double* Q = qA;
-        if (dt > O1.t) {
-            if (lb < w1.SL) { F = fA; Q = qA; }
-            else if (lb < w1.SC) { F = w1.FsL; Q = w1.QsL; }
-            else if (lb < w1.SR) { F = w1.FsR; Q = w1.QsR; }
-            else { F = w0.FsL; Q = w0.QsL; }
-        }
+        // dt <= O1.t or lb < SL: fA / qA; else by lb against SC and SR
+        const bool first = !(dt > O1.t) || lb < w1.SL;
+        double F[K], Q[K], F2[K], Q2[K];
+        sel3<NM>(lb < w1.SC, w1.FsL, lb < w1.SR, w1.FsR, w0.FsL, F2);
+        sel3<NM>(lb < w1.SC, w1.QsL, lb < w1.SR, w1.QsR, w0.QsL, Q2);
+        sel3<NM>(first, fA, true, F2, F2, F);
+        sel3<NM>(first, qA, true, Q2, Q2, Q);
         total2<NM>(F, Q, O1.t - dt, O1.x - 0.0, fLB, qLB, 0.0 - O1.t, 0.0 - O1.x, dt, out);
         return;
     }
@@ -666,8 +675,9 @@ __device__ __forceinline__ void crp_classic(const MZ<NM>& zLA, const MZ<NM>& zLB
             return;
         }
         const double lb = -O1.x / (dt - O1.t);
-        const double* F = lb < w1.SL ? fA : (lb < w1.SC ? w1.FsL : w1.FsR);
-        const double* Q = lb < w1.SL ? qA : (lb < w1.SC ? w1.QsL : w1.QsR);
+        double F[K], Q[K];
+        sel3<NM>(lb < w1.SL, fA, lb < w1.SC, w1.FsL, w1.FsR, F);
+        sel3<NM>(lb < w1.SL, qA, lb < w1.SC, w1.QsL, w1.QsR, Q);
         total2<NM>(F, Q, O1.t - dt, O1.x - 0.0, w0.FsL, w0.QsL, 0.0 - O1.t, 0.0 - O1.x, dt, out);
         return;
     }
@@ -707,14 +717,16 @@ __device__ __forceinline__ void crp_classic(const MZ<NM>& zLA, const MZ<NM>& zLB
             return;
         }
         const double lb = -O2.x / (dt - O2.t);
-        const double* F = lb < w2.SC ? w2.FsL : (lb < w2.SR ? w2.FsR : w0.FsR);
-        const double* Q = lb < w2.SC ? w2.QsL : (lb < w2.SR ? w2.QsR : w0.QsR);
+        double F[K], Q[K];
+        sel3<NM>(lb < w2.SC, w2.FsL, lb < w2.SR, w2.FsR, w0.FsR, F);
+        sel3<NM>(lb < w2.SC, w2.QsL, lb < w2.SR, w2.QsR, w0.QsR, Q);
         total2<NM>(F, Q, O2.t - dt, O2.x - 0.0, w0.FsR, w0.QsR, 0.0 - O2.t, 0.0 - O2.x, dt, out);
         return;
     }
     const double lb = -O3.x / (dt - O3.t);
-    const double* F = lb < w3.SL ? w1.FsL : (lb < w3.SC ? w3.FsL : w3.FsR);
-    const double* Q = lb < w3.SL ? w1.QsL : (lb < w3.SC ? w3.QsL : w3.QsR);
+    double F[K], Q[K];
+    sel3<NM>(lb < w3.SL, w1.FsL, lb < w3.SC, w3.FsL, w3.FsR, F);
+    sel3<NM>(lb < w3.SL, w1.QsL, lb < w3.SC, w3.QsL, w3.QsR, Q);
 #pragma unroll
     for (int k = 0; k < K; ++k) {
         const double g1 = F[k] * (O3.t - dt) - Q[k] * (O3.x - 0.0);
