@@ -139,6 +139,12 @@ void orc_mm_closure(int n_mat, const double* mats, double rho, double e, const d
                     double T0, double* out);
 void orc_mm_closure_rP(int n_mat, const double* mats, double rho, double P, const double* beta, double* out);
 void orc_mm_face_flux(const orc_mm* s, int cL, int cR, double nx, double ny, double* F);
+int orc_mm_crp_flux(const orc_mm* s, int cL, int cR, double hL, double hR, int iA, double a_sig, double dt,
+                    double* F);
+void orc_mm_wave_speeds(const orc_mm* s, int cL, int cR, double* S3);
+int orc_mm_crp_sub(const orc_mm* s, int inverse, int c0, int c1, int c2, double delta, double dt, double* F);
+void orc_mm_split(const orc_mm* s, int c, int iA, double* out);
+void orc_mm_plant(int k);   /* test-only planted mistakes in the composite-flux chain, 0 = none */
 void orc_mm_set_limiter(orc_mm* s, int limiter);
 void orc_mm_set_accuracy(orc_mm* s, int acc);   /* 1 or 2 (clamped) */
 void orc_mm_compute_grad(orc_mm* s);

@@ -574,6 +574,15 @@ inline double mix_sound_speed_re(const Mixture& mx, double rho, double e, const 
     return std::sqrt(c2);
 }
 
+// Planted mistakes for the tests that show the composite-flux checks can
+// fail (0 = none, the restatement as written): 1 swaps the inverse problem
+// of the aL == 0 branch of calc_crp_flux for a classic one, 2 flips the
+// mix-to-clear / clear-to-mix test of crp_classic, 3 gives the mixture
+// part of PState::split the energy of the first of its materials alone,
+// 4 takes the right star state where the first branch of crp_classic
+// takes the left one.
+inline int g_orc_plant = 0;
+
 // mmf::WaveConfig3 (hllc.h:21-27).
 struct Wave {
     double SL, SC, SR;
@@ -684,7 +693,10 @@ inline MArr crp_classic(const MState& zLA, const MState& zLB, const MState& zRB,
         MArr F = fLA, Q = qLA;
         if (dt > O1.t) {
             if (lb < w1.SL) { F = fLA; Q = qLA; }
-            else if (lb < w1.SC) { F = w1.FsL; Q = w1.QsL; }
+            else if (lb < w1.SC) {
+                F = g_orc_plant == 4 ? w1.FsR : w1.FsL;
+                Q = g_orc_plant == 4 ? w1.QsR : w1.QsL;
+            }
             else if (lb < w1.SR) { F = w1.FsR; Q = w1.QsR; }
             else { F = w0.FsL; Q = w0.QsL; }
         }
@@ -692,7 +704,7 @@ inline MArr crp_classic(const MState& zLA, const MState& zLB, const MState& zRB,
         const MArr G_O1O0 = seg(fLB, qLB, O0.t - O1.t, O0.x - O1.x);
         return total2(G_OdtO1, G_O1O0);
     }
-    if (w0.SC >= 0.0) {   // mix to clear
+    if ((w0.SC >= 0.0) != (g_orc_plant == 2)) {   // mix to clear
         if (dt <= O1.t && O1.x < 0.0) return w0.FsL;
         const MArr Q_L = m_qstate(zLA), F_L = m_flux(zLA);
         const Wave w1 = wave_config(mx, Q_L, F_L, w0.QsL, w0.FsL);
@@ -813,6 +825,12 @@ inline void m_split(const MState& z, const Mixture& mx, int iA, MState& zA, MSta
             zB.beta.a[i] = alpha.a[i] * z.rhos[i] / zB.rho;
             zB.e += zB.beta.a[i] * sg_energy_rT(mx.m[i], z.rhos[i], z.T);
         }
+    if (g_orc_plant == 3)
+        for (int i = 0; i < kMaxMat; ++i)
+            if (z.beta.has(i) && i != iA) {
+                zB.e = sg_energy_rT(mx.m[i], z.rhos[i], z.T);
+                break;
+            }
 }
 
 // master_component (mm_fluid.cpp:201-226): largest volume fraction of the upwind side.
@@ -844,7 +862,8 @@ inline MArr calc_crp_flux(const MState& zL, const MState& zR, double hL, double 
         MState zRA, zRB;
         m_split(zR, mx, iA, zRA, zRB);
         fA = m_hllc(zL, zRA, mx);
-        fB = crp_inverse(zL, zRB, zRA, mx, delta_R, dt, err);
+        fB = g_orc_plant == 1 ? crp_classic(zL, zRB, zRA, mx, delta_R, dt, err)
+                              : crp_inverse(zL, zRB, zRA, mx, delta_R, dt, err);
     } else if (aL == 1.0) {
         MState zRA, zRB;
         m_split(zR, mx, iA, zRA, zRB);
@@ -1583,6 +1602,54 @@ void orc_mm_face_flux(const orc_mm* s, int cL, int cR, double nx, double ny, dou
     MArr f = m_hllc(m_in_local(s->init[cL], n), m_in_local(s->init[cR], n), s->mix);
     for (int k = 0; k < 9; ++k) F[k] = f.a[k];
 }
+
+// calc_crp_flux (mm_fluid.cpp:259-337) between the states of cells cL and
+// cR across a face with normal +e_x (the face frame is the global one):
+// cell widths hL, hR, master material iA, face fraction a_sig, time step
+// dt; F: the 9 flux components; returns the "strange case" error flag.
+int orc_mm_crp_flux(const orc_mm* s, int cL, int cR, double hL, double hR, int iA, double a_sig, double dt,
+                    double* F) {
+    int err = 0;
+    const MArr f = calc_crp_flux(s->init[cL], s->init[cR], hL, hR, iA, a_sig, dt, s->mix, err);
+    for (int k = 0; k < 9; ++k) F[k] = f.a[k];
+    return err;
+}
+// wave_config (hllc.cpp:256-335) of the states of cells cL and cR in the
+// same frame: {S_L, S_C, S_R}.
+void orc_mm_wave_speeds(const orc_mm* s, int cL, int cR, double* S3) {
+    const MState& zL = s->init[cL];
+    const MState& zR = s->init[cR];
+    const Wave w = wave_config(s->mix, m_qstate(zL), m_flux(zL), m_qstate(zR), m_flux(zR));
+    S3[0] = w.SL;
+    S3[1] = w.SC;
+    S3[2] = w.SR;
+}
+// CrpFlux::classic (inverse = 0) or CrpFlux::inverse (inverse = 1) of the
+// states of cells c0, c1, c2 in that argument order (crp_flux.cpp:125-512),
+// across a +x face, with the distance delta and time step dt.
+int orc_mm_crp_sub(const orc_mm* s, int inverse, int c0, int c1, int c2, double delta, double dt, double* F) {
+    int err = 0;
+    const MArr f = inverse ? crp_inverse(s->init[c0], s->init[c1], s->init[c2], s->mix, delta, dt, err)
+                           : crp_classic(s->init[c0], s->init[c1], s->init[c2], s->mix, delta, dt, err);
+    for (int k = 0; k < 9; ++k) F[k] = f.a[k];
+    return err;
+}
+// PState::split (models.cpp:261-335) of cell c's state for material iA:
+// out[0..1] = {rho, e} of the pure part, out[2..3] = {rho, e} of the
+// mixture part, out[4..7] its mass fractions, out[8..9] the P and T both
+// parts carry.
+void orc_mm_split(const orc_mm* s, int c, int iA, double* out) {
+    MState zA, zB;
+    m_split(s->init[c], s->mix, iA, zA, zB);
+    out[0] = zA.rho;
+    out[1] = zA.e;
+    out[2] = zB.rho;
+    out[3] = zB.e;
+    for (int i = 0; i < kMaxMat; ++i) out[4 + i] = zB.beta.a[i];
+    out[8] = zB.P;
+    out[9] = zB.T;
+}
+void orc_mm_plant(int k) { g_orc_plant = k; }
 
 double orc_quad_volume_fraction(double p, double nx, double ny, double A, double B) {
     return quad_volume_fraction(p, nx, ny, A, B);

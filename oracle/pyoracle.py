@@ -87,6 +87,11 @@ def lib():
             "orc_mm_closure": (None, [_I, _dp, _D, _D, _dp, _D, _D, _dp]),
             "orc_mm_closure_rP": (None, [_I, _dp, _D, _D, _dp, _dp]),
             "orc_mm_face_flux": (None, [_P, _I, _I, _D, _D, _dp]),
+            "orc_mm_crp_flux": (_I, [_P, _I, _I, _D, _D, _I, _D, _D, _dp]),
+            "orc_mm_wave_speeds": (None, [_P, _I, _I, _dp]),
+            "orc_mm_plant": (None, [_I]),
+            "orc_mm_split": (None, [_P, _I, _I, _dp]),
+            "orc_mm_crp_sub": (_I, [_P, _I, _I, _I, _I, _D, _D, _dp]),
             "orc_mm_set_limiter": (None, [_P, _I]),
             "orc_mm_set_accuracy": (None, [_P, _I]),
             "orc_mm_compute_grad": (None, [_P]),
@@ -378,6 +383,32 @@ class MmFluid:
         lib().orc_mm_face_flux(self.h, cL, cR, nx, ny, _d(F))
         return F
 
+    def crp_flux(self, cL, cR, hL, hR, iA, a_sig, dt):
+        """calc_crp_flux between cells cL and cR across a +x face: (F[9], err)."""
+        F = np.zeros(9)
+        err = lib().orc_mm_crp_flux(self.h, cL, cR, hL, hR, iA, a_sig, dt, _d(F))
+        return F, err
+
+    def crp_sub(self, inverse, c0, c1, c2, delta, dt):
+        """CrpFlux::classic (inverse False) / inverse (True) of the states of
+        cells c0, c1, c2 across a +x face: (F[9], err)."""
+        F = np.zeros(9)
+        err = lib().orc_mm_crp_sub(self.h, int(bool(inverse)), c0, c1, c2, delta, dt, _d(F))
+        return F, err
+
+    def split(self, c, iA):
+        """PState::split of cell c for material iA: dict of the pure part's
+        rho / e, the mixture part's rho / e / beta, and their common P, T."""
+        out = np.zeros(10)
+        lib().orc_mm_split(self.h, c, iA, _d(out))
+        return {"rhoA": out[0], "eA": out[1], "rhoB": out[2], "eB": out[3], "betaB": out[4:8],
+                "P": out[8], "T": out[9]}
+
+    def wave_speeds(self, cL, cR):
+        S = np.zeros(3)
+        lib().orc_mm_wave_speeds(self.h, cL, cR, _d(S))
+        return S
+
     def set_limiter(self, limiter):
         lib().orc_mm_set_limiter(self.h, LIMITERS.index(limiter) if isinstance(limiter, str) else int(limiter))
 
@@ -404,6 +435,11 @@ class MmFluid:
         """EuMesh::refine with MmFluid's distributor: 1 changed, 0 not, -1 a
         state the reference rejects."""
         return lib().orc_mm_refine(self.h, max_level)
+
+
+def plant(k):
+    """Test-only planted mistake in the oracle's composite-flux chain (0 = none)."""
+    lib().orc_mm_plant(int(k))
 
 
 def mm_closure(materials, rho, e, beta, P0=float("nan"), T0=float("nan")):
