@@ -22,6 +22,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, HERE)
 
 HBM_PEAK_GBS = 8000.0
+RESERVE_PER_BASE_CELL = 4   # AMR workloads: reserved cells per base cell
 
 # Algorithmic bytes per cell-update of each stage (phi = unique faces / cells),
 # the stage-structured SoA accounting of the workload definition.
@@ -41,8 +42,25 @@ def kernel_bytes_per_cell(phi):
         "flux1": s["flux1"] + s["dt"],          # first-order update + fused next-step dt
         "grad_pred": s["grad"] + s["pred"],      # gradient + limiting + predictor (fused)
         "corr": s["corr"] + s["dt"],             # corrector + fused next-step dt
+        "tile": s["grad"] + s["pred"] + s["corr"] + s["dt"],   # the whole MUSCL step in one tiled kernel
         "dt": s["dt"],
     }
+
+
+# Bytes per cell the tiled MUSCL step itself must move (its fused minimum):
+# the state in (rho, u, v, p, e) and out, the isolation counter in and out,
+# volume and incircle; plus the halo of its tile shape (rows x 62 cells, a
+# two-cell ring of rho, u, v, p and a one-cell ring of e and the counter),
+# which neighbouring tiles read again.
+TILE_ROWS = {512: 6, 768: 10, 1024: 14}
+
+
+def tile_fused_bytes_per_cell(threads):
+    rows = TILE_ROWS[threads]
+    out = rows * 62.0
+    ring2 = (rows + 4) * 66.0 / out - 1.0
+    ring1 = (rows + 2) * 64.0 / out - 1.0
+    return 40 + 4 + 8 + 8 + 40 + 4 + 32 * ring2 + 12 * ring1
 
 
 WORKLOADS = {
@@ -97,6 +115,20 @@ WORKLOADS = {
 }
 
 
+def kernel_sources_sha():
+    """Short SHA-256 over the HIP kernel sources and headers (which the HBM
+    counter summaries record, so a summary is used only for the kernels it
+    measured)."""
+    import hashlib
+    d = os.path.join(HERE, "zephyr_amd", "csrc")
+    h = hashlib.sha256()
+    for f in sorted(os.listdir(d)):
+        if f.endswith((".hip", ".hpp")):
+            h.update(f.encode())
+            h.update(open(os.path.join(d, f), "rb").read())
+    return h.hexdigest()[:16]
+
+
 def ensemble_sizes(w, seed):
     """Side lengths s_i ~ U{smin..smax} from SplitMix64 (draw i -> problem i)."""
     from zephyr_amd import problems
@@ -139,13 +171,49 @@ def adapt_to_initial_data(solver_like, name, seed, centres, set_state, set_flags
     set_state(*initial_state(name, cx, cy, seed))
 
 
-def cpu_baseline(name, seed, budget_s):
+def host_info():
+    """Host cores and CPU model of the machine the CPU baselines run on."""
+    model = None
+    try:
+        import subprocess
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for line in out.splitlines():
+            if line.startswith("Model name:"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except Exception:
+        pass
+    if model is None and os.path.exists("/proc/cpuinfo"):
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    try:
+        affinity = len(os.sched_getaffinity(0))
+    except AttributeError:
+        affinity = os.cpu_count()
+    return {"nproc": os.cpu_count(), "affinity_cpus": affinity, "lscpu_model": model,
+            "OMP_NUM_THREADS": os.environ.get("OMP_NUM_THREADS"), "OMP_PROC_BIND": os.environ.get("OMP_PROC_BIND")}
+
+
+def cpu_threads():
+    """All the host cores this process may use: OMP_NUM_THREADS when set (the
+    CPU share of the machine), else the CPUs of the process's affinity mask."""
+    env = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
+    if env > 0:
+        return env
+    try:
+        return len(os.sched_getaffinity(0))
+    except AttributeError:
+        return os.cpu_count() or 1
+
+
+def cpu_run(name, seed, budget_s, threads):
     """Oracle (CPU restatement, OpenMP) on the same workload for as many steps
-    as fit in budget_s (at least 2)."""
+    as fit in budget_s (at least 2); returns (cell-updates/s, steps, seconds)."""
     from oracle import pyoracle as O
     w = WORKLOADS[name]
     nx, ny = w["nx"], w["ny"]
-    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(16, os.cpu_count() or 1)
     m = O.Mesh.rectangle(nx, ny)
     s = O.SmFluid(m, acc=w["acc"], flux=O.HLLC, limiter=w["limiter"], cfl=0.5)
     amr = w["kind"] == "amr"
@@ -171,10 +239,23 @@ def cpu_baseline(name, seed, budget_s):
         el = time.perf_counter() - t0
         if steps >= 2 and el >= budget_s:
             break
-    return {"value": cells / el, "unit": "cell-updates/s", "cores": threads, "kind": "port",
-            "sample": f"oracle (C++ -O3 -ffp-contract=off, OpenMP {threads} threads) on the same "
-                      f"{name} workload from the same initial data, {steps} steps"
-                      f"{' (adaptation included)' if amr else ''} in {el:.2f} s"}
+    return cells / el, steps, el
+
+
+def cpu_baseline(name, seed, budget_s):
+    """The oracle on all the host's cores (the reported baseline) and on one
+    thread, each for about budget_s / 2 seconds from the same initial data."""
+    w = WORKLOADS[name]
+    amr = w["kind"] == "amr"
+    nt = cpu_threads()
+    v, steps, el = cpu_run(name, seed, budget_s / 2, nt)
+    v1, steps1, el1 = cpu_run(name, seed, budget_s / 2, 1)
+    return {"value": v, "unit": "cell-updates/s", "cores": nt, "kind": "port",
+            "sample": f"oracle (C++ -O3 -ffp-contract=off, OpenMP {nt} threads, OMP_PROC_BIND "
+                      f"{os.environ.get('OMP_PROC_BIND', 'unset')}) on the same {name} workload from the same "
+                      f"initial data, {steps} steps{' (adaptation included)' if amr else ''} in {el:.2f} s",
+            "single_thread": {"value": v1, "steps": steps1, "seconds": el1},
+            "host": host_info()}
 
 
 class ChunkProfile:
@@ -448,14 +529,15 @@ def mm_run(args, w, Z):
     print(json.dumps(out), flush=True)
 
 
-def mm_cpu_baseline(w, st, mats, budget_s, seed):
-    """Oracle MmFluid (serial C++) on the same workload (same initial data,
-    same adaptation), whole steps until the budget is spent."""
+def mm_cpu_run(w, st, mats, budget_s, seed, threads):
+    """Oracle MmFluid (C++, OpenMP over cells) on the same workload (same
+    initial data, same adaptation), whole steps until the budget is spent."""
     from oracle import pyoracle as O
     O.build()
     m = O.Mesh.rectangle(w["nx"], w["ny"])
     o = O.MmFluid(m, mats, w["cfl"])
     o.set_accuracy(w.get("acc", 1))
+    o.set_threads(threads)
     amr = "max_level" in w
     if amr:
         def centres():
@@ -473,9 +555,22 @@ def mm_cpu_baseline(w, st, mats, budget_s, seed):
         o.update()
         steps += 1
     el = time.perf_counter() - t0
-    return {"value": cells / el, "unit": "cell-updates/s", "cores": 1, "kind": "port",
-            "sample": f"oracle MmFluid (C++ -O3 -ffp-contract=off, serial) on the same workload from the same "
-                      f"initial data, {steps} steps{' (adaptation included)' if amr else ''} in {el:.2f} s"}
+    return cells / el, steps, el
+
+
+def mm_cpu_baseline(w, st, mats, budget_s, seed):
+    """The MmFluid oracle on all the host's cores (the reported baseline) and
+    on one thread, each for about budget_s / 2 seconds."""
+    amr = "max_level" in w
+    nt = cpu_threads()
+    v, steps, el = mm_cpu_run(w, st, mats, budget_s / 2, seed, nt)
+    v1, steps1, el1 = mm_cpu_run(w, st, mats, budget_s / 2, seed, 1)
+    return {"value": v, "unit": "cell-updates/s", "cores": nt, "kind": "port",
+            "sample": f"oracle MmFluid (C++ -O3 -ffp-contract=off, OpenMP {nt} threads over cells) on the same "
+                      f"workload from the same initial data, {steps} steps{' (adaptation included)' if amr else ''} "
+                      f"in {el:.2f} s",
+            "single_thread": {"value": v1, "steps": steps1, "seconds": el1},
+            "host": host_info()}
 
 
 def ensemble_run(args, w, Z):
@@ -556,10 +651,15 @@ def ensemble_run(args, w, Z):
     looped = cells_per_step * args.steps / t_loop
     phi = 2.0 + sum(2.0 * s for s in sizes) / cells_per_step   # unique faces / cells, s x s meshes
     kb = kernel_bytes_per_cell(phi)
-    dom = max(("grad_pred", "corr"), key=lambda k: prof[k][0])
+    dom = max((k for k in ("grad_pred", "corr", "tile") if k in prof), key=lambda k: prof[k][0])
     dom_ms = prof[dom][0] / prof[dom][1]
     alg = kb[dom] * cells_per_step
     achieved = alg / (dom_ms * 1e-3) / 1e9
+    fused = None
+    if dom == "tile":
+        fb = tile_fused_bytes_per_cell(b.launch(1)[1]) * cells_per_step
+        fused = {"bytes_per_launch": fb, "achieved_GBs": fb / (dom_ms * 1e-3) / 1e9,
+                 "frac": fb / (dom_ms * 1e-3) / 1e9 / HBM_PEAK_GBS}
     out = {
         "metric": "FP64 cell-updates/s (HLLC+MUSCL, batched ensemble)",
         "value": value, "unit": "cell-updates/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
@@ -577,18 +677,18 @@ def ensemble_run(args, w, Z):
                    "kernel_ms_avg": {k: v[0] / v[1] for k, v in prof.items()}},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None, "algorithmic_bytes_per_launch": alg,
-                     "avg_launch_ms": dom_ms},
+                     "avg_launch_ms": dom_ms, "fused_kernel_minimum": fused},
     }
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = ensemble_cpu_baseline(w, sizes, args.seed, args.cpu_budget)
     print(json.dumps(out), flush=True)
 
 
-def ensemble_cpu_baseline(w, sizes, seed, budget_s):
-    """Oracle on the ensemble's problems, in order, one step each, until the budget is spent."""
+def ensemble_cpu_run(w, sizes, seed, budget_s, threads):
+    """Oracle on the ensemble's problems, in order, two steps each, until the
+    budget is spent; returns (cell-updates/s, problems, cell-updates, seconds)."""
     from oracle import pyoracle as O
-    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(16, os.cpu_count() or 1)
-    cells, steps, el = 0, 0, 0.0
+    cells, probs, el = 0, 0, 0.0
     for p, s in enumerate(sizes):
         m = O.Mesh.rectangle(s, s)
         f = O.SmFluid(m, acc=w["acc"], flux=O.HLLC, limiter=w["limiter"], cfl=0.5)
@@ -598,12 +698,23 @@ def ensemble_cpu_baseline(w, sizes, seed, budget_s):
         f.update(threads)
         el += time.perf_counter() - t0
         cells += 2 * s * s
-        steps += 1
+        probs += 1
         if el >= budget_s:
             break
-    return {"value": cells / el, "unit": "cell-updates/s", "cores": threads, "kind": "port",
-            "sample": f"oracle (C++ -O3 -ffp-contract=off, OpenMP {threads} threads): two steps of each of the "
-                      f"first {steps} ensemble problems ({cells} cell-updates, steps only) in {el:.2f} s"}
+    return cells / el, probs, cells, el
+
+
+def ensemble_cpu_baseline(w, sizes, seed, budget_s):
+    """The oracle on all the host's cores (the reported baseline) and on one
+    thread, each for about budget_s / 2 seconds of steps."""
+    nt = cpu_threads()
+    v, probs, cells, el = ensemble_cpu_run(w, sizes, seed, budget_s / 2, nt)
+    v1, probs1, cells1, el1 = ensemble_cpu_run(w, sizes, seed, budget_s / 2, 1)
+    return {"value": v, "unit": "cell-updates/s", "cores": nt, "kind": "port",
+            "sample": f"oracle (C++ -O3 -ffp-contract=off, OpenMP {nt} threads): two steps of each of the "
+                      f"first {probs} ensemble problems ({cells} cell-updates, steps only) in {el:.2f} s",
+            "single_thread": {"value": v1, "problems": probs1, "seconds": el1},
+            "host": host_info()}
 
 
 def main():
@@ -649,6 +760,10 @@ def main():
             return c["cx"], c["cy"]
         adapt_to_initial_data(ctx, args.workload, args.seed, centres, ctx.set_state, ctx.set_flags,
                               lambda ml, slope: ctx.refine(ml, slope), w["max_level"], w["threshold"])
+        # device capacity for the adapted meshes of the run (AmrCells::reserve_amr):
+        # the timed adaptations then allocate nothing (the mesh peaks near
+        # 2 x the base cell count on this workload)
+        ctx.reserve(RESERVE_PER_BASE_CELL * nx * ny)
     else:
         cx, cy = rect_centres(nx, ny)
         ctx.set_state(*initial_state(args.workload, cx, cy, args.seed))
@@ -696,26 +811,43 @@ def main():
     ctx.profile_enable(False)
 
     value = cells / el
-    step_kernels = ("flux1",) if acc == 1 else ("grad_pred", "corr")
+    step_kernels = ("flux1",) if acc == 1 else ("grad_pred", "corr", "tile")
     dom = max(step_kernels, key=lambda k: prof.ms.get(k, 0.0))
     dom_ms = prof.ms[dom] / prof.launches[dom]
     alg_bytes = prof.bytes[dom] / prof.launches[dom]
     achieved = alg_bytes / (dom_ms * 1e-3) / 1e9
+    # the tiled step fuses the gradient, predictor and corrector: the
+    # stage-structured bytes above are the workload's accounting; beside them
+    # the bytes the fused kernel itself must move
+    fused = None
+    if dom == "tile":
+        fb = tile_fused_bytes_per_cell(ctx.launch(1)[1]) * prof.cells[dom] / prof.launches[dom]
+        fused = {"bytes_per_launch": fb, "achieved_GBs": fb / (dom_ms * 1e-3) / 1e9,
+                 "frac": fb / (dom_ms * 1e-3) / 1e9 / HBM_PEAK_GBS}
     step_bytes = sum(prof.bytes.get(k, 0.0) for k in step_kernels)
     step_ms = sum(prof.ms.get(k, 0.0) for k in step_kernels)
 
-    # HBM bytes per launch of the dominant kernel, from the FETCH_SIZE /
-    # WRITE_SIZE counter runs summarised in profiles/traffic.json (counter
-    # KiB per dispatch, scaled by the copy-kernel calibration; per cell for
-    # the adaptive workload, whose launches vary in size)
+    # HBM bytes per launch of the dominant kernel.  rocprofv3 counters cannot
+    # be read from inside this run; they come from separate FETCH_SIZE /
+    # WRITE_SIZE counter runs of this same command summarised in
+    # profiles/traffic.json (KiB per dispatch, scaled by the copy-kernel
+    # calibration; per cell for the adaptive workload, whose launches vary
+    # in size), and are used only when those runs measured the kernel
+    # sources this run executes (same source hash); otherwise null.
     traffic, traffic_detail = None, None
     tpath = os.path.join(HERE, "profiles", "traffic.json")
+    src_sha = kernel_sources_sha()
     if os.path.exists(tpath):
         t = json.load(open(tpath)).get(args.workload, {}).get(dom)
         if t:
-            per_launch_cells = prof.cells[dom] / prof.launches[dom]
-            traffic = t["bytes_per_cell"] * per_launch_cells
-            traffic_detail = t
+            traffic_detail = dict(t)
+            if t.get("kernel_sources_sha") == src_sha:
+                per_launch_cells = prof.cells[dom] / prof.launches[dom]
+                traffic = t["bytes_per_cell"] * per_launch_cells
+                traffic_detail["measured_in"] = "separate counter runs of this command on the same kernel sources"
+            else:
+                traffic_detail["measured_in"] = (f"stale: counter runs of kernel sources {t.get('kernel_sources_sha')}, "
+                                                 f"this run executes {src_sha}; traffic not reported")
 
     scheme = "MUSCL+HLLC" if acc == 2 else "first-order HLLC"
     out = {
@@ -745,7 +877,7 @@ def main():
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "algorithmic_bytes_per_launch": alg_bytes, "avg_launch_ms": dom_ms,
-                     "traffic_detail": traffic_detail},
+                     "fused_kernel_minimum": fused, "traffic_detail": traffic_detail},
     }
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(args.workload, args.seed, args.cpu_budget)

@@ -29,8 +29,11 @@ enum {
     ZFV_ERR_STATE = 5        /* call order (no mesh, no state, not configured)        */
 };
 
-/* Numerical flux ids = zephyr::math::Fluxes values (math/cfd/flux/num_flux.h:11-21). */
-enum { ZFV_GODUNOV = 0, ZFV_RUSANOV = 1, ZFV_HLL = 2, ZFV_HLLC = 3, ZFV_CRP = 8 };
+/* Numerical flux ids = zephyr::math::Fluxes values (math/cfd/flux/num_flux.h:11-21).
+ * The single-material step takes GODUNOV..CIR2 (HLLC_M / HLLC_LM:
+ * hllc.cpp:427-572, CIR1 / CIR2: cir.cpp:16-216); CRP is MmFluid's. */
+enum { ZFV_GODUNOV = 0, ZFV_RUSANOV = 1, ZFV_HLL = 2, ZFV_HLLC = 3, ZFV_HLLC_M = 4, ZFV_HLLC_LM = 5,
+       ZFV_CIR1 = 6, ZFV_CIR2 = 7, ZFV_CRP = 8 };
 
 /* MmFluid's CrpMode (math/solver/mm_fluid.h:36-42) and DirSplit (:29-33). */
 enum { ZFV_CRP_NONE = 0, ZFV_CRP_V3 = 1, ZFV_CRP_V5 = 2, ZFV_CRP_VS = 3, ZFV_CRP_PLIC = 4, ZFV_CRP_MUSCL = 5 };
@@ -79,7 +82,7 @@ typedef struct {
  * and a stiffened-gas EOS {gamma, P0, e0} (IdealGas = P0 = e0 = 0). */
 typedef struct {
     int32_t acc;        /* 1 or 2 (clamped as set_accuracy)                 */
-    int32_t flux;       /* ZFV_GODUNOV..ZFV_HLLC                             */
+    int32_t flux;       /* ZFV_GODUNOV..ZFV_CIR2                             */
     int32_t limiter;    /* ZFV_LIM_*                                         */
     int32_t reserved;
     double cfl;         /* clamped to [0,1] as set_CFL                      */
@@ -116,6 +119,13 @@ int zfv_mesh_n_cells(zfv_ctx* ctx, int32_t* n);
  * boundary face sides; a hanging face counts once per fine side), the phi =
  * faces / cells of the bandwidth accounting.  No reference counterpart. */
 int zfv_mesh_n_faces(zfv_ctx* ctx, int64_t* n);
+/* Reserve device capacity for meshes of up to n_cells cells
+ * (AmrCells::reserve_amr, mesh/euler/amr_cells.cpp:566-575): every
+ * per-cell array, the adaptation's scratch and the curve-ordered copy of
+ * the mesh are sized for n_cells, so adaptations that stay below it
+ * allocate nothing.  Without it an adaptation that outgrows the capacity
+ * doubles it (one synchronisation).  Not available on a batch. */
+int zfv_mesh_reserve(zfv_ctx* ctx, int n_cells);
 /* Download the current mesh into caller arrays sized as zfv_mesh_soa
  * (pointers of a zfv_mesh_soa used as outputs; NULL members are skipped). */
 int zfv_mesh_download(zfv_ctx* ctx, const zfv_mesh_soa* out);
@@ -299,14 +309,22 @@ int zfv_mm_grad(zfv_ctx* ctx, double* gx, double* gy);
  * (one face each), 2 = one thread per cell with the main slots' loads
  * issued together (more registers per thread; for meshes that do not fill
  * the GPU), block = workgroup size 128, 256 or 512 (0 keeps the current
- * one).  All choices give bit-identical results. */
+ * one).  Layout 3 (stage 1 only) runs the whole MUSCL + HLLC step in one
+ * tiled kernel with the half states and gradients in LDS, on meshes built
+ * by zfv_mesh_rectangle / zfv_batch_rectangles (before any adaptation;
+ * ZFV_ERR_UNSUPPORTED otherwise); its block is the tile shape, 512, 768 or
+ * 1024 threads (6, 10 or 14 rows of 62 cells); it is the default there, and
+ * schemes it does not cover run the stage kernels.  All choices give
+ * bit-identical results. */
 int zfv_set_launch(zfv_ctx* ctx, int stage, int layout, int block);
 int zfv_get_launch(zfv_ctx* ctx, int stage, int* layout, int* block);
 /* Autotuner: times every (layout, workgroup size) of the configured scheme's
  * step kernels on the current state (`reps` launches each; the state is not
- * advanced) and keeps the fastest per stage.  ms_out (may be NULL): 27
- * doubles, ms per launch indexed [stage][layout][block 128/256/512]
- * (0 for stages the scheme does not use). */
+ * advanced), and the tiled step's shapes where it applies, and keeps the
+ * fastest per stage (the tiled step when it beats the best gradient +
+ * corrector pair).  ms_out (may be NULL): 36 doubles, ms per launch indexed
+ * [stage][layout 0-3][block 128/256/512, or for layout 3 the tile shape
+ * 512/768/1024] (0 where not timed). */
 int zfv_autotune(zfv_ctx* ctx, int reps, double* ms_out);
 /* Capture steps into HIP graphs (6 steps per graph, one graph per starting
  * phase, replayed by zfv_smf_step; re-captured after any change of mesh,

@@ -251,6 +251,187 @@ Flux rusanov(const PState& zL, const PState& zR, const Eos& eos) {
     return F;
 }
 
+// HLLC_M::calc_flux (math/cfd/flux/hllc.cpp:491-572): HLLC with the
+// tangential velocities averaged across the contact and the matching
+// energy correction; std::pow(x, 2) is x * x bit for bit.
+Flux hllc_m(const PState& zL, const PState& zR, const Eos& eos) {
+    double c_L = eos.sound_speed_rP(zL.rho, zL.p);
+    double c_R = eos.sound_speed_rP(zR.rho, zR.p);
+    double S_L = smin(zL.v.x - c_L, zR.v.x - c_R);
+    double S_R = smax(zL.v.x + c_L, zR.v.x + c_R);
+    if (S_L >= 0.0) { return phys_flux(zL); }
+    if (S_R <= 0.0) { return phys_flux(zR); }
+    double a_L = zL.rho * (S_L - zL.v.x);
+    double a_R = zR.rho * (S_R - zR.v.x);
+    double S_C = (zL.p - zR.p + a_R * zR.v.x - a_L * zL.v.x) / (a_R - a_L);
+    double V_C = (a_R * zR.v.y - a_L * zL.v.y) / (a_R - a_L);
+    double W_C = (a_R * zR.v.z - a_L * zL.v.z) / (a_R - a_L);
+    double rho_sL = a_L / (S_L - S_C);
+    double rho_sR = a_R / (S_R - S_C);
+    double E_sL = total_E(zL) + (S_C - zL.v.x) * (S_C + zL.p / a_L);
+    double E_sR = total_E(zR) + (S_C - zR.v.x) * (S_C + zR.p / a_R);
+    double v_L2 = zL.v.y * zL.v.y + zL.v.z * zL.v.z;
+    double v_R2 = zR.v.y * zR.v.y + zR.v.z * zR.v.z;
+    E_sL += 0.5 * ((a_R * v_R2 - a_L * v_L2) / (a_R - a_L) - v_L2);
+    E_sR += 0.5 * ((a_R * v_R2 - a_L * v_L2) / (a_R - a_L) - v_R2);
+    QState Q_sL{rho_sL, {rho_sL * S_C, rho_sL * V_C, rho_sL * W_C}, rho_sL * E_sL};
+    QState Q_sR{rho_sR, {rho_sR * S_C, rho_sR * V_C, rho_sR * W_C}, rho_sR * E_sR};
+    QState Q_L = to_qstate(zL), Q_R = to_qstate(zR);
+    Flux F_L = phys_flux(zL), F_R = phys_flux(zR);
+    double aSC = std::abs(S_C);
+    Flux F;
+    for (int k = 0; k < 5; ++k) {
+        double t = F_L.arr()[k] + F_R.arr()[k];
+        t = t + S_L * (Q_sL.arr()[k] - Q_L.arr()[k]);
+        t = t + aSC * (Q_sL.arr()[k] - Q_sR.arr()[k]);
+        t = t + S_R * (Q_sR.arr()[k] - Q_R.arr()[k]);
+        F.arr()[k] = 0.5 * t;
+        if (std::isnan(F.arr()[k])) { g_numeric_error = 1; }   // hllc.cpp:558-568 throws
+    }
+    return F;
+}
+
+// HLLC_LM::calc_flux (math/cfd/flux/hllc.cpp:427-488): HLLC whose outer
+// wave terms are scaled by phi = sin(pi/2 * Ma / 0.1) below Ma = 0.1 (1
+// above), Ma the larger |v_x| / c of the two sides.
+Flux hllc_lm(const PState& zL, const PState& zR, const Eos& eos) {
+    double c_L = eos.sound_speed_rP(zL.rho, zL.p);
+    double c_R = eos.sound_speed_rP(zR.rho, zR.p);
+    double S_L = smin(zL.v.x - c_L, zR.v.x - c_R);
+    double S_R = smax(zL.v.x + c_L, zR.v.x + c_R);
+    if (S_L >= 0.0) { return phys_flux(zL); }
+    if (S_R <= 0.0) { return phys_flux(zR); }
+    double a_L = zL.rho * (S_L - zL.v.x);
+    double a_R = zR.rho * (S_R - zR.v.x);
+    double S_C = (zL.p - zR.p + a_R * zR.v.x - a_L * zL.v.x) / (a_R - a_L);
+    double rho_sL = a_L / (S_L - S_C);
+    double rho_sR = a_R / (S_R - S_C);
+    double E_sL = total_E(zL) + (S_C - zL.v.x) * (S_C + zL.p / a_L);
+    double E_sR = total_E(zR) + (S_C - zR.v.x) * (S_C + zR.p / a_R);
+    QState Q_sL{rho_sL, {rho_sL * S_C, rho_sL * zL.v.y, rho_sL * zL.v.z}, rho_sL * E_sL};
+    QState Q_sR{rho_sR, {rho_sR * S_C, rho_sR * zR.v.y, rho_sR * zR.v.z}, rho_sR * E_sR};
+    QState Q_L = to_qstate(zL), Q_R = to_qstate(zR);
+    Flux F_L = phys_flux(zL), F_R = phys_flux(zR);
+    const double Ma_limit = 0.1;
+    double Ma_local = smax(std::abs(zL.v.x / c_L), std::abs(zR.v.x / c_R));
+    double xi = Ma_local / Ma_limit;
+    double phi = xi > 1.0 ? 1.0 : std::sin(M_PI_2 * xi);
+    double aSC = std::abs(S_C);
+    Flux F;
+    for (int k = 0; k < 5; ++k) {
+        double t = F_L.arr()[k] + F_R.arr()[k];
+        t = t + phi * S_L * (Q_sL.arr()[k] - Q_L.arr()[k]);
+        t = t + aSC * (Q_sL.arr()[k] - Q_sR.arr()[k]);
+        t = t + phi * S_R * (Q_sR.arr()[k] - Q_R.arr()[k]);
+        F.arr()[k] = 0.5 * t;
+        if (std::isnan(F.arr()[k])) { g_numeric_error = 1; }   // hllc.cpp:474-484 throws
+    }
+    return F;
+}
+
+// Arithmetic face state of the CIR fluxes (cir.cpp:22-27): the mean of the
+// two primitive vectors with the energy made consistent with (rho, p).
+inline PState cir_face_state(const PState& zL, const PState& zR, const Eos& eos) {
+    PState zE;
+    for (int k = 0; k < 6; ++k) zE.arr()[k] = 0.5 * (zL.arr()[k] + zR.arr()[k]);
+    zE.e = eos.energy_rP(zE.rho, zE.p);
+    return zE;
+}
+
+// CIR1::calc_flux (math/cfd/flux/cir.cpp:16-111): F = (F_L + F_R) / 2 +
+// |A| (q_L - q_R) / 2 with |A| = R |Lambda| L, the eigen-decomposition of
+// the flux Jacobian at the arithmetic face state.  The reference forms the
+// products with Eigen (OmR * abs_L * OmL, then abs_A * dq); this
+// restatement fixes the order: |A|(i,j) = sum over k ascending of
+// (R(i,k) |lambda_k|) L(k,j), (|A| dq)_i summed over j ascending.
+Flux cir1(const PState& zL, const PState& zR, const Eos& eos) {
+    const PState zE = cir_face_state(zL, zR, eos);
+    const double dPdE = (eos.gamma - 1.0) * zE.rho;   // pressure_re(..., deriv).dE
+    const double rho = zE.rho, u = zE.v.x, v = zE.v.y, w = zE.v.z, p = zE.p, eps = zE.e;
+    const double u2 = u * u, v2 = v * v, w2 = w * w;
+    const double q2 = u2 + v2 + w2;
+    const double E = rho * (eps + 0.5 * q2);
+    const double c = eos.sound_speed_rP(rho, p);
+    const double c2 = c * c;
+    const double h = (E + p) / rho;
+    const double b = dPdE / rho;
+    const double T = b * (q2 - h) + c2;
+    const double lam[5] = {std::abs(u - c), std::abs(u), std::abs(u), std::abs(u), std::abs(u + c)};
+    const double R[5][5] = {{1, 0, 0, 1, 1},
+                            {u - c, 0, 0, u, u + c},
+                            {v, 1, 0, v, v},
+                            {w, 0, 1, w, w},
+                            {h - u * c, v, w, h - c2 / b, h + u * c}};
+    double L[5][5] = {{T + u * c, -b * u - c, -b * v, -b * w, b},
+                      {-2 * v * c2, 0, 2 * c2, 0, 0},
+                      {-2 * w * c2, 0, 0, 2 * c2, 0},
+                      {2 * b * (h - q2), 2 * b * u, 2 * b * v, 2 * b * w, -2 * b},
+                      {T - u * c, -b * u + c, -b * v, -b * w, b}};
+    const double s2 = 2.0 * c2;
+    for (auto& row : L)
+        for (double& x : row) x = x / s2;
+    const QState qL = to_qstate(zL), qR = to_qstate(zR);
+    const Flux fL = phys_flux(zL), fR = phys_flux(zR);
+    double dq[5];
+    for (int k = 0; k < 5; ++k) dq[k] = qL.arr()[k] - qR.arr()[k];
+    Flux F;
+    for (int i = 0; i < 5; ++i) {
+        double t = 0.0;
+        for (int j = 0; j < 5; ++j) {
+            double a = 0.0;
+            for (int k = 0; k < 5; ++k) a = a + (R[i][k] * lam[k]) * L[k][j];
+            t = t + a * dq[j];
+        }
+        F.arr()[i] = 0.5 * (fL.arr()[i] + fR.arr()[i]) + 0.5 * t;
+    }
+    return F;
+}
+
+// CIR2::calc_flux (math/cfd/flux/cir.cpp:117-216): the same with the
+// pressure as a sixth variable, which needs no derivative of the EOS: a
+// 5 x 6 R, a 6 x 6 L, q extended by p.  The reference checks at run time
+// that R Lambda L is the extended Jacobian and R L the identity (1e-10
+// relative) and throws otherwise; both hold to round-off for every finite
+// state, and the restatement does not repeat the check.
+Flux cir2(const PState& zL, const PState& zR, const Eos& eos) {
+    const PState zE = cir_face_state(zL, zR, eos);
+    const double rho = zE.rho, u = zE.v.x, v = zE.v.y, w = zE.v.z, p = zE.p, eps = zE.e;
+    const double u2 = u * u, v2 = v * v, w2 = w * w;
+    const double q2 = u2 + v2 + w2;
+    const double E = rho * (eps + 0.5 * q2);
+    const double c = eos.sound_speed_rP(rho, p);
+    const double c2 = c * c;
+    const double h = (E + p) / rho;
+    const double lam[6] = {std::abs(u - c), std::abs(u), std::abs(u), std::abs(u), std::abs(u + c), std::abs(u)};
+    const double R[5][6] = {{1, 0, 0, 0, 1, 1},
+                            {u - c, 0, 0, 0, u + c, u},
+                            {v, 1, 0, 0, v, 0},
+                            {w, 0, 1, 0, w, 0},
+                            {h - u * c, 0, 0, 1, h + u * c, 0}};
+    const double L[6][6] = {{u / (2 * c), -1 / (2 * c), 0, 0, 0, 1 / (2 * c2)},
+                            {0, 0, 1, 0, 0, -v / c2},
+                            {0, 0, 0, 1, 0, -w / c2},
+                            {u2, -u, 0, 0, 1, -h / c2},
+                            {-u / (2 * c), 1 / (2 * c), 0, 0, 0, 1 / (2 * c2)},
+                            {1, 0, 0, 0, 0, -1 / c2}};
+    const QState qL = to_qstate(zL), qR = to_qstate(zR);
+    const Flux fL = phys_flux(zL), fR = phys_flux(zR);
+    double dq[6];
+    for (int k = 0; k < 5; ++k) dq[k] = qL.arr()[k] - qR.arr()[k];
+    dq[5] = zL.p - zR.p;
+    Flux F;
+    for (int i = 0; i < 5; ++i) {
+        double t = 0.0;
+        for (int j = 0; j < 6; ++j) {
+            double a = 0.0;
+            for (int k = 0; k < 6; ++k) a = a + (R[i][k] * lam[k]) * L[k][j];
+            t = t + a * dq[j];
+        }
+        F.arr()[i] = 0.5 * (fL.arr()[i] + fR.arr()[i]) + 0.5 * t;
+    }
+    return F;
+}
+
 // ---------------------------------------------------------------- exact Riemann
 // RiemannSolver (math/solver/riemann.cpp:12-460), stiffened-gas two-term EOS.
 inline double sqr(double x) { return x * x; }
@@ -429,6 +610,10 @@ Flux num_flux(int kind, const PState& zL, const PState& zR, const Eos& eos) {
         case ORC_GODUNOV: return godunov(zL, zR, eos);
         case ORC_RUSANOV: return rusanov(zL, zR, eos);
         case ORC_HLL:     return hll(zL, zR, eos);
+        case ORC_HLLC_M:  return hllc_m(zL, zR, eos);
+        case ORC_HLLC_LM: return hllc_lm(zL, zR, eos);
+        case ORC_CIR1:    return cir1(zL, zR, eos);
+        case ORC_CIR2:    return cir2(zL, zR, eos);
         default:          return hllc(zL, zR, eos);
     }
 }

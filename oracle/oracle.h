@@ -28,7 +28,9 @@ extern "C" {
 
 // Numerical flux ids: the values of zephyr::math::Fluxes
 // (reference zephyr/math/cfd/flux/num_flux.h:11-21).
-enum { ORC_GODUNOV = 0, ORC_RUSANOV = 1, ORC_HLL = 2, ORC_HLLC = 3 };
+/* Fluxes values (math/cfd/flux/num_flux.h:11-21) */
+enum { ORC_GODUNOV = 0, ORC_RUSANOV = 1, ORC_HLL = 2, ORC_HLLC = 3, ORC_HLLC_M = 4, ORC_HLLC_LM = 5, ORC_CIR1 = 6,
+       ORC_CIR2 = 7 };
 
 // Limiter ids (reference zephyr/math/cfd/limiter.cxx:163-252 name table).
 enum {
@@ -147,6 +149,7 @@ void orc_mm_split(const orc_mm* s, int c, int iA, double* out);
 void orc_mm_plant(int k);   /* test-only planted mistakes in the composite-flux chain, 0 = none */
 void orc_mm_set_limiter(orc_mm* s, int limiter);
 void orc_mm_set_accuracy(orc_mm* s, int acc);   /* 1 or 2 (clamped) */
+void orc_mm_set_threads(orc_mm* s, int nthreads);   /* OpenMP threads of the per-cell loops (default 1) */
 void orc_mm_compute_grad(orc_mm* s);
 void orc_mm_grad(const orc_mm* s, double* gx, double* gy);   /* [6 + 2 n_mat][n]: rho, u, v, e, beta, P, T, rhos */
 void orc_mm_set_flags(orc_mm* s);

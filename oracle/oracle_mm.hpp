@@ -927,6 +927,7 @@ struct orc_mm {
     std::vector<MState> init, half, next;   // MmFluid::Parts init / half / next (mm_fluid.h:53-67)
     std::vector<MG> d_dx, d_dy, d_dz; // Parts d_dx / d_dy / d_dz (components of MG)
     std::vector<double> nrm_x, nrm_y, pos;   // interface normals / positions [c][mat]
+    int nthreads = 1;   // OpenMP threads of the per-cell loops (each cell writes only itself)
 };
 
 namespace {
@@ -1248,8 +1249,12 @@ void orc_mm_set_accuracy(orc_mm* s, int acc) { s->acc = std::min(std::max(1, acc
 
 // MmFluid::compute_grad (mm_fluid.cpp:439-466) of the init layer.
 void orc_mm_compute_grad(orc_mm* s) {
-    for (int c = 0; c < s->m->n; ++c) mm_grad_cell(*s, c, s->d_dx[c], s->d_dy[c], s->d_dz[c]);
+    const int n = s->m->n;
+    #pragma omp parallel for num_threads(s->nthreads) schedule(static)
+    for (int c = 0; c < n; ++c) mm_grad_cell(*s, c, s->d_dx[c], s->d_dy[c], s->d_dz[c]);
 }
+
+void orc_mm_set_threads(orc_mm* s, int nthreads) { s->nthreads = std::max(1, nthreads); }
 
 // The stored gradients: rho, u, v, e, beta[n_mat], P, T, rhos[n_mat]
 // rows of n values each, x then y.
@@ -1332,8 +1337,13 @@ void orc_mm_state(const orc_mm* s, double* rho, double* u, double* v, double* P,
 // MmFluid::compute_dt (mm_fluid.cpp:166-194).
 double orc_mm_compute_dt(orc_mm* s) {
     const orc_mesh& m = *s->m;
-    double dtmin = std::numeric_limits<double>::max();
+    // per-cell candidates in parallel, then EuMesh::min over them in cell order
+    std::vector<double> cand(m.n);
+    int nerr = 0;
+    const int err0 = g_numeric_error;
+    #pragma omp parallel for num_threads(s->nthreads) schedule(static) reduction(|:nerr)
     for (int c = 0; c < m.n; ++c) {
+        g_numeric_error = 0;
         double dt = std::numeric_limits<double>::max();
         const MState& z = s->init[c];
         MixOpt o;
@@ -1348,8 +1358,12 @@ double orc_mm_compute_dt(orc_mm* s) {
             const double lambda = std::abs(vn) + cs;
             dt = smin(dt, m.vol[c] / (m.area[f] * lambda));
         }
-        dtmin = smin(dtmin, dt);   // EuMesh::min
+        cand[c] = dt;
+        nerr |= g_numeric_error;
     }
+    g_numeric_error = err0 | nerr;
+    double dtmin = std::numeric_limits<double>::max();
+    for (int c = 0; c < m.n; ++c) dtmin = smin(dtmin, cand[c]);   // EuMesh::min
     s->dt = smin(s->cfl * dtmin, s->max_dt);
     return s->dt;
 }
@@ -1376,8 +1390,11 @@ static int mm_integrate1(orc_mm* s, double dt, int dir) {
         ip.resize((size_t)nm * m.n);
         orc_mm_interface(s, inx.data(), iny.data(), ip.data());
     }
-    int neg = 0, crp_err = 0;
+    int neg = 0, crp_err = 0, nerr = 0;
+    const int err0 = g_numeric_error;   // the loop resets this thread's copy per cell
+    #pragma omp parallel for num_threads(s->nthreads) schedule(static) reduction(|:neg, crp_err, nerr)
     for (int c = 0; c < m.n; ++c) {
+        g_numeric_error = 0;   // thread-local; gathered below
         const double V_c = m.vol[c];
         const MState z_c = s->init[c];
         MArr flux;
@@ -1403,7 +1420,9 @@ static int mm_integrate1(orc_mm* s, double dt, int dir) {
         for (int q = 0; q < 9; ++q) q_c.a[q] -= coef * flux.a[q];
         if (q_c.a[0] < 0.0) neg = 1;
         s->next[c] = m_from_q(q_c, s->mix, z_c.P, z_c.T, z_c.rhos);
+        nerr |= g_numeric_error;
     }
+    g_numeric_error = err0 | nerr;
     std::swap(s->init, s->next);   // MmFluid::swap (mm_fluid.cpp:682-684)
     return (neg || crp_err) ? 1 : 0;
 }
@@ -1424,7 +1443,11 @@ static int mm_integrate2(orc_mm* s, double dt, int dir) {
     }
     // stage 1: mixed cells keep their state; a pure cell advances half a
     // step with the physical flux of its own extrapolated face states
+    int nerr = 0;
+    const int err0 = g_numeric_error;   // the loops reset this thread's copy per cell
+    #pragma omp parallel for num_threads(s->nthreads) schedule(static) reduction(|:nerr)
     for (int c = 0; c < m.n; ++c) {
+        g_numeric_error = 0;
         const MState z_c = s->init[c];
         if (z_c.beta.index() < 0) {
             s->half[c] = z_c;
@@ -1447,12 +1470,15 @@ static int mm_integrate2(orc_mm* s, double dt, int dir) {
         const double coef = 0.5 * dt / m.vol[c];
         for (int q = 0; q < 9; ++q) q_c.a[q] -= coef * flux.a[q];
         s->half[c] = m_from_q(q_c, s->mix, z_c.P, z_c.T, z_c.rhos);
+        nerr |= g_numeric_error;
     }
     // stage 2: face states extrapolated from the half states with the
     // gradients of init; a periodic face's neighbour centre is the
     // neighbour's own (face.is_boundary() is false there, :599-604)
     int neg = 0, crp_err = 0;
+    #pragma omp parallel for num_threads(s->nthreads) schedule(static) reduction(|:neg, crp_err, nerr)
     for (int c = 0; c < m.n; ++c) {
+        g_numeric_error = 0;
         const Vec3 cell_c{m.cx[c], m.cy[c], 0.0};
         const double V_c = m.vol[c];
         const MState z_c = s->init[c];
@@ -1489,7 +1515,9 @@ static int mm_integrate2(orc_mm* s, double dt, int dir) {
         for (int q = 0; q < 9; ++q) q_c.a[q] -= coef * flux.a[q];
         if (q_c.a[0] < 0.0) neg = 1;   // :671-673 throws
         s->next[c] = m_from_q(q_c, s->mix, z_c.P, z_c.T, z_c.rhos);
+        nerr |= g_numeric_error;
     }
+    g_numeric_error = err0 | nerr;
     std::swap(s->init, s->next);
     return (neg || crp_err) ? 1 : 0;
 }
@@ -1524,6 +1552,7 @@ double orc_mm_dt(const orc_mm* s) { return s->dt; }
 void orc_mm_interface(orc_mm* s, double* nx, double* ny, double* p) {
     const orc_mesh& m = *s->m;
     const int n = m.n, nm = s->mix.n;
+    #pragma omp parallel for num_threads(s->nthreads) schedule(static)
     for (int c = 0; c < n; ++c) {
         Vec3 ns[kMaxMat];
         double ps[kMaxMat];

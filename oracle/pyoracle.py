@@ -13,7 +13,7 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB_PATH = os.path.join(_HERE, "_build", "liborc.so")
 
-GODUNOV, RUSANOV, HLL, HLLC = 0, 1, 2, 3
+GODUNOV, RUSANOV, HLL, HLLC, HLLC_M, HLLC_LM, CIR1, CIR2 = 0, 1, 2, 3, 4, 5, 6, 7
 LIMITERS = ["none", "unit", "minmod", "MC", "vanLeer", "vanAlbada1", "vanAlbada2",
             "CHARM", "HCUS", "HQUICK", "Koren", "smart", "superbee"]
 UNDEFINED, INNER, WALL, ZOE, PERIODIC = -1, 0, 1, 2, 3
@@ -94,6 +94,7 @@ def lib():
             "orc_mm_crp_sub": (_I, [_P, _I, _I, _I, _I, _D, _D, _dp]),
             "orc_mm_set_limiter": (None, [_P, _I]),
             "orc_mm_set_accuracy": (None, [_P, _I]),
+            "orc_mm_set_threads": (None, [_P, _I]),
             "orc_mm_compute_grad": (None, [_P]),
             "orc_mm_grad": (None, [_P, _dp, _dp]),
             "orc_mm_set_flags": (None, [_P]),
@@ -414,6 +415,10 @@ class MmFluid:
 
     def compute_grad(self):
         lib().orc_mm_compute_grad(self.h)
+
+    def set_threads(self, nthreads):
+        """OpenMP threads of the per-cell loops (the results do not depend on it)."""
+        lib().orc_mm_set_threads(self.h, int(nthreads))
 
     def set_accuracy(self, acc):
         """1: the first-order update; 2: compute_grad + fluxes_stage1/2."""

@@ -140,3 +140,39 @@ def test_adaptive_run_within_tolerance(oracle, zfv):
                               m.slots()["adj"][m.slots()["boundary"] >= 0]), f"step {it}: adjacency"
         for a, b in zip(ctx.state(), s.state()):
             np.testing.assert_allclose(a, b, rtol=1e-12, atol=1e-12 * np.abs(b).max())
+
+
+def test_reserve_keeps_results(zfv):
+    """zfv_mesh_reserve (AmrCells::reserve_amr) changes the capacity only: an
+    adaptive MUSCL run with capacity reserved up front, or reserved again in
+    the middle of the run (state held in curve order), is bit-identical to
+    one whose capacity grows during its adaptations; a batch refuses it."""
+    from zephyr_amd import problems
+    out = []
+    for plan in ((None, None), (4096, None), (1100, 5000)):
+        ctx = zfv.Context()
+        ctx.mesh_rectangle(32, 32)
+        ctx.configure(acc=2, flux="hllc", limiter="minmod", cfl=0.5, gamma=GAMMA)
+        cells, _, _ = ctx.mesh_download()
+        ctx.set_state(*problems.circle_blast(cells["cx"], cells["cy"], 5))
+        if plan[0]:
+            ctx.reserve(plan[0])
+        for k in range(6):
+            ctx.set_flags(1, 0.1)
+            ctx.refine(3, slope=True)
+            ctx.step(3)
+            if k == 2 and plan[1]:
+                ctx.reserve(plan[1])
+        cells, slots, _ = ctx.mesh_download()
+        out.append((ctx.state(), ctx.dt(), cells["level"], slots["adj"]))
+        ctx.close()
+    assert out[0][2].max() == 3 and len(out[0][2]) > 1100
+    for st, dt, lvl, adj in out[1:]:
+        assert dt == out[0][1]
+        assert np.array_equal(lvl, out[0][2]) and np.array_equal(adj, out[0][3])
+        for a, b in zip(st, out[0][0]):
+            assert np.array_equal(a, b)
+    b = zfv.Context()
+    b.batch_rectangles([(8, 8), (4, 4)])
+    with pytest.raises(zfv.ZfvError):
+        b.reserve(1000)

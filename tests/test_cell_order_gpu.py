@@ -95,7 +95,7 @@ def test_curve_order_with_graphs_and_autotune(zfv):
     cur.step(2)
     ref.step(2)
     ms = cur.autotune(reps=2)
-    assert ms.shape == (3, 3, 3)
+    assert ms.shape == (3, 4, 3)
     for k in (13, 6, 7):
         cur.step(k)
         ref.step(k)

@@ -13,8 +13,9 @@ from . import build as _build
 
 LIB_PATH = _build.LIB_PATH
 
-GODUNOV, RUSANOV, HLL, HLLC = 0, 1, 2, 3
-FLUXES = {"godunov": GODUNOV, "rusanov": RUSANOV, "hll": HLL, "hllc": HLLC}
+GODUNOV, RUSANOV, HLL, HLLC, HLLC_M, HLLC_LM, CIR1, CIR2 = 0, 1, 2, 3, 4, 5, 6, 7
+FLUXES = {"godunov": GODUNOV, "rusanov": RUSANOV, "hll": HLL, "hllc": HLLC, "hllc_m": HLLC_M,
+          "hllc_lm": HLLC_LM, "cir1": CIR1, "cir2": CIR2}
 UNDEFINED, INNER, WALL, ZOE, PERIODIC, FUNCTION = -1, 0, 1, 2, 3, 4
 
 ZFV_OK, ZFV_ERR_ARG, ZFV_ERR_HIP, ZFV_ERR_NUMERIC, ZFV_ERR_UNSUPPORTED, ZFV_ERR_STATE = range(6)
@@ -29,7 +30,7 @@ EXPORTS = [
     "zfv_smf_compute_grad", "zfv_smf_compute_dt", "zfv_conserved_totals",
     "zfv_limiter_from_name", "zfv_profile_enable", "zfv_profile_read",
     "zfv_flags_upload", "zfv_flags_download", "zfv_set_flags", "zfv_balance_flags", "zfv_refine",
-    "zfv_mesh_n_faces", "zfv_set_launch", "zfv_get_launch", "zfv_autotune",
+    "zfv_mesh_n_faces", "zfv_mesh_reserve", "zfv_set_launch", "zfv_get_launch", "zfv_autotune",
     "zfv_batch_rectangles", "zfv_batch_layout", "zfv_batch_dt", "zfv_graph_enable",
     "zfv_set_cell_order", "zfv_get_cell_order",
     "zfv_mm_set_limiter", "zfv_mm_set_accuracy", "zfv_mm_set_method", "zfv_mm_set_crp_mode",
@@ -38,7 +39,7 @@ EXPORTS = [
     "zfv_mm_configure", "zfv_mm_set_crp", "zfv_mm_set_max_dt", "zfv_mm_set_state", "zfv_mm_state", "zfv_mm_step", "zfv_mm_dt", "zfv_mm_crp_faces",
     "zfv_mm_interface",
 ]
-CELL, QUAD, BATCH = 0, 1, 2   # step-kernel layouts
+CELL, QUAD, BATCH, TILE = 0, 1, 2, 3   # step-kernel layouts (TILE: the whole MUSCL step, stage 1)
 
 
 class ZfvError(RuntimeError):
@@ -94,6 +95,7 @@ def lib():
         L.zfv_set_flags.argtypes = [vp, C.c_int, d]
         L.zfv_balance_flags.argtypes = [vp, C.c_int]
         L.zfv_refine.argtypes = [vp, C.c_int, C.c_int]
+        L.zfv_mesh_reserve.argtypes = [vp, C.c_int]
         L.zfv_set_launch.argtypes = [vp, C.c_int, C.c_int, C.c_int]
         L.zfv_autotune.argtypes = [vp, C.c_int, vp]
         L.zfv_batch_rectangles.argtypes = [vp, C.c_int, vp, vp, vp, vp]
@@ -191,6 +193,10 @@ class Context:
         v = C.c_int64()
         _check(lib().zfv_mesh_n_faces(self.h, C.byref(v)))
         return v.value
+
+    def reserve(self, n_cells):
+        """Device capacity for meshes of up to n_cells cells (AmrCells::reserve_amr)."""
+        _check(lib().zfv_mesh_reserve(self.h, int(n_cells)))
 
     def mesh_download(self):
         n = C.c_int32()
@@ -307,10 +313,11 @@ class Context:
         return v.value, b.value
 
     def autotune(self, reps=5):
-        """ms per launch, [stage][layout][block 128/256/512]."""
-        ms = np.zeros(27)
+        """ms per launch, [stage][layout 0-3][block 128/256/512, or the tile
+        shape 512/768/1024 for layout 3]."""
+        ms = np.zeros(36)
         _check(lib().zfv_autotune(self.h, reps, _ptr(ms)))
-        return ms.reshape(3, 3, 3)
+        return ms.reshape(3, 4, 3)
 
     def graph_enable(self, on=True):
         _check(lib().zfv_graph_enable(self.h, 1 if on else 0))

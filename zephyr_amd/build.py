@@ -7,6 +7,7 @@ operations as the reference's x86-64 build (CMakeLists.txt:16-19, no -march).
 import os
 import subprocess
 import sys
+from concurrent.futures import ThreadPoolExecutor
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
@@ -19,6 +20,7 @@ SOURCES = [
     os.path.join(HERE, "csrc", "zfv_adjoint.hip"),
     os.path.join(HERE, "csrc", "zfv_mm.hip"),
     os.path.join(HERE, "csrc", "zfv_order.hip"),
+    os.path.join(HERE, "csrc", "zfv_tile.hip"),
     os.path.join(HERE, "csrc", "zfv_capi.cpp"),
 ]
 HEADERS = [
@@ -26,6 +28,7 @@ HEADERS = [
     os.path.join(HERE, "csrc", "zfv_internal.hpp"),
     os.path.join(HERE, "csrc", "zfv_adjoint_ops.hpp"),
     os.path.join(HERE, "csrc", "zfv_step_common.hpp"),
+    os.path.join(HERE, "csrc", "zfv_step_faces.hpp"),
     os.path.join(HERE, "csrc", "zfv_mm.hpp"),
     os.path.join(ROOT, "include", "zfv.h"),
 ]
@@ -35,11 +38,14 @@ ARCH = os.environ.get("ZFV_OFFLOAD_ARCH", "gfx950")
 
 FLAGS = [
     "--offload-arch=" + ARCH,
-    "-O3", "-std=c++17", "-fPIC", "-shared",
+    "-O3", "-std=c++17", "-fPIC",
     "-ffp-contract=off", "-fno-fast-math",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-I" + os.path.join(ROOT, "include"),
 ]
+# Sources compile one per process, then link; the translation units share no
+# device symbols, so no relocatable device code is needed.
+JOBS = max(1, min(len(SOURCES), int(os.environ.get("MAX_JOBS", os.cpu_count() or 1))))
 
 
 EXAMPLES = {
@@ -61,8 +67,15 @@ def _run(cmd, what, verbose):
 def build(verbose=False):
     """Compile libzfv.so and the host examples; always recompiles (the build check)."""
     os.makedirs(BUILD_DIR, exist_ok=True)
+    objs = [os.path.join(BUILD_DIR, os.path.basename(src) + ".o") for src in SOURCES]
+
+    def compile_one(i):
+        _run([HIPCC] + FLAGS + ["-c", SOURCES[i], "-o", objs[i]], "hipcc failed compiling " + SOURCES[i], verbose)
+
+    with ThreadPoolExecutor(JOBS) as pool:
+        list(pool.map(compile_one, range(len(SOURCES))))
     tmp = LIB_PATH + ".tmp"
-    _run([HIPCC] + FLAGS + ["-o", tmp] + SOURCES, "hipcc failed building libzfv.so", verbose)
+    _run([HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", tmp] + objs, "hipcc failed linking libzfv.so", verbose)
     os.replace(tmp, LIB_PATH)
     for name, src in EXAMPLES.items():
         _run(["g++", "-O2", "-std=c++17", "-I" + os.path.join(ROOT, "include"), "-I" + os.path.join(HERE, "host"),

@@ -206,7 +206,9 @@ __global__ void __launch_bounds__(256) k_amr_criterion(AmrMesh A, DevState z, De
 }
 
 // base_restriction (balancing_restrictions.h:21-47): reads the unrestricted
-// flags, writes the restricted ones to flag0 (snapshot of the sort phase).
+// flags (can_coarse looks at the siblings' ones), writes the restricted ones
+// to flag0; the balancing rounds then work on flag0 in place, and the caller
+// swaps flag and flag0.
 __global__ void __launch_bounds__(256) k_amr_restrict(AmrMesh A, int max_level) {
     const int ic = cell_id();
     if (ic >= A.n) return;
@@ -219,55 +221,54 @@ __global__ void __launch_bounds__(256) k_amr_restrict(AmrMesh A, int max_level) 
     A.flag0[ic] = flag;
 }
 
-__global__ void __launch_bounds__(256) k_amr_copy_flags(AmrMesh A) {
-    const int ic = cell_id();
-    if (ic < A.n) A.flag[ic] = A.flag0[ic];
+// Largest wanted level (level + flag) over the face neighbours of cell ic
+// (inner and periodic slots); every adjacency load is issued before any
+// neighbour's level and flag, so a cell costs two dependent round trips.
+__device__ __forceinline__ int max_wanted(const AmrMesh& A, const int* flag, int ic, int code) {
+    int jc[8];
+#pragma unroll
+    for (int s = 0; s < 8; ++s) {
+        const int t = (code >> (4 * s)) & 7;
+        jc[s] = (t == SC_INNER || t == SC_PERIODIC) ? A.adj[SI(A, ic, s)] : -1;
+    }
+    int w = -1;
+#pragma unroll
+    for (int s = 0; s < 8; ++s)
+        if (jc[s] >= 0) w = max(w, A.level[jc[s]] + flag[jc[s]]);
+    return w;
 }
 
-// One balancing round at level lvl (balancing_fast.h:273-390): round 1 on
-// the cells that kept 0, rounds 2-3 and 4 on the cells restricted to -1.
-__global__ void __launch_bounds__(256) k_amr_round(AmrMesh A, int lvl, int round) {
+// The balancing rounds of one level (balancing_fast.h:273-390) on the
+// restricted flags, in place.  The reference runs round 1 on the cells that
+// kept 0 (one becomes 1 if a neighbour wants more than one level finer),
+// rounds 2 and 3 on the cells restricted to -1 (1 as in round 1, else 0 if
+// a neighbour wants a finer level than the cell's), round 4 on the cells
+// still at -1 (0 if a sibling does not coarsen).  PHASE 0 runs rounds 1 and
+// 2 together, PHASE 1 round 3, PHASE 2 round 4.  The flags at the end equal
+// the reference's: a 1 depends only on finer neighbours, whose flags are
+// final before this level's rounds; a 0 of rounds 2-3 depends also on
+// same-level neighbours turning 1 during rounds 1-2, and round 3 reads them
+// all after the launch that set them, in both orders; a cell's own flag
+// changes only in its own thread (0 -> 1, -1 -> 0 or 1), so its value at
+// the start of a launch tells which round applies, as the snapshot did.
+template <int PHASE>
+__global__ void __launch_bounds__(256) k_amr_round(AmrMesh A, int lvl) {
     const int ic = cell_id();
-    if (ic >= A.n || A.level[ic] != lvl) return;
-    const int f0 = A.flag0[ic];
-    if (round == 1) {
-        if (f0 != 0) return;
-        for (int s = 0; s < 8; ++s) {
-            const int b = A.bnd[SI(A, ic, s)];
-            if (b == B_UNDEF || is_bnd(b)) continue;
-            const int jc = A.adj[SI(A, ic, s)];
-            if (A.level[jc] + A.flag[jc] > lvl + 1) {
-                A.flag[ic] = 1;
-                return;
-            }
-        }
-        return;
-    }
-    if (f0 != -1 || A.flag[ic] >= 0) return;
-    if (round == 4) {
+    if (ic >= A.n) return;
+    int* flag = A.flag0;
+    const int L = A.level[ic], f = flag[ic], code = A.code[ic];
+    if (L != lvl) return;
+    if (PHASE == 2) {
+        if (f != -1) return;
         int sibs[3];
         get_siblings(A, ic, sibs);
-        for (int k = 0; k < 3; ++k) {
-            if (A.flag[sibs[k]] >= 0) {
-                A.flag[ic] = 0;
-                break;
-            }
-        }
+        if (flag[sibs[0]] >= 0 || flag[sibs[1]] >= 0 || flag[sibs[2]] >= 0) flag[ic] = 0;
         return;
     }
-    for (int s = 0; s < 8; ++s) {
-        const int b = A.bnd[SI(A, ic, s)];
-        if (b == B_UNDEF || is_bnd(b)) continue;
-        const int jc = A.adj[SI(A, ic, s)];
-        const int wanted = A.level[jc] + A.flag[jc];
-        if (wanted > lvl) {
-            if (wanted > lvl + 1) {
-                A.flag[ic] = 1;
-                return;
-            }
-            A.flag[ic] = 0;
-        }
-    }
+    if (PHASE == 0 ? (f != 0 && f != -1) : f != -1) return;
+    const int w = max_wanted(A, flag, ic, code);
+    if (w > lvl + 1) flag[ic] = 1;
+    else if (f == -1 && w > lvl) flag[ic] = 0;
 }
 
 // Statistics (statistics.h:59-81), setup_positions (setup_positions.h:24-64)
@@ -354,44 +355,109 @@ __global__ void __launch_bounds__(256) k_amr_swapmap(AmrMesh A, int n_short, int
 // ---------------------------------------------------------------- geometry
 namespace {
 
-// retain_cell (retain.h:17-216).
+// setup_face_geom with view = SqQuad centre (faces.h:57-97), from the
+// cell's vertices held in registers.
+__device__ __forceinline__ void face_geom_v(const AmrMesh& A, int ic, int slot, const V2* V, int a, int b) {
+    double area, fx, fy, nx, ny;
+    line_geom(V[a], V[b], V[4], area, fx, fy, nx, ny);
+    const int f = SI(A, ic, slot);
+    A.area[f] = area;
+    A.fcx[f] = fx;
+    A.fcy[f] = fy;
+    A.nx[f] = nx;
+    A.ny[f] = ny;
+}
+
+// retain_cell (retain.h:17-216).  The loads come in three dependent batches
+// (the cell's slots; the neighbours' level, flag and next; the second
+// `next` lookups), all issued before the first store: the slot arrays are
+// written with the types of the arrays the lookups read, so stores between
+// them would serialise every side's chain.
 __device__ void retain_cell(const AmrMesh& A, int ic) {
-    const int lvl_c = A.level[ic];
+    enum { K_NONE, K_SET, K_SPLIT, K_MERGE, K_SUBS };
+    const int lvl_c = A.level[ic], z3 = A.z_idx[ic] & 3, next_c = A.next[ic];
+    int b0[4], b1[4], a0[4], a1[4];
+#pragma unroll
     for (int side = 0; side < 4; ++side) {
-        const int iface = SI(A, ic, side);
-        const int b = A.bnd[iface];
-        if (b == B_UNDEF) continue;
-        if (is_bnd(b)) {
-            A.adj[iface] = A.next[ic];
-            continue;
-        }
-        const int jc = A.adj[iface];
-        const int lvl_n = A.level[jc];
-        const int fl = A.flag[jc];
-        if (lvl_c == lvl_n) {
-            const int neib_next = A.next[jc];
-            if (fl > 0) {
-                split_face(A, ic, side);
-                A.adj[SI(A, ic, side)] = A.next[neib_next + cNeibChild0[side]];
-                A.adj[SI(A, ic, side + 4)] = A.next[neib_next + cNeibChild0[side + 4]];
-            } else if (fl == 0) {
-                A.adj[iface] = neib_next;
+        b0[side] = A.bnd[SI(A, ic, side)];
+        b1[side] = A.bnd[SI(A, ic, side + 4)];
+        a0[side] = A.adj[SI(A, ic, side)];
+        a1[side] = A.adj[SI(A, ic, side + 4)];
+    }
+    int ln[4], fl[4], nn[4];
+#pragma unroll
+    for (int side = 0; side < 4; ++side) {
+        const bool inner = b0[side] != B_UNDEF && !is_bnd(b0[side]);
+        const int jc = a0[side];
+        ln[side] = inner ? A.level[jc] : 0;
+        fl[side] = inner ? A.flag[jc] : 0;
+        nn[side] = inner ? A.next[jc] : 0;
+    }
+    int kind[4], r0[4], r1[4];
+    bool geo = false;
+#pragma unroll
+    for (int side = 0; side < 4; ++side) {
+        kind[side] = K_NONE;
+        r0[side] = r1[side] = -1;
+        if (b0[side] == B_UNDEF || is_bnd(b0[side])) continue;
+        if (lvl_c == ln[side]) {
+            if (fl[side] > 0) {
+                kind[side] = K_SPLIT;
+                r0[side] = A.next[nn[side] + cNeibChild0[side]];
+                r1[side] = A.next[nn[side] + cNeibChild0[side + 4]];
             } else {
-                A.adj[iface] = A.next[neib_next];
+                kind[side] = K_SET;
+                r0[side] = fl[side] == 0 ? nn[side] : A.next[nn[side]];
             }
-        } else if (lvl_c < lvl_n) {
-            if (fl < 0) {
-                merge_faces(A, ic, side);
-                A.adj[iface] = A.next[A.next[jc]];
+        } else if (lvl_c < ln[side]) {
+            if (fl[side] < 0) {
+                kind[side] = K_MERGE;
+                r0[side] = A.next[nn[side]];
             } else {
-                for (int sub = side; sub < 8; sub += 4) {
-                    const int jf = SI(A, ic, sub);
-                    if (A.bnd[jf] != B_UNDEF) A.adj[jf] = A.next[A.adj[jf]];
-                }
+                kind[side] = K_SUBS;
+                r0[side] = A.next[a0[side]];
+                if (b1[side] != B_UNDEF) r1[side] = A.next[a1[side]];
             }
         } else {
-            if (fl == 0) A.adj[iface] = A.next[jc];
-            else A.adj[iface] = A.next[A.next[jc] + adjacent_child(side, A.z_idx[ic] & 3)];
+            kind[side] = K_SET;
+            r0[side] = fl[side] == 0 ? nn[side] : A.next[nn[side] + adjacent_child(side, z3)];
+        }
+        geo = geo || kind[side] == K_SPLIT || kind[side] == K_MERGE;
+    }
+    V2 V[9];
+    if (geo) {
+#pragma unroll
+        for (int k = 0; k < 9; ++k) V[k] = vtx(A, ic, k);
+    }
+#pragma unroll
+    for (int side = 0; side < 4; ++side) {
+        const int f0 = SI(A, ic, side), f1 = SI(A, ic, side + 4);
+        if (b0[side] == B_UNDEF) continue;
+        if (is_bnd(b0[side])) {
+            A.adj[f0] = next_c;
+            continue;
+        }
+        switch (kind[side]) {
+            case K_SPLIT:   // split_face (faces.h:121-130), then the two sub-neighbours
+                A.bnd[f1] = b0[side];
+                face_geom_v(A, ic, side, V, cCF[side][0], cCF[side][1]);
+                face_geom_v(A, ic, side + 4, V, cCF[side + 4][0], cCF[side + 4][1]);
+                A.adj[f0] = r0[side];
+                A.adj[f1] = r1[side];
+                break;
+            case K_MERGE:   // merge_faces (faces.h:137-154)
+                A.bnd[f1] = B_UNDEF;
+                A.adj[f1] = -1;
+                face_geom_v(A, ic, side, V, cSF[side][0], cSF[side][1]);
+                A.adj[f0] = r0[side];
+                break;
+            case K_SUBS:
+                A.adj[f0] = r0[side];
+                if (b1[side] != B_UNDEF) A.adj[f1] = r1[side];
+                break;
+            default:
+                A.adj[f0] = r0[side];
+                break;
         }
     }
 }
@@ -598,34 +664,53 @@ __global__ void __launch_bounds__(256) k_amr_move(AmrMesh A, DevState z, int* wa
     const int n_swap = n_short >= A.n ? nz_total : rank[n_short].nz;   // the holes
     if (i >= n_swap) return;
     const int from = swap_src[i], to = swap_dst[i];
+    // each lane loads everything it copies before its first store (sources
+    // and destinations are distinct cells, but of the same array types)
     if (k < 8) {
         const int a = SI(A, from, k), b = SI(A, to, k);
-        A.bnd[b] = A.bnd[a];
-        A.adj[b] = A.adj[a];
-        A.nx[b] = A.nx[a];
-        A.ny[b] = A.ny[a];
-        A.fcx[b] = A.fcx[a];
-        A.fcy[b] = A.fcy[a];
-        A.area[b] = A.area[a];
+        const int bnd = A.bnd[a], adj = A.adj[a];
+        const double nx = A.nx[a], ny = A.ny[a], fcx = A.fcx[a], fcy = A.fcy[a], area = A.area[a];
+        A.bnd[b] = bnd;
+        A.adj[b] = adj;
+        A.nx[b] = nx;
+        A.ny[b] = ny;
+        A.fcx[b] = fcx;
+        A.fcy[b] = fcy;
+        A.area[b] = area;
     } else if (k == 8) {
+        double vx[9], vy[9];
+#pragma unroll
         for (int v = 0; v < 9; ++v) {
-            A.vx[v * A.stride + to] = A.vx[v * A.stride + from];
-            A.vy[v * A.stride + to] = A.vy[v * A.stride + from];
+            vx[v] = A.vx[v * A.stride + from];
+            vy[v] = A.vy[v * A.stride + from];
+        }
+#pragma unroll
+        for (int v = 0; v < 9; ++v) {
+            A.vx[v * A.stride + to] = vx[v];
+            A.vy[v * A.stride + to] = vy[v];
         }
     } else {
+        const int flag = A.flag[from], level = A.level[from], b_idx = A.b_idx[from], z_idx = A.z_idx[from];
+        const double cx = A.cx[from], cy = A.cy[from], vol = A.vol[from];
+        Z q{};
+        int w = 0;
+        if (z.r) {   // mesh only: the solver moves its own rows
+            q = Z{z.r[from], z.u[from], z.v[from], z.p[from], z.e[from]};
+            w = wait[from];
+        }
         A.next[to] = to;
         A.idx[to] = to;
-        A.flag[to] = A.flag[from];
-        A.level[to] = A.level[from];
-        A.b_idx[to] = A.b_idx[from];
-        A.z_idx[to] = A.z_idx[from];
-        A.cx[to] = A.cx[from];
-        A.cy[to] = A.cy[from];
-        A.vol[to] = A.vol[from];
+        A.flag[to] = flag;
+        A.level[to] = level;
+        A.b_idx[to] = b_idx;
+        A.z_idx[to] = z_idx;
+        A.cx[to] = cx;
+        A.cy[to] = cy;
+        A.vol[to] = vol;
         A.idx[from] = -1;
-        if (z.r) {   // mesh only: the solver moves its own rows
-            z.r[to] = z.r[from]; z.u[to] = z.u[from]; z.v[to] = z.v[from]; z.p[to] = z.p[from]; z.e[to] = z.e[from];
-            wait[to] = wait[from];
+        if (z.r) {
+            z.r[to] = q.r; z.u[to] = q.u; z.v[to] = q.v; z.p[to] = q.p; z.e[to] = q.e;
+            wait[to] = w;
         }
     }
 }
@@ -654,61 +739,81 @@ __global__ void __launch_bounds__(256) k_amr_derive(AmrMesh A, DevState z, Eos e
     double cand = INFINITY;
     if (c < A.n) {
         const double kNX[4] = {-1.0, 1.0, 0.0, 0.0}, kNY[4] = {0.0, 0.0, -1.0, 1.0};
-        int cw = 0;
+        // loads in three dependent batches (slot ids; the defined slots'
+        // data; the inner neighbours' centres), then the stores: the
+        // weights are stored into an array of the loads' type
         const double cx = A.cx[c], cy = A.cy[c];
+        const V2 va = vtx(A, c, 5), vb = vtx(A, c, 3), vp = vtx(A, c, 7), vq = vtx(A, c, 1);
+        Z zc{};
+        if (z.r) zc = Z{z.r[c], z.u[c], z.v[c], z.p[c], z.e[c]};
+        int t[8], nb[8];
+        double nx[8], ny[8], fcx[8], fcy[8], area[8], px[8], py[8];
+#pragma unroll
+        for (int s = 0; s < 8; ++s) {
+            switch (A.bnd[SI(A, c, s)]) {
+                case B_UNDEF: t[s] = SC_UNDEF; break;
+                case B_INNER: t[s] = SC_INNER; break;
+                case B_PERIODIC: t[s] = SC_PERIODIC; break;
+                case 1: t[s] = SC_WALL; break;
+                default: t[s] = SC_COPY; break;
+            }
+        }
+#pragma unroll
         for (int s = 0; s < 8; ++s) {
             const int f = SI(A, c, s);
-            const int b = A.bnd[f];
-            int t;
-            switch (b) {
-                case B_UNDEF: t = SC_UNDEF; break;
-                case B_INNER: t = SC_INNER; break;
-                case B_PERIODIC: t = SC_PERIODIC; break;
-                case 1: t = SC_WALL; break;
-                default: t = SC_COPY; break;
-            }
-            if (t == SC_UNDEF) {
+            const bool def = t[s] != SC_UNDEF;
+            nb[s] = def ? A.adj[f] : c;
+            nx[s] = def ? A.nx[f] : 0.0;
+            ny[s] = def ? A.ny[f] : 0.0;
+            fcx[s] = def ? A.fcx[f] : 0.0;
+            fcy[s] = def ? A.fcy[f] : 0.0;
+            area[s] = def ? A.area[f] : 0.0;
+        }
+        bool ok[8];
+#pragma unroll
+        for (int s = 0; s < 8; ++s) {
+            ok[s] = t[s] != SC_UNDEF && nb[s] >= 0 && nb[s] < A.n && !(t[s] >= SC_WALL && nb[s] != c);
+            const bool inner = ok[s] && t[s] == SC_INNER;
+            px[s] = inner ? A.cx[nb[s]] : 0.0;
+            py[s] = inner ? A.cy[nb[s]] : 0.0;
+        }
+        int cw = 0;
+#pragma unroll
+        for (int s = 0; s < 8; ++s) {
+            const int f = SI(A, c, s);
+            if (t[s] == SC_UNDEF) {
                 A.adj[f] = c;
                 continue;
             }
             const int side = s & 3;
-            const double nx = A.nx[f], ny = A.ny[f];
-            const bool gen = !(nx == kNX[side] && ny == kNY[side]);
-            cw |= (t | (gen ? kGeneralNormal : 0)) << (4 * s);
-            if (t >= SC_WALL) ++n_bnd;
+            const bool gen = !(nx[s] == kNX[side] && ny[s] == kNY[side]);
+            cw |= (t[s] | (gen ? kGeneralNormal : 0)) << (4 * s);
+            if (t[s] >= SC_WALL) ++n_bnd;
             else ++n_int;
-            const int nb = A.adj[f];
-            if (nb < 0 || nb >= A.n || (t >= SC_WALL && nb != c)) {   // inconsistent mesh: report, keep in bounds
+            if (!ok[s]) {   // inconsistent mesh: report, keep in bounds
                 A.adj[f] = c;
                 atomicOr(err, 2);
                 continue;
             }
-            double px, py;
-            if (t == SC_INNER) {
-                px = A.cx[nb];
-                py = A.cy[nb];
-            } else {
-                const double d2 = 2.0 * ((A.fcx[f] - cx) * nx + (A.fcy[f] - cy) * ny);
-                px = cx + d2 * nx;
-                py = cy + d2 * ny;
+            double qx = px[s], qy = py[s];
+            if (t[s] != SC_INNER) {
+                const double d2 = 2.0 * ((fcx[s] - cx) * nx[s] + (fcy[s] - cy) * ny[s]);
+                qx = cx + d2 * nx[s];
+                qy = cy + d2 * ny[s];
             }
-            const double drx = px - cx, dry = py - cy;
-            const double d = drx * nx + dry * ny;
-            const double dnx = d * nx, dny = d * ny;
+            const double drx = qx - cx, dry = qy - cy;
+            const double d = drx * nx[s] + dry * ny[s];
+            const double dnx = d * nx[s], dny = d * ny[s];
             const double len = sqrt(dnx * dnx + dny * dny);
-            A.w[f] = (1.0 * A.area[f]) / cube_cr(len);
+            A.w[f] = (1.0 * area[s]) / cube_cr(len);
         }
         A.code[c] = cw;
-        const V2 a = vtx(A, c, 5), b = vtx(A, c, 3), p = vtx(A, c, 7), q = vtx(A, c, 1);
-        const double ax = a.x - b.x, ay = a.y - b.y, bx = p.x - q.x, by = p.y - q.y;
+        const double ax = va.x - vb.x, ay = va.y - vb.y, bx = vp.x - vq.x, by = vp.y - vq.y;
         const double s1 = ax * ax + ay * ay, s2 = bx * bx + by * by;
         const double h = sqrt((s2 < s1) ? s2 : s1);
         A.h[c] = h;
         // CFL candidate of the adapted state (sm_fluid.cpp:104-114)
-        if (z.r) {
-            const Z zc{z.r[c], z.u[c], z.v[c], z.p[c], z.e[c]};
-            cand = h / (sqrt(vsq(zc)) + eos_c(eos, zc.r, zc.p));
-        }
+        if (z.r) cand = h / (sqrt(vsq(zc)) + eos_c(eos, zc.r, zc.p));
     }
     if (!(cand < INFINITY)) cand = INFINITY;   // NaN and +inf never win
     for (int off = 32; off > 0; off >>= 1) {
@@ -752,12 +857,10 @@ hipError_t amr_criterion(const AmrMesh& A, const DevState& z, const DevGrad& G, 
 
 hipError_t amr_balance(const AmrMesh& A, int max_level, hipStream_t s) {
     hipLaunchKernelGGL(k_amr_restrict, grid256(A.n), dim3(256), 0, s, A, max_level);
-    hipLaunchKernelGGL(k_amr_copy_flags, grid256(A.n), dim3(256), 0, s, A);
     for (int lvl = max_level - 1; lvl >= 0; --lvl) {
-        hipLaunchKernelGGL(k_amr_round, grid256(A.n), dim3(256), 0, s, A, lvl, 1);
-        hipLaunchKernelGGL(k_amr_round, grid256(A.n), dim3(256), 0, s, A, lvl, 2);
-        hipLaunchKernelGGL(k_amr_round, grid256(A.n), dim3(256), 0, s, A, lvl, 3);
-        hipLaunchKernelGGL(k_amr_round, grid256(A.n), dim3(256), 0, s, A, lvl, 4);
+        hipLaunchKernelGGL(k_amr_round<0>, grid256(A.n), dim3(256), 0, s, A, lvl);
+        hipLaunchKernelGGL(k_amr_round<1>, grid256(A.n), dim3(256), 0, s, A, lvl);
+        hipLaunchKernelGGL(k_amr_round<2>, grid256(A.n), dim3(256), 0, s, A, lvl);
     }
     return hipGetLastError();
 }

@@ -127,6 +127,28 @@ struct zfv_ctx {
     // update, gradient + predictor, corrector
     int layout[3] = {kLayoutCell, kLayoutCell, kLayoutCell};
     int block[3] = {kBlock, kBlock, kBlock};   // workgroup size per stage
+    bool layout_set[3] = {false, false, false};   // chosen by zfv_set_launch / zfv_autotune (else the default)
+
+    // Structured rectangles (zfv_tile.hip): the Rectangle problems the host
+    // built (set before the upload, empty for an uploaded mesh) and, when
+    // the uploaded slot arrays match per-column / per-row tables cell by
+    // cell, the tables of the tiled MUSCL step.
+    struct RectInfo {
+        int off, nx, ny;
+    };
+    std::vector<RectInfo> rects;
+    struct Tiles {
+        bool ok = false;
+        int n_probs = 0;
+        int n_tiles[kTileVariants] = {0, 0, 0};
+        DBuf<TileProb> prob;
+        DBuf<TileCol> col;
+        DBuf<TileRow> row;
+    } tiles;
+    int tile_variant = 0;
+    // The tiled step does not store its gradients: after one, the gradients
+    // of the last step are recomputed on demand from its input (layer 1 - cur).
+    bool grad_pending = false;
 
     // batch of independent problems (nseg > 1): padded cell offsets, valid
     // cell counts, device tables of the kernels' Tile lookup
@@ -240,6 +262,25 @@ struct zfv_ctx {
         M.seg_end = nullptr;
         return M;
     }
+    TileMesh tmesh() const {
+        TileMesh T;
+        T.n_probs = tiles.n_probs;
+        for (int v = 0; v < kTileVariants; ++v) T.n_tiles[v] = tiles.n_tiles[v];
+        T.prob = tiles.prob.p;
+        T.col = tiles.col.p;
+        T.row = tiles.row.p;
+        T.vol = vol.p;
+        T.h = h.p;
+        return T;
+    }
+    // Whether the steps run the tiled kernel: structured rectangles, MUSCL
+    // with HLLC, the state in the mesh's order.
+    bool tile_step() const {
+        return tiles.ok && prm.acc == 2 && prm.flux == ZFV_HLLC && layout[1] == kLayoutTile && !co.state;
+    }
+    // Launch configuration of the gradient kernel when it runs on its own.
+    int gp_layout() const { return layout[1] == kLayoutTile ? kLayoutCell : layout[1]; }
+    int gp_block() const { return layout[1] == kLayoutTile ? kBlock : block[1]; }
     AmrMesh amesh() const {
         AmrMesh A;
         A.n = hm.n;
@@ -293,6 +334,103 @@ double incircle_2d(const double* X, const double* Y) {
 }
 
 // Allocate device buffers and copy the validated host mesh.
+// Tables of the tiled step for the Rectangle problems of ctx->rects: per
+// column (i) and per row (j) the centres, face centres, LSM weights and
+// areas, taken from the first cell of each column / row; then every cell is
+// checked against them (slot codes, adjacency, centres, face geometry,
+// weights), so the tables give exactly the values of the slot arrays.  Any
+// difference leaves the tiled step off for this mesh.
+bool build_tiles(zfv_ctx* ctx, const std::vector<int>& code, const std::vector<double>& w) {
+    auto& T = ctx->tiles;
+    T.ok = false;
+    T.n_probs = 0;
+    if (ctx->rects.empty()) return false;
+    const HostMesh& m = ctx->hm;
+    const size_t n = (size_t)m.n;
+    std::vector<TileProb> probs;
+    std::vector<TileCol> cols;
+    std::vector<TileRow> rows;
+    int tile0[kTileVariants] = {0, 0, 0};
+    for (const auto& R : ctx->rects) {
+        const int nx = R.nx, ny = R.ny, off = R.off;
+        if (nx < 1 || ny < 1 || (size_t)off + (size_t)nx * ny > n) return false;
+        TileProb P{};
+        P.off = off;
+        P.nx = nx;
+        P.ny = ny;
+        P.tiles_j = (ny + kTileJ - 1) / kTileJ;
+        for (int v = 0; v < kTileVariants; ++v) {
+            P.tile0[v] = tile0[v];
+            tile0[v] += ((nx + kTileRows[v] - 1) / kTileRows[v]) * P.tiles_j;
+        }
+        P.col = (int)cols.size();
+        P.row = (int)rows.size();
+        const int c00 = off, cLast = off + (nx - 1) * ny + (ny - 1);
+        P.t[0] = code[c00] & 7;
+        P.t[1] = (code[cLast] >> 4) & 7;
+        P.t[2] = (code[c00] >> 8) & 7;
+        P.t[3] = (code[cLast] >> 12) & 7;
+        for (int s = 0; s < 4; ++s)
+            if (P.t[s] != SC_PERIODIC && P.t[s] != SC_WALL && P.t[s] != SC_COPY) return false;
+        P.per[0] = P.t[0] == SC_PERIODIC && P.t[1] == SC_PERIODIC;
+        P.per[1] = P.t[2] == SC_PERIODIC && P.t[3] == SC_PERIODIC;
+        auto W = [&](int s, size_t c) { return w[(size_t)s * n + c]; };
+        for (int i = 0; i < nx; ++i) {
+            const size_t c = (size_t)off + (size_t)i * ny, f = 8 * c;
+            cols.push_back(TileCol{m.cx[c], m.fcx[f + 0], m.fcx[f + 1], m.fcx[f + 2], m.fcx[f + 3], W(0, c), W(1, c),
+                                   m.area[f + 2], m.area[f + 3]});
+        }
+        for (int j = 0; j < ny; ++j) {
+            const size_t c = (size_t)off + j, f = 8 * c;
+            rows.push_back(TileRow{m.cy[c], m.fcy[f + 0], m.fcy[f + 1], m.fcy[f + 2], m.fcy[f + 3], W(2, c), W(3, c),
+                                   m.area[f + 0], m.area[f + 1]});
+        }
+        const TileCol* C = &cols[P.col];
+        const TileRow* Rw = &rows[P.row];
+        for (int i = 0; i < nx; ++i)
+            for (int j = 0; j < ny; ++j) {
+                const size_t c = (size_t)off + (size_t)i * ny + j, f = 8 * c;
+                const int ts[4] = {i > 0 ? (int)SC_INNER : P.t[0], i < nx - 1 ? (int)SC_INNER : P.t[1],
+                                   j > 0 ? (int)SC_INNER : P.t[2], j < ny - 1 ? (int)SC_INNER : P.t[3]};
+                const int want_code = ts[0] | (ts[1] << 4) | (ts[2] << 8) | (ts[3] << 12);
+                if (code[c] != want_code) return false;
+                const int self = (int)c;
+                const bool nb_l = ts[0] == SC_INNER || (ts[0] == SC_PERIODIC && P.per[0]);
+                const bool nb_r = ts[1] == SC_INNER || (ts[1] == SC_PERIODIC && P.per[0]);
+                const bool nb_b = ts[2] == SC_INNER || (ts[2] == SC_PERIODIC && P.per[1]);
+                const bool nb_t = ts[3] == SC_INNER || (ts[3] == SC_PERIODIC && P.per[1]);
+                const int adj[4] = {nb_l ? off + ((i - 1 + nx) % nx) * ny + j : self,
+                                    nb_r ? off + ((i + 1) % nx) * ny + j : self,
+                                    nb_b ? off + i * ny + (j - 1 + ny) % ny : self,
+                                    nb_t ? off + i * ny + (j + 1) % ny : self};
+                for (int sl = 0; sl < 4; ++sl)
+                    if (m.adj[f + sl] != adj[sl]) return false;
+                if (m.cx[c] != C[i].cx || m.cy[c] != Rw[j].cy) return false;
+                const double fx[4] = {C[i].fxL, C[i].fxR, C[i].fxB, C[i].fxT};
+                const double fy[4] = {Rw[j].fyL, Rw[j].fyR, Rw[j].fyB, Rw[j].fyT};
+                const double ar[4] = {Rw[j].aL, Rw[j].aR, C[i].aB, C[i].aT};
+                const double wt[4] = {C[i].wL, C[i].wR, Rw[j].wB, Rw[j].wT};
+                for (int sl = 0; sl < 4; ++sl) {
+                    if (m.fcx[f + sl] != fx[sl] || m.fcy[f + sl] != fy[sl] || m.area[f + sl] != ar[sl]) return false;
+                    if (W(sl, c) != wt[sl]) return false;
+                }
+            }
+        probs.push_back(P);
+    }
+    hipStream_t s = ctx->stream;
+    if (T.prob.alloc(probs.size()) != hipSuccess || T.col.alloc(cols.size()) != hipSuccess ||
+        T.row.alloc(rows.size()) != hipSuccess)
+        return false;
+    if (hipMemcpyAsync(T.prob.p, probs.data(), probs.size() * sizeof(TileProb), hipMemcpyHostToDevice, s) != hipSuccess ||
+        hipMemcpyAsync(T.col.p, cols.data(), cols.size() * sizeof(TileCol), hipMemcpyHostToDevice, s) != hipSuccess ||
+        hipMemcpyAsync(T.row.p, rows.data(), rows.size() * sizeof(TileRow), hipMemcpyHostToDevice, s) != hipSuccess)
+        return false;
+    T.n_probs = (int)probs.size();
+    for (int v = 0; v < kTileVariants; ++v) T.n_tiles[v] = tile0[v];
+    T.ok = true;
+    return true;
+}
+
 int upload_mesh(zfv_ctx* ctx) {
     ctx->mesh_epoch += 1;
     const HostMesh& m = ctx->hm;
@@ -408,8 +546,14 @@ int upload_mesh(zfv_ctx* ctx) {
     ZFV_HIP(ctx->err.alloc(1));
     ZFV_HIP(hipMemsetAsync(ctx->err.p, 0, sizeof(int), s));
     ZFV_HIP(ctx->partial.alloc(4 * (size_t)((n + kBlock - 1) / kBlock)));
+    build_tiles(ctx, code, w);
     ZFV_HIP(hipStreamSynchronize(s));
     ctx->graphs_clear();
+    ctx->grad_pending = false;
+    if (!ctx->layout_set[1]) {   // default: the tiled step where it applies
+        ctx->layout[1] = ctx->tiles.ok ? kLayoutTile : kLayoutCell;
+        ctx->block[1] = ctx->tiles.ok ? (kTileRows[0] + 2) * 64 : kBlock;
+    }
     ctx->have_mesh = true;
     ctx->host_stale = false;
     ctx->have_state = false;
@@ -454,55 +598,93 @@ int read_err(zfv_ctx* ctx) {
     return ZFV_OK;
 }
 
-// Re-allocate a [rows][cap] device array with a larger capacity, keeping
-// the first n_valid entries of every row.
-template <class T>
-hipError_t grow_rows(DBuf<T>& b, int rows, int old_cap, int new_cap, int n_valid, hipStream_t s) {
-    T* np = nullptr;
-    hipError_t e = hipMalloc(&np, (size_t)rows * new_cap * sizeof(T));
-    if (e != hipSuccess) return e;
-    if (b.p && n_valid > 0) {
-        e = hipMemcpy2DAsync(np, (size_t)new_cap * sizeof(T), b.p, (size_t)old_cap * sizeof(T),
-                             (size_t)n_valid * sizeof(T), rows, hipMemcpyDeviceToDevice, s);
+// Growth of the [rows][cap] device arrays to a larger capacity, keeping the
+// first n_valid entries of every row.  Transactional: every new array is
+// allocated before any is copied or swapped in, so a failed allocation
+// leaves the context as it was (old capacity, old arrays); the copies are
+// enqueued together and waited for once.
+struct GrowSet {
+    struct Item {
+        void** p;
+        size_t* n;
+        size_t elem;
+        int rows;
+        void* np;
+    };
+    std::vector<Item> items;
+    template <class T>
+    void add(DBuf<T>& b, int rows) {
+        items.push_back(Item{reinterpret_cast<void**>(&b.p), &b.n, sizeof(T), rows, nullptr});
+    }
+    void drop_new() {
+        for (auto& it : items)
+            if (it.np) (void)hipFree(it.np);
+        for (auto& it : items) it.np = nullptr;
+    }
+    hipError_t apply(int old_cap, int new_cap, int n_valid, hipStream_t s) {
+        for (auto& it : items) {
+            hipError_t e = hipMalloc(&it.np, (size_t)it.rows * new_cap * it.elem);
+            if (e != hipSuccess) {
+                it.np = nullptr;
+                drop_new();
+                return e;
+            }
+        }
+        if (n_valid > 0)
+            for (auto& it : items) {
+                if (!*it.p) continue;
+                hipError_t e = hipMemcpy2DAsync(it.np, (size_t)new_cap * it.elem, *it.p, (size_t)old_cap * it.elem,
+                                                (size_t)n_valid * it.elem, it.rows, hipMemcpyDeviceToDevice, s);
+                if (e != hipSuccess) {
+                    (void)hipStreamSynchronize(s);
+                    drop_new();
+                    return e;
+                }
+            }
+        hipError_t e = hipStreamSynchronize(s);
         if (e != hipSuccess) {
-            (void)hipFree(np);
+            drop_new();
             return e;
         }
-        e = hipStreamSynchronize(s);
-        if (e != hipSuccess) return e;
+        for (auto& it : items) {
+            if (*it.p) (void)hipFree(*it.p);
+            *it.p = it.np;
+            *it.n = (size_t)it.rows * new_cap;
+            it.np = nullptr;
+        }
+        return hipSuccess;
     }
-    b.release();
-    b.p = np;
-    b.n = (size_t)rows * new_cap;
-    return hipSuccess;
-}
+};
 
-// Grow every capacity-sized device array to hold new_cap cells.
+// Grow every capacity-sized device array to hold new_cap cells.  On failure
+// the context keeps its previous capacity and arrays.
 int grow_capacity(zfv_ctx* ctx, int new_cap) {
     if (new_cap <= ctx->cap) return ZFV_OK;
-    ctx->graphs_clear();   // buffers move
     const int oc = ctx->cap, n = ctx->hm.n;
-    hipStream_t s = ctx->stream;
-    for (auto* b : {&ctx->cx, &ctx->cy, &ctx->vol, &ctx->h}) ZFV_HIP(grow_rows(*b, 1, oc, new_cap, n, s));
+    GrowSet g;
+    for (auto* b : {&ctx->cx, &ctx->cy, &ctx->vol, &ctx->h}) g.add(*b, 1);
     for (auto* b : {&ctx->code, &ctx->level, &ctx->flag, &ctx->flag0, &ctx->b_idx, &ctx->z_idx, &ctx->next,
                     &ctx->idx, &ctx->wait[0], &ctx->wait[1]})
-        ZFV_HIP(grow_rows(*b, 1, oc, new_cap, n, s));
-    for (auto* b : {&ctx->area, &ctx->fcx, &ctx->fcy, &ctx->w, &ctx->nx, &ctx->ny})
-        ZFV_HIP(grow_rows(*b, 8, oc, new_cap, n, s));
-    for (auto* b : {&ctx->adj, &ctx->bnd}) ZFV_HIP(grow_rows(*b, 8, oc, new_cap, n, s));
-    for (auto* b : {&ctx->vx, &ctx->vy}) ZFV_HIP(grow_rows(*b, 9, oc, new_cap, n, s));
-    for (auto* b : {&ctx->gdx, &ctx->gdy}) ZFV_HIP(grow_rows(*b, 4, oc, new_cap, n, s));
+        g.add(*b, 1);
+    for (auto* b : {&ctx->area, &ctx->fcx, &ctx->fcy, &ctx->w, &ctx->nx, &ctx->ny}) g.add(*b, 8);
+    for (auto* b : {&ctx->adj, &ctx->bnd}) g.add(*b, 8);
+    for (auto* b : {&ctx->vx, &ctx->vy}) g.add(*b, 9);
+    for (auto* b : {&ctx->gdx, &ctx->gdy}) g.add(*b, 4);
     for (int l = 0; l < 3; ++l)
-        for (int k = 0; k < 5; ++k) ZFV_HIP(grow_rows(ctx->st[l][k], 1, oc, new_cap, n, s));
+        for (int k = 0; k < 5; ++k) g.add(ctx->st[l][k], 1);
     auto& M = ctx->mm;   // the multi-material rows keep their [rows][cap] layout
     if (M.configured) {
         const int nm = M.mix.n;
         for (auto* b : {&M.st[0], &M.st[1], &M.half})
-            if (b->p) ZFV_HIP(grow_rows(*b, 7 + 2 * nm, oc, new_cap, n, s));
+            if (b->p) g.add(*b, 7 + 2 * nm);
         for (auto* b : {&M.gx, &M.gy})
-            if (b->p) ZFV_HIP(grow_rows(*b, 6 + 2 * nm, oc, new_cap, n, s));
+            if (b->p) g.add(*b, 6 + 2 * nm);
     }
-    ZFV_HIP(ctx->partial.alloc(4 * (size_t)((new_cap + kBlock - 1) / kBlock)));
+    DBuf<double> partial;
+    ZFV_HIP(partial.alloc(4 * (size_t)((new_cap + kBlock - 1) / kBlock)));
+    ZFV_HIP(g.apply(oc, new_cap, n, ctx->stream));
+    ctx->graphs_clear();   // buffers moved
+    ctx->partial = std::move(partial);
     ctx->cap = new_cap;
     return ZFV_OK;
 }
@@ -840,6 +1022,7 @@ int zfv_mesh_upload(zfv_ctx* ctx, const zfv_mesh_soa* M) {
     m.fcx.assign(M->fcx, M->fcx + ns);
     m.fcy.assign(M->fcy, M->fcy + ns);
     m.area.assign(M->area, M->area + ns);
+    ctx->rects.clear();   // an uploaded mesh: no Rectangle record, no tiled step
     return upload_mesh(ctx);
 }
 
@@ -931,6 +1114,7 @@ int zfv_mesh_rectangle(zfv_ctx* ctx, int nx, int ny, double xmin, double xmax,
     ctx->nseg = 1;
     resize_host_mesh(ctx->hm, nx * ny);
     build_rectangle(ctx->hm, 0, nx, ny, xmin, xmax, ymin, ymax, bounds);
+    ctx->rects.assign(1, zfv_ctx::RectInfo{0, nx, ny});
     return upload_mesh(ctx);
 }
 
@@ -962,6 +1146,8 @@ int zfv_batch_rectangles(zfv_ctx* ctx, int n_problems, const int32_t* nx, const 
     ctx->nseg = n_problems;
     ctx->seg_begin_h = begin;
     ctx->seg_n_h = count;
+    ctx->rects.clear();
+    for (int p = 0; p < n_problems; ++p) ctx->rects.push_back(zfv_ctx::RectInfo{begin[p], nx[p], ny[p]});
     r = upload_mesh(ctx);
     if (r) return r;
     const int nchunk = (int)(total >> kChunkShift);
@@ -1024,6 +1210,47 @@ int zfv_mesh_n_faces(zfv_ctx* ctx, int64_t* n) {
     return ZFV_OK;
 }
 
+int zfv_mesh_reserve(zfv_ctx* ctx, int n_cells) {
+    int r = need_ready(ctx, false);
+    if (r) return r;
+    if (n_cells < 0 || n_cells > std::numeric_limits<int>::max() / 9) return fail(ZFV_ERR_ARG, "n_cells out of range");
+    if (ctx->nseg > 1) return fail(ZFV_ERR_UNSUPPORTED, "reserve on a batch of problems (no adaptation)");
+    if (n_cells <= ctx->cap && ctx->s_marks.n >= (size_t)n_cells) return ZFV_OK;
+    // the curve-ordered copy is rebuilt at the new stride: state and
+    // gradients back in the mesh's order first
+    r = to_mesh_order(ctx);
+    if (r) return r;
+    r = grad_to_mesh_order(ctx);
+    if (r) return r;
+    r = grow_capacity(ctx, n_cells);
+    if (r) return r;
+    const size_t cap = (size_t)ctx->cap, nr = (size_t)std::max(n_cells, 1);
+    // adaptation scratch (refine_impl) and the criterion's unlimited gradient
+    for (auto* b : {&ctx->s_marks, &ctx->s_rank})
+        if (b->n < nr) ZFV_HIP(b->alloc(nr));
+    for (auto* b : {&ctx->s_hole_at, &ctx->s_ref_list, &ctx->s_crs_list, &ctx->s_swap_src, &ctx->s_swap_dst, &ctx->s_fam})
+        if (b->n < nr) ZFV_HIP(b->alloc(nr));
+    if (ctx->s_faces.n < 2) ZFV_HIP(ctx->s_faces.alloc(2));
+    const size_t tiles = (size_t)std::max(amr_scan3_tiles(n_cells), 1);
+    if (ctx->s_tiles.n < tiles) ZFV_HIP(ctx->s_tiles.alloc(tiles));
+    if (!ctx->marks_total) ZFV_HIP(hipHostMalloc((void**)&ctx->marks_total, sizeof(Marks3), hipHostMallocDefault));
+    if (ctx->graw.n < 2 * cap) ZFV_HIP(ctx->graw.alloc(2 * cap));
+    // the curve-ordered copy at the new stride (built at the next step)
+    auto& co = ctx->co;
+    if (co.mode != 0 && co.keys_ok && co.cap != (int)cap) {
+        for (auto* b : {&co.cx, &co.cy, &co.vol, &co.h}) ZFV_HIP(b->alloc(cap));
+        for (auto* b : {&co.area, &co.fcx, &co.fcy, &co.w, &co.nx, &co.ny}) ZFV_HIP(b->alloc(8 * cap));
+        ZFV_HIP(co.code.alloc(cap));
+        ZFV_HIP(co.adj.alloc(8 * cap));
+        for (auto* b : {&co.perm, &co.iperm, &co.vals}) ZFV_HIP(b->alloc(cap));
+        for (auto& k : co.keys) ZFV_HIP(k.alloc(cap));
+        co.cap = (int)cap;
+        co.epoch = -1;
+    }
+    if (co.mode != 0 && co.keys_ok && co.gtmp.n < 4 * cap) ZFV_HIP(co.gtmp.alloc(4 * cap));
+    return ZFV_OK;
+}
+
 int zfv_mesh_download(zfv_ctx* ctx, const zfv_mesh_soa* o) {
     if (!ctx || !o) return fail(ZFV_ERR_ARG, "null argument");
     if (!ctx->have_mesh) return fail(ZFV_ERR_STATE, "no mesh uploaded");
@@ -1049,7 +1276,7 @@ int zfv_smf_configure(zfv_ctx* ctx, const zfv_smf_params* p) {
     int r = check_ctx(ctx);
     if (r) return r;
     if (!p) return fail(ZFV_ERR_ARG, "null params");
-    if (p->flux < ZFV_GODUNOV || p->flux > ZFV_HLLC)
+    if (p->flux < ZFV_GODUNOV || p->flux > ZFV_CIR2)
         return fail(ZFV_ERR_ARG, "unsupported numerical flux id " + std::to_string(p->flux));
     if (p->limiter < ZFV_LIM_NONE || p->limiter > ZFV_LIM_SUPERBEE)
         return fail(ZFV_ERR_ARG, "unknown limiter id " + std::to_string(p->limiter));
@@ -1088,10 +1315,16 @@ int zfv_smf_set_max_dt(zfv_ctx* ctx, double max_dt) {
     return put_max_dt(ctx);
 }
 
+static int materialize_grads(zfv_ctx* ctx);
+
 static int state_copy(zfv_ctx* ctx, const double* const* src, bool to_device, hipMemcpyKind kind) {
     const int n = ctx->hm.n;
     for (int k = 0; k < 5; ++k)
         if (!src[k]) return fail(ZFV_ERR_ARG, "null state array");
+    if (to_device) {   // the last step's input may be overwritten: its gradients first
+        int r = materialize_grads(ctx);
+        if (r) return r;
+    }
     for (int k = 0; k < 5; ++k) {
         double* d = ctx->st[ctx->cur][k].p;
         if (to_device) ZFV_HIP(hipMemcpyAsync(d, src[k], n * sizeof(double), kind, ctx->stream));
@@ -1183,6 +1416,8 @@ int zfv_wait_download(zfv_ctx* ctx, int32_t* wait) {
 int zfv_grad_download(zfv_ctx* ctx, double* dx, double* dy) {
     int r = need_ready(ctx);
     if (r) return r;
+    r = materialize_grads(ctx);
+    if (r) return r;
     r = grad_to_mesh_order(ctx);
     if (r) return r;
     if (!dx || !dy) return fail(ZFV_ERR_ARG, "null output");
@@ -1206,13 +1441,23 @@ static int step_once(zfv_ctx* ctx) {
         ZFV_HIP(timed(ctx, "flux1", [&] {
             return launch_flux1(P.flux, ctx->layout[0], ctx->block[0], M, in, out, eos, ctl, ctx->stream);
         }));
+    } else if (ctx->tile_step()) {
+        int* w_in = ctx->wait[ctx->wcur].p;
+        int* w_out = ctx->wait[1 - ctx->wcur].p;
+        const TileMesh T = ctx->tmesh();
+        ZFV_HIP(timed(ctx, "tile", [&] {
+            return launch_tile_step(P.limiter, ctx->tile_variant, T, in, out, w_in, w_out, eos, ctl, ctx->stream);
+        }));
+        ctx->wcur = 1 - ctx->wcur;
+        ctx->grad_pending = true;   // the step's gradients were not stored
+        ctx->co.grad = false;
     } else {
         DevGrad G{ctx->gdx.p, ctx->gdy.p};
         DevState half = ctx->layer(2);
         int* w_in = ctx->wait[ctx->wcur].p;
         int* w_out = ctx->wait[1 - ctx->wcur].p;
         ZFV_HIP(timed(ctx, "grad_pred", [&] {
-            return launch_grad_pred(P.limiter, true, ctx->layout[1], ctx->block[1], M, in, G, half, w_in, eos, ctl,
+            return launch_grad_pred(P.limiter, true, ctx->gp_layout(), ctx->gp_block(), M, in, G, half, w_in, eos, ctl,
                                     ctx->stream);
         }));
         ZFV_HIP(timed(ctx, "corr", [&] {
@@ -1221,9 +1466,25 @@ static int step_once(zfv_ctx* ctx) {
         }));
         ctx->wcur = 1 - ctx->wcur;
         ctx->co.grad = ctx->co.state;
+        ctx->grad_pending = false;
     }
     ctx->cur = nxt;
     ctx->step_no += 1;
+    return ZFV_OK;
+}
+
+// The gradients of the last step, when a tiled step did not store them:
+// the gradient kernel on that step's input, layer 1 - cur (intact until the
+// next step or upload, before which this runs).
+static int materialize_grads(zfv_ctx* ctx) {
+    if (!ctx->grad_pending) return ZFV_OK;
+    const DevGrad G{ctx->gdx.p, ctx->gdy.p};
+    const int prev = 1 - ctx->cur;
+    ZFV_HIP(launch_grad_pred(ctx->prm.limiter, false, ctx->gp_layout(), ctx->gp_block(), ctx->dmesh(),
+                             ctx->layer(prev), G, ctx->layer(2), ctx->wait[ctx->wcur].p, ctx->eos(), make_ctl(ctx),
+                             ctx->stream));
+    ctx->grad_pending = false;
+    ctx->co.grad = false;
     return ZFV_OK;
 }
 
@@ -1238,7 +1499,7 @@ static int recompute_step(zfv_ctx* ctx, const DevState& in, const DevState& half
     StepCtl ctl = make_ctl(ctx);
     int* w_in = ctx->wait[ctx->wcur].p;
     int* w_out = ctx->wait[1 - ctx->wcur].p;
-    ZFV_HIP(launch_grad_pred(P.limiter, true, ctx->layout[1], ctx->block[1], M, in, G, half, w_in, eos, ctl,
+    ZFV_HIP(launch_grad_pred(P.limiter, true, ctx->gp_layout(), ctx->gp_block(), M, in, G, half, w_in, eos, ctl,
                              ctx->stream));
     ZFV_HIP(launch_corr(P.flux, ctx->layout[2], ctx->block[2], M, in, half, G, w_in, w_out, out, eos, ctl,
                         ctx->stream));
@@ -1359,9 +1620,10 @@ int zfv_smf_compute_grad(zfv_ctx* ctx) {
     if (r) return r;
     StepCtl ctl = make_ctl(ctx);
     DevGrad G{ctx->gdx.p, ctx->gdy.p};
-    ZFV_HIP(launch_grad_pred(ctx->prm.limiter, false, ctx->layout[1], ctx->block[1], ctx->dmesh(), ctx->layer(ctx->cur), G,
+    ZFV_HIP(launch_grad_pred(ctx->prm.limiter, false, ctx->gp_layout(), ctx->gp_block(), ctx->dmesh(), ctx->layer(ctx->cur), G,
                              ctx->layer(2),
                              ctx->wait[ctx->wcur].p, ctx->eos(), ctl, ctx->stream));
+    ctx->grad_pending = false;
     ZFV_HIP(hipStreamSynchronize(ctx->stream));
     return ZFV_OK;
 }
@@ -1432,9 +1694,10 @@ int zfv_set_flags(zfv_ctx* ctx, int criterion, double threshold) {
     StepCtl ctl = make_ctl(ctx);
     DevGrad G{ctx->gdx.p, ctx->gdy.p};
     if (ctx->graw.n < 2 * (size_t)ctx->cap) ZFV_HIP(ctx->graw.alloc(2 * (size_t)ctx->cap));
-    ZFV_HIP(launch_grad_pred(ctx->prm.limiter, false, ctx->layout[1], ctx->block[1], ctx->dmesh(), ctx->layer(ctx->cur), G,
+    ZFV_HIP(launch_grad_pred(ctx->prm.limiter, false, ctx->gp_layout(), ctx->gp_block(), ctx->dmesh(), ctx->layer(ctx->cur), G,
                              ctx->layer(2),
                              ctx->wait[ctx->wcur].p, ctx->eos(), ctl, ctx->stream, ctx->graw.p));
+    ctx->grad_pending = false;
     const DevGrad R{ctx->graw.p, ctx->graw.p + ctx->cap};   // unlimited LSM gradient of rho
     ZFV_HIP(amr_criterion(ctx->amesh(), ctx->layer(ctx->cur), R, criterion, threshold, ctx->stream));
     ctx->host_stale = true;
@@ -1447,6 +1710,7 @@ int zfv_balance_flags(zfv_ctx* ctx, int max_level) {
     if (r) return r;
     if (max_level < 0 || max_level > 15) return fail(ZFV_ERR_ARG, "max_level out of [0, 15]");
     ZFV_HIP(amr_balance(ctx->amesh(), max_level, ctx->stream));
+    std::swap(ctx->flag, ctx->flag0);   // the balanced flags
     ctx->host_stale = true;
     return ZFV_OK;
 }
@@ -1494,11 +1758,14 @@ static int refine_impl(zfv_ctx* ctx, int max_level, int mode) {
     if (!mm) {
         r = to_mesh_order(ctx);
         if (r) return r;
-        r = grad_to_mesh_order(ctx);   // the slope distributor reads them
+        r = materialize_grads(ctx);   // the slope distributor reads them
+        if (r) return r;
+        r = grad_to_mesh_order(ctx);
         if (r) return r;
     }
     const int n = ctx->hm.n;
     ZFV_HIP(amr_balance(ctx->amesh(), max_level, s));
+    std::swap(ctx->flag, ctx->flag0);   // the balanced flags
     ctx->host_stale = true;
     for (auto* b : {&ctx->s_marks, &ctx->s_rank})
         if (b->n < (size_t)n) ZFV_HIP(b->alloc(n));
@@ -1519,7 +1786,7 @@ static int refine_impl(zfv_ctx* ctx, int max_level, int mode) {
     if (n_large <= n) return ZFV_OK;
     if (nz_total != n_refine + 4 * n_parents) return fail(ZFV_ERR_STATE, "unbalanced coarsening flags");
     if (n_large > ctx->cap) {
-        r = grow_capacity(ctx, std::max(n_large, ctx->cap + ctx->cap / 2));
+        r = grow_capacity(ctx, std::max(n_large, (int)std::min<long long>(2LL * ctx->cap, std::numeric_limits<int>::max() / 9)));
         if (r) return r;
     }
     for (auto* b : {&ctx->s_hole_at, &ctx->s_ref_list, &ctx->s_crs_list, &ctx->s_swap_src, &ctx->s_swap_dst})
@@ -1554,6 +1821,7 @@ static int refine_impl(zfv_ctx* ctx, int max_level, int mode) {
     ctx->faces_pending = true;
     ctx->graphs_clear();   // the cell count changed
     ctx->mesh_epoch += 1;
+    ctx->tiles.ok = false;   // no longer a plain Rectangle
     ctx->co.refined = ctx->co.keys_ok;
     ctx->co.lmax = std::max(ctx->co.lmax, max_level);
     if (mm) {
@@ -1691,6 +1959,8 @@ int zfv_adjoint_backward(zfv_ctx* ctx, double* lr, double* lu, double* lv, doubl
     auto& A = ctx->adjoint;
     if (!A.ready) return fail(ZFV_ERR_STATE, "adjoint_backward: no forward run to reverse");
     r = to_mesh_order(ctx);
+    if (r) return r;
+    r = materialize_grads(ctx);   // the reverse sweep reuses the layers
     if (r) return r;
     if (!lr || !lu || !lv || !lp || !le) return fail(ZFV_ERR_ARG, "adjoint_backward: null output");
     const int n = ctx->hm.n, cap = ctx->cap;
@@ -2154,14 +2424,36 @@ int zfv_limiter_from_name(const char* orig) {
 
 static bool valid_block(int b) { return b == 128 || b == 256 || b == 512; }
 
+static int tile_variant_of(int threads) {
+    for (int v = 0; v < kTileVariants; ++v)
+        if (threads == (kTileRows[v] + 2) * 64) return v;
+    return -1;
+}
+
 int zfv_set_launch(zfv_ctx* ctx, int stage, int layout, int block) {
     int r = check_ctx(ctx);
     if (r) return r;
     if (stage < 0 || stage > 2) return fail(ZFV_ERR_ARG, "stage must be 0 (first order), 1 (gradient), 2 (corrector)");
-    if (layout < 0 || layout >= kLayouts) return fail(ZFV_ERR_ARG, "layout must be 0 (cell), 1 (quad) or 2 (batched)");
+    if (layout == kLayoutTile) {
+        if (stage != 1) return fail(ZFV_ERR_ARG, "the tiled layout is a stage-1 layout (it runs the whole step)");
+        if (!ctx->have_mesh || !ctx->tiles.ok)
+            return fail(ZFV_ERR_UNSUPPORTED, "the tiled layout needs a Rectangle mesh (or a batch of them)");
+        const int v = block == 0 ? ctx->tile_variant : tile_variant_of(block);
+        if (v < 0) return fail(ZFV_ERR_ARG, "tiled layout: block must be 0 (keep), 512, 768 or 1024");
+        ctx->layout[1] = kLayoutTile;
+        ctx->block[1] = (kTileRows[v] + 2) * 64;
+        ctx->tile_variant = v;
+        ctx->layout_set[1] = true;
+        ctx->graphs_clear();
+        return ZFV_OK;
+    }
+    if (layout < 0 || layout >= kLayouts)
+        return fail(ZFV_ERR_ARG, "layout must be 0 (cell), 1 (quad), 2 (batched) or 3 (tiled, stage 1)");
     if (block != 0 && !valid_block(block)) return fail(ZFV_ERR_ARG, "block must be 0 (keep), 128, 256 or 512");
+    if (stage == 1 && ctx->layout[1] == kLayoutTile && block == 0) block = kBlock;
     ctx->layout[stage] = layout;
     if (block) ctx->block[stage] = block;
+    ctx->layout_set[stage] = true;
     ctx->graphs_clear();
     return ZFV_OK;
 }
@@ -2211,7 +2503,7 @@ int zfv_autotune(zfv_ctx* ctx, int reps, double* ms_out) {
         return ZFV_OK;
     };
     const int blocks[3] = {128, 256, 512};
-    double t[3][kLayouts][3] = {};   // [stage][layout][block]
+    double t[3][kLayouts + 1][3] = {};   // [stage][layout][block]; [1][kLayoutTile][v]: the tiled step, shape v
     const int stages[2] = {P.acc == 1 ? 0 : 1, P.acc == 1 ? -1 : 2};
     for (int st : stages) {
         if (st < 0) continue;
@@ -2242,13 +2534,33 @@ int zfv_autotune(zfv_ctx* ctx, int reps, double* ms_out) {
                 }
         ctx->layout[st] = bl;
         ctx->block[st] = blocks[bb];
+        ctx->layout_set[st] = true;
+    }
+    // the tiled step (one kernel for the whole step) against the best pair
+    // of gradient and corrector kernels
+    if (P.acc == 2 && P.flux == ZFV_HLLC && ctx->tiles.ok && !ctx->co.state) {
+        const TileMesh T = ctx->tmesh();
+        int bv = 0;
+        for (int v = 0; v < kTileVariants; ++v) {
+            r = time_it([&] { return launch_tile_step(P.limiter, v, T, in, out, w_in, w_out, eos, ctl, s); },
+                        t[1][kLayoutTile][v]);
+            if (r) return r;
+            if (t[1][kLayoutTile][v] < t[1][kLayoutTile][bv]) bv = v;
+        }
+        const double split = t[1][ctx->layout[1]][ctx->block[1] == 128 ? 0 : (ctx->block[1] == 256 ? 1 : 2)] +
+                             t[2][ctx->layout[2]][ctx->block[2] == 128 ? 0 : (ctx->block[2] == 256 ? 1 : 2)];
+        if (t[1][kLayoutTile][bv] < split) {
+            ctx->layout[1] = kLayoutTile;
+            ctx->block[1] = (kTileRows[bv] + 2) * 64;
+            ctx->tile_variant = bv;
+        }
     }
     ctx->event_pool.push_back(e0);
     ctx->event_pool.push_back(e1);
     if (ms_out)
         for (int st = 0; st < 3; ++st)
-            for (int L = 0; L < kLayouts; ++L)
-                for (int bi = 0; bi < 3; ++bi) ms_out[(st * kLayouts + L) * 3 + bi] = t[st][L][bi];
+            for (int L = 0; L <= kLayouts; ++L)
+                for (int bi = 0; bi < 3; ++bi) ms_out[(st * (kLayouts + 1) + L) * 3 + bi] = t[st][L][bi];
     ctx->co.grad = ctx->co.state;
     ctx->graphs_clear();
     return read_err(ctx);

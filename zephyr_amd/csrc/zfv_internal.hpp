@@ -74,7 +74,7 @@ struct StepCtl {
 // arrays plus the AMR bookkeeping of the reference's AmrCells
 // (mesh/euler/amr_cells.h:74-104): per slot the Boundary id, per cell
 // level / flag / b_idx / z_idx / next / idx (idx < 0: undefined cell) and the
-// 9 SqQuad vertices ([9][stride]).  flag0 is the restricted-flag snapshot of
+// 9 SqQuad vertices ([9][stride]).  flag0 receives the restricted flags of
 // the balancing's sort phase.
 struct AmrMesh {
     int n;
@@ -88,6 +88,8 @@ struct AmrMesh {
 
 hipError_t amr_criterion(const AmrMesh& A, const DevState& z, const DevGrad& G, int criterion, double thr,
                          hipStream_t s);
+// balance_flags: the restricted, balanced flags are left in A.flag0 (A.flag
+// keeps the unrestricted ones); the caller swaps the two arrays.
 hipError_t amr_balance(const AmrMesh& A, int max_level, hipStream_t s);
 // Per-cell marks scanned by the adaptation: refine, coarsening-family head,
 // flag != 0.
@@ -159,6 +161,9 @@ hipError_t launch_dt(const DevMesh& M, const DevState& z, const Eos& eos, unsign
 // resident waves, fewer dependent memory round trips per cell).
 constexpr int kLayoutCell = 0, kLayoutQuad = 1, kLayoutBatch = 2;
 constexpr int kLayouts = 3;
+// Stage 1 only: the whole MUSCL step in one tiled kernel (structured
+// rectangles, HLLC); stage 2 is then not launched.
+constexpr int kLayoutTile = 3;
 
 hipError_t launch_flux1(int flux, int layout, int block, const DevMesh& M, const DevState& in, const DevState& out,
                         const Eos& eos, const StepCtl& ctl, hipStream_t s);
@@ -169,6 +174,47 @@ hipError_t launch_corr(int flux, int layout, int block, const DevMesh& M, const 
                        const DevGrad& g, const int* wait_in, int* wait_out, const DevState& next,
                        const Eos& eos, const StepCtl& ctl, hipStream_t s);
 hipError_t launch_totals(const DevMesh& M, const DevState& z, double* partial, int nblocks, hipStream_t s);
+
+// Structured rectangles (zfv_tile.hip): a Rectangle of nx x ny cells (cell
+// off + i ny + j, i along x) whose slot arrays the host has checked against
+// per-column and per-row tables, cell by cell, so that the tables give the
+// same values the slot arrays hold.  The tiled MUSCL step reads the state,
+// the isolation counters, volume and incircle per cell and the geometry from
+// the tables.
+struct TileCol {   // column i: centre x; face-centre x of slots L, R, B, T; weights of L, R; areas of B, T
+    double cx, fxL, fxR, fxB, fxT, wL, wR, aB, aT;
+};
+struct TileRow {   // row j: centre y; face-centre y of slots L, R, B, T; weights of B, T; areas of L, R
+    double cy, fyL, fyR, fyB, fyT, wB, wT, aL, aR;
+};
+// Tile shapes: kTileJ output cells along j (64-lane wavefronts with a halo
+// lane on each side) by kTileRows[v] along i, with (rows + 2) wavefronts
+// per workgroup (one per row of the tile and its two halo rows).
+constexpr int kTileJ = 62;
+constexpr int kTileVariants = 3;
+constexpr int kTileRows[kTileVariants] = {6, 10, 14};
+struct TileProb {
+    int off, nx, ny;     // first cell, columns, rows
+    int tiles_j;         // tiles along j
+    int tile0[kTileVariants];   // first tile (workgroup) of the problem, per tile shape
+    int col, row;        // first column / row table entry
+    int t[4];            // slot code of the domain sides L, R, B, T (periodic, wall or copy)
+    int per[2];          // x / y periodic (both sides periodic): edge neighbours wrap, else the cell itself
+};
+struct TileMesh {
+    int n_probs;
+    int n_tiles[kTileVariants];
+    const TileProb* __restrict__ prob;
+    const TileCol* __restrict__ col;
+    const TileRow* __restrict__ row;
+    const double* __restrict__ vol;
+    const double* __restrict__ h;
+};
+// One MUSCL + HLLC step (gradient, limiting, predictor, corrector and the
+// next step's CFL candidate) per tile, intermediate values in LDS.
+// variant: the tile shape (kTileRows); workgroups of (kTileRows[v] + 2) * 64 threads.
+hipError_t launch_tile_step(int limiter, int variant, const TileMesh& T, const DevState& in, const DevState& out,
+                            const int* wait_in, int* wait_out, const Eos& eos, const StepCtl& ctl, hipStream_t s);
 
 // Discrete adjoint of the MUSCL + HLLC step (zfv_adjoint.hip).
 // One reverse step: z the step's input, znext its output, half / G its half

@@ -91,30 +91,50 @@ __global__ void __launch_bounds__(256) k_curve_mesh(DevMesh src, CurveMesh dst, 
     if (i >= src.n) return;
     const int o = perm[i];
     const int code = src.code[o];
-    dst.cx[i] = src.cx[o];
-    dst.cy[i] = src.cy[o];
-    dst.vol[i] = src.vol[o];
-    dst.h[i] = src.h[o];
-    dst.code[i] = code;
+    // every load is issued before the first store (the source and copy
+    // arrays are of the same types, so stores would otherwise order them)
+    const double cx = src.cx[o], cy = src.cy[o], vol = src.vol[o], h = src.h[o];
+    int adj[8];
+    double area[8], fcx[8], fcy[8], w[8], nx[8], ny[8];
     // undefined slots are never read past their code, canonical normals
     // never past it either: only what the step kernels read is copied
 #pragma unroll
     for (int s = 0; s < 8; ++s) {
-        const int so = s * src.stride + o, si = s * dst.stride + i;
+        const int so = s * src.stride + o;
         const int nib = (code >> (4 * s)) & 15;
-        if ((nib & 7) == SC_UNDEF) {
-            dst.adj[si] = i;
-            continue;
-        }
-        const int a = src.adj[so];
-        dst.adj[si] = (a >= 0 && a < src.n) ? iperm[a] : i;
-        dst.area[si] = src.area[so];
-        dst.fcx[si] = src.fcx[so];
-        dst.fcy[si] = src.fcy[so];
-        dst.w[si] = src.w[so];
+        const bool def = (nib & 7) != SC_UNDEF;
+        adj[s] = def ? src.adj[so] : -1;
+        area[s] = def ? src.area[so] : 0.0;
+        fcx[s] = def ? src.fcx[so] : 0.0;
+        fcy[s] = def ? src.fcy[so] : 0.0;
+        w[s] = def ? src.w[so] : 0.0;
+        const bool gen = def && (nib & kGeneralNormal);
+        nx[s] = gen ? src.nx[so] : 0.0;
+        ny[s] = gen ? src.ny[so] : 0.0;
+    }
+#pragma unroll
+    for (int s = 0; s < 8; ++s) {
+        const int a = adj[s];
+        adj[s] = (a >= 0 && a < src.n) ? iperm[a] : i;
+    }
+    dst.cx[i] = cx;
+    dst.cy[i] = cy;
+    dst.vol[i] = vol;
+    dst.h[i] = h;
+    dst.code[i] = code;
+#pragma unroll
+    for (int s = 0; s < 8; ++s) {
+        const int si = s * dst.stride + i;
+        const int nib = (code >> (4 * s)) & 15;
+        dst.adj[si] = adj[s];
+        if ((nib & 7) == SC_UNDEF) continue;
+        dst.area[si] = area[s];
+        dst.fcx[si] = fcx[s];
+        dst.fcy[si] = fcy[s];
+        dst.w[si] = w[s];
         if (nib & kGeneralNormal) {
-            dst.nx[si] = src.nx[so];
-            dst.ny[si] = src.ny[so];
+            dst.nx[si] = nx[s];
+            dst.ny[si] = ny[s];
         }
     }
 }

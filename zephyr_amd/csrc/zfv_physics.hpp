@@ -421,11 +421,202 @@ __device__ inline Q flux_godunov(const Z& L, const Z& R, const Eos& eos, bool& f
     return phys_flux(z);
 }
 
+// HLLC_M::calc_flux (math/cfd/flux/hllc.cpp:491-572): HLLC with the
+// tangential velocity averaged across the contact (V_C) and the matching
+// energy correction of the star states.  v_z is zero in 2D: the reference's
+// v_y^2 + v_z^2 is v_y^2 and its W_C only feeds the z momentum.
+__device__ __forceinline__ Q flux_hllc_m(const Z& L, const Z& R, const Eos& eos, bool& nan_out) {
+    double cL = eos_c(eos, L.r, L.p);
+    double cR = eos_c(eos, R.r, R.p);
+    double SL = smin(L.u - cL, R.u - cR);
+    double SR = smax(L.u + cL, R.u + cR);
+    if (SL >= 0.0) return phys_flux(L);
+    if (SR <= 0.0) return phys_flux(R);
+    double aL = L.r * (SL - L.u);
+    double aR = R.r * (SR - R.u);
+    double SC = (L.p - R.p + aR * R.u - aL * L.u) / (aR - aL);
+    double VC = (aR * R.v - aL * L.v) / (aR - aL);
+    double rsL = aL / (SL - SC);
+    double rsR = aR / (SR - SC);
+    double hL = L.e + 0.5 * vsq(L);
+    double hR = R.e + 0.5 * vsq(R);
+    double EsL = hL + (SC - L.u) * (SC + L.p / aL);
+    double EsR = hR + (SC - R.u) * (SC + R.p / aR);
+    double vL2 = L.v * L.v, vR2 = R.v * R.v;
+    EsL = EsL + 0.5 * ((aR * vR2 - aL * vL2) / (aR - aL) - vL2);
+    EsR = EsR + 0.5 * ((aR * vR2 - aL * vL2) / (aR - aL) - vR2);
+    double qsL[4] = {rsL, rsL * SC, rsL * VC, rsL * EsL};
+    double qsR[4] = {rsR, rsR * SC, rsR * VC, rsR * EsR};
+    double ruL = L.r * L.u, ruR = R.r * R.u;
+    double EL = L.r * hL, ER = R.r * hR;
+    double qL[4] = {L.r, ruL, L.r * L.v, EL};
+    double qR[4] = {R.r, ruR, R.r * R.v, ER};
+    double fL[4] = {ruL, ruL * L.u + L.p, ruL * L.v, (EL + L.p) * L.u};
+    double fR[4] = {ruR, ruR * R.u + R.p, ruR * R.v, (ER + R.p) * R.u};
+    double aSC = fabs(SC);
+    double F[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        double t = fL[k] + fR[k];
+        t = t + SL * (qsL[k] - qL[k]);
+        t = t + aSC * (qsL[k] - qsR[k]);
+        t = t + SR * (qsR[k] - qR[k]);
+        F[k] = 0.5 * t;
+    }
+    nan_out |= isnan(F[0]) | isnan(F[1]) | isnan(F[2]) | isnan(F[3]);
+    return Q{F[0], F[1], F[2], F[3]};
+}
+
+// HLLC_LM::calc_flux (math/cfd/flux/hllc.cpp:427-488): the outer wave terms
+// of HLLC scaled by phi = sin(pi/2 Ma / 0.1) below Ma = 0.1, Ma the larger
+// |v_x| / c of the two sides.  sin is the device's (ocml), the reference
+// calls glibc's: phi may differ in the last bit.
+__device__ __forceinline__ Q flux_hllc_lm(const Z& L, const Z& R, const Eos& eos, bool& nan_out) {
+    double cL = eos_c(eos, L.r, L.p);
+    double cR = eos_c(eos, R.r, R.p);
+    double SL = smin(L.u - cL, R.u - cR);
+    double SR = smax(L.u + cL, R.u + cR);
+    if (SL >= 0.0) return phys_flux(L);
+    if (SR <= 0.0) return phys_flux(R);
+    double aL = L.r * (SL - L.u);
+    double aR = R.r * (SR - R.u);
+    double SC = (L.p - R.p + aR * R.u - aL * L.u) / (aR - aL);
+    double rsL = aL / (SL - SC);
+    double rsR = aR / (SR - SC);
+    double hL = L.e + 0.5 * vsq(L);
+    double hR = R.e + 0.5 * vsq(R);
+    double EsL = hL + (SC - L.u) * (SC + L.p / aL);
+    double EsR = hR + (SC - R.u) * (SC + R.p / aR);
+    double qsL[4] = {rsL, rsL * SC, rsL * L.v, rsL * EsL};
+    double qsR[4] = {rsR, rsR * SC, rsR * R.v, rsR * EsR};
+    double ruL = L.r * L.u, ruR = R.r * R.u;
+    double EL = L.r * hL, ER = R.r * hR;
+    double qL[4] = {L.r, ruL, L.r * L.v, EL};
+    double qR[4] = {R.r, ruR, R.r * R.v, ER};
+    double fL[4] = {ruL, ruL * L.u + L.p, ruL * L.v, (EL + L.p) * L.u};
+    double fR[4] = {ruR, ruR * R.u + R.p, ruR * R.v, (ER + R.p) * R.u};
+    const double Ma = smax(fabs(L.u / cL), fabs(R.u / cR));
+    const double xi = Ma / 0.1;
+    const double phi = xi > 1.0 ? 1.0 : sin(1.5707963267948966 * xi);
+    double aSC = fabs(SC);
+    double F[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        double t = fL[k] + fR[k];
+        t = t + phi * SL * (qsL[k] - qL[k]);
+        t = t + aSC * (qsL[k] - qsR[k]);
+        t = t + phi * SR * (qsR[k] - qR[k]);
+        F[k] = 0.5 * t;
+    }
+    nan_out |= isnan(F[0]) | isnan(F[1]) | isnan(F[2]) | isnan(F[3]);
+    return Q{F[0], F[1], F[2], F[3]};
+}
+
+// CIR1 / CIR2 (math/cfd/flux/cir.cpp:16-216): F = (F_L + F_R)/2 +
+// |A| (q_L - q_R)/2, |A| = R |Lambda| L at the arithmetic face state (energy
+// from (rho, p)).  CIR1 decomposes the 5 x 5 Jacobian (it needs dP/de =
+// (gamma - 1) rho); CIR2 carries the pressure as a sixth variable (5 x 6 R,
+// 6 x 6 L).  The z velocity (zero in 2D) and the z momentum are carried as
+// in the reference's vectors so that every sum has the same terms in the
+// same order: |A|(i,j) = sum_k (R(i,k) |lambda_k|) L(k,j), then sum_j.
+template <bool EXT>
+__device__ __forceinline__ Q flux_cir(const Z& ZL, const Z& ZR, const Eos& eos) {
+    constexpr int N = EXT ? 6 : 5;
+    const double w = 0.0;   // v_z of both sides and of the face state
+    const double rho = 0.5 * (ZL.r + ZR.r), u = 0.5 * (ZL.u + ZR.u), v = 0.5 * (ZL.v + ZR.v);
+    const double p = 0.5 * (ZL.p + ZR.p);
+    const double eps = eos_e(eos, rho, p);
+    const double u2 = u * u, v2 = v * v, w2 = w * w;
+    const double q2 = u2 + v2 + w2;
+    const double E = rho * (eps + 0.5 * q2);
+    const double c = eos_c(eos, rho, p);
+    const double c2 = c * c;
+    const double h = (E + p) / rho;
+    double lam[N], R[5][N], L[N][N];
+    if constexpr (!EXT) {
+        const double b = ((eos.g - 1.0) * rho) / rho;
+        const double T = b * (q2 - h) + c2;
+        const double l5[5] = {fabs(u - c), fabs(u), fabs(u), fabs(u), fabs(u + c)};
+        const double r5[5][5] = {{1, 0, 0, 1, 1},
+                                 {u - c, 0, 0, u, u + c},
+                                 {v, 1, 0, v, v},
+                                 {w, 0, 1, w, w},
+                                 {h - u * c, v, w, h - c2 / b, h + u * c}};
+        const double s2 = 2.0 * c2;
+        const double L5[5][5] = {{T + u * c, -b * u - c, -b * v, -b * w, b},
+                                 {-2 * v * c2, 0, 2 * c2, 0, 0},
+                                 {-2 * w * c2, 0, 0, 2 * c2, 0},
+                                 {2 * b * (h - q2), 2 * b * u, 2 * b * v, 2 * b * w, -2 * b},
+                                 {T - u * c, -b * u + c, -b * v, -b * w, b}};
+#pragma unroll
+        for (int i = 0; i < 5; ++i) {
+            lam[i] = l5[i];
+#pragma unroll
+            for (int j = 0; j < 5; ++j) {
+                R[i][j] = r5[i][j];
+                L[i][j] = L5[i][j] / s2;
+            }
+        }
+    } else {
+        const double l6[6] = {fabs(u - c), fabs(u), fabs(u), fabs(u), fabs(u + c), fabs(u)};
+        const double r6[5][6] = {{1, 0, 0, 0, 1, 1},
+                                 {u - c, 0, 0, 0, u + c, u},
+                                 {v, 1, 0, 0, v, 0},
+                                 {w, 0, 1, 0, w, 0},
+                                 {h - u * c, 0, 0, 1, h + u * c, 0}};
+        const double L6[6][6] = {{u / (2 * c), -1 / (2 * c), 0, 0, 0, 1 / (2 * c2)},
+                                 {0, 0, 1, 0, 0, -v / c2},
+                                 {0, 0, 0, 1, 0, -w / c2},
+                                 {u2, -u, 0, 0, 1, -h / c2},
+                                 {-u / (2 * c), 1 / (2 * c), 0, 0, 0, 1 / (2 * c2)},
+                                 {1, 0, 0, 0, 0, -1 / c2}};
+#pragma unroll
+        for (int i = 0; i < 6; ++i) {
+            lam[i] = l6[i];
+#pragma unroll
+            for (int j = 0; j < 6; ++j) {
+                if (i < 5) R[i][j] = r6[i][j];
+                L[i][j] = L6[i][j];
+            }
+        }
+    }
+    // conserved vectors (z momentum rho v_z = rho * 0) and physical fluxes
+    const double EL = ZL.r * (ZL.e + 0.5 * vsq(ZL)), ER = ZR.r * (ZR.e + 0.5 * vsq(ZR));
+    const double qL[5] = {ZL.r, ZL.r * ZL.u, ZL.r * ZL.v, ZL.r * w, EL};
+    const double qR[5] = {ZR.r, ZR.r * ZR.u, ZR.r * ZR.v, ZR.r * w, ER};
+    const double ruL = ZL.r * ZL.u, ruR = ZR.r * ZR.u;
+    const double fL[5] = {ruL, ruL * ZL.u + ZL.p, ruL * ZL.v, ruL * w, (EL + ZL.p) * ZL.u};
+    const double fR[5] = {ruR, ruR * ZR.u + ZR.p, ruR * ZR.v, ruR * w, (ER + ZR.p) * ZR.u};
+    double dq[N];
+#pragma unroll
+    for (int k = 0; k < 5; ++k) dq[k] = qL[k] - qR[k];
+    if constexpr (EXT) dq[5] = ZL.p - ZR.p;
+    double F[5];
+#pragma unroll
+    for (int i = 0; i < 5; ++i) {
+        double t = 0.0;
+#pragma unroll
+        for (int j = 0; j < N; ++j) {
+            double a = 0.0;
+#pragma unroll
+            for (int k = 0; k < N; ++k) a = a + (R[i][k] * lam[k]) * L[k][j];
+            t = t + a * dq[j];
+        }
+        F[i] = 0.5 * (fL[i] + fR[i]) + 0.5 * t;
+    }
+    return Q{F[0], F[1], F[2], F[4]};
+}
+
+// Fluxes values (math/cfd/flux/num_flux.h:11-21).
 template <int FLUX>
 __device__ __forceinline__ Q num_flux(const Z& L, const Z& R, const Eos& eos, bool& err) {
     if constexpr (FLUX == 0) return flux_godunov(L, R, eos, err);
     else if constexpr (FLUX == 1) return flux_rusanov(L, R, eos);
     else if constexpr (FLUX == 2) return flux_hll(L, R, eos, err);
+    else if constexpr (FLUX == 4) return flux_hllc_m(L, R, eos, err);
+    else if constexpr (FLUX == 5) return flux_hllc_lm(L, R, eos, err);
+    else if constexpr (FLUX == 6) return flux_cir<false>(L, R, eos);
+    else if constexpr (FLUX == 7) return flux_cir<true>(L, R, eos);
     else return flux_hllc(L, R, eos, err);
 }
 

@@ -32,7 +32,10 @@ inline void check(int code) {
 enum class Boundary : int { UNDEFINED = -1, INNER = 0, WALL = 1, ZOE = 2, PERIODIC = 3, FUNCTION = 4 };
 
 // math::Fluxes, the members the step implements (math/cfd/flux/num_flux.h:11-21).
-enum class Fluxes : int { GODUNOV = ZFV_GODUNOV, RUSANOV = ZFV_RUSANOV, HLL = ZFV_HLL, HLLC = ZFV_HLLC };
+enum class Fluxes : int {
+    GODUNOV = ZFV_GODUNOV, RUSANOV = ZFV_RUSANOV, HLL = ZFV_HLL, HLLC = ZFV_HLLC,
+    HLLC_M = ZFV_HLLC_M, HLLC_LM = ZFV_HLLC_LM, CIR1 = ZFV_CIR1, CIR2 = ZFV_CIR2
+};
 
 // Equations of state: StiffenedGas (phys/matter/eos/stiffened_gas.cpp:103-175);
 // IdealGas is P0 = e0 = 0 (ideal_gas.cpp:32-104).
