@@ -182,6 +182,15 @@ int zfv_refine(zfv_ctx* ctx, int max_level, int distributor);
 
 /* Conserved totals {mass, mom_x, mom_y, energy} = sum V*q (synchronises). */
 int zfv_conserved_totals(zfv_ctx* ctx, double* out4);
+/* zfv_conserved_totals without any device allocation, for callers that
+ * cannot allocate (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  No counterpart in the reference.  The
+ * same sums in the same order (bit-identical results), with the partial
+ * sums in the library's static per-device scratch (4 MiB) instead of a
+ * context buffer: meshes of at most 33,554,432 cells (ZFV_ERR_ARG beyond).
+ * Every call on one device shares that scratch: calls that overlap on one
+ * device race. */
+int zfv_conserved_totals_noalloc(zfv_ctx* ctx, double* out4);
 
 /* Limiter name -> id with the reference's parsing rules; -1 if unknown
  * (the reference throws, limiter.cxx:249-251). */

@@ -27,7 +27,7 @@ EXPORTS = [
     "zfv_mesh_download", "zfv_smf_configure", "zfv_smf_set_max_dt", "zfv_state_upload",
     "zfv_state_download", "zfv_state_upload_device", "zfv_state_download_device",
     "zfv_wait_upload", "zfv_wait_download", "zfv_grad_download", "zfv_smf_step", "zfv_smf_dt",
-    "zfv_smf_compute_grad", "zfv_smf_compute_dt", "zfv_conserved_totals",
+    "zfv_smf_compute_grad", "zfv_smf_compute_dt", "zfv_conserved_totals", "zfv_conserved_totals_noalloc",
     "zfv_limiter_from_name", "zfv_profile_enable", "zfv_profile_read",
     "zfv_flags_upload", "zfv_flags_download", "zfv_set_flags", "zfv_balance_flags", "zfv_refine",
     "zfv_mesh_n_faces", "zfv_mesh_reserve", "zfv_set_launch", "zfv_get_launch", "zfv_autotune",
@@ -295,6 +295,12 @@ class Context:
     def totals(self):
         out = np.zeros(4)
         _check(lib().zfv_conserved_totals(self.h, _ptr(out)))
+        return out
+
+    def totals_noalloc(self):
+        """zfv_conserved_totals_noalloc: the same totals without device allocation."""
+        out = np.zeros(4)
+        _check(lib().zfv_conserved_totals_noalloc(self.h, _ptr(out)))
         return out
 
     def synchronize(self):

@@ -9,6 +9,7 @@
 #include <cstring>
 #include <limits>
 #include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -920,6 +921,27 @@ int grad_to_mesh_order(zfv_ctx* ctx) {
     return ZFV_OK;
 }
 
+// Device address of the static scratch array of each device, taken once per
+// device at zfv_create (the no-allocation entry points use only this).
+std::mutex g_scratch_mu;
+std::vector<double*> g_scratch;
+
+int scratch_setup(int device) {
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if ((int)g_scratch.size() <= device) g_scratch.resize(device + 1, nullptr);
+    if (g_scratch[device]) return ZFV_OK;
+    double* p = nullptr;
+    ZFV_HIP(scratch_address(&p));
+    if (!p) return fail(ZFV_ERR_HIP, "hipGetSymbolAddress(scratch) returned a null address");
+    g_scratch[device] = p;
+    return ZFV_OK;
+}
+
+double* scratch_of(int device) {
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    return device < (int)g_scratch.size() ? g_scratch[device] : nullptr;
+}
+
 }  // namespace
 
 extern "C" {
@@ -934,6 +956,8 @@ int zfv_create(int device, zfv_ctx** out) {
     if (e != hipSuccess || count == 0) return fail(ZFV_ERR_HIP, "no HIP device available");
     if (device < 0 || device >= count) return fail(ZFV_ERR_ARG, "device index out of range");
     ZFV_HIP(hipSetDevice(device));
+    int rs = scratch_setup(device);   // once per device, with the device current
+    if (rs) return rs;
     zfv_ctx* ctx = new zfv_ctx();
     ctx->device = device;
     e = hipStreamCreateWithFlags(&ctx->own_stream, hipStreamNonBlocking);
@@ -1658,6 +1682,28 @@ int zfv_conserved_totals(zfv_ctx* ctx, double* out4) {
     for (int k = 0; k < 4; ++k) out4[k] = 0.0;
     for (int b = 0; b < nb; ++b)
         for (int k = 0; k < 4; ++k) out4[k] += part[4 * (size_t)b + k];
+    return read_err(ctx);
+}
+
+int zfv_conserved_totals_noalloc(zfv_ctx* ctx, double* out4) {
+    int r = need_ready(ctx, true, true);
+    if (r) return r;
+    if (!out4) return fail(ZFV_ERR_ARG, "null output");
+    const int nb = (ctx->hm.n + kBlock - 1) / kBlock;
+    if (nb > kScratchBlocks)
+        return fail(ZFV_ERR_ARG, "mesh of " + std::to_string(ctx->hm.n) + " cells exceeds the " +
+                                     std::to_string((long long)kScratchBlocks * kBlock) +
+                                     " cells of the no-allocation scratch");
+    double* sp = scratch_of(ctx->device);
+    if (!sp) return fail(ZFV_ERR_STATE, "no scratch address for this device");
+    r = to_mesh_order(ctx);   // the partial sums run in the mesh's order
+    if (r) return r;
+    ZFV_HIP(launch_totals(ctx->dmesh(), ctx->layer(ctx->cur), sp + 4, nb, ctx->stream));
+    ZFV_HIP(launch_totals_sum(sp + 4, nb, sp, ctx->stream));
+    double res[4];
+    ZFV_HIP(hipMemcpyAsync(res, sp, sizeof(res), hipMemcpyDeviceToHost, ctx->stream));
+    ZFV_HIP(hipStreamSynchronize(ctx->stream));
+    for (int k = 0; k < 4; ++k) out4[k] = res[k];
     return read_err(ctx);
 }
 

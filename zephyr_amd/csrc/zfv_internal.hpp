@@ -215,6 +215,12 @@ struct TileMesh {
 // variant: the tile shape (kTileRows); workgroups of (kTileRows[v] + 2) * 64 threads.
 hipError_t launch_tile_step(int limiter, int variant, const TileMesh& T, const DevState& in, const DevState& out,
                             const int* wait_in, int* wait_out, const Eos& eos, const StepCtl& ctl, hipStream_t s);
+// Scratch of the no-allocation entry points (zfv_kernels.hip): 4 result
+// doubles, then up to kScratchBlocks workgroups' 4 partial sums (4 MiB).
+constexpr int kScratchBlocks = 131072;
+constexpr int kScratchDoubles = 4 + 4 * kScratchBlocks;
+hipError_t scratch_address(double** out);
+hipError_t launch_totals_sum(const double* partial, int nblocks, double* out, hipStream_t s);
 
 // Discrete adjoint of the MUSCL + HLLC step (zfv_adjoint.hip).
 // One reverse step: z the step's input, znext its output, half / G its half

@@ -808,6 +808,36 @@ __global__ void __launch_bounds__(kBlock) k_totals(DevMesh M, DevState z, double
     if (threadIdx.x < 4) partial[4 * blockIdx.x + threadIdx.x] = red[threadIdx.x][0];
 }
 
+// ---------------------------------------------------------------- scratch
+// Device memory of the entry points that allocate nothing
+// (zfv_conserved_totals_noalloc): [0, 4) the result, then the per-workgroup
+// partial sums.  One copy per device, held until the process ends and shared
+// by every call on that device (calls that overlap on one device race).  In
+// host code `scratch` names a host-side stand-in, not device memory: the
+// host uses only the address scratch_address() returns.
+static __device__ __align__(256) double scratch[kScratchDoubles];
+
+// The four totals, each summed over the workgroups' partials in workgroup
+// order from 0.0: the additions zfv_conserved_totals makes on the host.
+__global__ void __launch_bounds__(64) k_totals_sum(const double* __restrict__ partial, int nblocks,
+                                                   double* __restrict__ out) {
+    const int k = (int)threadIdx.x;
+    if (k >= 4) return;
+    double acc = 0.0;
+    for (int b = 0; b < nblocks; ++b) acc = acc + partial[4 * (size_t)b + k];
+    out[k] = acc;
+}
+
+// Device address of `scratch` on the current device.
+hipError_t scratch_address(double** out) {
+    return hipGetSymbolAddress(reinterpret_cast<void**>(out), HIP_SYMBOL(scratch));
+}
+
+hipError_t launch_totals_sum(const double* partial, int nblocks, double* out, hipStream_t s) {
+    hipLaunchKernelGGL(k_totals_sum, dim3(1), dim3(64), 0, s, partial, nblocks, out);
+    return hipGetLastError();
+}
+
 // ---------------------------------------------------------------- launchers
 // Grids: `block` threads per workgroup, a workgroup covers `block` cells
 // (cell layout) or block / 4 cells (quad layout).
