@@ -115,17 +115,25 @@ WORKLOADS = {
 }
 
 
-def kernel_sources_sha():
-    """Short SHA-256 over the HIP kernel sources and headers (which the HBM
-    counter summaries record, so a summary is used only for the kernels it
-    measured)."""
+# The sources a step kernel is compiled from: its own file and the headers
+# that file includes.
+STEP_HEADERS = ("zfv_internal.hpp", "zfv_physics.hpp", "zfv_mm.hpp", "zfv_step_common.hpp", "zfv_step_faces.hpp")
+KERNEL_FILES = {"tile": "zfv_tile.hip", "grad_pred": "zfv_kernels.hip", "corr": "zfv_kernels.hip",
+                "flux1": "zfv_kernels.hip"}
+
+
+def kernel_sources_sha(kernel):
+    """Short SHA-256 over the HIP sources a step kernel is compiled from
+    (which the HBM counter summaries record, so a summary is used only for
+    the kernel code it measured); None for a kernel outside the table."""
     import hashlib
+    if kernel not in KERNEL_FILES:
+        return None
     d = os.path.join(HERE, "zephyr_amd", "csrc")
     h = hashlib.sha256()
-    for f in sorted(os.listdir(d)):
-        if f.endswith((".hip", ".hpp")):
-            h.update(f.encode())
-            h.update(open(os.path.join(d, f), "rb").read())
+    for f in (KERNEL_FILES[kernel],) + STEP_HEADERS:
+        h.update(f.encode())
+        h.update(open(os.path.join(d, f), "rb").read())
     return h.hexdigest()[:16]
 
 
@@ -471,6 +479,7 @@ def mm_run(args, w, Z):
     ctx.mm_configure(mats, w["cfl"])
     acc = w.get("acc", 1)
     ctx.mm_set_accuracy(acc)
+    ctx.mm_set_overlap(args.mm_overlap)
     amr = "max_level" in w
     st = None
     if amr:
@@ -500,7 +509,9 @@ def mm_run(args, w, Z):
               "mm_crp_mark": mm_crp_mark_bytes_per_cell(2) * n, "mm_interface": mm_interface_bytes_per_cell(2) * n,
               "mm_grad": mm_grad_bytes_per_cell(2) * n, "mm_stage1": mm_stage1_bytes_per_cell(2) * n,
               "mm_flux2": mm_flux2_bytes_per_cell(2) * n}
-    dom = max(prof, key=lambda k: prof[k][0])
+    # the deferred cells' update (overlapped steps) has no fixed cell count:
+    # it is timed but not priced
+    dom = max((k for k in prof if k in alg_of), key=lambda k: prof[k][0])
     avg_ms = prof[dom][0] / prof[dom][1]
     alg = alg_of[dom]
     achieved = alg / (avg_ms * 1e-3) / 1e9
@@ -516,7 +527,8 @@ def mm_run(args, w, Z):
                    "mach": problems.mm_bubble_shock_params(args.seed), "mixed_cells_final": mixed,
                    "crp_faces_last_step": n_crp,
                    "kernel_hbm_frac": {k: alg_of[k] / (prof[k][0] / prof[k][1] * 1e-3) / 1e9 / HBM_PEAK_GBS
-                                       for k in prof},
+                                       for k in prof if k in alg_of},
+                   "stream_mode": args.mm_overlap,
                    "dt_final": ctx.mm_dt(),
                    "kernel_ms_avg": {k: v[0] / v[1] for k, v in prof.items()},
                    "kernel_timing": f"HIP events, {args.prof_steps} steps following the timed region"},
@@ -730,6 +742,9 @@ def main():
     ap.add_argument("--fd-dirs", type=int, default=0, help="adjoint workload: finite-difference directions")
     ap.add_argument("--prof-steps", type=int, default=100,
                     help="steps of the event-timed kernel pass after the timed region")
+    ap.add_argument("--mm-overlap", type=int, default=0, choices=(0, 1, 2),
+                    help="multi-material workloads: 0 one stream, 1 interface on a second stream, 2 also the "
+                         "composite fluxes beside the other cells' update")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run device-to-device copies of the state (known bytes) for HBM counter scaling")
     args = ap.parse_args()
@@ -836,9 +851,13 @@ def main():
     # sources this run executes (same source hash); otherwise null.
     traffic, traffic_detail = None, None
     tpath = os.path.join(HERE, "profiles", "traffic.json")
-    src_sha = kernel_sources_sha()
+    src_sha = kernel_sources_sha(dom)
+    # the entry of the launch form in use: kernel, layout, workgroup size
+    # (or tile shape), whose register use and access pattern set the bytes
+    lay, blk = ctx.launch(2 if dom == "corr" else 1)
+    tkey = f"{dom}:{lay}:{blk}"
     if os.path.exists(tpath):
-        t = json.load(open(tpath)).get(args.workload, {}).get(dom)
+        t = json.load(open(tpath)).get(args.workload, {}).get(tkey)
         if t:
             traffic_detail = dict(t)
             if t.get("kernel_sources_sha") == src_sha:
@@ -848,6 +867,8 @@ def main():
             else:
                 traffic_detail["measured_in"] = (f"stale: counter runs of kernel sources {t.get('kernel_sources_sha')}, "
                                                  f"this run executes {src_sha}; traffic not reported")
+        else:
+            traffic_detail = {"measured_in": f"no counter run of {tkey} (kernel:layout:workgroup) for this workload"}
 
     scheme = "MUSCL+HLLC" if acc == 2 else "first-order HLLC"
     out = {

@@ -84,7 +84,9 @@ typedef struct {
     int32_t acc;        /* 1 or 2 (clamped as set_accuracy)                 */
     int32_t flux;       /* ZFV_GODUNOV..ZFV_CIR2                             */
     int32_t limiter;    /* ZFV_LIM_*                                         */
-    int32_t reserved;
+    int32_t axial;      /* set_axial (sm_fluid.cpp:45-47): 1 takes face.area(true),
+                           cell.volume(true) and the pressure source of
+                           sm_fluid.cpp:166-169; needs zfv_mesh_rectangle_axial */
     double cfl;         /* clamped to [0,1] as set_CFL                      */
     double max_dt;      /* set_max_dt                                        */
     double gamma, P0, e0;
@@ -113,6 +115,22 @@ int zfv_mesh_upload(zfv_ctx* ctx, const zfv_mesh_soa* mesh);
  * bounds = {left, right, bottom, top} Boundary ids. */
 int zfv_mesh_rectangle(zfv_ctx* ctx, int nx, int ny, double xmin, double xmax,
                        double ymin, double ymax, const int32_t* bounds);
+/* The same Rectangle with set_axial(true) (rectangle.cpp:97-103, 590-602):
+ * y is the radius (ymin >= 0; a bottom on the axis, ymin = 0, becomes a
+ * wall), the cells carry volume_alt = hx hy y_c and the faces area_alt =
+ * length * y_face (the y-weighted volume and areas of the body of
+ * revolution per radian), and the cell and left / right face centres move
+ * to their y-weighted centroids (y + hy^2 / (12 y)).  Such a mesh is not
+ * adapted (zfv_refine returns ZFV_ERR_UNSUPPORTED). */
+int zfv_mesh_rectangle_axial(zfv_ctx* ctx, int nx, int ny, double xmin, double xmax,
+                             double ymin, double ymax, const int32_t* bounds);
+/* zfv_mesh_upload of a mesh with the axisymmetric geometry (an axial
+ * EuMesh, amr_cells.h:288-290, amr_faces.h:183-185): volume_alt (n, > 0)
+ * and area_alt (8n); the centres in `mesh` are the y-weighted ones. */
+int zfv_mesh_upload_axial(zfv_ctx* ctx, const zfv_mesh_soa* mesh, const double* vol_alt, const double* area_alt);
+/* volume_alt (n) and area_alt (8n, slot s of cell c at 8c + s) of an
+ * axisymmetric mesh (AmrCells::volume_alt, AmrFaces::area_alt). */
+int zfv_mesh_axial_download(zfv_ctx* ctx, double* vol_alt, double* area_alt);
 /* Number of cells of the current mesh (EuMesh::n_cells). */
 int zfv_mesh_n_cells(zfv_ctx* ctx, int32_t* n);
 /* Number of unique faces of the current mesh (interior face sides / 2 +
@@ -132,6 +150,8 @@ int zfv_mesh_download(zfv_ctx* ctx, const zfv_mesh_soa* out);
 
 /* Configure the solver (SmFluid(Eos) + setters + add_types, sm_fluid.cpp:16-67). */
 int zfv_smf_configure(zfv_ctx* ctx, const zfv_smf_params* params);
+/* The solver's parameters as set (SmFluid's getters: CFL(), ...). */
+int zfv_smf_get_params(zfv_ctx* ctx, zfv_smf_params* out);
 /* set_max_dt (sm_fluid.cpp:65-67). */
 int zfv_smf_set_max_dt(zfv_ctx* ctx, double max_dt);
 
@@ -180,7 +200,8 @@ int zfv_balance_flags(zfv_ctx* ctx, int max_level);
  * (uses the gradient of the last zfv_set_flags / zfv_smf_compute_grad). */
 int zfv_refine(zfv_ctx* ctx, int max_level, int distributor);
 
-/* Conserved totals {mass, mom_x, mom_y, energy} = sum V*q (synchronises). */
+/* Conserved totals {mass, mom_x, mom_y, energy} = sum V*q (synchronises);
+ * V = volume_alt with set_axial, the volume the update divides by. */
 int zfv_conserved_totals(zfv_ctx* ctx, double* out4);
 /* zfv_conserved_totals without any device allocation, for callers that
  * cannot allocate (code under HIP graph capture, tight serving loops,
@@ -264,6 +285,19 @@ int zfv_mm_set_crp_mode(zfv_ctx* ctx, int mode);
 int zfv_mm_set_splitting(zfv_ctx* ctx, int split);
 /* MmFluid::set_max_dt (mm_fluid.cpp:94-96). */
 int zfv_mm_set_max_dt(zfv_ctx* ctx, double max_dt);
+/* Launch control of the multi-material step, no counterpart in the
+ * reference (MmFluid::fluxes evaluates every face in one loop,
+ * mm_fluid.cpp:373-445).  mode 0 (the default): every kernel on the
+ * context's stream; 1: the interface reconstruction on a second stream of
+ * the context, beside the gradient, predictor and composite-face marking;
+ * 2: also the composite fluxes there, beside the update of the cells that
+ * have no composite face, the other cells after them.  Results are the
+ * same bit for bit in every mode; on MI355X modes 1 and 2 measured slower
+ * than 0 (the cross-stream joins cost more than the overlap gains).
+ * crp_groups: one-wavefront workgroups of the composite-flux kernel (0:
+ * 2048). */
+int zfv_mm_set_overlap(zfv_ctx* ctx, int mode, int crp_groups);
+
 /* Cells from {rho, u, v, P, beta}: mmf::PState(rho, v, P, Fractions, mixture)
  * (models.cpp:176-185), beta normalised as Fractions does. */
 int zfv_mm_set_state(zfv_ctx* ctx, const double* rho, const double* u, const double* v, const double* P,
@@ -311,6 +345,40 @@ int zfv_mm_refine(zfv_ctx* ctx, int max_level);
  * that material).  zfv_mm_set_flags computes only the rows its distributor
  * reads (rho, u, v, e, beta); the others come back as NaN after it. */
 int zfv_mm_grad(zfv_ctx* ctx, double* gx, double* gy);
+
+/* ---- output and checkpoints (io/, EuMesh::backup) ------------------------- */
+/* Cell fields of a VTU file (VtuFile variables): density, velocity (3
+ * components, v_z = 0), pressure, specific internal energy, isolation
+ * counter, refinement level, refinement flag, cell index. */
+enum {
+    ZFV_VTU_DENSITY = 1, ZFV_VTU_VELOCITY = 2, ZFV_VTU_PRESSURE = 4, ZFV_VTU_ENERGY = 8, ZFV_VTU_WAIT = 16,
+    ZFV_VTU_LEVEL = 32, ZFV_VTU_FLAG = 64, ZFV_VTU_INDEX = 128,
+    ZFV_VTU_STATE_FIELDS = 31, ZFV_VTU_ALL = 255
+};
+/* VtuFile::save (io/vtu_file.cpp:439-584): the mesh and the chosen cell
+ * fields as a VTK UnstructuredGrid with raw appended data.  polygons = 0:
+ * every cell a quad of its four corners (AmrCells written with hanging
+ * nodes); 1: polygons through the midpoints of split faces
+ * (PvdFile::polyhedral). */
+int zfv_vtu_write(zfv_ctx* ctx, const char* filename, unsigned fields, int polygons);
+/* PvdFile (io/pvd_file.h:12-75): zfv_pvd_open makes directory/filename/
+ * (directory NULL or empty: "output"); every zfv_pvd_save writes the next
+ * VTU file there and rewrites directory/filename.pvd listing all saved
+ * files with their times. */
+typedef struct zfv_pvd zfv_pvd;
+int zfv_pvd_open(const char* filename, const char* directory, zfv_pvd** out);
+int zfv_pvd_save(zfv_pvd* pvd, zfv_ctx* ctx, double time, unsigned fields, int polygons);
+int zfv_pvd_close(zfv_pvd* pvd);
+/* EuMesh::backup (mesh/euler/eu_mesh.cpp:1017-1061, AmrCells::backup
+ * amr_cells.cpp:1074-1172) into directory `root`: backup.json and one raw
+ * file per array in the reference's layout, plus the solver's parameters
+ * and state (cells/data/init as PState {rho, v, p, e}, cells/data/wait).
+ * zfv_restore reads such a directory into a context: mesh, solver
+ * parameters, state, isolation counters and flags; the continued run is
+ * the same as the original's (the reference has no reader,
+ * tests/mesh/backup_read.cpp:23-36). */
+int zfv_backup(zfv_ctx* ctx, const char* root);
+int zfv_restore(zfv_ctx* ctx, const char* root);
 
 /* ---- launch configuration (no reference counterpart) --------------------- */
 /* Step kernels of a stage (0 = first-order update, 1 = gradient + predictor,

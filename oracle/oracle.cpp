@@ -713,6 +713,10 @@ struct orc_mesh {
     // face slots, 8 per cell (amr_faces.h:83-101)
     std::vector<int> bnd, adj;
     std::vector<double> fnx, fny, fcx, fcy, area;
+    // axisymmetric geometry (amr_cells.h volume_alt, amr_faces.h area_alt):
+    // y-weighted volume and face areas, with y-shifted centres
+    bool axial = false;
+    std::vector<double> vol_alt, area_alt;
 };
 
 namespace {
@@ -744,10 +748,19 @@ inline PState boundary_value(const PState& zc, const Vec3& normal, int flag) {
 extern "C" {
 
 // AmrCells(const Rectangle&) / Rectangle::initialize (mesh/euler/amr_cells_gen.cpp:128-281,
-// geom/generator/rectangle.cpp:465-...), axial = false.
-orc_mesh* orc_mesh_rectangle(int nx, int ny, double xmin, double xmax,
-                             double ymin, double ymax, const int* bounds) {
+// geom/generator/rectangle.cpp:465-603); axial: Rectangle::set_axial
+// (rectangle.cpp:97-103, a bottom on the axis becomes a wall) and the
+// alternative volume, face areas and shifted centres (rectangle.cpp:590-602).
+orc_mesh* orc_mesh_rectangle_ax(int nx, int ny, double xmin, double xmax,
+                                double ymin, double ymax, const int* bounds_in, int axial) {
+    int bounds[4] = {bounds_in[0], bounds_in[1], bounds_in[2], bounds_in[3]};
+    if (axial && ymin == 0.0) bounds[2] = ORC_WALL;
     orc_mesh* m = new orc_mesh();
+    m->axial = axial != 0;
+    if (m->axial) {
+        m->vol_alt.resize((size_t)nx * ny);
+        m->area_alt.assign(8 * (size_t)nx * ny, 0.0);
+    }
     int n = nx * ny;
     m->n = n;
     m->cx.resize(n); m->cy.resize(n); m->vol.resize(n);
@@ -802,8 +815,31 @@ orc_mesh* orc_mesh_rectangle(int nx, int ny, double xmin, double xmax,
         m->fcx[f + 3] = q[7].x; m->fcy[f + 3] = q[7].y;
         m->area[f + 0] = hy; m->area[f + 1] = hy;
         m->area[f + 2] = hx; m->area[f + 3] = hx;
+        if (m->axial) {
+            m->vol_alt[ic] = hx * hy * q[4].y;
+            m->area_alt[f + 0] = hy * q[3].y;
+            m->area_alt[f + 1] = hy * q[5].y;
+            m->area_alt[f + 2] = hx * q[1].y;
+            m->area_alt[f + 3] = hx * q[7].y;
+            m->cy[ic] += hy * hy / (12.0 * q[4].y);
+            m->fcy[f + 0] += hy * hy / (12.0 * q[3].y);
+            m->fcy[f + 1] += hy * hy / (12.0 * q[5].y);
+        }
     }
     return m;
+}
+
+orc_mesh* orc_mesh_rectangle(int nx, int ny, double xmin, double xmax,
+                             double ymin, double ymax, const int* bounds) {
+    return orc_mesh_rectangle_ax(nx, ny, xmin, xmax, ymin, ymax, bounds, 0);
+}
+
+int orc_mesh_axial(const orc_mesh* m) { return m->axial ? 1 : 0; }
+
+void orc_mesh_get_axial(const orc_mesh* m, double* vol_alt, double* area_alt) {
+    if (!m->axial) return;
+    std::memcpy(vol_alt, m->vol_alt.data(), (size_t)m->n * 8);
+    std::memcpy(area_alt, m->area_alt.data(), 8 * (size_t)m->n * 8);
 }
 
 void orc_mesh_free(orc_mesh* m) { delete m; }
@@ -870,6 +906,7 @@ struct orc_smf {
     // SmFluid::Parts (math/solver/sm_fluid.h:26-36)
     std::vector<PState> init, half, next, d_dx, d_dy, d_dz;
     std::vector<int> wait, wait_next;
+    bool axial = false;   // SmFluid::set_axial (sm_fluid.cpp:45-47)
 };
 
 namespace {
@@ -981,7 +1018,21 @@ inline PState extrap(const PState& u, const PState& gx, const PState& gy, const 
     return r;
 }
 
-// SmFluid::fluxes, first order (sm_fluid.cpp:127-174), axial = false.
+// face.area(m_axial) and cell.volume(m_axial) (amr_faces.h:183-185,
+// amr_cells.h:288-290).
+inline double flux_area(const orc_smf& S, size_t s) { return S.axial ? S.m->area_alt[s] : S.m->area[s]; }
+inline double flux_volume(const orc_smf& S, int c) { return S.axial ? S.m->vol_alt[c] : S.m->vol[c]; }
+
+// The axisymmetric pressure source: momentum.y += k * coeff * p * dt with
+// coeff = volume / volume_alt (sm_fluid.cpp:166-169, 242-245, 347-350;
+// k = 0.5 in the predictor).
+inline void axial_source(const orc_smf& S, int c, double k, double p, QState& q) {
+    if (!S.axial) return;
+    const double coeff = S.m->vol[c] / S.m->vol_alt[c];
+    q.m.y += (k == 1.0 ? coeff : k * coeff) * p * S.dt;
+}
+
+// SmFluid::fluxes, first order (sm_fluid.cpp:127-174).
 void fluxes_cell(orc_smf& S, int c) {
     const orc_mesh& m = *S.m;
     PState z_c = S.init[c];
@@ -997,10 +1048,11 @@ void fluxes_cell(orc_smf& S, int c) {
         PState zp = in_local(z_n, normal);
         Flux lf = num_flux(S.flux, zm, zp, S.eos);
         flux_to_global(lf, normal);
-        for (int k = 0; k < 5; ++k) flux.arr()[k] = flux.arr()[k] + lf.arr()[k] * m.area[s];
+        for (int k = 0; k < 5; ++k) flux.arr()[k] = flux.arr()[k] + lf.arr()[k] * flux_area(S, s);
     }
-    double coef = S.dt / m.vol[c];
+    double coef = S.dt / flux_volume(S, c);
     for (int k = 0; k < 5; ++k) q_c.arr()[k] = q_c.arr()[k] - coef * flux.arr()[k];
+    axial_source(S, c, 1.0, z_c.p, q_c);
     S.next[c] = to_pstate(q_c, S.eos);
 }
 
@@ -1029,10 +1081,11 @@ void stage1_cell(orc_smf& S, int c) {
         zm.v = rot_local(zm.v, normal);
         Flux lf = phys_flux(zm);
         flux_to_global(lf, normal);
-        for (int k = 0; k < 5; ++k) flux.arr()[k] = flux.arr()[k] + lf.arr()[k] * m.area[s];
+        for (int k = 0; k < 5; ++k) flux.arr()[k] = flux.arr()[k] + lf.arr()[k] * flux_area(S, s);
     }
-    double coef = 0.5 * S.dt / m.vol[c];
+    double coef = 0.5 * S.dt / flux_volume(S, c);
     for (int k = 0; k < 5; ++k) q_c.arr()[k] = q_c.arr()[k] - coef * flux.arr()[k];
+    axial_source(S, c, 0.5, z_c.p, q_c);
     S.half[c] = to_pstate(q_c, S.eos);
     if (is_bad(S.half[c], S.eos)) S.half[c] = z_c;
 }
@@ -1087,10 +1140,11 @@ void stage2_cell(orc_smf& S, int c) {
         zp.v = rot_local(zp.v, normal);
         Flux lf = robust ? hll(zm, zp, S.eos) : num_flux(S.flux, zm, zp, S.eos);
         flux_to_global(lf, normal);
-        for (int k = 0; k < 5; ++k) flux.arr()[k] = flux.arr()[k] + lf.arr()[k] * m.area[s];
+        for (int k = 0; k < 5; ++k) flux.arr()[k] = flux.arr()[k] + lf.arr()[k] * flux_area(S, s);
     }
-    double coef = S.dt / m.vol[c];
+    double coef = S.dt / flux_volume(S, c);
     for (int k = 0; k < 5; ++k) q_c.arr()[k] = q_c.arr()[k] - coef * flux.arr()[k];
+    axial_source(S, c, 1.0, z_ch.p, q_c);
     PState zn = to_pstate(q_c, S.eos);
     if (is_bad(zn, S.eos)) {
         S.next[c] = z_c;
@@ -1140,6 +1194,12 @@ orc_smf* orc_smf_create(orc_mesh* m, double gamma, double P0, double e0,
 
 void orc_smf_free(orc_smf* s) { delete s; }
 void orc_smf_set_max_dt(orc_smf* s, double max_dt) { s->max_dt = max_dt; }
+
+int orc_smf_set_axial(orc_smf* s, int axial) {
+    if (axial && !s->m->axial) return -1;   // no alternative geometry on this mesh
+    s->axial = axial != 0;
+    return 0;
+}
 
 void orc_smf_set_state(orc_smf* s, const double* rho, const double* u, const double* v,
                        const double* p, const double* e) {
@@ -1217,7 +1277,7 @@ void orc_smf_totals(const orc_smf* s, double* out) {
     double t[4] = {0, 0, 0, 0};
     for (int c = 0; c < s->m->n; ++c) {
         QState q = to_qstate(s->init[c]);
-        double V = s->m->vol[c];
+        double V = flux_volume(*s, c);
         t[0] += V * q.rho; t[1] += V * q.m.x; t[2] += V * q.m.y; t[3] += V * q.E;
     }
     for (int k = 0; k < 4; ++k) out[k] = t[k];

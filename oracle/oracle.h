@@ -63,6 +63,14 @@ void orc_eos(const double* eos, double rho, double e, double p, double* out);
 typedef struct orc_mesh orc_mesh;
 orc_mesh* orc_mesh_rectangle(int nx, int ny, double xmin, double xmax,
                              double ymin, double ymax, const int* bounds /*L,R,B,T*/);
+// The same with the axisymmetric geometry when axial != 0: y-weighted
+// volume and face areas, y-shifted cell and side-face centres, a bottom on
+// the axis (ymin = 0) made a wall (Rectangle::set_axial).
+orc_mesh* orc_mesh_rectangle_ax(int nx, int ny, double xmin, double xmax,
+                                double ymin, double ymax, const int* bounds, int axial);
+int orc_mesh_axial(const orc_mesh* m);
+// volume_alt (n) and area_alt (8n) of an axial mesh.
+void orc_mesh_get_axial(const orc_mesh* m, double* vol_alt, double* area_alt);
 void orc_mesh_free(orc_mesh* m);
 int  orc_mesh_size(const orc_mesh* m);
 // Cell arrays (n): cx, cy, volume, level, flag, b_idx, z_idx.
@@ -84,6 +92,8 @@ orc_smf* orc_smf_create(orc_mesh* m, double gamma, double P0, double e0,
                         int flux, int limiter, int acc, double cfl);
 void orc_smf_free(orc_smf* s);
 void orc_smf_set_max_dt(orc_smf* s, double max_dt);
+// SmFluid::set_axial; -1 if the mesh has no axial geometry.
+int orc_smf_set_axial(orc_smf* s, int axial);
 void orc_smf_set_state(orc_smf* s, const double* rho, const double* u, const double* v,
                        const double* p, const double* e);
 void orc_smf_get_state(const orc_smf* s, double* rho, double* u, double* v,
@@ -108,6 +118,7 @@ void orc_smf_set_flags(orc_smf* s, int criterion, double threshold, int nthreads
 void orc_mesh_set_flags(orc_mesh* m, const int* flags);
 void orc_mesh_balance_flags(orc_mesh* m, int max_level);
 // EuMesh::refine: balance + apply; s may be NULL (mesh only); slope: distributor.
+// Returns 1 changed, 0 unchanged, -2 for an axial mesh (not adapted here).
 int orc_mesh_refine(orc_mesh* m, orc_smf* s, int max_level, int slope);
 void orc_mesh_get_next(const orc_mesh* m, int* next);
 

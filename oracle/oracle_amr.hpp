@@ -611,6 +611,7 @@ void orc_mesh_balance_flags(orc_mesh* m, int max_level) { balance_flags(*m, max_
 // EuMesh::refine (eu_mesh.cpp:290-359): balance, then apply with the solver's
 // distributor (slope = 1: "slope", 0: "const").  S may be NULL (mesh only).
 int orc_mesh_refine(orc_mesh* m, orc_smf* S, int max_level, int slope) {
+    if (m->axial) return -2;   // the restatement adapts planar meshes only
     balance_flags(*m, max_level);
     if (!S) return apply_flags(*m, nullptr) ? 1 : 0;
     SmfData D(S, slope != 0);

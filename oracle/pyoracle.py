@@ -45,6 +45,10 @@ def lib():
             "orc_riemann_sample": (None, [_D] * 9 + [_dp]),
             "orc_eos": (None, [_dp, _D, _D, _D, _dp]),
             "orc_mesh_rectangle": (_P, [_I, _I, _D, _D, _D, _D, _ip]),
+            "orc_mesh_rectangle_ax": (_P, [_I, _I, _D, _D, _D, _D, _ip, _I]),
+            "orc_mesh_axial": (_I, [_P]),
+            "orc_mesh_get_axial": (None, [_P, _dp, _dp]),
+            "orc_smf_set_axial": (_I, [_P, _I]),
             "orc_mesh_free": (None, [_P]),
             "orc_mesh_size": (_I, [_P]),
             "orc_mesh_get_cells": (None, [_P, _dp, _dp, _dp, _ip, _ip, _ip, _ip]),
@@ -172,9 +176,15 @@ class Mesh:
         self.h = handle
 
     @classmethod
-    def rectangle(cls, nx, ny, xmin=0.0, xmax=1.0, ymin=0.0, ymax=1.0, bounds=(ZOE,) * 4):
+    def rectangle(cls, nx, ny, xmin=0.0, xmax=1.0, ymin=0.0, ymax=1.0, bounds=(ZOE,) * 4, axial=False):
         b = np.ascontiguousarray(bounds, dtype=np.int32)
-        return cls(lib().orc_mesh_rectangle(nx, ny, xmin, xmax, ymin, ymax, _i(b)))
+        return cls(lib().orc_mesh_rectangle_ax(nx, ny, xmin, xmax, ymin, ymax, _i(b), 1 if axial else 0))
+
+    def axial_geometry(self):
+        """(volume_alt (n), area_alt (8n)) of an axial mesh."""
+        v, a = np.zeros(self.n), np.zeros(8 * self.n)
+        lib().orc_mesh_get_axial(self.h, _d(v), _d(a))
+        return v, a
 
     def __del__(self):
         if getattr(self, "h", None) and _lib is not None:
@@ -280,6 +290,11 @@ class SmFluid:
 
     def set_max_dt(self, v):
         lib().orc_smf_set_max_dt(self.h, v)
+
+    def set_axial(self, axial=True):
+        """SmFluid::set_axial; the mesh must carry the axial geometry."""
+        if lib().orc_smf_set_axial(self.h, 1 if axial else 0) != 0:
+            raise ValueError("the mesh has no axial geometry")
 
     def update(self, nthreads=None):
         nt = nthreads or min(8, os.cpu_count() or 1)

@@ -23,7 +23,8 @@ ZFV_OK, ZFV_ERR_ARG, ZFV_ERR_HIP, ZFV_ERR_NUMERIC, ZFV_ERR_UNSUPPORTED, ZFV_ERR_
 # Every symbol include/zfv.h declares.
 EXPORTS = [
     "zfv_last_error", "zfv_version", "zfv_create", "zfv_destroy", "zfv_set_stream",
-    "zfv_synchronize", "zfv_mesh_upload", "zfv_mesh_rectangle", "zfv_mesh_n_cells",
+    "zfv_synchronize", "zfv_mesh_upload", "zfv_mesh_rectangle", "zfv_mesh_rectangle_axial",
+    "zfv_mesh_axial_download", "zfv_mesh_upload_axial", "zfv_mesh_n_cells",
     "zfv_mesh_download", "zfv_smf_configure", "zfv_smf_set_max_dt", "zfv_state_upload",
     "zfv_state_download", "zfv_state_upload_device", "zfv_state_download_device",
     "zfv_wait_upload", "zfv_wait_download", "zfv_grad_download", "zfv_smf_step", "zfv_smf_dt",
@@ -34,11 +35,16 @@ EXPORTS = [
     "zfv_batch_rectangles", "zfv_batch_layout", "zfv_batch_dt", "zfv_graph_enable",
     "zfv_set_cell_order", "zfv_get_cell_order",
     "zfv_mm_set_limiter", "zfv_mm_set_accuracy", "zfv_mm_set_method", "zfv_mm_set_crp_mode",
-    "zfv_mm_set_splitting", "zfv_mm_set_flags", "zfv_mm_refine", "zfv_mm_grad",
+    "zfv_mm_set_splitting", "zfv_mm_set_overlap", "zfv_mm_set_flags", "zfv_mm_refine", "zfv_mm_grad",
     "zfv_adjoint_forward", "zfv_adjoint_backward",
     "zfv_mm_configure", "zfv_mm_set_crp", "zfv_mm_set_max_dt", "zfv_mm_set_state", "zfv_mm_state", "zfv_mm_step", "zfv_mm_dt", "zfv_mm_crp_faces",
     "zfv_mm_interface",
+    "zfv_smf_get_params", "zfv_vtu_write", "zfv_pvd_open", "zfv_pvd_save", "zfv_pvd_close",
+    "zfv_backup", "zfv_restore",
 ]
+VTU_DENSITY, VTU_VELOCITY, VTU_PRESSURE, VTU_ENERGY, VTU_WAIT, VTU_LEVEL, VTU_FLAG, VTU_INDEX = (
+    1, 2, 4, 8, 16, 32, 64, 128)
+VTU_ALL = 255
 CELL, QUAD, BATCH, TILE = 0, 1, 2, 3   # step-kernel layouts (TILE: the whole MUSCL step, stage 1)
 
 
@@ -62,7 +68,7 @@ class MeshSoA(C.Structure):
 
 class SmfParams(C.Structure):
     _fields_ = [
-        ("acc", C.c_int32), ("flux", C.c_int32), ("limiter", C.c_int32), ("reserved", C.c_int32),
+        ("acc", C.c_int32), ("flux", C.c_int32), ("limiter", C.c_int32), ("axial", C.c_int32),
         ("cfl", C.c_double), ("max_dt", C.c_double),
         ("gamma", C.c_double), ("P0", C.c_double), ("e0", C.c_double),
     ]
@@ -89,6 +95,8 @@ def lib():
                 f.restype = C.c_int
         L.zfv_create.argtypes = [C.c_int, C.POINTER(vp)]
         L.zfv_mesh_rectangle.argtypes = [vp, C.c_int, C.c_int, d, d, d, d, vp]
+        L.zfv_mesh_rectangle_axial.argtypes = [vp, C.c_int, C.c_int, d, d, d, d, vp]
+        L.zfv_mesh_axial_download.argtypes = [vp, vp, vp]
         L.zfv_smf_set_max_dt.argtypes = [vp, d]
         L.zfv_smf_step.argtypes = [vp, C.c_int]
         L.zfv_profile_enable.argtypes = [vp, C.c_int]
@@ -96,6 +104,13 @@ def lib():
         L.zfv_balance_flags.argtypes = [vp, C.c_int]
         L.zfv_refine.argtypes = [vp, C.c_int, C.c_int]
         L.zfv_mesh_reserve.argtypes = [vp, C.c_int]
+        L.zfv_vtu_write.argtypes = [vp, C.c_char_p, C.c_uint, C.c_int]
+        L.zfv_pvd_open.argtypes = [C.c_char_p, C.c_char_p, C.POINTER(vp)]
+        L.zfv_pvd_save.argtypes = [vp, vp, d, C.c_uint, C.c_int]
+        L.zfv_pvd_close.argtypes = [vp]
+        L.zfv_backup.argtypes = [vp, C.c_char_p]
+        L.zfv_restore.argtypes = [vp, C.c_char_p]
+        L.zfv_smf_get_params.argtypes = [vp, vp]
         L.zfv_set_launch.argtypes = [vp, C.c_int, C.c_int, C.c_int]
         L.zfv_autotune.argtypes = [vp, C.c_int, vp]
         L.zfv_batch_rectangles.argtypes = [vp, C.c_int, vp, vp, vp, vp]
@@ -115,6 +130,7 @@ def lib():
         L.zfv_mm_set_method.argtypes = [vp, C.c_int]
         L.zfv_mm_set_crp_mode.argtypes = [vp, C.c_int]
         L.zfv_mm_set_splitting.argtypes = [vp, C.c_int]
+        L.zfv_mm_set_overlap.argtypes = [vp, C.c_int, C.c_int]
         L.zfv_mm_set_flags.argtypes = [vp]
         L.zfv_mm_refine.argtypes = [vp, C.c_int]
         L.zfv_mm_grad.argtypes = [vp, vp, vp]
@@ -140,6 +156,28 @@ def limiter_id(name):
 
 def _ptr(a):
     return C.c_void_p(a.ctypes.data) if a is not None else None
+
+
+class PvdFile:
+    """PvdFile (io/pvd_file.h:12-75): a series of VTU files and their PVD collection."""
+
+    def __init__(self, filename, directory="output"):
+        self.h = C.c_void_p()
+        _check(lib().zfv_pvd_open(os.fsencode(filename), os.fsencode(directory), C.byref(self.h)))
+
+    def save(self, ctx, time, fields=VTU_ALL, polygons=False):
+        _check(lib().zfv_pvd_save(self.h, ctx.h, float(time), fields, 1 if polygons else 0))
+
+    def close(self):
+        if self.h:
+            lib().zfv_pvd_close(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class Context:
@@ -169,6 +207,18 @@ class Context:
         _check(lib().zfv_mesh_rectangle(self.h, nx, ny, xmin, xmax, ymin, ymax, _ptr(b)))
         self.n = nx * ny
 
+    def mesh_rectangle_axial(self, nx, ny, xmin=0.0, xmax=1.0, ymin=0.0, ymax=1.0, bounds=(ZOE,) * 4):
+        """Rectangle with set_axial(true): y is the radius."""
+        b = np.ascontiguousarray(bounds, dtype=np.int32)
+        _check(lib().zfv_mesh_rectangle_axial(self.h, nx, ny, xmin, xmax, ymin, ymax, _ptr(b)))
+        self.n = nx * ny
+
+    def mesh_axial(self):
+        """(volume_alt (n), area_alt (8n)) of an axisymmetric mesh."""
+        v, a = np.zeros(self.n), np.zeros(8 * self.n)
+        _check(lib().zfv_mesh_axial_download(self.h, _ptr(v), _ptr(a)))
+        return v, a
+
     def mesh_upload(self, cells, slots, verts):
         n = len(cells["cx"])
         arrs = dict(
@@ -194,6 +244,28 @@ class Context:
         _check(lib().zfv_mesh_n_faces(self.h, C.byref(v)))
         return v.value
 
+    def vtu_write(self, filename, fields=VTU_ALL, polygons=False):
+        """VtuFile::save: the mesh and cell fields as a VTU file."""
+        _check(lib().zfv_vtu_write(self.h, os.fsencode(filename), fields, 1 if polygons else 0))
+
+    def backup(self, root):
+        """EuMesh::backup of the mesh and the solver state into directory root."""
+        _check(lib().zfv_backup(self.h, os.fsencode(root)))
+
+    def restore(self, root):
+        """Mesh, solver parameters and state from a zfv_backup directory."""
+        _check(lib().zfv_restore(self.h, os.fsencode(root)))
+        n = C.c_int32()
+        _check(lib().zfv_mesh_n_cells(self.h, C.byref(n)))
+        self.n = n.value
+
+    def params(self):
+        """The solver's parameters (acc, flux, limiter, axial, cfl, max_dt, gamma, P0, e0)."""
+        p = SmfParams()
+        _check(lib().zfv_smf_get_params(self.h, C.byref(p)))
+        return dict(acc=p.acc, flux=p.flux, limiter=p.limiter, axial=p.axial, cfl=p.cfl, max_dt=p.max_dt,
+                    gamma=p.gamma, P0=p.P0, e0=p.e0)
+
     def reserve(self, n_cells):
         """Device capacity for meshes of up to n_cells cells (AmrCells::reserve_amr)."""
         _check(lib().zfv_mesh_reserve(self.h, int(n_cells)))
@@ -215,10 +287,12 @@ class Context:
         return cells, slots, (vx, vy)
 
     # ---- solver
-    def configure(self, acc=1, flux=HLLC, limiter="MC", cfl=0.5, gamma=1.4, P0=0.0, e0=0.0, max_dt=None):
+    def configure(self, acc=1, flux=HLLC, limiter="MC", cfl=0.5, gamma=1.4, P0=0.0, e0=0.0, max_dt=None,
+                  axial=False):
+        """SmFluid setters; axial = set_axial (needs mesh_rectangle_axial)."""
         if isinstance(flux, str):
             flux = FLUXES[flux.lower()]
-        p = SmfParams(acc=acc, flux=flux, limiter=limiter_id(limiter), reserved=0, cfl=cfl,
+        p = SmfParams(acc=acc, flux=flux, limiter=limiter_id(limiter), axial=1 if axial else 0, cfl=cfl,
                       max_dt=np.finfo(np.float64).max if max_dt is None else max_dt,
                       gamma=gamma, P0=P0, e0=e0)
         _check(lib().zfv_smf_configure(self.h, C.byref(p)))
@@ -410,6 +484,12 @@ class Context:
         """MmFluid::set_crp_mode by name ("V3", "V5", "VS", "PLIC"; "NONE" is
         ignored, as in the reference)."""
         _check(lib().zfv_mm_set_crp_mode(self.h, self.CRP_MODES.index(mode)))
+
+    def mm_set_overlap(self, mode=0, crp_groups=0):
+        """0 (default): one stream; 1: the interface on a second stream; 2:
+        also the composite fluxes, beside the update of the other cells;
+        crp_groups 0 = the default grid of the composite-flux kernel."""
+        _check(lib().zfv_mm_set_overlap(self.h, int(mode), crp_groups))
 
     def mm_set_splitting(self, split):
         """MmFluid::set_splitting: "NONE", "SIMPLE" or "STRANG"."""

@@ -22,6 +22,7 @@ SOURCES = [
     os.path.join(HERE, "csrc", "zfv_order.hip"),
     os.path.join(HERE, "csrc", "zfv_tile.hip"),
     os.path.join(HERE, "csrc", "zfv_capi.cpp"),
+    os.path.join(HERE, "csrc", "zfv_io.cpp"),
 ]
 HEADERS = [
     os.path.join(HERE, "csrc", "zfv_physics.hpp"),

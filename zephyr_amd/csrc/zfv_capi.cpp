@@ -77,6 +77,9 @@ struct HostMesh {
     std::vector<int32_t> level, flag, b_idx, z_idx;
     std::vector<int32_t> bnd, adj;
     std::vector<double> nx, ny, fcx, fcy, area;
+    // axisymmetric geometry (AmrCells::volume_alt, AmrFaces::area_alt),
+    // empty for a planar mesh
+    std::vector<double> vol_alt, area_alt;
 };
 
 struct KernelTimer {
@@ -99,6 +102,9 @@ struct zfv_ctx {
     bool host_stale = false;   // hm out of date after a device-side adaptation
     DBuf<double> cx, cy, vol, h, area, fcx, fcy, w, nx, ny, vx, vy;
     DBuf<int> code, adj, bnd, level, flag, flag0, b_idx, z_idx, next, idx;
+    // axisymmetric geometry: volume_alt, area_alt [8][cap], volume / volume_alt
+    bool axial_mesh = false;
+    DBuf<double> vol_alt, area_alt, axc;
     // AMR scratch
     DBuf<Marks3> s_marks, s_rank;
     DBuf<unsigned long long> s_faces;
@@ -201,9 +207,16 @@ struct zfv_ctx {
         int grad_rows = 0;            // rows of gx / gy the last compute_grad filled
         int limiter = 3;              // MC (mm_fluid.cpp:20)
         int acc = 1;                  // m_acc (mm_fluid.h:157)
-        DBuf<int> crp_list;
-        DBuf<unsigned int> crp_count;
+        DBuf<int> crp_list, crp_cells;
+        DBuf<unsigned int> crp_count;   // [0] listed faces, [1] listed cells
         DBuf<unsigned long long> dtmin;
+        // The composite fluxes on a second stream (same priority), beside
+        // the update of the cells without a composite face: `side` and the
+        // fork / join events are created on the first overlapped step.
+        int overlap = 0;        // zfv_mm_set_overlap mode
+        int crp_groups = 0;     // k_mm_crp workgroups, 0: 2048
+        hipStream_t side = nullptr;
+        hipEvent_t ev_in = nullptr, ev_mark = nullptr, ev_crp = nullptr;
     } mm;
     long long mesh_epoch = 0;   // incremented whenever the mesh changes
 
@@ -247,11 +260,25 @@ struct zfv_ctx {
         M.chunk_pid = nseg > 1 ? chunk_pid.p : nullptr;
         M.seg_begin = nseg > 1 ? seg_begin.p : nullptr;
         M.seg_end = nseg > 1 ? seg_end.p : nullptr;
+        M.axc = nullptr;
         return M;
     }
-    // The mesh in the order the state is currently held in.
+    // The mesh of the SmFluid update: with set_axial, cell.volume(true) and
+    // face.area(true) in place of the planar ones and the source
+    // coefficient (an axisymmetric mesh is never adapted, so never curve
+    // ordered).
+    DevMesh ax(DevMesh M) const {
+        if (prm.axial && axial_mesh) {
+            M.vol = vol_alt.p;
+            M.area = area_alt.p;
+            M.axc = axc.p;
+        }
+        return M;
+    }
+    // The mesh in the order the state is currently held in, as the step
+    // kernels read it.
     DevMesh smesh() const {
-        if (!co.state) return dmesh();
+        if (!co.state) return ax(dmesh());
         DevMesh M;
         M.n = hm.n;
         M.stride = co.cap;
@@ -261,6 +288,7 @@ struct zfv_ctx {
         M.chunk_pid = nullptr;
         M.seg_begin = nullptr;
         M.seg_end = nullptr;
+        M.axc = nullptr;
         return M;
     }
     TileMesh tmesh() const {
@@ -277,7 +305,8 @@ struct zfv_ctx {
     // Whether the steps run the tiled kernel: structured rectangles, MUSCL
     // with HLLC, the state in the mesh's order.
     bool tile_step() const {
-        return tiles.ok && prm.acc == 2 && prm.flux == ZFV_HLLC && layout[1] == kLayoutTile && !co.state;
+        return tiles.ok && prm.acc == 2 && prm.flux == ZFV_HLLC && layout[1] == kLayoutTile && !co.state &&
+               !prm.axial;
     }
     // Launch configuration of the gradient kernel when it runs on its own.
     int gp_layout() const { return layout[1] == kLayoutTile ? kLayoutCell : layout[1]; }
@@ -547,6 +576,17 @@ int upload_mesh(zfv_ctx* ctx) {
     ZFV_HIP(ctx->err.alloc(1));
     ZFV_HIP(hipMemsetAsync(ctx->err.p, 0, sizeof(int), s));
     ZFV_HIP(ctx->partial.alloc(4 * (size_t)((n + kBlock - 1) / kBlock)));
+    ctx->axial_mesh = !m.vol_alt.empty();
+    if (ctx->axial_mesh) {   // the alternative geometry, slot-major, and volume / volume_alt per cell
+        std::vector<double> aa(ns, 0.0), axc(n, 0.0);
+        for (int c = 0; c < n; ++c) {
+            axc[c] = m.vol[c] / m.vol_alt[c];
+            for (int s8 = 0; s8 < 8; ++s8) aa[(size_t)s8 * n + c] = m.area_alt[8 * (size_t)c + s8];
+        }
+        ZFV_HIP(put(ctx->vol_alt, m.vol_alt));
+        ZFV_HIP(put(ctx->area_alt, aa));
+        ZFV_HIP(put(ctx->axc, axc));
+    }
     build_tiles(ctx, code, w);
     ZFV_HIP(hipStreamSynchronize(s));
     ctx->graphs_clear();
@@ -763,17 +803,22 @@ hipEvent_t pooled_event(zfv_ctx* ctx) {
 }
 
 template <class F>
-hipError_t timed(zfv_ctx* ctx, const char* name, F&& launch) {
+hipError_t timed_on(zfv_ctx* ctx, hipStream_t s, const char* name, F&& launch) {
     if (!ctx->profile) return launch();
     KernelTimer t;
     t.name = name;
     t.t0 = pooled_event(ctx);
     t.t1 = pooled_event(ctx);
-    (void)hipEventRecord(t.t0, ctx->stream);
+    (void)hipEventRecord(t.t0, s);
     hipError_t e = launch();
-    (void)hipEventRecord(t.t1, ctx->stream);
+    (void)hipEventRecord(t.t1, s);
     ctx->timers.push_back(t);
     return e;
+}
+
+template <class F>
+hipError_t timed(zfv_ctx* ctx, const char* name, F&& launch) {
+    return timed_on(ctx, ctx->stream, name, launch);
 }
 
 int ensure_dt(zfv_ctx* ctx) {
@@ -807,6 +852,8 @@ int need_ready(zfv_ctx* ctx, bool state = true, bool config = false) {
     if (!ctx->have_mesh) return fail(ZFV_ERR_STATE, "no mesh uploaded");
     if (state && !ctx->have_state) return fail(ZFV_ERR_STATE, "no state uploaded");
     if (config && !ctx->configured) return fail(ZFV_ERR_STATE, "solver not configured");
+    if (config && ctx->prm.axial && !ctx->axial_mesh)
+        return fail(ZFV_ERR_STATE, "axisymmetric solver on a mesh without the axisymmetric geometry");
     return ZFV_OK;
 }
 
@@ -944,6 +991,10 @@ double* scratch_of(int device) {
 
 }  // namespace
 
+namespace zfv {
+int set_error(int code, const char* msg) { return fail(code, msg ? msg : ""); }
+}  // namespace zfv
+
 extern "C" {
 
 const char* zfv_last_error(void) { return g_last_error.c_str(); }
@@ -983,6 +1034,12 @@ int zfv_destroy(zfv_ctx* ctx) {
     collect_timers(ctx);
     ctx->graphs_clear();
     for (hipEvent_t e : ctx->event_pool) (void)hipEventDestroy(e);
+    for (hipEvent_t e : {ctx->mm.ev_in, ctx->mm.ev_mark, ctx->mm.ev_crp})
+        if (e) (void)hipEventDestroy(e);
+    if (ctx->mm.side) {
+        (void)hipStreamSynchronize(ctx->mm.side);
+        (void)hipStreamDestroy(ctx->mm.side);
+    }
     if (ctx->marks_total) (void)hipHostFree(ctx->marks_total);
     hipStream_t s = ctx->own_stream;
     delete ctx;
@@ -1007,7 +1064,16 @@ int zfv_synchronize(zfv_ctx* ctx) {
     return ZFV_OK;
 }
 
-int zfv_mesh_upload(zfv_ctx* ctx, const zfv_mesh_soa* M) {
+static int mesh_upload_impl(zfv_ctx* ctx, const zfv_mesh_soa* M, const double* vol_alt, const double* area_alt);
+
+int zfv_mesh_upload(zfv_ctx* ctx, const zfv_mesh_soa* M) { return mesh_upload_impl(ctx, M, nullptr, nullptr); }
+
+int zfv_mesh_upload_axial(zfv_ctx* ctx, const zfv_mesh_soa* M, const double* vol_alt, const double* area_alt) {
+    if (!vol_alt || !area_alt) return fail(ZFV_ERR_ARG, "null volume_alt or area_alt");
+    return mesh_upload_impl(ctx, M, vol_alt, area_alt);
+}
+
+static int mesh_upload_impl(zfv_ctx* ctx, const zfv_mesh_soa* M, const double* vol_alt, const double* area_alt) {
     int r = check_ctx(ctx);
     if (r) return r;
     if (!M) return fail(ZFV_ERR_ARG, "null mesh");
@@ -1018,10 +1084,20 @@ int zfv_mesh_upload(zfv_ctx* ctx, const zfv_mesh_soa* M) {
     if (!M->cx || !M->cy || !M->volume || !M->vx || !M->vy || !M->boundary || !M->adj || !M->nx ||
         !M->ny || !M->fcx || !M->fcy || !M->area)
         return fail(ZFV_ERR_ARG, "mesh array missing");
+    if (vol_alt)
+        for (int c = 0; c < n; ++c)
+            if (!(vol_alt[c] > 0.0) || !std::isfinite(vol_alt[c]))
+                return fail(ZFV_ERR_ARG, "volume_alt must be positive and finite (cells off the axis)");
     HostMesh& m = ctx->hm;
     const size_t ns = 8 * (size_t)n, nv = 9 * (size_t)n;
     ctx->nseg = 1;
     m.n = n;
+    m.vol_alt.clear();
+    m.area_alt.clear();
+    if (vol_alt) {
+        m.vol_alt.assign(vol_alt, vol_alt + n);
+        m.area_alt.assign(area_alt, area_alt + ns);
+    }
     m.cx.assign(M->cx, M->cx + n);
     m.cy.assign(M->cy, M->cy + n);
     m.vol.assign(M->volume, M->volume + n);
@@ -1046,7 +1122,16 @@ int zfv_mesh_upload(zfv_ctx* ctx, const zfv_mesh_soa* M) {
     m.fcx.assign(M->fcx, M->fcx + ns);
     m.fcy.assign(M->fcy, M->fcy + ns);
     m.area.assign(M->area, M->area + ns);
-    ctx->rects.clear();   // an uploaded mesh: no Rectangle record, no tiled step
+    // An uploaded mesh may be a Rectangle (a restored checkpoint of one, or
+    // the reference's own): unrefined cells whose first column holds the
+    // same centre x.  build_tiles then checks every cell against the
+    // Rectangle's tables and takes the tiled step only if all match.
+    ctx->rects.clear();
+    bool flat = true;
+    for (int c = 0; c < n && flat; ++c) flat = m.level[c] == 0;
+    int ny = 1;
+    while (flat && ny < n && m.cx[ny] == m.cx[0]) ++ny;
+    if (flat && n % ny == 0) ctx->rects.assign(1, zfv_ctx::RectInfo{0, n / ny, ny});
     return upload_mesh(ctx);
 }
 
@@ -1078,10 +1163,19 @@ static void resize_host_mesh(HostMesh& m, int n) {
     m.bnd.assign(ns, ZFV_UNDEFINED); m.adj.assign(ns, -1);
     m.nx.assign(ns, 0.0); m.ny.assign(ns, 0.0); m.fcx.assign(ns, 0.0); m.fcy.assign(ns, 0.0);
     m.area.assign(ns, 0.0);
+    m.vol_alt.clear();
+    m.area_alt.clear();
 }
 
+// AmrCells of a Rectangle (rectangle.cpp:465-603); axial: the y-weighted
+// volume and face areas and the y-shifted cell and side-face centres of
+// Rectangle::set_axial(true) (rectangle.cpp:590-602).
 static void build_rectangle(HostMesh& m, int off, int nx, int ny, double xmin, double xmax, double ymin,
-                            double ymax, const int32_t* bounds) {
+                            double ymax, const int32_t* bounds, bool axial = false) {
+    if (axial && m.vol_alt.empty()) {
+        m.vol_alt.assign(m.n, 0.0);
+        m.area_alt.assign(8 * (size_t)m.n, 0.0);
+    }
     const bool xper = bounds[0] == ZFV_PERIODIC && bounds[1] == ZFV_PERIODIC;
     const bool yper = bounds[2] == ZFV_PERIODIC && bounds[3] == ZFV_PERIODIC;
     const double hx = (xmax - xmin) / nx, hy = (ymax - ymin) / ny;
@@ -1125,6 +1219,16 @@ static void build_rectangle(HostMesh& m, int off, int nx, int ny, double xmin, d
                 m.fcy[f + s] = Y[cen[s]];
                 m.area[f + s] = s < 2 ? hy : hx;
             }
+            if (axial) {
+                m.vol_alt[c] = hx * hy * Y[4];
+                m.area_alt[f + 0] = hy * Y[3];
+                m.area_alt[f + 1] = hy * Y[5];
+                m.area_alt[f + 2] = hx * Y[1];
+                m.area_alt[f + 3] = hx * Y[7];
+                m.cy[c] += hy * hy / (12.0 * Y[4]);
+                m.fcy[f + 0] += hy * hy / (12.0 * Y[3]);
+                m.fcy[f + 1] += hy * hy / (12.0 * Y[5]);
+            }
         }
     }
 }
@@ -1140,6 +1244,33 @@ int zfv_mesh_rectangle(zfv_ctx* ctx, int nx, int ny, double xmin, double xmax,
     build_rectangle(ctx->hm, 0, nx, ny, xmin, xmax, ymin, ymax, bounds);
     ctx->rects.assign(1, zfv_ctx::RectInfo{0, nx, ny});
     return upload_mesh(ctx);
+}
+
+int zfv_mesh_rectangle_axial(zfv_ctx* ctx, int nx, int ny, double xmin, double xmax, double ymin, double ymax,
+                             const int32_t* bounds) {
+    int r = check_ctx(ctx);
+    if (r) return r;
+    r = check_rectangle(nx, ny, xmin, xmax, ymin, ymax, bounds);
+    if (r) return r;
+    if (!(ymin >= 0.0)) return fail(ZFV_ERR_ARG, "axisymmetric rectangle: ymin must be >= 0 (y is the radius)");
+    int32_t b[4] = {bounds[0], bounds[1], bounds[2], bounds[3]};
+    if (ymin == 0.0) b[2] = ZFV_WALL;   // Rectangle::set_axial (rectangle.cpp:97-103)
+    ctx->nseg = 1;
+    resize_host_mesh(ctx->hm, nx * ny);
+    build_rectangle(ctx->hm, 0, nx, ny, xmin, xmax, ymin, ymax, b, true);
+    ctx->rects.assign(1, zfv_ctx::RectInfo{0, nx, ny});
+    return upload_mesh(ctx);
+}
+
+int zfv_mesh_axial_download(zfv_ctx* ctx, double* vol_alt, double* area_alt) {
+    int r = check_ctx(ctx);
+    if (r) return r;
+    if (!ctx->have_mesh) return fail(ZFV_ERR_STATE, "no mesh uploaded");
+    if (!ctx->axial_mesh) return fail(ZFV_ERR_STATE, "the mesh has no axisymmetric geometry");
+    const HostMesh& m = ctx->hm;
+    if (vol_alt) std::memcpy(vol_alt, m.vol_alt.data(), (size_t)m.n * sizeof(double));
+    if (area_alt) std::memcpy(area_alt, m.area_alt.data(), 8 * (size_t)m.n * sizeof(double));
+    return ZFV_OK;
 }
 
 int zfv_batch_rectangles(zfv_ctx* ctx, int n_problems, const int32_t* nx, const int32_t* ny, const double* box,
@@ -1305,6 +1436,9 @@ int zfv_smf_configure(zfv_ctx* ctx, const zfv_smf_params* p) {
     if (p->limiter < ZFV_LIM_NONE || p->limiter > ZFV_LIM_SUPERBEE)
         return fail(ZFV_ERR_ARG, "unknown limiter id " + std::to_string(p->limiter));
     if (!(p->gamma > 1.0)) return fail(ZFV_ERR_ARG, "EOS gamma must be > 1");
+    if (p->axial != 0 && p->axial != 1) return fail(ZFV_ERR_ARG, "axial must be 0 or 1");
+    if (p->axial && ctx->have_mesh && !ctx->axial_mesh)
+        return fail(ZFV_ERR_STATE, "axisymmetric solver on a mesh without the axisymmetric geometry");
     ctx->prm = *p;
     ctx->prm.acc = std::min(std::max(1, p->acc), 2);               // set_accuracy
     ctx->prm.cfl = std::max(0.0, std::min(p->cfl, 1.0));            // set_CFL
@@ -1331,6 +1465,15 @@ int put_max_dt(zfv_ctx* ctx) {
 }  // namespace
 
 extern "C" {
+
+int zfv_smf_get_params(zfv_ctx* ctx, zfv_smf_params* out) {
+    int r = check_ctx(ctx);
+    if (r) return r;
+    if (!out) return fail(ZFV_ERR_ARG, "null output");
+    if (!ctx->configured) return fail(ZFV_ERR_STATE, "solver not configured");
+    *out = ctx->prm;
+    return ZFV_OK;
+}
 
 int zfv_smf_set_max_dt(zfv_ctx* ctx, double max_dt) {
     int r = check_ctx(ctx);
@@ -1675,7 +1818,7 @@ int zfv_conserved_totals(zfv_ctx* ctx, double* out4) {
     if (r) return r;
     if (!out4) return fail(ZFV_ERR_ARG, "null output");
     const int nb = (ctx->hm.n + kBlock - 1) / kBlock;
-    ZFV_HIP(launch_totals(ctx->dmesh(), ctx->layer(ctx->cur), ctx->partial.p, nb, ctx->stream));
+    ZFV_HIP(launch_totals(ctx->ax(ctx->dmesh()), ctx->layer(ctx->cur), ctx->partial.p, nb, ctx->stream));
     std::vector<double> part(4 * (size_t)nb);
     ZFV_HIP(hipMemcpyAsync(part.data(), ctx->partial.p, part.size() * 8, hipMemcpyDeviceToHost, ctx->stream));
     ZFV_HIP(hipStreamSynchronize(ctx->stream));
@@ -1698,7 +1841,7 @@ int zfv_conserved_totals_noalloc(zfv_ctx* ctx, double* out4) {
     if (!sp) return fail(ZFV_ERR_STATE, "no scratch address for this device");
     r = to_mesh_order(ctx);   // the partial sums run in the mesh's order
     if (r) return r;
-    ZFV_HIP(launch_totals(ctx->dmesh(), ctx->layer(ctx->cur), sp + 4, nb, ctx->stream));
+    ZFV_HIP(launch_totals(ctx->ax(ctx->dmesh()), ctx->layer(ctx->cur), sp + 4, nb, ctx->stream));
     ZFV_HIP(launch_totals_sum(sp + 4, nb, sp, ctx->stream));
     double res[4];
     ZFV_HIP(hipMemcpyAsync(res, sp, sizeof(res), hipMemcpyDeviceToHost, ctx->stream));
@@ -1797,6 +1940,7 @@ static int refine_impl(zfv_ctx* ctx, int max_level, int mode) {
     if (r) return r;
     if (max_level < 0 || max_level > 15) return fail(ZFV_ERR_ARG, "max_level out of [0, 15]");
     if (ctx->nseg > 1) return fail(ZFV_ERR_UNSUPPORTED, "adaptation of a batch of problems");
+    if (ctx->axial_mesh) return fail(ZFV_ERR_UNSUPPORTED, "adaptation of an axisymmetric mesh");
     if (mm && S.grad_epoch != ctx->mesh_epoch)
         return fail(ZFV_ERR_STATE, "multi-material refine needs the gradients of zfv_mm_set_flags on this mesh");
     hipStream_t s = ctx->stream;
@@ -1911,6 +2055,7 @@ static int adjoint_supported(zfv_ctx* ctx) {
     if (!(L == ZFV_LIM_NONE || L == ZFV_LIM_UNIT || L == ZFV_LIM_MINMOD || L == ZFV_LIM_MC || L == ZFV_LIM_SUPERBEE))
         return fail(ZFV_ERR_UNSUPPORTED, "adjoint: limiter must be none, unit, minmod, superbee or MC");
     if (ctx->nseg != 1) return fail(ZFV_ERR_UNSUPPORTED, "adjoint of a batch");
+    if (P.axial) return fail(ZFV_ERR_UNSUPPORTED, "adjoint of the axisymmetric step");
     int r = sync_host_mesh(ctx);
     if (r) return r;
     const HostMesh& m = ctx->hm;
@@ -2189,6 +2334,17 @@ int zfv_mm_set_splitting(zfv_ctx* ctx, int split) {
     return ZFV_OK;
 }
 
+int zfv_mm_set_overlap(zfv_ctx* ctx, int mode, int crp_groups) {
+    int r = mm_ready(ctx, false);
+    if (r) return r;
+    if (mode < 0 || mode > 2) return fail(ZFV_ERR_ARG, "mm_set_overlap: mode 0, 1 or 2");
+    if (crp_groups < 0 || crp_groups > 65536) return fail(ZFV_ERR_ARG, "mm_set_overlap: crp_groups in 0..65536");
+    ZFV_HIP(hipStreamSynchronize(ctx->stream));
+    ctx->mm.overlap = mode;
+    ctx->mm.crp_groups = crp_groups;
+    return ZFV_OK;
+}
+
 int zfv_mm_set_state(zfv_ctx* ctx, const double* rho, const double* u, const double* v, const double* P,
                      const double* beta) {
     int r = mm_ready(ctx, false);
@@ -2264,8 +2420,20 @@ int zfv_mm_step(zfv_ctx* ctx, int n_steps) {
         if (S.inter.n != 3 * nm * cap) ZFV_HIP(S.inter.alloc(3 * nm * cap));
         if (S.cf.n != 8 * (4 + nm) * cap) ZFV_HIP(S.cf.alloc(8 * (4 + nm) * cap));
         if (S.crp_list.n != 8 * cap) ZFV_HIP(S.crp_list.alloc(8 * cap));
-        if (S.crp_count.n != 1) ZFV_HIP(S.crp_count.alloc(1));
+        if (S.crp_count.n != 2) ZFV_HIP(S.crp_count.alloc(2));
+        if (S.overlap) {
+            if (S.overlap == 2 && S.crp_cells.n != cap) ZFV_HIP(S.crp_cells.alloc(cap));
+            if (!S.side) ZFV_HIP(hipStreamCreateWithFlags(&S.side, hipStreamNonBlocking));
+            for (hipEvent_t* e : {&S.ev_in, &S.ev_mark, &S.ev_crp})
+                if (!*e) ZFV_HIP(hipEventCreateWithFlags(e, hipEventDisableTiming));
+        }
     }
+    // 0: one stream; 1: the interface on the side stream (it reads only
+    // `in`) beside the gradient, predictor and marking; 2: also the
+    // composite fluxes there, beside the update of the cells without a
+    // composite face, and the other cells after them
+    const int mode = S.crp ? S.overlap : 0;
+    const int groups = S.crp_groups > 0 ? S.crp_groups : 2048;
     // MmFluid::update (mm_fluid.cpp:112-137): the sweeps of one step
     struct Sweep { int dir; double scale; };
     const Sweep none[] = {{0, 1.0}}, simple[] = {{1, 1.0}, {2, 1.0}}, strang[] = {{1, 0.5}, {2, 1.0}, {1, 0.5}};
@@ -2293,40 +2461,65 @@ int zfv_mm_step(zfv_ctx* ctx, int n_steps) {
             const MmLayer in = mm_layer(ctx, S.cur), out = mm_layer(ctx, 1 - S.cur);
             const MmLayer half = acc == 2 ? mm_rows(ctx, S.half.p) : in;
             const double* CF = nullptr;
+            double* ir = S.inter.p;
+            hipStream_t s1 = ctx->stream, s2 = mode ? S.side : ctx->stream;
+            // Every cell is updated by the same code whatever the mode.
+            if (mode) {
+                ZFV_HIP(hipEventRecord(S.ev_in, s1));   // `in` is complete
+                ZFV_HIP(hipStreamWaitEvent(s2, S.ev_in, 0));
+            }
+            if (plic) {   // interface_recovery (mm_fluid.cpp:143-146, 160-163)
+                ZFV_HIP(timed_on(ctx, s2, "mm_interface", [&] {
+                    return mm_interface(M, in, S.mix, ctx->vx.p, ctx->vy.p, ir, ir + nm * cap, ir + 2 * nm * cap, s2);
+                }));
+            }
             if (acc == 2) {   // compute_grad of init (mm_fluid.cpp:154)
                 ZFV_HIP(timed(ctx, "mm_grad", [&] {
-                    return mm_grad(M, in, S.mix, S.crp, S.limiter, 6 + 2 * (int)nm, S.gx.p, S.gy.p, ctx->stream);
+                    return mm_grad(M, in, S.mix, S.crp, S.limiter, 6 + 2 * (int)nm, S.gx.p, S.gy.p, s1);
                 }));
-            }
-            double* ir = S.inter.p;
-            if (plic) {   // interface_recovery (mm_fluid.cpp:143-146, 160-163)
-                ZFV_HIP(timed(ctx, "mm_interface", [&] {
-                    return mm_interface(M, in, S.mix, ctx->vx.p, ctx->vy.p, ir, ir + nm * cap, ir + 2 * nm * cap,
-                                        ctx->stream);
-                }));
-            }
-            if (acc == 2) {   // fluxes_stage1 (:165)
+                // fluxes_stage1 (:165)
                 ZFV_HIP(timed(ctx, "mm_stage1", [&] {
-                    return mm_stage1(M, in, half, S.mix, c, o, S.gx.p, S.gy.p, ctx->stream);
+                    return mm_stage1(M, in, half, S.mix, c, o, S.gx.p, S.gy.p, s1);
                 }));
             }
             if (S.crp) {   // the composite fluxes of the sweep's faces
                 ZFV_HIP(timed(ctx, "mm_crp_mark", [&] {
-                    return mm_crp_mark(M, in, half, S.mix, acc, o, S.gx.p, S.gy.p, S.crp_list.p, S.crp_count.p,
-                                       ctx->stream);
+                    return mm_crp_mark(M, in, half, S.mix, acc, o, S.gx.p, S.gy.p, S.crp_list.p,
+                                       mode == 2 ? S.crp_cells.p : nullptr, S.crp_count.p, s1);
                 }));
-                ZFV_HIP(timed(ctx, "mm_crp", [&] {
+                // the composite fluxes run where the interface ran (s2),
+                // after the marking; mode 1 joins before them
+                hipStream_t sc = mode == 2 ? s2 : s1;
+                if (mode == 2) {   // s2 waits for the list
+                    ZFV_HIP(hipEventRecord(S.ev_mark, s1));
+                    ZFV_HIP(hipStreamWaitEvent(s2, S.ev_mark, 0));
+                } else if (mode == 1) {   // s1 waits for the interface
+                    ZFV_HIP(hipEventRecord(S.ev_crp, s2));
+                    ZFV_HIP(hipStreamWaitEvent(s1, S.ev_crp, 0));
+                }
+                ZFV_HIP(timed_on(ctx, sc, "mm_crp", [&] {
                     return mm_crp(M, in, half, S.mix, acc, o, S.gx.p, S.gy.p, c, ir, ir + nm * cap, S.crp_list.p,
-                                  S.crp_count.p, S.cf.p, ctx->stream);
+                                  S.crp_count.p, S.cf.p, groups, sc);
                 }));
+                if (mode == 2) ZFV_HIP(hipEventRecord(S.ev_crp, s2));
                 CF = S.cf.p;
             }
-            if (acc == 2)   // fluxes_stage2 (:167)
-                ZFV_HIP(timed(ctx, "mm_flux2", [&] {
-                    return mm_flux2(M, in, half, out, S.mix, c, o, S.gx.p, S.gy.p, CF, ctx->stream);
-                }));
-            else
-                ZFV_HIP(timed(ctx, "mm_flux1", [&] { return mm_flux1(M, in, out, S.mix, c, o, CF, ctx->stream); }));
+            // fluxes_stage2 (:167) or fluxes (:148): every cell, or (mode
+            // 2) the cells without a composite face, then, after the join,
+            // the others
+            const char* name = acc == 2 ? "mm_flux2" : "mm_flux1";
+            auto update = [&](int pass) {
+                return timed(ctx, pass == 2 ? (acc == 2 ? "mm_flux2_list" : "mm_flux1_list") : name, [&] {
+                    return acc == 2 ? mm_flux2(M, in, half, out, S.mix, c, o, S.gx.p, S.gy.p, CF, pass,
+                                               S.crp_cells.p, S.crp_count.p, s1)
+                                    : mm_flux1(M, in, out, S.mix, c, o, CF, pass, S.crp_cells.p, S.crp_count.p, s1);
+                });
+            };
+            ZFV_HIP(update(mode == 2 ? 1 : 0));
+            if (mode == 2) {
+                ZFV_HIP(hipStreamWaitEvent(s1, S.ev_crp, 0));
+                ZFV_HIP(update(2));
+            }
             S.cur = 1 - S.cur;   // MmFluid::swap, after every integrate
         }
         S.step_no += 1;

@@ -31,6 +31,9 @@ struct DevMesh {
     const double* __restrict__ w;       // [8][stride] LSM weight area / |drn|^3
     const double* __restrict__ nx;      // [8][stride] outward normal
     const double* __restrict__ ny;
+    // Axisymmetric step (SmFluid::set_axial): volume / volume_alt per cell,
+    // with vol and area then volume_alt and area_alt; null for a planar step.
+    const double* __restrict__ axc;
     // Batches of independent problems (null for one problem): problem of each
     // 64-cell chunk, first cell and end of each problem's cells.  Problems
     // start on 512-cell boundaries.
@@ -253,25 +256,32 @@ struct MmOpts {
 hipError_t mm_prepare(const DevMesh& M, const MmLayer& L, const mm::Mix& mx, unsigned long long* slot,
                       hipStream_t s);
 // CF: composite Riemann fluxes per [slot][4 + n_mat][stride] from mm_crp,
-// or null without composite Riemann problems.
+// or null without composite Riemann problems.  pass 0: every cell; pass 1:
+// the cells without a composite face (CF is not read: mm_crp may still be
+// writing it on another stream); pass 2: the cells listed by mm_crp_mark
+// (cells, count[1]), which pass 1 left.
 hipError_t mm_flux1(const DevMesh& M, const MmLayer& in, const MmLayer& out, const mm::Mix& mx,
-                    const StepCtl& ctl, const MmOpts& o, const double* CF, hipStream_t s);
+                    const StepCtl& ctl, const MmOpts& o, const double* CF, int pass, const int* cells,
+                    const unsigned int* count, hipStream_t s);
 // The faces that take composite Riemann fluxes: list (8 * stride ints of
-// work space) and their number (one counter), then the fluxes into CF.
-// acc 1 reads the cells of `in`; acc 2 the face states extrapolated from
-// `half` with the gradients gx / gy ([6 + 2 n_mat][stride], k_mm_grad).
+// work space) and their number (count[0]), with `cells` (stride ints, or
+// null) the cells that have such a face and their number (count[1]); then
+// the fluxes into CF, over `groups` one-wavefront workgroups.  acc 1 reads
+// the cells of `in`; acc 2 the face states extrapolated from `half` with
+// the gradients gx / gy ([6 + 2 n_mat][stride], k_mm_grad).
 hipError_t mm_crp_mark(const DevMesh& M, const MmLayer& in, const MmLayer& half, const mm::Mix& mx, int acc,
-                       const MmOpts& o, const double* gx, const double* gy, int* list, unsigned int* count,
-                       hipStream_t s);
+                       const MmOpts& o, const double* gx, const double* gy, int* list, int* cells,
+                       unsigned int* count, hipStream_t s);
 hipError_t mm_crp(const DevMesh& M, const MmLayer& in, const MmLayer& half, const mm::Mix& mx, int acc,
                   const MmOpts& o, const double* gx, const double* gy, const StepCtl& ctl, const double* nx,
-                  const double* ny, const int* list, const unsigned int* count, double* CF, hipStream_t s);
+                  const double* ny, const int* list, const unsigned int* count, double* CF, int groups,
+                  hipStream_t s);
 // Second order: the predictor into `half`, then the corrector into `out`.
 hipError_t mm_stage1(const DevMesh& M, const MmLayer& in, const MmLayer& half, const mm::Mix& mx, const StepCtl& ctl,
                      const MmOpts& o, const double* gx, const double* gy, hipStream_t s);
 hipError_t mm_flux2(const DevMesh& M, const MmLayer& in, const MmLayer& half, const MmLayer& out, const mm::Mix& mx,
                     const StepCtl& ctl, const MmOpts& o, const double* gx, const double* gy, const double* CF,
-                    hipStream_t s);
+                    int pass, const int* cells, const unsigned int* count, hipStream_t s);
 hipError_t mm_interface(const DevMesh& M, const MmLayer& L, const mm::Mix& mx, const double* vx, const double* vy,
                         double* nx, double* ny, double* p, hipStream_t s);
 // Adaptation of the multi-material state: the limited gradient (rows rho,

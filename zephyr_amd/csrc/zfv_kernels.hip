@@ -121,6 +121,7 @@ k_flux1(DevMesh M, DevState in, DevState out, Eos eos, StepCtl ctl) {
         q.mx = q.mx - coef * F1;
         q.my = q.my - coef * F2;
         q.E = q.E - coef * F3;
+        axial_source<false>(M, c, zc.p, dt, q);
         const Z zn = to_z(q, eos);
         store_z(out, c, zn);
         if (bad) *ctl.err = 1;
@@ -284,6 +285,7 @@ k_grad_pred(DevMesh M, DevState in, DevGrad G, DevState half,
         q.mx = q.mx - coef * F1;
         q.my = q.my - coef * F2;
         q.E = q.E - coef * F3;
+        axial_source<true>(M, c, zc.p, dt, q);
         zh = to_z(q, eos);
         if (is_bad(zh, eos)) zh = zc;
     }
@@ -387,6 +389,7 @@ k_corr(DevMesh M, DevState init, DevState half, DevGrad G, const int* __restrict
         q.mx = q.mx - coef * F1;
         q.my = q.my - coef * F2;
         q.E = q.E - coef * F3;
+        axial_source<false>(M, c, zch.p, dt, q);
         Z zn = to_z(q, eos);
         int wn;
         if (is_bad(zn, eos)) {
@@ -535,6 +538,7 @@ k_flux1_q(DevMesh M, DevState in, DevState out, Eos eos, StepCtl ctl) {
         q.mx = q.mx - coef * F1;
         q.my = q.my - coef * F2;
         q.E = q.E - coef * F3;
+        axial_source<false>(M, c, zc.p, dt, q);
         const Z zn = to_z(q, eos);
         store_z_quad(out, c, side, zn);
         if (bad) *ctl.err = 1;
@@ -697,6 +701,7 @@ __device__ __forceinline__ void grad_pred_quad(const DevMesh& M, const DevState&
         q.mx = q.mx - coef * F1;
         q.my = q.my - coef * F2;
         q.E = q.E - coef * F3;
+        axial_source<true>(M, c, zc.p, dt, q);
         zh = to_z(q, eos);
         if (is_bad(zh, eos)) zh = zc;
     }
@@ -765,6 +770,7 @@ k_corr_q(DevMesh M, DevState init, DevState half, DevGrad G, const int* __restri
         q.mx = q.mx - coef * F1;
         q.my = q.my - coef * F2;
         q.E = q.E - coef * F3;
+        axial_source<false>(M, c, zch.p, dt, q);
         Z zn = to_z(q, eos);
         int wn;
         if (is_bad(zn, eos)) {

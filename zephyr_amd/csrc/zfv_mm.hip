@@ -251,21 +251,29 @@ __device__ __forceinline__ bool crp_face(const double* CF, const MZ<NM>& zm, con
 // state on either side (MmFluid::calc_flux, mm_fluid.cpp:347-371; a
 // boundary face's other side is the boundary value of the cell's own).  acc
 // 1: the cells' states; acc 2: the extrapolated, normalised half states.
-// Appends slot-major pairs s * stride + c to `list`; `count` starts at zero.
+// Appends slot-major pairs s * stride + c to `list` (counter count[0]) and,
+// when `cells` is given, each cell with such a face to `cells` (counter
+// count[1]); the counters start at zero.
 template <int NM, int ACC>
 __global__ void __launch_bounds__(kBlock) k_mm_crp_mark(DevMesh M, MmLayer in, MmLayer half,
                                                         const double* __restrict__ gx, const double* __restrict__ gy,
-                                                        int dir, int* __restrict__ list, unsigned int* count) {
+                                                        int dir, int* __restrict__ list, int* __restrict__ cells,
+                                                        unsigned int* count) {
     const Tile tl = tile_of(M, (int)blockDim.x);
     const int c = tl.c0 + (int)threadIdx.x;
     if (c >= tl.end) return;
     const int st = M.stride;
     const int code = M.code[c];
+    bool any = false;
     for (int s = 0; s < 8; ++s) {
         const int t = (code >> (4 * s)) & 7;
         if (t == SC_UNDEF || !mm_swept(M, s * st + c, dir)) continue;
-        if (mm_crp_slot<NM, ACC>(M, in, half, gx, gy, c, s * st + c, t)) list[atomicAdd(count, 1u)] = s * st + c;
+        if (mm_crp_slot<NM, ACC>(M, in, half, gx, gy, c, s * st + c, t)) {
+            list[atomicAdd(count, 1u)] = s * st + c;
+            any = true;
+        }
     }
+    if (any && cells) cells[atomicAdd(count + 1, 1u)] = c;
 }
 
 // The composite Riemann flux of each listed face, one face per thread: the
@@ -339,77 +347,104 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 1)))
     if (bad) atomicOr(ctl.err, 1);
 }
 
-// MmFluid::fluxes (mm_fluid.cpp:373-445): per face the mmf HLLC flux of the
-// face-frame states (own, neighbour or boundary value, :98-108) or the
-// composite flux from CF, q -= dt / V sum F S, the new state by the PT
-// closure from the cell's previous P, T and true densities
-// (models.cpp:187-201), its sound speed and the next step's CFL candidate.
-// PLAIN: 1 = the plain faces take HLLC (the default, compiled alone), 2 =
-// the flux of opt.flux.
+// MmFluid::fluxes (mm_fluid.cpp:373-445) of cell c: per face the mmf HLLC
+// flux of the face-frame states (own, neighbour or boundary value, :98-108)
+// or the composite flux from CF, q -= dt / V sum F S, the new state by the
+// PT closure from the cell's previous P, T and true densities
+// (models.cpp:187-201) and its sound speed; returns the next step's CFL
+// candidate.  With `defer` a cell with a composite face is left alone
+// (+inf, nothing stored, CF not read): k_mm_flux1_list updates it once CF
+// is complete.  PLAIN: 1 = the plain faces take HLLC (the default, compiled
+// alone), 2 = the flux of opt.flux.
+template <int NM, int PLAIN>
+__device__ __forceinline__ double mm_flux1_cell(const DevMesh& M, const MmLayer& in, const MmLayer& out,
+                                                const Mix& mx, const StepCtl& ctl, const MmOpts& opt,
+                                                const double* __restrict__ CF, int c, double dt, bool defer) {
+    const int st = M.stride;
+    const MZ<NM> zc = load_mz<NM>(in, c, st);
+    const int code = M.code[c];
+    double acc[4 + NM];
+#pragma unroll
+    for (int k = 0; k < 4 + NM; ++k) acc[k] = 0.0;
+    bool bad = false;
+#pragma unroll 1
+    for (int s = 0; s < 8; ++s) {
+        const int nib = (code >> (4 * s)) & 15;
+        const int t = nib & 7;
+        if (t == SC_UNDEF || !mm_swept(M, s * st + c, opt.dir)) continue;
+        const int so = s * st + c;
+        const Frame fr = slot_frame(M, c, s, nib);
+        MZ<NM> zp;
+        if (t <= SC_PERIODIC) {
+            zp = load_mz<NM>(in, M.adj[so], st);
+        } else {
+            zp = zc;
+            const Z w = frame_boundary(fr, Z{zc.r, zc.u, zc.v, zc.P, zc.e}, t == SC_WALL);
+            zp.u = w.u;
+            zp.v = w.v;
+        }
+        MZ<NM> zm = zc;
+        rotate_uv<NM>(fr, zm);
+        rotate_uv<NM>(fr, zp);
+        double F[4 + NM];
+        if (crp_face<NM>(CF, zm, zp)) {
+            if (defer) return INFINITY;
+#pragma unroll
+            for (int k = 0; k < 4 + NM; ++k) F[k] = CF[(s * (4 + NM) + k) * st + c];
+        } else {
+            if (PLAIN == 1) hllc_mm<NM>(zm, zp, F, bad);
+            else plain_flux_mm<NM>(opt.flux, zm, zp, F, bad);
+            const Q g = frame_global(fr, Q{F[0], F[1], F[2], F[3]});
+            F[1] = g.mx;
+            F[2] = g.my;
+        }
+        const double S = M.area[so];
+#pragma unroll
+        for (int k = 0; k < 4 + NM; ++k) acc[k] = acc[k] + F[k] * S;
+    }
+    double q[4 + NM];
+    m_qstate<NM>(zc, q);
+    const double coef = dt / M.vol[c];
+#pragma unroll
+    for (int k = 0; k < 4 + NM; ++k) q[k] = q[k] - coef * acc[k];
+    if (q[0] < 0.0) bad = true;   // mm_fluid.cpp:440-442 throws
+    MZ<NM> zn = mm_from_q<NM>(mx, q, zc);
+    zn.c = sound_speed<NM>(mx, zn.r, zn.P, zn.b, zn.T, zn.rh);
+    store_mz<NM>(out, c, st, zn);
+    if (bad) atomicOr(ctl.err, 1);
+    return mm_dt_cand<NM>(M, c, zn);
+}
+
+// Every cell (defer = false), or every cell without a composite face
+// (defer = true, while k_mm_crp fills CF on another stream).
 template <int NM, int PLAIN>
 __global__ void __launch_bounds__(kBlock) k_mm_flux1(DevMesh M, MmLayer in, MmLayer out, Mix mx, StepCtl ctl,
-                                                     MmOpts opt, const double* __restrict__ CF) {
+                                                     MmOpts opt, const double* __restrict__ CF, int defer) {
     const Tile tl = tile_of(M, (int)blockDim.x);
     const int c = tl.c0 + (int)threadIdx.x;
     const double step = step_dt(ctl, tl.pid);
     step_prologue(ctl, tl, step);
     const double dt = opt.dt_scale * step;
     double cand = INFINITY;
-    if (c < tl.end) {
-        const int st = M.stride;
-        const MZ<NM> zc = load_mz<NM>(in, c, st);
-        const int code = M.code[c];
-        double acc[4 + NM];
-#pragma unroll
-        for (int k = 0; k < 4 + NM; ++k) acc[k] = 0.0;
-        bool bad = false;
-#pragma unroll 1
-        for (int s = 0; s < 8; ++s) {
-            const int nib = (code >> (4 * s)) & 15;
-            const int t = nib & 7;
-            if (t == SC_UNDEF || !mm_swept(M, s * st + c, opt.dir)) continue;
-            const int so = s * st + c;
-            const Frame fr = slot_frame(M, c, s, nib);
-            MZ<NM> zp;
-            if (t <= SC_PERIODIC) {
-                zp = load_mz<NM>(in, M.adj[so], st);
-            } else {
-                zp = zc;
-                const Z w = frame_boundary(fr, Z{zc.r, zc.u, zc.v, zc.P, zc.e}, t == SC_WALL);
-                zp.u = w.u;
-                zp.v = w.v;
-            }
-            MZ<NM> zm = zc;
-            rotate_uv<NM>(fr, zm);
-            rotate_uv<NM>(fr, zp);
-            double F[4 + NM];
-            if (crp_face<NM>(CF, zm, zp)) {
-#pragma unroll
-                for (int k = 0; k < 4 + NM; ++k) F[k] = CF[(s * (4 + NM) + k) * st + c];
-            } else {
-                if (PLAIN == 1) hllc_mm<NM>(zm, zp, F, bad);
-                else plain_flux_mm<NM>(opt.flux, zm, zp, F, bad);
-                const Q g = frame_global(fr, Q{F[0], F[1], F[2], F[3]});
-                F[1] = g.mx;
-                F[2] = g.my;
-            }
-            const double S = M.area[so];
-#pragma unroll
-            for (int k = 0; k < 4 + NM; ++k) acc[k] = acc[k] + F[k] * S;
-        }
-        double q[4 + NM];
-        m_qstate<NM>(zc, q);
-        const double coef = dt / M.vol[c];
-#pragma unroll
-        for (int k = 0; k < 4 + NM; ++k) q[k] = q[k] - coef * acc[k];
-        if (q[0] < 0.0) bad = true;   // mm_fluid.cpp:440-442 throws
-        MZ<NM> zn = mm_from_q<NM>(mx, q, zc);
-        zn.c = sound_speed<NM>(mx, zn.r, zn.P, zn.b, zn.T, zn.rh);
-        store_mz<NM>(out, c, st, zn);
-        if (bad) atomicOr(ctl.err, 1);
-        cand = mm_dt_cand<NM>(M, c, zn);
-    }
+    if (c < tl.end) cand = mm_flux1_cell<NM, PLAIN>(M, in, out, mx, ctl, opt, CF, c, dt, defer != 0);
     block_min_atomic(&ctl.dtmin[3 * tl.pid + ctl.wr], cand);
+}
+
+// The cells k_mm_flux1 deferred: cells[0 .. count[1]) from k_mm_crp_mark,
+// grid-stride.  The multi-material solver runs one problem (dt slot 0).
+template <int NM, int PLAIN>
+__global__ void __launch_bounds__(kBlock) k_mm_flux1_list(DevMesh M, MmLayer in, MmLayer out, Mix mx, StepCtl ctl,
+                                                          MmOpts opt, const double* __restrict__ CF,
+                                                          const int* __restrict__ cells,
+                                                          const unsigned int* __restrict__ count) {
+    const double dt = opt.dt_scale * step_dt(ctl, 0);
+    const int n = (int)count[1];
+    double cand = INFINITY;
+    for (int e = (int)(blockIdx.x * blockDim.x + threadIdx.x); e < n; e += (int)(gridDim.x * blockDim.x)) {
+        const double v = mm_flux1_cell<NM, PLAIN>(M, in, out, mx, ctl, opt, CF, cells[e], dt, false);
+        cand = v < cand ? v : cand;   // a NaN candidate never wins, as in block_min_atomic
+    }
+    block_min_atomic(&ctl.dtmin[ctl.wr], cand);
 }
 
 // MmFluid::fluxes_stage1 (mm_fluid.cpp:521-578): a mixed cell keeps its
@@ -460,68 +495,95 @@ __global__ void __launch_bounds__(kBlock) k_mm_stage1(DevMesh M, MmLayer in, MmL
     store_mz<NM>(half, c, st, mm_from_q<NM>(mx, q, zc));
 }
 
-// MmFluid::fluxes_stage2 (mm_fluid.cpp:580-680): per face the states
-// extrapolated from the half layer with the gradients of the old one and
-// re-closed, the mmf HLLC flux or the composite flux from CF, q -= dt / V
-// sum F S on the old conserved vector, the closure from the old P, T and
-// true densities, the sound speed and the next step's CFL candidate.
+// MmFluid::fluxes_stage2 (mm_fluid.cpp:580-680) of cell c: per face the
+// states extrapolated from the half layer with the gradients of the old one
+// and re-closed, the mmf HLLC flux or the composite flux from CF, q -= dt /
+// V sum F S on the old conserved vector, the closure from the old P, T and
+// true densities and the sound speed; returns the next step's CFL
+// candidate.  `defer` as in mm_flux1_cell.
+template <int NM, int PLAIN>
+__device__ __forceinline__ double mm_flux2_cell(const DevMesh& M, const MmLayer& in, const MmLayer& half,
+                                                const MmLayer& out, const Mix& mx, const StepCtl& ctl,
+                                                const MmOpts& opt, const double* __restrict__ gx,
+                                                const double* __restrict__ gy, const double* __restrict__ CF,
+                                                int c, double dt, bool defer) {
+    const int st = M.stride;
+    const int code = M.code[c];
+    double acc[4 + NM];
+#pragma unroll
+    for (int k = 0; k < 4 + NM; ++k) acc[k] = 0.0;
+    bool bad = false;
+#pragma unroll 1
+    for (int s = 0; s < 8; ++s) {
+        const int nib = (code >> (4 * s)) & 15;
+        const int t = nib & 7;
+        if (t == SC_UNDEF || !mm_swept(M, s * st + c, opt.dir)) continue;
+        const int so = s * st + c;
+        double F[4 + NM];
+        if (CF != nullptr && mm_crp_slot<NM, 2>(M, in, half, gx, gy, c, so, t)) {
+            if (defer) return INFINITY;
+#pragma unroll
+            for (int k = 0; k < 4 + NM; ++k) F[k] = CF[(s * (4 + NM) + k) * st + c];
+        } else {
+            const Frame fr = slot_frame(M, c, s, nib);
+            MZ<NM> zm, zp;
+            mm_face_pair<NM, 2>(M, in, half, mx, gx, gy, c, so, t, fr, zm, zp);
+            rotate_uv<NM>(fr, zm);
+            rotate_uv<NM>(fr, zp);
+            if (PLAIN == 1) hllc_mm<NM>(zm, zp, F, bad);
+            else plain_flux_mm<NM>(opt.flux, zm, zp, F, bad);
+            const Q g = frame_global(fr, Q{F[0], F[1], F[2], F[3]});
+            F[1] = g.mx;
+            F[2] = g.my;
+        }
+        const double S = M.area[so];
+#pragma unroll
+        for (int k = 0; k < 4 + NM; ++k) acc[k] = acc[k] + F[k] * S;
+    }
+    const MZ<NM> zc = load_mz<NM>(in, c, st);
+    double q[4 + NM];
+    m_qstate<NM>(zc, q);
+    const double coef = dt / M.vol[c];
+#pragma unroll
+    for (int k = 0; k < 4 + NM; ++k) q[k] = q[k] - coef * acc[k];
+    if (q[0] < 0.0) bad = true;   // mm_fluid.cpp:671-673 throws
+    MZ<NM> zn = mm_from_q<NM>(mx, q, zc);
+    zn.c = sound_speed<NM>(mx, zn.r, zn.P, zn.b, zn.T, zn.rh);
+    store_mz<NM>(out, c, st, zn);
+    if (bad) atomicOr(ctl.err, 1);
+    return mm_dt_cand<NM>(M, c, zn);
+}
+
 template <int NM, int PLAIN>
 __global__ void __launch_bounds__(kBlock) k_mm_flux2(DevMesh M, MmLayer in, MmLayer half, MmLayer out, Mix mx,
                                                      StepCtl ctl, MmOpts opt, const double* __restrict__ gx,
-                                                     const double* __restrict__ gy, const double* __restrict__ CF) {
+                                                     const double* __restrict__ gy, const double* __restrict__ CF,
+                                                     int defer) {
     const Tile tl = tile_of(M, (int)blockDim.x);
     const int c = tl.c0 + (int)threadIdx.x;
     const double step = step_dt(ctl, tl.pid);
     step_prologue(ctl, tl, step);
     const double dt = opt.dt_scale * step;
     double cand = INFINITY;
-    if (c < tl.end) {
-        const int st = M.stride;
-        const int code = M.code[c];
-        double acc[4 + NM];
-#pragma unroll
-        for (int k = 0; k < 4 + NM; ++k) acc[k] = 0.0;
-        bool bad = false;
-#pragma unroll 1
-        for (int s = 0; s < 8; ++s) {
-            const int nib = (code >> (4 * s)) & 15;
-            const int t = nib & 7;
-            if (t == SC_UNDEF || !mm_swept(M, s * st + c, opt.dir)) continue;
-            const int so = s * st + c;
-            double F[4 + NM];
-            if (CF != nullptr && mm_crp_slot<NM, 2>(M, in, half, gx, gy, c, so, t)) {
-#pragma unroll
-                for (int k = 0; k < 4 + NM; ++k) F[k] = CF[(s * (4 + NM) + k) * st + c];
-            } else {
-                const Frame fr = slot_frame(M, c, s, nib);
-                MZ<NM> zm, zp;
-                mm_face_pair<NM, 2>(M, in, half, mx, gx, gy, c, so, t, fr, zm, zp);
-                rotate_uv<NM>(fr, zm);
-                rotate_uv<NM>(fr, zp);
-                if (PLAIN == 1) hllc_mm<NM>(zm, zp, F, bad);
-                else plain_flux_mm<NM>(opt.flux, zm, zp, F, bad);
-                const Q g = frame_global(fr, Q{F[0], F[1], F[2], F[3]});
-                F[1] = g.mx;
-                F[2] = g.my;
-            }
-            const double S = M.area[so];
-#pragma unroll
-            for (int k = 0; k < 4 + NM; ++k) acc[k] = acc[k] + F[k] * S;
-        }
-        const MZ<NM> zc = load_mz<NM>(in, c, st);
-        double q[4 + NM];
-        m_qstate<NM>(zc, q);
-        const double coef = dt / M.vol[c];
-#pragma unroll
-        for (int k = 0; k < 4 + NM; ++k) q[k] = q[k] - coef * acc[k];
-        if (q[0] < 0.0) bad = true;   // mm_fluid.cpp:671-673 throws
-        MZ<NM> zn = mm_from_q<NM>(mx, q, zc);
-        zn.c = sound_speed<NM>(mx, zn.r, zn.P, zn.b, zn.T, zn.rh);
-        store_mz<NM>(out, c, st, zn);
-        if (bad) atomicOr(ctl.err, 1);
-        cand = mm_dt_cand<NM>(M, c, zn);
-    }
+    if (c < tl.end) cand = mm_flux2_cell<NM, PLAIN>(M, in, half, out, mx, ctl, opt, gx, gy, CF, c, dt, defer != 0);
     block_min_atomic(&ctl.dtmin[3 * tl.pid + ctl.wr], cand);
+}
+
+template <int NM, int PLAIN>
+__global__ void __launch_bounds__(kBlock) k_mm_flux2_list(DevMesh M, MmLayer in, MmLayer half, MmLayer out, Mix mx,
+                                                          StepCtl ctl, MmOpts opt, const double* __restrict__ gx,
+                                                          const double* __restrict__ gy,
+                                                          const double* __restrict__ CF,
+                                                          const int* __restrict__ cells,
+                                                          const unsigned int* __restrict__ count) {
+    const double dt = opt.dt_scale * step_dt(ctl, 0);
+    const int n = (int)count[1];
+    double cand = INFINITY;
+    for (int e = (int)(blockIdx.x * blockDim.x + threadIdx.x); e < n; e += (int)(gridDim.x * blockDim.x)) {
+        const double v = mm_flux2_cell<NM, PLAIN>(M, in, half, out, mx, ctl, opt, gx, gy, CF, cells[e], dt, false);
+        cand = v < cand ? v : cand;
+    }
+    block_min_atomic(&ctl.dtmin[ctl.wr], cand);
 }
 
 // MmFluid::interface_recovery (mm_fluid.cpp:484-519): per mixed cell and
@@ -949,6 +1011,8 @@ __global__ void __launch_bounds__(kBlock) k_mm_cache(DevMesh M, MmLayer L, Mix m
 
 // ---- launchers --------------------------------------------------------------
 static inline dim3 mm_grid(int n) { return dim3((n + kBlock - 1) / kBlock); }
+// workgroups of the deferred-cell passes (a few thousand cells at most, grid-stride)
+constexpr int kListGroups = 64;
 
 #define ZFV_MM_DISPATCH(NMAT, KERNEL, ...)                                                              \
     switch (NMAT) {                                                                                     \
@@ -978,31 +1042,39 @@ hipError_t mm_prepare(const DevMesh& M, const MmLayer& L, const Mix& mx, unsigne
 }
 
 hipError_t mm_flux1(const DevMesh& M, const MmLayer& in, const MmLayer& out, const Mix& mx, const StepCtl& ctl,
-                    const MmOpts& o, const double* CF, hipStream_t s) {
-    ZFV_MM_DISPATCH_ACC(mx.n, o.flux == kFluxHllc ? 1 : 2, k_mm_flux1, mm_grid(M.n), dim3(kBlock), 0, s, M, in, out,
-                        mx, ctl, o, CF);
+                    const MmOpts& o, const double* CF, int pass, const int* cells, const unsigned int* count,
+                    hipStream_t s) {
+    const int plain = o.flux == kFluxHllc ? 1 : 2;
+    if (pass == 2) {
+        ZFV_MM_DISPATCH_ACC(mx.n, plain, k_mm_flux1_list, dim3(kListGroups), dim3(kBlock), 0, s, M, in, out, mx, ctl,
+                            o, CF, cells, count);
+    } else {
+        ZFV_MM_DISPATCH_ACC(mx.n, plain, k_mm_flux1, mm_grid(M.n), dim3(kBlock), 0, s, M, in, out, mx, ctl, o, CF,
+                            pass);
+    }
     return hipGetLastError();
 }
 
 __global__ void __launch_bounds__(64) k_zero_u32(unsigned int* p) {
-    if (threadIdx.x == 0) *p = 0u;
+    if (threadIdx.x < 2) p[threadIdx.x] = 0u;
 }
 
 hipError_t mm_crp_mark(const DevMesh& M, const MmLayer& in, const MmLayer& half, const Mix& mx, int acc,
-                       const MmOpts& o, const double* gx, const double* gy, int* list, unsigned int* count,
-                       hipStream_t s) {
-    // the counter is cleared by a kernel, not a memset node, so the step can be captured
+                       const MmOpts& o, const double* gx, const double* gy, int* list, int* cells,
+                       unsigned int* count, hipStream_t s) {
+    // the counters are cleared by a kernel, not a memset node, so the step can be captured
     hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, s, count);
     ZFV_MM_DISPATCH_ACC(mx.n, acc, k_mm_crp_mark, mm_grid(M.n), dim3(kBlock), 0, s, M, in, half, gx, gy, o.dir,
-                        list, count);
+                        list, cells, count);
     return hipGetLastError();
 }
 
 hipError_t mm_crp(const DevMesh& M, const MmLayer& in, const MmLayer& half, const Mix& mx, int acc,
                   const MmOpts& o, const double* gx, const double* gy, const StepCtl& ctl, const double* nx,
-                  const double* ny, const int* list, const unsigned int* count, double* CF, hipStream_t s) {
-    // 2048 one-wavefront workgroups, grid-stride over any list length
-    ZFV_MM_DISPATCH_ACC(mx.n, acc, k_mm_crp, dim3(2048), dim3(64), 0, s, M, in, half, gx, gy, mx, ctl, o, nx, ny,
+                  const double* ny, const int* list, const unsigned int* count, double* CF, int groups,
+                  hipStream_t s) {
+    // `groups` one-wavefront workgroups, grid-stride over any list length
+    ZFV_MM_DISPATCH_ACC(mx.n, acc, k_mm_crp, dim3(groups), dim3(64), 0, s, M, in, half, gx, gy, mx, ctl, o, nx, ny,
                         list, count, CF);
     return hipGetLastError();
 }
@@ -1015,9 +1087,15 @@ hipError_t mm_stage1(const DevMesh& M, const MmLayer& in, const MmLayer& half, c
 
 hipError_t mm_flux2(const DevMesh& M, const MmLayer& in, const MmLayer& half, const MmLayer& out, const Mix& mx,
                     const StepCtl& ctl, const MmOpts& o, const double* gx, const double* gy, const double* CF,
-                    hipStream_t s) {
-    ZFV_MM_DISPATCH_ACC(mx.n, o.flux == kFluxHllc ? 1 : 2, k_mm_flux2, mm_grid(M.n), dim3(kBlock), 0, s, M, in, half,
-                        out, mx, ctl, o, gx, gy, CF);
+                    int pass, const int* cells, const unsigned int* count, hipStream_t s) {
+    const int plain = o.flux == kFluxHllc ? 1 : 2;
+    if (pass == 2) {
+        ZFV_MM_DISPATCH_ACC(mx.n, plain, k_mm_flux2_list, dim3(kListGroups), dim3(kBlock), 0, s, M, in, half, out,
+                            mx, ctl, o, gx, gy, CF, cells, count);
+    } else {
+        ZFV_MM_DISPATCH_ACC(mx.n, plain, k_mm_flux2, mm_grid(M.n), dim3(kBlock), 0, s, M, in, half, out, mx, ctl, o,
+                            gx, gy, CF, pass);
+    }
     return hipGetLastError();
 }
 

@@ -66,6 +66,16 @@ __device__ __forceinline__ double step_dt(const StepCtl& ctl, int pid) {
     return smin(ctl.cfl * dmin, *ctl.max_dt);
 }
 
+// The axisymmetric pressure source of SmFluid: momentum.y += k coeff p dt
+// with coeff = volume / volume_alt (M.axc; nothing for a planar step) and
+// k = 0.5 in the predictor (sm_fluid.cpp:166-169, 242-245, 347-350).
+template <bool HALF>
+__device__ __forceinline__ void axial_source(const DevMesh& M, int c, double p, double dt, Q& q) {
+    if (M.axc == nullptr) return;
+    const double coeff = M.axc[c];
+    q.my = q.my + (HALF ? 0.5 * coeff : coeff) * p * dt;
+}
+
 __device__ __forceinline__ void step_prologue(const StepCtl& ctl, const Tile& t, double dt) {
     if (t.first && threadIdx.x == 0) {
         ctl.dt_out[t.pid] = dt;
