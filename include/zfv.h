@@ -268,7 +268,8 @@ int zfv_mm_set_crp(zfv_ctx* ctx, int mode);
 /* MmFluid::set_method (mm_fluid.cpp:50-61): ZFV_CRP = CrpMode::PLIC with
  * HLLC; ZFV_HLLC, ZFV_HLL or ZFV_RUSANOV = CrpMode::NONE with that mmf flux
  * (hllc.cpp:339-421, hll.cpp:125-160, rusanov.cpp:46-83).  The mmf Godunov
- * flux and the other Fluxes are not built (ZFV_ERR_UNSUPPORTED). */
+ * flux (godunov.cpp:39-77) and the other Fluxes are not built
+ * (ZFV_ERR_UNSUPPORTED). */
 int zfv_mm_set_method(zfv_ctx* ctx, int flux);
 /* MmFluid::set_crp_mode (mm_fluid.cpp:63-70): ZFV_CRP_V3 / V5 / VS (the
  * heuristic face fractions, sections.cpp:52-84) or ZFV_CRP_PLIC, each with
@@ -318,6 +319,16 @@ int zfv_mm_crp_faces(zfv_ctx* ctx, int64_t* n);
  * per material the PLIC normal (nx, ny) and section position p,
  * [n_mat][n] each; zero for pure cells and absent materials. */
 int zfv_mm_interface(zfv_ctx* ctx, double* nx, double* ny, double* p);
+/* MmFluid::domain (mm_fluid.cpp:686-712) after interface_recovery of the
+ * current state: the region material idx occupies, one polygon per cell
+ * holding it -- the cell, or the cell clipped by its PLIC line (Polygon::clip,
+ * polygon.cpp:266-292).  Outputs, each optional: the polygon count, the
+ * vertex count, offsets (n_polys + 1: polygon k has vertices offsets[k] ..
+ * offsets[k + 1] - 1, counter-clockwise) and the vertices xs, ys.  Call with
+ * null arrays for the counts, then with arrays of those sizes; no
+ * counterpart of the returned EuMesh's other fields.  Host code. */
+int zfv_mm_domain(zfv_ctx* ctx, int idx, int64_t* n_polys, int64_t* n_verts, int32_t* offsets, double* xs,
+                  double* ys);
 /* MmFluid::set_limiter (mm_fluid.cpp:45-47): limiter number as
  * zfv_limiter_from_name gives it (default MC). */
 int zfv_mm_set_limiter(zfv_ctx* ctx, int limiter);

@@ -1023,13 +1023,23 @@ inline PState extrap(const PState& u, const PState& gx, const PState& gy, const 
 inline double flux_area(const orc_smf& S, size_t s) { return S.axial ? S.m->area_alt[s] : S.m->area[s]; }
 inline double flux_volume(const orc_smf& S, int c) { return S.axial ? S.m->vol_alt[c] : S.m->vol[c]; }
 
+// Test-only planted mistakes of the axisymmetric step (0 none): 1 drops
+// the source, 2 drops the predictor's 1/2 in it, 3 divides the update by
+// the planar volume.
+int g_plant_axial = 0;
+
 // The axisymmetric pressure source: momentum.y += k * coeff * p * dt with
 // coeff = volume / volume_alt (sm_fluid.cpp:166-169, 242-245, 347-350;
 // k = 0.5 in the predictor).
 inline void axial_source(const orc_smf& S, int c, double k, double p, QState& q) {
-    if (!S.axial) return;
+    if (!S.axial || g_plant_axial == 1) return;
+    if (g_plant_axial == 2) k = 1.0;
     const double coeff = S.m->vol[c] / S.m->vol_alt[c];
     q.m.y += (k == 1.0 ? coeff : k * coeff) * p * S.dt;
+}
+
+inline double update_volume(const orc_smf& S, int c) {
+    return g_plant_axial == 3 ? S.m->vol[c] : flux_volume(S, c);
 }
 
 // SmFluid::fluxes, first order (sm_fluid.cpp:127-174).
@@ -1050,7 +1060,7 @@ void fluxes_cell(orc_smf& S, int c) {
         flux_to_global(lf, normal);
         for (int k = 0; k < 5; ++k) flux.arr()[k] = flux.arr()[k] + lf.arr()[k] * flux_area(S, s);
     }
-    double coef = S.dt / flux_volume(S, c);
+    double coef = S.dt / update_volume(S, c);
     for (int k = 0; k < 5; ++k) q_c.arr()[k] = q_c.arr()[k] - coef * flux.arr()[k];
     axial_source(S, c, 1.0, z_c.p, q_c);
     S.next[c] = to_pstate(q_c, S.eos);
@@ -1083,7 +1093,7 @@ void stage1_cell(orc_smf& S, int c) {
         flux_to_global(lf, normal);
         for (int k = 0; k < 5; ++k) flux.arr()[k] = flux.arr()[k] + lf.arr()[k] * flux_area(S, s);
     }
-    double coef = 0.5 * S.dt / flux_volume(S, c);
+    double coef = 0.5 * S.dt / update_volume(S, c);
     for (int k = 0; k < 5; ++k) q_c.arr()[k] = q_c.arr()[k] - coef * flux.arr()[k];
     axial_source(S, c, 0.5, z_c.p, q_c);
     S.half[c] = to_pstate(q_c, S.eos);
@@ -1142,7 +1152,7 @@ void stage2_cell(orc_smf& S, int c) {
         flux_to_global(lf, normal);
         for (int k = 0; k < 5; ++k) flux.arr()[k] = flux.arr()[k] + lf.arr()[k] * flux_area(S, s);
     }
-    double coef = S.dt / flux_volume(S, c);
+    double coef = S.dt / update_volume(S, c);
     for (int k = 0; k < 5; ++k) q_c.arr()[k] = q_c.arr()[k] - coef * flux.arr()[k];
     axial_source(S, c, 1.0, z_ch.p, q_c);
     PState zn = to_pstate(q_c, S.eos);
@@ -1194,6 +1204,8 @@ orc_smf* orc_smf_create(orc_mesh* m, double gamma, double P0, double e0,
 
 void orc_smf_free(orc_smf* s) { delete s; }
 void orc_smf_set_max_dt(orc_smf* s, double max_dt) { s->max_dt = max_dt; }
+
+void orc_smf_plant_axial(int k) { g_plant_axial = k; }
 
 int orc_smf_set_axial(orc_smf* s, int axial) {
     if (axial && !s->m->axial) return -1;   // no alternative geometry on this mesh
@@ -1382,3 +1394,4 @@ void orc_init_quadrants(const orc_mesh* m, uint64_t seed, double gamma,
 
 #include "oracle_amr.hpp"
 #include "oracle_mm.hpp"
+#include "oracle_transfer.hpp"

@@ -94,6 +94,7 @@ void orc_smf_free(orc_smf* s);
 void orc_smf_set_max_dt(orc_smf* s, double max_dt);
 // SmFluid::set_axial; -1 if the mesh has no axial geometry.
 int orc_smf_set_axial(orc_smf* s, int axial);
+void orc_smf_plant_axial(int k);   /* test-only planted mistakes of the axial step, 0 = none */
 void orc_smf_set_state(orc_smf* s, const double* rho, const double* u, const double* v,
                        const double* p, const double* e);
 void orc_smf_get_state(const orc_smf* s, double* rho, double* u, double* v,
@@ -152,6 +153,9 @@ void orc_mm_closure(int n_mat, const double* mats, double rho, double e, const d
                     double T0, double* out);
 void orc_mm_closure_rP(int n_mat, const double* mats, double rho, double P, const double* beta, double* out);
 void orc_mm_face_flux(const orc_mm* s, int cL, int cR, double nx, double ny, double* F);
+int orc_mm_face_flux_kind(const orc_mm* s, int kind, int cL, int cR, double nx, double ny, double* F);
+void orc_mm_stiffened_gas(const orc_mm* s, int c, double* out);   /* {gamma, P0} of cell c's state */
+double orc_mm_sound_speed(const orc_mm* s, int c);
 int orc_mm_crp_flux(const orc_mm* s, int cL, int cR, double hL, double hR, int iA, double a_sig, double dt,
                     double* F);
 void orc_mm_wave_speeds(const orc_mm* s, int cL, int cR, double* S3);
@@ -169,6 +173,24 @@ double orc_quad_volume_fraction(double p, double nx, double ny, double A, double
 double orc_quad_find_section(double alpha, double nx, double ny, double A, double B);
 double orc_face_fraction(int kind, double a1, double a2);
 double orc_average_flux(double alpha, double cs, double CFL);
+
+/* Transfer (math/solver/transfer.h): a volume fraction carried by a velocity
+ * sampled at the face centres (fvx, fvy: 8n, slot s of cell c at 8c + s)
+ * and cell centres (cvx, cvy: n).  Methods (Transfer::Method numbers):
+ * 0 CRP_V3, 1 CRP_V5, 2 CRP_SE, 3 CRP_N1, 7 MUSCLd, 9 MUSCLd_CRP,
+ * 11 MUSCL_MC, 12 MUSCL_MC_CRP; create returns NULL for the others.
+ * dir: 0 ANY, 1 X, 2 Y. */
+typedef struct orc_tr orc_tr;
+orc_tr* orc_tr_create(orc_mesh* m, int method, double cfl);
+void orc_tr_free(orc_tr* T);
+void orc_tr_set_velocity(orc_tr* T, const double* fvx, const double* fvy, const double* cvx, const double* cvy);
+void orc_tr_set_state(orc_tr* T, const double* u);
+void orc_tr_state(const orc_tr* T, double* u, double* nx, double* ny);
+void orc_tr_set_dt(orc_tr* T, double dt);
+double orc_tr_compute_dt(const orc_tr* T);
+void orc_tr_update(orc_tr* T, int dir);
+void orc_tr_slopes(const orc_tr* T, double* dx, double* dy);
+void orc_tr_set_flags(orc_tr* T);   /* into the mesh's flags */
 
 #ifdef __cplusplus
 }

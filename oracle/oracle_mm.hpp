@@ -912,7 +912,7 @@ constexpr int kMG = 6 + 2 * kMaxMat;
 // CrpMode (mm_fluid.h:36-42) and Fluxes (num_flux.h:11-21) by their
 // positions in the reference's enums.
 constexpr int kCrpNone = 0, kCrpV3 = 1, kCrpV5 = 2, kCrpVS = 3, kCrpPlic = 4;
-constexpr int kFluxRusanov = 1, kFluxHll = 2, kFluxHllc = 3;
+constexpr int kFluxGodunov = 0, kFluxRusanov = 1, kFluxHll = 2, kFluxHllc = 3;
 struct MG { double a[kMG] = {}; };
 
 struct orc_mm {
@@ -1175,6 +1175,59 @@ void m_interpolation_update(MState& z, const Mixture& mx) {
     for (int i = 0; i < kMaxMat; ++i) z.rhos[i] = rh[i];
 }
 
+// MixturePT::stiffened_gas (mixture_pt.cpp:152-169): the stiffened gas
+// tangent to the mixture EOS at (rho, P) -- a pure state's own material,
+// else gamma = 1 + 1 / (rho e_P), P0 = -(rho e_R + P e_P) / (gamma e_P)
+// with e_R, e_P from find_reT_rP with derivatives (e0 is not needed).
+inline void mix_stiffened_gas(const Mixture& mx, double rho, double P, const Frac& b, const MixOpt& o0, double& g,
+                              double& p0) {
+    const int idx = b.index();
+    if (idx >= 0) {   // StiffenedGas::stiffened_gas (stiffened_gas.cpp:169-171)
+        g = mx.m[idx].g;
+        p0 = mx.m[idx].P0;
+        return;
+    }
+    MixOpt o = o0;
+    o.deriv = true;
+    double rh[kMaxMat], e, edR, edP, T;
+    find_reT_rP(mx, rho, P, b, o, rh, e, edR, edP, T);
+    g = 1.0 + 1.0 / (rho * edP);
+    p0 = -(rho * edR + P * edP) / (g * edP);
+}
+
+// Godunov::calc_flux, mmf (math/cfd/flux/godunov.cpp:39-77): the exact
+// Riemann solver between the two sides' tangent stiffened gases; the face
+// state takes the upwind side's (U >= 0: left) tangential velocity and
+// fractions, its energy and temperature from get_reT at the face's
+// (rho, P) with the side's T as the guess; the physical flux of it.
+inline MArr m_godunov(const MState& zL, const MState& zR, const Mixture& mx) {
+    if (m_is_bad(zL) || m_is_bad(zR)) g_numeric_error = 1;   // :40-47 throw
+    MixOpt oL, oR;
+    oL.T0 = zL.T;
+    oL.rhos = zL.rhos;
+    oR.T0 = zR.T;
+    oR.rhos = zR.rhos;
+    double gL, p0L, gR, p0R;
+    mix_stiffened_gas(mx, zL.rho, zL.P, zL.beta, oL, gL, p0L);
+    mix_stiffened_gas(mx, zR.rho, zR.P, zR.beta, oR, gR, p0R);
+    const RSol sol = riemann_solve(zL.rho, zL.v.x, zL.P, gL, p0L, zR.rho, zR.v.x, zR.P, gR, p0R);
+    const MState& zs = sol.U >= 0 ? zL : zR;
+    MixOpt os;
+    os.T0 = zs.T;
+    MState z;
+    z.rho = sol.rho;
+    z.v = {sol.Uf, zs.v.y, zs.v.z};
+    z.P = sol.Pf;
+    double rh[kMaxMat];
+    get_reT(mx, sol.rho, sol.Pf, zs.beta, os, rh, z.e, z.T);
+    z.beta = zs.beta;
+    for (int i = 0; i < kMaxMat; ++i) z.rhos[i] = NAN;
+    MArr F = m_flux(z);
+    for (int k = 0; k < 5; ++k)
+        if (!std::isfinite(F.a[k])) g_numeric_error = 1;   // Flux::is_bad, :65-75 exits
+    return F;
+}
+
 // MmFluid::calc_flux (mm_fluid.cpp:347-371) with alpha_sigma for
 // CrpMode::PLIC (:228-256): face-frame states z_m / z_p of slot f of cell
 // c; the face-frame flux.  inx / iny: the interface normals of
@@ -1183,6 +1236,7 @@ MArr mm_calc_flux(const orc_mm& S, int c, size_t f, const MState& z_m, const MSt
                   double dt, const double* inx, const double* iny, int& crp_err) {
     const orc_mesh& m = *S.m;
     if (!S.crp || (z_m.beta.index() >= 0 && z_p.beta.index() >= 0)) {
+        if (S.flux == kFluxGodunov) return m_godunov(z_m, z_p, S.mix);
         if (S.flux == kFluxRusanov) return m_rusanov(z_m, z_p, S.mix);
         if (S.flux == kFluxHll) return m_hll(z_m, z_p, S.mix);
         return m_hllc(z_m, z_p, S.mix);
@@ -1630,6 +1684,38 @@ void orc_mm_face_flux(const orc_mm* s, int cL, int cR, double nx, double ny, dou
     const Vec3 n{nx, ny, 0.0};
     MArr f = m_hllc(m_in_local(s->init[cL], n), m_in_local(s->init[cR], n), s->mix);
     for (int k = 0; k < 9; ++k) F[k] = f.a[k];
+}
+
+// The plain mmf flux `kind` (0 Godunov, 1 Rusanov, 2 HLL, 3 HLLC) between
+// the states of cells cL and cR across a face with normal (nx, ny), in the
+// face frame; returns 1 where the reference throws.
+int orc_mm_face_flux_kind(const orc_mm* s, int kind, int cL, int cR, double nx, double ny, double* F) {
+    const Vec3 n{nx, ny, 0.0};
+    const MState zL = m_in_local(s->init[cL], n), zR = m_in_local(s->init[cR], n);
+    g_numeric_error = 0;
+    MArr f = kind == kFluxGodunov ? m_godunov(zL, zR, s->mix)
+           : kind == kFluxRusanov ? m_rusanov(zL, zR, s->mix)
+           : kind == kFluxHll ? m_hll(zL, zR, s->mix) : m_hllc(zL, zR, s->mix);
+    for (int k = 0; k < 9; ++k) F[k] = f.a[k];
+    return g_numeric_error;
+}
+
+// MixturePT::stiffened_gas of cell c's state: out = {gamma, P0}.
+void orc_mm_stiffened_gas(const orc_mm* s, int c, double* out) {
+    const MState& z = s->init[c];
+    MixOpt o;
+    o.T0 = z.T;
+    o.rhos = z.rhos;
+    mix_stiffened_gas(s->mix, z.rho, z.P, z.beta, o, out[0], out[1]);
+}
+
+// sound_speed_rP of cell c's state.
+double orc_mm_sound_speed(const orc_mm* s, int c) {
+    const MState& z = s->init[c];
+    MixOpt o;
+    o.T0 = z.T;
+    o.rhos = z.rhos;
+    return mix_sound_speed_rP(s->mix, z.rho, z.P, z.beta, o);
 }
 
 // calc_crp_flux (mm_fluid.cpp:259-337) between the states of cells cL and

@@ -49,6 +49,7 @@ def lib():
             "orc_mesh_axial": (_I, [_P]),
             "orc_mesh_get_axial": (None, [_P, _dp, _dp]),
             "orc_smf_set_axial": (_I, [_P, _I]),
+            "orc_smf_plant_axial": (None, [_I]),
             "orc_mesh_free": (None, [_P]),
             "orc_mesh_size": (_I, [_P]),
             "orc_mesh_get_cells": (None, [_P, _dp, _dp, _dp, _ip, _ip, _ip, _ip]),
@@ -81,6 +82,19 @@ def lib():
             "orc_mm_set_max_dt": (None, [_P, _D]),
             "orc_mm_set_crp": (None, [_P, _I]),
             "orc_mm_set_flux": (None, [_P, _I]),
+            "orc_tr_create": (_P, [_P, _I, _D]),
+            "orc_tr_free": (None, [_P]),
+            "orc_tr_set_velocity": (None, [_P, _dp, _dp, _dp, _dp]),
+            "orc_tr_set_state": (None, [_P, _dp]),
+            "orc_tr_state": (None, [_P, _dp, _dp, _dp]),
+            "orc_tr_set_dt": (None, [_P, _D]),
+            "orc_tr_compute_dt": (_D, [_P]),
+            "orc_tr_update": (None, [_P, _I]),
+            "orc_tr_slopes": (None, [_P, _dp, _dp]),
+            "orc_tr_set_flags": (None, [_P]),
+            "orc_mm_face_flux_kind": (_I, [_P, _I, _I, _I, _D, _D, _dp]),
+            "orc_mm_stiffened_gas": (None, [_P, _I, _dp]),
+            "orc_mm_sound_speed": (_D, [_P, _I]),
             "orc_mm_set_splitting": (None, [_P, _I]),
             "orc_mm_set_state": (None, [_P, _dp, _dp, _dp, _dp, _dp]),
             "orc_mm_state": (None, [_P] + [_dp] * 8),
@@ -346,13 +360,14 @@ class MmFluid:
 
     def set_method(self, flux):
         """set_method (mm_fluid.cpp:50-61): "CRP" = PLIC with HLLC, else
-        CrpMode NONE with the named mmf flux ("HLLC", "HLL", "Rusanov")."""
+        CrpMode NONE with the named mmf flux ("HLLC", "HLL", "Rusanov",
+        "Godunov")."""
         if flux == "CRP":
             lib().orc_mm_set_crp(self.h, 4)
             lib().orc_mm_set_flux(self.h, 3)
         else:
             lib().orc_mm_set_crp(self.h, 0)
-            lib().orc_mm_set_flux(self.h, {"Rusanov": 1, "HLL": 2, "HLLC": 3}[flux])
+            lib().orc_mm_set_flux(self.h, {"Godunov": 0, "Rusanov": 1, "HLL": 2, "HLLC": 3}[flux])
 
     def set_splitting(self, split):
         lib().orc_mm_set_splitting(self.h, self.SPLITS.index(split))
@@ -398,6 +413,23 @@ class MmFluid:
         F = np.zeros(9)
         lib().orc_mm_face_flux(self.h, cL, cR, nx, ny, _d(F))
         return F
+
+    def face_flux_kind(self, kind, cL, cR, nx=1.0, ny=0.0):
+        """The plain mmf flux by name ("Godunov", "Rusanov", "HLL", "HLLC")
+        in the face frame, and the reference-throws flag."""
+        F = np.zeros(9)
+        k = {"Godunov": 0, "Rusanov": 1, "HLL": 2, "HLLC": 3}[kind]
+        err = lib().orc_mm_face_flux_kind(self.h, k, cL, cR, nx, ny, _d(F))
+        return F, err
+
+    def stiffened_gas(self, c):
+        """(gamma, P0) of MixturePT::stiffened_gas at cell c's state."""
+        out = np.zeros(2)
+        lib().orc_mm_stiffened_gas(self.h, c, _d(out))
+        return out
+
+    def sound_speed(self, c):
+        return lib().orc_mm_sound_speed(self.h, c)
 
     def crp_flux(self, cL, cR, hL, hR, iA, a_sig, dt):
         """calc_crp_flux between cells cL and cR across a +x face: (F[9], err)."""
@@ -499,3 +531,56 @@ def face_fraction(a1, a2, kind="plic"):
 
 def average_flux(alpha, cos, cfl):
     return lib().orc_average_flux(alpha, cos, cfl)
+
+
+# ---- Transfer (math/solver/transfer.h) -------------------------------------
+TRANSFER_METHODS = ["CRP_V3", "CRP_V5", "CRP_SE", "CRP_N1", "CRP_N2", "VOF", "VOF_CRP", "MUSCLd", "MUSCLn",
+                    "MUSCLd_CRP", "MUSCLn_CRP", "MUSCL_MC", "MUSCL_MC_CRP", "WENO", "WENO_CRP"]
+
+
+class Transfer:
+    """Oracle Transfer: u1 carried by a velocity sampled at the face and cell
+    centres (the values the reference's virtual velocity() gives there)."""
+
+    def __init__(self, mesh, method="MUSCL_MC", cfl=0.5):
+        self.mesh = mesh
+        k = TRANSFER_METHODS.index(method)
+        self.h = lib().orc_tr_create(mesh.h, k, cfl)
+        if not self.h:
+            raise ValueError(f"Transfer method {method} is not restated")
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.orc_tr_free(self.h)
+            self.h = None
+
+    def set_velocity(self, fvx, fvy, cvx, cvy):
+        arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (fvx, fvy, cvx, cvy)]
+        lib().orc_tr_set_velocity(self.h, *[_d(a) for a in arrs])
+
+    def set_state(self, u):
+        lib().orc_tr_set_state(self.h, _d(np.ascontiguousarray(u, dtype=np.float64)))
+
+    def state(self):
+        n = self.mesh.n
+        u, nx, ny = np.zeros(n), np.zeros(n), np.zeros(n)
+        lib().orc_tr_state(self.h, _d(u), _d(nx), _d(ny))
+        return u, nx, ny
+
+    def set_dt(self, dt):
+        lib().orc_tr_set_dt(self.h, dt)
+
+    def compute_dt(self):
+        return lib().orc_tr_compute_dt(self.h)
+
+    def update(self, direction="ANY"):
+        lib().orc_tr_update(self.h, ["ANY", "X", "Y"].index(direction))
+
+    def slopes(self):
+        n = self.mesh.n
+        dx, dy = np.zeros(n), np.zeros(n)
+        lib().orc_tr_slopes(self.h, _d(dx), _d(dy))
+        return dx, dy
+
+    def set_flags(self):
+        lib().orc_tr_set_flags(self.h)

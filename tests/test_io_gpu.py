@@ -162,3 +162,29 @@ def test_io_errors(zfv, tmp_path):
         ctx.vtu_write(str(tmp_path / "x.vtu"))   # no mesh
     with pytest.raises(zfv.ZfvError):
         ctx.restore(str(tmp_path / "missing"))
+
+
+def test_axial_backup_restores_the_axial_run(zfv, tmp_path):
+    """An axisymmetric mesh and solver: the backup holds volume_alt and
+    area_alt (written when present, as AmrCells::backup does) and the
+    axial flag; the restored run continues bit for bit."""
+    from zephyr_amd import problems
+    ctx = zfv.Context()
+    ctx.mesh_rectangle_axial(40, 30, 0.0, 2.0, 0.0, 1.0, bounds=(2, 2, 2, 1))
+    ctx.configure(acc=2, flux="hllc", limiter="MC", cfl=0.45, gamma=GAMMA, axial=True)
+    i, j = np.divmod(np.arange(40 * 30), 30)
+    ctx.set_state(*problems.four_quadrant((i + 0.5) / 40, (j + 0.5) / 30, 8))
+    ctx.step(3)
+    ctx.backup(str(tmp_path / "ax"))
+    desc = json.load(open(tmp_path / "ax" / "backup.json"))
+    assert desc["mesh"]["cells"]["axial"] is True and desc["solver"]["axial"] is True
+    assert "volume_alt" in desc["mesh"]["cells"] and "area_alt" in desc["mesh"]["cells"]["faces"]
+    res = zfv.Context()
+    res.restore(str(tmp_path / "ax"))
+    assert res.params()["axial"] == 1
+    for a, b in zip(res.mesh_axial(), ctx.mesh_axial()):
+        assert np.array_equal(a, b)
+    for c in (ctx, res):
+        c.step(4)
+    for a, b in zip(res.state(), ctx.state()):
+        assert np.array_equal(a, b)

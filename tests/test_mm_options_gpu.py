@@ -95,11 +95,35 @@ def test_splitting_on_an_adapted_mesh(oracle, zfv):
     assert np.max(m.cells()["level"]) == 2
 
 
+@pytest.mark.skip(reason="the device multi-material Godunov flux is not built: its second-order step "
+                         "made an illegal memory access on the device whose cause was not found; the oracle's "
+                         "flux stays pinned in test_oracle_mm_godunov.py")
+@pytest.mark.parametrize("acc", [1, 2])
+@pytest.mark.parametrize("mats", [[GAS, HEAVY], [WATER, AIR]])
+def test_mmf_godunov_matches_oracle(oracle, zfv, acc, mats):
+    """set_method(Godunov): the exact Riemann solver takes pow in its
+    rarefaction and shock functions, whose device and glibc values may
+    differ in the last bit, so the bar is relative 1e-10 per field after 6
+    steps (the same as the single-material Godunov step's 1e-12 over 30
+    steps, with the closure's Newton iterations in between)."""
+    _, o, ctx = configured_pair(oracle, zfv, mats, acc, method="Godunov", bounds=(1, 2, 2, 1))
+    for step in range(6):
+        assert o.update() == 0, f"step {step}: the oracle rejected a state"
+        ctx.mm_step(1)
+        np.testing.assert_allclose(ctx.mm_dt(), o.dt(), rtol=1e-10)
+    zg, zo = ctx.mm_state(), o.state()
+    for k in ("rho", "u", "v", "P", "e", "T", "beta"):
+        a, b = np.nan_to_num(zg[k]), np.nan_to_num(zo[k])
+        np.testing.assert_allclose(a, b, rtol=1e-10, atol=1e-10 * np.abs(b).max(), err_msg=k)
+    assert ctx.mm_crp_faces() == 0
+
+
 def test_unsupported_options_are_refused(zfv):
     ctx = zfv.Context()
     ctx.mesh_rectangle(8, 8)
     ctx.mm_configure([GAS, HEAVY], 0.5)
-    for bad in (lambda: ctx.mm_set_method("Godunov"), lambda: ctx.mm_set_crp_mode("MUSCL")):
+    for bad in (lambda: ctx.mm_set_method("Godunov"), lambda: ctx.mm_set_method(4),
+                lambda: ctx.mm_set_crp_mode("MUSCL")):
         with pytest.raises(zfv.ZfvError, match="error 4"):
             bad()
     ctx.mm_set_crp_mode("NONE")   # ignored, as MmFluid::set_crp_mode does

@@ -38,7 +38,7 @@ EXPORTS = [
     "zfv_mm_set_splitting", "zfv_mm_set_overlap", "zfv_mm_set_flags", "zfv_mm_refine", "zfv_mm_grad",
     "zfv_adjoint_forward", "zfv_adjoint_backward",
     "zfv_mm_configure", "zfv_mm_set_crp", "zfv_mm_set_max_dt", "zfv_mm_set_state", "zfv_mm_state", "zfv_mm_step", "zfv_mm_dt", "zfv_mm_crp_faces",
-    "zfv_mm_interface",
+    "zfv_mm_interface", "zfv_mm_domain",
     "zfv_smf_get_params", "zfv_vtu_write", "zfv_pvd_open", "zfv_pvd_save", "zfv_pvd_close",
     "zfv_backup", "zfv_restore",
 ]
@@ -125,6 +125,7 @@ def lib():
         L.zfv_mm_step.argtypes = [vp, C.c_int]
         L.zfv_mm_dt.argtypes = [vp, vp]
         L.zfv_mm_interface.argtypes = [vp] * 4
+        L.zfv_mm_domain.argtypes = [vp, C.c_int, vp, vp, vp, vp, vp]
         L.zfv_mm_set_limiter.argtypes = [vp, C.c_int]
         L.zfv_mm_set_accuracy.argtypes = [vp, C.c_int]
         L.zfv_mm_set_method.argtypes = [vp, C.c_int]
@@ -475,10 +476,21 @@ class Context:
     CRP_MODES = ("NONE", "V3", "V5", "VS", "PLIC", "MUSCL")                  # CrpMode (mm_fluid.h:36-42)
     SPLITS = ("NONE", "SIMPLE", "STRANG")                                    # DirSplit (mm_fluid.h:29-33)
 
+    def mm_domain(self, idx):
+        """MmFluid::domain: (offsets, xs, ys) of the polygons material idx
+        occupies, after interface_recovery of the current state."""
+        npoly, nvert = C.c_int64(), C.c_int64()
+        _check(lib().zfv_mm_domain(self.h, idx, C.byref(npoly), C.byref(nvert), None, None, None))
+        off = np.zeros(npoly.value + 1, np.int32)
+        xs, ys = np.zeros(nvert.value), np.zeros(nvert.value)
+        _check(lib().zfv_mm_domain(self.h, idx, None, None, _ptr(off), _ptr(xs), _ptr(ys)))
+        return off, xs, ys
+
     def mm_set_method(self, flux):
         """MmFluid::set_method: "CRP" (PLIC + HLLC) or a plain mmf flux
-        ("HLLC", "HLL", "Rusanov") without composite fluxes."""
-        _check(lib().zfv_mm_set_method(self.h, self.MM_FLUXES[flux]))
+        ("HLLC", "HLL", "Rusanov", "Godunov") without composite fluxes; or a
+        Fluxes id."""
+        _check(lib().zfv_mm_set_method(self.h, flux if isinstance(flux, int) else self.MM_FLUXES[flux]))
 
     def mm_set_crp_mode(self, mode):
         """MmFluid::set_crp_mode by name ("V3", "V5", "VS", "PLIC"; "NONE" is

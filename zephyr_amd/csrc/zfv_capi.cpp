@@ -2555,6 +2555,89 @@ int zfv_mm_dt(zfv_ctx* ctx, double* dt) {
     return read_err(ctx);
 }
 
+// MmFluid::domain (mm_fluid.cpp:686-712) after interface_recovery of the
+// current state: per cell the part material idx occupies, as a polygon --
+// none (alpha <= 0, or alpha < 1/2 without an interface normal), the whole
+// cell (alpha >= 1, or alpha > 1/2 without a normal), else the cell clipped
+// by the PLIC line through centre + p n with normal n, keeping the side
+// (v - point) . n <= 1e-14 (Polygon::clip, polygon.cpp:266-292).  A
+// Rectangle-born mesh is linear: a cell's polygon is its four corners
+// (AmrCells::polygon, amr_cells.cpp:147-183).  Host code over the
+// downloaded state and interface.
+int zfv_mm_domain(zfv_ctx* ctx, int idx, int64_t* n_polys, int64_t* n_verts, int32_t* offsets, double* xs,
+                  double* ys) {
+    int r = mm_ready(ctx, true);
+    if (r) return r;
+    const int nm = ctx->mm.mix.n;
+    if (idx < 0 || idx >= nm) return fail(ZFV_ERR_ARG, "mm_domain: no such material");
+    r = sync_host_mesh(ctx);
+    if (r) return r;
+    const HostMesh& m = ctx->hm;
+    const int n = m.n;
+    const size_t sn = (size_t)n, sm = (size_t)nm * n;
+    std::vector<double> rho(sn), u(sn), v(sn), P(sn), e(sn), T(sn), beta(sm), rhos(sm), inx(sm), iny(sm), ip(sm);
+    r = zfv_mm_state(ctx, rho.data(), u.data(), v.data(), P.data(), e.data(), T.data(), beta.data(), rhos.data());
+    if (r) return r;
+    r = zfv_mm_interface(ctx, inx.data(), iny.data(), ip.data());
+    if (r) return r;
+    std::vector<int32_t> off{0};
+    std::vector<double> px, py;
+    double qx[8], qy[8];
+    for (int c = 0; c < n; ++c) {
+        const size_t k = (size_t)idx * n + c;
+        double alpha = 0.0;   // PState::alpha (models.cpp:224-227)
+        if (!std::isnan(rhos[k])) {
+            const double a = beta[k] * rho[c] / rhos[k];
+            const double lo = (1.0 < a) ? 1.0 : a;
+            alpha = (0.0 < lo) ? lo : 0.0;
+        }
+        const double nx = inx[k], ny = iny[k];
+        const bool zero_n = std::abs(nx) <= 1.0e-12 && std::abs(ny) <= 1.0e-12;   // isZero()
+        if (alpha <= 0.0 || (alpha < 0.5 && zero_n)) continue;
+        const double* X = &m.vx[9 * (size_t)c];
+        const double* Y = &m.vy[9 * (size_t)c];
+        const int ring[4] = {0, 2, 8, 6};   // vs<-1,-1>, vs<+1,-1>, vs<+1,+1>, vs<-1,+1>
+        int cnt = 0;
+        if (alpha >= 1.0 || (alpha > 0.5 && zero_n)) {
+            for (int j : ring) {
+                qx[cnt] = X[j];
+                qy[cnt] = Y[j];
+                ++cnt;
+            }
+        } else {
+            const double ptx = m.cx[c] + ip[k] * nx, pty = m.cy[c] + ip[k] * ny;
+            for (int i = 0; i < 4; ++i) {
+                const double x1 = X[ring[i]], y1 = Y[ring[i]];
+                const double x2 = X[ring[(i + 1) % 4]], y2 = Y[ring[(i + 1) % 4]];
+                const double dot1 = (x1 - ptx) * nx + (y1 - pty) * ny;
+                const double dot2 = (x2 - ptx) * nx + (y2 - pty) * ny;
+                const double det = (x2 - x1) * nx + (y2 - y1) * ny;
+                if (dot1 <= 1.0e-14) {
+                    qx[cnt] = x1;
+                    qy[cnt] = y1;
+                    ++cnt;
+                }
+                if (dot1 * dot2 < 0.0) {
+                    qx[cnt] = (dot2 * x1 - dot1 * x2) / det;
+                    qy[cnt] = (dot2 * y1 - dot1 * y2) / det;
+                    ++cnt;
+                }
+            }
+        }
+        for (int i = 0; i < cnt; ++i) {
+            px.push_back(qx[i]);
+            py.push_back(qy[i]);
+        }
+        off.push_back((int32_t)px.size());
+    }
+    if (n_polys) *n_polys = (int64_t)off.size() - 1;
+    if (n_verts) *n_verts = (int64_t)px.size();
+    if (offsets) std::memcpy(offsets, off.data(), off.size() * sizeof(int32_t));
+    if (xs) std::memcpy(xs, px.data(), px.size() * sizeof(double));
+    if (ys) std::memcpy(ys, py.data(), py.size() * sizeof(double));
+    return ZFV_OK;
+}
+
 int zfv_mm_interface(zfv_ctx* ctx, double* nx, double* ny, double* p) {
     int r = mm_ready(ctx, true);
     if (r) return r;

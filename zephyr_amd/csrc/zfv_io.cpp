@@ -65,6 +65,8 @@ struct Snapshot {
     int n = 0;
     std::vector<double> cx, cy, vol, vx, vy, nx, ny, fcx, fcy, area;
     std::vector<int32_t> level, flag, b_idx, z_idx, bnd, adj;
+    bool axial = false;                  // the mesh carries volume_alt / area_alt
+    std::vector<double> vol_alt, area_alt;
     bool state = false;
     std::vector<double> rho, u, v, p, e;
     std::vector<int32_t> wait;
@@ -89,6 +91,17 @@ int take(zfv_ctx* ctx, Snapshot& S, bool need_state) {
     m.boundary = S.bnd.data(); m.adj = S.adj.data();
     m.nx = S.nx.data(); m.ny = S.ny.data(); m.fcx = S.fcx.data(); m.fcy = S.fcy.data(); m.area = S.area.data();
     ZIO(zfv_mesh_download(ctx, &m));
+    S.vol_alt.resize(n);
+    S.area_alt.resize(ns);
+    const int ra = zfv_mesh_axial_download(ctx, S.vol_alt.data(), S.area_alt.data());
+    if (ra == ZFV_OK) {
+        S.axial = true;
+    } else if (ra == ZFV_ERR_STATE) {   // a planar mesh
+        S.vol_alt.clear();
+        S.area_alt.clear();
+    } else {
+        return ra;
+    }
     S.rho.resize(n); S.u.resize(n); S.v.resize(n); S.p.resize(n); S.e.resize(n); S.wait.resize(n);
     const int rs = zfv_state_download(ctx, S.rho.data(), S.u.data(), S.v.data(), S.p.data(), S.e.data());
     if (rs == ZFV_OK) {
@@ -448,7 +461,8 @@ int zfv_backup(zfv_ctx* ctx, const char* root) {
     for (int c = 0; c < n; ++c) lmax = S.level[c] > lmax ? S.level[c] : lmax;
     // Vector3d arrays (centres, normals, vertices) and the PState layer
     std::vector<double> center(3 * (size_t)n), fnorm(3 * ns), fcent(3 * ns), verts(27 * (size_t)n), init(6 * (size_t)n);
-    std::vector<int32_t> face_begin(n + 1), node_begin(n + 1), index(n), rank(n, 0), arank(ns, 0), alien(ns, -1);
+    std::vector<int32_t> face_begin(n + 1), node_begin(n + 1), index(n), rank(n, 0), arank(ns, 0), alien(ns, -1),
+        basic(ns);
     for (int c = 0; c < n; ++c) {
         center[3 * c] = S.cx[c];
         center[3 * c + 1] = S.cy[c];
@@ -467,6 +481,7 @@ int zfv_backup(zfv_ctx* ctx, const char* root) {
         node_begin[c] = 9 * c;
     }
     for (size_t f = 0; f < ns; ++f) {
+        basic[f] = (int32_t)(f / 8);   // the face's own cell
         fnorm[3 * f] = S.nx[f];
         fnorm[3 * f + 1] = S.ny[f];
         fnorm[3 * f + 2] = 0.0;
@@ -484,7 +499,7 @@ int zfv_backup(zfv_ctx* ctx, const char* root) {
     js << "    \"structured\": false,\n";
     js << "    \"cells\": {\n";
     const std::string t = "      ", t2 = "        ", t3 = "          ";
-    js << t << "\"dim\":      2,\n" << t << "\"adaptive\": true,\n" << t << "\"axial\":    false,\n"
+    js << t << "\"dim\":      2,\n" << t << "\"adaptive\": true,\n" << t << "\"axial\":    " << S.axial << ",\n"
        << t << "\"linear\":   true,\n" << t << "\"size\":     " << n << ",\n";
     W.arr("cells", t, "rank", rank.data(), rank.size(), 4, ",\n");
     W.arr("cells", t, "index", index.data(), index.size(), 4, ",\n");
@@ -494,18 +509,21 @@ int zfv_backup(zfv_ctx* ctx, const char* root) {
     W.arr("cells", t, "z_idx", S.z_idx.data(), S.z_idx.size(), 4, ",\n");
     W.arr("cells", t, "center", center.data(), center.size(), 24, ",\n");
     W.arr("cells", t, "volume", S.vol.data(), S.vol.size(), 8, ",\n");
+    if (S.axial) W.arr("cells", t, "volume_alt", S.vol_alt.data(), S.vol_alt.size(), 8, ",\n");
     W.arr("cells", t, "face_begin", face_begin.data(), face_begin.size(), 4, ",\n");
     W.arr("cells", t, "node_begin", node_begin.data(), node_begin.size(), 4, ",\n");
     js << t << "\"faces\": {\n";
     js << t2 << "\"adjacent\": {\n";
     W.arr("cells/faces/adjacent", t3, "rank", arank.data(), arank.size(), 4, ",\n");
     W.arr("cells/faces/adjacent", t3, "index", S.adj.data(), S.adj.size(), 4, ",\n");
-    W.arr("cells/faces/adjacent", t3, "alien", alien.data(), alien.size(), 4, "\n");
+    W.arr("cells/faces/adjacent", t3, "alien", alien.data(), alien.size(), 4, ",\n");
+    W.arr("cells/faces/adjacent", t3, "basic", basic.data(), basic.size(), 4, "\n");
     js << t2 << "},\n";
     W.arr("cells/faces", t2, "boundary", S.bnd.data(), S.bnd.size(), 4, ",\n");
     W.arr("cells/faces", t2, "normal", fnorm.data(), fnorm.size(), 24, ",\n");
     W.arr("cells/faces", t2, "center", fcent.data(), fcent.size(), 24, ",\n");
-    W.arr("cells/faces", t2, "area", S.area.data(), S.area.size(), 8, "\n");
+    W.arr("cells/faces", t2, "area", S.area.data(), S.area.size(), 8, S.axial ? ",\n" : "\n");
+    if (S.axial) W.arr("cells/faces", t2, "area_alt", S.area_alt.data(), S.area_alt.size(), 8, "\n");
     js << t << "},\n";
     W.arr("cells", t, "verts", verts.data(), verts.size(), 24, ",\n");
     js << t << "\"data\": {\n";
@@ -516,7 +534,8 @@ int zfv_backup(zfv_ctx* ctx, const char* root) {
     js << "  \"solver\": {\n";
     js << "    \"kind\": \"SmFluid\",\n";
     js << "    \"accuracy\": " << P.acc << ",\n    \"flux\": " << P.flux << ",\n    \"limiter\": " << P.limiter
-       << ",\n    \"CFL\": " << P.cfl << ",\n    \"max_dt\": " << P.max_dt << ",\n";
+       << ",\n    \"CFL\": " << P.cfl << ",\n    \"max_dt\": " << P.max_dt << ",\n    \"axial\": " << (P.axial != 0)
+       << ",\n";
     js << "    \"eos\": { \"gamma\": " << P.gamma << ", \"P0\": " << P.P0 << ", \"e0\": " << P.e0 << " }\n";
     js << "  }\n}\n";
     if (W.fail_code) return fail(W.fail_code, W.fail_msg);
@@ -608,7 +627,15 @@ int zfv_restore(zfv_ctx* ctx, const char* root) {
     m.level = level.data(); m.flag = flag.data(); m.b_idx = b_idx.data(); m.z_idx = z_idx.data();
     m.boundary = bnd.data(); m.adj = adj.data(); m.nx = nx.data(); m.ny = ny.data();
     m.fcx = fcx.data(); m.fcy = fcy.data(); m.area = area.data();
-    ZIO(zfv_mesh_upload(ctx, &m));
+    const JVal* va = cells->get("volume_alt");
+    if (va) {   // an axisymmetric mesh: its y-weighted volume and face areas
+        std::vector<double> vol_alt(N), area_alt(NS);
+        ZIO(load(va, 8, N, vol_alt.data()));
+        ZIO(load(faces->get("area_alt"), 8, NS, area_alt.data()));
+        ZIO(zfv_mesh_upload_axial(ctx, &m, vol_alt.data(), area_alt.data()));
+    } else {
+        ZIO(zfv_mesh_upload(ctx, &m));
+    }
     auto num = [&](const JVal* o, const char* k, double& out) -> bool {
         const JVal* v = o ? o->get(k) : nullptr;
         if (!v || v->kind != JVal::NUM) return false;
@@ -630,6 +657,8 @@ int zfv_restore(zfv_ctx* ctx, const char* root) {
     P.gamma = g;
     P.P0 = P0;
     P.e0 = e0;
+    const JVal* ax = solver->get("axial");
+    P.axial = ax && ax->kind == JVal::BOOL && ax->b ? 1 : 0;
     ZIO(zfv_smf_configure(ctx, &P));
     std::vector<double> rho(N), u(N), v(N), p(N), e(N);
     for (size_t c = 0; c < N; ++c) {

@@ -865,17 +865,6 @@ __global__ void __launch_bounds__(kBlock) k_mm_flags(DevMesh M, MmLayer L, int* 
     flag[c] = split ? 1 : -1;
 }
 
-// PState::is_bad, mmf (models.cpp:393-403).
-template <int NM>
-__device__ __forceinline__ bool mm_is_bad(const MZ<NM>& z) {
-    bool empty = true;   // Fractions::empty (fractions.cpp:91-99)
-#pragma unroll
-    for (int i = 0; i < NM; ++i)
-        if (z.b[i] > kFracMinimal) empty = false;
-    auto bad = [](double x) { return isinf(x) || isnan(x); };
-    return bad(z.r) || bad(z.u) || bad(z.v) || bad(z.P) || bad(z.e) || bad(z.T) || empty || z.r < 0.0;
-}
-
 // MmFluid::distributor split (mm_fluid.cpp:768-800), one thread per child
 // (ref_list: the refined cells; child i of parent ip at next[ip] + i, its
 // geometry already written): child = parent + grad . dr with momentum- and

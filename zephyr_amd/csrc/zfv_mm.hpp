@@ -408,6 +408,17 @@ __device__ void hll_mm(const MZ<NM>& zL, const MZ<NM>& zR, double* F, bool& bad)
     }
 }
 
+// PState::is_bad, mmf (models.cpp:393-403).
+template <int NM>
+__device__ __forceinline__ bool mm_is_bad(const MZ<NM>& z) {
+    bool empty = true;   // Fractions::empty (fractions.cpp:91-99)
+#pragma unroll
+    for (int i = 0; i < NM; ++i)
+        if (z.b[i] > kFracMinimal) empty = false;
+    auto bad = [](double x) { return isinf(x) || isnan(x); };
+    return bad(z.r) || bad(z.u) || bad(z.v) || bad(z.P) || bad(z.e) || bad(z.T) || empty || z.r < 0.0;
+}
+
 // The flux of the faces without a composite Riemann problem: MmFluid's m_nf
 // (HLLC by default; HLL or Rusanov after set_method with CrpMode::NONE),
 // numbered as the reference's Fluxes (num_flux.h:11-21).

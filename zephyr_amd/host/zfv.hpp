@@ -78,6 +78,10 @@ public:
         nx_ = nx;
         ny_ = ny;
     }
+    // set_axial (rectangle.cpp:97-103): y is the radius; the mesh then
+    // carries the axisymmetric geometry (a bottom on the axis is a wall).
+    void set_axial(bool axial = true) { axial_ = axial; }
+    bool axial() const { return axial_; }
     int nx() const { return nx_; }
     int ny() const { return ny_; }
 
@@ -85,6 +89,7 @@ private:
     friend class EuMesh;
     double x0_, x1_, y0_, y1_;
     int nx_ = 0, ny_ = 0;
+    bool axial_ = false;
     std::array<Boundary, 4> bounds_{Boundary::ZOE, Boundary::ZOE, Boundary::ZOE, Boundary::ZOE};
 };
 
@@ -99,7 +104,8 @@ public:
         check(zfv_create(device, &ctx_));
         int32_t b[4] = {(int32_t)gen.bounds_[0], (int32_t)gen.bounds_[1], (int32_t)gen.bounds_[2],
                         (int32_t)gen.bounds_[3]};
-        check(zfv_mesh_rectangle(ctx_, gen.nx_, gen.ny_, gen.x0_, gen.x1_, gen.y0_, gen.y1_, b));
+        check(gen.axial_ ? zfv_mesh_rectangle_axial(ctx_, gen.nx_, gen.ny_, gen.x0_, gen.x1_, gen.y0_, gen.y1_, b)
+                         : zfv_mesh_rectangle(ctx_, gen.nx_, gen.ny_, gen.x0_, gen.x1_, gen.y0_, gen.y1_, b));
     }
     ~EuMesh() {
         if (ctx_) zfv_destroy(ctx_);
@@ -168,6 +174,12 @@ public:
     void set_max_dt(double dt) {
         max_dt_ = dt;
         if (mesh_) check(zfv_smf_set_max_dt(mesh_->context(), dt));
+    }
+    // set_axial (sm_fluid.cpp:45-47): the axisymmetric areas, volumes and
+    // pressure source; the mesh must come from an axial Rectangle.
+    void set_axial(bool axial = true) {
+        axial_ = axial;
+        reconfigure();
     }
     double CFL() const { return cfl_; }
     int accuracy() const { return acc_; }
@@ -240,6 +252,7 @@ private:
         p.acc = acc_;
         p.flux = (int)flux_;
         p.limiter = limiter_;
+        p.axial = axial_ ? 1 : 0;
         p.cfl = cfl_;
         p.max_dt = max_dt_;
         p.gamma = eos_->gamma;
@@ -252,6 +265,7 @@ private:
     EuMesh* mesh_ = nullptr;
     double cfl_ = 0.5, max_dt_ = std::numeric_limits<double>::max();
     int acc_ = 1, limiter_ = ZFV_LIM_MC;
+    bool axial_ = false;
     Fluxes flux_ = Fluxes::HLLC;
 };
 
