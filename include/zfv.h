@@ -319,6 +319,38 @@ int zfv_mm_crp_faces(zfv_ctx* ctx, int64_t* n);
  * per material the PLIC normal (nx, ny) and section position p,
  * [n_mat][n] each; zero for pure cells and absent materials. */
 int zfv_mm_interface(zfv_ctx* ctx, double* nx, double* ny, double* p);
+/* ---- Transfer (math/solver/transfer.h:21-153, transfer.cpp) ---------------
+ * A volume fraction u carried by a velocity field over the context's mesh.
+ * The reference takes the velocity from a virtual velocity(point); here the
+ * caller gives its samples at the face centres (fvx, fvy: 8n, slot s of cell
+ * c at 8c + s) and at the cell centres (cvx, cvy: n), the points the
+ * reference evaluates it at.  Methods (Transfer::Method numbering): 0
+ * CRP_V3, 1 CRP_V5, 2 CRP_SE, 3 CRP_N1, 7 MUSCLd, 9 MUSCLd_CRP, 11 MUSCL_MC,
+ * 12 MUSCL_MC_CRP; 4 CRP_N2, 5 VOF, 6 VOF_CRP, 8 MUSCLn, 10 MUSCLn_CRP
+ * (sub-cell polygon reconstruction) and 13 WENO, 14 WENO_CRP (structured
+ * stencils) are not built (ZFV_ERR_UNSUPPORTED).  The interface normals are
+ * those of InterfaceRecovery::compute_normals after each update (no
+ * smoothing); an adaptation invalidates the state (set it again). */
+/* Transfer(), set_method, set_CFL (transfer.cpp:43-71). */
+int zfv_transfer_configure(zfv_ctx* ctx, int method, double cfl);
+/* The velocity samples (replace Transfer::velocity). */
+int zfv_transfer_set_velocity(zfv_ctx* ctx, const double* fvx, const double* fvy, const double* cvx,
+                              const double* cvy);
+/* data.u1 of every cell, then the interface normals. */
+int zfv_transfer_set_state(zfv_ctx* ctx, const double* u);
+/* u1 and the interface normals n (each optional). */
+int zfv_transfer_state(zfv_ctx* ctx, double* u, double* nx, double* ny);
+/* du/dx, du/dy of the last MUSCL update (compute_slopes). */
+int zfv_transfer_slopes(zfv_ctx* ctx, double* dx, double* dy);
+/* set_dt / get_dt (transfer.cpp:78-84) and compute_dt (:86-102). */
+int zfv_transfer_set_dt(zfv_ctx* ctx, double dt);
+int zfv_transfer_get_dt(zfv_ctx* ctx, double* dt);
+int zfv_transfer_compute_dt(zfv_ctx* ctx, double* dt);
+/* n_steps calls of update(mesh, dir), dir 0 ANY, 1 X, 2 Y (transfer.cpp:474-535). */
+int zfv_transfer_update(zfv_ctx* ctx, int dir, int n_steps);
+/* set_flags (transfer.cpp:668-690) into the mesh's adaptation flags. */
+int zfv_transfer_set_flags(zfv_ctx* ctx);
+
 /* MmFluid::domain (mm_fluid.cpp:686-712) after interface_recovery of the
  * current state: the region material idx occupies, one polygon per cell
  * holding it -- the cell, or the cell clipped by its PLIC line (Polygon::clip,

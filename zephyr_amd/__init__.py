@@ -39,6 +39,9 @@ EXPORTS = [
     "zfv_adjoint_forward", "zfv_adjoint_backward",
     "zfv_mm_configure", "zfv_mm_set_crp", "zfv_mm_set_max_dt", "zfv_mm_set_state", "zfv_mm_state", "zfv_mm_step", "zfv_mm_dt", "zfv_mm_crp_faces",
     "zfv_mm_interface", "zfv_mm_domain",
+    "zfv_transfer_configure", "zfv_transfer_set_velocity", "zfv_transfer_set_state", "zfv_transfer_state",
+    "zfv_transfer_slopes", "zfv_transfer_set_dt", "zfv_transfer_get_dt", "zfv_transfer_compute_dt",
+    "zfv_transfer_update", "zfv_transfer_set_flags",
     "zfv_smf_get_params", "zfv_vtu_write", "zfv_pvd_open", "zfv_pvd_save", "zfv_pvd_close",
     "zfv_backup", "zfv_restore",
 ]
@@ -126,6 +129,16 @@ def lib():
         L.zfv_mm_dt.argtypes = [vp, vp]
         L.zfv_mm_interface.argtypes = [vp] * 4
         L.zfv_mm_domain.argtypes = [vp, C.c_int, vp, vp, vp, vp, vp]
+        L.zfv_transfer_configure.argtypes = [vp, C.c_int, d]
+        L.zfv_transfer_set_velocity.argtypes = [vp] * 5
+        L.zfv_transfer_set_state.argtypes = [vp, vp]
+        L.zfv_transfer_state.argtypes = [vp] * 4
+        L.zfv_transfer_slopes.argtypes = [vp] * 3
+        L.zfv_transfer_set_dt.argtypes = [vp, d]
+        L.zfv_transfer_get_dt.argtypes = [vp, vp]
+        L.zfv_transfer_compute_dt.argtypes = [vp, vp]
+        L.zfv_transfer_update.argtypes = [vp, C.c_int, C.c_int]
+        L.zfv_transfer_set_flags.argtypes = [vp]
         L.zfv_mm_set_limiter.argtypes = [vp, C.c_int]
         L.zfv_mm_set_accuracy.argtypes = [vp, C.c_int]
         L.zfv_mm_set_method.argtypes = [vp, C.c_int]
@@ -475,6 +488,53 @@ class Context:
     MM_FLUXES = {"Godunov": 0, "Rusanov": 1, "HLL": 2, "HLLC": 3, "CRP": 8}   # Fluxes (num_flux.h:11-21)
     CRP_MODES = ("NONE", "V3", "V5", "VS", "PLIC", "MUSCL")                  # CrpMode (mm_fluid.h:36-42)
     SPLITS = ("NONE", "SIMPLE", "STRANG")                                    # DirSplit (mm_fluid.h:29-33)
+
+    # ---- Transfer
+    TRANSFER_METHODS = ["CRP_V3", "CRP_V5", "CRP_SE", "CRP_N1", "CRP_N2", "VOF", "VOF_CRP", "MUSCLd", "MUSCLn",
+                        "MUSCLd_CRP", "MUSCLn_CRP", "MUSCL_MC", "MUSCL_MC_CRP", "WENO", "WENO_CRP"]
+
+    def transfer_configure(self, method="MUSCL_MC", cfl=0.5):
+        k = method if isinstance(method, int) else self.TRANSFER_METHODS.index(method)
+        _check(lib().zfv_transfer_configure(self.h, k, cfl))
+
+    def transfer_set_velocity(self, fvx, fvy, cvx, cvy):
+        """Velocity samples at the face centres (8n) and cell centres (n)."""
+        arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (fvx, fvy, cvx, cvy)]
+        _check(lib().zfv_transfer_set_velocity(self.h, *[_ptr(a) for a in arrs]))
+
+    def transfer_set_state(self, u):
+        _check(lib().zfv_transfer_set_state(self.h, _ptr(np.ascontiguousarray(u, dtype=np.float64))))
+
+    def transfer_state(self):
+        """(u, nx, ny): the fraction and the interface normals."""
+        u, nx, ny = np.zeros(self.n), np.zeros(self.n), np.zeros(self.n)
+        _check(lib().zfv_transfer_state(self.h, _ptr(u), _ptr(nx), _ptr(ny)))
+        return u, nx, ny
+
+    def transfer_slopes(self):
+        dx, dy = np.zeros(self.n), np.zeros(self.n)
+        _check(lib().zfv_transfer_slopes(self.h, _ptr(dx), _ptr(dy)))
+        return dx, dy
+
+    def transfer_set_dt(self, dt):
+        _check(lib().zfv_transfer_set_dt(self.h, dt))
+
+    def transfer_dt(self):
+        v = C.c_double()
+        _check(lib().zfv_transfer_get_dt(self.h, C.byref(v)))
+        return v.value
+
+    def transfer_compute_dt(self):
+        v = C.c_double()
+        _check(lib().zfv_transfer_compute_dt(self.h, C.byref(v)))
+        return v.value
+
+    def transfer_update(self, direction="ANY", n_steps=1):
+        d = direction if isinstance(direction, int) else ["ANY", "X", "Y"].index(direction)
+        _check(lib().zfv_transfer_update(self.h, d, n_steps))
+
+    def transfer_set_flags(self):
+        _check(lib().zfv_transfer_set_flags(self.h))
 
     def mm_domain(self, idx):
         """MmFluid::domain: (offsets, xs, ys) of the polygons material idx

@@ -21,6 +21,7 @@ SOURCES = [
     os.path.join(HERE, "csrc", "zfv_mm.hip"),
     os.path.join(HERE, "csrc", "zfv_order.hip"),
     os.path.join(HERE, "csrc", "zfv_tile.hip"),
+    os.path.join(HERE, "csrc", "zfv_transfer.hip"),
     os.path.join(HERE, "csrc", "zfv_capi.cpp"),
     os.path.join(HERE, "csrc", "zfv_io.cpp"),
 ]

@@ -218,6 +218,17 @@ struct zfv_ctx {
         hipStream_t side = nullptr;
         hipEvent_t ev_in = nullptr, ev_mark = nullptr, ev_crp = nullptr;
     } mm;
+    // Transfer solver: u in two layers, the interface normals, the slopes,
+    // the velocity samples (faces slot-major, cells), its dt and CFL
+    struct TransferSolver {
+        bool configured = false, have_state = false, have_velocity = false;
+        long long epoch = -1;   // mesh epoch of the state and velocity
+        int method = 11;        // Transfer::Method (MUSCL_MC)
+        double cfl = 0.5, dt = 1.0e300;   // Transfer::Transfer (transfer.cpp:43-48)
+        int cur = 0;
+        DBuf<double> u[2], nx, ny, gx, gy, fvx, fvy, cvx, cvy;
+        DBuf<unsigned long long> slot;
+    } tr;
     long long mesh_epoch = 0;   // incremented whenever the mesh changes
 
     // Curve order of the step kernels (zfv_order.hip): after a device
@@ -567,6 +578,10 @@ int upload_mesh(zfv_ctx* ctx) {
         for (int k = 0; k < 5; ++k) ZFV_HIP(ctx->st[l][k].alloc(n));
     ZFV_HIP(ctx->gdx.alloc(4 * (size_t)n));
     ZFV_HIP(ctx->gdy.alloc(4 * (size_t)n));
+    // a batch's padding cells are never written by the gradient kernels:
+    // zero, so a gradient download is deterministic there
+    ZFV_HIP(hipMemsetAsync(ctx->gdx.p, 0, 4 * (size_t)n * sizeof(double), s));
+    ZFV_HIP(hipMemsetAsync(ctx->gdy.p, 0, 4 * (size_t)n * sizeof(double), s));
     for (int k = 0; k < 2; ++k) {
         ZFV_HIP(ctx->wait[k].alloc(n));
         ZFV_HIP(hipMemsetAsync(ctx->wait[k].p, 0, n * sizeof(int), s));
@@ -2553,6 +2568,179 @@ int zfv_mm_dt(zfv_ctx* ctx, double* dt) {
     ZFV_HIP(hipMemcpyAsync(dt, ctx->mm.dt_out.p, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
     ZFV_HIP(hipStreamSynchronize(ctx->stream));
     return read_err(ctx);
+}
+
+// ---- Transfer (math/solver/transfer.h:21-153) -------------------------------
+static int tr_ready(zfv_ctx* ctx, bool state) {
+    int r = check_ctx(ctx);
+    if (r) return r;
+    if (!ctx->have_mesh) return fail(ZFV_ERR_STATE, "no mesh uploaded");
+    auto& T = ctx->tr;
+    if (!T.configured) return fail(ZFV_ERR_STATE, "transfer: not configured");
+    if (ctx->nseg != 1) return fail(ZFV_ERR_UNSUPPORTED, "transfer on a batch of problems");
+    if (state && (!T.have_state || T.epoch != ctx->mesh_epoch))
+        return fail(ZFV_ERR_STATE, "transfer: no state on the current mesh");
+    return ZFV_OK;
+}
+
+int zfv_transfer_configure(zfv_ctx* ctx, int method, double cfl) {
+    int r = check_ctx(ctx);
+    if (r) return r;
+    if (!ctx->have_mesh) return fail(ZFV_ERR_STATE, "no mesh uploaded");
+    if (method < 0 || method > 14) return fail(ZFV_ERR_ARG, "transfer: unknown method");
+    if (!tr_method_built(method))
+        return fail(ZFV_ERR_UNSUPPORTED, "transfer: CRP_N2, VOF, VOF_CRP, MUSCLn, MUSCLn_CRP, WENO, WENO_CRP not built");
+    auto& T = ctx->tr;
+    T.method = method;
+    T.cfl = std::max(0.0, std::min(cfl, 1.0));   // set_CFL
+    T.configured = true;
+    return ZFV_OK;
+}
+
+int zfv_transfer_set_velocity(zfv_ctx* ctx, const double* fvx, const double* fvy, const double* cvx,
+                              const double* cvy) {
+    int r = tr_ready(ctx, false);
+    if (r) return r;
+    if (!fvx || !fvy || !cvx || !cvy) return fail(ZFV_ERR_ARG, "transfer: null velocity array");
+    r = to_mesh_order(ctx);
+    if (r) return r;
+    auto& T = ctx->tr;
+    const int n = ctx->hm.n;
+    const size_t cap = (size_t)ctx->cap;
+    std::vector<double> sx(8 * cap, 0.0), sy(8 * cap, 0.0);
+    for (int c = 0; c < n; ++c)
+        for (int k = 0; k < 8; ++k) {   // host layout 8c + k to slot-major
+            sx[(size_t)k * cap + c] = fvx[8 * (size_t)c + k];
+            sy[(size_t)k * cap + c] = fvy[8 * (size_t)c + k];
+        }
+    for (auto* b : {&T.fvx, &T.fvy}) ZFV_HIP(b->alloc(8 * cap));
+    for (auto* b : {&T.cvx, &T.cvy}) ZFV_HIP(b->alloc(cap));
+    hipStream_t s = ctx->stream;
+    ZFV_HIP(hipMemcpyAsync(T.fvx.p, sx.data(), 8 * cap * sizeof(double), hipMemcpyHostToDevice, s));
+    ZFV_HIP(hipMemcpyAsync(T.fvy.p, sy.data(), 8 * cap * sizeof(double), hipMemcpyHostToDevice, s));
+    ZFV_HIP(hipMemcpyAsync(T.cvx.p, cvx, n * sizeof(double), hipMemcpyHostToDevice, s));
+    ZFV_HIP(hipMemcpyAsync(T.cvy.p, cvy, n * sizeof(double), hipMemcpyHostToDevice, s));
+    ZFV_HIP(hipStreamSynchronize(s));
+    T.have_velocity = true;
+    if (T.epoch != ctx->mesh_epoch) T.have_state = false;
+    T.epoch = ctx->mesh_epoch;
+    return ZFV_OK;
+}
+
+// The fraction u1 (data.u1), then the interface normals (update_interface
+// without smoothing, as update_CRP ends, transfer.cpp:493-504).
+int zfv_transfer_set_state(zfv_ctx* ctx, const double* u) {
+    int r = tr_ready(ctx, false);
+    if (r) return r;
+    if (!u) return fail(ZFV_ERR_ARG, "transfer: null state");
+    auto& T = ctx->tr;
+    if (!T.have_velocity || T.epoch != ctx->mesh_epoch) return fail(ZFV_ERR_STATE, "transfer: set the velocity first");
+    const size_t cap = (size_t)ctx->cap;
+    for (auto* b : {&T.u[0], &T.u[1], &T.nx, &T.ny, &T.gx, &T.gy}) ZFV_HIP(b->alloc(cap));
+    if (T.slot.n < 1) ZFV_HIP(T.slot.alloc(1));
+    T.cur = 0;
+    hipStream_t s = ctx->stream;
+    ZFV_HIP(hipMemcpyAsync(T.u[0].p, u, ctx->hm.n * sizeof(double), hipMemcpyHostToDevice, s));
+    ZFV_HIP(tr_normals(ctx->dmesh(), T.u[0].p, T.nx.p, T.ny.p, s));
+    ZFV_HIP(hipStreamSynchronize(s));
+    T.have_state = true;
+    return ZFV_OK;
+}
+
+int zfv_transfer_state(zfv_ctx* ctx, double* u, double* nx, double* ny) {
+    int r = tr_ready(ctx, true);
+    if (r) return r;
+    auto& T = ctx->tr;
+    const size_t b = (size_t)ctx->hm.n * sizeof(double);
+    hipStream_t s = ctx->stream;
+    if (u) ZFV_HIP(hipMemcpyAsync(u, T.u[T.cur].p, b, hipMemcpyDeviceToHost, s));
+    if (nx) ZFV_HIP(hipMemcpyAsync(nx, T.nx.p, b, hipMemcpyDeviceToHost, s));
+    if (ny) ZFV_HIP(hipMemcpyAsync(ny, T.ny.p, b, hipMemcpyDeviceToHost, s));
+    ZFV_HIP(hipStreamSynchronize(s));
+    return ZFV_OK;
+}
+
+int zfv_transfer_slopes(zfv_ctx* ctx, double* dx, double* dy) {
+    int r = tr_ready(ctx, true);
+    if (r) return r;
+    if (!dx || !dy) return fail(ZFV_ERR_ARG, "transfer: null output");
+    auto& T = ctx->tr;
+    const size_t b = (size_t)ctx->hm.n * sizeof(double);
+    ZFV_HIP(hipMemcpyAsync(dx, T.gx.p, b, hipMemcpyDeviceToHost, ctx->stream));
+    ZFV_HIP(hipMemcpyAsync(dy, T.gy.p, b, hipMemcpyDeviceToHost, ctx->stream));
+    ZFV_HIP(hipStreamSynchronize(ctx->stream));
+    return ZFV_OK;
+}
+
+int zfv_transfer_set_dt(zfv_ctx* ctx, double dt) {
+    int r = tr_ready(ctx, false);
+    if (r) return r;
+    ctx->tr.dt = dt;
+    return ZFV_OK;
+}
+
+int zfv_transfer_get_dt(zfv_ctx* ctx, double* dt) {
+    int r = tr_ready(ctx, false);
+    if (r) return r;
+    if (!dt) return fail(ZFV_ERR_ARG, "transfer: null output");
+    *dt = ctx->tr.dt;
+    return ZFV_OK;
+}
+
+// compute_dt (transfer.cpp:95-102): CFL times the minimum over the cells.
+int zfv_transfer_compute_dt(zfv_ctx* ctx, double* dt) {
+    int r = tr_ready(ctx, false);
+    if (r) return r;
+    if (!dt) return fail(ZFV_ERR_ARG, "transfer: null output");
+    auto& T = ctx->tr;
+    if (!T.have_velocity || T.epoch != ctx->mesh_epoch) return fail(ZFV_ERR_STATE, "transfer: set the velocity first");
+    if (T.slot.n < 1) ZFV_HIP(T.slot.alloc(1));
+    hipStream_t s = ctx->stream;
+    ZFV_HIP(launch_fill_u64(T.slot.p, 1, 0x7ff0000000000000ULL, s));
+    ZFV_HIP(tr_dt(ctx->dmesh(), T.cvx.p, T.cvy.p, T.slot.p, s));
+    unsigned long long bits = 0;
+    ZFV_HIP(hipMemcpyAsync(&bits, T.slot.p, sizeof(bits), hipMemcpyDeviceToHost, s));
+    ZFV_HIP(hipStreamSynchronize(s));
+    double tau;
+    std::memcpy(&tau, &bits, sizeof(tau));
+    if (!(tau < INFINITY)) tau = std::numeric_limits<double>::max();   // no finite candidate
+    *dt = T.cfl * tau;
+    return ZFV_OK;
+}
+
+// n_steps calls of Transfer::update(mesh, dir) with the dt as set
+// (transfer.cpp:474-535; dir 0 ANY, 1 X, 2 Y).
+int zfv_transfer_update(zfv_ctx* ctx, int dir, int n_steps) {
+    int r = tr_ready(ctx, true);
+    if (r) return r;
+    if (dir < 0 || dir > 2) return fail(ZFV_ERR_ARG, "transfer: dir 0 (ANY), 1 (X) or 2 (Y)");
+    if (n_steps < 0) return fail(ZFV_ERR_ARG, "negative step count");
+    r = to_mesh_order(ctx);
+    if (r) return r;
+    auto& T = ctx->tr;
+    const DevMesh M = ctx->dmesh();
+    hipStream_t s = ctx->stream;
+    for (int k = 0; k < n_steps; ++k) {
+        ZFV_HIP(timed(ctx, "tr_step", [&] {
+            return tr_step(M, T.method, T.u[T.cur].p, T.u[1 - T.cur].p, T.nx.p, T.ny.p, T.gx.p, T.gy.p, T.fvx.p,
+                           T.fvy.p, T.dt, dir, s);
+        }));
+        T.cur = 1 - T.cur;
+        ZFV_HIP(timed(ctx, "tr_normals", [&] { return tr_normals(M, T.u[T.cur].p, T.nx.p, T.ny.p, s); }));
+    }
+    return ZFV_OK;
+}
+
+// Transfer::set_flags (transfer.cpp:668-690) into the mesh's flags.
+int zfv_transfer_set_flags(zfv_ctx* ctx) {
+    int r = tr_ready(ctx, true);
+    if (r) return r;
+    r = to_mesh_order(ctx);
+    if (r) return r;
+    ZFV_HIP(tr_flags(ctx->dmesh(), ctx->tr.u[ctx->tr.cur].p, ctx->flag.p, ctx->stream));
+    ZFV_HIP(hipStreamSynchronize(ctx->stream));
+    ctx->host_stale = true;
+    return ZFV_OK;
 }
 
 // MmFluid::domain (mm_fluid.cpp:686-712) after interface_recovery of the

@@ -301,6 +301,16 @@ hipError_t mm_move(const MmLayer& L, const mm::Mix& mx, int stride, int n, int n
 hipError_t mm_cache(const DevMesh& M, const MmLayer& L, const mm::Mix& mx, unsigned long long* slot,
                     hipStream_t s);
 
+// Transfer (zfv_transfer.hip): the slopes (MUSCL methods) and the fluxes
+// of one update from u into out (mesh order), the interface normals, the
+// CFL candidates' minimum into *slot, set_flags into flag.
+bool tr_method_built(int method);
+hipError_t tr_step(const DevMesh& M, int method, const double* u, double* out, const double* tnx, const double* tny,
+                   double* gx, double* gy, const double* fvx, const double* fvy, double dt, int dir, hipStream_t s);
+hipError_t tr_normals(const DevMesh& M, const double* u, double* nx, double* ny, hipStream_t s);
+hipError_t tr_dt(const DevMesh& M, const double* cvx, const double* cvy, unsigned long long* slot, hipStream_t s);
+hipError_t tr_flags(const DevMesh& M, const double* u, int* flag, hipStream_t s);
+
 // Error-word bits: 1 numeric, 2 inconsistent mesh (errors); kErrRobust and
 // kErrIsolated are notices for the adjoint (a cell in the robust isolation
 // mode, a cell isolated by the corrector's bad-state fallback).
