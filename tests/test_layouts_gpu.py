@@ -63,14 +63,15 @@ def test_layout_equals_cell_on_adapted_mesh(zfv, layout, acc, flux, limiter, blo
         assert_same(cell.grad(), other.grad(), "gradients")
 
 
-def test_quad_equals_cell_set_flags(zfv):
+@pytest.mark.parametrize("layout", ["QUAD", "BATCH"])
+def test_layout_equals_cell_set_flags(zfv, layout):
     cell = adapted_context(zfv, zfv.CELL, 2, 3, "MC", seed=5)
-    quad = adapted_context(zfv, zfv.QUAD, 2, 3, "MC", seed=5)
-    for ctx in (cell, quad):
+    other = adapted_context(zfv, getattr(zfv, layout), 2, 3, "MC", seed=5)
+    for ctx in (cell, other):
         ctx.step(3)
         ctx.set_flags(1, 0.1)
-    assert np.array_equal(cell.flags(), quad.flags())
-    assert_same(cell.grad(), quad.grad(), "limited gradients of set_flags")
+    assert np.array_equal(cell.flags(), other.flags())
+    assert_same(cell.grad(), other.grad(), "limited gradients of set_flags")
 
 
 def test_autotune_keeps_state_and_results(zfv):

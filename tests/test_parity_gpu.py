@@ -66,10 +66,12 @@ def test_mesh_upload_of_oracle_mesh(oracle, zfv):
     assert np.array_equal(slots["adj"], m.slots()["adj"])
 
 
+@pytest.mark.parametrize("layout", ["CELL", "QUAD", "BATCH"])
 @pytest.mark.parametrize("acc", [1, 2])
-def test_general_normals_bit_exact(oracle, zfv, acc):
+def test_general_normals_bit_exact(oracle, zfv, acc, layout):
     """Slots whose normals are tilted by round-off (as AMR faces built from
-    vertices can be) take the general rotation / reflection / LSM path."""
+    vertices can be) take the general rotation / reflection / LSM path, in
+    every step-kernel layout."""
     m = oracle.Mesh.rectangle(14, 11, bounds=(1, 2, 1, 3))
     sl = m.slots()
     rng = np.random.default_rng(3)
@@ -82,13 +84,15 @@ def test_general_normals_bit_exact(oracle, zfv, acc):
     ctx = zfv.Context()
     ctx.mesh_upload(m.cells(), sl, m.verts())
     ctx.configure(acc=acc, flux=3, limiter="MC", cfl=0.5, gamma=GAMMA)
+    for st in range(3):
+        ctx.set_launch(st, getattr(zfv, layout), 256)
     st = m.quadrant_state(8)
     s.set_state(*st)
     ctx.set_state(*st)
     for _ in range(15):
         assert s.update(4) == 0
     ctx.step(15)
-    assert_state_equal(ctx.state(), s.state(), f"general normals acc={acc}")
+    assert_state_equal(ctx.state(), s.state(), f"general normals acc={acc} layout={layout}")
 
 
 def test_dt_matches_oracle(oracle, zfv):

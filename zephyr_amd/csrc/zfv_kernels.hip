@@ -147,6 +147,180 @@ __device__ __forceinline__ SlotL slot_load(const DevMesh& M, const DevState& in,
     return L;
 }
 
+// ---- batched loads (the BATCH kernels)
+// Four slots s0..s0+3 of a cell are read in two memory round trips: the
+// slots' own arrays first, then everything at the neighbours.  No load sits
+// under a condition: an undefined slot (adj -1, or anything outside the
+// mesh) reads the cell itself, and the slot types only select among the
+// loaded values afterwards, so all of a cell's loads of one round are in
+// flight together.  The general normals are read only when one of the four
+// slots has one (canonical normals are not stored per slot).
+__device__ __forceinline__ int slot_nb(const DevMesh& M, int c, int a) {
+    return (unsigned)a < (unsigned)M.n ? a : c;
+}
+
+__device__ __forceinline__ bool any_general(int code, int s0) {
+    return ((code >> (4 * s0)) & 0x8888) != 0;
+}
+
+struct LsmRaw4 {
+    double fx[4], fy[4], w[4], a[4], nx[4], ny[4];
+    double r[4], u[4], v[4], p[4], cx[4], cy[4];
+};
+
+template <bool AREA>
+__device__ __forceinline__ void lsm_load4(const DevMesh& M, const DevState& in, int c, int s0, int code,
+                                          LsmRaw4& R) {
+    int adj[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const int so = (s0 + k) * M.stride + c;
+        adj[k] = M.adj[so];
+        R.fx[k] = M.fcx[so];
+        R.fy[k] = M.fcy[so];
+        R.w[k] = M.w[so];
+        R.a[k] = AREA ? M.area[so] : 0.0;
+        R.nx[k] = 0.0;
+        R.ny[k] = 0.0;
+    }
+    if (any_general(code, s0)) {
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            R.nx[k] = M.nx[(s0 + k) * M.stride + c];
+            R.ny[k] = M.ny[(s0 + k) * M.stride + c];
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const int nb = slot_nb(M, c, adj[k]);
+        R.r[k] = in.r[nb];
+        R.u[k] = in.u[nb];
+        R.v[k] = in.v[nb];
+        R.p[k] = in.p[nb];
+        R.cx[k] = M.cx[nb];
+        R.cy[k] = M.cy[nb];
+    }
+}
+
+// slot_load's values from the loads of slot s = s0 + k: the same operations
+// in the same order (lsm_neib, lsm_drn).
+__device__ __forceinline__ SlotL slot_eval(const LsmRaw4& R, int k, int s, int nib, const Z& zc, double ccx,
+                                           double ccy) {
+    SlotL L;
+    const bool gen = (nib & kGeneralNormal) != 0;
+    const Frame fr = frame_of(s & 3, gen, gen ? R.nx[k] : 0.0, gen ? R.ny[k] : 0.0);
+    const int t = nib & 7;
+    const Z zl{R.r[k], R.u[k], R.v[k], R.p[k], 0.0};
+    const Z zn = (t == SC_INNER || t == SC_PERIODIC) ? zl : frame_boundary(fr, zc, t == SC_WALL);
+    double px = R.cx[k], py = R.cy[k];
+    if (t != SC_INNER) frame_symm(fr, ccx, ccy, R.fx[k], R.fy[k], px, py);
+    L.drx = px - ccx;
+    L.dry = py - ccy;
+    lsm_drn(fr, L.drx, L.dry, L.dnx, L.dny);
+    L.w = R.w[k];
+    L.dF[0] = zn.r - zc.r;
+    L.dF[1] = zn.u - zc.u;
+    L.dF[2] = zn.v - zc.v;
+    L.dF[3] = zn.p - zc.p;
+    L.xp = fr.gen || fr.side < 2;
+    L.yp = fr.gen || fr.side >= 2;
+    return L;
+}
+
+// Predictor face geometry of four slots (face centre, area, normal), one
+// round trip.
+struct PredRaw4 {
+    double fx[4], fy[4], a[4], nx[4], ny[4];
+};
+
+__device__ __forceinline__ void pred_load4(const DevMesh& M, int c, int s0, int code, PredRaw4& P) {
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const int so = (s0 + k) * M.stride + c;
+        P.fx[k] = M.fcx[so];
+        P.fy[k] = M.fcy[so];
+        P.a[k] = M.area[so];
+        P.nx[k] = 0.0;
+        P.ny[k] = 0.0;
+    }
+    if (any_general(code, s0)) {
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            P.nx[k] = M.nx[(s0 + k) * M.stride + c];
+            P.ny[k] = M.ny[(s0 + k) * M.stride + c];
+        }
+    }
+}
+
+__device__ __forceinline__ Frame raw_frame(int s, int nib, double nx, double ny) {
+    const bool gen = (nib & kGeneralNormal) != 0;
+    return frame_of(s & 3, gen, gen ? nx : 0.0, gen ? ny : 0.0);
+}
+
+// Everything four corrector faces read (corr_load of slots s0..s0+3), with
+// the face normals as stored: the frames are made when a face is evaluated
+// (face_of), which keeps the registers of four faces' loads within what two
+// resident waves per SIMD allow.
+struct FaceRaw {
+    Z hn;
+    double ngx[4], ngy[4];
+    double ncx, ncy, fx, fy, a, nx, ny;
+    int wnb;
+};
+
+__device__ __forceinline__ void corr_load4(const DevMesh& M, const DevState& half, const DevGrad& G,
+                                           const int* __restrict__ wait_in, int c, int s0, int code, FaceRaw* F) {
+    int adj[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const int so = (s0 + k) * M.stride + c;
+        adj[k] = M.adj[so];
+        F[k].fx = M.fcx[so];
+        F[k].fy = M.fcy[so];
+        F[k].a = M.area[so];
+        F[k].nx = 0.0;
+        F[k].ny = 0.0;
+    }
+    if (any_general(code, s0)) {
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            F[k].nx = M.nx[(s0 + k) * M.stride + c];
+            F[k].ny = M.ny[(s0 + k) * M.stride + c];
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const int nb = slot_nb(M, c, adj[k]);
+        F[k].hn = load_z(half, nb);
+        F[k].wnb = wait_in[nb];
+        F[k].ncx = M.cx[nb];
+        F[k].ncy = M.cy[nb];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            F[k].ngx[j] = G.dx[j * M.stride + nb];
+            F[k].ngy[j] = G.dy[j * M.stride + nb];
+        }
+    }
+}
+
+__device__ __forceinline__ FaceC face_of(const FaceRaw& R, int s, int nib) {
+    FaceC F;
+    F.hn = R.hn;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        F.ngx[j] = R.ngx[j];
+        F.ngy[j] = R.ngy[j];
+    }
+    F.ncx = R.ncx;
+    F.ncy = R.ncy;
+    F.fx = R.fx;
+    F.fy = R.fy;
+    F.a = R.a;
+    F.fr = raw_frame(s, nib, R.nx, R.ny);
+    F.wnb = R.wnb;
+    return F;
+}
+
 __device__ __forceinline__ Q pred_face(const DevMesh& M, int c, int s, int nib, const Z& zc, double ccx, double ccy,
                                        const double* gx, const double* gy, const Eos& eos) {
     const int so = s * M.stride + c;
@@ -164,9 +338,10 @@ __device__ __forceinline__ Q pred_face(const DevMesh& M, int c, int s, int nib, 
 // det = A11*A00 + (-A01)*A10.
 // Without the predictor, `raw` (if not null) receives the unlimited LSM
 // gradient of rho ([2][stride]) for the refinement criterion.  BATCH (the
-// batched layout): the predictor's face geometry of the main slots is
-// loaded with the LSM loads at the start, so the predictor makes no further
-// memory round trips on cells without hanging faces.
+// batched layout): the four main slots' loads, the predictor's face
+// geometry included, are issued together and unconditionally (lsm_load4):
+// two memory round trips per cell instead of two per slot, and two more per
+// pass for the sub-face slots of a cell with hanging faces.
 template <int LIM, bool PRED, bool BATCH>
 __global__ void __launch_bounds__(kMaxBlock)
 __attribute__((amdgpu_waves_per_eu(BATCH ? ZFV_WAVES_GRADB : ZFV_WAVES_GRAD)))
@@ -184,25 +359,32 @@ k_grad_pred(DevMesh M, DevState in, DevGrad G, DevState half,
     const Z zc = load_z(in, c);
     const double ccx = M.cx[c], ccy = M.cy[c];
     const int code = M.code[c];
+    const int wc = PRED ? wait[c] : 0;
     const bool sub = (code >> 16) != 0;   // some second sub-face is defined
 
-    // The four main slots are loaded once, in one batch, and kept for the
-    // limiting pass; sub-face slots (hanging faces) follow in slot order.
+    // The four main slots are loaded once and kept for the limiting pass;
+    // sub-face slots (hanging faces) follow in slot order.  BATCH: two
+    // round trips for the four main slots (lsm_load4), two for the four
+    // sub-face slots of a cell that has any.
     SlotL S[4];
-#pragma unroll
-    for (int s = 0; s < 4; ++s) {
-        const int nib = (code >> (4 * s)) & 15;
-        S[s] = (nib & 7) != SC_UNDEF ? slot_load(M, in, c, s, nib, zc, ccx, ccy) : SlotL{};
-    }
     double pdx[4], pdy[4], pa[4];
-    if (BATCH && PRED) {
+    if (BATCH) {
+        LsmRaw4 R;
+        lsm_load4<PRED>(M, in, c, 0, code, R);
 #pragma unroll
         for (int s = 0; s < 4; ++s) {
-            const int so = s * M.stride + c;
-            const bool def = ((code >> (4 * s)) & 7) != SC_UNDEF;
-            pdx[s] = def ? M.fcx[so] - ccx : 0.0;
-            pdy[s] = def ? M.fcy[so] - ccy : 0.0;
-            pa[s] = def ? M.area[so] : 0.0;
+            const int nib = (code >> (4 * s)) & 15;
+            const bool def = (nib & 7) != SC_UNDEF;
+            S[s] = def ? slot_eval(R, s, s, nib, zc, ccx, ccy) : SlotL{};
+            pdx[s] = def ? R.fx[s] - ccx : 0.0;
+            pdy[s] = def ? R.fy[s] - ccy : 0.0;
+            pa[s] = def ? R.a[s] : 0.0;
+        }
+    } else {
+#pragma unroll
+        for (int s = 0; s < 4; ++s) {
+            const int nib = (code >> (4 * s)) & 15;
+            S[s] = (nib & 7) != SC_UNDEF ? slot_load(M, in, c, s, nib, zc, ccx, ccy) : SlotL{};
         }
     }
     double A00 = 0.0, A01 = 0.0, A10 = 0.0, A11 = 0.0;
@@ -211,10 +393,21 @@ k_grad_pred(DevMesh M, DevState in, DevGrad G, DevState half,
     for (int s = 0; s < 4; ++s)
         if (((code >> (4 * s)) & 7) != SC_UNDEF) lsm_add(S[s], A00, A01, A10, A11, Fx, Fy);
     if (sub) {
-        for (int s = 4; s < 8; ++s) {
-            const int nib = (code >> (4 * s)) & 15;
-            if ((nib & 7) == SC_UNDEF) continue;
-            lsm_add(slot_load(M, in, c, s, nib, zc, ccx, ccy), A00, A01, A10, A11, Fx, Fy);
+        if (BATCH) {
+            LsmRaw4 R;
+            lsm_load4<false>(M, in, c, 4, code, R);
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const int nib = (code >> (4 * (4 + k))) & 15;
+                if ((nib & 7) == SC_UNDEF) continue;
+                lsm_add(slot_eval(R, k, 4 + k, nib, zc, ccx, ccy), A00, A01, A10, A11, Fx, Fy);
+            }
+        } else {
+            for (int s = 4; s < 8; ++s) {
+                const int nib = (code >> (4 * s)) & 15;
+                if ((nib & 7) == SC_UNDEF) continue;
+                lsm_add(slot_load(M, in, c, s, nib, zc, ccx, ccy), A00, A01, A10, A11, Fx, Fy);
+            }
         }
     }
     if (fabs(A11) < 1e-8) A11 = 1.0;   // A(2,2) = 0 -> 1 always holds in 2D (gradient.h:268-273)
@@ -239,10 +432,21 @@ k_grad_pred(DevMesh M, DevState in, DevGrad G, DevState half,
     for (int s = 0; s < 4; ++s)
         if (((code >> (4 * s)) & 7) != SC_UNDEF) lim_add<LIM>(S[s], gx, gy, Lx, Ly);
     if (sub) {
-        for (int s = 4; s < 8; ++s) {
-            const int nib = (code >> (4 * s)) & 15;
-            if ((nib & 7) == SC_UNDEF) continue;
-            lim_add<LIM>(slot_load(M, in, c, s, nib, zc, ccx, ccy), gx, gy, Lx, Ly);
+        if (BATCH) {
+            LsmRaw4 R;
+            lsm_load4<false>(M, in, c, 4, code, R);
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const int nib = (code >> (4 * (4 + k))) & 15;
+                if ((nib & 7) == SC_UNDEF) continue;
+                lim_add<LIM>(slot_eval(R, k, 4 + k, nib, zc, ccx, ccy), gx, gy, Lx, Ly);
+            }
+        } else {
+            for (int s = 4; s < 8; ++s) {
+                const int nib = (code >> (4 * s)) & 15;
+                if ((nib & 7) == SC_UNDEF) continue;
+                lim_add<LIM>(slot_load(M, in, c, s, nib, zc, ccx, ccy), gx, gy, Lx, Ly);
+            }
         }
     }
 #pragma unroll
@@ -256,7 +460,7 @@ k_grad_pred(DevMesh M, DevState in, DevGrad G, DevState half,
 
     // predictor: half = PState(q - (dt/2V) sum Flux(z_c + grad.(x_f - x_c)) S)
     Z zh = zc;
-    if (wait[c] <= 0) {
+    if (wc <= 0) {
         double F0 = 0.0, F1 = 0.0, F2 = 0.0, F3 = 0.0;
         if (BATCH) {
 #pragma unroll
@@ -269,15 +473,31 @@ k_grad_pred(DevMesh M, DevState in, DevGrad G, DevState half,
                 F2 = F2 + f.my;
                 F3 = F3 + f.E;
             }
-        }
-        for (int s = BATCH ? 4 : 0; s < 8; ++s) {
-            const int nib = (code >> (4 * s)) & 15;
-            if ((nib & 7) == SC_UNDEF) continue;
-            const Q f = pred_face(M, c, s, nib, zc, ccx, ccy, gx, gy, eos);
-            F0 = F0 + f.r;
-            F1 = F1 + f.mx;
-            F2 = F2 + f.my;
-            F3 = F3 + f.E;
+            if (sub) {
+                PredRaw4 P;
+                pred_load4(M, c, 4, code, P);
+#pragma unroll
+                for (int k = 0; k < 4; ++k) {
+                    const int nib = (code >> (4 * (4 + k))) & 15;
+                    if ((nib & 7) == SC_UNDEF) continue;
+                    const Q f = pred_eval(raw_frame(4 + k, nib, P.nx[k], P.ny[k]), P.fx[k] - ccx, P.fy[k] - ccy,
+                                          P.a[k], zc, gx, gy, eos);
+                    F0 = F0 + f.r;
+                    F1 = F1 + f.mx;
+                    F2 = F2 + f.my;
+                    F3 = F3 + f.E;
+                }
+            }
+        } else {
+            for (int s = 0; s < 8; ++s) {
+                const int nib = (code >> (4 * s)) & 15;
+                if ((nib & 7) == SC_UNDEF) continue;
+                const Q f = pred_face(M, c, s, nib, zc, ccx, ccy, gx, gy, eos);
+                F0 = F0 + f.r;
+                F1 = F1 + f.mx;
+                F2 = F2 + f.my;
+                F3 = F3 + f.E;
+            }
         }
         Q q = to_q(zc);
         const double coef = 0.5 * dt / M.vol[c];
@@ -324,12 +544,12 @@ __device__ __forceinline__ Q corr_face(const DevMesh& M, const DevState& half, c
 // SmFluid::fluxes_stage2, corrector (sm_fluid.cpp:255-366), plus the next
 // step's CFL candidate.  `wait` is double-buffered: neighbours' isolation is
 // read from the pre-pass array.  BATCH (the batched layout): the loads of
-// the four main slots are issued together before any face is evaluated, two
-// dependent memory round trips per cell instead of two per face, at twice
-// the registers (fewer resident waves; for meshes too small to fill the
-// GPU); otherwise one face at a time.  Sub-face slots of hanging faces
-// follow the main ones, and the flux terms are summed in slot order either
-// way.
+// the four main slots are issued together, unconditionally, before any face
+// is evaluated (corr_load4), two dependent memory round trips per cell
+// instead of two per face, and two more for the four sub-face slots of a
+// cell with hanging faces, at twice the registers (fewer resident waves; for
+// meshes too small to fill the GPU); otherwise one face at a time.  The flux
+// terms are summed in slot order either way.
 template <int FLUX, bool BATCH>
 __global__ void __launch_bounds__(kMaxBlock)
 __attribute__((amdgpu_waves_per_eu(BATCH ? ZFV_WAVES_CORRB : ZFV_WAVES_CORR)))
@@ -354,25 +574,33 @@ k_corr(DevMesh M, DevState init, DevState half, DevGrad G, const int* __restrict
         double F0 = 0.0, F1 = 0.0, F2 = 0.0, F3 = 0.0;
         bool bad = false;
         if (BATCH) {
-            FaceC D[4];
-#pragma unroll
-            for (int s = 0; s < 4; ++s) {
-                const int nib = (code >> (4 * s)) & 15;
-                if ((nib & 7) != SC_UNDEF) D[s] = corr_load(M, half, G, wait_in, c, s, nib);
-            }
+            FaceRaw D[4];
+            corr_load4(M, half, G, wait_in, c, 0, code, D);
 #pragma unroll
             for (int s = 0; s < 4; ++s) {
                 const int nib = (code >> (4 * s)) & 15;
                 if ((nib & 7) == SC_UNDEF) continue;
-                const Q f = corr_eval<FLUX>(D[s], nib, zch, wc, ccx, ccy, gx, gy, eos, bad);
+                const Q f = corr_eval<FLUX>(face_of(D[s], s, nib), nib, zch, wc, ccx, ccy, gx, gy, eos, bad);
                 F0 = F0 + f.r;
                 F1 = F1 + f.mx;
                 F2 = F2 + f.my;
                 F3 = F3 + f.E;
             }
-        }
-        if (!BATCH || (code >> 16) != 0) {
-            for (int s = BATCH ? 4 : 0; s < 8; ++s) {
+            if ((code >> 16) != 0) {
+                corr_load4(M, half, G, wait_in, c, 4, code, D);
+#pragma unroll
+                for (int k = 0; k < 4; ++k) {
+                    const int nib = (code >> (4 * (4 + k))) & 15;
+                    if ((nib & 7) == SC_UNDEF) continue;
+                    const Q f = corr_eval<FLUX>(face_of(D[k], 4 + k, nib), nib, zch, wc, ccx, ccy, gx, gy, eos, bad);
+                    F0 = F0 + f.r;
+                    F1 = F1 + f.mx;
+                    F2 = F2 + f.my;
+                    F3 = F3 + f.E;
+                }
+            }
+        } else {
+            for (int s = 0; s < 8; ++s) {
                 const int nib = (code >> (4 * s)) & 15;
                 if ((nib & 7) == SC_UNDEF) continue;
                 const Q f = corr_face<FLUX>(M, half, G, wait_in, c, s, nib, zch, wc, ccx, ccy, gx, gy, eos, bad);
@@ -926,11 +1154,11 @@ hipError_t launch_grad_pred(int limiter, bool predictor, int layout, int block, 
         dim3 g(grid_cells(M.n, block / 4));
         if (predictor) launch_gp_t<true, kLayoutQuad>(limiter, g, b, s, M, in, G, half, wait, eos, ctl, nullptr);
         else launch_gp_t<false, kLayoutQuad>(limiter, g, b, s, M, in, G, half, wait, eos, ctl, raw);
-    } else if (layout == kLayoutBatch && predictor) {
+    } else if (layout == kLayoutBatch) {
         dim3 g(grid_cells(M.n, block));
-        launch_gp_t<true, kLayoutBatch>(limiter, g, b, s, M, in, G, half, wait, eos, ctl, nullptr);
+        if (predictor) launch_gp_t<true, kLayoutBatch>(limiter, g, b, s, M, in, G, half, wait, eos, ctl, nullptr);
+        else launch_gp_t<false, kLayoutBatch>(limiter, g, b, s, M, in, G, half, wait, eos, ctl, raw);
     } else {
-        // without the predictor the batched layout has nothing to batch
         dim3 g(grid_cells(M.n, block));
         if (predictor) launch_gp_t<true, kLayoutCell>(limiter, g, b, s, M, in, G, half, wait, eos, ctl, nullptr);
         else launch_gp_t<false, kLayoutCell>(limiter, g, b, s, M, in, G, half, wait, eos, ctl, raw);
