@@ -453,14 +453,22 @@ int zfv_autotune(zfv_ctx* ctx, int reps, double* ms_out);
 int zfv_graph_enable(zfv_ctx* ctx, int on);
 /* Cell order of the step kernels.  mode 1 (default): on a mesh with refined
  * cells the steps run on a copy of the mesh sorted along a space-filling
- * curve (base cell, then Morton index inside it), rebuilt after each
- * adaptation; the state is gathered into that order before steps and back
- * into the mesh's order by every entry point that reads or writes per-cell
- * data, so the order is never observable and the results are bit-identical
- * to mode 0 (the mesh's own order throughout).  active: 1 while the state
- * is held in curve order. */
+ * curve (base cell, then Morton index inside it), the order carried across
+ * each zfv_refine (children where their parent was, a parent where its
+ * children were) and the copy rebuilt at the next step; the state is
+ * gathered into that order before steps and back into the mesh's order by
+ * every entry point that reads or writes per-cell data, so the order is
+ * never observable and the results are bit-identical to mode 0 (the mesh's
+ * own order throughout).  mode 2: as mode 1 with the order computed afresh
+ * (sorted) after each adaptation.  active: 1 while the state is held in
+ * curve order.  No counterpart in the reference (its cells stay in storage
+ * order, amr_cells.h:74-104). */
 int zfv_set_cell_order(zfv_ctx* ctx, int mode);
 int zfv_get_cell_order(zfv_ctx* ctx, int* mode, int* active);
+/* The steps' cell order of the current mesh: perm[i] (n entries) is the
+ * mesh index of the i-th cell along the curve.  ZFV_ERR_STATE when the steps
+ * run in the mesh's own order (mode 0, no refined cells, or a batch). */
+int zfv_get_curve_order(zfv_ctx* ctx, int32_t* perm);
 
 /* Kernel timing: average device time (ms) per launch of each step kernel
  * over the last zfv_smf_step call when enabled (HIP events on the context

@@ -149,3 +149,33 @@ def test_mode_switch_and_rectangles(zfv):
     rect.set_state(*problems.four_quadrant(cells["cx"], cells["cy"], 1))
     rect.step(2)
     assert rect.cell_order() == (1, 0)
+
+
+def curve_keys(cells, lmax=3):
+    """Host restatement of the curve key: base cell, then the Morton index of
+    the cell's corner at level lmax inside it."""
+    L = cells["level"].astype(np.int64)
+    return (cells["b_idx"].astype(np.int64) << (2 * lmax)) | (cells["z_idx"].astype(np.int64) << (2 * (lmax - L)))
+
+
+@pytest.mark.parametrize("every", [1, 4])
+def test_carried_curve_order_equals_rebuilt_order(zfv, every):
+    """The order carried across each adaptation (children where their parent
+    was, a parent where its children were) is the sorted curve order, and
+    equals the order computed afresh after each adaptation."""
+    carried = refined_context(zfv, True, nx=40, ny=36, seed=5)
+    rebuilt = refined_context(zfv, True, nx=40, ny=36, seed=5)
+    rebuilt.set_cell_order(True, rebuild=True)
+    for cycle in range(6):
+        for ctx in (carried, rebuilt):
+            ctx.step(every)
+            ctx.set_flags(1, 0.1)
+            ctx.refine(3, slope=True)
+        cells, _, _ = carried.mesh_download()
+        pc, pr = carried.curve_order(), rebuilt.curve_order()
+        assert np.array_equal(pc, pr), f"cycle {cycle}: carried order differs from the rebuilt one"
+        assert np.array_equal(np.sort(pc), np.arange(carried.n)), f"cycle {cycle}: not a permutation"
+        keys = curve_keys(cells)[pc]
+        assert np.all(np.diff(keys) > 0), f"cycle {cycle}: not sorted along the curve"
+        assert_same(carried, rebuilt, f"cycle {cycle}")
+    assert carried.cell_order()[0] == 1 and rebuilt.cell_order()[0] == 2

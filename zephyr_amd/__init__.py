@@ -33,7 +33,7 @@ EXPORTS = [
     "zfv_flags_upload", "zfv_flags_download", "zfv_set_flags", "zfv_balance_flags", "zfv_refine",
     "zfv_mesh_n_faces", "zfv_mesh_reserve", "zfv_set_launch", "zfv_get_launch", "zfv_autotune",
     "zfv_batch_rectangles", "zfv_batch_layout", "zfv_batch_dt", "zfv_graph_enable",
-    "zfv_set_cell_order", "zfv_get_cell_order",
+    "zfv_set_cell_order", "zfv_get_cell_order", "zfv_get_curve_order",
     "zfv_mm_set_limiter", "zfv_mm_set_accuracy", "zfv_mm_set_method", "zfv_mm_set_crp_mode",
     "zfv_mm_set_splitting", "zfv_mm_set_overlap", "zfv_mm_set_flags", "zfv_mm_refine", "zfv_mm_grad",
     "zfv_adjoint_forward", "zfv_adjoint_backward",
@@ -120,6 +120,7 @@ def lib():
         L.zfv_graph_enable.argtypes = [vp, C.c_int]
         L.zfv_set_cell_order.argtypes = [vp, C.c_int]
         L.zfv_get_cell_order.argtypes = [vp, vp, vp]
+        L.zfv_get_curve_order.argtypes = [vp, vp]
         L.zfv_mm_configure.argtypes = [vp, C.c_int, vp, C.c_double]
         L.zfv_mm_set_max_dt.argtypes = [vp, C.c_double]
         L.zfv_mm_set_crp.argtypes = [vp, C.c_int]
@@ -416,10 +417,18 @@ class Context:
     def graph_enable(self, on=True):
         _check(lib().zfv_graph_enable(self.h, 1 if on else 0))
 
-    def set_cell_order(self, curve=True):
-        """Curve order for the step kernels of refined meshes (default) or
-        the mesh's own order throughout; results are identical."""
-        _check(lib().zfv_set_cell_order(self.h, 1 if curve else 0))
+    def set_cell_order(self, curve=True, rebuild=False):
+        """Curve order for the step kernels of refined meshes (default;
+        carried across adaptations, or with rebuild=True computed afresh
+        after each) or the mesh's own order throughout; results are
+        identical."""
+        _check(lib().zfv_set_cell_order(self.h, (2 if rebuild else 1) if curve else 0))
+
+    def curve_order(self):
+        """The steps' cell order: mesh index of each cell along the curve."""
+        out = np.empty(self.n, np.int32)
+        _check(lib().zfv_get_curve_order(self.h, out.ctypes.data))
+        return out
 
     def cell_order(self):
         """(mode, active): mode 1 = curve order for steps; active = the

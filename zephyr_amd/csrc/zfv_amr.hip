@@ -26,7 +26,6 @@ namespace {
 __constant__ int cSF[4][2] = {{0, 6}, {2, 8}, {0, 2}, {6, 8}};
 __constant__ int cCF[8][2] = {{0, 3}, {2, 5}, {0, 1}, {6, 7}, {3, 6}, {5, 8}, {1, 2}, {7, 8}};
 __constant__ int cChildBySubface[8] = {0, 1, 0, 2, 2, 3, 1, 3};
-__constant__ int cChildrenBySide[4][2] = {{0, 2}, {1, 3}, {0, 1}, {2, 3}};
 __constant__ int cSubfaceBySideChild[4][4] = {{0, -1, 4, -1}, {-1, 1, -1, 5}, {2, 6, -1, -1}, {-1, -1, 3, 7}};
 __constant__ int cNeibChild0[8] = {1, 0, 2, 0, 3, 2, 3, 1};
 __constant__ int cNextSiblingSide[4] = {1, 3, 2, 0};
@@ -72,23 +71,22 @@ __device__ void line_geom(V2 v0, V2 v1, V2 view, double& area, double& fx, doubl
     }
 }
 
-__device__ __forceinline__ void slot_undef(const AmrMesh& A, int c, int s) {
-    A.bnd[SI(A, c, s)] = B_UNDEF;
-    A.adj[SI(A, c, s)] = -1;
-}
+// AmrCells::set_cell(ic, Quad, axial = false) (amr_cells.cpp:725-785),
+// computed in registers: area, centroid, the SqQuad's nine vertices and the
+// four sides' face geometry (normals seen from the centroid).
+struct QuadG {
+    double vol;
+    V2 C;
+    V2 V[9];
+    double area[4], fx[4], fy[4], nx[4], ny[4];
+};
 
-// AmrCells::set_cell(ic, Quad, axial = false) (amr_cells.cpp:725-785).
-__device__ void set_cell(const AmrMesh& A, int ic, V2 v00, V2 v01, V2 v10, V2 v11) {
-    A.idx[ic] = -1;
-    A.flag[ic] = 0;
-    A.b_idx[ic] = -1;
-    A.z_idx[ic] = -1;
-    A.level[ic] = -1;
+__device__ void quad_geom(V2 v00, V2 v01, V2 v10, V2 v11, QuadG& G) {
     // Quad::area: 0.5 |(v11 - v00) x (v10 - v01)| (quad.cpp:322-324)
     const double ax = v11.x - v00.x, ay = v11.y - v00.y, bx = v10.x - v01.x, by = v10.y - v01.y;
     const double cz = ax * by - ay * bx;
     const double area = 0.5 * sqrt(cz * cz);
-    A.vol[ic] = area;
+    G.vol = area;
     // Quad::centroid_oxy over vertices 0, 1, 3, 2 (quad.cpp:358-377)
     const V2 q[4] = {v00, v01, v11, v10};
     double Cx = 0.0, Cy = 0.0;
@@ -99,55 +97,68 @@ __device__ void set_cell(const AmrMesh& A, int ic, V2 v00, V2 v01, V2 v10, V2 v1
         Cy = Cy - (b.x - a.x) * (b.y * b.y + b.y * a.y + a.y * a.y);
     }
     const double d6 = 6.0 * area;
-    const V2 C{Cx / d6, Cy / d6};
-    A.cx[ic] = C.x;
-    A.cy[ic] = C.y;
+    G.C = V2{Cx / d6, Cy / d6};
     // SqQuad(const Quad&): bilinear vertices (quad.cpp:452-457)
-    const V2 V[9] = {v00,
-                     {0.5 * (v00.x + v01.x), 0.5 * (v00.y + v01.y)},
-                     v01,
-                     {0.5 * (v00.x + v10.x), 0.5 * (v00.y + v10.y)},
-                     {0.25 * (((v00.x + v01.x) + v10.x) + v11.x), 0.25 * (((v00.y + v01.y) + v10.y) + v11.y)},
-                     {0.5 * (v01.x + v11.x), 0.5 * (v01.y + v11.y)},
-                     v10,
-                     {0.5 * (v10.x + v11.x), 0.5 * (v10.y + v11.y)},
-                     v11};
+    G.V[0] = v00;
+    G.V[1] = V2{0.5 * (v00.x + v01.x), 0.5 * (v00.y + v01.y)};
+    G.V[2] = v01;
+    G.V[3] = V2{0.5 * (v00.x + v10.x), 0.5 * (v00.y + v10.y)};
+    G.V[4] = V2{0.25 * (((v00.x + v01.x) + v10.x) + v11.x), 0.25 * (((v00.y + v01.y) + v10.y) + v11.y)};
+    G.V[5] = V2{0.5 * (v01.x + v11.x), 0.5 * (v01.y + v11.y)};
+    G.V[6] = v10;
+    G.V[7] = V2{0.5 * (v10.x + v11.x), 0.5 * (v10.y + v11.y)};
+    G.V[8] = v11;
+#pragma unroll
+    for (int side = 0; side < 4; ++side)
+        line_geom(G.V[cSF[side][0]], G.V[cSF[side][1]], G.C, G.area[side], G.fx[side], G.fy[side], G.nx[side],
+                  G.ny[side]);
+}
+
+// The cell arrays and vertices of a new cell ic (index ic, flag 0, the given
+// tree indices and level).
+__device__ void write_cell(const AmrMesh& A, int ic, const QuadG& G, int b_idx, int level, int z_idx) {
+    A.idx[ic] = ic;
+    A.flag[ic] = 0;
+    A.b_idx[ic] = b_idx;
+    A.level[ic] = level;
+    A.z_idx[ic] = z_idx;
+    A.vol[ic] = G.vol;
+    A.cx[ic] = G.C.x;
+    A.cy[ic] = G.C.y;
 #pragma unroll
     for (int k = 0; k < 9; ++k) {
-        A.vx[k * A.stride + ic] = V[k].x;
-        A.vy[k * A.stride + ic] = V[k].y;
+        A.vx[k * A.stride + ic] = G.V[k].x;
+        A.vy[k * A.stride + ic] = G.V[k].y;
     }
-    for (int s = 0; s < 8; ++s) slot_undef(A, ic, s);
+}
+
+// The eight slots of a new cell: types b, neighbours a; the main sides get
+// set_cell's face geometry, except the sides split into two sub-faces
+// (split bit per side), which get setup_face_geom's halves with the SqQuad
+// centre as view (faces.h:57-97, 121-130).
+__device__ void write_slots(const AmrMesh& A, int ic, const QuadG& G, const int* b, const int* a, unsigned split) {
+#pragma unroll
+    for (int s = 0; s < 8; ++s) {
+        A.bnd[SI(A, ic, s)] = b[s];
+        A.adj[SI(A, ic, s)] = a[s];
+    }
+#pragma unroll
     for (int side = 0; side < 4; ++side) {
-        const int f = SI(A, ic, side);
-        line_geom(V[cSF[side][0]], V[cSF[side][1]], C, A.area[f], A.fcx[f], A.fcy[f], A.nx[f], A.ny[f]);
-        A.bnd[f] = B_INNER;
+        const int f0 = SI(A, ic, side);
+        if (split & (1u << side)) {
+            const int f1 = SI(A, ic, side + 4);
+            line_geom(G.V[cCF[side][0]], G.V[cCF[side][1]], G.V[4], A.area[f0], A.fcx[f0], A.fcy[f0], A.nx[f0],
+                      A.ny[f0]);
+            line_geom(G.V[cCF[side + 4][0]], G.V[cCF[side + 4][1]], G.V[4], A.area[f1], A.fcx[f1], A.fcy[f1],
+                      A.nx[f1], A.ny[f1]);
+        } else {
+            A.area[f0] = G.area[side];
+            A.fcx[f0] = G.fx[side];
+            A.fcy[f0] = G.fy[side];
+            A.nx[f0] = G.nx[side];
+            A.ny[f0] = G.ny[side];
+        }
     }
-}
-
-__device__ __forceinline__ bool simple_face(const AmrMesh& A, int ic, int side) {
-    return A.bnd[SI(A, ic, side + 4)] == B_UNDEF;
-}
-
-// setup_face_geom with view = SqQuad centre (faces.h:57-97).
-__device__ void face_geom(const AmrMesh& A, int ic, int slot, int a, int b) {
-    const int f = SI(A, ic, slot);
-    line_geom(vtx(A, ic, a), vtx(A, ic, b), vtx(A, ic, 4), A.area[f], A.fcx[f], A.fcy[f], A.nx[f], A.ny[f]);
-}
-
-// split_face_impl (faces.h:121-130).
-__device__ void split_face(const AmrMesh& A, int ic, int side) {
-    const int f0 = SI(A, ic, side), f1 = SI(A, ic, side + 4);
-    A.bnd[f1] = A.bnd[f0];
-    A.adj[f1] = A.adj[f0];
-    face_geom(A, ic, side, cCF[side][0], cCF[side][1]);
-    face_geom(A, ic, side + 4, cCF[side + 4][0], cCF[side + 4][1]);
-}
-
-// merge_faces_impl (faces.h:137-154).
-__device__ void merge_faces(const AmrMesh& A, int ic, int side) {
-    slot_undef(A, ic, side + 4);
-    face_geom(A, ic, side, cSF[side][0], cSF[side][1]);
 }
 
 // can_coarse / get_siblings (mesh/amr/siblings.h:36-131), flags read from `flag`.
@@ -287,11 +298,12 @@ __global__ void __launch_bounds__(256) k_amr_mark(AmrMesh A, Marks3* marks) {
 
 // Positions: retained cells keep their index, parents go to n + rank,
 // children to n + n_parents + 4 rank.  Also the compacted refine / coarsen
-// lists and the hole table (k-th hole ascending).  Thread 0 clears the
+// lists, the four children of each coarsened family in z order (fam) and the
+// hole table (k-th hole ascending).  Thread 0 clears the
 // face counters and the dt slots the derive pass fills.
 __global__ void __launch_bounds__(256) k_amr_positions(AmrMesh A, int n_large, int n_short, int n_parents,
                                                        const Marks3* rank, int* hole_at, int* ref_list,
-                                                       int* crs_list, unsigned long long* faces,
+                                                       int* crs_list, int* fam, unsigned long long* faces,
                                                        unsigned long long* dt_rd, unsigned long long* dt_wr) {
     const int ic = cell_id();
     const int n = A.n;
@@ -320,6 +332,8 @@ __global__ void __launch_bounds__(256) k_amr_positions(AmrMesh A, int n_large, i
         A.next[ic] = pos;
         for (int k = 0; k < 3; ++k) A.next[sibs[k]] = pos;
         crs_list[rk.head] = ic;
+        fam[4 * rk.head] = ic;
+        for (int k = 0; k < 3; ++k) fam[4 * rk.head + (A.z_idx[sibs[k]] & 3)] = sibs[k];
     }
     if (ic < n_short && f != 0) hole_at[rk.nz] = ic;
 }
@@ -462,145 +476,252 @@ __device__ void retain_cell(const AmrMesh& A, int ic) {
     }
 }
 
-// One child of refine_cell / make_children / link_siblings (refine.h:36-444):
-// child i of parent ip, written only into the child's own cell and slots.
-__device__ void refine_child(const AmrMesh& A, int ip, int i) {
-    const int mc = A.next[ip];
-    const int ich = mc + i;
-    set_cell(A, ich, vtx(A, ip, cCorners[i][0]), vtx(A, ip, cCorners[i][1]), vtx(A, ip, cCorners[i][2]),
-             vtx(A, ip, cCorners[i][3]));
-    const int lvl = A.level[ip], zi = A.z_idx[ip];
-    A.idx[ich] = ich;
-    A.flag[ich] = 0;
-    A.b_idx[ich] = A.b_idx[ip];
-    A.level[ich] = lvl + 1;
-    A.z_idx[ich] = 4 * zi + i;
-    for (int l = 0; l < 8; ++l) {
-        if (cLinks[l][0] != i) continue;
-        const int f = SI(A, ich, cLinks[l][1]);
-        A.bnd[f] = B_INNER;
-        A.adj[f] = A.next[mc + cLinks[l][2]];
-    }
-    for (int side = 0; side < 4; ++side) {
-        const int sub = cSubfaceBySideChild[side][i];
-        if (sub < 0) continue;   // child i does not touch this side
-        const int pf = SI(A, ip, side);
-        const int bflag = A.bnd[pf];
-        if (bflag == B_UNDEF) continue;
-        const int chf = SI(A, ich, side);
-        A.bnd[chf] = bflag;
-        if (is_bnd(bflag)) {
-            A.adj[chf] = A.next[ich];
-            continue;
-        }
-        if (simple_face(A, ip, side)) {
-            const int jc = A.adj[pf];
-            if (lvl > A.level[jc]) A.adj[chf] = A.next[A.next[jc] + adjacent_child(side, zi & 3)];
-            else if (A.flag[jc] == 0) A.adj[chf] = A.next[jc];
-            else A.adj[chf] = A.next[A.next[jc] + adjacent_child(side, i)];
-        } else {
-            const int jc = A.adj[SI(A, ip, sub)];
-            const int fl = A.flag[jc];
-            if (fl == 0) A.adj[chf] = A.next[jc];
-            else if (fl < 0) A.adj[chf] = A.next[A.next[jc]];
-            else A.adj[chf] = jc;
-            if (A.level[jc] + fl > lvl + 1) {
-                split_face(A, ich, side);
-                const int neib_next = A.next[jc];
-                A.adj[SI(A, ich, side)] = A.next[neib_next + cNeibChild0[side]];
-                A.adj[SI(A, ich, side + 4)] = A.next[neib_next + cNeibChild0[side + 4]];
-            }
-        }
-    }
-}
-
-// SmFluid distributor split "slope" (sm_fluid.cpp:395-458) for child c of
-// parent ip: returns false if the reconstructed state is bad.
-__device__ bool split_slope(const AmrMesh& A, const DevState& z, const DevGrad& G, const Eos& eos, int ip, int c,
-                            Z& q) {
-    const Z zp{z.r[ip], z.u[ip], z.v[ip], z.p[ip], z.e[ip]};
-    const int st = A.stride;
-    const double gxr = G.dx[ip], gxu = G.dx[st + ip], gxv = G.dx[2 * st + ip], gxp = G.dx[3 * st + ip];
-    const double gyr = G.dy[ip], gyu = G.dy[st + ip], gyv = G.dy[2 * st + ip], gyp = G.dy[3 * st + ip];
+// SmFluid distributor split "slope" (sm_fluid.cpp:395-458) for a child
+// whose centroid is (drx, dry) from the parent's, from the parent's state zp
+// and limited gradient (gx, gy of rho, u, v, p): false if the reconstructed
+// state is bad.
+__device__ bool split_slope(const Z& zp, const double* gx, const double* gy, double drx, double dry,
+                            const Eos& eos, Z& q) {
     const double dR = (eos.g - 1.0) * (zp.e - eos.e0);
     const double dE = (eos.g - 1.0) * zp.r;
-    const double gex = (gxp - dR * gxr) / dE, gey = (gyp - dR * gyr) / dE;
-    const double drx = A.cx[c] - A.cx[ip], dry = A.cy[c] - A.cy[ip];
+    const double gex = (gx[3] - dR * gx[0]) / dE, gey = (gy[3] - dR * gy[0]) / dE;
     q = zp;
-    q.r = zp.r + gxr * drx + gyr * dry;
+    q.r = zp.r + gx[0] * drx + gy[0] * dry;
     const double ratio = zp.r / q.r;
-    q.u = zp.u + ratio * (gxu * drx + gyu * dry);
-    q.v = zp.v + ratio * (gxv * drx + gyv * dry);
+    q.u = zp.u + ratio * (gx[1] * drx + gy[1] * dry);
+    q.v = zp.v + ratio * (gx[2] * drx + gy[2] * dry);
     const double du = zp.u - q.u, dv = zp.v - q.v;
     q.e = zp.e - 0.5 * (du * du + dv * dv) + ratio * (gex * drx + gey * dry);
     q.p = eos_p(eos, q.r, q.e);
     return !is_bad(q, eos);
 }
 
-// coarse_cell / select_children / make_parent (coarse.h:95-404) and the
-// conservative merge (sm_fluid.cpp:382-393), for the family whose first
-// child (z_idx % 4 == 0) is ich.
-__device__ void coarse_family(const AmrMesh& A, const DevState& z, int* wait, const Eos& eos, int ich, int* fam) {
-    int ch[4];
-    ch[0] = ich;
-    int sibs[3];
-    get_siblings(A, ich, sibs);
-    for (int k = 0; k < 3; ++k) ch[A.z_idx[sibs[k]] & 3] = sibs[k];
-    if (fam)
-        for (int k = 0; k < 4; ++k) fam[k] = ch[k];
-    const int ip = A.next[ich];
-    set_cell(A, ip, vtx(A, ch[0], 0), vtx(A, ch[1], 2), vtx(A, ch[2], 6), vtx(A, ch[3], 8));
-    const int ip_next = A.next[ip];
-    A.idx[ip] = ip;
-    A.flag[ip] = 0;
-    A.b_idx[ip] = A.b_idx[ch[0]];
-    A.level[ip] = A.level[ch[0]] - 1;
-    A.z_idx[ip] = A.z_idx[ch[0]] / 4;
+// One child of refine_cell / make_children / link_siblings (refine.h:36-444):
+// child i of parent ip, written only into the child's own cell and slots,
+// plus (z.r not null) the child's state by the const or slope distributor
+// (sm_fluid.cpp:372-458), decided for all four children together (lanes
+// 4k..4k+3 of a wavefront hold one family).  The loads come in three
+// dependent batches before the first store: the parent's data and slots;
+// the new positions and the outer neighbours' level, flag and position;
+// then the positions of those neighbours' children or parents.
+__device__ void refine_one(const AmrMesh& A, const DevState& z, const DevGrad& Gd, int* wait, const Eos& eos,
+                           bool slope, int ip, int i) {
+    // batch 1: the parent
+    const int mc = A.next[ip];
+    const int lvl = A.level[ip], zi = A.z_idx[ip], bi = A.b_idx[ip];
+    const V2 q0 = vtx(A, ip, cCorners[i][0]), q1 = vtx(A, ip, cCorners[i][1]);
+    const V2 q2 = vtx(A, ip, cCorners[i][2]), q3 = vtx(A, ip, cCorners[i][3]);
+    int pb0[4], pb1[4], pa0[4], pas[4];
+#pragma unroll
     for (int side = 0; side < 4; ++side) {
-        const int some_ch = ch[cChildBySubface[side]];
-        const int scf = SI(A, some_ch, side);
-        const int pf = SI(A, ip, side);
-        A.bnd[pf] = A.bnd[scf];
-        if (is_bnd(A.bnd[scf])) {
-            A.adj[pf] = ip_next;
+        const int sub = cSubfaceBySideChild[side][i];
+        pb0[side] = A.bnd[SI(A, ip, side)];
+        pb1[side] = A.bnd[SI(A, ip, side + 4)];
+        pa0[side] = A.adj[SI(A, ip, side)];
+        pas[side] = A.adj[SI(A, ip, sub >= 0 ? sub : side)];
+    }
+    Z zp{};
+    double gx[4] = {0.0, 0.0, 0.0, 0.0}, gy[4] = {0.0, 0.0, 0.0, 0.0};
+    double pcx = 0.0, pcy = 0.0;
+    if (z.r) {
+        zp = Z{z.r[ip], z.u[ip], z.v[ip], z.p[ip], z.e[ip]};
+        if (slope) {
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                gx[k] = Gd.dx[k * A.stride + ip];
+                gy[k] = Gd.dy[k * A.stride + ip];
+            }
+            pcx = A.cx[ip];
+            pcy = A.cy[ip];
+        }
+    }
+    // batch 2: positions of the siblings and of the child itself; the outer
+    // neighbours (through the parent's side, or its sub-face on a split side)
+    const int ich = mc + i;
+    const int lk0 = A.next[mc + cLinks[2 * i][2]], lk1 = A.next[mc + cLinks[2 * i + 1][2]];
+    const int nself = A.next[ich];
+    int jc[4], jl[4], jf[4], jn[4];
+#pragma unroll
+    for (int side = 0; side < 4; ++side) {
+        const int j = pb1[side] == B_UNDEF ? pa0[side] : pas[side];
+        jc[side] = (unsigned)j < (unsigned)A.n ? j : ip;
+        jl[side] = A.level[jc[side]];
+        jf[side] = A.flag[jc[side]];
+        jn[side] = A.next[jc[side]];
+    }
+    // batch 3: the positions behind them (x0 / x1 < 0: not needed)
+    int y0[4], y1[4];
+    bool splits[4];
+#pragma unroll
+    for (int side = 0; side < 4; ++side) {
+        const bool simple = pb1[side] == B_UNDEF;
+        int x0 = -1, x1 = -1;
+        splits[side] = false;
+        if (simple) {
+            if (lvl > jl[side]) x0 = jn[side] + adjacent_child(side, zi & 3);
+            else if (jf[side] != 0) x0 = jn[side] + adjacent_child(side, i);
+        } else if (jl[side] + jf[side] > lvl + 1) {
+            splits[side] = true;
+            x0 = jn[side] + cNeibChild0[side];
+            x1 = jn[side] + cNeibChild0[side + 4];
+        } else if (jf[side] < 0) {
+            x0 = jn[side];
+        }
+        y0[side] = A.next[(unsigned)x0 < (unsigned)A.stride ? x0 : ip];
+        y1[side] = A.next[(unsigned)x1 < (unsigned)A.stride ? x1 : ip];
+    }
+
+    QuadG G;
+    quad_geom(q0, q1, q2, q3, G);
+    int b[8], a[8];
+#pragma unroll
+    for (int s = 0; s < 8; ++s) {
+        b[s] = s < 4 ? B_INNER : B_UNDEF;
+        a[s] = -1;
+    }
+    a[cLinks[2 * i][1]] = lk0;
+    a[cLinks[2 * i + 1][1]] = lk1;
+    unsigned split = 0u;
+#pragma unroll
+    for (int side = 0; side < 4; ++side) {
+        if (cSubfaceBySideChild[side][i] < 0) continue;   // child i does not touch this side
+        const int bflag = pb0[side];
+        if (bflag == B_UNDEF) continue;
+        b[side] = bflag;
+        if (is_bnd(bflag)) {
+            a[side] = nself;
             continue;
         }
-        const int sn = A.adj[scf];
-        const int wanted = A.level[sn] + A.flag[sn];
-        if (wanted <= A.level[ip]) {
-            A.adj[pf] = A.flag[sn] == 0 ? A.next[sn] : A.next[A.next[sn]];
-            slot_undef(A, ip, side + 4);
+        if (pb1[side] == B_UNDEF) {
+            a[side] = (lvl > jl[side] || jf[side] != 0) ? y0[side] : jn[side];
+        } else if (splits[side]) {
+            b[side + 4] = bflag;
+            a[side] = y0[side];
+            a[side + 4] = y1[side];
+            split |= 1u << side;
+        } else {
+            a[side] = jf[side] == 0 ? jn[side] : (jf[side] < 0 ? y0[side] : jc[side]);
+        }
+    }
+    write_cell(A, ich, G, bi, lvl + 1, 4 * zi + i);
+    write_slots(A, ich, G, b, a, split);
+    if (z.r) {
+        Z q;
+        bool ok = false;
+        if (slope) ok = split_slope(zp, gx, gy, G.C.x - pcx, G.C.y - pcy, eos, q);
+        const unsigned long long m = __ballot(!ok);
+        const bool any_bad = ((m >> (threadIdx.x & 60)) & 0xFULL) != 0ULL;
+        if (any_bad) q = zp;
+        z.r[ich] = q.r; z.u[ich] = q.u; z.v[ich] = q.v; z.p[ich] = q.p; z.e[ich] = q.e;
+        wait[ich] = 0;
+    }
+    if (i == 0) A.idx[ip] = -1;
+}
+
+// coarse_cell / select_children / make_parent (coarse.h:95-404) and the
+// conservative merge (sm_fluid.cpp:382-393) for the family ch[0..3]
+// (children in z order, from the positions pass).  Loads in three dependent
+// batches (the children; the parent's position and the outer neighbours;
+// the positions behind them), then the parent's cell and slots.
+__device__ void coarse_one(const AmrMesh& A, const DevState& z, int* wait, const Eos& eos, const int* ch) {
+    // batch 1: the children
+    const int ip = A.next[ch[0]];
+    const V2 q0 = vtx(A, ch[0], 0), q1 = vtx(A, ch[1], 2), q2 = vtx(A, ch[2], 6), q3 = vtx(A, ch[3], 8);
+    const int lc = A.level[ch[0]], zc0 = A.z_idx[ch[0]], bc = A.b_idx[ch[0]];
+    int sb[4], sa[4], sa2[4];
+#pragma unroll
+    for (int side = 0; side < 4; ++side) {
+        const int c1 = ch[cChildBySubface[side]], c2 = ch[cChildBySubface[side + 4]];
+        sb[side] = A.bnd[SI(A, c1, side)];
+        sa[side] = A.adj[SI(A, c1, side)];
+        sa2[side] = A.adj[SI(A, c2, side)];
+    }
+    Z zk[4];
+    double Vk[4];
+    if (z.r) {
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const int c = ch[k];
+            zk[k] = Z{z.r[c], z.u[c], z.v[c], z.p[c], z.e[c]};
+            Vk[k] = A.vol[c];
+        }
+    }
+    // batch 2: the parent's final position, the neighbours across each side
+    const int ip_next = A.next[ip];
+    int nl[4], nf[4], nn[4], ml[4], mf[4], mn[4];
+#pragma unroll
+    for (int side = 0; side < 4; ++side) {
+        const int j = (unsigned)sa[side] < (unsigned)A.n ? sa[side] : ch[0];
+        const int k = (unsigned)sa2[side] < (unsigned)A.n ? sa2[side] : ch[0];
+        nl[side] = A.level[j];
+        nf[side] = A.flag[j];
+        nn[side] = A.next[j];
+        ml[side] = A.level[k];
+        mf[side] = A.flag[k];
+        mn[side] = A.next[k];
+    }
+    // batch 3: the positions behind them
+    const int lp = lc - 1;
+    int y0[4], y1[4];
+#pragma unroll
+    for (int side = 0; side < 4; ++side) {
+        int x0 = -1, x1 = -1;
+        if (nl[side] + nf[side] <= lp) {
+            if (nf[side] != 0) x0 = nn[side];
+        } else {
+            if (lc > nl[side]) x0 = nn[side] + cNeibChild0[side];
+            else if (nf[side] != 0) x0 = nn[side];
+            if (lc > ml[side]) x1 = mn[side] + cNeibChild0[side + 4];
+            else if (mf[side] != 0) x1 = mn[side];
+        }
+        y0[side] = A.next[(unsigned)x0 < (unsigned)A.stride ? x0 : ip];
+        y1[side] = A.next[(unsigned)x1 < (unsigned)A.stride ? x1 : ip];
+    }
+
+    QuadG G;
+    quad_geom(q0, q1, q2, q3, G);
+    int b[8], a[8];
+#pragma unroll
+    for (int s = 0; s < 8; ++s) {
+        b[s] = s < 4 ? B_INNER : B_UNDEF;
+        a[s] = -1;
+    }
+    unsigned split = 0u;
+#pragma unroll
+    for (int side = 0; side < 4; ++side) {
+        b[side] = sb[side];
+        if (is_bnd(sb[side])) {
+            a[side] = ip_next;
             continue;
         }
-        split_face(A, ip, side);
-        for (int sub = side; sub < 8; sub += 4) {
-            const int c2 = ch[cChildBySubface[sub]];
-            const int jc = A.adj[SI(A, c2, side)];
-            const int sf = SI(A, ip, sub);
-            if (A.level[c2] > A.level[jc]) A.adj[sf] = A.next[A.next[jc] + cNeibChild0[sub]];
-            else if (A.flag[jc] == 0) A.adj[sf] = A.next[jc];
-            else A.adj[sf] = A.next[A.next[jc]];
+        if (nl[side] + nf[side] <= lp) {
+            a[side] = nf[side] == 0 ? nn[side] : y0[side];
+            continue;
         }
+        b[side + 4] = sb[side];
+        a[side] = (lc > nl[side] || nf[side] != 0) ? y0[side] : nn[side];
+        a[side + 4] = (lc > ml[side] || mf[side] != 0) ? y1[side] : mn[side];
+        split |= 1u << side;
     }
-    if (!z.r) {   // mesh only: the solver's data are merged by its own kernel
-        for (int k = 0; k < 4; ++k) A.idx[ch[k]] = -1;
-        return;
+    write_cell(A, ip, G, bc, lc - 1, zc0 / 4);
+    write_slots(A, ip, G, b, a, split);
+    if (z.r) {
+        // merge: q_p = sum_k q(child_k) V_k / V_p
+        double q[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const Q qc = to_q(zk[k]);
+            q[0] = q[0] + qc.r * Vk[k];
+            q[1] = q[1] + qc.mx * Vk[k];
+            q[2] = q[2] + qc.my * Vk[k];
+            q[3] = q[3] + qc.E * Vk[k];
+        }
+        const double Vp = G.vol;
+        const Z zp = to_z(Q{q[0] / Vp, q[1] / Vp, q[2] / Vp, q[3] / Vp}, eos);
+        z.r[ip] = zp.r; z.u[ip] = zp.u; z.v[ip] = zp.v; z.p[ip] = zp.p; z.e[ip] = zp.e;
+        wait[ip] = 0;
     }
-    // merge: q_p = sum_k q(child_k) V_k / V_p
-    double q[4] = {0.0, 0.0, 0.0, 0.0};
-    for (int k = 0; k < 4; ++k) {
-        const int c = ch[k];
-        const Q qc = to_q(Z{z.r[c], z.u[c], z.v[c], z.p[c], z.e[c]});
-        const double V = A.vol[c];
-        q[0] = q[0] + qc.r * V;
-        q[1] = q[1] + qc.mx * V;
-        q[2] = q[2] + qc.my * V;
-        q[3] = q[3] + qc.E * V;
-    }
-    const double Vp = A.vol[ip];
-    const Z zp = to_z(Q{q[0] / Vp, q[1] / Vp, q[2] / Vp, q[3] / Vp}, eos);
-    z.r[ip] = zp.r; z.u[ip] = zp.u; z.v[ip] = zp.v; z.p[ip] = zp.p; z.e[ip] = zp.e;
-    wait[ip] = 0;
+#pragma unroll
     for (int k = 0; k < 4; ++k) A.idx[ch[k]] = -1;
 }
 
@@ -610,13 +731,12 @@ __device__ void coarse_family(const AmrMesh& A, const DevState& z, int* wait, co
 // blocks_retain workgroups take the retained cells (one thread each, flag 0),
 // the next blocks_refine the refined cells (a lane quad per parent, one lane
 // per child; the slope distributor's all-or-nothing fallback is decided
-// across the quad), the rest the coarsened families (one thread per family).
-// With z.r null the kernel does the mesh only, and `fam` receives the four
-// children of each coarsened family for the solver's own merge.
+// across the quad), the rest the coarsened families (one thread per family,
+// its children from `fam`).  With z.r null the kernel does the mesh only.
 __global__ void __launch_bounds__(256) k_amr_geometry(AmrMesh A, DevState z, DevGrad G, int* wait, Eos eos,
                                                       int slope, const int* ref_list, int n_refine,
-                                                      const int* crs_list, int n_parents, int blocks_retain,
-                                                      int blocks_refine, int* fam) {
+                                                      const int* fam, int n_parents, int blocks_retain,
+                                                      int blocks_refine) {
     const int b = (int)blockIdx.x;
     if (b < blocks_retain) {
         const int ic = b * 256 + (int)threadIdx.x;
@@ -628,27 +748,15 @@ __global__ void __launch_bounds__(256) k_amr_geometry(AmrMesh A, DevState z, Dev
         const int t = (b - blocks_retain) * 256 + (int)threadIdx.x;
         const int pi = t >> 2, i = t & 3;
         if (pi >= n_refine) return;   // whole quads leave together
-        const int ip = ref_list[pi];
-        refine_child(A, ip, i);
-        const int c = A.next[ip] + i;
-        if (!z.r) {
-            if (i == 0) A.idx[ip] = -1;
-            return;
-        }
-        Z q;
-        bool ok = false;
-        if (slope) ok = split_slope(A, z, G, eos, ip, c, q);
-        const unsigned long long m = __ballot(!ok);
-        const bool any_bad = ((m >> (threadIdx.x & 60)) & 0xFULL) != 0ULL;
-        if (any_bad) q = Z{z.r[ip], z.u[ip], z.v[ip], z.p[ip], z.e[ip]};
-        z.r[c] = q.r; z.u[c] = q.u; z.v[c] = q.v; z.p[c] = q.p; z.e[c] = q.e;
-        wait[c] = 0;
-        if (i == 0) A.idx[ip] = -1;
+        refine_one(A, z, G, wait, eos, slope != 0, ref_list[pi], i);
         return;
     }
     const int t = (b - blocks_retain - blocks_refine) * 256 + (int)threadIdx.x;
     if (t >= n_parents) return;
-    coarse_family(A, z, wait, eos, crs_list[t], fam ? fam + 4 * t : nullptr);
+    int ch[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) ch[k] = fam[4 * t + k];
+    coarse_one(A, z, wait, eos, ch);
 }
 
 // SwapLists::move_elements (swap_lists.h:129-138) + AmrCells::move_item
@@ -739,43 +847,55 @@ __global__ void __launch_bounds__(256) k_amr_derive(AmrMesh A, DevState z, Eos e
     double cand = INFINITY;
     if (c < A.n) {
         const double kNX[4] = {-1.0, 1.0, 0.0, 0.0}, kNY[4] = {0.0, 0.0, -1.0, 1.0};
-        // loads in three dependent batches (slot ids; the defined slots'
-        // data; the inner neighbours' centres), then the stores: the
-        // weights are stored into an array of the loads' type
+        // loads in two dependent batches, every load unconditional (the
+        // slot arrays are read for undefined slots too, and a neighbour
+        // index out of range reads the cell itself), the slot types
+        // selecting afterwards: the cell's own and slot data, then the
+        // neighbours' centres; then the stores (the weights are stored into
+        // an array of the loads' type)
         const double cx = A.cx[c], cy = A.cy[c];
         const V2 va = vtx(A, c, 5), vb = vtx(A, c, 3), vp = vtx(A, c, 7), vq = vtx(A, c, 1);
         Z zc{};
         if (z.r) zc = Z{z.r[c], z.u[c], z.v[c], z.p[c], z.e[c]};
-        int t[8], nb[8];
+        int t[8], nb[8], bl[8], al[8];
         double nx[8], ny[8], fcx[8], fcy[8], area[8], px[8], py[8];
 #pragma unroll
         for (int s = 0; s < 8; ++s) {
-            switch (A.bnd[SI(A, c, s)]) {
+            const int f = SI(A, c, s);
+            bl[s] = A.bnd[f];
+            al[s] = A.adj[f];
+            nx[s] = A.nx[f];
+            ny[s] = A.ny[f];
+            fcx[s] = A.fcx[f];
+            fcy[s] = A.fcy[f];
+            area[s] = A.area[f];
+        }
+#pragma unroll
+        for (int s = 0; s < 8; ++s) {
+            switch (bl[s]) {
                 case B_UNDEF: t[s] = SC_UNDEF; break;
                 case B_INNER: t[s] = SC_INNER; break;
                 case B_PERIODIC: t[s] = SC_PERIODIC; break;
                 case 1: t[s] = SC_WALL; break;
                 default: t[s] = SC_COPY; break;
             }
-        }
-#pragma unroll
-        for (int s = 0; s < 8; ++s) {
-            const int f = SI(A, c, s);
             const bool def = t[s] != SC_UNDEF;
-            nb[s] = def ? A.adj[f] : c;
-            nx[s] = def ? A.nx[f] : 0.0;
-            ny[s] = def ? A.ny[f] : 0.0;
-            fcx[s] = def ? A.fcx[f] : 0.0;
-            fcy[s] = def ? A.fcy[f] : 0.0;
-            area[s] = def ? A.area[f] : 0.0;
+            nb[s] = def ? al[s] : c;
+            nx[s] = def ? nx[s] : 0.0;
+            ny[s] = def ? ny[s] : 0.0;
+            fcx[s] = def ? fcx[s] : 0.0;
+            fcy[s] = def ? fcy[s] : 0.0;
+            area[s] = def ? area[s] : 0.0;
         }
         bool ok[8];
 #pragma unroll
         for (int s = 0; s < 8; ++s) {
             ok[s] = t[s] != SC_UNDEF && nb[s] >= 0 && nb[s] < A.n && !(t[s] >= SC_WALL && nb[s] != c);
             const bool inner = ok[s] && t[s] == SC_INNER;
-            px[s] = inner ? A.cx[nb[s]] : 0.0;
-            py[s] = inner ? A.cy[nb[s]] : 0.0;
+            const int j = inner ? nb[s] : c;
+            const double qx = A.cx[j], qy = A.cy[j];
+            px[s] = inner ? qx : 0.0;
+            py[s] = inner ? qy : 0.0;
         }
         int cw = 0;
 #pragma unroll
@@ -886,10 +1006,10 @@ hipError_t amr_mark(const AmrMesh& A, Marks3* marks, hipStream_t s) {
 }
 
 hipError_t amr_positions(const AmrMesh& A, int n_large, int n_short, int n_parents, const Marks3* rank,
-                         int* hole_at, int* ref_list, int* crs_list, unsigned long long* faces,
+                         int* hole_at, int* ref_list, int* crs_list, int* fam, unsigned long long* faces,
                          unsigned long long* dt_rd, unsigned long long* dt_wr, hipStream_t s) {
     hipLaunchKernelGGL(k_amr_positions, grid256(n_large), dim3(256), 0, s, A, n_large, n_short, n_parents, rank,
-                       hole_at, ref_list, crs_list, faces, dt_rd, dt_wr);
+                       hole_at, ref_list, crs_list, fam, faces, dt_rd, dt_wr);
     return hipGetLastError();
 }
 
@@ -902,11 +1022,11 @@ hipError_t amr_swapmap(const AmrMesh& A, int n_short, int n_large, int nz_total,
 }
 
 hipError_t amr_geometry(const AmrMesh& A, const DevState& z, const DevGrad& G, int* wait, const Eos& eos,
-                        bool slope, const int* ref_list, int n_refine, const int* crs_list, int n_parents,
-                        int* fam, hipStream_t s) {
+                        bool slope, const int* ref_list, int n_refine, const int* fam, int n_parents,
+                        hipStream_t s) {
     const int br = (A.n + 255) / 256, bf = (4 * n_refine + 255) / 256, bc = (n_parents + 255) / 256;
     hipLaunchKernelGGL(k_amr_geometry, dim3(br + bf + bc), dim3(256), 0, s, A, z, G, wait, eos, slope ? 1 : 0,
-                       ref_list, n_refine, crs_list, n_parents, br, bf, fam);
+                       ref_list, n_refine, fam, n_parents, br, bf);
     return hipGetLastError();
 }
 

@@ -110,7 +110,7 @@ struct zfv_ctx {
     DBuf<unsigned long long> s_faces;
     DBuf<int> s_hole_at, s_ref_list, s_crs_list, s_swap_src, s_swap_dst;
     DBuf<Marks3> s_tiles;             // tile sums of the marks scan
-    DBuf<int> s_fam;                  // children of each coarsened family (multi-material merge)
+    DBuf<int> s_fam;                  // children of each coarsened family in z order (amr_positions)
     Marks3* marks_total = nullptr;    // pinned host: totals of the marks scan
     bool faces_pending = false;   // n_faces still on the device (s_faces)
     // device state
@@ -239,7 +239,8 @@ struct zfv_ctx {
     // observes or changes per-cell data (downloads, uploads, flags,
     // adaptation, totals, the adjoint) puts them back in the mesh's order.
     struct CurveOrder {
-        int mode = 1;             // 0: always the mesh's order; 1: curve order for steps of refined meshes
+        int mode = 1;             // 0: always the mesh's order; 1: curve order for steps of refined meshes,
+                                  // carried across adaptations; 2: as 1, rebuilt in full after each adaptation
         bool refined = false;     // the mesh has refined cells (curve order differs from row order)
         bool keys_ok = false;     // levels, z and base indices of the uploaded mesh form valid curve keys
         int lmax = 0;             // finest level the keys resolve
@@ -247,11 +248,13 @@ struct zfv_ctx {
         long long n_base = 0;     // base cells (largest b_idx + 1)
         bool slots_clean = false; // the dense slot array holds -1 everywhere
         long long epoch = -1;     // mesh epoch of the sorted copy
+        long long perm_epoch = -1;  // mesh epoch whose order perm / iperm already hold (carried by zfv_refine)
         int cap = 0;              // stride of the copy (= the mesh's capacity when built)
         bool state = false;       // state layer cur and wait[wcur] in curve order
         bool grad = false;        // gdx / gdy last written in curve order
         DBuf<double> cx, cy, vol, h, area, fcx, fcy, w, nx, ny, gtmp;
         DBuf<int> code, adj, perm, iperm, vals, slots, counts, offsets, tiles;
+        DBuf<int> ccount, coffset, ctiles;   // the order carried across an adaptation (curve_counts)
         DBuf<unsigned long long> keys[2];
         DBuf<unsigned char> tmp;
     } co;
@@ -635,6 +638,7 @@ int upload_mesh(zfv_ctx* ctx) {
     ctx->co.state = false;
     ctx->co.grad = false;
     ctx->co.epoch = -1;
+    ctx->co.perm_epoch = -1;
     return ZFV_OK;
 }
 
@@ -887,14 +891,18 @@ int build_curve(zfv_ctx* ctx) {
         for (auto* b : {&co.perm, &co.iperm, &co.vals}) ZFV_HIP(b->alloc(cap));
         for (auto& k : co.keys) ZFV_HIP(k.alloc(cap));
         co.cap = cap;
+        co.perm_epoch = -1;
     }
-    // The order: dense slots (one per finest-level Morton cell of every base
-    // cell) compacted in order when they are at most 64 per cell, else a
-    // radix sort of the same keys.
+    // The order: carried across the last adaptation (zfv_refine), or dense
+    // slots (one per finest-level Morton cell of every base cell) compacted
+    // in order when they are at most 64 per cell, else a radix sort of the
+    // same keys.
     const long long n_slots = co.n_base << (2 * co.lmax);
     const long long n_chunks = (n_slots + 63) / 64;
     size_t bytes = 0;
-    if (n_slots <= 64LL * std::max(n, 1) && n_chunks * 64 < (1LL << 31)) {
+    if (co.perm_epoch == ctx->mesh_epoch) {
+        // perm / iperm hold this mesh's order already
+    } else if (n_slots <= 64LL * std::max(n, 1) && n_chunks * 64 < (1LL << 31)) {
         if (co.slots.n < (size_t)(n_chunks * 64)) {
             ZFV_HIP(co.slots.alloc((size_t)(n_chunks * 64)));
             co.slots_clean = false;
@@ -1416,8 +1424,15 @@ int zfv_mesh_reserve(zfv_ctx* ctx, int n_cells) {
         for (auto& k : co.keys) ZFV_HIP(k.alloc(cap));
         co.cap = (int)cap;
         co.epoch = -1;
+        co.perm_epoch = -1;
     }
     if (co.mode != 0 && co.keys_ok && co.gtmp.n < 4 * cap) ZFV_HIP(co.gtmp.alloc(4 * cap));
+    if (co.mode == 1 && co.keys_ok) {
+        for (auto* b : {&co.ccount, &co.coffset})
+            if (b->n < nr) ZFV_HIP(b->alloc(nr));
+        const size_t ct = (size_t)std::max(curve_counts_tiles((int)nr), 1);
+        if (co.ctiles.n < ct) ZFV_HIP(co.ctiles.alloc(ct));
+    }
     return ZFV_OK;
 }
 
@@ -1763,7 +1778,8 @@ int zfv_graph_enable(zfv_ctx* ctx, int on) {
 int zfv_set_cell_order(zfv_ctx* ctx, int mode) {
     int r = check_ctx(ctx);
     if (r) return r;
-    if (mode != 0 && mode != 1) return fail(ZFV_ERR_ARG, "cell order must be 0 (mesh) or 1 (curve for steps)");
+    if (mode < 0 || mode > 2)
+        return fail(ZFV_ERR_ARG, "cell order must be 0 (mesh), 1 (curve for steps) or 2 (curve, rebuilt in full)");
     if (mode == 0 && ctx->have_mesh) {
         r = to_mesh_order(ctx);
         if (r) return r;
@@ -1772,6 +1788,22 @@ int zfv_set_cell_order(zfv_ctx* ctx, int mode) {
     }
     ctx->co.mode = mode;
     ctx->graphs_clear();
+    return ZFV_OK;
+}
+
+int zfv_get_curve_order(zfv_ctx* ctx, int32_t* perm) {
+    int r = check_ctx(ctx);
+    if (r) return r;
+    if (!perm) return fail(ZFV_ERR_ARG, "null output");
+    if (!ctx->have_mesh) return fail(ZFV_ERR_STATE, "no mesh uploaded");
+    auto& co = ctx->co;
+    if (co.mode == 0 || !co.refined || ctx->nseg > 1) return fail(ZFV_ERR_STATE, "the steps run in the mesh's order");
+    if (co.epoch != ctx->mesh_epoch || co.cap != ctx->cap) {
+        r = build_curve(ctx);
+        if (r) return r;
+    }
+    ZFV_HIP(hipMemcpyAsync(perm, co.perm.p, (size_t)ctx->hm.n * sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    ZFV_HIP(hipStreamSynchronize(ctx->stream));
     return ZFV_OK;
 }
 
@@ -1997,19 +2029,38 @@ static int refine_impl(zfv_ctx* ctx, int max_level, int mode) {
     for (auto* b : {&ctx->s_hole_at, &ctx->s_ref_list, &ctx->s_crs_list, &ctx->s_swap_src, &ctx->s_swap_dst})
         if (b->n < (size_t)std::max(n, 1)) ZFV_HIP(b->alloc(std::max(n, 1)));
     if (ctx->s_faces.n < 2) ZFV_HIP(ctx->s_faces.alloc(2));
-    if (mm && ctx->s_fam.n < 4 * (size_t)std::max(n_parents, 1)) ZFV_HIP(ctx->s_fam.alloc(4 * (size_t)std::max(n_parents, 1)));
+    if (ctx->s_fam.n < 4 * (size_t)std::max(n_parents, 1)) ZFV_HIP(ctx->s_fam.alloc(4 * (size_t)std::max(n_parents, 1)));
 
     AmrMesh A = ctx->amesh();   // A.n = n (cells before adaptation)
     const int rd = (int)(ctx->step_no % 3), wr = (int)((ctx->step_no + 1) % 3);
     ZFV_HIP(amr_positions(A, n_large, n_short, n_parents, ctx->s_rank.p, ctx->s_hole_at.p, ctx->s_ref_list.p,
-                          ctx->s_crs_list.p, ctx->s_faces.p, ctx->dtmin.p + rd, ctx->dtmin.p + wr, s));
+                          ctx->s_crs_list.p, ctx->s_fam.p, ctx->s_faces.p, ctx->dtmin.p + rd, ctx->dtmin.p + wr, s));
     ZFV_HIP(amr_swapmap(A, n_short, n_large, nz_total, ctx->s_rank.p, ctx->s_hole_at.p, ctx->s_swap_src.p,
                         ctx->s_swap_dst.p, ctx->err.p, s));
+    // The steps' curve order after this adaptation, carried from the one
+    // before it while the balanced flags and the final positions are at
+    // hand (before the moves overwrite flags): no rebuild at the next step.
+    bool carried = false;
+    {
+        auto& co = ctx->co;
+        co.perm_epoch = -1;
+        if (!mm && co.mode == 1 && co.keys_ok && co.refined && co.epoch == ctx->mesh_epoch && co.cap == ctx->cap &&
+            co.vals.n >= (size_t)n_short) {
+            for (auto* b : {&co.ccount, &co.coffset})
+                if (b->n < (size_t)n) ZFV_HIP(b->alloc(n));
+            const size_t ct = (size_t)std::max(curve_counts_tiles(n), 1);
+            if (co.ctiles.n < ct) ZFV_HIP(co.ctiles.alloc(ct));
+            ZFV_HIP(curve_counts(co.perm.p, n, A.flag, A.z_idx, co.ccount.p, co.coffset.p, co.ctiles.p, s));
+            ZFV_HIP(curve_expand(co.perm.p, n, A.flag, A.z_idx, A.next, co.coffset.p, co.vals.p, co.iperm.p, s));
+            std::swap(co.perm, co.vals);
+            carried = true;
+        }
+    }
     const DevState z = mm ? DevState{} : ctx->layer(ctx->cur);
     int* wait = mm ? nullptr : ctx->wait[ctx->wcur].p;
     DevGrad G{ctx->gdx.p, ctx->gdy.p};
     ZFV_HIP(amr_geometry(A, z, G, wait, ctx->eos(), mode == kRefineSlope, ctx->s_ref_list.p, n_refine,
-                         ctx->s_crs_list.p, n_parents, mm ? ctx->s_fam.p : nullptr, s));
+                         ctx->s_fam.p, n_parents, s));
     const size_t cap = (size_t)ctx->cap;
     if (mm)
         ZFV_HIP(mm_distribute(A, mm_layer(ctx, S.cur), S.mix, S.gx.p, S.gy.p, ctx->s_ref_list.p, n_refine,
@@ -2026,6 +2077,7 @@ static int refine_impl(zfv_ctx* ctx, int max_level, int mode) {
     ctx->faces_pending = true;
     ctx->graphs_clear();   // the cell count changed
     ctx->mesh_epoch += 1;
+    ctx->co.perm_epoch = carried ? ctx->mesh_epoch : -1;
     ctx->tiles.ok = false;   // no longer a plain Rectangle
     ctx->co.refined = ctx->co.keys_ok;
     ctx->co.lmax = std::max(ctx->co.lmax, max_level);

@@ -104,17 +104,17 @@ struct Marks3 {
 int amr_scan3_tiles(int count);
 hipError_t amr_scan3(const Marks3* in, Marks3* out, Marks3* tile_sums, int count, Marks3* total, hipStream_t s);
 hipError_t amr_mark(const AmrMesh& A, Marks3* marks, hipStream_t s);
+// fam receives the four children of each coarsened family in z order.
 hipError_t amr_positions(const AmrMesh& A, int n_large, int n_short, int n_parents, const Marks3* rank,
-                         int* hole_at, int* ref_list, int* crs_list, unsigned long long* faces,
+                         int* hole_at, int* ref_list, int* crs_list, int* fam, unsigned long long* faces,
                          unsigned long long* dt_rd, unsigned long long* dt_wr, hipStream_t s);
 hipError_t amr_swapmap(const AmrMesh& A, int n_short, int n_large, int nz_total, const Marks3* rank,
                        const int* hole_at, int* swap_src, int* swap_dst, int* err, hipStream_t s);
 // z.r null: mesh only (the solver's data are distributed and moved by its
-// own kernels); fam (may be null) receives the four children of each
-// coarsened family.
+// own kernels); fam: the coarsened families of amr_positions.
 hipError_t amr_geometry(const AmrMesh& A, const DevState& z, const DevGrad& G, int* wait, const Eos& eos,
-                        bool slope, const int* ref_list, int n_refine, const int* crs_list, int n_parents,
-                        int* fam, hipStream_t s);
+                        bool slope, const int* ref_list, int n_refine, const int* fam, int n_parents,
+                        hipStream_t s);
 hipError_t amr_move(const AmrMesh& A, const DevState& z, int* wait, int n_short, int nz_total, const Marks3* rank,
                     int bound, const int* swap_src, const int* swap_dst, hipStream_t s);
 hipError_t amr_derive(const AmrMesh& A, const DevState& z, const Eos& eos, unsigned long long* counts,
@@ -143,6 +143,16 @@ int curve_scan_tiles(int n_chunks);
 hipError_t curve_compact(int* slots, int n_chunks, int* counts, int* offsets, int* tile_sums, int* perm, int* iperm,
                          hipStream_t s);
 hipError_t curve_inverse(const int* perm, int* iperm, int n, hipStream_t s);
+// The order after one adaptation from the order before it (perm of the n
+// cells before), with the adaptation's balanced flags, z indices and final
+// positions (next): curve_counts before the cells move (counts and their
+// exclusive scan, tile_sums: curve_counts_tiles(n) ints), curve_expand after
+// the swap map, into perm_new / iperm_new (not aliasing perm).
+int curve_counts_tiles(int n);
+hipError_t curve_counts(const int* perm, int n, const int* flag, const int* z_idx, int* counts, int* offsets,
+                        int* tile_sums, hipStream_t s);
+hipError_t curve_expand(const int* perm, int n, const int* flag, const int* z_idx, const int* next, const int* offsets,
+                        int* perm_new, int* iperm_new, hipStream_t s);
 // Radix sort of (key, value) pairs on bits [0, end_bit); tmp == nullptr
 // sizes the scratch.
 hipError_t curve_sort(const unsigned long long* keys_in, unsigned long long* keys_out, const int* vals_in,

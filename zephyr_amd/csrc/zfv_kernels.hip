@@ -1154,11 +1154,13 @@ hipError_t launch_grad_pred(int limiter, bool predictor, int layout, int block, 
         dim3 g(grid_cells(M.n, block / 4));
         if (predictor) launch_gp_t<true, kLayoutQuad>(limiter, g, b, s, M, in, G, half, wait, eos, ctl, nullptr);
         else launch_gp_t<false, kLayoutQuad>(limiter, g, b, s, M, in, G, half, wait, eos, ctl, raw);
-    } else if (layout == kLayoutBatch) {
+    } else if (layout == kLayoutBatch && predictor) {
         dim3 g(grid_cells(M.n, block));
-        if (predictor) launch_gp_t<true, kLayoutBatch>(limiter, g, b, s, M, in, G, half, wait, eos, ctl, nullptr);
-        else launch_gp_t<false, kLayoutBatch>(limiter, g, b, s, M, in, G, half, wait, eos, ctl, raw);
+        launch_gp_t<true, kLayoutBatch>(limiter, g, b, s, M, in, G, half, wait, eos, ctl, nullptr);
     } else {
+        // without the predictor (the refinement criterion's gradient, on the
+        // mesh's own order) the one-thread-per-cell kernel measures faster:
+        // it keeps three waves per SIMD against the batched kernel's two
         dim3 g(grid_cells(M.n, block));
         if (predictor) launch_gp_t<true, kLayoutCell>(limiter, g, b, s, M, in, G, half, wait, eos, ctl, nullptr);
         else launch_gp_t<false, kLayoutCell>(limiter, g, b, s, M, in, G, half, wait, eos, ctl, raw);
