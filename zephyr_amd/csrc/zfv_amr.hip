@@ -289,11 +289,19 @@ __global__ void __launch_bounds__(256) k_amr_round(AmrMesh A, int lvl) {
 // positions of children and parents, the rank of each hole (a changed cell
 // below n_short) and, in closed form, the number of surviving positions
 // above each position >= n_short.
-__global__ void __launch_bounds__(256) k_amr_mark(AmrMesh A, Marks3* marks) {
+// cc (perm not null): the entries the cell at curve position ic becomes in
+// the carried curve order (see k_amr_geometry).
+__global__ void __launch_bounds__(256) k_amr_mark(AmrMesh A, AmrMarks* marks, const int* __restrict__ perm) {
     const int ic = cell_id();
     if (ic >= A.n) return;
     const int f = A.flag[ic];
-    marks[ic] = Marks3{f > 0 ? 1 : 0, (f < 0 && (A.z_idx[ic] & 3) == 0) ? 1 : 0, f != 0 ? 1 : 0};
+    int cc = 0;
+    if (perm) {
+        const int o = perm[ic];
+        const int fo = A.flag[o];
+        cc = fo > 0 ? 4 : (fo < 0 ? ((A.z_idx[o] & 3) == 0 ? 1 : 0) : 1);
+    }
+    marks[ic] = AmrMarks{f > 0 ? 1 : 0, (f < 0 && (A.z_idx[ic] & 3) == 0) ? 1 : 0, f != 0 ? 1 : 0, cc};
 }
 
 // Positions: retained cells keep their index, parents go to n + rank,
@@ -302,7 +310,7 @@ __global__ void __launch_bounds__(256) k_amr_mark(AmrMesh A, Marks3* marks) {
 // hole table (k-th hole ascending).  Thread 0 clears the
 // face counters and the dt slots the derive pass fills.
 __global__ void __launch_bounds__(256) k_amr_positions(AmrMesh A, int n_large, int n_short, int n_parents,
-                                                       const Marks3* rank, int* hole_at, int* ref_list,
+                                                       const AmrMarks* rank, int* hole_at, int* ref_list,
                                                        int* crs_list, int* fam, unsigned long long* faces,
                                                        unsigned long long* dt_rd, unsigned long long* dt_wr) {
     const int ic = cell_id();
@@ -319,7 +327,7 @@ __global__ void __launch_bounds__(256) k_amr_positions(AmrMesh A, int n_large, i
         return;
     }
     const int f = A.flag[ic];
-    const Marks3 rk = rank[ic];
+    const AmrMarks rk = rank[ic];
     if (f == 0) {
         A.next[ic] = ic;
     } else if (f > 0) {
@@ -343,7 +351,7 @@ __global__ void __launch_bounds__(256) k_amr_positions(AmrMesh A, int n_large, i
 // every position survives; for p < n the survivors above p are the new
 // cells plus the retained cells in (p, n).  n_swap = number of holes.
 __global__ void __launch_bounds__(256) k_amr_swapmap(AmrMesh A, int n_short, int n_large, int nz_total,
-                                                     const Marks3* rank, const int* hole_at, int* swap_src,
+                                                     const AmrMarks* rank, const int* hole_at, int* swap_src,
                                                      int* swap_dst, int* err) {
     const int p = n_short + cell_id();
     if (p >= n_large) return;
@@ -727,17 +735,54 @@ __device__ void coarse_one(const AmrMesh& A, const DevState& z, int* wait, const
 
 }  // namespace
 
+// The steps' curve order carried across the adaptation.  Refining a cell
+// puts its four children, in z order, exactly where the cell was on the
+// curve; coarsening a family puts the parent where its four children were
+// (the first of them, z_idx % 4 == 0, leads); every other cell keeps its
+// place.  So the new order is the old one with each entry expanded to the
+// cc entries of its marks, at the scanned position rank[j].cc, and each
+// entry's mesh index is the final position the adaptation gives it:
+// next[o] for a kept cell, next[next[o] + k] for child k, next[next[o]] for
+// a family's parent.
+__device__ void carry_curve(const AmrMesh& A, const int* perm, const AmrMarks* rank, int* perm_new, int* iperm_new,
+                            int j) {
+    const int o = perm[j];
+    const int f = A.flag[o];
+    const int at = rank[j].cc;
+    if (f > 0) {
+        const int mc = A.next[o];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const int p = A.next[mc + k];
+            perm_new[at + k] = p;
+            iperm_new[p] = at + k;
+        }
+    } else if (f == 0 || (A.z_idx[o] & 3) == 0) {
+        const int p = f == 0 ? A.next[o] : A.next[A.next[o]];
+        perm_new[at] = p;
+        iperm_new[p] = at;
+    }
+}
+
 // One launch for the three kinds of cells, by workgroup: the first
 // blocks_retain workgroups take the retained cells (one thread each, flag 0),
 // the next blocks_refine the refined cells (a lane quad per parent, one lane
 // per child; the slope distributor's all-or-nothing fallback is decided
-// across the quad), the rest the coarsened families (one thread per family,
-// its children from `fam`).  With z.r null the kernel does the mesh only.
+// across the quad), then blocks_coarse the coarsened families (one thread
+// per family, its children from `fam`), the rest the carried curve order
+// (one thread per old curve position).  With z.r null the kernel does the
+// mesh only.
 __global__ void __launch_bounds__(256) k_amr_geometry(AmrMesh A, DevState z, DevGrad G, int* wait, Eos eos,
                                                       int slope, const int* ref_list, int n_refine,
                                                       const int* fam, int n_parents, int blocks_retain,
-                                                      int blocks_refine) {
+                                                      int blocks_refine, int blocks_coarse, const int* perm,
+                                                      const AmrMarks* rank, int* perm_new, int* iperm_new) {
     const int b = (int)blockIdx.x;
+    if (b >= blocks_retain + blocks_refine + blocks_coarse) {
+        const int j = (b - blocks_retain - blocks_refine - blocks_coarse) * 256 + (int)threadIdx.x;
+        if (j < A.n) carry_curve(A, perm, rank, perm_new, iperm_new, j);
+        return;
+    }
     if (b < blocks_retain) {
         const int ic = b * 256 + (int)threadIdx.x;
         if (ic >= A.n || A.flag[ic] != 0) return;
@@ -764,7 +809,7 @@ __global__ void __launch_bounds__(256) k_amr_geometry(AmrMesh A, DevState z, Dev
 // per move: lanes 0-7 copy one face slot each, lane 8 the vertices, lane 9
 // the cell arrays and the state.
 __global__ void __launch_bounds__(256) k_amr_move(AmrMesh A, DevState z, int* wait, int n_short, int nz_total,
-                                                  const Marks3* rank, int bound, const int* swap_src,
+                                                  const AmrMarks* rank, int bound, const int* swap_src,
                                                   const int* swap_dst) {
     const int t = cell_id();
     const int i = t >> 4, k = t & 15;
@@ -985,27 +1030,28 @@ hipError_t amr_balance(const AmrMesh& A, int max_level, hipStream_t s) {
     return hipGetLastError();
 }
 
-struct Add3 {
-    __device__ __forceinline__ Marks3 operator()(const Marks3& a, const Marks3& b) const {
-        return Marks3{a.ref + b.ref, a.head + b.head, a.nz + b.nz};
+struct AddMarks {
+    __device__ __forceinline__ AmrMarks operator()(const AmrMarks& a, const AmrMarks& b) const {
+        return AmrMarks{a.ref + b.ref, a.head + b.head, a.nz + b.nz, a.cc + b.cc};
     }
-    __device__ static __forceinline__ Marks3 shfl(const Marks3& v, int d) {
-        return Marks3{__shfl_up(v.ref, d, 64), __shfl_up(v.head, d, 64), __shfl_up(v.nz, d, 64)};
+    __device__ static __forceinline__ AmrMarks shfl(const AmrMarks& v, int d) {
+        return AmrMarks{__shfl_up(v.ref, d, 64), __shfl_up(v.head, d, 64), __shfl_up(v.nz, d, 64),
+                        __shfl_up(v.cc, d, 64)};
     }
 };
 
 int amr_scan3_tiles(int count) { return scan::tiles_of(count); }
 
-hipError_t amr_scan3(const Marks3* in, Marks3* out, Marks3* tile_sums, int count, Marks3* total, hipStream_t s) {
-    return scan::exclusive(in, out, tile_sums, count, Add3(), Marks3{0, 0, 0}, total, s);
+hipError_t amr_scan3(const AmrMarks* in, AmrMarks* out, AmrMarks* tile_sums, int count, AmrMarks* total, hipStream_t s) {
+    return scan::exclusive(in, out, tile_sums, count, AddMarks(), AmrMarks{0, 0, 0, 0}, total, s);
 }
 
-hipError_t amr_mark(const AmrMesh& A, Marks3* marks, hipStream_t s) {
-    hipLaunchKernelGGL(k_amr_mark, grid256(A.n), dim3(256), 0, s, A, marks);
+hipError_t amr_mark(const AmrMesh& A, AmrMarks* marks, const int* perm, hipStream_t s) {
+    hipLaunchKernelGGL(k_amr_mark, grid256(A.n), dim3(256), 0, s, A, marks, perm);
     return hipGetLastError();
 }
 
-hipError_t amr_positions(const AmrMesh& A, int n_large, int n_short, int n_parents, const Marks3* rank,
+hipError_t amr_positions(const AmrMesh& A, int n_large, int n_short, int n_parents, const AmrMarks* rank,
                          int* hole_at, int* ref_list, int* crs_list, int* fam, unsigned long long* faces,
                          unsigned long long* dt_rd, unsigned long long* dt_wr, hipStream_t s) {
     hipLaunchKernelGGL(k_amr_positions, grid256(n_large), dim3(256), 0, s, A, n_large, n_short, n_parents, rank,
@@ -1013,7 +1059,7 @@ hipError_t amr_positions(const AmrMesh& A, int n_large, int n_short, int n_paren
     return hipGetLastError();
 }
 
-hipError_t amr_swapmap(const AmrMesh& A, int n_short, int n_large, int nz_total, const Marks3* rank,
+hipError_t amr_swapmap(const AmrMesh& A, int n_short, int n_large, int nz_total, const AmrMarks* rank,
                        const int* hole_at, int* swap_src, int* swap_dst, int* err, hipStream_t s) {
     if (n_large <= n_short) return hipSuccess;
     hipLaunchKernelGGL(k_amr_swapmap, grid256(n_large - n_short), dim3(256), 0, s, A, n_short, n_large, nz_total,
@@ -1023,14 +1069,15 @@ hipError_t amr_swapmap(const AmrMesh& A, int n_short, int n_large, int nz_total,
 
 hipError_t amr_geometry(const AmrMesh& A, const DevState& z, const DevGrad& G, int* wait, const Eos& eos,
                         bool slope, const int* ref_list, int n_refine, const int* fam, int n_parents,
-                        hipStream_t s) {
+                        const int* perm, const AmrMarks* rank, int* perm_new, int* iperm_new, hipStream_t s) {
     const int br = (A.n + 255) / 256, bf = (4 * n_refine + 255) / 256, bc = (n_parents + 255) / 256;
-    hipLaunchKernelGGL(k_amr_geometry, dim3(br + bf + bc), dim3(256), 0, s, A, z, G, wait, eos, slope ? 1 : 0,
-                       ref_list, n_refine, fam, n_parents, br, bf);
+    const int be = perm_new ? (A.n + 255) / 256 : 0;
+    hipLaunchKernelGGL(k_amr_geometry, dim3(br + bf + bc + be), dim3(256), 0, s, A, z, G, wait, eos, slope ? 1 : 0,
+                       ref_list, n_refine, fam, n_parents, br, bf, bc, perm, rank, perm_new, iperm_new);
     return hipGetLastError();
 }
 
-hipError_t amr_move(const AmrMesh& A, const DevState& z, int* wait, int n_short, int nz_total, const Marks3* rank,
+hipError_t amr_move(const AmrMesh& A, const DevState& z, int* wait, int n_short, int nz_total, const AmrMarks* rank,
                     int bound, const int* swap_src, const int* swap_dst, hipStream_t s) {
     if (bound <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_amr_move, grid256(16 * bound), dim3(256), 0, s, A, z, wait, n_short, nz_total, rank, bound,

@@ -106,12 +106,12 @@ struct zfv_ctx {
     bool axial_mesh = false;
     DBuf<double> vol_alt, area_alt, axc;
     // AMR scratch
-    DBuf<Marks3> s_marks, s_rank;
+    DBuf<AmrMarks> s_marks, s_rank;
     DBuf<unsigned long long> s_faces;
     DBuf<int> s_hole_at, s_ref_list, s_crs_list, s_swap_src, s_swap_dst;
-    DBuf<Marks3> s_tiles;             // tile sums of the marks scan
+    DBuf<AmrMarks> s_tiles;           // tile sums of the marks scan
     DBuf<int> s_fam;                  // children of each coarsened family in z order (amr_positions)
-    Marks3* marks_total = nullptr;    // pinned host: totals of the marks scan
+    AmrMarks* marks_total = nullptr;  // pinned host: totals of the marks scan
     bool faces_pending = false;   // n_faces still on the device (s_faces)
     // device state
     DBuf<double> st[3][5];   // init / next / half layers
@@ -254,7 +254,6 @@ struct zfv_ctx {
         bool grad = false;        // gdx / gdy last written in curve order
         DBuf<double> cx, cy, vol, h, area, fcx, fcy, w, nx, ny, gtmp;
         DBuf<int> code, adj, perm, iperm, vals, slots, counts, offsets, tiles;
-        DBuf<int> ccount, coffset, ctiles;   // the order carried across an adaptation (curve_counts)
         DBuf<unsigned long long> keys[2];
         DBuf<unsigned char> tmp;
     } co;
@@ -878,8 +877,11 @@ int need_ready(zfv_ctx* ctx, bool state = true, bool config = false) {
 
 int put_max_dt(zfv_ctx* ctx);
 
-// Sorted copy of the mesh for the step kernels (state in the mesh's order).
-int build_curve(zfv_ctx* ctx) {
+// Sorted copy of the mesh for the step kernels (state in the mesh's order);
+// with gather, the same launch also gathers state layer cur and wait[wcur]
+// into curve order in the other layer and wait buffer, which become the
+// current ones (gather_current).
+int build_curve(zfv_ctx* ctx, bool gather = false) {
     auto& co = ctx->co;
     const int n = ctx->hm.n, cap = ctx->cap;
     hipStream_t s = ctx->stream;
@@ -934,7 +936,13 @@ int build_curve(zfv_ctx* ctx) {
     }
     const CurveMesh C{co.cap, co.cx.p, co.cy.p, co.vol.p, co.h.p, co.code.p, co.adj.p,
                       co.area.p, co.fcx.p, co.fcy.p, co.w.p, co.nx.p, co.ny.p};
-    ZFV_HIP(curve_mesh(ctx->dmesh(), C, co.perm.p, co.iperm.p, s));
+    const int nxt = 1 - ctx->cur;
+    ZFV_HIP(curve_mesh(ctx->dmesh(), C, co.perm.p, co.iperm.p, gather ? ctx->layer(ctx->cur) : DevState{},
+                       ctx->layer(nxt), ctx->wait[ctx->wcur].p, ctx->wait[1 - ctx->wcur].p, s));
+    if (gather) {
+        ctx->cur = nxt;
+        ctx->wcur = 1 - ctx->wcur;
+    }
     co.epoch = ctx->mesh_epoch;
     ctx->graphs_clear();   // captured steps read the previous copy
     return ZFV_OK;
@@ -956,11 +964,12 @@ int to_curve_order(zfv_ctx* ctx) {
     auto& co = ctx->co;
     if (co.state || co.mode == 0 || !co.refined || ctx->nseg > 1) return ZFV_OK;
     if (co.epoch != ctx->mesh_epoch || co.cap != ctx->cap) {
-        int r = build_curve(ctx);
+        int r = build_curve(ctx, true);   // gathers the state in the same launch
+        if (r) return r;
+    } else {
+        int r = gather_current(ctx, co.perm.p);
         if (r) return r;
     }
-    int r = gather_current(ctx, co.perm.p);
-    if (r) return r;
     co.state = true;
     return ZFV_OK;
 }
@@ -1411,7 +1420,7 @@ int zfv_mesh_reserve(zfv_ctx* ctx, int n_cells) {
     if (ctx->s_faces.n < 2) ZFV_HIP(ctx->s_faces.alloc(2));
     const size_t tiles = (size_t)std::max(amr_scan3_tiles(n_cells), 1);
     if (ctx->s_tiles.n < tiles) ZFV_HIP(ctx->s_tiles.alloc(tiles));
-    if (!ctx->marks_total) ZFV_HIP(hipHostMalloc((void**)&ctx->marks_total, sizeof(Marks3), hipHostMallocDefault));
+    if (!ctx->marks_total) ZFV_HIP(hipHostMalloc((void**)&ctx->marks_total, sizeof(AmrMarks), hipHostMallocDefault));
     if (ctx->graw.n < 2 * cap) ZFV_HIP(ctx->graw.alloc(2 * cap));
     // the curve-ordered copy at the new stride (built at the next step)
     auto& co = ctx->co;
@@ -1427,12 +1436,6 @@ int zfv_mesh_reserve(zfv_ctx* ctx, int n_cells) {
         co.perm_epoch = -1;
     }
     if (co.mode != 0 && co.keys_ok && co.gtmp.n < 4 * cap) ZFV_HIP(co.gtmp.alloc(4 * cap));
-    if (co.mode == 1 && co.keys_ok) {
-        for (auto* b : {&co.ccount, &co.coffset})
-            if (b->n < nr) ZFV_HIP(b->alloc(nr));
-        const size_t ct = (size_t)std::max(curve_counts_tiles((int)nr), 1);
-        if (co.ctiles.n < ct) ZFV_HIP(co.ctiles.alloc(ct));
-    }
     return ZFV_OK;
 }
 
@@ -1953,10 +1956,10 @@ int zfv_balance_flags(zfv_ctx* ctx, int max_level) {
 
 // Exclusive scan of the marks; the totals land in pinned host memory
 // (ctx->marks_total), valid after the next synchronisation.
-static int scan3(zfv_ctx* ctx, const Marks3* in, Marks3* out, int count) {
+static int scan3(zfv_ctx* ctx, const AmrMarks* in, AmrMarks* out, int count) {
     const int tiles = amr_scan3_tiles(count);
     if (ctx->s_tiles.n < (size_t)std::max(tiles, 1)) ZFV_HIP(ctx->s_tiles.alloc(std::max(tiles, 1)));
-    if (!ctx->marks_total) ZFV_HIP(hipHostMalloc((void**)&ctx->marks_total, sizeof(Marks3), hipHostMallocDefault));
+    if (!ctx->marks_total) ZFV_HIP(hipHostMalloc((void**)&ctx->marks_total, sizeof(AmrMarks), hipHostMallocDefault));
     ZFV_HIP(amr_scan3(in, out, ctx->s_tiles.p, count, ctx->marks_total, ctx->stream));
     return ZFV_OK;
 }
@@ -2006,11 +2009,18 @@ static int refine_impl(zfv_ctx* ctx, int max_level, int mode) {
     ctx->host_stale = true;
     for (auto* b : {&ctx->s_marks, &ctx->s_rank})
         if (b->n < (size_t)n) ZFV_HIP(b->alloc(n));
-    ZFV_HIP(amr_mark(ctx->amesh(), ctx->s_marks.p, s));
+    // The steps' curve order is carried across the adaptation (k_amr_mark
+    // counts, k_amr_geometry places) when the copy of this mesh is current
+    // and the capacity does not grow.
+    auto& co = ctx->co;
+    const bool carry_ok = !mm && co.mode == 1 && co.keys_ok && co.refined && co.epoch == ctx->mesh_epoch &&
+                          co.cap == ctx->cap;
+    co.perm_epoch = -1;
+    ZFV_HIP(amr_mark(ctx->amesh(), ctx->s_marks.p, carry_ok ? co.perm.p : nullptr, s));
     r = scan3(ctx, ctx->s_marks.p, ctx->s_rank.p, n);
     if (r) return r;
     ZFV_HIP(hipStreamSynchronize(s));
-    const volatile Marks3* tot = ctx->marks_total;
+    const volatile AmrMarks* tot = ctx->marks_total;
     const int n_refine = tot->ref;
     const int n_parents = tot->head;
     const int nz_total = tot->nz;   // refined + coarsened cells
@@ -2037,30 +2047,14 @@ static int refine_impl(zfv_ctx* ctx, int max_level, int mode) {
                           ctx->s_crs_list.p, ctx->s_fam.p, ctx->s_faces.p, ctx->dtmin.p + rd, ctx->dtmin.p + wr, s));
     ZFV_HIP(amr_swapmap(A, n_short, n_large, nz_total, ctx->s_rank.p, ctx->s_hole_at.p, ctx->s_swap_src.p,
                         ctx->s_swap_dst.p, ctx->err.p, s));
-    // The steps' curve order after this adaptation, carried from the one
-    // before it while the balanced flags and the final positions are at
-    // hand (before the moves overwrite flags): no rebuild at the next step.
-    bool carried = false;
-    {
-        auto& co = ctx->co;
-        co.perm_epoch = -1;
-        if (!mm && co.mode == 1 && co.keys_ok && co.refined && co.epoch == ctx->mesh_epoch && co.cap == ctx->cap &&
-            co.vals.n >= (size_t)n_short) {
-            for (auto* b : {&co.ccount, &co.coffset})
-                if (b->n < (size_t)n) ZFV_HIP(b->alloc(n));
-            const size_t ct = (size_t)std::max(curve_counts_tiles(n), 1);
-            if (co.ctiles.n < ct) ZFV_HIP(co.ctiles.alloc(ct));
-            ZFV_HIP(curve_counts(co.perm.p, n, A.flag, A.z_idx, co.ccount.p, co.coffset.p, co.ctiles.p, s));
-            ZFV_HIP(curve_expand(co.perm.p, n, A.flag, A.z_idx, A.next, co.coffset.p, co.vals.p, co.iperm.p, s));
-            std::swap(co.perm, co.vals);
-            carried = true;
-        }
-    }
     const DevState z = mm ? DevState{} : ctx->layer(ctx->cur);
     int* wait = mm ? nullptr : ctx->wait[ctx->wcur].p;
     DevGrad G{ctx->gdx.p, ctx->gdy.p};
+    const bool carried = carry_ok && co.cap == ctx->cap && co.vals.n >= (size_t)n_short && tot->cc == n_short;
     ZFV_HIP(amr_geometry(A, z, G, wait, ctx->eos(), mode == kRefineSlope, ctx->s_ref_list.p, n_refine,
-                         ctx->s_fam.p, n_parents, s));
+                         ctx->s_fam.p, n_parents, co.perm.p, ctx->s_rank.p, carried ? co.vals.p : nullptr,
+                         co.iperm.p, s));
+    if (carried) std::swap(co.perm, co.vals);
     const size_t cap = (size_t)ctx->cap;
     if (mm)
         ZFV_HIP(mm_distribute(A, mm_layer(ctx, S.cur), S.mix, S.gx.p, S.gy.p, ctx->s_ref_list.p, n_refine,

@@ -96,26 +96,34 @@ hipError_t amr_criterion(const AmrMesh& A, const DevState& z, const DevGrad& G, 
 hipError_t amr_balance(const AmrMesh& A, int max_level, hipStream_t s);
 // Per-cell marks scanned by the adaptation: refine, coarsening-family head,
 // flag != 0.
-struct Marks3 {
-    int ref, head, nz;
+// Per-cell marks of an adaptation: refined, head of a coarsened family,
+// changed; cc: the entries curve position i of the steps' order becomes
+// (4 refined, 1 kept or family head, 0 other family member; 0 when the
+// order is not carried).
+struct AmrMarks {
+    int ref, head, nz, cc;
 };
 // Exclusive scan of the marks (tile_sums: amr_scan3_tiles(count) entries);
 // the total of all marks goes to `total` (device or pinned host memory).
 int amr_scan3_tiles(int count);
-hipError_t amr_scan3(const Marks3* in, Marks3* out, Marks3* tile_sums, int count, Marks3* total, hipStream_t s);
-hipError_t amr_mark(const AmrMesh& A, Marks3* marks, hipStream_t s);
+hipError_t amr_scan3(const AmrMarks* in, AmrMarks* out, AmrMarks* tile_sums, int count, AmrMarks* total, hipStream_t s);
+// perm (may be null): the steps' curve order of the cells, for cc.
+hipError_t amr_mark(const AmrMesh& A, AmrMarks* marks, const int* perm, hipStream_t s);
 // fam receives the four children of each coarsened family in z order.
-hipError_t amr_positions(const AmrMesh& A, int n_large, int n_short, int n_parents, const Marks3* rank,
+hipError_t amr_positions(const AmrMesh& A, int n_large, int n_short, int n_parents, const AmrMarks* rank,
                          int* hole_at, int* ref_list, int* crs_list, int* fam, unsigned long long* faces,
                          unsigned long long* dt_rd, unsigned long long* dt_wr, hipStream_t s);
-hipError_t amr_swapmap(const AmrMesh& A, int n_short, int n_large, int nz_total, const Marks3* rank,
+hipError_t amr_swapmap(const AmrMesh& A, int n_short, int n_large, int nz_total, const AmrMarks* rank,
                        const int* hole_at, int* swap_src, int* swap_dst, int* err, hipStream_t s);
 // z.r null: mesh only (the solver's data are distributed and moved by its
-// own kernels); fam: the coarsened families of amr_positions.
+// own kernels); fam: the coarsened families of amr_positions.  perm_new not
+// null: the same launch also carries the steps' curve order (perm, of the
+// A.n cells before) into perm_new / iperm_new, at the positions the scanned
+// marks' cc give (rank).
 hipError_t amr_geometry(const AmrMesh& A, const DevState& z, const DevGrad& G, int* wait, const Eos& eos,
                         bool slope, const int* ref_list, int n_refine, const int* fam, int n_parents,
-                        hipStream_t s);
-hipError_t amr_move(const AmrMesh& A, const DevState& z, int* wait, int n_short, int nz_total, const Marks3* rank,
+                        const int* perm, const AmrMarks* rank, int* perm_new, int* iperm_new, hipStream_t s);
+hipError_t amr_move(const AmrMesh& A, const DevState& z, int* wait, int n_short, int nz_total, const AmrMarks* rank,
                     int bound, const int* swap_src, const int* swap_dst, hipStream_t s);
 hipError_t amr_derive(const AmrMesh& A, const DevState& z, const Eos& eos, unsigned long long* counts,
                       unsigned long long* dt_slot, int* err, hipStream_t s);
@@ -143,22 +151,15 @@ int curve_scan_tiles(int n_chunks);
 hipError_t curve_compact(int* slots, int n_chunks, int* counts, int* offsets, int* tile_sums, int* perm, int* iperm,
                          hipStream_t s);
 hipError_t curve_inverse(const int* perm, int* iperm, int n, hipStream_t s);
-// The order after one adaptation from the order before it (perm of the n
-// cells before), with the adaptation's balanced flags, z indices and final
-// positions (next): curve_counts before the cells move (counts and their
-// exclusive scan, tile_sums: curve_counts_tiles(n) ints), curve_expand after
-// the swap map, into perm_new / iperm_new (not aliasing perm).
-int curve_counts_tiles(int n);
-hipError_t curve_counts(const int* perm, int n, const int* flag, const int* z_idx, int* counts, int* offsets,
-                        int* tile_sums, hipStream_t s);
-hipError_t curve_expand(const int* perm, int n, const int* flag, const int* z_idx, const int* next, const int* offsets,
-                        int* perm_new, int* iperm_new, hipStream_t s);
 // Radix sort of (key, value) pairs on bits [0, end_bit); tmp == nullptr
 // sizes the scratch.
 hipError_t curve_sort(const unsigned long long* keys_in, unsigned long long* keys_out, const int* vals_in,
                       int* vals_out, int n, int end_bit, void* tmp, size_t& tmp_bytes, hipStream_t s);
-// The curve-ordered mesh copy (perm: curve -> mesh index, iperm: inverse).
-hipError_t curve_mesh(const DevMesh& src, const CurveMesh& dst, const int* perm, const int* iperm, hipStream_t s);
+// The curve-ordered mesh copy (perm: curve -> mesh index, iperm: inverse);
+// with zsrc.r not null the same launch gathers the state rows and wait
+// counters into curve order (zdst, wdst).
+hipError_t curve_mesh(const DevMesh& src, const CurveMesh& dst, const int* perm, const int* iperm,
+                      const DevState& zsrc, const DevState& zdst, const int* wsrc, int* wdst, hipStream_t s);
 // dst[i] = src[idx[i]]: the five state rows and the wait counters, or
 // `rows` rows of a [rows][stride] array.
 hipError_t gather_state(const DevState& src, const DevState& dst, const int* wsrc, int* wdst, const int* idx, int n,
@@ -307,7 +308,7 @@ hipError_t mm_flags(const DevMesh& M, const MmLayer& L, const mm::Mix& mx, int* 
 hipError_t mm_distribute(const AmrMesh& A, const MmLayer& L, const mm::Mix& mx, const double* gx, const double* gy,
                          const int* ref_list, int n_refine, const int* fam, int n_parents, int* err, hipStream_t s);
 hipError_t mm_move(const MmLayer& L, const mm::Mix& mx, int stride, int n, int n_short, int nz_total,
-                   const Marks3* rank, int bound, const int* swap_src, const int* swap_dst, hipStream_t s);
+                   const AmrMarks* rank, int bound, const int* swap_src, const int* swap_dst, hipStream_t s);
 hipError_t mm_cache(const DevMesh& M, const MmLayer& L, const mm::Mix& mx, unsigned long long* slot,
                     hipStream_t s);
 

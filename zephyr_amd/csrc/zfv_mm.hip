@@ -970,7 +970,7 @@ __global__ void __launch_bounds__(256) k_mm_merge(AmrMesh A, MmLayer L, Mix mx, 
 // SwapLists::move_elements (swap_lists.h:129-138) for the state rows.
 template <int NM>
 __global__ void __launch_bounds__(256) k_mm_move(MmLayer L, int st, int n_short, int nz_total, int n,
-                                                 const Marks3* __restrict__ rank, int bound,
+                                                 const AmrMarks* __restrict__ rank, int bound,
                                                  const int* __restrict__ swap_src, const int* __restrict__ swap_dst) {
     const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
     if (i >= bound) return;
@@ -1129,7 +1129,7 @@ hipError_t mm_distribute(const AmrMesh& A, const MmLayer& L, const Mix& mx, cons
     return hipGetLastError();
 }
 
-hipError_t mm_move(const MmLayer& L, const Mix& mx, int stride, int n, int n_short, int nz_total, const Marks3* rank,
+hipError_t mm_move(const MmLayer& L, const Mix& mx, int stride, int n, int n_short, int nz_total, const AmrMarks* rank,
                    int bound, const int* swap_src, const int* swap_dst, hipStream_t s) {
     if (bound <= 0) return hipSuccess;
     ZFV_MM_DISPATCH(mx.n, k_mm_move, dim3((bound + 255) / 256), dim3(256), 0, s, L, stride, n_short, nz_total, n,

@@ -77,48 +77,6 @@ __global__ void __launch_bounds__(256) k_curve_write(int* __restrict__ slots, in
     if (lane == 0) counts[chunk] = 0;
 }
 
-// Incremental curve order across one adaptation.  Refining a cell puts its
-// four children, in z order, exactly where the cell was on the curve;
-// coarsening a family puts the parent where its four children were (the
-// first of them, z_idx % 4 == 0, leads); every other cell keeps its place.
-// So the new order is the old one with each entry expanded to 4 (refined),
-// 1 (kept, or the head of a coarsened family) or 0 entries, at positions
-// given by a scan of those counts, and each entry's mesh index is the
-// final position the adaptation gives it: next[o] for a kept cell,
-// next[next[o] + k] for child k, next[next[o]] for a family's parent.
-__global__ void __launch_bounds__(256) k_curve_counts(const int* __restrict__ perm, int n, const int* __restrict__ flag,
-                                                      const int* __restrict__ z_idx, int* __restrict__ counts) {
-    const int j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= n) return;
-    const int o = perm[j];
-    const int f = flag[o];
-    counts[j] = f > 0 ? 4 : (f < 0 ? ((z_idx[o] & 3) == 0 ? 1 : 0) : 1);
-}
-
-__global__ void __launch_bounds__(256) k_curve_expand(const int* __restrict__ perm, int n, const int* __restrict__ flag,
-                                                      const int* __restrict__ z_idx, const int* __restrict__ next,
-                                                      const int* __restrict__ offsets, int* __restrict__ perm_new,
-                                                      int* __restrict__ iperm_new) {
-    const int j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= n) return;
-    const int o = perm[j];
-    const int f = flag[o];
-    const int at = offsets[j];
-    if (f > 0) {
-        const int mc = next[o];
-#pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            const int p = next[mc + k];
-            perm_new[at + k] = p;
-            iperm_new[p] = at + k;
-        }
-    } else if (f == 0 || (z_idx[o] & 3) == 0) {
-        const int p = f == 0 ? next[o] : next[next[o]];
-        perm_new[at] = p;
-        iperm_new[p] = at;
-    }
-}
-
 __global__ void __launch_bounds__(256) k_curve_inverse(const int* __restrict__ perm, int* __restrict__ iperm, int n) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) iperm[perm[i]] = i;
@@ -128,10 +86,19 @@ __global__ void __launch_bounds__(256) k_curve_inverse(const int* __restrict__ p
 // cell perm[i] of the mesh; neighbour indices are mapped through iperm
 // (boundary slots point at the cell itself, undefined slots too).
 __global__ void __launch_bounds__(256) k_curve_mesh(DevMesh src, CurveMesh dst, const int* __restrict__ perm,
-                                                    const int* __restrict__ iperm) {
+                                                    const int* __restrict__ iperm, DevState zsrc, DevState zdst,
+                                                    const int* __restrict__ wsrc, int* __restrict__ wdst) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= src.n) return;
     const int o = perm[i];
+    if (zsrc.r) {   // the state into curve order (gather_state of the same permutation)
+        zdst.r[i] = zsrc.r[o];
+        zdst.u[i] = zsrc.u[o];
+        zdst.v[i] = zsrc.v[o];
+        zdst.p[i] = zsrc.p[o];
+        zdst.e[i] = zsrc.e[o];
+        wdst[i] = wsrc[o];
+    }
     const int code = src.code[o];
     // every load is issued before the first store (the source and copy
     // arrays are of the same types, so stores would otherwise order them)
@@ -237,34 +204,16 @@ hipError_t curve_sort(const unsigned long long* keys_in, unsigned long long* key
     return hipcub::DeviceRadixSort::SortPairs(tmp, tmp_bytes, keys_in, keys_out, vals_in, vals_out, n, 0, end_bit, s);
 }
 
-int curve_counts_tiles(int n) { return scan::tiles_of(n); }
-
-hipError_t curve_counts(const int* perm, int n, const int* flag, const int* z_idx, int* counts, int* offsets,
-                        int* tile_sums, hipStream_t s) {
-    if (n <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_curve_counts, grid256(n), dim3(256), 0, s, perm, n, flag, z_idx, counts);
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return e;
-    return scan::exclusive(counts, offsets, tile_sums, n, scan::AddInt(), 0, (int*)nullptr, s);
-}
-
-hipError_t curve_expand(const int* perm, int n, const int* flag, const int* z_idx, const int* next, const int* offsets,
-                        int* perm_new, int* iperm_new, hipStream_t s) {
-    if (n <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_curve_expand, grid256(n), dim3(256), 0, s, perm, n, flag, z_idx, next, offsets, perm_new,
-                       iperm_new);
-    return hipGetLastError();
-}
-
 hipError_t curve_inverse(const int* perm, int* iperm, int n, hipStream_t s) {
     if (n <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_curve_inverse, grid256(n), dim3(256), 0, s, perm, iperm, n);
     return hipGetLastError();
 }
 
-hipError_t curve_mesh(const DevMesh& src, const CurveMesh& dst, const int* perm, const int* iperm, hipStream_t s) {
+hipError_t curve_mesh(const DevMesh& src, const CurveMesh& dst, const int* perm, const int* iperm,
+                      const DevState& zsrc, const DevState& zdst, const int* wsrc, int* wdst, hipStream_t s) {
     if (src.n <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_curve_mesh, grid256(src.n), dim3(256), 0, s, src, dst, perm, iperm);
+    hipLaunchKernelGGL(k_curve_mesh, grid256(src.n), dim3(256), 0, s, src, dst, perm, iperm, zsrc, zdst, wsrc, wdst);
     return hipGetLastError();
 }
 
