@@ -685,6 +685,7 @@ def ensemble_run(args, w, Z):
                                           "one_shared_stream": t_loop_one * 1e3 / args.steps},
                    "batched_over_looped_one_stream": t_loop_one / t_batch,
                    "launch": {str(st): b.launch(st) for st in (1, 2)},
+                   "tile_persistent": b.tile_schedule(),
                    "autotune_ms": tune.tolist(),
                    "kernel_ms_avg": {k: v[0] / v[1] for k, v in prof.items()}},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -892,6 +893,7 @@ def main():
                    "kernel_launches": dict(prof.launches),
                    "kernel_timing": f"HIP events, {args.prof_steps} steps following the timed region",
                    "launch": {str(st): ctx.launch(st) for st in ((0,) if acc == 1 else (1, 2))},
+                   "tile_persistent": ctx.tile_schedule(),
                    "autotune_ms": tune.tolist(),
                    "step_kernels_hbm_frac": step_bytes / (step_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
                    "step_kernels_ms_share_of_wall": (step_ms / max(args.prof_steps, 1)) / (el * 1e3 / args.steps)},

@@ -438,13 +438,22 @@ int zfv_restore(zfv_ctx* ctx, const char* root);
  * bit-identical results. */
 int zfv_set_launch(zfv_ctx* ctx, int stage, int layout, int block);
 int zfv_get_launch(zfv_ctx* ctx, int stage, int* layout, int* block);
+/* Schedule of the tiled step (layout 3): 0 (default) one workgroup per
+ * tile; 1 persistent workgroups, as many as the GPU holds at once, each
+ * walking the tiles of its XCD's range.  Results are bit-identical.  No
+ * counterpart in the reference (its SmFluid::update loops over cells on the
+ * CPU, sm_fluid.cpp:81-102). */
+int zfv_set_tile_schedule(zfv_ctx* ctx, int persistent);
+int zfv_get_tile_schedule(zfv_ctx* ctx, int* persistent);
 /* Autotuner: times every (layout, workgroup size) of the configured scheme's
  * step kernels on the current state (`reps` launches each; the state is not
- * advanced), and the tiled step's shapes where it applies, and keeps the
- * fastest per stage (the tiled step when it beats the best gradient +
- * corrector pair).  ms_out (may be NULL): 36 doubles, ms per launch indexed
- * [stage][layout 0-3][block 128/256/512, or for layout 3 the tile shape
- * 512/768/1024] (0 where not timed). */
+ * advanced), and the tiled step's shapes and schedules where it applies, and
+ * keeps the fastest per stage (the tiled step when it beats the best
+ * gradient + corrector pair).  ms_out (may be NULL): 36 doubles, ms per
+ * launch indexed [stage][layout 0-3][block 128/256/512, or for layout 3 the
+ * tile shape 512/768/1024], where [1][3][shape] is the tiled step with one
+ * workgroup per tile and [2][3][shape] with persistent workgroups (0 where
+ * not timed). */
 int zfv_autotune(zfv_ctx* ctx, int reps, double* ms_out);
 /* Capture steps into HIP graphs (6 steps per graph, one graph per starting
  * phase, replayed by zfv_smf_step; re-captured after any change of mesh,

@@ -170,3 +170,35 @@ def test_tiled_layout_refused_where_it_does_not_apply(zfv):
     with pytest.raises(zfv.ZfvError):
         ctx.set_launch(1, zfv.TILE, 512)
     ctx.step(2)   # adapted mesh: the stage kernels
+
+
+@pytest.mark.parametrize("shape", SHAPES)
+def test_persistent_schedule_bit_identical(zfv, shape):
+    """The tiled step by persistent workgroups (each walking the tiles of its
+    XCD's range) against one workgroup per tile: a mesh of many tiles, a
+    mesh smaller than the resident grid, and a batch of problems."""
+    for nx, ny, bounds in ((300, 260, (2, 2, 3, 3)), (9, 40, (1, 2, 2, 1))):
+        _, b = pair(zfv, nx, ny, bounds, "MC", shape, seed=9)
+        ref = pair(zfv, nx, ny, bounds, "MC", shape, seed=9)[1]   # tiled, one workgroup per tile
+        b.set_tile_schedule(True)
+        assert b.tile_schedule() and not ref.tile_schedule()
+        run_compare(ref, b, 4, f"persistent {nx}x{ny} shape {shape}")
+    sizes = [(64, 64), (17, 9), (100, 130), (5, 70), (1, 1)]
+    out = []
+    for persistent in (False, True):
+        ctx = zfv.Context()
+        ctx.batch_rectangles(sizes)
+        ctx.configure(acc=2, flux="hllc", limiter="MC", cfl=0.5, gamma=GAMMA)
+        ctx.set_launch(1, zfv.TILE, shape)
+        ctx.set_tile_schedule(persistent)
+        begin, count = ctx.batch_layout()
+        full = [np.ones(ctx.n), np.zeros(ctx.n), np.zeros(ctx.n), np.ones(ctx.n), np.full(ctx.n, 2.5)]
+        for p, (nx, ny) in enumerate(sizes):
+            st = quadrants(nx, ny, 60 + p)
+            for k in range(5):
+                full[k][begin[p]:begin[p] + count[p]] = st[k]
+        ctx.set_state(*full)
+        ctx.step(5)
+        out.append((ctx.state(), ctx.batch_dt(), ctx.wait()))
+    assert_same(out[0][0], out[1][0], "batch state, persistent schedule")
+    assert np.array_equal(out[0][1], out[1][1]) and np.array_equal(out[0][2], out[1][2])

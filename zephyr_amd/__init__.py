@@ -34,6 +34,7 @@ EXPORTS = [
     "zfv_mesh_n_faces", "zfv_mesh_reserve", "zfv_set_launch", "zfv_get_launch", "zfv_autotune",
     "zfv_batch_rectangles", "zfv_batch_layout", "zfv_batch_dt", "zfv_graph_enable",
     "zfv_set_cell_order", "zfv_get_cell_order", "zfv_get_curve_order",
+    "zfv_set_tile_schedule", "zfv_get_tile_schedule",
     "zfv_mm_set_limiter", "zfv_mm_set_accuracy", "zfv_mm_set_method", "zfv_mm_set_crp_mode",
     "zfv_mm_set_splitting", "zfv_mm_set_overlap", "zfv_mm_set_flags", "zfv_mm_refine", "zfv_mm_grad",
     "zfv_adjoint_forward", "zfv_adjoint_backward",
@@ -121,6 +122,8 @@ def lib():
         L.zfv_set_cell_order.argtypes = [vp, C.c_int]
         L.zfv_get_cell_order.argtypes = [vp, vp, vp]
         L.zfv_get_curve_order.argtypes = [vp, vp]
+        L.zfv_set_tile_schedule.argtypes = [vp, C.c_int]
+        L.zfv_get_tile_schedule.argtypes = [vp, vp]
         L.zfv_mm_configure.argtypes = [vp, C.c_int, vp, C.c_double]
         L.zfv_mm_set_max_dt.argtypes = [vp, C.c_double]
         L.zfv_mm_set_crp.argtypes = [vp, C.c_int]
@@ -416,6 +419,16 @@ class Context:
 
     def graph_enable(self, on=True):
         _check(lib().zfv_graph_enable(self.h, 1 if on else 0))
+
+    def set_tile_schedule(self, persistent):
+        """Tiled step by persistent workgroups (True) or one workgroup per
+        tile (False); results are identical."""
+        _check(lib().zfv_set_tile_schedule(self.h, 1 if persistent else 0))
+
+    def tile_schedule(self):
+        p = C.c_int(0)
+        _check(lib().zfv_get_tile_schedule(self.h, C.byref(p)))
+        return bool(p.value)
 
     def set_cell_order(self, curve=True, rebuild=False):
         """Curve order for the step kernels of refined meshes (default;

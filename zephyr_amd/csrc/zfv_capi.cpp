@@ -153,6 +153,7 @@ struct zfv_ctx {
         DBuf<TileRow> row;
     } tiles;
     int tile_variant = 0;
+    bool tile_persistent = false;   // tiled step by persistent workgroups (else one workgroup per tile)
     // The tiled step does not store its gradients: after one, the gradients
     // of the last step are recomputed on demand from its input (layer 1 - cur).
     bool grad_pending = false;
@@ -1646,7 +1647,8 @@ static int step_once(zfv_ctx* ctx) {
         int* w_out = ctx->wait[1 - ctx->wcur].p;
         const TileMesh T = ctx->tmesh();
         ZFV_HIP(timed(ctx, "tile", [&] {
-            return launch_tile_step(P.limiter, ctx->tile_variant, T, in, out, w_in, w_out, eos, ctl, ctx->stream);
+            return launch_tile_step(P.limiter, ctx->tile_variant, ctx->tile_persistent, T, in, out, w_in, w_out, eos,
+                                    ctl, ctx->stream);
         }));
         ctx->wcur = 1 - ctx->wcur;
         ctx->grad_pending = true;   // the step's gradients were not stored
@@ -3014,6 +3016,23 @@ int zfv_set_launch(zfv_ctx* ctx, int stage, int layout, int block) {
     return ZFV_OK;
 }
 
+int zfv_set_tile_schedule(zfv_ctx* ctx, int persistent) {
+    int r = check_ctx(ctx);
+    if (r) return r;
+    if (persistent != 0 && persistent != 1) return fail(ZFV_ERR_ARG, "tile schedule must be 0 or 1");
+    ctx->tile_persistent = persistent != 0;
+    ctx->graphs_clear();
+    return ZFV_OK;
+}
+
+int zfv_get_tile_schedule(zfv_ctx* ctx, int* persistent) {
+    int r = check_ctx(ctx);
+    if (r) return r;
+    if (!persistent) return fail(ZFV_ERR_ARG, "null output");
+    *persistent = ctx->tile_persistent ? 1 : 0;
+    return ZFV_OK;
+}
+
 int zfv_get_launch(zfv_ctx* ctx, int stage, int* layout, int* block) {
     int r = check_ctx(ctx);
     if (r) return r;
@@ -3095,20 +3114,26 @@ int zfv_autotune(zfv_ctx* ctx, int reps, double* ms_out) {
     // the tiled step (one kernel for the whole step) against the best pair
     // of gradient and corrector kernels
     if (P.acc == 2 && P.flux == ZFV_HLLC && ctx->tiles.ok && !ctx->co.state) {
+        // [1][tile][v]: one workgroup per tile; [2][tile][v]: persistent workgroups
         const TileMesh T = ctx->tmesh();
-        int bv = 0;
-        for (int v = 0; v < kTileVariants; ++v) {
-            r = time_it([&] { return launch_tile_step(P.limiter, v, T, in, out, w_in, w_out, eos, ctl, s); },
-                        t[1][kLayoutTile][v]);
-            if (r) return r;
-            if (t[1][kLayoutTile][v] < t[1][kLayoutTile][bv]) bv = v;
-        }
+        int bv = 0, bp = 0;
+        for (int pers = 0; pers < 2; ++pers)
+            for (int v = 0; v < kTileVariants; ++v) {
+                r = time_it([&] { return launch_tile_step(P.limiter, v, pers != 0, T, in, out, w_in, w_out, eos, ctl, s); },
+                            t[1 + pers][kLayoutTile][v]);
+                if (r) return r;
+                if (t[1 + pers][kLayoutTile][v] < t[1 + bp][kLayoutTile][bv]) {
+                    bv = v;
+                    bp = pers;
+                }
+            }
         const double split = t[1][ctx->layout[1]][ctx->block[1] == 128 ? 0 : (ctx->block[1] == 256 ? 1 : 2)] +
                              t[2][ctx->layout[2]][ctx->block[2] == 128 ? 0 : (ctx->block[2] == 256 ? 1 : 2)];
-        if (t[1][kLayoutTile][bv] < split) {
+        if (t[1 + bp][kLayoutTile][bv] < split) {
             ctx->layout[1] = kLayoutTile;
             ctx->block[1] = (kTileRows[bv] + 2) * 64;
             ctx->tile_variant = bv;
+            ctx->tile_persistent = bp != 0;
         }
     }
     ctx->event_pool.push_back(e0);

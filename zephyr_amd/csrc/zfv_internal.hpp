@@ -227,8 +227,9 @@ struct TileMesh {
 // One MUSCL + HLLC step (gradient, limiting, predictor, corrector and the
 // next step's CFL candidate) per tile, intermediate values in LDS.
 // variant: the tile shape (kTileRows); workgroups of (kTileRows[v] + 2) * 64 threads.
-hipError_t launch_tile_step(int limiter, int variant, const TileMesh& T, const DevState& in, const DevState& out,
-                            const int* wait_in, int* wait_out, const Eos& eos, const StepCtl& ctl, hipStream_t s);
+hipError_t launch_tile_step(int limiter, int variant, bool persistent, const TileMesh& T, const DevState& in,
+                            const DevState& out, const int* wait_in, int* wait_out, const Eos& eos,
+                            const StepCtl& ctl, hipStream_t s);
 // Scratch of the no-allocation entry points (zfv_kernels.hip): 4 result
 // doubles, then up to kScratchBlocks workgroups' 4 partial sums (4 MiB).
 constexpr int kScratchBlocks = 131072;

@@ -18,6 +18,8 @@
 // r takes row i = i0 - 1 + r of the halo box and lane l the cell
 // j = j0 - 1 + l, so a wavefront reads 64 consecutive cells; the outputs are
 // rows 1..kTileI and lanes 1..kTileJ.  Cell index c = off + i * ny + j.
+#include <algorithm>
+
 #include "zfv_internal.hpp"
 #include "zfv_step_common.hpp"
 #include "zfv_step_faces.hpp"
@@ -143,21 +145,30 @@ __device__ __forceinline__ SlotL tile_slot(const Frame& fr, int t, const Z& zc, 
 // (sm_fluid.cpp:116-125, gradient.h:226-282, 412-474), fluxes_stage1
 // (:176-253) and fluxes_stage2 (:255-366) with HLLC, plus the next step's
 // CFL candidate (:104-114).
-// Workgroups of (TI + 2) wavefronts; WAVES = the resident waves per SIMD the
-// register allocation is sized for.
-template <int LIM, int V, int TI, int WAVES>
-__global__ void __launch_bounds__((TI + 2) * 64) __attribute__((amdgpu_waves_per_eu(WAVES)))
-k_tile_step(TileMesh T, DevState in, DevState out, const int* __restrict__ wait_in, int* __restrict__ wait_out,
-            Eos eos, StepCtl ctl) {
+// The workgroup's LDS: the state box, the half states, limited gradients
+// and isolation counters of the halo box, the CFL reduction.
+template <int TI>
+struct TileLds {
+    double zs[4][(TI + 4) * kBJ];   // rho, u, v, p of the state box
+    double hs[5][(TI + 2) * kHJ];   // half state of the halo box
+    double gs[8][(TI + 2) * kHJ];   // limited gradients: d/dx then d/dy of rho, u, v, p
+    int ws[(TI + 2) * kHJ];         // isolation counters (wait_in)
+    double red[TI + 2];
+};
+
+// One tile b (of the tile list of shape V) by the whole workgroup.
+template <int LIM, int V, int TI>
+__device__ __forceinline__ void tile_body(const TileMesh& T, const DevState& in, const DevState& out,
+                                          const int* __restrict__ wait_in, int* __restrict__ wait_out,
+                                          const Eos& eos, const StepCtl& ctl, int b, TileLds<TI>& S) {
     constexpr int kTileI = TI, kTileThreads = (TI + 2) * 64;
     constexpr int kBI = TI + 4, kHI = TI + 2;
-    __shared__ double zs[4][kBI * kBJ];   // rho, u, v, p of the state box
-    __shared__ double hs[5][kHI * kHJ];   // half state of the halo box
-    __shared__ double gs[8][kHI * kHJ];   // limited gradients: d/dx then d/dy of rho, u, v, p
-    __shared__ int ws[kHI * kHJ];         // isolation counters (wait_in)
-    __shared__ double red[TI + 2];
+    auto& zs = S.zs;
+    auto& hs = S.hs;
+    auto& gs = S.gs;
+    auto& ws = S.ws;
+    auto& red = S.red;
 
-    const int b = xcd_block((int)blockIdx.x, (int)gridDim.x);
     const int pid = problem_of<V>(T, b);
     const TileProb P = T.prob[pid];
     const int t = b - P.tile0[V];
@@ -333,12 +344,76 @@ k_tile_step(TileMesh T, DevState in, DevState out, const int* __restrict__ wait_
     tile_min_atomic<TI + 2>(&ctl.dtmin[3 * pid + ctl.wr], cand, red);
 }
 
+// Workgroups of (TI + 2) wavefronts; WAVES = the resident waves per SIMD the
+// register allocation is sized for.  One workgroup per tile, dealt to the
+// XCDs in contiguous ranges of tiles (xcd_block).
+template <int LIM, int V, int TI, int WAVES>
+__global__ void __launch_bounds__((TI + 2) * 64) __attribute__((amdgpu_waves_per_eu(WAVES)))
+k_tile_step(TileMesh T, DevState in, DevState out, const int* __restrict__ wait_in, int* __restrict__ wait_out,
+            Eos eos, StepCtl ctl) {
+    __shared__ TileLds<TI> S;
+    tile_body<LIM, V, TI>(T, in, out, wait_in, wait_out, eos, ctl, xcd_block((int)blockIdx.x, (int)gridDim.x), S);
+}
+
+// The same step by persistent workgroups: the grid is the number of
+// workgroups the GPU holds at once (a multiple of 8), and workgroup g, on
+// XCD g % 8 (workgroups are dealt to the XCDs round-robin), walks the tiles
+// of that XCD's contiguous range (the ranges of xcd_block) with a stride of
+// the XCD's workgroup count.  Every workgroup leaves after its last tile.
+template <int LIM, int V, int TI, int WAVES>
+__global__ void __launch_bounds__((TI + 2) * 64) __attribute__((amdgpu_waves_per_eu(WAVES)))
+k_tile_step_p(TileMesh T, DevState in, DevState out, const int* __restrict__ wait_in, int* __restrict__ wait_out,
+              Eos eos, StepCtl ctl) {
+    __shared__ TileLds<TI> S;
+    const int n = T.n_tiles[V];
+    const int g = (int)blockIdx.x, x = g & 7, wgs = (int)gridDim.x >> 3;
+    const int q = n >> 3, rem = n & 7;
+    const int begin = x * q + (x < rem ? x : rem), end = begin + q + (x < rem ? 1 : 0);
+    for (int b = begin + (g >> 3); b < end; b += wgs) {
+        tile_body<LIM, V, TI>(T, in, out, wait_in, wait_out, eos, ctl, b, S);
+        __syncthreads();   // the next tile's loads overwrite the LDS boxes
+    }
+}
+
+// Workgroups the GPU holds at once for a persistent tiled-step kernel
+// (a multiple of 8, at least 8).
+template <int LIM, int V, int TI, int WAVES>
+static hipError_t resident_grid(int* grid) {
+    static int per_cu = 0;   // one value per kernel instance
+    if (per_cu == 0) {
+        int nb = 0;
+        hipError_t e =
+            hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_tile_step_p<LIM, V, TI, WAVES>, (TI + 2) * 64, 0);
+        if (e != hipSuccess) return e;
+        per_cu = nb > 0 ? nb : 1;
+    }
+    int dev = 0, cus = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    if (e != hipSuccess) return e;
+    *grid = std::max(8, (per_cu * cus) / 8 * 8);
+    return hipSuccess;
+}
+
 template <int V, int TI, int WAVES>
-static void launch_tile_t(int limiter, const TileMesh& T, const DevState& in, const DevState& out,
-                          const int* wait_in, int* wait_out, const Eos& eos, const StepCtl& ctl, hipStream_t s) {
-    const dim3 g(T.n_tiles[V]), b((TI + 2) * 64);
-#define ZFV_TILE(L) \
-    hipLaunchKernelGGL((k_tile_step<L, V, TI, WAVES>), g, b, 0, s, T, in, out, wait_in, wait_out, eos, ctl)
+static hipError_t launch_tile_t(int limiter, bool persistent, const TileMesh& T, const DevState& in,
+                                const DevState& out, const int* wait_in, int* wait_out, const Eos& eos,
+                                const StepCtl& ctl, hipStream_t s) {
+    const dim3 b((TI + 2) * 64);
+    dim3 g(T.n_tiles[V]);
+#define ZFV_TILE(L)                                                                                              \
+    do {                                                                                                         \
+        if (persistent) {                                                                                        \
+            int grid = 0;                                                                                        \
+            const hipError_t e = resident_grid<L, V, TI, WAVES>(&grid);                                          \
+            if (e != hipSuccess) return e;                                                                       \
+            hipLaunchKernelGGL((k_tile_step_p<L, V, TI, WAVES>), dim3(grid), b, 0, s, T, in, out, wait_in,       \
+                               wait_out, eos, ctl);                                                              \
+        } else {                                                                                                 \
+            hipLaunchKernelGGL((k_tile_step<L, V, TI, WAVES>), g, b, 0, s, T, in, out, wait_in, wait_out, eos,   \
+                               ctl);                                                                             \
+        }                                                                                                        \
+    } while (0)
     switch (limiter) {
         case 0: ZFV_TILE(0); break;
         case 1: ZFV_TILE(1); break;
@@ -355,21 +430,22 @@ static void launch_tile_t(int limiter, const TileMesh& T, const DevState& in, co
         default: ZFV_TILE(12); break;
     }
 #undef ZFV_TILE
+    return hipGetLastError();
 }
 
 // Tile shapes: 6 rows at 4 waves per SIMD (two workgroups per CU), 10 rows
 // at 3 (one workgroup, no register spills), 14 rows at 4 (one workgroup).
-hipError_t launch_tile_step(int limiter, int variant, const TileMesh& T, const DevState& in, const DevState& out,
-                            const int* wait_in, int* wait_out, const Eos& eos, const StepCtl& ctl, hipStream_t s) {
+hipError_t launch_tile_step(int limiter, int variant, bool persistent, const TileMesh& T, const DevState& in,
+                            const DevState& out, const int* wait_in, int* wait_out, const Eos& eos,
+                            const StepCtl& ctl, hipStream_t s) {
     if (variant < 0 || variant >= kTileVariants) return hipErrorInvalidValue;
     if (T.n_tiles[variant] <= 0) return hipSuccess;
     static_assert(kTileRows[0] == 6 && kTileRows[1] == 10 && kTileRows[2] == 14, "tile shapes");
     switch (variant) {
-        case 0: launch_tile_t<0, 6, 4>(limiter, T, in, out, wait_in, wait_out, eos, ctl, s); break;
-        case 1: launch_tile_t<1, 10, 3>(limiter, T, in, out, wait_in, wait_out, eos, ctl, s); break;
-        default: launch_tile_t<2, 14, 4>(limiter, T, in, out, wait_in, wait_out, eos, ctl, s); break;
+        case 0: return launch_tile_t<0, 6, 4>(limiter, persistent, T, in, out, wait_in, wait_out, eos, ctl, s);
+        case 1: return launch_tile_t<1, 10, 3>(limiter, persistent, T, in, out, wait_in, wait_out, eos, ctl, s);
+        default: return launch_tile_t<2, 14, 4>(limiter, persistent, T, in, out, wait_in, wait_out, eos, ctl, s);
     }
-    return hipGetLastError();
 }
 
 }  // namespace zfv
