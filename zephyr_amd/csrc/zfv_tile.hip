@@ -145,29 +145,27 @@ __device__ __forceinline__ SlotL tile_slot(const Frame& fr, int t, const Z& zc, 
 // (sm_fluid.cpp:116-125, gradient.h:226-282, 412-474), fluxes_stage1
 // (:176-253) and fluxes_stage2 (:255-366) with HLLC, plus the next step's
 // CFL candidate (:104-114).
-// The workgroup's LDS: the state box, the half states, limited gradients
-// and isolation counters of the halo box, the CFL reduction.
-template <int TI>
-struct TileLds {
-    double zs[4][(TI + 4) * kBJ];   // rho, u, v, p of the state box
-    double hs[5][(TI + 2) * kHJ];   // half state of the halo box
-    double gs[8][(TI + 2) * kHJ];   // limited gradients: d/dx then d/dy of rho, u, v, p
-    int ws[(TI + 2) * kHJ];         // isolation counters (wait_in)
-    double red[TI + 2];
-};
+// The workgroup's LDS (declared by each kernel, as separate arrays): the
+// state box zs (rho, u, v, p), the half states hs, limited gradients gs
+// (d/dx then d/dy of rho, u, v, p) and isolation counters ws of the halo
+// box, the CFL reduction red.
+#define ZFV_TILE_LDS(TI)                                  \
+    __shared__ double zs[4][((TI) + 4) * kBJ];            \
+    __shared__ double hs[5][((TI) + 2) * kHJ];            \
+    __shared__ double gs[8][((TI) + 2) * kHJ];            \
+    __shared__ int ws[((TI) + 2) * kHJ];                  \
+    __shared__ double red[(TI) + 2]
 
 // One tile b (of the tile list of shape V) by the whole workgroup.
 template <int LIM, int V, int TI>
 __device__ __forceinline__ void tile_body(const TileMesh& T, const DevState& in, const DevState& out,
                                           const int* __restrict__ wait_in, int* __restrict__ wait_out,
-                                          const Eos& eos, const StepCtl& ctl, int b, TileLds<TI>& S) {
+                                          const Eos& eos, const StepCtl& ctl, int b,
+                                          double (&zs)[4][(TI + 4) * kBJ], double (&hs)[5][(TI + 2) * kHJ],
+                                          double (&gs)[8][(TI + 2) * kHJ], int (&ws)[(TI + 2) * kHJ],
+                                          double (&red)[TI + 2]) {
     constexpr int kTileI = TI, kTileThreads = (TI + 2) * 64;
     constexpr int kBI = TI + 4, kHI = TI + 2;
-    auto& zs = S.zs;
-    auto& hs = S.hs;
-    auto& gs = S.gs;
-    auto& ws = S.ws;
-    auto& red = S.red;
 
     const int pid = problem_of<V>(T, b);
     const TileProb P = T.prob[pid];
@@ -351,8 +349,9 @@ template <int LIM, int V, int TI, int WAVES>
 __global__ void __launch_bounds__((TI + 2) * 64) __attribute__((amdgpu_waves_per_eu(WAVES)))
 k_tile_step(TileMesh T, DevState in, DevState out, const int* __restrict__ wait_in, int* __restrict__ wait_out,
             Eos eos, StepCtl ctl) {
-    __shared__ TileLds<TI> S;
-    tile_body<LIM, V, TI>(T, in, out, wait_in, wait_out, eos, ctl, xcd_block((int)blockIdx.x, (int)gridDim.x), S);
+    ZFV_TILE_LDS(TI);
+    tile_body<LIM, V, TI>(T, in, out, wait_in, wait_out, eos, ctl, xcd_block((int)blockIdx.x, (int)gridDim.x), zs, hs,
+                          gs, ws, red);
 }
 
 // The same step by persistent workgroups: the grid is the number of
@@ -364,13 +363,13 @@ template <int LIM, int V, int TI, int WAVES>
 __global__ void __launch_bounds__((TI + 2) * 64) __attribute__((amdgpu_waves_per_eu(WAVES)))
 k_tile_step_p(TileMesh T, DevState in, DevState out, const int* __restrict__ wait_in, int* __restrict__ wait_out,
               Eos eos, StepCtl ctl) {
-    __shared__ TileLds<TI> S;
+    ZFV_TILE_LDS(TI);
     const int n = T.n_tiles[V];
     const int g = (int)blockIdx.x, x = g & 7, wgs = (int)gridDim.x >> 3;
     const int q = n >> 3, rem = n & 7;
     const int begin = x * q + (x < rem ? x : rem), end = begin + q + (x < rem ? 1 : 0);
     for (int b = begin + (g >> 3); b < end; b += wgs) {
-        tile_body<LIM, V, TI>(T, in, out, wait_in, wait_out, eos, ctl, b, S);
+        tile_body<LIM, V, TI>(T, in, out, wait_in, wait_out, eos, ctl, b, zs, hs, gs, ws, red);
         __syncthreads();   // the next tile's loads overwrite the LDS boxes
     }
 }
