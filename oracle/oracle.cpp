@@ -1395,3 +1395,4 @@ void orc_init_quadrants(const orc_mesh* m, uint64_t seed, double gamma,
 #include "oracle_amr.hpp"
 #include "oracle_mm.hpp"
 #include "oracle_transfer.hpp"
+#include "oracle_3d.hpp"

@@ -192,6 +192,31 @@ void orc_tr_update(orc_tr* T, int dir);
 void orc_tr_slopes(const orc_tr* T, double* dx, double* dy);
 void orc_tr_set_flags(orc_tr* T);   /* into the mesh's flags */
 
+/* Three-dimensional Cuboid meshes and the SmFluid step on them
+ * (oracle_3d.hpp); 24 face slots per cell, cell-major. */
+typedef struct orc_mesh3 orc_mesh3;
+typedef struct orc_smf3 orc_smf3;
+orc_mesh3* orc_mesh3_cuboid(int nx, int ny, int nz, double xmin, double xmax, double ymin, double ymax,
+                            double zmin, double zmax, const int* bounds);
+void orc_mesh3_free(orc_mesh3* m);
+int orc_mesh3_size(const orc_mesh3* m);
+void orc_mesh3_get_cells(const orc_mesh3* m, double* cx, double* cy, double* cz, double* vol, double* h);
+void orc_mesh3_get_slots(const orc_mesh3* m, int* boundary, int* adj, double* nx, double* ny, double* nz,
+                         double* fcx, double* fcy, double* fcz, double* area, double* w);
+orc_smf3* orc_smf3_create(orc_mesh3* m, double gamma, double P0, double e0, int flux, int limiter, int acc,
+                          double cfl);
+void orc_smf3_free(orc_smf3* s);
+void orc_smf3_set_state(orc_smf3* s, const double* rho, const double* u, const double* v, const double* w,
+                        const double* p, const double* e);
+void orc_smf3_get_state(const orc_smf3* s, double* rho, double* u, double* v, double* w, double* p, double* e);
+void orc_smf3_get_wait(const orc_smf3* s, int* wait);
+void orc_smf3_get_grad(const orc_smf3* s, double* g);
+double orc_smf3_compute_dt(orc_smf3* s, int nthreads);
+int orc_smf3_update(orc_smf3* s, int nthreads);
+double orc_smf3_dt(const orc_smf3* s);
+void orc_smf3_totals(const orc_smf3* s, double* out);
+void orc_rotate3(const double* v, const double* n, int to_local, double* out);
+
 #ifdef __cplusplus
 }
 #endif

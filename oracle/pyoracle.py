@@ -121,6 +121,22 @@ def lib():
             "orc_quad_find_section": (_D, [_D] * 5),
             "orc_face_fraction": (_D, [_I, _D, _D]),
             "orc_average_flux": (_D, [_D] * 3),
+            "orc_mesh3_cuboid": (_P, [_I, _I, _I, _D, _D, _D, _D, _D, _D, _ip]),
+            "orc_mesh3_free": (None, [_P]),
+            "orc_mesh3_size": (_I, [_P]),
+            "orc_mesh3_get_cells": (None, [_P] + [_dp] * 5),
+            "orc_mesh3_get_slots": (None, [_P, _ip, _ip] + [_dp] * 8),
+            "orc_smf3_create": (_P, [_P, _D, _D, _D, _I, _I, _I, _D]),
+            "orc_smf3_free": (None, [_P]),
+            "orc_smf3_set_state": (None, [_P] + [_dp] * 6),
+            "orc_smf3_get_state": (None, [_P] + [_dp] * 6),
+            "orc_smf3_get_wait": (None, [_P, _ip]),
+            "orc_smf3_get_grad": (None, [_P, _dp]),
+            "orc_smf3_compute_dt": (_D, [_P, _I]),
+            "orc_smf3_update": (_I, [_P, _I]),
+            "orc_smf3_dt": (_D, [_P]),
+            "orc_smf3_totals": (None, [_P, _dp]),
+            "orc_rotate3": (None, [_dp, _dp, _I, _dp]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -326,6 +342,98 @@ class SmFluid:
     def totals(self):
         out = np.zeros(4)
         lib().orc_smf_totals(self.h, _d(out))
+        return out
+
+
+# ---- three-dimensional Cuboid meshes and SmFluid -----------------------------
+def rotate3(v, n, to_local=True):
+    """Rotate::to_local / to_global of one 3-vector about the normal n."""
+    a, b, out = np.asarray(v, np.float64), np.asarray(n, np.float64), np.zeros(3)
+    lib().orc_rotate3(_d(a), _d(b), 1 if to_local else 0, _d(out))
+    return out
+
+
+class Mesh3:
+    """Cuboid mesh of AMR-3D cells (24 face slots per cell, the six main
+    ones defined): cell c = nz (ny i + j) + k; bounds L, R, B, T, Z, F."""
+
+    def __init__(self, nx, ny, nz, xmin=0.0, xmax=1.0, ymin=0.0, ymax=1.0, zmin=0.0, zmax=1.0,
+                 bounds=(ZOE,) * 6):
+        b = np.ascontiguousarray(bounds, dtype=np.int32)
+        self.shape = (nx, ny, nz)
+        self.h = lib().orc_mesh3_cuboid(nx, ny, nz, xmin, xmax, ymin, ymax, zmin, zmax, _i(b))
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.orc_mesh3_free(self.h)
+            self.h = None
+
+    @property
+    def n(self):
+        return lib().orc_mesh3_size(self.h)
+
+    def cells(self):
+        n = self.n
+        a = {k: np.zeros(n) for k in ("cx", "cy", "cz", "volume", "h")}
+        lib().orc_mesh3_get_cells(self.h, *[_d(a[k]) for k in ("cx", "cy", "cz", "volume", "h")])
+        return a
+
+    def slots(self):
+        ns = 24 * self.n
+        a = {"boundary": np.zeros(ns, np.int32), "adj": np.zeros(ns, np.int32)}
+        for k in ("nx", "ny", "nz", "fcx", "fcy", "fcz", "area", "w"):
+            a[k] = np.zeros(ns)
+        lib().orc_mesh3_get_slots(self.h, _i(a["boundary"]), _i(a["adj"]),
+                                  *[_d(a[k]) for k in ("nx", "ny", "nz", "fcx", "fcy", "fcz", "area", "w")])
+        return a
+
+
+class SmFluid3:
+    """Oracle SmFluid on a Cuboid mesh (oracle_3d.hpp)."""
+
+    def __init__(self, mesh, gamma=1.4, P0=0.0, e0=0.0, flux=HLLC, limiter="MC", acc=1, cfl=0.5):
+        self.mesh = mesh
+        lim = LIMITERS.index(limiter) if isinstance(limiter, str) else int(limiter)
+        self.h = lib().orc_smf3_create(mesh.h, gamma, P0, e0, flux, lim, acc, cfl)
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.orc_smf3_free(self.h)
+            self.h = None
+
+    def set_state(self, rho, u, v, w, p, e):
+        arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (rho, u, v, w, p, e)]
+        lib().orc_smf3_set_state(self.h, *[_d(a) for a in arrs])
+
+    def state(self):
+        arrs = [np.zeros(self.mesh.n) for _ in range(6)]
+        lib().orc_smf3_get_state(self.h, *[_d(a) for a in arrs])
+        return arrs
+
+    def wait(self):
+        w = np.zeros(self.mesh.n, np.int32)
+        lib().orc_smf3_get_wait(self.h, _i(w))
+        return w
+
+    def grad(self):
+        """d/dx, d/dy, d/dz of rho, u, v, w, p: [3][5][n]."""
+        g = np.zeros(15 * self.mesh.n)
+        lib().orc_smf3_get_grad(self.h, _d(g))
+        return g.reshape(3, 5, self.mesh.n)
+
+    def update(self, nthreads=None):
+        nt = nthreads or min(8, os.cpu_count() or 1)
+        return lib().orc_smf3_update(self.h, nt)
+
+    def dt(self):
+        return lib().orc_smf3_dt(self.h)
+
+    def compute_dt(self, nthreads=1):
+        return lib().orc_smf3_compute_dt(self.h, nthreads)
+
+    def totals(self):
+        out = np.zeros(5)
+        lib().orc_smf3_totals(self.h, _d(out))
         return out
 
 
