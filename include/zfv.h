@@ -485,6 +485,57 @@ int zfv_get_curve_order(zfv_ctx* ctx, int32_t* perm);
 int zfv_profile_enable(zfv_ctx* ctx, int on);
 int zfv_profile_read(zfv_ctx* ctx, const char** names, double* ms, int32_t* launches, int32_t* count);
 
+/* ---- three-dimensional meshes (EuMesh over AMR-3D cells, one level) ---- */
+/* A context holding a three-dimensional Cuboid mesh and its SmFluid state.
+ * No reference counterpart as a handle (EuMesh(Cuboid) + SmFluid,
+ * eu_mesh.cpp:70-104, sm_fluid.cpp:16-35). */
+typedef struct zfv3_ctx zfv3_ctx;
+int zfv3_create(int device, zfv3_ctx** out);
+int zfv3_destroy(zfv3_ctx* ctx);
+/* Build the Cuboid mesh on the host and upload it (replaces
+ * generator::Cuboid + Cuboid::initialize, geom/generator/cuboid.cpp:300-442):
+ * cell c = nz (ny i + j) + k; bounds = {left, right, bottom, top, back,
+ * front} Boundary ids (WALL, ZOE, PERIODIC, FUNCTION; NULL = ZOE; a periodic
+ * side makes its direction periodic on both ends). */
+int zfv3_mesh_cuboid(zfv3_ctx* ctx, int nx, int ny, int nz, double xmin, double xmax, double ymin, double ymax,
+                     double zmin, double zmax, const int32_t* bounds);
+int zfv3_mesh_n_cells(zfv3_ctx* ctx, int32_t* n);
+/* The mesh in the AMR-3D layout (mesh/euler/amr_cells.h:74-104,
+ * amr_faces.h:83-101): per cell cx, cy, cz, volume, incircle diameter h
+ * (amr_cells.cpp:107-121); 24 face slots per cell (slot s of cell c at
+ * 24 c + s, sides L, R, B, T, Z, F then their sub-faces, which a Cuboid
+ * leaves undefined: boundary -1, adj -1, zeros), with the Boundary id, the
+ * neighbour (the cell itself on a non-periodic boundary), the outward
+ * normal, face centre, area and least-squares weight w = area / (cbrt(V)
+ * |drn|^3) (gradient.h:231-256).  NULL arrays are skipped. */
+int zfv3_mesh_download(zfv3_ctx* ctx, double* cx, double* cy, double* cz, double* vol, double* h,
+                       int32_t* boundary, int32_t* adj, double* nx, double* ny, double* nz, double* fcx,
+                       double* fcy, double* fcz, double* area, double* w);
+/* SmFluid setters as zfv_smf_configure; flux Rusanov, HLL or HLLC
+ * (ZFV_ERR_UNSUPPORTED otherwise), axial 0. */
+int zfv3_smf_configure(zfv3_ctx* ctx, const zfv_smf_params* params);
+int zfv3_smf_set_max_dt(zfv3_ctx* ctx, double max_dt);
+/* Primitive state PState{rho, {u, v, w}, p, e} of the "init" layer, n
+ * doubles per field; e NULL = energy_rP(rho, p) (stiffened_gas.cpp:121-122). */
+int zfv3_state_upload(zfv3_ctx* ctx, const double* rho, const double* u, const double* v, const double* w,
+                      const double* p, const double* e);
+int zfv3_state_download(zfv3_ctx* ctx, double* rho, double* u, double* v, double* w, double* p, double* e);
+/* Isolation counters (Parts::wait, sm_fluid.h:34-35). */
+int zfv3_wait_upload(zfv3_ctx* ctx, const int32_t* wait);
+int zfv3_wait_download(zfv3_ctx* ctx, int32_t* wait);
+/* Limited gradients of {rho, u, v, w, p} of the last second-order step:
+ * [3][5][n] (d/dx, d/dy, d/dz; compute_grad, sm_fluid.cpp:116-125). */
+int zfv3_grad_download(zfv3_ctx* ctx, double* g);
+/* `n_steps` calls of SmFluid::update (sm_fluid.cpp:81-102), enqueued; NaN
+ * fluxes are sticky and reported by the next synchronising call. */
+int zfv3_smf_step(zfv3_ctx* ctx, int n_steps);
+/* SmFluid::dt() of the last step; compute_dt (sm_fluid.cpp:104-114) alone. */
+int zfv3_smf_dt(zfv3_ctx* ctx, double* dt);
+int zfv3_smf_compute_dt(zfv3_ctx* ctx, double* dt);
+/* {mass, mom_x, mom_y, mom_z, energy} = sum V q (synchronises). */
+int zfv3_conserved_totals(zfv3_ctx* ctx, double* out5);
+int zfv3_synchronize(zfv3_ctx* ctx);
+
 #ifdef __cplusplus
 }
 #endif

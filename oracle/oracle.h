@@ -210,6 +210,7 @@ void orc_smf3_set_state(orc_smf3* s, const double* rho, const double* u, const d
                         const double* p, const double* e);
 void orc_smf3_get_state(const orc_smf3* s, double* rho, double* u, double* v, double* w, double* p, double* e);
 void orc_smf3_get_wait(const orc_smf3* s, int* wait);
+void orc_smf3_set_wait(orc_smf3* s, const int* wait);
 void orc_smf3_get_grad(const orc_smf3* s, double* g);
 double orc_smf3_compute_dt(orc_smf3* s, int nthreads);
 int orc_smf3_update(orc_smf3* s, int nthreads);

@@ -412,6 +412,7 @@ void orc_smf3_get_state(const orc_smf3* s, double* rho, double* u, double* v, do
 }
 
 void orc_smf3_get_wait(const orc_smf3* s, int* wait) { std::memcpy(wait, s->wait.data(), sizeof(int) * s->m->n); }
+void orc_smf3_set_wait(orc_smf3* s, const int* wait) { std::memcpy(s->wait.data(), wait, sizeof(int) * s->m->n); }
 
 // d/dx, d/dy, d/dz of rho, u, v, w, p: [3][5][n]
 void orc_smf3_get_grad(const orc_smf3* s, double* g) {

@@ -131,6 +131,7 @@ def lib():
             "orc_smf3_set_state": (None, [_P] + [_dp] * 6),
             "orc_smf3_get_state": (None, [_P] + [_dp] * 6),
             "orc_smf3_get_wait": (None, [_P, _ip]),
+            "orc_smf3_set_wait": (None, [_P, _ip]),
             "orc_smf3_get_grad": (None, [_P, _dp]),
             "orc_smf3_compute_dt": (_D, [_P, _I]),
             "orc_smf3_update": (_I, [_P, _I]),
@@ -414,6 +415,10 @@ class SmFluid3:
         w = np.zeros(self.mesh.n, np.int32)
         lib().orc_smf3_get_wait(self.h, _i(w))
         return w
+
+    def set_wait(self, w):
+        w = np.ascontiguousarray(w, dtype=np.int32)
+        lib().orc_smf3_set_wait(self.h, _i(w))
 
     def grad(self):
         """d/dx, d/dy, d/dz of rho, u, v, w, p: [3][5][n]."""

@@ -16,14 +16,14 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 def header_symbols():
     src = open(os.path.join(ROOT, "include", "zfv.h")).read()
     src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
-    return sorted(set(re.findall(r"\b(zfv_[a-z0-9_]+)\s*\(", src)))
+    return sorted(set(re.findall(r"\b(zfv3?_[a-z0-9_]+)\s*\(", src)))
 
 
 def test_library_exports_every_header_symbol(zfv):
     syms = header_symbols()
     assert len(syms) >= 25
     out = subprocess.run(["nm", "-D", "--defined-only", zfv.LIB_PATH], capture_output=True, text=True).stdout
-    exported = set(re.findall(r" T (zfv_\w+)", out))
+    exported = set(re.findall(r" T (zfv3?_\w+)", out))
     missing = [s for s in syms if s not in exported]
     assert not missing, missing
     assert sorted(zfv.EXPORTS) == syms

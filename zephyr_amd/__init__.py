@@ -45,6 +45,10 @@ EXPORTS = [
     "zfv_transfer_update", "zfv_transfer_set_flags",
     "zfv_smf_get_params", "zfv_vtu_write", "zfv_pvd_open", "zfv_pvd_save", "zfv_pvd_close",
     "zfv_backup", "zfv_restore",
+    "zfv3_create", "zfv3_destroy", "zfv3_mesh_cuboid", "zfv3_mesh_n_cells", "zfv3_mesh_download",
+    "zfv3_smf_configure", "zfv3_smf_set_max_dt", "zfv3_state_upload", "zfv3_state_download",
+    "zfv3_wait_upload", "zfv3_wait_download", "zfv3_grad_download", "zfv3_smf_step", "zfv3_smf_dt",
+    "zfv3_smf_compute_dt", "zfv3_conserved_totals", "zfv3_synchronize",
 ]
 VTU_DENSITY, VTU_VELOCITY, VTU_PRESSURE, VTU_ENERGY, VTU_WAIT, VTU_LEVEL, VTU_FLAG, VTU_INDEX = (
     1, 2, 4, 8, 16, 32, 64, 128)
@@ -154,6 +158,13 @@ def lib():
         L.zfv_mm_grad.argtypes = [vp, vp, vp]
         L.zfv_adjoint_forward.argtypes = [vp, C.c_int, C.c_int, vp, vp]
         L.zfv_adjoint_backward.argtypes = [vp, vp, vp, vp, vp, vp]
+        L.zfv3_create.argtypes = [C.c_int, C.POINTER(vp)]
+        L.zfv3_mesh_cuboid.argtypes = [vp, C.c_int, C.c_int, C.c_int, d, d, d, d, d, d, vp]
+        L.zfv3_mesh_download.argtypes = [vp] * 16
+        L.zfv3_smf_set_max_dt.argtypes = [vp, d]
+        L.zfv3_state_upload.argtypes = [vp] * 7
+        L.zfv3_state_download.argtypes = [vp] * 7
+        L.zfv3_smf_step.argtypes = [vp, C.c_int]
         _lib = L
     return _lib
 
@@ -661,3 +672,102 @@ class Context:
         count = C.c_int32(cap)
         _check(lib().zfv_profile_read(self.h, names, ms, cnt, C.byref(count)))
         return {names[i].decode(): (ms[i], cnt[i]) for i in range(count.value)}
+
+
+class Context3:
+    """A three-dimensional Cuboid mesh + SmFluid state on the device (a zfv3_ctx)."""
+
+    def __init__(self, device=0):
+        h = C.c_void_p()
+        _check(lib().zfv3_create(device, C.byref(h)))
+        self.h = h
+        self.n = 0
+
+    def close(self):
+        if getattr(self, "h", None) and self.h.value:
+            lib().zfv3_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def mesh_cuboid(self, nx, ny, nz, xmin=0.0, xmax=1.0, ymin=0.0, ymax=1.0, zmin=0.0, zmax=1.0,
+                    bounds=(ZOE,) * 6):
+        """Cuboid (geom/generator/cuboid.cpp): bounds left, right, bottom, top, back, front."""
+        b = np.ascontiguousarray(bounds, dtype=np.int32)
+        _check(lib().zfv3_mesh_cuboid(self.h, nx, ny, nz, xmin, xmax, ymin, ymax, zmin, zmax, _ptr(b)))
+        self.n = nx * ny * nz
+
+    def mesh(self):
+        """(cells, slots): cells cx, cy, cz, vol, h (n); slots boundary, adj, nx, ny, nz, fcx, fcy,
+        fcz, area, w (24 per cell, cell-major)."""
+        n = self.n
+        cells = {k: np.zeros(n) for k in ("cx", "cy", "cz", "vol", "h")}
+        slots = {"boundary": np.zeros(24 * n, np.int32), "adj": np.zeros(24 * n, np.int32)}
+        slots.update({k: np.zeros(24 * n) for k in ("nx", "ny", "nz", "fcx", "fcy", "fcz", "area", "w")})
+        args = [cells[k] for k in ("cx", "cy", "cz", "vol", "h")] + [
+            slots[k] for k in ("boundary", "adj", "nx", "ny", "nz", "fcx", "fcy", "fcz", "area", "w")]
+        _check(lib().zfv3_mesh_download(self.h, *[_ptr(a) for a in args]))
+        return cells, slots
+
+    def configure(self, acc=1, flux=HLLC, limiter="MC", cfl=0.5, gamma=1.4, P0=0.0, e0=0.0, max_dt=None):
+        if isinstance(flux, str):
+            flux = FLUXES[flux.lower()]
+        p = SmfParams(acc=acc, flux=flux, limiter=limiter_id(limiter), axial=0, cfl=cfl,
+                      max_dt=np.finfo(np.float64).max if max_dt is None else max_dt,
+                      gamma=gamma, P0=P0, e0=e0)
+        _check(lib().zfv3_smf_configure(self.h, C.byref(p)))
+
+    def set_max_dt(self, v):
+        _check(lib().zfv3_smf_set_max_dt(self.h, v))
+
+    def set_state(self, rho, u, v, w, p, e=None):
+        arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (rho, u, v, w, p)]
+        ea = None if e is None else np.ascontiguousarray(e, dtype=np.float64)
+        _check(lib().zfv3_state_upload(self.h, *[_ptr(a) for a in arrs], _ptr(ea)))
+
+    def state(self):
+        """rho, u, v, w, p, e."""
+        arrs = [np.zeros(self.n) for _ in range(6)]
+        _check(lib().zfv3_state_download(self.h, *[_ptr(a) for a in arrs]))
+        return arrs
+
+    def set_wait(self, w):
+        w = np.ascontiguousarray(w, dtype=np.int32)
+        _check(lib().zfv3_wait_upload(self.h, _ptr(w)))
+
+    def wait(self):
+        w = np.zeros(self.n, np.int32)
+        _check(lib().zfv3_wait_download(self.h, _ptr(w)))
+        return w
+
+    def grad(self):
+        """[3][5][n]: d/dx, d/dy, d/dz of rho, u, v, w, p."""
+        g = np.zeros(15 * self.n)
+        _check(lib().zfv3_grad_download(self.h, _ptr(g)))
+        return g.reshape(3, 5, self.n)
+
+    def step(self, n=1):
+        _check(lib().zfv3_smf_step(self.h, n))
+
+    def dt(self):
+        v = C.c_double()
+        _check(lib().zfv3_smf_dt(self.h, C.byref(v)))
+        return v.value
+
+    def compute_dt(self):
+        v = C.c_double()
+        _check(lib().zfv3_smf_compute_dt(self.h, C.byref(v)))
+        return v.value
+
+    def totals(self):
+        """mass, momentum x, y, z, energy."""
+        out = np.zeros(5)
+        _check(lib().zfv3_conserved_totals(self.h, _ptr(out)))
+        return out
+
+    def synchronize(self):
+        _check(lib().zfv3_synchronize(self.h))

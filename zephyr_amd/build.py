@@ -22,6 +22,8 @@ SOURCES = [
     os.path.join(HERE, "csrc", "zfv_order.hip"),
     os.path.join(HERE, "csrc", "zfv_tile.hip"),
     os.path.join(HERE, "csrc", "zfv_transfer.hip"),
+    os.path.join(HERE, "csrc", "zfv_3d.hip"),
+    os.path.join(HERE, "csrc", "zfv_3d.cpp"),
     os.path.join(HERE, "csrc", "zfv_capi.cpp"),
     os.path.join(HERE, "csrc", "zfv_io.cpp"),
 ]
@@ -32,6 +34,7 @@ HEADERS = [
     os.path.join(HERE, "csrc", "zfv_step_common.hpp"),
     os.path.join(HERE, "csrc", "zfv_step_faces.hpp"),
     os.path.join(HERE, "csrc", "zfv_mm.hpp"),
+    os.path.join(HERE, "csrc", "zfv_3d.hpp"),
     os.path.join(ROOT, "include", "zfv.h"),
 ]
 
