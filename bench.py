@@ -112,6 +112,11 @@ WORKLOADS = {
              "four-quadrant Riemann data (seed + problem index), MUSCL-Hancock + MC + HLLC, FP64, CFL 0.5, "
              "per-problem dt; one context, one launch per stage for all problems, captured in HIP graphs, "
              "launch configuration autotuned; compared with the plain per-problem path looped over 256 contexts"),
+    "cube128_muscl": dict(
+        kind="cube", nx=128, ny=128, nz=128, acc=2, limiter="MC", steps=100,
+        desc="128x128x128 Cuboid mesh (AMR-3D cells, one level), MUSCL-Hancock + MC + HLLC, CFL 0.5, FP64, ZOE "
+             "bounds; seeded spherical blast (radius 0.25: p = 10, rho = 1.5 inside; p = rho = 1 outside) with "
+             "a 1% seeded sinusoidal density perturbation and a smooth velocity field of amplitude 0.05"),
 }
 
 
@@ -730,6 +735,100 @@ def ensemble_cpu_baseline(w, sizes, seed, budget_s):
             "host": host_info()}
 
 
+# Compulsory bytes per cell of the three-dimensional step kernels: every
+# array the kernel touches, once (neighbour values served from cache).
+# Slot-major mesh: per slot type + adj (8), face centre (24), area (8) and,
+# for the gradient, the weight (8); cell centre 24, volume 8, incircle 8;
+# state rho, u, v, w, p, e (48); gradients 3 x 5 doubles (120); wait 4.
+# The first-order update and the corrector also reduce the next step's CFL
+# candidate (incircle, 8), so the standalone reduction runs only after a new
+# state is set.
+CUBE_BYTES = {
+    "dt": 40 + 8,
+    "flux1": 48 + 6 * 16 + 8 + 48 + 8,
+    "grad_pred": 48 + 24 + 6 * 48 + 4 + 8 + 120 + 48,
+    "corr": 48 + 48 + 120 + 4 + 4 + 24 + 8 + 6 * 40 + 48 + 8,
+}
+
+
+def cube_state(cx, cy, cz, seed):
+    rng = np.random.default_rng(seed)
+    k = rng.uniform(2.0, 6.0, size=(4, 3))
+    ph = rng.uniform(0.0, 2.0 * np.pi, size=4)
+    r = np.sqrt((cx - 0.5) ** 2 + (cy - 0.5) ** 2 + (cz - 0.5) ** 2)
+    inside = r < 0.25
+    rho = np.where(inside, 1.5, 1.0) * (1.0 + 0.01 * np.sin(k[0, 0] * cx + k[0, 1] * cy + k[0, 2] * cz + ph[0]))
+    u = 0.05 * np.sin(k[1, 0] * cx + k[1, 1] * cy + k[1, 2] * cz + ph[1])
+    v = 0.05 * np.sin(k[2, 0] * cx + k[2, 1] * cy + k[2, 2] * cz + ph[2])
+    w = 0.05 * np.sin(k[3, 0] * cx + k[3, 1] * cy + k[3, 2] * cz + ph[3])
+    p = np.where(inside, 10.0, 1.0)
+    return rho, u, v, w, p, p / (0.4 * rho)
+
+
+def cube_cpu_baseline(w, seed, budget_s):
+    """The oracle's SmFluid3 (C++, OpenMP) on the same mesh and initial data,
+    whole steps until about budget_s seconds are spent."""
+    from oracle import pyoracle as O
+    nt = cpu_threads()
+    m = O.Mesh3(w["nx"], w["ny"], w["nz"])
+    f = O.SmFluid3(m, flux=O.HLLC, acc=w["acc"], limiter=w["limiter"], cfl=0.5)
+    c = m.cells()
+    f.set_state(*cube_state(c["cx"], c["cy"], c["cz"], seed))
+    steps, el = 0, 0.0
+    while el < budget_s:
+        t0 = time.perf_counter()
+        f.update(nt)
+        el += time.perf_counter() - t0
+        steps += 1
+    n = m.n
+    return {"value": n * steps / el, "unit": "cell-updates/s", "cores": nt, "kind": "port",
+            "sample": f"oracle SmFluid3 (C++ -O3 -ffp-contract=off, OpenMP {nt} threads): the first {steps} "
+                      f"steps of the same {w['nx']}x{w['ny']}x{w['nz']} workload from the same initial data in "
+                      f"{el:.2f} s", "host": host_info()}
+
+
+def cube_run(args, w, Z):
+    ctx = Z.Context3()
+    ctx.mesh_cuboid(w["nx"], w["ny"], w["nz"])
+    ctx.configure(acc=w["acc"], flux="hllc", limiter=w["limiter"], cfl=0.5, gamma=1.4)
+    cells, _ = ctx.mesh()
+    ctx.set_state(*cube_state(cells["cx"], cells["cy"], cells["cz"], args.seed))
+    del cells
+    n = ctx.n
+    ctx.step(args.warmup)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    ctx.step(args.steps)
+    ctx.synchronize()
+    el = time.perf_counter() - t0
+    # kernel timing pass: the workload continues with events around each launch
+    ctx.profile_enable(True)
+    ctx.step(min(args.prof_steps, 20))
+    prof = ctx.profile_read()
+    ctx.profile_enable(False)
+    avg = {k: v[0] / v[1] for k, v in prof.items()}
+    dom = max((k for k in ("grad_pred", "corr", "flux1") if k in avg), key=lambda k: avg[k])
+    alg = CUBE_BYTES[dom] * n
+    achieved = alg / (avg[dom] * 1e-3) / 1e9
+    value = n * args.steps / el
+    out = {
+        "metric": "FP64 cell-updates/s (HLLC+MUSCL, 3D Cuboid)",
+        "value": value, "unit": "cell-updates/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": el * 1e3 / args.steps, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": args.workload, "description": w["desc"], "cells": n,
+                   "kernel_ms_avg": avg, "bytes_per_cell": CUBE_BYTES,
+                   "step_compulsory_GBs": sum(CUBE_BYTES[k] for k in ("grad_pred", "corr")) * n
+                   / (el / args.steps) / 1e9},
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "algorithmic_bytes_per_launch": alg,
+                     "avg_launch_ms": avg[dom]},
+    }
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cube_cpu_baseline(w, args.seed, args.cpu_budget)
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -766,6 +865,8 @@ def main():
         return adjoint_run(args, w, Z)
     if w["kind"] == "mm":
         return mm_run(args, w, Z)
+    if w["kind"] == "cube":
+        return cube_run(args, w, Z)
     nx, ny, acc, limiter = w["nx"], w["ny"], w["acc"], w["limiter"]
     ctx = Z.Context()
     ctx.mesh_rectangle(nx, ny)

@@ -326,9 +326,12 @@ int zfv_mm_interface(zfv_ctx* ctx, double* nx, double* ny, double* p);
  * c at 8c + s) and at the cell centres (cvx, cvy: n), the points the
  * reference evaluates it at.  Methods (Transfer::Method numbering): 0
  * CRP_V3, 1 CRP_V5, 2 CRP_SE, 3 CRP_N1, 7 MUSCLd, 9 MUSCLd_CRP, 11 MUSCL_MC,
- * 12 MUSCL_MC_CRP; 4 CRP_N2, 5 VOF, 6 VOF_CRP, 8 MUSCLn, 10 MUSCLn_CRP
- * (sub-cell polygon reconstruction) and 13 WENO, 14 WENO_CRP (structured
- * stencils) are not built (ZFV_ERR_UNSUPPORTED).  The interface normals are
+ * 12 MUSCL_MC_CRP, 13 WENO, 14 WENO_CRP (update_WENO, transfer.cpp:541-666:
+ * five-cell stencils on the (i, j) grid of an unadapted Rectangle of at
+ * least 3 x 3 cells, indices wrapped periodically; ZFV_ERR_UNSUPPORTED at
+ * update on other meshes); 4 CRP_N2, 5 VOF, 6 VOF_CRP, 8 MUSCLn, 10
+ * MUSCLn_CRP (sub-cell polygon reconstruction) are not built
+ * (ZFV_ERR_UNSUPPORTED).  The interface normals are
  * those of InterfaceRecovery::compute_normals after each update (no
  * smoothing); an adaptation invalidates the state (set it again). */
 /* Transfer(), set_method, set_CFL (transfer.cpp:43-71). */
@@ -535,6 +538,11 @@ int zfv3_smf_compute_dt(zfv3_ctx* ctx, double* dt);
 /* {mass, mom_x, mom_y, mom_z, energy} = sum V q (synchronises). */
 int zfv3_conserved_totals(zfv3_ctx* ctx, double* out5);
 int zfv3_synchronize(zfv3_ctx* ctx);
+/* Kernel timing as zfv_profile_enable / zfv_profile_read: total device time
+ * (ms) and launches of each step kernel (dt, flux1, grad_pred, corr) since
+ * timing was enabled or last read (HIP events on the context stream). */
+int zfv3_profile_enable(zfv3_ctx* ctx, int on);
+int zfv3_profile_read(zfv3_ctx* ctx, const char** names, double* ms, int32_t* launches, int32_t* count);
 
 #ifdef __cplusplus
 }

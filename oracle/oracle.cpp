@@ -706,6 +706,9 @@ double lim_binary(int kind, double a, double b) {
 // =============================================================== mesh
 struct orc_mesh {
     int n = 0;
+    // the structured (i, j) grid of a Rectangle, cell = gny i + j (EuMesh::nx,
+    // ny and operator()(i, j), eu_mesh.cpp:826-830); 0 on other meshes
+    int gnx = 0, gny = 0;
     // cells (amr_cells.h:74-104); idx < 0 marks an undefined cell during AMR
     std::vector<double> cx, cy, vol;
     std::vector<int> level, flag, b_idx, z_idx, next, idx;
@@ -763,6 +766,8 @@ orc_mesh* orc_mesh_rectangle_ax(int nx, int ny, double xmin, double xmax,
     }
     int n = nx * ny;
     m->n = n;
+    m->gnx = nx;
+    m->gny = ny;
     m->cx.resize(n); m->cy.resize(n); m->vol.resize(n);
     m->level.assign(n, 0); m->flag.assign(n, 0); m->b_idx.resize(n); m->z_idx.assign(n, 0);
     m->next.resize(n); m->idx.resize(n);

@@ -189,6 +189,7 @@ void orc_tr_state(const orc_tr* T, double* u, double* nx, double* ny);
 void orc_tr_set_dt(orc_tr* T, double dt);
 double orc_tr_compute_dt(const orc_tr* T);
 void orc_tr_update(orc_tr* T, int dir);
+double orc_weno5(const double* u, int right);
 void orc_tr_slopes(const orc_tr* T, double* dx, double* dy);
 void orc_tr_set_flags(orc_tr* T);   /* into the mesh's flags */
 

@@ -9,9 +9,9 @@
 // included at the end of oracle.cpp.
 //
 // Built methods (Transfer::Method numbering): CRP_V3 0, CRP_V5 1, CRP_SE 2,
-// CRP_N1 3, MUSCLd 7, MUSCLd_CRP 9, MUSCL_MC 11, MUSCL_MC_CRP 12.  CRP_N2,
-// VOF, VOF_CRP, MUSCLn, MUSCLn_CRP (sub-cell polygon reconstruction) and
-// WENO / WENO_CRP (structured stencils) are not.
+// CRP_N1 3, MUSCLd 7, MUSCLd_CRP 9, MUSCL_MC 11, MUSCL_MC_CRP 12, WENO 13,
+// WENO_CRP 14 (on a Rectangle's structured grid).  CRP_N2, VOF, VOF_CRP,
+// MUSCLn, MUSCLn_CRP (sub-cell polygon reconstruction) are not.
 
 struct orc_tr {
     orc_mesh* m = nullptr;
@@ -32,7 +32,37 @@ enum {
 
 inline bool tr_built(int k) {
     return k == kTrV3 || k == kTrV5 || k == kTrSE || k == kTrN1 || k == kTrMUSCLd || k == kTrMUSCLdCRP ||
-           k == kTrMC || k == kTrMCCRP;
+           k == kTrMC || k == kTrMCCRP || k == kTrWENO || k == kTrWENOCRP;
+}
+
+// WENO<b, e> (math/calc/weno.h:39-120) on three cell averages a, b, c of a
+// uniform grid: the face values u(+1/2) (p) and u(-1/2) (m) of the
+// quadratic interpolant, w1 a + w2 b + w3 c summed left to right, and the
+// smoothness indicator 13/12 d2^2 + 1/4 d1^2.
+inline double weno_p_m2(double a, double b, double c) { return ((1.0 / 3.0) * a + (-7.0 / 6.0) * b) + (11.0 / 6.0) * c; }
+inline double weno_p_m1(double a, double b, double c) { return ((-1.0 / 6.0) * a + (5.0 / 6.0) * b) + (1.0 / 3.0) * c; }
+inline double weno_p_0(double a, double b, double c) { return ((1.0 / 3.0) * a + (5.0 / 6.0) * b) + (-1.0 / 6.0) * c; }
+inline double weno_m_m2(double a, double b, double c) { return ((-1.0 / 6.0) * a + (5.0 / 6.0) * b) + (1.0 / 3.0) * c; }
+inline double weno_m_m1(double a, double b, double c) { return ((1.0 / 3.0) * a + (5.0 / 6.0) * b) + (-1.0 / 6.0) * c; }
+inline double weno_m_0(double a, double b, double c) { return ((11.0 / 6.0) * a + (-7.0 / 6.0) * b) + (1.0 / 3.0) * c; }
+inline double weno_beta(double d1, double d2) { return (13.0 / 12.0) * d2 * d2 + (1.0 / 4.0) * d1 * d1; }
+inline double weno_b_m2(double a, double b, double c) { return weno_beta((a - 4.0 * b) + 3.0 * c, (a - 2.0 * b) + c); }
+inline double weno_b_m1(double a, double b, double c) { return weno_beta(a - c, (a - 2.0 * b) + c); }
+inline double weno_b_0(double a, double b, double c) { return weno_beta((3.0 * a - 4.0 * b) + c, (a - 2.0 * b) + c); }
+
+// WENO5 (weno.h:126-170) on u1..u5: the sub-stencils we2 = {u1, u2, u3},
+// we1 = {u2, u3, u4}, we0 = {u3, u4, u5}, linear weights 3/10, 6/10, 1/10
+// (p) or 1/10, 6/10, 3/10 (m) divided by (1e-8 + beta)^2 and normalised.
+inline double weno5(const double* u, bool right) {
+    const double b0 = weno_b_0(u[2], u[3], u[4]);
+    const double b1 = weno_b_m1(u[1], u[2], u[3]);
+    const double b2 = weno_b_m2(u[0], u[1], u[2]);
+    const double d0c = right ? 3.0 / 10.0 : 1.0 / 10.0, d1c = 6.0 / 10.0, d2c = right ? 1.0 / 10.0 : 3.0 / 10.0;
+    const double e0 = 1.0e-8 + b0, e1 = 1.0e-8 + b1, e2 = 1.0e-8 + b2;
+    const double d0 = d0c / (e0 * e0), d1 = d1c / (e1 * e1), d2 = d2c / (e2 * e2);
+    const double w0 = d0 / (d0 + d1 + d2), w1 = d1 / (d0 + d1 + d2), w2 = d2 / (d0 + d1 + d2);
+    if (right) return (w0 * weno_p_0(u[2], u[3], u[4]) + w1 * weno_p_m1(u[1], u[2], u[3])) + w2 * weno_p_m2(u[0], u[1], u[2]);
+    return (w0 * weno_m_0(u[2], u[3], u[4]) + w1 * weno_m_m1(u[1], u[2], u[3])) + w2 * weno_m_m2(u[0], u[1], u[2]);
 }
 
 // flux_2D (transfer.cpp:110-126): the volume through a face with the face
@@ -144,6 +174,45 @@ void tr_grad(const orc_tr& T, int c, bool limit, double& gx_out, double& gy_out)
     gy_out = (B[1][0] * Lx + B[1][1] * Ly) + B[1][2] * Lz;
 }
 
+// update_WENO's flux sum of one cell (transfer.cpp:541-658) on the
+// Rectangle's grid, cell = gny i + j, indices wrapped periodically
+// (EuMesh::operator()(i, j)): for each swept side L, R, B, T (no boundary
+// test: a boundary face takes the wrapped stencil too) the WENO5 face value
+// of the upwind cell -- the cell itself when vn > 0, else the neighbour
+// across the side -- on its five cells along the side's axis, m() on its
+// lower face and p() on its upper one, bounded to [0, 1]; the flux
+// a_sig vn dt S, with WENO_CRP corrected by flux_CRP.  Returns u2.
+double tr_cell_weno(const orc_tr& T, int c, int dir) {
+    const orc_mesh& m = *T.m;
+    const int nx = m.gnx, ny = m.gny;
+    const int i = c / ny, j = c % ny;
+    const double a1 = T.u1[c];
+    const double V1 = m.vol[c];
+    double fluxes = 0.0;
+    for (int s = 0; s < 4; ++s) {
+        if ((s < 2 && dir == 2) || (s >= 2 && dir == 1)) continue;
+        const size_t f = 8 * (size_t)c + s;
+        const int nb = m.adj[f];
+        const double vn = (T.fvx[f] * m.fnx[f] + T.fvy[f] * m.fny[f]) + 0.0;
+        const bool upper = (s & 1) != 0;
+        const int k = s < 2 ? i : j;
+        const int K = vn > 0.0 ? k : (upper ? k + 1 : k - 1);
+        double st[5];
+        for (int q = 0; q < 5; ++q) {
+            const int kk = K - 2 + q;
+            const int ii = s < 2 ? (kk + nx) % nx : i, jj = s < 2 ? j : (kk + ny) % ny;
+            st[q] = T.u1[(size_t)ny * ii + jj];
+        }
+        const bool right = upper ? vn > 0.0 : !(vn > 0.0);
+        const double a_sig = f_between(weno5(st, right), 0.0, 1.0);
+        const double S = m.area[f];
+        double Flux = a_sig * vn * T.dt * S;
+        if (T.method == kTrWENOCRP) Flux = tr_flux_CRP(a1, T.u1[nb], S, V1, m.vol[nb], vn, T.dt, Flux);
+        fluxes += Flux;
+    }
+    return T.u1[c] - fluxes / V1;
+}
+
 // The flux sum of one cell for the built methods (fluxes_CRP,
 // transfer.cpp:208-251; fluxes_MUSCL, :331-395), u2 = u1 - sum / V.
 double tr_cell(const orc_tr& T, int c, int dir) {
@@ -253,6 +322,8 @@ extern "C" {
 
 orc_tr* orc_tr_create(orc_mesh* m, int method, double cfl) {
     if (!tr_built(method)) return nullptr;
+    if ((method == kTrWENO || method == kTrWENOCRP) && (m->gnx < 3 || m->gny < 3 || m->gnx * m->gny != m->n))
+        return nullptr;   // the five-cell stencils need a Rectangle of at least 3 x 3
     orc_tr* T = new orc_tr();
     T->m = m;
     T->method = method;
@@ -311,9 +382,10 @@ double orc_tr_compute_dt(const orc_tr* T) {
 // MUSCL methods, the fluxes, u1 = between(u2, 0, 1), the normals.
 void orc_tr_update(orc_tr* T, int dir) {
     const int n = T->m->n;
-    if (T->method >= kTrMUSCLd)
+    const bool weno = T->method == kTrWENO || T->method == kTrWENOCRP;
+    if (T->method >= kTrMUSCLd && !weno)
         for (int c = 0; c < n; ++c) tr_grad(*T, c, T->method == kTrMC || T->method == kTrMCCRP, T->dudx[c], T->dudy[c]);
-    for (int c = 0; c < n; ++c) T->u2[c] = tr_cell(*T, c, dir);
+    for (int c = 0; c < n; ++c) T->u2[c] = weno ? tr_cell_weno(*T, c, dir) : tr_cell(*T, c, dir);
     for (int c = 0; c < n; ++c) {
         T->u1[c] = f_between(T->u2[c], 0.0, 1.0);
         T->u2[c] = 0.0;
@@ -344,3 +416,8 @@ void orc_tr_set_flags(orc_tr* T) {
 }
 
 }  // extern "C"
+
+extern "C" {
+// WENO5::p() (right != 0) or m() of five cell averages (the WENO test hook).
+double orc_weno5(const double* u, int right) { return weno5(u, right != 0); }
+}

@@ -83,6 +83,7 @@ def lib():
             "orc_mm_set_crp": (None, [_P, _I]),
             "orc_mm_set_flux": (None, [_P, _I]),
             "orc_tr_create": (_P, [_P, _I, _D]),
+            "orc_weno5": (_D, [_dp, _I]),
             "orc_tr_free": (None, [_P]),
             "orc_tr_set_velocity": (None, [_P, _dp, _dp, _dp, _dp]),
             "orc_tr_set_state": (None, [_P, _dp]),
@@ -649,6 +650,13 @@ def average_flux(alpha, cos, cfl):
 # ---- Transfer (math/solver/transfer.h) -------------------------------------
 TRANSFER_METHODS = ["CRP_V3", "CRP_V5", "CRP_SE", "CRP_N1", "CRP_N2", "VOF", "VOF_CRP", "MUSCLd", "MUSCLn",
                     "MUSCLd_CRP", "MUSCLn_CRP", "MUSCL_MC", "MUSCL_MC_CRP", "WENO", "WENO_CRP"]
+
+
+def weno5(u, right=True):
+    """WENO5::p() (right) or m() of five cell averages u1..u5."""
+    a = np.ascontiguousarray(u, dtype=np.float64)
+    assert a.size == 5
+    return lib().orc_weno5(_d(a), 1 if right else 0)
 
 
 class Transfer:

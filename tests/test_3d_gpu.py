@@ -170,3 +170,18 @@ def test_errors(zfv):
     ctx.configure(flux="hllc")
     with pytest.raises(zfv.ZfvError):
         ctx.step(1)   # no state
+
+
+def test_kernel_timing(zfv):
+    ctx = zfv.Context3()
+    ctx.mesh_cuboid(16, 16, 16)
+    ctx.configure(acc=2, flux="hllc", limiter="MC", gamma=GAMMA)
+    n = ctx.n
+    ctx.set_state(np.ones(n), np.zeros(n), np.zeros(n), np.zeros(n), np.ones(n))
+    ctx.profile_enable(True)
+    ctx.step(3)
+    prof = ctx.profile_read()
+    # the CFL reduction runs alone once after a new state, then inside the corrector
+    assert {k: v[1] for k, v in prof.items()} == {"dt": 1, "grad_pred": 3, "corr": 3}
+    assert all(v[0] > 0.0 for v in prof.values())
+    ctx.profile_enable(False)

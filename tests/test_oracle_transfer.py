@@ -15,16 +15,22 @@ mathematics fixes:
   flow tangent to the walls for every method while u stays inside (0, 1) (each inner face's
   flux is the same from both sides);
 * direction splitting: an x sweep moves nothing with a velocity along y;
-* set_flags marks exactly the cells whose closed neighbourhood varies.
-A swapped upwind side, a wrong bound or a wrong extrapolation offset fails
-one of these.
+* set_flags marks exactly the cells whose closed neighbourhood varies;
+* WENO5 (weno.h) reproduces the face values of a quadratic from its cell
+  averages (every sub-stencil is exact on quadratics), keeps a constant,
+  takes the smooth side of a jump (non-oscillatory), and its m() is the
+  mirror of p(); WENO and WENO_CRP are in the lists above (WENO_CRP is
+  CRP-bounded, so Courant 1 moves the profile exactly one cell).
+A swapped upwind side, a wrong bound, a wrong extrapolation offset or a
+swapped interpolation weight fails one of these.
 """
 import numpy as np
 import pytest
 
 ZOE, PERIODIC = 2, 3
-BUILT = ["CRP_V3", "CRP_V5", "CRP_SE", "CRP_N1", "MUSCLd", "MUSCLd_CRP", "MUSCL_MC", "MUSCL_MC_CRP"]
-BOUNDED = ["CRP_V3", "CRP_V5", "CRP_SE", "CRP_N1", "MUSCLd_CRP", "MUSCL_MC_CRP"]
+BUILT = ["CRP_V3", "CRP_V5", "CRP_SE", "CRP_N1", "MUSCLd", "MUSCLd_CRP", "MUSCL_MC", "MUSCL_MC_CRP", "WENO",
+         "WENO_CRP"]
+BOUNDED = ["CRP_V3", "CRP_V5", "CRP_SE", "CRP_N1", "MUSCLd_CRP", "MUSCL_MC_CRP", "WENO_CRP"]
 
 
 def velocity_field(m, f):
@@ -46,9 +52,11 @@ def uniform(vx, vy):
 
 def test_unbuilt_methods_are_refused(oracle):
     m = oracle.Mesh.rectangle(4, 4)
-    for meth in ("CRP_N2", "VOF", "VOF_CRP", "MUSCLn", "MUSCLn_CRP", "WENO", "WENO_CRP"):
+    for meth in ("CRP_N2", "VOF", "VOF_CRP", "MUSCLn", "MUSCLn_CRP"):
         with pytest.raises(ValueError):
             oracle.Transfer(m, meth)
+    with pytest.raises(ValueError):   # five-cell stencils on a 2-cell-wide grid
+        oracle.Transfer(oracle.Mesh.rectangle(2, 4), "WENO")
 
 
 def test_compute_dt_closed_form(oracle):
@@ -156,3 +164,28 @@ def test_set_flags(oracle):
     for i, j in ((3, 2), (2, 2), (4, 2), (3, 1), (3, 3)):
         want[i, j] = 1
     assert np.array_equal(flag, want)
+
+
+def test_weno5_exact_on_quadratics(oracle):
+    rng = np.random.default_rng(7)
+    k = np.arange(-2, 3, dtype=float)
+    for _ in range(50):
+        a, b, c = rng.normal(size=3)
+        avg = a + b * k + c * (k * k + 1.0 / 12.0)   # averages over [k - 1/2, k + 1/2]
+        q = lambda x: a + b * x + c * x * x
+        scale = abs(a) + abs(b) + abs(c)
+        assert abs(oracle.weno5(avg, True) - q(0.5)) < 1e-13 * scale
+        assert abs(oracle.weno5(avg, False) - q(-0.5)) < 1e-13 * scale
+
+
+def test_weno5_constant_jump_and_mirror(oracle):
+    for v in (0.0, 1.0, 0.3, 0.7):
+        assert abs(oracle.weno5([v] * 5, True) - v) <= 4e-16   # a few ulps: the weights sum to 1 in rounding
+    step = np.array([0.0, 0.0, 1.0, 1.0, 1.0])   # jump on the left face of the centre cell
+    assert abs(oracle.weno5(step, True) - 1.0) < 1e-6
+    assert abs(oracle.weno5(step, False) - 1.0) < 1e-6
+    assert abs(oracle.weno5(step[::-1], True) - 1.0) < 1e-6
+    rng = np.random.default_rng(3)
+    for _ in range(50):
+        u = rng.random(5)
+        assert oracle.weno5(u[::-1], False) == pytest.approx(oracle.weno5(u, True), rel=1e-13, abs=1e-15)

@@ -13,7 +13,8 @@ import pytest
 pytestmark = pytest.mark.gpu
 
 ZOE, PERIODIC = 2, 3
-BUILT = ["CRP_V3", "CRP_V5", "CRP_SE", "CRP_N1", "MUSCLd", "MUSCLd_CRP", "MUSCL_MC", "MUSCL_MC_CRP"]
+BUILT = ["CRP_V3", "CRP_V5", "CRP_SE", "CRP_N1", "MUSCLd", "MUSCLd_CRP", "MUSCL_MC", "MUSCL_MC_CRP", "WENO",
+         "WENO_CRP"]
 
 
 def rotation(x, y):
@@ -65,7 +66,7 @@ def test_updates_match_oracle(oracle, zfv, method, split):
     assert 0.0 < np.sum(u) and np.any((u > 0.0) & (u < 1.0))
 
 
-@pytest.mark.parametrize("method", ["CRP_N1", "MUSCL_MC_CRP"])
+@pytest.mark.parametrize("method", ["CRP_N1", "MUSCL_MC_CRP", "WENO", "WENO_CRP"])
 def test_open_box_and_several_updates_per_call(oracle, zfv, method):
     m, T, ctx = pair(oracle, zfv, method, nx=40, ny=32, bounds=(ZOE,) * 4,
                      field=lambda x, y: (np.full_like(x, 0.7), np.full_like(y, -0.3)))
@@ -84,9 +85,16 @@ def test_open_box_and_several_updates_per_call(oracle, zfv, method):
 def test_refusals(zfv):
     ctx = zfv.Context()
     ctx.mesh_rectangle(8, 8)
-    for meth in ("CRP_N2", "VOF", "VOF_CRP", "MUSCLn", "MUSCLn_CRP", "WENO", "WENO_CRP"):
+    for meth in ("CRP_N2", "VOF", "VOF_CRP", "MUSCLn", "MUSCLn_CRP"):
         with pytest.raises(zfv.ZfvError, match="error 4"):
             ctx.transfer_configure(meth)
+    narrow = zfv.Context()   # WENO's five-cell stencils need 3 x 3 cells
+    narrow.mesh_rectangle(2, 8)
+    narrow.transfer_configure("WENO")
+    narrow.transfer_set_velocity(np.ones(128), np.zeros(128), np.ones(16), np.zeros(16))
+    narrow.transfer_set_state(np.full(16, 0.5))
+    with pytest.raises(zfv.ZfvError, match="error 4"):
+        narrow.transfer_update("ANY")
     ctx.transfer_configure("CRP_V3")
     with pytest.raises(zfv.ZfvError):
         ctx.transfer_update("ANY")   # no state

@@ -48,7 +48,8 @@ EXPORTS = [
     "zfv3_create", "zfv3_destroy", "zfv3_mesh_cuboid", "zfv3_mesh_n_cells", "zfv3_mesh_download",
     "zfv3_smf_configure", "zfv3_smf_set_max_dt", "zfv3_state_upload", "zfv3_state_download",
     "zfv3_wait_upload", "zfv3_wait_download", "zfv3_grad_download", "zfv3_smf_step", "zfv3_smf_dt",
-    "zfv3_smf_compute_dt", "zfv3_conserved_totals", "zfv3_synchronize",
+    "zfv3_smf_compute_dt", "zfv3_conserved_totals", "zfv3_synchronize", "zfv3_profile_enable",
+    "zfv3_profile_read",
 ]
 VTU_DENSITY, VTU_VELOCITY, VTU_PRESSURE, VTU_ENERGY, VTU_WAIT, VTU_LEVEL, VTU_FLAG, VTU_INDEX = (
     1, 2, 4, 8, 16, 32, 64, 128)
@@ -165,6 +166,7 @@ def lib():
         L.zfv3_state_upload.argtypes = [vp] * 7
         L.zfv3_state_download.argtypes = [vp] * 7
         L.zfv3_smf_step.argtypes = [vp, C.c_int]
+        L.zfv3_profile_enable.argtypes = [vp, C.c_int]
         _lib = L
     return _lib
 
@@ -771,3 +773,16 @@ class Context3:
 
     def synchronize(self):
         _check(lib().zfv3_synchronize(self.h))
+
+    def profile_enable(self, on=True):
+        _check(lib().zfv3_profile_enable(self.h, 1 if on else 0))
+
+    def profile_read(self):
+        """{kernel: (total ms, launches)} since enabled or last read."""
+        cap = 8
+        names = (C.c_char_p * cap)()
+        ms = (C.c_double * cap)()
+        cnt = (C.c_int32 * cap)()
+        count = C.c_int32(cap)
+        _check(lib().zfv3_profile_read(self.h, names, ms, cnt, C.byref(count)))
+        return {names[i].decode(): (ms[i], cnt[i]) for i in range(count.value)}

@@ -64,18 +64,37 @@ struct zfv3_ctx {
     double* d_z[3] = {nullptr, nullptr, nullptr};   // three states of [6][n]
     double* d_G = nullptr;        // [15][n]
     int* d_wait[2] = {nullptr, nullptr};
-    unsigned long long* d_slots = nullptr;   // two CFL slots
+    unsigned long long* d_slots = nullptr;   // three rotating CFL slots
     double* d_dt = nullptr;
     int* d_err = nullptr;
     double* d_part = nullptr;     // totals partial sums [5][blocks]
     int blocks = 0;
     // solver
     bool configured = false, have_state = false, dt_valid = false;
+    bool dt_ready = false;        // slot `cur` holds the CFL minimum of the current state
     zfv_smf_params prm{};
     int zi = 0, hi = 1, ni = 2;   // init / half / next buffers
     int wi = 0, cur = 0;
+    // kernel timing: an event pair around every launch while enabled
+    struct Timed {
+        const char* name;
+        hipEvent_t a, b;
+    };
+    bool prof_on = false;
+    std::vector<Timed> timed;
 
-    ~zfv3_ctx() { release(); if (stream) (void)hipStreamDestroy(stream); }
+    ~zfv3_ctx() {
+        drop_events();
+        release();
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+    void drop_events() {
+        for (auto& t : timed) {
+            (void)hipEventDestroy(t.a);
+            (void)hipEventDestroy(t.b);
+        }
+        timed.clear();
+    }
     void release() {
         for (void* p : allocs) (void)hipFree(p);
         allocs.clear();
@@ -102,6 +121,7 @@ struct zfv3_ctx {
         return Dev3State{b, b + N, b + 2 * N, b + 3 * N, b + 4 * N, b + 5 * N};
     }
     Eos eos() const { return Eos{prm.gamma, prm.P0, prm.e0}; }
+    unsigned long long* slot(int k) const { return d_slots + (size_t)k * kSlotWords; }
 };
 
 namespace {
@@ -229,7 +249,7 @@ int upload_mesh(zfv3_ctx* ctx) {
         (r = ctx->alloc(&ctx->d_z[0], 6 * N)) || (r = ctx->alloc(&ctx->d_z[1], 6 * N)) ||
         (r = ctx->alloc(&ctx->d_z[2], 6 * N)) || (r = ctx->alloc(&ctx->d_G, 15 * N)) ||
         (r = ctx->alloc(&ctx->d_wait[0], N)) || (r = ctx->alloc(&ctx->d_wait[1], N)) ||
-        (r = ctx->alloc(&ctx->d_slots, 2)) || (r = ctx->alloc(&ctx->d_dt, 1)) || (r = ctx->alloc(&ctx->d_err, 1)))
+        (r = ctx->alloc(&ctx->d_slots, 3 * (size_t)kSlotWords)) || (r = ctx->alloc(&ctx->d_dt, 1)) || (r = ctx->alloc(&ctx->d_err, 1)))
         return r;
     ctx->blocks = (int)((N + 255) / 256);
     if ((r = ctx->alloc(&ctx->d_part, 5 * (size_t)ctx->blocks))) return r;
@@ -250,14 +270,24 @@ int upload_mesh(zfv3_ctx* ctx) {
     Z3HIP(hipMemcpy(ctx->d_slot, slot.data(), slot.size() * sizeof(double), hipMemcpyHostToDevice));
     Z3HIP(hipMemcpy(ctx->d_type, type.data(), S * sizeof(int), hipMemcpyHostToDevice));
     Z3HIP(hipMemcpy(ctx->d_adj, adj.data(), S * sizeof(int), hipMemcpyHostToDevice));
-    const unsigned long long init[2] = {kDblMaxBits, kDblMaxBits};
-    Z3HIP(hipMemcpy(ctx->d_slots, init, sizeof(init), hipMemcpyHostToDevice));
+    const std::vector<unsigned long long> init(3 * (size_t)kSlotWords, kDblMaxBits);
+    Z3HIP(hipMemcpy(ctx->d_slots, init.data(), init.size() * sizeof(unsigned long long), hipMemcpyHostToDevice));
     Z3HIP(hipMemset(ctx->d_err, 0, sizeof(int)));
     Z3HIP(hipMemset(ctx->d_wait[0], 0, N * sizeof(int)));
     Z3HIP(hipMemset(ctx->d_wait[1], 0, N * sizeof(int)));
     Z3HIP(hipMemset(ctx->d_G, 0, 15 * N * sizeof(double)));
     ctx->zi = 0; ctx->hi = 1; ctx->ni = 2; ctx->wi = 0; ctx->cur = 0;
     ctx->have_state = false;
+    ctx->dt_valid = false;
+    ctx->dt_ready = false;
+    return ZFV_OK;
+}
+
+// A new state: the current slot is cleared for a fresh CFL reduction.
+int state_changed(zfv3_ctx* ctx) {
+    const std::vector<unsigned long long> v(kSlotWords, kDblMaxBits);
+    Z3HIP(hipMemcpy(ctx->slot(ctx->cur), v.data(), v.size() * sizeof(unsigned long long), hipMemcpyHostToDevice));
+    ctx->dt_ready = false;
     ctx->dt_valid = false;
     return ZFV_OK;
 }
@@ -393,8 +423,7 @@ int zfv3_state_upload(zfv3_ctx* ctx, const double* rho, const double* u, const d
     for (int k = 0; k < 6; ++k)
         Z3HIP(hipMemcpy(ctx->d_z[ctx->zi] + k * N, src[k], N * sizeof(double), hipMemcpyHostToDevice));
     ctx->have_state = true;
-    ctx->dt_valid = false;
-    return ZFV_OK;
+    return state_changed(ctx);
 }
 
 int zfv3_state_download(zfv3_ctx* ctx, double* rho, double* u, double* v, double* w, double* p, double* e) {
@@ -435,6 +464,35 @@ int zfv3_grad_download(zfv3_ctx* ctx, double* g) {
     return read_err(ctx);
 }
 
+}  // extern "C"
+
+namespace {
+
+// A launch, between two recorded events while kernel timing is on.
+template <class F>
+int timed_launch(zfv3_ctx* ctx, const char* name, F&& launch) {
+    if (!ctx->prof_on) {
+        Z3HIP(launch());
+        return ZFV_OK;
+    }
+    zfv3_ctx::Timed t{name, nullptr, nullptr};
+    Z3HIP(hipEventCreate(&t.a));
+    hipError_t e = hipEventCreate(&t.b);
+    if (e != hipSuccess) {
+        (void)hipEventDestroy(t.a);
+        return fail(ZFV_ERR_HIP, std::string("hipEventCreate: ") + hipGetErrorString(e));
+    }
+    ctx->timed.push_back(t);
+    Z3HIP(hipEventRecord(t.a, ctx->stream));
+    Z3HIP(launch());
+    Z3HIP(hipEventRecord(t.b, ctx->stream));
+    return ZFV_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
 int zfv3_smf_step(zfv3_ctx* ctx, int n_steps) {
     int r = need_ready(ctx, true);
     if (r) return r;
@@ -442,24 +500,32 @@ int zfv3_smf_step(zfv3_ctx* ctx, int n_steps) {
     const Dev3Mesh M = ctx->mesh();
     const Eos eos = ctx->eos();
     const int flux = ctx->prm.flux, lim = ctx->prm.limiter;
+    hipStream_t s = ctx->stream;
     for (int it = 0; it < n_steps; ++it) {
-        const Dev3Ctl ctl{ctx->d_slots + ctx->cur, ctx->d_slots + (ctx->cur ^ 1), ctx->prm.cfl, ctx->prm.max_dt,
-                          ctx->d_dt, ctx->d_err};
-        Z3HIP(launch3_dt(M, ctx->state(ctx->zi), eos, ctl.dtmin, ctx->stream));
+        const Dev3Ctl ctl{ctx->slot(ctx->cur), ctx->slot((ctx->cur + 1) % 3), ctx->slot((ctx->cur + 2) % 3),
+                          ctx->prm.cfl, ctx->prm.max_dt, ctx->d_dt, ctx->d_err};
+        const Dev3State z = ctx->state(ctx->zi), h = ctx->state(ctx->hi), nx = ctx->state(ctx->ni);
+        int* w0 = ctx->d_wait[ctx->wi];
+        int* w1 = ctx->d_wait[ctx->wi ^ 1];
+        if (!ctx->dt_ready && (r = timed_launch(ctx, "dt", [&] { return launch3_dt(M, z, eos, ctl.dtmin, s); })))
+            return r;
         if (ctx->prm.acc == 1) {
-            Z3HIP(launch3_flux1(flux, M, ctx->state(ctx->zi), ctx->state(ctx->ni), eos, ctl, ctx->stream));
+            if ((r = timed_launch(ctx, "flux1", [&] { return launch3_flux1(flux, M, z, nx, eos, ctl, s); }))) return r;
         } else {
-            Z3HIP(launch3_grad_pred(lim, M, ctx->state(ctx->zi), ctx->d_G, ctx->state(ctx->hi), ctx->d_wait[ctx->wi],
-                                    eos, ctl, ctx->stream));
-            Z3HIP(launch3_corr(flux, M, ctx->state(ctx->zi), ctx->state(ctx->hi), ctx->d_G, ctx->d_wait[ctx->wi],
-                               ctx->d_wait[ctx->wi ^ 1], ctx->state(ctx->ni), eos, ctl, ctx->stream));
+            if ((r = timed_launch(ctx, "grad_pred",
+                                  [&] { return launch3_grad_pred(lim, M, z, ctx->d_G, h, w0, eos, ctl, s); })))
+                return r;
+            if ((r = timed_launch(ctx, "corr",
+                                  [&] { return launch3_corr(flux, M, z, h, ctx->d_G, w0, w1, nx, eos, ctl, s); })))
+                return r;
             ctx->wi ^= 1;
         }
         const int t = ctx->zi;
         ctx->zi = ctx->ni;
         ctx->ni = t;
-        ctx->cur ^= 1;
+        ctx->cur = (ctx->cur + 1) % 3;
         ctx->dt_valid = true;
+        ctx->dt_ready = true;
     }
     return ZFV_OK;
 }
@@ -477,15 +543,21 @@ int zfv3_smf_compute_dt(zfv3_ctx* ctx, double* dt) {
     int r = need_ready(ctx, true);
     if (r) return r;
     if (!dt) return fail(ZFV_ERR_ARG, "null output");
-    unsigned long long* slot = ctx->d_slots + ctx->cur;
-    Z3HIP(launch3_dt(ctx->mesh(), ctx->state(ctx->zi), ctx->eos(), slot, ctx->stream));
-    unsigned long long bits = 0;
-    Z3HIP(hipMemcpyAsync(&bits, slot, sizeof(bits), hipMemcpyDeviceToHost, ctx->stream));
+    unsigned long long* slot = ctx->slot(ctx->cur);
+    if (!ctx->dt_ready) {
+        Z3HIP(launch3_dt(ctx->mesh(), ctx->state(ctx->zi), ctx->eos(), slot, ctx->stream));
+        ctx->dt_ready = true;
+    }
+    std::vector<unsigned long long> bits(kSlotWords);
+    Z3HIP(hipMemcpyAsync(bits.data(), slot, bits.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                         ctx->stream));
     Z3HIP(hipStreamSynchronize(ctx->stream));
-    const unsigned long long reset = kDblMaxBits;
-    Z3HIP(hipMemcpy(slot, &reset, sizeof(reset), hipMemcpyHostToDevice));
-    double dmin;
-    std::memcpy(&dmin, &bits, sizeof(dmin));
+    double dmin = 0.0;
+    for (int k = 0; k < kSubSlots; ++k) {
+        double v;
+        std::memcpy(&v, &bits[(size_t)kSubStride * k], sizeof(v));
+        if (k == 0 || v < dmin) dmin = v;
+    }
     const double t = ctx->prm.cfl * dmin;
     *dt = (ctx->prm.max_dt < t) ? ctx->prm.max_dt : t;
     return read_err(ctx);
@@ -504,6 +576,47 @@ int zfv3_conserved_totals(zfv3_ctx* ctx, double* out5) {
     for (int k = 0; k < 5; ++k) out5[k] = 0.0;
     for (int b = 0; b < ctx->blocks; ++b)
         for (int k = 0; k < 5; ++k) out5[k] += part[5 * (size_t)b + k];
+    return ZFV_OK;
+}
+
+int zfv3_profile_enable(zfv3_ctx* ctx, int on) {
+    if (!ctx) return fail(ZFV_ERR_ARG, "null context");
+    (void)hipSetDevice(ctx->device);
+    Z3HIP(hipStreamSynchronize(ctx->stream));
+    ctx->drop_events();
+    ctx->prof_on = on != 0;
+    return ZFV_OK;
+}
+
+int zfv3_profile_read(zfv3_ctx* ctx, const char** names, double* ms, int32_t* launches, int32_t* count) {
+    if (!ctx || !count) return fail(ZFV_ERR_ARG, "null argument");
+    (void)hipSetDevice(ctx->device);
+    Z3HIP(hipStreamSynchronize(ctx->stream));
+    std::vector<const char*> nm;
+    std::vector<double> tot;
+    std::vector<int> cnt;
+    for (auto& t : ctx->timed) {
+        float el = 0.0f;
+        Z3HIP(hipEventElapsedTime(&el, t.a, t.b));
+        size_t k = 0;
+        while (k < nm.size() && std::strcmp(nm[k], t.name) != 0) ++k;
+        if (k == nm.size()) {
+            nm.push_back(t.name);
+            tot.push_back(0.0);
+            cnt.push_back(0);
+        }
+        tot[k] += el;
+        cnt[k] += 1;
+    }
+    ctx->drop_events();
+    const int cap = *count;
+    const int used = (int)nm.size() < cap ? (int)nm.size() : cap;
+    for (int k = 0; k < used; ++k) {
+        if (names) names[k] = nm[k];
+        if (ms) ms[k] = tot[k];
+        if (launches) launches[k] = cnt[k];
+    }
+    *count = used;
     return ZFV_OK;
 }
 

@@ -21,6 +21,7 @@
 // B_aa = cof_aa / ((a11 a22) a00) with cof_00 = a11 a22, cof_11 = a22 a00,
 // cof_22 = a00 a11, which is what is computed.
 #include "zfv_3d.hpp"
+#include "zfv_step_common.hpp"
 
 namespace zfv {
 
@@ -217,46 +218,80 @@ __device__ __forceinline__ Z3 extrap3(const Z3& z, const double (*g)[5], double 
     return r;
 }
 
-__device__ __forceinline__ double dt3(const Dev3Ctl& ctl) {
-    const double dmin = __longlong_as_double((long long)*ctl.dtmin);
-    return smin(ctl.cfl * dmin, ctl.max_dt);
+// The CFL minimum of slot ctl.dtmin (the least of its partial minima), then
+// dt = min(CFL dmin, max_dt) (sm_fluid.cpp:104-114).  Every thread of the
+// workgroup calls it.
+__device__ __forceinline__ double block_dt(const Dev3Ctl& ctl) {
+    __shared__ double s_min;
+    if (threadIdx.x < kSubSlots) {
+        double v = __longlong_as_double((long long)ctl.dtmin[kSubStride * threadIdx.x]);
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) {
+            const double o = __shfl_xor(v, off, 64);
+            v = (o < v) ? o : v;
+        }
+        if (threadIdx.x == 0) s_min = v;
+    }
+    __syncthreads();
+    return smin(ctl.cfl * s_min, ctl.max_dt);
 }
 
-// The least candidate of a wavefront into *slot; infinities and NaNs are
-// left out, as the reference's `if (t < res)` over res = DBL_MAX leaves them.
-__device__ __forceinline__ void min_atomic(unsigned long long* slot, double v) {
+// Workgroup 0 publishes dt and clears the slot the previous step read.
+__device__ __forceinline__ void step_epilogue0(const Dev3Ctl& ctl, double dt) {
+    if (blockIdx.x != 0) return;
+    if (threadIdx.x == 0) *ctl.dt_out = dt;
+    if (threadIdx.x < kSubSlots) ctl.reset[kSubStride * threadIdx.x] = kDblMaxBits;
+}
+
+// The least candidate of a workgroup of 256 into its partial minimum of the
+// slot, one atomic per workgroup; infinities and NaNs are left out, as the reference's
+// `if (t < res)` over res = DBL_MAX leaves them.  Every thread of the
+// workgroup calls it.
+__device__ __forceinline__ void block_min(unsigned long long* slot, double v) {
     if (!(v < INFINITY)) v = INFINITY;
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) {
         const double o = __shfl_xor(v, off, 64);
         v = (o < v) ? o : v;
     }
-    if ((threadIdx.x & 63) == 0 && v < INFINITY) atomicMin(slot, (unsigned long long)__double_as_longlong(v));
+    __shared__ double red[4];
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double m = red[0];
+        for (int i = 1; i < 4; ++i) m = (red[i] < m) ? red[i] : m;
+        if (m < INFINITY)
+            atomicMin(slot + kSubStride * (blockIdx.x & (kSubSlots - 1)), (unsigned long long)__double_as_longlong(m));
+    }
+}
+
+// The cell of this thread: workgroups dealt out so that each XCD (blockIdx
+// mod 8) takes one contiguous eighth of the cells, whose y and z neighbours
+// (and most x neighbours) then sit in the same L2.
+__device__ __forceinline__ int cell_of_thread() {
+    return xcd_block((int)blockIdx.x, (int)gridDim.x) * (int)blockDim.x + (int)threadIdx.x;
+}
+
+// h / (|v| + c) (sm_fluid.cpp:105-110).
+__device__ __forceinline__ double cell_dt3(const Z3& q, double h, const Eos& eos) {
+    return h / (sqrt(vsq3(q)) + eos_c(eos, q.r, q.p));
 }
 
 }  // namespace
 
 // SmFluid::compute_dt (sm_fluid.cpp:104-114): h / (|v| + c), min over cells.
 __global__ void __launch_bounds__(256) k3_dt(Dev3Mesh M, Dev3State z, Eos eos, unsigned long long* slot) {
-    const int c = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    const int c = cell_of_thread();
     double cand = INFINITY;
-    if (c < M.n) {
-        const Z3 q = load3(z, c);
-        cand = M.h[c] / (sqrt(vsq3(q)) + eos_c(eos, q.r, q.p));
-    }
-    min_atomic(slot, cand);
+    if (c < M.n) cand = cell_dt3(load3(z, c), M.h[c], eos);
+    block_min(slot, cand);
 }
 
-// SmFluid::fluxes, first order (sm_fluid.cpp:127-174).
+// SmFluid::fluxes, first order (sm_fluid.cpp:127-174); returns the next
+// step's CFL candidate of the new state.
 template <int FLUX>
-__global__ void __launch_bounds__(256) k3_flux1(Dev3Mesh M, Dev3State in, Dev3State out, Eos eos, Dev3Ctl ctl) {
-    const int c = (int)(blockIdx.x * blockDim.x + threadIdx.x);
-    if (c >= M.n) return;
-    const double dt = dt3(ctl);
-    if (c == 0) {
-        *ctl.dt_out = dt;
-        *ctl.reset = kDblMaxBits;
-    }
+__device__ __forceinline__ double flux1_cell(const Dev3Mesh& M, const Dev3State& in, const Dev3State& out,
+                                             const Eos& eos, const Dev3Ctl& ctl, double dt, int c) {
     const Z3 zc = load3(in, c);
     Q3 F{0.0, 0.0, 0.0, 0.0, 0.0};
     bool bad = false;
@@ -285,8 +320,19 @@ __global__ void __launch_bounds__(256) k3_flux1(Dev3Mesh M, Dev3State in, Dev3St
     q.my = q.my - coef * F.my;
     q.mz = q.mz - coef * F.mz;
     q.E = q.E - coef * F.E;
-    store3(out, c, to_z3(q, eos));
+    const Z3 zn = to_z3(q, eos);
+    store3(out, c, zn);
     if (bad) *ctl.err = 1;
+    return cell_dt3(zn, M.h[c], eos);
+}
+
+template <int FLUX>
+__global__ void __launch_bounds__(256) k3_flux1(Dev3Mesh M, Dev3State in, Dev3State out, Eos eos, Dev3Ctl ctl) {
+    const int c = cell_of_thread();
+    const double dt = block_dt(ctl);
+    step_epilogue0(ctl, dt);
+    const double cand = c < M.n ? flux1_cell<FLUX>(M, in, out, eos, ctl, dt, c) : INFINITY;
+    block_min(ctl.next, cand);
 }
 
 // SmFluid::compute_grad (gradient::LSM, gradient.h:226-282, then
@@ -296,7 +342,8 @@ __global__ void __launch_bounds__(256) k3_flux1(Dev3Mesh M, Dev3State in, Dev3St
 template <int LIM>
 __global__ void __launch_bounds__(256) k3_grad_pred(Dev3Mesh M, Dev3State in, double* __restrict__ G, Dev3State half,
                                                     const int* __restrict__ wait, Eos eos, Dev3Ctl ctl) {
-    const int c = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    const int c = cell_of_thread();
+    const double dt = block_dt(ctl);
     if (c >= M.n) return;
     const Z3 zc = load3(in, c);
     const double cc[3] = {M.cx[c], M.cy[c], M.cz[c]};
@@ -369,7 +416,6 @@ __global__ void __launch_bounds__(256) k3_grad_pred(Dev3Mesh M, Dev3State in, do
     // predictor
     Z3 zh = zc;
     if (wait[c] <= 0) {
-        const double dt = dt3(ctl);
         Q3 Fl{0.0, 0.0, 0.0, 0.0, 0.0};
         for (int s = 0; s < 6; ++s) {
             const int so = s * M.stride + c;
@@ -399,18 +445,13 @@ __global__ void __launch_bounds__(256) k3_grad_pred(Dev3Mesh M, Dev3State in, do
 }
 
 // SmFluid::fluxes_stage2, corrector (sm_fluid.cpp:255-366): wait read from
-// the pre-pass array, written to the other.
+// the pre-pass array, written to the other; returns the next step's CFL
+// candidate of the new state.
 template <int FLUX>
-__global__ void __launch_bounds__(256) k3_corr(Dev3Mesh M, Dev3State init, Dev3State half, const double* __restrict__ G,
-                                               const int* __restrict__ wait_in, int* __restrict__ wait_out,
-                                               Dev3State next, Eos eos, Dev3Ctl ctl) {
-    const int c = (int)(blockIdx.x * blockDim.x + threadIdx.x);
-    if (c >= M.n) return;
-    const double dt = dt3(ctl);
-    if (c == 0) {
-        *ctl.dt_out = dt;
-        *ctl.reset = kDblMaxBits;
-    }
+__device__ __forceinline__ double corr_cell(const Dev3Mesh& M, const Dev3State& init, const Dev3State& half,
+                                            const double* __restrict__ G, const int* __restrict__ wait_in,
+                                            int* __restrict__ wait_out, const Dev3State& next, const Eos& eos,
+                                            const Dev3Ctl& ctl, double dt, int c) {
     const Z3 zc = load3(init, c);
     const Z3 zch = load3(half, c);
     const double cc[3] = {M.cx[c], M.cy[c], M.cz[c]};
@@ -487,11 +528,24 @@ __global__ void __launch_bounds__(256) k3_corr(Dev3Mesh M, Dev3State init, Dev3S
     store3(next, c, zn);
     wait_out[c] = wn;
     if (bad) *ctl.err = 1;
+    return cell_dt3(zn, M.h[c], eos);
+}
+
+template <int FLUX>
+__global__ void __launch_bounds__(256) k3_corr(Dev3Mesh M, Dev3State init, Dev3State half, const double* __restrict__ G,
+                                               const int* __restrict__ wait_in, int* __restrict__ wait_out,
+                                               Dev3State next, Eos eos, Dev3Ctl ctl) {
+    const int c = cell_of_thread();
+    const double dt = block_dt(ctl);
+    step_epilogue0(ctl, dt);
+    const double cand =
+        c < M.n ? corr_cell<FLUX>(M, init, half, G, wait_in, wait_out, next, eos, ctl, dt, c) : INFINITY;
+    block_min(ctl.next, cand);
 }
 
 // Per-workgroup partial sums of V q: mass, momentum x, y, z, energy.
 __global__ void __launch_bounds__(256) k3_totals(Dev3Mesh M, Dev3State z, double* partial) {
-    const int c = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    const int c = cell_of_thread();
     double v[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
     if (c < M.n) {
         const Q3 q = to_q3(load3(z, c));

@@ -23,11 +23,14 @@ struct Dev3State {
     double *r, *u, *v, *w, *p, *e;
 };
 
-// Step control: dtmin is the CFL slot this step reads (the bits of a
-// positive double, reduced with atomicMin); the stage that ends a step sets
-// `reset`, the next step's slot, back to the largest finite double.
+// Step control, three rotating CFL slots of kSlotWords words (the bits of
+// positive doubles, reduced with atomicMin): dtmin is the slot this step reads; the stage
+// that ends a step reduces the new state's h / (|v| + c) into `next`, the
+// slot the following step reads, and sets `reset`, the slot the previous
+// step read, back to the largest finite double.
 struct Dev3Ctl {
     unsigned long long* dtmin;
+    unsigned long long* next;
     unsigned long long* reset;
     double cfl, max_dt;
     double* dt_out;
@@ -35,6 +38,12 @@ struct Dev3Ctl {
 };
 
 constexpr unsigned long long kDblMaxBits = 0x7fefffffffffffffULL;
+// A CFL slot is kSubSlots partial minima, kSubStride words (128 bytes) apart,
+// so that the workgroups' atomics spread over as many cache lines; the
+// slot's value is their least.
+constexpr int kSubSlots = 64;
+constexpr int kSubStride = 16;
+constexpr int kSlotWords = kSubSlots * kSubStride;
 
 hipError_t launch3_dt(const Dev3Mesh& M, const Dev3State& z, const Eos& eos, unsigned long long* slot,
                       hipStream_t s);

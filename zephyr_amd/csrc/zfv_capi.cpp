@@ -2075,6 +2075,7 @@ static int refine_impl(zfv_ctx* ctx, int max_level, int mode) {
     ctx->mesh_epoch += 1;
     ctx->co.perm_epoch = carried ? ctx->mesh_epoch : -1;
     ctx->tiles.ok = false;   // no longer a plain Rectangle
+    ctx->rects.clear();
     ctx->co.refined = ctx->co.keys_ok;
     ctx->co.lmax = std::max(ctx->co.lmax, max_level);
     if (mm) {
@@ -2637,7 +2638,7 @@ int zfv_transfer_configure(zfv_ctx* ctx, int method, double cfl) {
     if (!ctx->have_mesh) return fail(ZFV_ERR_STATE, "no mesh uploaded");
     if (method < 0 || method > 14) return fail(ZFV_ERR_ARG, "transfer: unknown method");
     if (!tr_method_built(method))
-        return fail(ZFV_ERR_UNSUPPORTED, "transfer: CRP_N2, VOF, VOF_CRP, MUSCLn, MUSCLn_CRP, WENO, WENO_CRP not built");
+        return fail(ZFV_ERR_UNSUPPORTED, "transfer: CRP_N2, VOF, VOF_CRP, MUSCLn, MUSCLn_CRP not built");
     auto& T = ctx->tr;
     T.method = method;
     T.cfl = std::max(0.0, std::min(cfl, 1.0));   // set_CFL
@@ -2763,15 +2764,25 @@ int zfv_transfer_update(zfv_ctx* ctx, int dir, int n_steps) {
     if (r) return r;
     if (dir < 0 || dir > 2) return fail(ZFV_ERR_ARG, "transfer: dir 0 (ANY), 1 (X) or 2 (Y)");
     if (n_steps < 0) return fail(ZFV_ERR_ARG, "negative step count");
+    auto& T = ctx->tr;
+    // WENO's stencils index the Rectangle's (i, j) grid (EuMesh::operator()(i, j))
+    int gnx = 0, gny = 0;
+    if (tr_method_structured(T.method)) {
+        if (ctx->rects.size() != 1 || ctx->rects[0].off != 0 ||
+            (long long)ctx->rects[0].nx * ctx->rects[0].ny != ctx->hm.n || ctx->rects[0].nx < 3 ||
+            ctx->rects[0].ny < 3)
+            return fail(ZFV_ERR_UNSUPPORTED, "transfer: WENO needs an unadapted Rectangle of at least 3 x 3 cells");
+        gnx = ctx->rects[0].nx;
+        gny = ctx->rects[0].ny;
+    }
     r = to_mesh_order(ctx);
     if (r) return r;
-    auto& T = ctx->tr;
     const DevMesh M = ctx->dmesh();
     hipStream_t s = ctx->stream;
     for (int k = 0; k < n_steps; ++k) {
         ZFV_HIP(timed(ctx, "tr_step", [&] {
             return tr_step(M, T.method, T.u[T.cur].p, T.u[1 - T.cur].p, T.nx.p, T.ny.p, T.gx.p, T.gy.p, T.fvx.p,
-                           T.fvy.p, T.dt, dir, s);
+                           T.fvy.p, T.dt, dir, gnx, gny, s);
         }));
         T.cur = 1 - T.cur;
         ZFV_HIP(timed(ctx, "tr_normals", [&] { return tr_normals(M, T.u[T.cur].p, T.nx.p, T.ny.p, s); }));

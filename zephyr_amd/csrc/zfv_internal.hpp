@@ -318,7 +318,10 @@ hipError_t mm_cache(const DevMesh& M, const MmLayer& L, const mm::Mix& mx, unsig
 // CFL candidates' minimum into *slot, set_flags into flag.
 bool tr_method_built(int method);
 hipError_t tr_step(const DevMesh& M, int method, const double* u, double* out, const double* tnx, const double* tny,
-                   double* gx, double* gy, const double* fvx, const double* fvy, double dt, int dir, hipStream_t s);
+                   double* gx, double* gy, const double* fvx, const double* fvy, double dt, int dir, int gnx,
+                   int gny, hipStream_t s);
+// WENO / WENO_CRP: five-cell stencils on a Rectangle's (i, j) grid.
+bool tr_method_structured(int method);
 hipError_t tr_normals(const DevMesh& M, const double* u, double* nx, double* ny, hipStream_t s);
 hipError_t tr_dt(const DevMesh& M, const double* cvx, const double* cvy, unsigned long long* slot, hipStream_t s);
 hipError_t tr_flags(const DevMesh& M, const double* u, int* flag, hipStream_t s);

@@ -10,7 +10,7 @@
 // interface normals (InterfaceRecovery::compute_normal), the CFL reduction
 // and set_flags.  Built methods, Transfer::Method numbering: CRP_V3 0,
 // CRP_V5 1, CRP_SE 2, CRP_N1 3, MUSCLd 7, MUSCLd_CRP 9, MUSCL_MC 11,
-// MUSCL_MC_CRP 12.
+// MUSCL_MC_CRP 12, WENO 13 and WENO_CRP 14 (on a Rectangle's grid).
 #include "zfv_internal.hpp"
 #include "zfv_step_common.hpp"
 #include "zfv_step_faces.hpp"
@@ -21,7 +21,8 @@ using namespace mm;
 
 namespace {
 
-constexpr int kTrV3 = 0, kTrV5 = 1, kTrSE = 2, kTrN1 = 3, kTrMUSCLd = 7, kTrMUSCLdCRP = 9, kTrMC = 11, kTrMCCRP = 12;
+constexpr int kTrV3 = 0, kTrV5 = 1, kTrSE = 2, kTrN1 = 3, kTrMUSCLd = 7, kTrMUSCLdCRP = 9, kTrMC = 11, kTrMCCRP = 12,
+              kTrWENO = 13, kTrWENOCRP = 14;
 constexpr int kLimMC = 3, kLimVanAlbada2 = 6;   // limiter ids (the MC of m_limiter, ATvL's vanAlbada2)
 
 __device__ __forceinline__ bool tr_swept(const DevMesh& M, int so, int dir) {   // EuCell::faces(dir)
@@ -224,6 +225,92 @@ __global__ void __launch_bounds__(256) k_tr_flux(DevMesh M, const double* __rest
     out[c] = f_between(a1 - fluxes / V1, 0.0, 1.0);
 }
 
+// WENO<b, e> (math/calc/weno.h:39-120): the quadratic interpolant's face
+// values of three cell averages, p = u(+1/2) and m = u(-1/2), as
+// w1 a + w2 b + w3 c left to right, and the smoothness indicator
+// 13/12 d2^2 + 1/4 d1^2.  S is the sub-stencil's first offset (-2, -1, 0).
+template <int S, bool RIGHT>
+__device__ __forceinline__ double weno3(double a, double b, double c) {
+    if (RIGHT) {
+        if (S == -2) return ((1.0 / 3.0) * a + (-7.0 / 6.0) * b) + (11.0 / 6.0) * c;
+        if (S == -1) return ((-1.0 / 6.0) * a + (5.0 / 6.0) * b) + (1.0 / 3.0) * c;
+        return ((1.0 / 3.0) * a + (5.0 / 6.0) * b) + (-1.0 / 6.0) * c;
+    }
+    if (S == -2) return ((-1.0 / 6.0) * a + (5.0 / 6.0) * b) + (1.0 / 3.0) * c;
+    if (S == -1) return ((1.0 / 3.0) * a + (5.0 / 6.0) * b) + (-1.0 / 6.0) * c;
+    return ((11.0 / 6.0) * a + (-7.0 / 6.0) * b) + (1.0 / 3.0) * c;
+}
+template <int S>
+__device__ __forceinline__ double weno_beta(double a, double b, double c) {
+    const double d1 = S == -2 ? (a - 4.0 * b) + 3.0 * c : (S == -1 ? a - c : (3.0 * a - 4.0 * b) + c);
+    const double d2 = (a - 2.0 * b) + c;
+    return (13.0 / 12.0) * d2 * d2 + (1.0 / 4.0) * d1 * d1;
+}
+
+// WENO5::p() / m() (weno.h:126-170) on u[0..4]: sub-stencils {u0 u1 u2}
+// (offset -2), {u1 u2 u3} (-1), {u2 u3 u4} (0), linear weights 1/10, 6/10,
+// 3/10 in that order for p (3/10, 6/10, 1/10 for m) over (1e-8 + beta)^2,
+// normalised.
+template <bool RIGHT>
+__device__ __forceinline__ double weno5(const double* u) {
+    const double b0 = weno_beta<0>(u[2], u[3], u[4]);
+    const double b1 = weno_beta<-1>(u[1], u[2], u[3]);
+    const double b2 = weno_beta<-2>(u[0], u[1], u[2]);
+    const double d0c = RIGHT ? 3.0 / 10.0 : 1.0 / 10.0, d1c = 6.0 / 10.0, d2c = RIGHT ? 1.0 / 10.0 : 3.0 / 10.0;
+    const double e0 = 1.0e-8 + b0, e1 = 1.0e-8 + b1, e2 = 1.0e-8 + b2;
+    const double d0 = d0c / (e0 * e0), d1 = d1c / (e1 * e1), d2 = d2c / (e2 * e2);
+    const double w0 = d0 / (d0 + d1 + d2), w1 = d1 / (d0 + d1 + d2), w2 = d2 / (d0 + d1 + d2);
+    return (w0 * weno3<0, RIGHT>(u[2], u[3], u[4]) + w1 * weno3<-1, RIGHT>(u[1], u[2], u[3])) +
+           w2 * weno3<-2, RIGHT>(u[0], u[1], u[2]);
+}
+
+// update_WENO (transfer.cpp:541-658) on a Rectangle's grid (cell = ny i + j,
+// indices wrapped periodically as EuMesh::operator()(i, j) does): for each
+// swept side L, R, B, T -- boundary faces included, with the wrapped
+// stencil -- the WENO5 face value of the upwind cell (the cell itself when
+// vn > 0, else the neighbour across the side) on its five cells along the
+// side's axis, m() on its lower face and p() on its upper one, bounded to
+// [0, 1]; flux a_sig vn dt S, corrected by flux_CRP for WENO_CRP; then
+// out = between(u - sum F / V, 0, 1).
+template <bool CRP>
+__global__ void __launch_bounds__(256) k_tr_weno(DevMesh M, const double* __restrict__ u, const double* __restrict__ fvx,
+                                                 const double* __restrict__ fvy, double dt, int dir, int gnx, int gny,
+                                                 double* __restrict__ out) {
+    const int c = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (c >= M.n) return;
+    const int st = M.stride;
+    const int i = c / gny, j = c - (c / gny) * gny;
+    const double a1 = u[c];
+    const double V1 = M.vol[c];
+    double fluxes = 0.0;
+#pragma unroll 1
+    for (int s = 0; s < 4; ++s) {
+        if ((s < 2 && dir == 2) || (s >= 2 && dir == 1)) continue;
+        const int so = s * st + c;
+        const double vn = (fvx[so] * M.nx[so] + fvy[so] * M.ny[so]) + 0.0;
+        const bool upper = (s & 1) != 0;
+        const int k = s < 2 ? i : j;
+        const int K = vn > 0.0 ? k : (upper ? k + 1 : k - 1);
+        double sten[5];
+#pragma unroll
+        for (int q = 0; q < 5; ++q) {
+            const int kk = K - 2 + q;
+            const int ii = s < 2 ? (kk + gnx) % gnx : i, jj = s < 2 ? j : (kk + gny) % gny;
+            sten[q] = u[gny * ii + jj];
+        }
+        const bool right = upper ? vn > 0.0 : !(vn > 0.0);
+        const double a_sig = f_between(right ? weno5<true>(sten) : weno5<false>(sten), 0.0, 1.0);
+        const double S = M.area[so];
+        double Flux = a_sig * vn * dt * S;
+        if (CRP) {
+            const int nb = M.adj[so];
+            Flux = tr_flux_CRP(a1, u[nb], S, V1, M.vol[nb], vn, dt, Flux);
+        }
+        fluxes += Flux;
+    }
+    out[c] = f_between(a1 - fluxes / V1, 0.0, 1.0);
+}
+
 // InterfaceRecovery::compute_normal (interface_recovery.cpp:17-39): zero in
 // a pure cell, else -sum face_fraction(uc, un) S n_f over the inner faces,
 // normalised.
@@ -299,12 +386,22 @@ inline dim3 tr_grid(int n) { return dim3((unsigned)((n + 255) / 256)); }
 
 bool tr_method_built(int m) {
     return m == kTrV3 || m == kTrV5 || m == kTrSE || m == kTrN1 || m == kTrMUSCLd || m == kTrMUSCLdCRP ||
-           m == kTrMC || m == kTrMCCRP;
+           m == kTrMC || m == kTrMCCRP || m == kTrWENO || m == kTrWENOCRP;
 }
 
+bool tr_method_structured(int m) { return m == kTrWENO || m == kTrWENOCRP; }
+
 hipError_t tr_step(const DevMesh& M, int method, const double* u, double* out, const double* tnx, const double* tny,
-                   double* gx, double* gy, const double* fvx, const double* fvy, double dt, int dir, hipStream_t s) {
+                   double* gx, double* gy, const double* fvx, const double* fvy, double dt, int dir, int gnx,
+                   int gny, hipStream_t s) {
     const dim3 g = tr_grid(M.n), b(256);
+    if (method == kTrWENO || method == kTrWENOCRP) {
+        if (method == kTrWENO)
+            hipLaunchKernelGGL(k_tr_weno<false>, g, b, 0, s, M, u, fvx, fvy, dt, dir, gnx, gny, out);
+        else
+            hipLaunchKernelGGL(k_tr_weno<true>, g, b, 0, s, M, u, fvx, fvy, dt, dir, gnx, gny, out);
+        return hipGetLastError();
+    }
     if (method >= kTrMUSCLd) {
         if (method == kTrMC || method == kTrMCCRP)
             hipLaunchKernelGGL((k_tr_slopes<kLimMC, true>), g, b, 0, s, M, u, gx, gy);
