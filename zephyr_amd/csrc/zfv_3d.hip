@@ -461,16 +461,28 @@ __device__ __forceinline__ double corr_cell(const Dev3Mesh& M, const Dev3State& 
     for (int a = 0; a < 3; ++a)
 #pragma unroll
         for (int k = 0; k < 5; ++k) g[a][k] = G[(size_t)(a * 5 + k) * M.stride + c];
+    // the six slots' codes, neighbours and neighbours' counters first: one
+    // round trip for all of them instead of two per face
+    int ts[6], nbs[6];
+#pragma unroll
+    for (int s = 0; s < 6; ++s) {
+        ts[s] = M.type[s * M.stride + c];
+        nbs[s] = M.adj[s * M.stride + c];
+    }
+    bool rob[6];
+#pragma unroll
+    for (int s = 0; s < 6; ++s) rob[s] = wc > 0 || wait_in[nbs[s]] > 0;
     Q3 F{0.0, 0.0, 0.0, 0.0, 0.0};
     bool bad = false;
+#pragma unroll
     for (int s = 0; s < 6; ++s) {
         const int so = s * M.stride + c;
-        const int t = M.type[so];
-        const int nb = M.adj[so];
+        const int t = ts[s];
+        const int nb = nbs[s];
         const bool bound = t >= SC_WALL;
         const double fc[3] = {M.fcx[so], M.fcy[so], M.fcz[so]};
         const Z3 znh = bound ? (t == SC_WALL ? wall3(s, zch) : zch) : load3(half, nb);
-        const bool robust = wc > 0 || wait_in[nb] > 0;
+        const bool robust = rob[s];
         Z3 zm, zp;
         if (robust) {
             zm = zch;
