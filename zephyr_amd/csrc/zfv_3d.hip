@@ -73,7 +73,12 @@ __device__ __forceinline__ void qset(Q3& q, int k, double v) {
     else q.E = v;
 }
 
-// HLLC::calc_flux, smf (math/cfd/flux/hllc.cpp:23-83).
+// HLLC::calc_flux, smf (math/cfd/flux/hllc.cpp:23-83); MODE 1 is HLLC_M
+// (hllc.cpp:491-572: the tangential velocities averaged across the contact,
+// V_C and W_C, with the matching energy correction), MODE 2 HLLC_LM
+// (hllc.cpp:427-489: the outer waves scaled by phi = sin(pi/2 Ma / 0.1)
+// below Ma = 0.1; sin is the device's, within an ulp of the host's).
+template <int MODE>
 __device__ Q3 hllc3(const Z3& zL, const Z3& zR, const Eos& eos, bool& bad) {
     const double c_L = eos_c(eos, zL.r, zL.p), c_R = eos_c(eos, zR.r, zR.p);
     const double S_L = smin(zL.u - c_L, zR.u - c_R);
@@ -85,22 +90,41 @@ __device__ Q3 hllc3(const Z3& zL, const Z3& zR, const Eos& eos, bool& bad) {
     const double S_C = (zL.p - zR.p + a_R * zR.u - a_L * zL.u) / (a_R - a_L);
     const double rho_sL = a_L / (S_L - S_C);
     const double rho_sR = a_R / (S_R - S_C);
-    const double E_sL = (zL.e + 0.5 * vsq3(zL)) + (S_C - zL.u) * (S_C + zL.p / a_L);
-    const double E_sR = (zR.e + 0.5 * vsq3(zR)) + (S_C - zR.u) * (S_C + zR.p / a_R);
-    const Q3 Q_sL{rho_sL, rho_sL * S_C, rho_sL * zL.v, rho_sL * zL.w, rho_sL * E_sL};
-    const Q3 Q_sR{rho_sR, rho_sR * S_C, rho_sR * zR.v, rho_sR * zR.w, rho_sR * E_sR};
+    double E_sL = (zL.e + 0.5 * vsq3(zL)) + (S_C - zL.u) * (S_C + zL.p / a_L);
+    double E_sR = (zR.e + 0.5 * vsq3(zR)) + (S_C - zR.u) * (S_C + zR.p / a_R);
+    double vL = zL.v, wL = zL.w, vR = zR.v, wR = zR.w;
+    if (MODE == 1) {
+        const double V_C = (a_R * zR.v - a_L * zL.v) / (a_R - a_L);
+        const double W_C = (a_R * zR.w - a_L * zL.w) / (a_R - a_L);
+        const double v_L2 = zL.v * zL.v + zL.w * zL.w;
+        const double v_R2 = zR.v * zR.v + zR.w * zR.w;
+        E_sL += 0.5 * ((a_R * v_R2 - a_L * v_L2) / (a_R - a_L) - v_L2);
+        E_sR += 0.5 * ((a_R * v_R2 - a_L * v_L2) / (a_R - a_L) - v_R2);
+        vL = vR = V_C;
+        wL = wR = W_C;
+    }
+    const Q3 Q_sL{rho_sL, rho_sL * S_C, rho_sL * vL, rho_sL * wL, rho_sL * E_sL};
+    const Q3 Q_sR{rho_sR, rho_sR * S_C, rho_sR * vR, rho_sR * wR, rho_sR * E_sR};
     const Q3 Q_L = to_q3(zL), Q_R = to_q3(zR);
     const Q3 F_L = phys_flux3(zL), F_R = phys_flux3(zR);
+    double phi = 1.0;
+    if (MODE == 2) {
+        const double Ma_local = smax(fabs(zL.u / c_L), fabs(zR.u / c_R));
+        const double xi = Ma_local / 0.1;
+        phi = xi > 1.0 ? 1.0 : sin(1.5707963267948966 * xi);
+    }
     const double aSC = fabs(S_C);
     Q3 F;
 #pragma unroll
     for (int k = 0; k < 5; ++k) {
         double t = qk(F_L, k) + qk(F_R, k);
-        t = t + S_L * (qk(Q_sL, k) - qk(Q_L, k));
+        if (MODE == 2) t = t + phi * S_L * (qk(Q_sL, k) - qk(Q_L, k));
+        else t = t + S_L * (qk(Q_sL, k) - qk(Q_L, k));
         t = t + aSC * (qk(Q_sL, k) - qk(Q_sR, k));
-        t = t + S_R * (qk(Q_sR, k) - qk(Q_R, k));
+        if (MODE == 2) t = t + phi * S_R * (qk(Q_sR, k) - qk(Q_R, k));
+        else t = t + S_R * (qk(Q_sR, k) - qk(Q_R, k));
         const double f = 0.5 * t;
-        if (isnan(f)) bad = true;   // hllc.cpp:72-80 throws
+        if (isnan(f)) bad = true;   // hllc.cpp:72-80, 474-484, 558-568 throw
         qset(F, k, f);
     }
     return F;
@@ -148,7 +172,9 @@ template <int FLUX>
 __device__ __forceinline__ Q3 num_flux3(const Z3& zL, const Z3& zR, const Eos& eos, bool& bad) {
     if (FLUX == 1) return rusanov3(zL, zR, eos);
     if (FLUX == 2) return hll3(zL, zR, eos, bad);
-    return hllc3(zL, zR, eos, bad);
+    if (FLUX == 4) return hllc3<1>(zL, zR, eos, bad);
+    if (FLUX == 5) return hllc3<2>(zL, zR, eos, bad);
+    return hllc3<0>(zL, zR, eos, bad);
 }
 
 // Velocity to the frame of side s (L, R, B, T, Z, F) and back.
@@ -592,6 +618,8 @@ hipError_t launch3_flux1(int flux, const Dev3Mesh& M, const Dev3State& in, const
     switch (flux) {
         case 1: hipLaunchKernelGGL(k3_flux1<1>, g256(M.n), dim3(256), 0, s, M, in, out, eos, ctl); break;
         case 2: hipLaunchKernelGGL(k3_flux1<2>, g256(M.n), dim3(256), 0, s, M, in, out, eos, ctl); break;
+        case 4: hipLaunchKernelGGL(k3_flux1<4>, g256(M.n), dim3(256), 0, s, M, in, out, eos, ctl); break;
+        case 5: hipLaunchKernelGGL(k3_flux1<5>, g256(M.n), dim3(256), 0, s, M, in, out, eos, ctl); break;
         default: hipLaunchKernelGGL(k3_flux1<3>, g256(M.n), dim3(256), 0, s, M, in, out, eos, ctl); break;
     }
     return hipGetLastError();
@@ -627,6 +655,8 @@ hipError_t launch3_corr(int flux, const Dev3Mesh& M, const Dev3State& init, cons
     switch (flux) {
         case 1: ZFV3_C(1); break;
         case 2: ZFV3_C(2); break;
+        case 4: ZFV3_C(4); break;
+        case 5: ZFV3_C(5); break;
         default: ZFV3_C(3); break;
     }
 #undef ZFV3_C
