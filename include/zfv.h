@@ -514,8 +514,8 @@ int zfv3_mesh_n_cells(zfv3_ctx* ctx, int32_t* n);
 int zfv3_mesh_download(zfv3_ctx* ctx, double* cx, double* cy, double* cz, double* vol, double* h,
                        int32_t* boundary, int32_t* adj, double* nx, double* ny, double* nz, double* fcx,
                        double* fcy, double* fcz, double* area, double* w);
-/* SmFluid setters as zfv_smf_configure; flux Rusanov, HLL, HLLC, HLLC_M or
- * HLLC_LM (ZFV_ERR_UNSUPPORTED otherwise), axial 0. */
+/* SmFluid setters as zfv_smf_configure; flux GODUNOV..CIR2
+ * (ZFV_ERR_UNSUPPORTED for CRP), axial 0. */
 int zfv3_smf_configure(zfv3_ctx* ctx, const zfv_smf_params* params);
 int zfv3_smf_set_max_dt(zfv3_ctx* ctx, double max_dt);
 /* Primitive state PState{rho, {u, v, w}, p, e} of the "init" layer, n

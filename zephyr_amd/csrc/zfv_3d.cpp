@@ -386,9 +386,8 @@ int zfv3_mesh_download(zfv3_ctx* ctx, double* cx, double* cy, double* cz, double
 
 int zfv3_smf_configure(zfv3_ctx* ctx, const zfv_smf_params* p) {
     if (!ctx || !p) return fail(ZFV_ERR_ARG, "null argument");
-    if (p->flux != ZFV_RUSANOV && p->flux != ZFV_HLL && p->flux != ZFV_HLLC && p->flux != ZFV_HLLC_M &&
-        p->flux != ZFV_HLLC_LM)
-        return fail(ZFV_ERR_UNSUPPORTED, "the 3D step takes the Rusanov, HLL, HLLC, HLLC_M and HLLC_LM fluxes");
+    if (p->flux < ZFV_GODUNOV || p->flux > ZFV_CIR2)
+        return fail(ZFV_ERR_UNSUPPORTED, "the 3D step takes the single-material fluxes GODUNOV..CIR2");
     if (p->limiter < ZFV_LIM_NONE || p->limiter > ZFV_LIM_SUPERBEE) return fail(ZFV_ERR_ARG, "unknown limiter");
     if (p->axial) return fail(ZFV_ERR_UNSUPPORTED, "set_axial applies to two-dimensional meshes");
     if (!(p->gamma > 1.0)) return fail(ZFV_ERR_ARG, "gamma must be > 1");

@@ -168,10 +168,38 @@ __device__ Q3 rusanov3(const Z3& zL, const Z3& zR, const Eos& eos) {
     return F;
 }
 
+// CIR1 / CIR2 (math/cfd/flux/cir.cpp:16-216) with the sides' w.
+template <bool EXT>
+__device__ __forceinline__ Q3 cir3(const Z3& zL, const Z3& zR, const Eos& eos) {
+    const double zl[6] = {zL.r, zL.u, zL.v, zL.w, zL.p, zL.e}, zr[6] = {zR.r, zR.u, zR.v, zR.w, zR.p, zR.e};
+    double F[5];
+    cir_flux5<EXT>(zl, zr, eos, F);
+    return Q3{F[0], F[1], F[2], F[3], F[4]};
+}
+
+// Godunov (math/cfd/flux/godunov.cpp): the exact Riemann solution at the
+// face (riemann.cpp:118-459), tangential velocities from the upwind side of
+// the contact.
+__device__ Q3 godunov3(const Z3& zL, const Z3& zR, const Eos& eos, bool& bad) {
+    const RiemannFace sol = riemann_face(zL.r, zL.u, zL.p, zR.r, zR.u, zR.p, eos.g, eos.P0);
+    bad |= sol.fail;
+    Z3 z;
+    z.r = sol.rho;
+    z.u = sol.Uf;
+    z.v = sol.U > 0.0 ? zL.v : zR.v;
+    z.w = sol.U > 0.0 ? zL.w : zR.w;
+    z.p = sol.Pf;
+    z.e = eos_e(eos, sol.rho, sol.Pf);
+    return phys_flux3(z);
+}
+
 template <int FLUX>
 __device__ __forceinline__ Q3 num_flux3(const Z3& zL, const Z3& zR, const Eos& eos, bool& bad) {
+    if (FLUX == 0) return godunov3(zL, zR, eos, bad);
     if (FLUX == 1) return rusanov3(zL, zR, eos);
     if (FLUX == 2) return hll3(zL, zR, eos, bad);
+    if (FLUX == 6) return cir3<false>(zL, zR, eos);
+    if (FLUX == 7) return cir3<true>(zL, zR, eos);
     if (FLUX == 4) return hllc3<1>(zL, zR, eos, bad);
     if (FLUX == 5) return hllc3<2>(zL, zR, eos, bad);
     return hllc3<0>(zL, zR, eos, bad);
@@ -616,7 +644,10 @@ hipError_t launch3_dt(const Dev3Mesh& M, const Dev3State& z, const Eos& eos, uns
 hipError_t launch3_flux1(int flux, const Dev3Mesh& M, const Dev3State& in, const Dev3State& out, const Eos& eos,
                          const Dev3Ctl& ctl, hipStream_t s) {
     switch (flux) {
+        case 0: hipLaunchKernelGGL(k3_flux1<0>, g256(M.n), dim3(256), 0, s, M, in, out, eos, ctl); break;
         case 1: hipLaunchKernelGGL(k3_flux1<1>, g256(M.n), dim3(256), 0, s, M, in, out, eos, ctl); break;
+        case 6: hipLaunchKernelGGL(k3_flux1<6>, g256(M.n), dim3(256), 0, s, M, in, out, eos, ctl); break;
+        case 7: hipLaunchKernelGGL(k3_flux1<7>, g256(M.n), dim3(256), 0, s, M, in, out, eos, ctl); break;
         case 2: hipLaunchKernelGGL(k3_flux1<2>, g256(M.n), dim3(256), 0, s, M, in, out, eos, ctl); break;
         case 4: hipLaunchKernelGGL(k3_flux1<4>, g256(M.n), dim3(256), 0, s, M, in, out, eos, ctl); break;
         case 5: hipLaunchKernelGGL(k3_flux1<5>, g256(M.n), dim3(256), 0, s, M, in, out, eos, ctl); break;
@@ -653,7 +684,10 @@ hipError_t launch3_corr(int flux, const Dev3Mesh& M, const Dev3State& init, cons
 #define ZFV3_C(F) \
     hipLaunchKernelGGL((k3_corr<F>), g256(M.n), dim3(256), 0, s, M, init, half, G, wait_in, wait_out, next, eos, ctl)
     switch (flux) {
+        case 0: ZFV3_C(0); break;
         case 1: ZFV3_C(1); break;
+        case 6: ZFV3_C(6); break;
+        case 7: ZFV3_C(7); break;
         case 2: ZFV3_C(2); break;
         case 4: ZFV3_C(4); break;
         case 5: ZFV3_C(5); break;

@@ -519,12 +519,14 @@ __device__ __forceinline__ Q flux_hllc_lm(const Z& L, const Z& R, const Eos& eos
 // 6 x 6 L).  The z velocity (zero in 2D) and the z momentum are carried as
 // in the reference's vectors so that every sum has the same terms in the
 // same order: |A|(i,j) = sum_k (R(i,k) |lambda_k|) L(k,j), then sum_j.
+// cir_flux5 takes the sides as {rho, u, v, w, p, e} and gives all five
+// components (the three-dimensional step passes its w; the planar one 0).
 template <bool EXT>
-__device__ __forceinline__ Q flux_cir(const Z& ZL, const Z& ZR, const Eos& eos) {
+__device__ __forceinline__ void cir_flux5(const double* zl, const double* zr, const Eos& eos, double* F) {
     constexpr int N = EXT ? 6 : 5;
-    const double w = 0.0;   // v_z of both sides and of the face state
-    const double rho = 0.5 * (ZL.r + ZR.r), u = 0.5 * (ZL.u + ZR.u), v = 0.5 * (ZL.v + ZR.v);
-    const double p = 0.5 * (ZL.p + ZR.p);
+    const double rho = 0.5 * (zl[0] + zr[0]), u = 0.5 * (zl[1] + zr[1]), v = 0.5 * (zl[2] + zr[2]);
+    const double w = 0.5 * (zl[3] + zr[3]);
+    const double p = 0.5 * (zl[4] + zr[4]);
     const double eps = eos_e(eos, rho, p);
     const double u2 = u * u, v2 = v * v, w2 = w * w;
     const double q2 = u2 + v2 + w2;
@@ -580,18 +582,19 @@ __device__ __forceinline__ Q flux_cir(const Z& ZL, const Z& ZR, const Eos& eos) 
             }
         }
     }
-    // conserved vectors (z momentum rho v_z = rho * 0) and physical fluxes
-    const double EL = ZL.r * (ZL.e + 0.5 * vsq(ZL)), ER = ZR.r * (ZR.e + 0.5 * vsq(ZR));
-    const double qL[5] = {ZL.r, ZL.r * ZL.u, ZL.r * ZL.v, ZL.r * w, EL};
-    const double qR[5] = {ZR.r, ZR.r * ZR.u, ZR.r * ZR.v, ZR.r * w, ER};
-    const double ruL = ZL.r * ZL.u, ruR = ZR.r * ZR.u;
-    const double fL[5] = {ruL, ruL * ZL.u + ZL.p, ruL * ZL.v, ruL * w, (EL + ZL.p) * ZL.u};
-    const double fR[5] = {ruR, ruR * ZR.u + ZR.p, ruR * ZR.v, ruR * w, (ER + ZR.p) * ZR.u};
+    // conserved vectors and physical fluxes
+    const double sqL = (zl[1] * zl[1] + zl[2] * zl[2]) + zl[3] * zl[3];
+    const double sqR = (zr[1] * zr[1] + zr[2] * zr[2]) + zr[3] * zr[3];
+    const double EL = zl[0] * (zl[5] + 0.5 * sqL), ER = zr[0] * (zr[5] + 0.5 * sqR);
+    const double qL[5] = {zl[0], zl[0] * zl[1], zl[0] * zl[2], zl[0] * zl[3], EL};
+    const double qR[5] = {zr[0], zr[0] * zr[1], zr[0] * zr[2], zr[0] * zr[3], ER};
+    const double ruL = zl[0] * zl[1], ruR = zr[0] * zr[1];
+    const double fL[5] = {ruL, ruL * zl[1] + zl[4], ruL * zl[2], ruL * zl[3], (EL + zl[4]) * zl[1]};
+    const double fR[5] = {ruR, ruR * zr[1] + zr[4], ruR * zr[2], ruR * zr[3], (ER + zr[4]) * zr[1]};
     double dq[N];
 #pragma unroll
     for (int k = 0; k < 5; ++k) dq[k] = qL[k] - qR[k];
-    if constexpr (EXT) dq[5] = ZL.p - ZR.p;
-    double F[5];
+    if constexpr (EXT) dq[5] = zl[4] - zr[4];
 #pragma unroll
     for (int i = 0; i < 5; ++i) {
         double t = 0.0;
@@ -604,6 +607,15 @@ __device__ __forceinline__ Q flux_cir(const Z& ZL, const Z& ZR, const Eos& eos) 
         }
         F[i] = 0.5 * (fL[i] + fR[i]) + 0.5 * t;
     }
+}
+
+// The planar CIR flux: v_z = 0 on both sides (every z term an exact zero,
+// (u u + v v) + 0 0 the reference's squaredNorm), the z momentum dropped.
+template <bool EXT>
+__device__ __forceinline__ Q flux_cir(const Z& ZL, const Z& ZR, const Eos& eos) {
+    const double zl[6] = {ZL.r, ZL.u, ZL.v, 0.0, ZL.p, ZL.e}, zr[6] = {ZR.r, ZR.u, ZR.v, 0.0, ZR.p, ZR.e};
+    double F[5];
+    cir_flux5<EXT>(zl, zr, eos, F);
     return Q{F[0], F[1], F[2], F[4]};
 }
 
