@@ -124,7 +124,9 @@ WORKLOADS = {
 # that file includes.
 STEP_HEADERS = ("zfv_internal.hpp", "zfv_physics.hpp", "zfv_mm.hpp", "zfv_step_common.hpp", "zfv_step_faces.hpp")
 KERNEL_FILES = {"tile": "zfv_tile.hip", "grad_pred": "zfv_kernels.hip", "corr": "zfv_kernels.hip",
-                "flux1": "zfv_kernels.hip"}
+                "flux1": "zfv_kernels.hip", "k3_grad_pred": "zfv_3d.hip", "k3_corr": "zfv_3d.hip",
+                "k3_flux1": "zfv_3d.hip"}
+KERNEL_HEADERS = {"k3_grad_pred": ("zfv_3d.hpp",), "k3_corr": ("zfv_3d.hpp",), "k3_flux1": ("zfv_3d.hpp",)}
 
 
 def kernel_sources_sha(kernel):
@@ -136,7 +138,7 @@ def kernel_sources_sha(kernel):
         return None
     d = os.path.join(HERE, "zephyr_amd", "csrc")
     h = hashlib.sha256()
-    for f in (KERNEL_FILES[kernel],) + STEP_HEADERS:
+    for f in (KERNEL_FILES[kernel],) + STEP_HEADERS + KERNEL_HEADERS.get(kernel, ()):
         h.update(f.encode())
         h.update(open(os.path.join(d, f), "rb").read())
     return h.hexdigest()[:16]
@@ -801,6 +803,21 @@ def cube_run(args, w, Z):
     ctx.step(args.steps)
     ctx.synchronize()
     el = time.perf_counter() - t0
+    if args.copy_calibration:
+        # 3 device-to-device copies of n doubles: 8n bytes read and 8n bytes
+        # written per copy dispatch (the scale of the HBM counters)
+        import ctypes as C
+        hip = C.CDLL("libamdhip64.so")
+        src, dst = C.c_void_p(), C.c_void_p()
+        nbytes = C.c_size_t(8 * n)
+        if hip.hipMalloc(C.byref(src), nbytes) != 0 or hip.hipMalloc(C.byref(dst), nbytes) != 0:
+            raise RuntimeError("hipMalloc failed")
+        for _ in range(3):
+            if hip.hipMemcpy(dst, src, nbytes, 3) != 0:   # hipMemcpyDeviceToDevice
+                raise RuntimeError("hipMemcpy failed")
+        hip.hipDeviceSynchronize()
+        hip.hipFree(src)
+        hip.hipFree(dst)
     # kernel timing pass: the workload continues with events around each launch
     ctx.profile_enable(True)
     ctx.step(min(args.prof_steps, 20))
@@ -811,6 +828,19 @@ def cube_run(args, w, Z):
     alg = CUBE_BYTES[dom] * n
     achieved = alg / (avg[dom] * 1e-3) / 1e9
     value = n * args.steps / el
+    # HBM bytes per launch from the counter runs of the same kernel sources
+    traffic, traffic_detail = None, {"measured_in": f"no counter run of k3_{dom} for this workload"}
+    tpath = os.path.join(HERE, "profiles", "traffic.json")
+    t = json.load(open(tpath)).get(args.workload, {}).get("k3_" + dom) if os.path.exists(tpath) else None
+    if t:
+        traffic_detail = dict(t)
+        src_sha = kernel_sources_sha("k3_" + dom)
+        if t.get("kernel_sources_sha") == src_sha:
+            traffic = t["bytes_per_cell"] * n
+            traffic_detail["measured_in"] = "separate counter runs of this command on the same kernel sources"
+        else:
+            traffic_detail["measured_in"] = (f"stale: counter runs of kernel sources {t.get('kernel_sources_sha')}, "
+                                             f"this run executes {src_sha}; traffic not reported")
     out = {
         "metric": "FP64 cell-updates/s (HLLC+MUSCL, 3D Cuboid)",
         "value": value, "unit": "cell-updates/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
@@ -821,8 +851,8 @@ def cube_run(args, w, Z):
                    "step_compulsory_GBs": sum(CUBE_BYTES[k] for k in ("grad_pred", "corr")) * n
                    / (el / args.steps) / 1e9},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "algorithmic_bytes_per_launch": alg,
-                     "avg_launch_ms": avg[dom]},
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "traffic_detail": traffic_detail,
+                     "algorithmic_bytes_per_launch": alg, "avg_launch_ms": avg[dom]},
     }
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cube_cpu_baseline(w, args.seed, args.cpu_budget)
