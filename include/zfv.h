@@ -503,6 +503,15 @@ int zfv3_destroy(zfv3_ctx* ctx);
 int zfv3_mesh_cuboid(zfv3_ctx* ctx, int nx, int ny, int nz, double xmin, double xmax, double ymin, double ymax,
                      double zmin, double zmax, const int32_t* bounds);
 int zfv3_mesh_n_cells(zfv3_ctx* ctx, int32_t* n);
+/* Geometry tables: a Cuboid's centres, face centres, weights, areas and
+ * volume depend on one index each, so the step kernels can read 1D tables
+ * (nx + ny + nz entries per quantity) instead of the per-slot arrays; the
+ * tables are used only when every cell's values equal them bit for bit.
+ * on = 0 keeps the per-slot arrays for meshes built afterwards (default 1);
+ * active: whether the current mesh uses the tables.  No reference
+ * counterpart. */
+int zfv3_set_geometry_tables(zfv3_ctx* ctx, int on);
+int zfv3_get_geometry_tables(zfv3_ctx* ctx, int* active);
 /* The mesh in the AMR-3D layout (mesh/euler/amr_cells.h:74-104,
  * amr_faces.h:83-101): per cell cx, cy, cz, volume, incircle diameter h
  * (amr_cells.cpp:107-121); 24 face slots per cell (slot s of cell c at

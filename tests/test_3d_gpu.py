@@ -202,3 +202,30 @@ def test_kernel_timing(zfv):
     assert {k: v[1] for k, v in prof.items()} == {"dt": 1, "grad_pred": 3, "corr": 3}
     assert all(v[0] > 0.0 for v in prof.values())
     ctx.profile_enable(False)
+
+
+@pytest.mark.parametrize("bounds", [(1, 2, 3, 3, 2, 1), (3,) * 6])
+@pytest.mark.parametrize("acc", [1, 2])
+def test_geometry_tables_and_slot_arrays_agree(oracle, zfv, bounds, acc):
+    """The step kernels read a Cuboid's 1D geometry tables (the default) or
+    the per-slot arrays; both give the oracle's states."""
+    m = oracle.Mesh3(7, 6, 9, bounds=bounds)
+    s = oracle.SmFluid3(m, flux=3, acc=acc, limiter="MC", cfl=0.5)
+    c = m.cells()
+    z = blast_state(c["cx"], c["cy"], c["cz"], 8)
+    s.set_state(*z)
+    ctxs = []
+    for on in (True, False):
+        ctx = zfv.Context3()
+        ctx.set_geometry_tables(on)
+        ctx.mesh_cuboid(7, 6, 9, bounds=bounds)
+        assert ctx.geometry_tables() == on
+        ctx.configure(acc=acc, flux="hllc", limiter="MC", cfl=0.5, gamma=GAMMA)
+        ctx.set_state(*z)
+        ctxs.append(ctx)
+    for step in range(3):
+        s.update()
+        for ctx in ctxs:
+            ctx.step(1)
+            assert ctx.dt() == s.dt()
+            assert_equal(ctx.state(), s.state(), STATE, f"tables {ctx.geometry_tables()} step {step}")

@@ -49,7 +49,7 @@ EXPORTS = [
     "zfv3_smf_configure", "zfv3_smf_set_max_dt", "zfv3_state_upload", "zfv3_state_download",
     "zfv3_wait_upload", "zfv3_wait_download", "zfv3_grad_download", "zfv3_smf_step", "zfv3_smf_dt",
     "zfv3_smf_compute_dt", "zfv3_conserved_totals", "zfv3_synchronize", "zfv3_profile_enable",
-    "zfv3_profile_read",
+    "zfv3_profile_read", "zfv3_set_geometry_tables", "zfv3_get_geometry_tables",
 ]
 VTU_DENSITY, VTU_VELOCITY, VTU_PRESSURE, VTU_ENERGY, VTU_WAIT, VTU_LEVEL, VTU_FLAG, VTU_INDEX = (
     1, 2, 4, 8, 16, 32, 64, 128)
@@ -167,6 +167,7 @@ def lib():
         L.zfv3_state_download.argtypes = [vp] * 7
         L.zfv3_smf_step.argtypes = [vp, C.c_int]
         L.zfv3_profile_enable.argtypes = [vp, C.c_int]
+        L.zfv3_set_geometry_tables.argtypes = [vp, C.c_int]
         _lib = L
     return _lib
 
@@ -776,6 +777,15 @@ class Context3:
 
     def profile_enable(self, on=True):
         _check(lib().zfv3_profile_enable(self.h, 1 if on else 0))
+
+    def set_geometry_tables(self, on=True):
+        """Use the Cuboid's 1D geometry tables for meshes built afterwards."""
+        _check(lib().zfv3_set_geometry_tables(self.h, 1 if on else 0))
+
+    def geometry_tables(self):
+        v = C.c_int()
+        _check(lib().zfv3_get_geometry_tables(self.h, C.byref(v)))
+        return bool(v.value)
 
     def profile_read(self):
         """{kernel: (total ms, launches)} since enabled or last read."""

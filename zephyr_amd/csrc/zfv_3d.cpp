@@ -68,6 +68,10 @@ struct zfv3_ctx {
     double* d_dt = nullptr;
     int* d_err = nullptr;
     double* d_part = nullptr;     // totals partial sums [5][blocks]
+    double* d_tab = nullptr;      // the Cuboid's geometry tables, when every cell matches them
+    int tab_bcode[6] = {0, 0, 0, 0, 0, 0};
+    int tab_per[3] = {0, 0, 0};
+    bool use_tables = true;
     int blocks = 0;
     // solver
     bool configured = false, have_state = false, dt_valid = false;
@@ -100,7 +104,7 @@ struct zfv3_ctx {
         allocs.clear();
         d_cell = nullptr; d_type = nullptr; d_adj = nullptr; d_slot = nullptr;
         d_z[0] = d_z[1] = d_z[2] = nullptr; d_G = nullptr; d_wait[0] = d_wait[1] = nullptr;
-        d_slots = nullptr; d_dt = nullptr; d_err = nullptr; d_part = nullptr;
+        d_slots = nullptr; d_dt = nullptr; d_err = nullptr; d_part = nullptr; d_tab = nullptr;
     }
     template <class T>
     int alloc(T** p, size_t count) {
@@ -112,8 +116,19 @@ struct zfv3_ctx {
     }
     Dev3Mesh mesh() const {
         const size_t N = (size_t)n, S = (size_t)kSides * n;
-        return Dev3Mesh{n, n, d_cell, d_cell + N, d_cell + 2 * N, d_cell + 3 * N, d_cell + 4 * N, d_type, d_adj,
-                        d_slot, d_slot + S, d_slot + 2 * S, d_slot + 3 * S, d_slot + 4 * S};
+        Dev3Mesh M{};
+        M.n = n;
+        M.stride = n;
+        if (d_tab) {
+            M.tnx = nx; M.tny = ny; M.tnz = nz;
+            M.tab = d_tab;
+            for (int k = 0; k < 6; ++k) M.bcode[k] = tab_bcode[k];
+            for (int a = 0; a < 3; ++a) M.per[a] = tab_per[a];
+        }
+        M.cx = d_cell; M.cy = d_cell + N; M.cz = d_cell + 2 * N; M.vol = d_cell + 3 * N; M.h = d_cell + 4 * N;
+        M.type = d_type; M.adj = d_adj;
+        M.fcx = d_slot; M.fcy = d_slot + S; M.fcz = d_slot + 2 * S; M.area = d_slot + 3 * S; M.w = d_slot + 4 * S;
+        return M;
     }
     Dev3State state(int k) const {
         double* b = d_z[k];
@@ -239,6 +254,53 @@ void build_cuboid(zfv3_ctx* m, int nx, int ny, int nz, double xmin, double xmax,
     }
 }
 
+// The Cuboid's geometry as 1D tables (the layout of Dev3Mesh::tab): each
+// value taken from the first cell along its axis, then every cell and slot
+// compared with them bit for bit; false (no tables) on any difference.
+bool build_tables(const zfv3_ctx* m, std::vector<double>& tab, int bcode[6], int per[3]) {
+    const int nx = m->nx, ny = m->ny, nz = m->nz, L = nx + ny + nz;
+    const int len[3] = {nx, ny, nz}, base[3] = {0, nx, nx + ny}, st[3] = {ny * nz, nz, 1};
+    tab.assign(tab3_size(nx, ny, nz), 0.0);
+    auto slot = [](int c, int s) { return (size_t)kSides * c + s; };
+    const std::vector<double>* fcs[3] = {&m->fcx, &m->fcy, &m->fcz};
+    const std::vector<double>* ccs[3] = {&m->cx, &m->cy, &m->cz};
+    for (int a = 0; a < 3; ++a)
+        for (int t = 0; t < len[a]; ++t) {
+            const int c = t * st[a];
+            tab[18 * (size_t)L + base[a] + t] = (*ccs[a])[c];
+            for (int s = 0; s < kSides; ++s) tab[(size_t)(3 * s + a) * L + base[a] + t] = (*fcs[a])[slot(c, s)];
+        }
+    for (int s = 0; s < kSides; ++s) {
+        const int b = s >> 1;
+        for (int t = 0; t < len[b]; ++t) tab[19 * (size_t)L + (size_t)s * L + base[b] + t] = m->w[slot(t * st[b], s)];
+        tab[25 * (size_t)L + s] = m->area[slot(0, s)];
+        const int edge_cell = (s & 1) ? (len[b] - 1) * st[b] : 0;
+        bcode[s] = slot_code(m->bnd[slot(edge_cell, s)]);
+    }
+    tab[25 * (size_t)L + 6] = m->vol[0];
+    for (int a = 0; a < 3; ++a) per[a] = m->bnd[slot((len[a] - 1) * st[a], 2 * a + 1)] == ZFV_PERIODIC;
+    for (int c = 0; c < m->n; ++c) {
+        const int idx[3] = {c / st[0], (c % st[0]) / nz, c % nz};
+        if (m->vol[c] != tab[25 * (size_t)L + 6]) return false;
+        for (int a = 0; a < 3; ++a)
+            if ((*ccs[a])[c] != tab[18 * (size_t)L + base[a] + idx[a]]) return false;
+        for (int s = 0; s < kSides; ++s) {
+            const int b = s >> 1, up = s & 1;
+            const bool edge = up ? idx[b] == len[b] - 1 : idx[b] == 0;
+            const size_t o = slot(c, s);
+            const int code = edge ? bcode[s] : SC_INNER;
+            int nb = up ? c + st[b] : c - st[b];
+            if (edge) nb = per[b] ? (up ? c - (len[b] - 1) * st[b] : c + (len[b] - 1) * st[b]) : c;
+            if (slot_code(m->bnd[o]) != code || m->adj[o] != nb) return false;
+            if (m->area[o] != tab[25 * (size_t)L + s]) return false;
+            if (m->w[o] != tab[19 * (size_t)L + (size_t)s * L + base[b] + idx[b]]) return false;
+            for (int a = 0; a < 3; ++a)
+                if ((*fcs[a])[o] != tab[(size_t)(3 * s + a) * L + base[a] + idx[a]]) return false;
+        }
+    }
+    return true;
+}
+
 int upload_mesh(zfv3_ctx* ctx) {
     const int n = ctx->n;
     const size_t N = (size_t)n, S = (size_t)kSides * n;
@@ -253,6 +315,11 @@ int upload_mesh(zfv3_ctx* ctx) {
         return r;
     ctx->blocks = (int)((N + 255) / 256);
     if ((r = ctx->alloc(&ctx->d_part, 5 * (size_t)ctx->blocks))) return r;
+    std::vector<double> tab;
+    if (ctx->use_tables && build_tables(ctx, tab, ctx->tab_bcode, ctx->tab_per)) {
+        if ((r = ctx->alloc(&ctx->d_tab, tab.size()))) return r;
+        Z3HIP(hipMemcpy(ctx->d_tab, tab.data(), tab.size() * sizeof(double), hipMemcpyHostToDevice));
+    }
     // host staging in the device layout
     std::vector<double> cell(5 * N), slot(5 * S);
     std::vector<int> type(S), adj(S);
@@ -343,6 +410,18 @@ int zfv3_mesh_cuboid(zfv3_ctx* ctx, int nx, int ny, int nz, double xmin, double 
     Z3HIP(hipStreamSynchronize(ctx->stream));
     build_cuboid(ctx, nx, ny, nz, xmin, xmax, ymin, ymax, zmin, zmax, bd);
     return upload_mesh(ctx);
+}
+
+int zfv3_set_geometry_tables(zfv3_ctx* ctx, int on) {
+    if (!ctx) return fail(ZFV_ERR_ARG, "null context");
+    ctx->use_tables = on != 0;
+    return ZFV_OK;
+}
+
+int zfv3_get_geometry_tables(zfv3_ctx* ctx, int* active) {
+    if (!ctx || !active) return fail(ZFV_ERR_ARG, "null argument");
+    *active = ctx->d_tab != nullptr;
+    return ZFV_OK;
 }
 
 int zfv3_mesh_n_cells(zfv3_ctx* ctx, int32_t* n) {

@@ -331,6 +331,73 @@ __device__ __forceinline__ double cell_dt3(const Z3& q, double h, const Eos& eos
     return h / (sqrt(vsq3(q)) + eos_c(eos, q.r, q.p));
 }
 
+// The geometry of cell c: the slot arrays, or (TAB) the Cuboid's tables.
+template <bool TAB>
+struct Geo3 {
+    const Dev3Mesh& M;
+    int c, i, j, k, L;
+    __device__ __forceinline__ Geo3(const Dev3Mesh& m, int cell) : M(m), c(cell), i(0), j(0), k(0), L(0) {
+        if constexpr (TAB) {
+            const int plane = M.tny * M.tnz;
+            i = c / plane;
+            const int r = c - i * plane;
+            j = r / M.tnz;
+            k = r - j * M.tnz;
+            L = M.tnx + M.tny + M.tnz;
+        }
+    }
+    __device__ __forceinline__ int idx(int a) const { return a == 0 ? i : (a == 1 ? j : k); }
+    __device__ __forceinline__ int len(int a) const { return a == 0 ? M.tnx : (a == 1 ? M.tny : M.tnz); }
+    __device__ __forceinline__ int base(int a) const { return a == 0 ? 0 : (a == 1 ? M.tnx : M.tnx + M.tny); }
+    __device__ __forceinline__ int stride(int a) const { return a == 0 ? M.tny * M.tnz : (a == 1 ? M.tnz : 1); }
+    __device__ __forceinline__ double centre(int a) const {
+        if constexpr (TAB) return M.tab[18 * L + base(a) + idx(a)];
+        return a == 0 ? M.cx[c] : (a == 1 ? M.cy[c] : M.cz[c]);
+    }
+    __device__ __forceinline__ double vol() const {
+        if constexpr (TAB) return M.tab[25 * L + 6];
+        return M.vol[c];
+    }
+    __device__ __forceinline__ bool edge(int s) const {
+        const int a = s >> 1;
+        return (s & 1) ? idx(a) == len(a) - 1 : idx(a) == 0;
+    }
+    __device__ __forceinline__ int type(int s) const {
+        if constexpr (TAB) return edge(s) ? M.bcode[s] : SC_INNER;
+        return M.type[s * M.stride + c];
+    }
+    __device__ __forceinline__ int adj(int s) const {
+        if constexpr (TAB) {
+            const int a = s >> 1, st = stride(a), up = s & 1;
+            if (!edge(s)) return up ? c + st : c - st;
+            if (!M.per[a]) return c;
+            return up ? c - (len(a) - 1) * st : c + (len(a) - 1) * st;
+        }
+        return M.adj[s * M.stride + c];
+    }
+    __device__ __forceinline__ double fc(int s, int a) const {
+        if constexpr (TAB) return M.tab[(3 * s + a) * L + base(a) + idx(a)];
+        const int so = s * M.stride + c;
+        return a == 0 ? M.fcx[so] : (a == 1 ? M.fcy[so] : M.fcz[so]);
+    }
+    __device__ __forceinline__ double area(int s) const {
+        if constexpr (TAB) return M.tab[25 * L + s];
+        return M.area[s * M.stride + c];
+    }
+    __device__ __forceinline__ double w(int s) const {
+        if constexpr (TAB) return M.tab[19 * L + s * L + base(s >> 1) + idx(s >> 1)];
+        return M.w[s * M.stride + c];
+    }
+    // component a of the centre of the neighbour nb across inner side s
+    __device__ __forceinline__ double ncentre(int s, int nb, int a) const {
+        if constexpr (TAB) {
+            if (a != (s >> 1)) return centre(a);
+            return M.tab[18 * L + base(a) + idx(a) + ((s & 1) ? 1 : -1)];
+        }
+        return a == 0 ? M.cx[nb] : (a == 1 ? M.cy[nb] : M.cz[nb]);
+    }
+};
+
 }  // namespace
 
 // SmFluid::compute_dt (sm_fluid.cpp:104-114): h / (|v| + c), min over cells.
@@ -343,24 +410,25 @@ __global__ void __launch_bounds__(256) k3_dt(Dev3Mesh M, Dev3State z, Eos eos, u
 
 // SmFluid::fluxes, first order (sm_fluid.cpp:127-174); returns the next
 // step's CFL candidate of the new state.
-template <int FLUX>
+template <int FLUX, bool TAB>
 __device__ __forceinline__ double flux1_cell(const Dev3Mesh& M, const Dev3State& in, const Dev3State& out,
                                              const Eos& eos, const Dev3Ctl& ctl, double dt, int c) {
+    const Geo3<TAB> geo(M, c);
     const Z3 zc = load3(in, c);
     Q3 F{0.0, 0.0, 0.0, 0.0, 0.0};
     bool bad = false;
     for (int s = 0; s < 6; ++s) {
         const int so = s * M.stride + c;
-        const int t = M.type[so];
+        const int t = geo.type(s);
         Z3 zn = zc;
-        if (t == SC_INNER || t == SC_PERIODIC) zn = load3(in, M.adj[so]);
+        if (t == SC_INNER || t == SC_PERIODIC) zn = load3(in, geo.adj(s));
         else if (t == SC_WALL) zn = wall3(s, zc);
         Z3 lm = zc, lp = zn;
         to_local3(s, lm);
         to_local3(s, lp);
         Q3 f = num_flux3<FLUX>(lm, lp, eos, bad);
         to_global3(s, f);
-        const double a = M.area[so];
+        const double a = geo.area(s);
         F.r = F.r + f.r * a;
         F.mx = F.mx + f.mx * a;
         F.my = F.my + f.my * a;
@@ -368,7 +436,7 @@ __device__ __forceinline__ double flux1_cell(const Dev3Mesh& M, const Dev3State&
         F.E = F.E + f.E * a;
     }
     Q3 q = to_q3(zc);
-    const double coef = dt / M.vol[c];
+    const double coef = dt / geo.vol();
     q.r = q.r - coef * F.r;
     q.mx = q.mx - coef * F.mx;
     q.my = q.my - coef * F.my;
@@ -380,12 +448,12 @@ __device__ __forceinline__ double flux1_cell(const Dev3Mesh& M, const Dev3State&
     return cell_dt3(zn, M.h[c], eos);
 }
 
-template <int FLUX>
+template <int FLUX, bool TAB>
 __global__ void __launch_bounds__(256) k3_flux1(Dev3Mesh M, Dev3State in, Dev3State out, Eos eos, Dev3Ctl ctl) {
     const int c = cell_of_thread();
     const double dt = block_dt(ctl);
     step_epilogue0(ctl, dt);
-    const double cand = c < M.n ? flux1_cell<FLUX>(M, in, out, eos, ctl, dt, c) : INFINITY;
+    const double cand = c < M.n ? flux1_cell<FLUX, TAB>(M, in, out, eos, ctl, dt, c) : INFINITY;
     block_min(ctl.next, cand);
 }
 
@@ -393,28 +461,29 @@ __global__ void __launch_bounds__(256) k3_flux1(Dev3Mesh M, Dev3State in, Dev3St
 // gradient::limiting, gradient.h:412-474) fused with fluxes_stage1
 // (sm_fluid.cpp:176-253).  G: [3][5][stride], d/dx, d/dy, d/dz of rho, u,
 // v, w, p.
-template <int LIM>
+template <int LIM, bool TAB>
 __global__ void __launch_bounds__(256) k3_grad_pred(Dev3Mesh M, Dev3State in, double* __restrict__ G, Dev3State half,
                                                     const int* __restrict__ wait, Eos eos, Dev3Ctl ctl) {
     const int c = cell_of_thread();
     const double dt = block_dt(ctl);
     if (c >= M.n) return;
+    const Geo3<TAB> geo(M, c);
     const Z3 zc = load3(in, c);
-    const double cc[3] = {M.cx[c], M.cy[c], M.cz[c]};
+    const double cc[3] = {geo.centre(0), geo.centre(1), geo.centre(2)};
     // per slot: dr along its axis (zero across), the state across it
     double dra[6], w[6];
     Z3 zn[6];
 #pragma unroll
     for (int s = 0; s < 6; ++s) {
         const int so = s * M.stride + c;
-        const int t = M.type[so];
-        const int nb = M.adj[so];
-        w[s] = M.w[so];
+        const int t = geo.type(s);
+        const int nb = geo.adj(s);
+        w[s] = geo.w(s);
         zn[s] = (t == SC_INNER || t == SC_PERIODIC) ? load3(in, nb) : (t == SC_WALL ? wall3(s, zc) : zc);
-        const double fc[3] = {M.fcx[so], M.fcy[so], M.fcz[so]};
+        const double fc[3] = {geo.fc(s, 0), geo.fc(s, 1), geo.fc(s, 2)};
         const double na = sgn_of(s);
         if (t == SC_INNER) {
-            const double nc[3] = {M.cx[nb], M.cy[nb], M.cz[nb]};
+            const double nc[3] = {geo.ncentre(s, nb, 0), geo.ncentre(s, nb, 1), geo.ncentre(s, nb, 2)};
             dra[s] = axis_of(nc, s) - axis_of(cc, s);
         } else {   // symm_point: p + (2 (f - p).n) n
             const double d = 2.0 * ((axis_of(fc, s) - axis_of(cc, s)) * na);
@@ -473,12 +542,12 @@ __global__ void __launch_bounds__(256) k3_grad_pred(Dev3Mesh M, Dev3State in, do
         Q3 Fl{0.0, 0.0, 0.0, 0.0, 0.0};
         for (int s = 0; s < 6; ++s) {
             const int so = s * M.stride + c;
-            Z3 zm = extrap3(zc, g, M.fcx[so] - cc[0], M.fcy[so] - cc[1], M.fcz[so] - cc[2], eos);
+            Z3 zm = extrap3(zc, g, geo.fc(s, 0) - cc[0], geo.fc(s, 1) - cc[1], geo.fc(s, 2) - cc[2], eos);
             if (is_bad3(zm, eos)) zm = zc;
             to_local3(s, zm);
             Q3 f = phys_flux3(zm);
             to_global3(s, f);
-            const double a = M.area[so];
+            const double a = geo.area(s);
             Fl.r = Fl.r + f.r * a;
             Fl.mx = Fl.mx + f.mx * a;
             Fl.my = Fl.my + f.my * a;
@@ -486,7 +555,7 @@ __global__ void __launch_bounds__(256) k3_grad_pred(Dev3Mesh M, Dev3State in, do
             Fl.E = Fl.E + f.E * a;
         }
         Q3 q = to_q3(zc);
-        const double coef = 0.5 * dt / M.vol[c];
+        const double coef = 0.5 * dt / geo.vol();
         q.r = q.r - coef * Fl.r;
         q.mx = q.mx - coef * Fl.mx;
         q.my = q.my - coef * Fl.my;
@@ -501,14 +570,15 @@ __global__ void __launch_bounds__(256) k3_grad_pred(Dev3Mesh M, Dev3State in, do
 // SmFluid::fluxes_stage2, corrector (sm_fluid.cpp:255-366): wait read from
 // the pre-pass array, written to the other; returns the next step's CFL
 // candidate of the new state.
-template <int FLUX>
+template <int FLUX, bool TAB>
 __device__ __forceinline__ double corr_cell(const Dev3Mesh& M, const Dev3State& init, const Dev3State& half,
                                             const double* __restrict__ G, const int* __restrict__ wait_in,
                                             int* __restrict__ wait_out, const Dev3State& next, const Eos& eos,
                                             const Dev3Ctl& ctl, double dt, int c) {
+    const Geo3<TAB> geo(M, c);
     const Z3 zc = load3(init, c);
     const Z3 zch = load3(half, c);
-    const double cc[3] = {M.cx[c], M.cy[c], M.cz[c]};
+    const double cc[3] = {geo.centre(0), geo.centre(1), geo.centre(2)};
     const int wc = wait_in[c];
     double g[3][5];
 #pragma unroll
@@ -520,8 +590,8 @@ __device__ __forceinline__ double corr_cell(const Dev3Mesh& M, const Dev3State& 
     int ts[6], nbs[6];
 #pragma unroll
     for (int s = 0; s < 6; ++s) {
-        ts[s] = M.type[s * M.stride + c];
-        nbs[s] = M.adj[s * M.stride + c];
+        ts[s] = geo.type(s);
+        nbs[s] = geo.adj(s);
     }
     bool rob[6];
 #pragma unroll
@@ -534,7 +604,7 @@ __device__ __forceinline__ double corr_cell(const Dev3Mesh& M, const Dev3State& 
         const int t = ts[s];
         const int nb = nbs[s];
         const bool bound = t >= SC_WALL;
-        const double fc[3] = {M.fcx[so], M.fcy[so], M.fcz[so]};
+        const double fc[3] = {geo.fc(s, 0), geo.fc(s, 1), geo.fc(s, 2)};
         const Z3 znh = bound ? (t == SC_WALL ? wall3(s, zch) : zch) : load3(half, nb);
         const bool robust = rob[s];
         Z3 zm, zp;
@@ -544,7 +614,7 @@ __device__ __forceinline__ double corr_cell(const Dev3Mesh& M, const Dev3State& 
         } else {
             double nc[3];
             if (t == SC_INNER) {
-                nc[0] = M.cx[nb]; nc[1] = M.cy[nb]; nc[2] = M.cz[nb];
+                nc[0] = geo.ncentre(s, nb, 0); nc[1] = geo.ncentre(s, nb, 1); nc[2] = geo.ncentre(s, nb, 2);
             } else {   // the mirror point of the cell centre
                 const double na = sgn_of(s);
                 const double d = 2.0 * ((axis_of(fc, s) - axis_of(cc, s)) * na);
@@ -569,7 +639,7 @@ __device__ __forceinline__ double corr_cell(const Dev3Mesh& M, const Dev3State& 
         to_local3(s, zp);
         Q3 f = robust ? hll3(zm, zp, eos, bad) : num_flux3<FLUX>(zm, zp, eos, bad);
         to_global3(s, f);
-        const double a = M.area[so];
+        const double a = geo.area(s);
         F.r = F.r + f.r * a;
         F.mx = F.mx + f.mx * a;
         F.my = F.my + f.my * a;
@@ -577,7 +647,7 @@ __device__ __forceinline__ double corr_cell(const Dev3Mesh& M, const Dev3State& 
         F.E = F.E + f.E * a;
     }
     Q3 q = to_q3(zc);
-    const double coef = dt / M.vol[c];
+    const double coef = dt / geo.vol();
     q.r = q.r - coef * F.r;
     q.mx = q.mx - coef * F.mx;
     q.my = q.my - coef * F.my;
@@ -597,7 +667,7 @@ __device__ __forceinline__ double corr_cell(const Dev3Mesh& M, const Dev3State& 
     return cell_dt3(zn, M.h[c], eos);
 }
 
-template <int FLUX>
+template <int FLUX, bool TAB>
 __global__ void __launch_bounds__(256) k3_corr(Dev3Mesh M, Dev3State init, Dev3State half, const double* __restrict__ G,
                                                const int* __restrict__ wait_in, int* __restrict__ wait_out,
                                                Dev3State next, Eos eos, Dev3Ctl ctl) {
@@ -605,7 +675,7 @@ __global__ void __launch_bounds__(256) k3_corr(Dev3Mesh M, Dev3State init, Dev3S
     const double dt = block_dt(ctl);
     step_epilogue0(ctl, dt);
     const double cand =
-        c < M.n ? corr_cell<FLUX>(M, init, half, G, wait_in, wait_out, next, eos, ctl, dt, c) : INFINITY;
+        c < M.n ? corr_cell<FLUX, TAB>(M, init, half, G, wait_in, wait_out, next, eos, ctl, dt, c) : INFINITY;
     block_min(ctl.next, cand);
 }
 
@@ -643,22 +713,30 @@ hipError_t launch3_dt(const Dev3Mesh& M, const Dev3State& z, const Eos& eos, uns
 
 hipError_t launch3_flux1(int flux, const Dev3Mesh& M, const Dev3State& in, const Dev3State& out, const Eos& eos,
                          const Dev3Ctl& ctl, hipStream_t s) {
+    const bool tab = M.tnx > 0;
+#define ZFV3_F1(F)                                                                                   \
+    if (tab) hipLaunchKernelGGL((k3_flux1<F, true>), g256(M.n), dim3(256), 0, s, M, in, out, eos, ctl); \
+    else hipLaunchKernelGGL((k3_flux1<F, false>), g256(M.n), dim3(256), 0, s, M, in, out, eos, ctl)
     switch (flux) {
-        case 0: hipLaunchKernelGGL(k3_flux1<0>, g256(M.n), dim3(256), 0, s, M, in, out, eos, ctl); break;
-        case 1: hipLaunchKernelGGL(k3_flux1<1>, g256(M.n), dim3(256), 0, s, M, in, out, eos, ctl); break;
-        case 6: hipLaunchKernelGGL(k3_flux1<6>, g256(M.n), dim3(256), 0, s, M, in, out, eos, ctl); break;
-        case 7: hipLaunchKernelGGL(k3_flux1<7>, g256(M.n), dim3(256), 0, s, M, in, out, eos, ctl); break;
-        case 2: hipLaunchKernelGGL(k3_flux1<2>, g256(M.n), dim3(256), 0, s, M, in, out, eos, ctl); break;
-        case 4: hipLaunchKernelGGL(k3_flux1<4>, g256(M.n), dim3(256), 0, s, M, in, out, eos, ctl); break;
-        case 5: hipLaunchKernelGGL(k3_flux1<5>, g256(M.n), dim3(256), 0, s, M, in, out, eos, ctl); break;
-        default: hipLaunchKernelGGL(k3_flux1<3>, g256(M.n), dim3(256), 0, s, M, in, out, eos, ctl); break;
+        case 0: ZFV3_F1(0); break;
+        case 1: ZFV3_F1(1); break;
+        case 6: ZFV3_F1(6); break;
+        case 7: ZFV3_F1(7); break;
+        case 2: ZFV3_F1(2); break;
+        case 4: ZFV3_F1(4); break;
+        case 5: ZFV3_F1(5); break;
+        default: ZFV3_F1(3); break;
     }
+#undef ZFV3_F1
     return hipGetLastError();
 }
 
 hipError_t launch3_grad_pred(int limiter, const Dev3Mesh& M, const Dev3State& in, double* G, const Dev3State& half,
                              const int* wait, const Eos& eos, const Dev3Ctl& ctl, hipStream_t s) {
-#define ZFV3_GP(L) hipLaunchKernelGGL((k3_grad_pred<L>), g256(M.n), dim3(256), 0, s, M, in, G, half, wait, eos, ctl)
+    const bool tab = M.tnx > 0;
+#define ZFV3_GP(L)                                                                                          \
+    if (tab) hipLaunchKernelGGL((k3_grad_pred<L, true>), g256(M.n), dim3(256), 0, s, M, in, G, half, wait, eos, ctl); \
+    else hipLaunchKernelGGL((k3_grad_pred<L, false>), g256(M.n), dim3(256), 0, s, M, in, G, half, wait, eos, ctl)
     switch (limiter) {
         case 0: ZFV3_GP(0); break;
         case 1: ZFV3_GP(1); break;
@@ -681,8 +759,12 @@ hipError_t launch3_grad_pred(int limiter, const Dev3Mesh& M, const Dev3State& in
 hipError_t launch3_corr(int flux, const Dev3Mesh& M, const Dev3State& init, const Dev3State& half, const double* G,
                         const int* wait_in, int* wait_out, const Dev3State& next, const Eos& eos, const Dev3Ctl& ctl,
                         hipStream_t s) {
-#define ZFV3_C(F) \
-    hipLaunchKernelGGL((k3_corr<F>), g256(M.n), dim3(256), 0, s, M, init, half, G, wait_in, wait_out, next, eos, ctl)
+    const bool tab = M.tnx > 0;
+#define ZFV3_C(F)                                                                                               \
+    if (tab) hipLaunchKernelGGL((k3_corr<F, true>), g256(M.n), dim3(256), 0, s, M, init, half, G, wait_in, wait_out, \
+                                next, eos, ctl);                                                                  \
+    else hipLaunchKernelGGL((k3_corr<F, false>), g256(M.n), dim3(256), 0, s, M, init, half, G, wait_in, wait_out,  \
+                            next, eos, ctl)
     switch (flux) {
         case 0: ZFV3_C(0); break;
         case 1: ZFV3_C(1); break;

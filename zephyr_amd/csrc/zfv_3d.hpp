@@ -13,6 +13,20 @@ namespace zfv {
 // on a non-periodic boundary); w: the least-squares weight of the slot.
 struct Dev3Mesh {
     int n, stride;
+    // A Cuboid's geometry as 1D tables (tnx > 0): every centre, face-centre
+    // component and least-squares weight depends on one index only, so the
+    // kernels read tables of nx + ny + nz entries instead of the slot arrays
+    // (whose values the host checked equal to the tables').  Layout, with
+    // L = tnx + tny + tnz and base(a) = 0, tnx, tnx + tny: face centre
+    // component a of side s at (3 s + a) L + base(a) + index_a; cell centre
+    // component a at 18 L + base(a) + index_a; weight of side s at
+    // 19 L + s L + base(s / 2) + index_{s/2}; area of side s at 25 L + s;
+    // volume at 25 L + 6.  bcode: the slot code of each side at the edge;
+    // per: periodic directions.
+    int tnx, tny, tnz;
+    const double* tab;
+    int bcode[6];
+    int per[3];
     const double *cx, *cy, *cz, *vol, *h;
     const int *type, *adj;
     const double *fcx, *fcy, *fcz, *area, *w;
@@ -44,6 +58,9 @@ constexpr unsigned long long kDblMaxBits = 0x7fefffffffffffffULL;
 constexpr int kSubSlots = 64;
 constexpr int kSubStride = 16;
 constexpr int kSlotWords = kSubSlots * kSubStride;
+
+// The number of doubles of a Cuboid's geometry tables.
+inline size_t tab3_size(int nx, int ny, int nz) { return 25 * (size_t)(nx + ny + nz) + 7; }
 
 hipError_t launch3_dt(const Dev3Mesh& M, const Dev3State& z, const Eos& eos, unsigned long long* slot,
                       hipStream_t s);
